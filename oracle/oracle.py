@@ -1,0 +1,294 @@
+"""ORACLE — test infrastructure only.
+
+ctypes bindings and numpy helpers over liboracle.so, the plain-C CPU restatement of the
+LFRic Core LMA-apply and Krylov-solve path (sources and per-function reference citations in
+oracle/src/).  Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import
+this module, and only as the checker; the product (lfric_core_amd) never does.
+"""
+import ctypes
+import os
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "liboracle.so"
+
+_c_int_p = ctypes.POINTER(ctypes.c_int)
+_VP = ctypes.c_void_p
+_I = ctypes.c_int
+_D = ctypes.c_double
+_U64 = ctypes.c_uint64
+_I64 = ctypes.c_int64
+
+
+class OrcProblem(ctypes.Structure):
+    _fields_ = [
+        ("ncells", _I), ("nlayers", _I), ("ncolours", _I), ("maxcc", _I), ("ndf", _I),
+        ("undf", _I), ("map", _VP), ("ncells_per_colour", _VP), ("cells_in_colour", _VP),
+        ("matrix", _VP), ("diag", _VP), ("apply_fn", _VP), ("prec_fn", _VP), ("ctx", _VP),
+    ]
+
+
+class OrcResult(ctypes.Structure):
+    _fields_ = [("iterations", _I), ("converged", _I), ("r_nrm_0", _D), ("r_nrm", _D)]
+
+
+CALLBACK = ctypes.CFUNCTYPE(None, _VP, ctypes.POINTER(_D), ctypes.POINTER(_D))
+
+_lib = None
+
+
+def build():
+    """Compile liboracle.so with the committed Makefile (gcc)."""
+    import subprocess
+    subprocess.run(["make", "-s", "-B", "-C", str(HERE)], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            build()
+        L = ctypes.CDLL(str(LIB_PATH))
+        sig = {
+            "orc_cubedsphere_topology": (_I, [_I, _VP, _VP]),
+            "orc_local_vert_lids": (_I, [_I, _VP, _VP]),
+            "orc_mesh_connectivity": (_I, [_I, _VP, _VP, _VP, _VP, _VP, _VP]),
+            "orc_ndf": (_I, [_I, _I]),
+            "orc_dofmap": (_I, [_I, _I, _I, _I, _I, _I, _VP, _VP, _VP, _VP, _VP]),
+            "orc_colour_cubedsphere": (_I, [_I, _VP, _VP, _VP, _VP]),
+            "orc_colour_validate": (_I, [_I, _VP, _VP]),
+            "orc_rng_bits": (_U64, [_U64, _U64, _U64]),
+            "orc_rng_uniform": (_D, [_U64, _U64, _U64]),
+            "orc_rng_normal": (_D, [_U64, _U64, _U64]),
+            "orc_log": (_D, [_D]),
+            "orc_exp": (_D, [_D]),
+            "orc_gen_spd": (None, [_I, _I64, _U64, _I, _VP]),
+            "orc_gen_uniform": (None, [_I64, _U64, _U64, _VP]),
+            "orc_lma_apply": (None, [_I, _I, _I, _VP, _VP, _I, _I, _I, _VP, _I, _I, _VP, _VP, _VP, _VP]),
+            "orc_lma_apply_r32": (None, [_I, _I, _I, _VP, _VP, _I, _I, _I, _VP, _I, _I, _VP, _VP, _VP, _VP]),
+            "orc_transpose_apply": (None, [_I, _I, _I, _VP, _VP, _I, _I, _I, _VP, _I, _I, _VP, _VP, _VP, _VP]),
+            "orc_dg_apply": (None, [_I, _I, _I, _I, _VP, _I, _I, _VP, _VP, _VP, _VP]),
+            "orc_mm_diagonal": (None, [_I, _I, _I, _I, _VP, _VP, _VP]),
+            "orc_cg": (_I, [ctypes.POINTER(OrcProblem), _VP, _VP, _D, _D, _I, _I, ctypes.POINTER(OrcResult), _VP]),
+            "orc_gmres": (_I, [ctypes.POINTER(OrcProblem), _VP, _VP, _I, _D, _D, _I, _I, ctypes.POINTER(OrcResult), _VP]),
+            "orc_givens_lsq": (_I, [_I, _VP, _VP, _VP]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return ctypes.c_void_p(a.ctypes.data) if a is not None else None
+
+
+W0, W2, W3 = 0, 2, 3
+
+
+def cubedsphere_topology(n):
+    nc = 6 * n * n
+    cn = np.zeros((nc, 4), np.int32)
+    vc = np.zeros((nc, 4), np.int32)
+    rc = lib().orc_cubedsphere_topology(n, _p(cn), _p(vc))
+    assert rc == 0
+    return cn, vc
+
+
+def local_vert_lids(verts_on_cell):
+    lid = np.zeros_like(verts_on_cell, dtype=np.int32)
+    nv = lib().orc_local_vert_lids(verts_on_cell.shape[0], _p(np.ascontiguousarray(verts_on_cell, np.int32)), _p(lid))
+    return lid, nv
+
+
+def mesh_connectivity(cell_next, vert_on_cell):
+    nc = cell_next.shape[0]
+    foc = np.zeros((nc, 6), np.int32)
+    eoc = np.zeros((nc, 12), np.int32)
+    nf, ne = _I(0), _I(0)
+    lib().orc_mesh_connectivity(nc, _p(np.ascontiguousarray(cell_next, np.int32)),
+                                _p(np.ascontiguousarray(vert_on_cell, np.int32)), _p(foc), _p(eoc),
+                                ctypes.byref(nf), ctypes.byref(ne))
+    return foc, eoc, nf.value, ne.value
+
+
+def dofmap(space, order, nlayers, vert_on_cell, face_on_cell, edge_on_cell, nverts_2d, nedges_2d):
+    nc = vert_on_cell.shape[0]
+    ndf = lib().orc_ndf(space, order)
+    dm = np.zeros((nc, ndf), np.int32)
+    undf = _I(0)
+    rc = lib().orc_dofmap(space, order, nc, nlayers, nverts_2d, nedges_2d,
+                          _p(np.ascontiguousarray(vert_on_cell, np.int32)),
+                          _p(np.ascontiguousarray(face_on_cell, np.int32)),
+                          _p(np.ascontiguousarray(edge_on_cell, np.int32)), _p(dm), ctypes.byref(undf))
+    assert rc == 0, rc
+    return dm, undf.value
+
+
+def colour_cubedsphere(n, cell_next):
+    nc = 6 * n * n
+    cm = np.zeros(nc, np.int32)
+    npc = np.zeros(4, np.int32)
+    maxc = (nc + 3) // 4
+    cic = np.zeros((4, maxc), np.int32)
+    rc = lib().orc_colour_cubedsphere(n, _p(np.ascontiguousarray(cell_next, np.int32)), _p(cm), _p(npc), _p(cic))
+    return rc, cm, npc, cic
+
+
+def colour_validate(cell_next, colour_map):
+    return lib().orc_colour_validate(cell_next.shape[0], _p(np.ascontiguousarray(cell_next, np.int32)),
+                                     _p(np.ascontiguousarray(colour_map, np.int32)))
+
+
+class Mesh:
+    """One serial-partition extruded cubed-sphere function space built by the oracle."""
+
+    def __init__(self, n, nlayers, space=W0, order=0):
+        self.n, self.nlayers, self.space, self.order = n, nlayers, space, order
+        self.cell_next, gvoc = cubedsphere_topology(n)
+        self.vert_on_cell, self.nverts = local_vert_lids(gvoc)
+        self.face_on_cell, self.edge_on_cell, self.nfaces_2d, self.nedges_2d = mesh_connectivity(
+            self.cell_next, self.vert_on_cell)
+        self.dofmap, self.undf = dofmap(space, order, nlayers, self.vert_on_cell, self.face_on_cell,
+                                        self.edge_on_cell, self.nverts, self.nedges_2d)
+        rc, self.colour_map, self.ncells_per_colour, self.cells_in_colour = colour_cubedsphere(n, self.cell_next)
+        assert rc == 0, "closed-form colouring did not validate"
+        self.ncells = 6 * n * n
+        self.ndf = self.dofmap.shape[1]
+        self.ncell_3d = self.ncells * nlayers
+
+
+def gen_spd(ndf, ncell_3d, seed, scaled=False):
+    """Element matrices in the reference layout matrix(ncell_3d, ndf, ndf) (Fortran order)."""
+    A = np.zeros(ndf * ndf * ncell_3d, np.float64)
+    lib().orc_gen_spd(ndf, ncell_3d, seed, int(scaled), _p(A))
+    return A.reshape((ndf, ndf, ncell_3d)).transpose(2, 1, 0)  # view as (ncell_3d, ndf, ndf)
+
+
+def flat_matrix(A):
+    """Fortran-order flat buffer of a (ncell_3d, ndf1, ndf2) matrix view."""
+    return np.ascontiguousarray(np.asarray(A).transpose(2, 1, 0)).ravel()
+
+
+def gen_uniform(n, seed, stream):
+    v = np.zeros(n, np.float64)
+    lib().orc_gen_uniform(n, seed, stream, _p(v))
+    return v
+
+
+def lma_apply(mesh, A, x, to_mesh=None, r32=False):
+    to = to_mesh or mesh
+    Af = flat_matrix(A)
+    if r32:
+        Af = Af.astype(np.float32)
+        x = np.ascontiguousarray(x, np.float32)
+        y = np.zeros(to.undf, np.float32)
+        fn = lib().orc_lma_apply_r32
+    else:
+        x = np.ascontiguousarray(x, np.float64)
+        y = np.zeros(to.undf, np.float64)
+        fn = lib().orc_lma_apply
+    fn(mesh.ncells, mesh.nlayers, 4, _p(to.ncells_per_colour), _p(to.cells_in_colour),
+       to.cells_in_colour.shape[1], to.ndf, to.undf, _p(to.dofmap), mesh.ndf, mesh.undf,
+       _p(mesh.dofmap), _p(Af), _p(x), _p(y))
+    return y
+
+
+def mm_diagonal(mesh, A):
+    d = np.zeros(mesh.undf, np.float64)
+    lib().orc_mm_diagonal(mesh.ncells, mesh.nlayers, mesh.ndf, mesh.undf, _p(mesh.dofmap),
+                          _p(flat_matrix(A)), _p(d))
+    return d
+
+
+def make_problem(mesh, A, diag=None):
+    Af = flat_matrix(A)
+    keep = [Af, diag]
+    prob = OrcProblem(mesh.ncells, mesh.nlayers, 4, mesh.cells_in_colour.shape[1], mesh.ndf,
+                      mesh.undf, _p(mesh.dofmap).value, _p(mesh.ncells_per_colour).value,
+                      _p(mesh.cells_in_colour).value, _p(Af).value,
+                      _p(diag).value if diag is not None else None, None, None, None)
+    prob._keep = keep
+    return prob
+
+
+def cg(mesh, A, b, x0=None, diag=None, r_tol=1e-8, a_tol=1e-16, max_iter=500, monitor=True):
+    prob = make_problem(mesh, A, diag)
+    x = np.zeros(mesh.undf) if x0 is None else np.array(x0, np.float64)
+    b = np.ascontiguousarray(b, np.float64)
+    res = OrcResult()
+    hist = np.zeros(max_iter + 1)
+    lib().orc_cg(ctypes.byref(prob), _p(b), _p(x), r_tol, a_tol, max_iter, int(monitor), ctypes.byref(res), _p(hist))
+    return x, res, hist[: res.iterations + 1]
+
+
+def gmres_callbacks(n, apply, prec, b, x0=None, gcrk=30, r_tol=1e-8, a_tol=1e-16, max_iter=50, monitor=True):
+    """GMRES(gcrk) of the oracle with python-callable operator and preconditioner."""
+
+    def wrap(f):
+        def cb(ctx, xp, yp):
+            xv = np.ctypeslib.as_array(xp, shape=(n,))
+            yv = np.ctypeslib.as_array(yp, shape=(n,))
+            yv[:] = f(xv.copy())
+        return CALLBACK(cb)
+
+    ca, cp = wrap(apply), wrap(prec)
+    prob = OrcProblem(0, 0, 0, 0, 0, n, None, None, None, None, None,
+                      ctypes.cast(ca, _VP).value, ctypes.cast(cp, _VP).value, None)
+    x = np.zeros(n) if x0 is None else np.array(x0, np.float64)
+    b = np.ascontiguousarray(b, np.float64)
+    res = OrcResult()
+    hist = np.zeros(max_iter + 1)
+    lib().orc_gmres(ctypes.byref(prob), _p(b), _p(x), gcrk, r_tol, a_tol, max_iter, int(monitor),
+                    ctypes.byref(res), _p(hist))
+    return x, res, hist[: res.iterations + 1]
+
+
+def cg_callbacks(n, apply, prec, b, x0=None, r_tol=1e-8, a_tol=1e-16, max_iter=500, monitor=True):
+    def wrap(f):
+        def cb(ctx, xp, yp):
+            xv = np.ctypeslib.as_array(xp, shape=(n,))
+            yv = np.ctypeslib.as_array(yp, shape=(n,))
+            yv[:] = f(xv.copy())
+        return CALLBACK(cb)
+
+    ca, cp = wrap(apply), wrap(prec)
+    prob = OrcProblem(0, 0, 0, 0, 0, n, None, None, None, None, None,
+                      ctypes.cast(ca, _VP).value, ctypes.cast(cp, _VP).value, None)
+    x = np.zeros(n) if x0 is None else np.array(x0, np.float64)
+    b = np.ascontiguousarray(b, np.float64)
+    res = OrcResult()
+    hist = np.zeros(max_iter + 1)
+    lib().orc_cg(ctypes.byref(prob), _p(b), _p(x), r_tol, a_tol, max_iter, int(monitor), ctypes.byref(res), _p(hist))
+    return x, res, hist[: res.iterations + 1]
+
+
+def givens_lsq(h, g):
+    m = h.shape[1]
+    hf = np.asfortranarray(h, np.float64).ravel(order="F")
+    u = np.zeros(m)
+    lib().orc_givens_lsq(m, _p(np.ascontiguousarray(hf)), _p(np.ascontiguousarray(g, np.float64)), _p(u))
+    return u
+
+
+def assembled_csr(mesh, A, to_mesh=None):
+    """Global sparse matrix sum_c P_c^T A_c P_c assembled with scipy (brute-force check)."""
+    import scipy.sparse as sp
+    to = to_mesh or mesh
+    A = np.asarray(A)
+    nl = mesh.nlayers
+    rows, cols, vals = [], [], []
+    cells = np.arange(mesh.ncells)
+    for k in range(nl):
+        ik = cells * nl + k
+        for i in range(to.ndf):
+            for j in range(mesh.ndf):
+                rows.append(to.dofmap[:, i] - 1 + k)
+                cols.append(mesh.dofmap[:, j] - 1 + k)
+                vals.append(A[ik, i, j])
+    return sp.csr_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))),
+                         shape=(to.undf, mesh.undf))

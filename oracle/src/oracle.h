@@ -1,0 +1,108 @@
+/*
+ * ORACLE — test infrastructure only.
+ *
+ * A plain-C, CPU-only restatement of the LFRic Core LMA-apply and Krylov-solve path, used
+ * ONLY by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg as the checker.
+ * Nothing in the product (lfric_core_amd/, include/) links, loads or calls this code.
+ *
+ * Every function cites the reference file:line it restates (paths relative to the
+ * lfric_core source tree).  Integer arrays use the reference's 1-based ids as values;
+ * C indexing is 0-based.
+ *
+ * Parity pins (what checks each restatement against something other than itself):
+ *   orc_cubedsphere_topology  netCDF UGRID fixtures C4/C8/C12/C16/C24/C32 (face_nodes, face_links)
+ *   orc_local_vert_lids       invariant: lid == first encounter; nverts == 6n^2+2
+ *   orc_mesh_connectivity     canned bi-periodic 3x3 mesh edge_on_cell / face_on_cell tables
+ *   orc_dofmap                fs0k0 / fs2k0 / fs3k0 dofmap fixtures; undf closed forms
+ *   orc_colour_cubedsphere    closed form + validator invariants + brute-force shared-vertex check
+ *   orc_rng_*                 libm log/exp to a few ulp; N(0,1) / U(-1,1) moments
+ *   orc_lma_apply             known-answer test of matrix_vector_mod_test.pf; assembled
+ *                             sparse matrix (scipy) brute force; oracle/_ref (the reference
+ *                             Fortran kernel compiled from source) bit-level comparison
+ *   orc_cg / orc_gmres        numpy.linalg.solve on tiny SPD/nonsymmetric systems; scipy cg
+ *                             solutions; Givens least-squares vs numpy.linalg.lstsq
+ */
+#ifndef LFRIC_ORACLE_H
+#define LFRIC_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- mesh / topology ---------------------------------------------------------------- */
+int orc_cubedsphere_topology(int n, int *cell_next, int *verts_on_cell);
+int orc_local_vert_lids(int ncells, const int *verts_on_cell_gid, int *vert_on_cell_lid);
+int orc_mesh_connectivity(int ncells, const int *cell_next, const int *vert_on_cell,
+                          int *face_on_cell, int *edge_on_cell, int *nfaces_2d,
+                          int *nedges_2d);
+
+/* function space ids (match the reference's fs_continuity enumeration by meaning only) */
+enum { ORC_W0 = 0, ORC_W2 = 2, ORC_W3 = 3 };
+int orc_ndf(int space, int order);
+int orc_dofmap(int space, int order, int ncells, int nlayers, int nverts_2d, int nedges_2d,
+               const int *vert_on_cell, const int *face_on_cell, const int *edge_on_cell,
+               int *dofmap, int *undf);
+
+int orc_colour_cubedsphere(int n, const int *cell_next, int *colour_map,
+                           int *ncells_per_colour, int *cells_in_colour);
+int orc_colour_validate(int ncells, const int *cell_next, const int *colour_map);
+
+/* ---- counter-based synthetic inputs -------------------------------------------------- */
+uint64_t orc_rng_bits(uint64_t seed, uint64_t stream, uint64_t ctr);
+double orc_rng_uniform(uint64_t seed, uint64_t stream, uint64_t ctr);
+double orc_rng_normal(uint64_t seed, uint64_t stream, uint64_t ctr);
+double orc_log(double x);
+double orc_exp(double x);
+void orc_gen_spd(int ndf, int64_t ncell_3d, uint64_t seed, int scaled, double *matrix);
+void orc_gen_uniform(int64_t n, uint64_t seed, uint64_t stream, double *v);
+
+/* ---- LMA kernels ---------------------------------------------------------------------- */
+void orc_lma_apply(int ncells, int nlayers, int ncolours, const int *ncells_per_colour,
+                   const int *cells_in_colour, int maxcc, int ndf1, int undf1,
+                   const int *map1, int ndf2, int undf2, const int *map2,
+                   const double *matrix, const double *x, double *y);
+void orc_lma_apply_r32(int ncells, int nlayers, int ncolours, const int *ncells_per_colour,
+                       const int *cells_in_colour, int maxcc, int ndf1, int undf1,
+                       const int *map1, int ndf2, int undf2, const int *map2,
+                       const float *matrix, const float *x, float *y);
+void orc_transpose_apply(int ncells, int nlayers, int ncolours, const int *ncells_per_colour,
+                         const int *cells_in_colour, int maxcc, int ndf1, int undf1,
+                         const int *map1, int ndf2, int undf2, const int *map2,
+                         const double *matrix, const double *x, double *y);
+void orc_dg_apply(int ncells, int nlayers, int ndf1, int undf1, const int *map1, int ndf2,
+                  int undf2, const int *map2, const double *matrix, const double *x,
+                  double *y);
+void orc_mm_diagonal(int ncells, int nlayers, int ndf, int undf, const int *map,
+                     const double *matrix, double *diag);
+
+/* ---- Krylov solvers ------------------------------------------------------------------- */
+typedef struct orc_problem {
+  /* operator y = A x (LMA, W->W) */
+  int ncells, nlayers, ncolours, maxcc, ndf, undf;
+  const int *map, *ncells_per_colour, *cells_in_colour;
+  const double *matrix;
+  const double *diag; /* Jacobi diagonal, NULL for identity preconditioner */
+  /* optional callbacks replacing the LMA operator / diagonal preconditioner */
+  void (*apply_fn)(const void *ctx, const double *x, double *y);
+  void (*prec_fn)(const void *ctx, const double *x, double *y);
+  const void *ctx;
+} orc_problem;
+
+typedef struct orc_result {
+  int iterations;
+  int converged;
+  double r_nrm_0;
+  double r_nrm;
+} orc_result;
+
+int orc_cg(const orc_problem *p, const double *b, double *x, double r_tol, double a_tol,
+           int max_iter, int monitor, orc_result *res, double *history);
+int orc_gmres(const orc_problem *p, const double *b, double *x, int gcrk, double r_tol,
+              double a_tol, int max_iter, int monitor, orc_result *res, double *history);
+int orc_givens_lsq(int m, const double *h_in, const double *g_in, double *u);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,130 @@
+"""Write the golden fixtures under tests/golden/ from data files and test tables held by the
+lfric_core reference tree.  Run once in a container that has the reference mounted:
+
+    python tests/golden/make_fixtures.py /root/reference
+
+Everything written is data (inputs and expected outputs); no reference source text is kept.
+
+* mesh_fixtures.npz   face_node / face_link connectivity of the committed UGRID meshes
+                      (infrastructure/unit-test/data/mesh_C{4,8,16,32}.nc,
+                      applications/coupled/example/mesh_C12.nc,
+                      applications/io_demo/example/mesh_C24.nc), read with scipy's netCDF3
+                      reader; the C16/C24/C32 arrays are stored as SHA-256 digests only.
+* canned_mesh.json    the bi-periodic 3x3x3 unit-test mesh of
+                      infrastructure/source/mesh/mesh_mod.F90:3625-3777 (layer-1 cell_next,
+                      vert_on_cell, edge_on_cell, face_on_cell) and the W0/W2/W3 dofmaps
+                      that infrastructure/unit-test/support/fs_test_data_mod.F90 expects on
+                      it (fs0k0_dofmap :24-33, fs2k0_dofmap :1026-1033, fs3k0_dofmap :1824-1826).
+* kat.json            the matrix_vector known-answer test
+                      (components/science/unit-test/kernel/algebra/matrix_vector_mod_test.pf:57-89)
+                      and the C3 mesh-generator summary counts
+                      (rose-stem/app/check_summary/file/summary_cubedsphere_c3_ugrid.kgo.txt:8-10).
+"""
+import hashlib
+import json
+import re
+import sys
+from pathlib import Path
+
+import numpy as np
+from scipy.io import netcdf_file
+
+HERE = Path(__file__).resolve().parent
+
+MESHES = {
+    4: "infrastructure/unit-test/data/mesh_C4.nc",
+    8: "infrastructure/unit-test/data/mesh_C8.nc",
+    12: "applications/coupled/example/mesh_C12.nc",
+    16: "infrastructure/unit-test/data/mesh_C16.nc",
+    24: "applications/io_demo/example/mesh_C24.nc",
+    32: "infrastructure/unit-test/data/mesh_C32.nc",
+}
+
+
+def digest(a):
+    return hashlib.sha256(np.ascontiguousarray(a, dtype="<i4").tobytes()).hexdigest()
+
+
+def read_mesh(path):
+    nc = netcdf_file(path, "r", mmap=False)
+    pre = [k for k in nc.variables if k.endswith("_face_nodes")][0][: -len("_face_nodes")]
+    fn = np.array(nc.variables[pre + "_face_nodes"][:], dtype=np.int32)
+    fl = np.array(nc.variables[pre + "_face_links"][:], dtype=np.int32)
+    nc.close()
+    return fn, fl
+
+
+def fortran_int_table(text, name):
+    """Integers of `integer ... :: name(r,0:c) = reshape( [ ... ], [r,c] )`."""
+    m = re.search(re.escape(name) + r"\s*\(([^)]*)\)\s*=\s*reshape\(\s*\[(.*?)\]", text, re.S)
+    vals = [int(v) for v in re.findall(r"-?\d+", m.group(2))]
+    return vals
+
+
+def canned_rows(text, start_marker, array, ncells=9):
+    i = text.index("self%" + start_marker)
+    rows = {}
+    for m in re.finditer(r"self%" + array + r"\(:\s*,\s*(\d+)\)\s*=\s*\[([^\]]*)\]", text[i:]):
+        c = int(m.group(1))
+        if c > ncells or c in rows:
+            if len(rows) == ncells:
+                break
+            continue
+        rows[c] = [int(v) for v in re.findall(r"-?\d+", m.group(2))]
+    return [rows[c] for c in range(1, ncells + 1)]
+
+
+def main(ref):
+    ref = Path(ref)
+    arrays, digests = {}, {}
+    for n, rel in MESHES.items():
+        fn, fl = read_mesh(ref / rel)
+        digests[f"C{n}_face_nodes"] = digest(fn)
+        digests[f"C{n}_face_links"] = digest(fl)
+        if n <= 12:
+            arrays[f"C{n}_face_nodes"] = fn
+            arrays[f"C{n}_face_links"] = fl
+    np.savez_compressed(HERE / "mesh_fixtures.npz", **arrays)
+    (HERE / "mesh_digests.json").write_text(json.dumps(digests, indent=1, sort_keys=True) + "\n")
+
+    mesh_src = (ref / "infrastructure/source/mesh/mesh_mod.F90").read_text()
+    marker = "else if (mesh_cfg == PLANE_BI_PERIODIC) then"
+    # the second occurrence holds the connectivity tables
+    first = mesh_src.index(marker)
+    start = mesh_src.index(marker, first + 1)
+    sub = mesh_src[start:]
+    cell_next = [r[:4] for r in canned_rows(sub, "cell_next", "cell_next")]
+    vert_on_cell = [r[:4] for r in canned_rows(sub, "vert_on_cell", "vert_on_cell")]
+    edge_on_cell = canned_rows(sub, "edge_on_cell", "edge_on_cell")
+    face_on_cell = canned_rows(sub, "face_on_cell", "face_on_cell")
+    fs = (ref / "infrastructure/unit-test/support/fs_test_data_mod.F90").read_text()
+    canned = {
+        "ncells": 9,
+        "nlayers": 3,
+        "cell_next": cell_next,
+        "vert_on_cell": vert_on_cell,
+        "edge_on_cell": edge_on_cell,
+        "face_on_cell": face_on_cell,
+        "fs0k0_dofmap": np.array(fortran_int_table(fs, "fs0k0_dofmap")).reshape(10, 8)[1:].tolist(),
+        "fs2k0_dofmap": np.array(fortran_int_table(fs, "fs2k0_dofmap")).reshape(10, 6)[1:].tolist(),
+        "fs3k0_dofmap": np.array(fortran_int_table(fs, "fs3k0_dofmap")).reshape(10, 1)[1:].tolist(),
+    }
+    (HERE / "canned_mesh.json").write_text(json.dumps(canned) + "\n")
+
+    d = np.array([[2.0, -2.0], [3.0, -1.0], [0.4, 1.3], [-4.5, 7.9]])
+    x = np.array([0.25, 1.5])
+    summary = (ref / "rose-stem/app/check_summary/file/summary_cubedsphere_c3_ugrid.kgo.txt").read_text()
+    counts = {k: int(re.search(k + r":\s*(\d+)", summary).group(1)) for k in ("Nodes", "Edges", "Faces")}
+    kat = {
+        "matrix_vector": {
+            "d": d.tolist(), "x": x.tolist(), "answer": [-2.5, -0.75, 2.05, 10.725],
+            "map1": [1, 2, 3, 4], "map2": [1, 2], "tol": 1e-14,
+        },
+        "c3_summary": counts,
+    }
+    (HERE / "kat.json").write_text(json.dumps(kat, indent=1) + "\n")
+    print("wrote fixtures to", HERE)
+
+
+if __name__ == "__main__":
+    main(sys.argv[1] if len(sys.argv) > 1 else "/root/reference")
