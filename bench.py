@@ -1,0 +1,197 @@
+#!/usr/bin/env python3
+"""Headline benchmark: solved dofs/s at relative residual 1e-8, mixed precision vs FP64, LMA
+GB/s and p50/p99 solve ms on one MI355X.
+
+Workload (one step = one solve): the C96 cubed sphere with 70 quadratically stretched levels,
+lowest-order W0 (8 dofs per cell, 3,926,158 dofs), synthetic SPD element matrices
+A_e = B_e B_e^T + 0.1 I with B_e ~ N(0,1)/8 and each column scaled by 10^U(-2,2), b ~ U(-1,1),
+seed 0, Jacobi-preconditioned CG to ||r||/||r0|| <= 1e-8.  The timed solve is FP32 (r_solver)
+CG inside FP64 iterative refinement; the plain FP64 CG solve of the same system is timed
+beside it.  The level stretching does not enter the synthetic element matrices.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1]
+"""
+import argparse
+import ctypes
+import json
+import math
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+
+
+def apply_bytes(fs, ndf, s_a, s_x):
+    """Algorithmic bytes of one LMA apply: matrix once, x once, y once, dofmap once."""
+    return fs.ncell_3d * ndf * ndf * s_a + 2 * fs.undf * s_x + 4 * fs.ncells * ndf
+
+
+def time_applies(L, op, x, y, reps):
+    op.apply(x, y)
+    L.synchronize()
+    L.timer_start()
+    for _ in range(reps):
+        op.apply(x, y)
+    return L.timer_stop() / reps
+
+
+def cpu_baseline(fs, op64, b_host, iters_needed, budget_s):
+    """The reference's own matrix_vector_code (compiled from its Fortran source into
+    oracle/_ref/libref.so) in a coloured OpenMP loop, inside cg_solve's operation order,
+    on the host cores.  Bounded sample: a fixed number of CG iterations at full size; the
+    solve time is that per-iteration time times the iterations the FP64 GPU solve needed."""
+    lib_path = ROOT / "oracle" / "_ref" / "libref.so"
+    if not lib_path.exists():
+        return None
+    from oracle import oracle as O
+    R = ctypes.CDLL(str(lib_path))
+    R.ref_cg_r64.restype = ctypes.c_int
+    R.ref_num_threads.restype = ctypes.c_int
+    m = O.Mesh(int(round(math.sqrt(fs.ncells / 6))), fs.nlayers)
+    A3 = op64.download()
+    A = O.flat_matrix(A3)
+    diag = O.mm_diagonal(m, A3)
+    P = lambda a: ctypes.c_void_p(a.ctypes.data)
+    I = ctypes.c_int
+
+    def run(iters):
+        x = np.zeros(m.undf)
+        t = time.perf_counter()
+        R.ref_cg_r64(I(m.ncells), I(m.nlayers), I(4), P(m.ncells_per_colour), P(m.cells_in_colour),
+                     I(m.cells_in_colour.shape[1]), I(8), I(m.undf), P(m.dofmap), P(A), P(diag), P(b_host), P(x),
+                     ctypes.c_double(1e-30), I(iters))
+        return time.perf_counter() - t
+
+    t2 = run(2)
+    n = max(3, min(iters_needed, int(budget_s / max(t2 / 2, 1e-6))))
+    t = run(n)
+    per_iter = t / n
+    solve_s = per_iter * iters_needed
+    return {
+        "value": fs.undf / solve_s,
+        "unit": "solved dofs/s",
+        "cores": R.ref_num_threads(),
+        "kind": "reference",
+        "sample": (f"{n} FP64 Jacobi-PCG iterations of the full {m.undf}-dof system ({t:.1f} s, "
+                   f"{per_iter * 1e3:.1f} ms/iteration) with the reference matrix_vector_code in a coloured "
+                   f"OpenMP loop; solve time = per-iteration time x {iters_needed} iterations of the FP64 solve"),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--edge-cells", type=int, default=96)
+    ap.add_argument("--nlayers", type=int, default=70)
+    ap.add_argument("--apply-reps", type=int, default=20)
+    ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only: --gpus {args.gpus} is not supported (the hot path is a "
+              "single mesh partition; see the sharding design for more than one GPU)", file=sys.stderr)
+        sys.exit(2)
+
+    import lfric_core_amd as L
+    from lfric_core_amd.solvers import residual_norm
+
+    L.init(0)
+    fs = L.FunctionSpace.cubedsphere(args.edge_cells, args.nlayers)
+    op64 = L.LmaOperator(fs).generate_spd(0, scaled=True)
+    op32 = op64.convert(L.R32)
+    prec64 = L.DiagonalPreconditioner.from_operator(op64)
+    prec32 = L.DiagonalPreconditioner.from_operator(op32)
+    b = L.FieldVector(fs).fill_uniform(0, 3)
+    x = L.FieldVector(fs)
+    undf = fs.undf
+
+    mixed = L.MixedPrecisionCG(op64, op32, prec32, 1e-8, 0.0, 20000, inner_r_tol=1e-4, check_every=16)
+    plain = L.ConjugateGradient(op64, prec64, 1e-8, 0.0, 20000, check_every=16)
+
+    def run(solver, k):
+        times, res = [], None
+        for _ in range(k):
+            x.set_scalar(0.0)
+            L.synchronize()
+            t = time.perf_counter()
+            res = solver.apply(x, b)
+            L.synchronize()
+            times.append(time.perf_counter() - t)
+        return times, res
+
+    run(mixed, args.warmup)
+    L.synchronize()
+    t0 = time.perf_counter()
+    mixed_times, mres = run(mixed, args.steps)
+    L.synchronize()
+    total = time.perf_counter() - t0
+    x_mixed = x.export_field()
+    mixed_true = residual_norm(op64, x, b) / mres.r_nrm_0
+
+    run(plain, args.warmup)
+    plain_times, pres = run(plain, args.steps)
+    x_plain = x.export_field()
+    plain_true = residual_norm(op64, x, b) / pres.r_nrm_0
+    err_mixed_vs_fp64 = float(np.linalg.norm(x_mixed - x_plain) / np.linalg.norm(x_plain))
+
+    # LMA apply bandwidth (HIP events on the library stream)
+    xs32, ys32 = L.FieldVector(fs, L.R32).fill_uniform(1, 3), L.FieldVector(fs, L.R32)
+    xs64, ys64 = L.FieldVector(fs).fill_uniform(1, 3), L.FieldVector(fs)
+    t32 = time_applies(L, op32, xs32, ys32, args.apply_reps)
+    t64 = time_applies(L, op64, xs64, ys64, args.apply_reps)
+    b32 = apply_bytes(fs, 8, 4, 4)
+    b64 = apply_bytes(fs, 8, 8, 8)
+    gbs32 = b32 / (t32 * 1e-3) / 1e9
+    gbs64 = b64 / (t64 * 1e-3) / 1e9
+
+    mean_mixed = total / args.steps
+    value = undf / mean_mixed
+    line = {
+        "metric": "solved dofs/s at rel-res 1e-8 (mixed vs FP64), LMA GB/s, p50/p99 solve ms",
+        "value": value,
+        "unit": "solved dofs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": mean_mixed * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 inner CG / f64 refinement",
+        "data": "synthetic (counter-based generator, seed 0)",
+        "config": {"workload": f"C{args.edge_cells}x{args.nlayers} quadratic-stretched W0 order 0, scaled SPD "
+                               f"LMA operator, Jacobi PCG to rel-res 1e-8", "undf": undf,
+                   "ncell_3d": fs.ncell_3d, "parallelism": "single mesh partition, one GPU"},
+        "fp64_solved_dofs_per_s": undf / (sum(plain_times) / len(plain_times)),
+        "mixed_vs_fp64_speedup": (sum(plain_times) / len(plain_times)) / mean_mixed,
+        "solve_ms": {"mixed_p50": float(np.percentile(mixed_times, 50) * 1e3),
+                     "mixed_p99": float(np.percentile(mixed_times, 99) * 1e3),
+                     "fp64_p50": float(np.percentile(plain_times, 50) * 1e3),
+                     "fp64_p99": float(np.percentile(plain_times, 99) * 1e3)},
+        "iterations": {"mixed_inner": mres.iterations, "mixed_outer": mres.outer_iterations,
+                       "fp64": pres.iterations},
+        "verification": {"mixed_true_rel_res_fp64": mixed_true, "fp64_true_rel_res": plain_true,
+                         "mixed_vs_fp64_solution_rel_err": err_mixed_vs_fp64},
+        "lma_gbs": {"fp32": gbs32, "fp64": gbs64, "fp32_ms": t32, "fp64_ms": t64,
+                    "fp32_bytes": b32, "fp64_bytes": b64},
+        "roofline": {"bound": "hbm", "achieved": gbs32, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": gbs32 / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_lma_colour<float,8,8> (4 colour launches per apply; bytes and time per apply)"},
+    }
+    if not args.no_cpu_baseline:
+        bh = b.export_field()
+        line["cpu_baseline"] = cpu_baseline(fs, op64, bh, pres.iterations, args.cpu_budget)
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,200 @@
+/*
+ * lfric_hip.h — C ABI of the MI355X-native LMA operator-apply and Krylov-solve library
+ * (liblfric_hip.so, built from lfric_core_amd/csrc).
+ *
+ * Plain C: opaque handles, plain pointers and sizes, int status codes.  Field and operator
+ * data live in GPU memory owned by the handles; host data crosses only in the *_upload /
+ * *_download calls (the import_field / export_field points of the reference's
+ * field_vector_type, components/science/source/algorithm/solver/sci_field_vector_mod.x90:294-334).
+ *
+ * Every entry point names the reference interface it replaces.  Reference paths are relative
+ * to the lfric_core source tree.
+ *
+ * Status codes (replace log_event(..., LOG_LEVEL_ERROR) aborts, infrastructure/source/
+ * utilities/log_mod.F90:423-471, and the solver's converged / not-converged / NaN exits,
+ * components/science/source/solver/sci_iterative_solver_mod.f90:577-590, 980-988).
+ */
+#ifndef LFRIC_HIP_H
+#define LFRIC_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LFRIC_OK 0
+#define LFRIC_NOT_CONVERGED 1 /* max_iter reached without meeting r_tol / a_tol */
+#define LFRIC_NAN 2           /* NaN in a residual norm */
+#define LFRIC_BREAKDOWN 3     /* zero / negative curvature or zero Krylov norm */
+#define LFRIC_E_ARG (-1)      /* invalid argument or shape mismatch */
+#define LFRIC_E_HIP (-2)      /* HIP runtime error */
+#define LFRIC_E_NOMEM (-3)    /* device allocation failed */
+#define LFRIC_E_STATE (-4)    /* library not initialised / handle misuse */
+#define LFRIC_E_UNSUPPORTED (-5)
+
+/* storage precisions: r_def / r_solver kinds of infrastructure/source/utilities/
+ * constants_mod.F90:55-61 (r_solver = real32 by default, infrastructure/build/lfric.mk:126) */
+#define LFRIC_R64 0
+#define LFRIC_R32 1
+#define LFRIC_BF16 2
+#define LFRIC_F16 3
+
+/* function spaces of infrastructure/source/function_space/fs_continuity_mod (by meaning) */
+#define LFRIC_W0 0
+#define LFRIC_W2 2
+#define LFRIC_W3 3
+
+typedef struct lfric_fs lfric_fs;     /* mesh + function space: dofmap, colouring (device) */
+typedef struct lfric_op lfric_op;     /* LMA operator local_stencil (device) */
+typedef struct lfric_vec lfric_vec;   /* field vector data (device) */
+typedef struct lfric_prec lfric_prec; /* preconditioner */
+
+/* ---- library ------------------------------------------------------------------------ */
+/* Select the device and create the library stream (default priority). */
+int lfric_init(int device);
+/* Last error message of the calling thread ("" when none). */
+const char *lfric_last_error(void);
+/* Wait for all work queued by the library. */
+int lfric_synchronize(void);
+/* Time work on the library stream with HIP events: start, then stop returns milliseconds. */
+int lfric_timer_start(void);
+int lfric_timer_stop(double *ms);
+/* Bytes of device memory held by live handles. */
+long long lfric_device_bytes(void);
+
+/* ---- function spaces (mesh_type + function_space_type inputs of the hot path) ---------- */
+/* Wrap a caller-held dofmap and colouring.  Replaces the reads of
+ * function_space_type%get_whole_dofmap (infrastructure/source/function_space/
+ * function_space_mod.F90:748-757) and mesh_type%get_colour_map /
+ * get_last_halo_cell_per_colour (infrastructure/source/mesh/mesh_mod.F90:2202-2245) made by
+ * the PSy layer.  dofmap is (ndf, ncells) column-major with 1-based dof ids at the column
+ * base; cells_in_colour is (maxcc, ncolours) column-major with 1-based cell ids. */
+int lfric_fs_create(int ncells, int nlayers, int ndf, int undf, const int *dofmap,
+                    int ncolours, const int *ncells_per_colour, const int *cells_in_colour,
+                    int maxcc, lfric_fs **fs);
+/* Build the serial-partition extruded cubed-sphere C<edge_cells> function space of the
+ * reference's mesh generator and dofmap setup (mesh_tools/source/support/gencube_ps_mod.F90:
+ * 336-720, infrastructure/source/function_space/
+ * function_space_constructor_helper_functions_mod.F90:1715-2300) and its 4-colouring
+ * (infrastructure/source/mesh/mesh_colouring_mod.F90:151-308). */
+int lfric_fs_create_cubedsphere(int edge_cells, int nlayers, int space, int order,
+                                lfric_fs **fs);
+/* Same mesh as `like` (same cells, layers and colouring), another space/order. */
+int lfric_fs_create_sibling(const lfric_fs *like, int space, int order, lfric_fs **fs);
+int lfric_fs_info(const lfric_fs *fs, int *ncells, int *nlayers, int *ndf, int *undf,
+                  int *ncolours);
+int lfric_fs_get_dofmap(const lfric_fs *fs, int *dofmap);
+int lfric_fs_get_colouring(const lfric_fs *fs, int *ncells_per_colour, int *cells_in_colour);
+void lfric_fs_destroy(lfric_fs *fs);
+
+/* ---- field vectors (vector_mod abstract_vector_type, infrastructure/source/field/
+ *      vector_mod.F90:17-59; field_vector_type / r_solver_field_vector_type) ----------- */
+int lfric_vec_create(const lfric_fs *fs, int precision, int nrhs, lfric_vec **v);
+int lfric_vec_upload(lfric_vec *v, const double *host);    /* import_field */
+int lfric_vec_download(const lfric_vec *v, double *host);  /* export_field */
+int lfric_vec_upload_r32(lfric_vec *v, const float *host);
+int lfric_vec_download_r32(const lfric_vec *v, float *host);
+int lfric_vec_size(const lfric_vec *v, int *undf, int *nrhs); /* vector_size */
+void lfric_vec_destroy(lfric_vec *v);
+int lfric_vec_set_scalar(lfric_vec *v, double scalar);                /* setval_c */
+int lfric_vec_axpy(lfric_vec *y, double alpha, const lfric_vec *x);   /* y = y + a x */
+int lfric_vec_aypx(lfric_vec *y, double alpha, const lfric_vec *x);   /* y = a y + x */
+int lfric_vec_axpby(lfric_vec *z, double a, const lfric_vec *x, double b,
+                    const lfric_vec *y);                              /* z = a x + b y */
+int lfric_vec_scale(lfric_vec *v, double alpha);                      /* v = a v */
+int lfric_vec_copy(lfric_vec *dst, const lfric_vec *src);             /* setval_X */
+int lfric_vec_divide(lfric_vec *z, const lfric_vec *x, const lfric_vec *d); /* X_divideby_Y */
+/* FP64-accumulated inner products (components/science/source/psy/sci_psykal_light_mod.f90:
+ * 72-214; single partition so the global sum is the identity). */
+int lfric_vec_dot(const lfric_vec *x, const lfric_vec *y, double *out);
+int lfric_vec_norm(const lfric_vec *x, double *out);
+/* Fill with U(-1,1) from the counter-based generator (seed, stream), dof index counter. */
+int lfric_vec_fill_uniform(lfric_vec *v, unsigned long long seed, unsigned long long stream);
+
+/* ---- LMA operators (operator_type, infrastructure/source/operator/operator_mod.t90:31-137) */
+int lfric_op_create(const lfric_fs *to, const lfric_fs *from, int precision, lfric_op **op);
+/* local_stencil(ncell_3d, ndf_to, ndf_from), column-major, FP64. */
+int lfric_op_upload(lfric_op *op, const double *local_stencil);
+int lfric_op_download(const lfric_op *op, double *local_stencil);
+/* Synthetic SPD element matrices A_e = B_e B_e^T + shift I, B_e ~ N(0,1)/8 (scaled: column j
+ * times 10^u_j, u_j ~ U(-2,2)); defined by the counter-based generator (shift 0.1 for the
+ * benchmark W0 operators, 1.0 for the W2 mass matrix C C^T + I). */
+int lfric_op_generate_spd(lfric_op *op, unsigned long long seed, int scaled, double shift);
+/* General element entries scale*U(lo,hi) (optionally times a per-column sign table). */
+int lfric_op_generate_uniform(lfric_op *op, unsigned long long seed, unsigned long long stream,
+                              double lo, double hi, const double *col_sign);
+/* Round an operator into another precision (e.g. the r_solver FP32 copy of an FP64 master). */
+int lfric_op_convert(const lfric_op *src, lfric_op *dst);
+void lfric_op_destroy(lfric_op *op);
+
+/* invoke(setval_c(y,0), matrix_vector_kernel_type(y, x, A)): y = A x over the coloured
+ * mesh with a colour-ordered, atomic-free scatter.  Replaces the whole invoke of
+ * components/science/source/kernel/algebra/matrix_vector_kernel_mod.F90:68-140 as called by
+ * components/science/source/algorithm/solver/sci_mass_matrix_operator_alg_mod.x90:98-100. */
+int lfric_lma_apply(const lfric_op *op, const lfric_vec *x, lfric_vec *y);
+/* Same product through the colour-ordered gather kernel (one launch, same summation order
+ * per dof, bit-identical to lfric_lma_apply). */
+int lfric_lma_apply_gather(const lfric_op *op, const lfric_vec *x, lfric_vec *y);
+/* transpose_matrix_vector_kernel (components/science/source/kernel/algebra/
+ * transpose_matrix_vector_kernel_mod.F90): y = A^T x, A maps `from` -> `to`, y on `from`. */
+int lfric_lma_apply_transpose(const lfric_op *op, const lfric_vec *x, lfric_vec *y);
+/* dg_matrix_vector_kernel (components/science/source/kernel/algebra/
+ * dg_matrix_vector_kernel_mod.F90): discontinuous output, no colouring. */
+int lfric_dg_apply(const lfric_op *op, const lfric_vec *x, lfric_vec *y);
+/* mm_diagonal_kernel (components/science/source/kernel/algebra/sci_mm_diagonal_kernel_mod.F90
+ * :68-125): assembled diagonal of a square LMA operator. */
+int lfric_mm_diagonal(const lfric_op *op, lfric_vec *diag);
+
+/* ---- preconditioners (abstract_preconditioner_type, components/science/source/solver/
+ *      sci_preconditioner_mod.f90:19-38) ------------------------------------------------ */
+/* diagonal_preconditioner_type (components/science/source/algorithm/solver/
+ * sci_diagonal_preconditioner_alg_mod.x90:71-106): y = x / diag. */
+int lfric_prec_create_diagonal(const lfric_vec *diag, lfric_prec **p);
+/* null_preconditioner_type (sci_null_preconditioner_alg_mod.x90): y = x. */
+int lfric_prec_create_identity(lfric_prec **p);
+int lfric_prec_apply(const lfric_prec *p, const lfric_vec *x, lfric_vec *y);
+void lfric_prec_destroy(lfric_prec *p);
+
+/* ---- iterative solvers (abstract_iterative_solver_type and its CG / GMRES extensions,
+ *      components/science/source/solver/sci_iterative_solver_mod.f90:38-57) ----------- */
+typedef struct lfric_solver_opts {
+  double r_tol;              /* relative tolerance */
+  double a_tol;              /* absolute tolerance */
+  int max_iter;              /* CG iterations / GMRES restart cycles */
+  int gcrk;                  /* GMRES Krylov dimension */
+  int monitor_convergence;   /* test convergence (reference: only when true) */
+  int fail_on_non_converged; /* return LFRIC_NOT_CONVERGED instead of LFRIC_OK */
+  int check_every;           /* host polls the device convergence flag every n iterations */
+  double inner_r_tol;        /* mixed precision: inner (FP32) relative reduction per cycle */
+  int max_outer;             /* mixed precision: maximum refinement cycles */
+} lfric_solver_opts;
+
+typedef struct lfric_solver_result {
+  int status;           /* LFRIC_OK / LFRIC_NOT_CONVERGED / LFRIC_NAN / LFRIC_BREAKDOWN */
+  int iterations;       /* total (inner) iterations */
+  int outer_iterations; /* refinement cycles (mixed precision), GMRES restart cycles */
+  double r_nrm_0;       /* initial residual norm */
+  double r_nrm;         /* final residual norm the stopping test used */
+  double true_rel_res;  /* ||b - A x|| / ||b - A x0|| in FP64 on exit (mixed and plain) */
+} lfric_solver_result;
+
+void lfric_solver_opts_default(lfric_solver_opts *o);
+
+/* ||b - A x|| in FP64-accumulated arithmetic of A's precision (verification helper). */
+int lfric_residual_norm(const lfric_op *A, const lfric_vec *x, const lfric_vec *b, double *nrm);
+
+/* cg_solve (sci_iterative_solver_mod.f90:495-592), device resident: x and b are FP64 or
+ * FP32 vectors of A's precision; scalars in FP64 on the device. */
+int lfric_cg_solve(const lfric_op *A, const lfric_prec *P, lfric_vec *x, const lfric_vec *b,
+                   const lfric_solver_opts *opts, lfric_solver_result *res);
+/* FP64 iterative refinement around FP32 (r_solver) CG: r = b - A64 x in FP64, inner CG on
+ * A32 with FP32 vectors and FP64 dots, x += d in FP64.  No reference counterpart as one
+ * call; the inner solve is cg_solve on r_solver_field_vector_type
+ * (components/science/source/algorithm/solver/sci_r_solver_field_vector_mod.x90:119-252). */
+int lfric_ir_cg_solve(const lfric_op *A64, const lfric_op *A32, const lfric_prec *P32,
+                      lfric_vec *x, const lfric_vec *b, const lfric_solver_opts *opts,
+                      lfric_solver_result *res);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

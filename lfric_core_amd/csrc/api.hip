@@ -1,0 +1,805 @@
+// C ABI of liblfric_hip: handles, set-up and the device-resident solver drivers.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <mutex>
+#include <unordered_map>
+
+#include "internal.h"
+#include "kernels.h"
+
+namespace lfric {
+
+// ---- errors and context ---------------------------------------------------------------------
+static thread_local std::string g_err;
+void set_error(const std::string &m) { g_err = m; }
+int fail(int code, const std::string &m) {
+  g_err = m;
+  return code;
+}
+
+static Context g_ctx;
+Context &ctx() { return g_ctx; }
+
+int require_init() {
+  if (g_ctx.device < 0) return fail(LFRIC_E_STATE, "lfric_init has not been called");
+  return LFRIC_OK;
+}
+
+int dmalloc(void **p, size_t bytes) {
+  *p = nullptr;
+  if (bytes == 0) bytes = 64;
+  hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    return fail(LFRIC_E_NOMEM, "hipMalloc of " + std::to_string(bytes) + " bytes failed: " + hipGetErrorString(e));
+  }
+  g_ctx.bytes += static_cast<long long>(bytes);
+  return LFRIC_OK;
+}
+
+void dfree(void *p, size_t bytes) {
+  if (!p) return;
+  (void)hipFree(p);
+  g_ctx.bytes -= static_cast<long long>(bytes == 0 ? 64 : bytes);
+}
+
+FunctionSpace::~FunctionSpace() {
+  dfree(d_map, map_bytes);
+  dfree(d_first, flag_bytes);
+  dfree(d_last, flag_bytes);
+}
+
+// Scratch vectors of the solvers, kept between solves (grown on demand).
+struct Workspace {
+  void *p = nullptr;
+  size_t bytes = 0;
+};
+static Workspace g_ws[8];
+static SolverState *g_state = nullptr;
+static SolverState *g_state_host = nullptr;
+
+static int ws_get(int slot, size_t bytes, void **out) {
+  Workspace &w = g_ws[slot];
+  if (w.bytes < bytes) {
+    dfree(w.p, w.bytes);
+    w.p = nullptr;
+    w.bytes = 0;
+    if (int rc = dmalloc(&w.p, bytes)) return rc;
+    w.bytes = bytes;
+  }
+  *out = w.p;
+  return LFRIC_OK;
+}
+
+static size_t esize(int prec) {
+  switch (prec) {
+    case LFRIC_R64: return 8;
+    case LFRIC_R32: return 4;
+    case LFRIC_BF16:
+    case LFRIC_F16: return 2;
+  }
+  return 0;
+}
+
+// ---- function spaces -----------------------------------------------------------------------
+static int finish_space(std::shared_ptr<FunctionSpace> &fs) {
+  FunctionSpace &f = *fs;
+  const int nc = f.ncells, ndf = f.ndf;
+  LF_CHECK(ndf >= 1 && ndf <= 32, LFRIC_E_UNSUPPORTED, "ndf must be in [1, 32]");
+  // colour-major positions
+  f.colour_start.assign(f.ncolours + 1, 0);
+  for (int c = 0; c < f.ncolours; ++c) f.colour_start[c + 1] = f.colour_start[c] + f.npc[c];
+  LF_CHECK(f.colour_start[f.ncolours] == nc, LFRIC_E_ARG, "colouring does not cover every cell once");
+  f.cell_of_pos.resize(nc);
+  std::vector<char> seen_cell(nc, 0);
+  for (int c = 0; c < f.ncolours; ++c)
+    for (int i = 0; i < f.npc[c]; ++i) {
+      int cell = f.cic[static_cast<size_t>(c) * f.maxcc + i];
+      LF_CHECK(cell >= 1 && cell <= nc && !seen_cell[cell - 1], LFRIC_E_ARG, "invalid or repeated cell in colouring");
+      seen_cell[cell - 1] = 1;
+      f.cell_of_pos[f.colour_start[c] + i] = cell - 1;
+    }
+  for (int i = 0; i < nc * ndf; ++i)
+    LF_CHECK(f.dofmap[i] >= 1 && f.dofmap[i] + f.nlayers - 1 <= f.undf, LFRIC_E_ARG, "dofmap entry out of range");
+  // column roles.  Two dofs of one cell whose columns overlap must be a bottom/top pair
+  // (map(top) = map(bottom) + 1, the dofmap convention of dofmap_setup); the pair set must be
+  // the same in every cell.  Non-overlapping columns need no role.
+  f.top_mask = 0;
+  f.partner.assign(ndf, -1);
+  for (int c = 0; c < nc; ++c) {
+    const int *row = &f.dofmap[static_cast<size_t>(c) * ndf];
+    for (int a = 0; a < ndf; ++a)
+      for (int b = 0; b < ndf; ++b) {
+        const int d = row[b] - row[a];
+        if (a == b || d <= 0 || d >= f.nlayers) continue;
+        LF_CHECK(d == 1, LFRIC_E_UNSUPPORTED, "dofmap columns of one cell overlap by more than one level");
+        if (c == 0) {
+          LF_CHECK(f.partner[a] < 0 && !((f.top_mask >> b) & 1u), LFRIC_E_UNSUPPORTED,
+                   "dofmap has a dof with two top partners");
+          f.partner[a] = b;
+          f.top_mask |= 1u << b;
+        } else {
+          LF_CHECK(f.partner[a] == b, LFRIC_E_UNSUPPORTED, "bottom/top dof pairs differ between cells");
+        }
+      }
+  }
+  for (int a = 0; a < ndf; ++a)
+    if (f.partner[a] >= 0 && ((f.top_mask >> a) & 1u))
+      return fail(LFRIC_E_UNSUPPORTED, "dofmap has chained top/bottom dofs");
+  for (int a = 0; a < ndf; ++a)  // every pair of cell 0 must exist in all cells
+    if (f.partner[a] >= 0)
+      for (int c = 1; c < nc; ++c)
+        LF_CHECK(f.dofmap[static_cast<size_t>(c) * ndf + f.partner[a]] == f.dofmap[static_cast<size_t>(c) * ndf + a] + 1,
+                 LFRIC_E_UNSUPPORTED, "bottom/top dof pairs differ between cells");
+  // colouring must keep every column to one cell per colour (race-free scatter)
+  {
+    std::vector<int> stamp(static_cast<size_t>(f.undf) + 2, -1);
+    for (int c = 0; c < f.ncolours; ++c)
+      for (int p = f.colour_start[c]; p < f.colour_start[c + 1]; ++p) {
+        const int cell = f.cell_of_pos[p];
+        for (int d = 0; d < ndf; ++d) {
+          int &s = stamp[f.dofmap[cell * ndf + d]];
+          if (s == c) return fail(LFRIC_E_ARG, "two cells of one colour share a dof column");
+          s = c;
+        }
+      }
+  }
+  // first / last touch of each bottom-role column in processing order
+  std::vector<uint32_t> first(nc, 0), last(nc, 0);
+  {
+    std::vector<int> last_pos(static_cast<size_t>(f.undf) + 2, -1);
+    for (int p = 0; p < nc; ++p) {
+      const int cell = f.cell_of_pos[p];
+      for (int d = 0; d < ndf; ++d) {
+        if ((f.top_mask >> d) & 1u) continue;
+        int &lp = last_pos[f.dofmap[cell * ndf + d]];
+        if (lp < 0) first[p] |= 1u << d;
+        lp = p;
+      }
+    }
+    for (int p = 0; p < nc; ++p) {
+      const int cell = f.cell_of_pos[p];
+      for (int d = 0; d < ndf; ++d)
+        if (!((f.top_mask >> d) & 1u) && last_pos[f.dofmap[cell * ndf + d]] == p) last[p] |= 1u << d;
+    }
+  }
+  std::vector<int> mpos(static_cast<size_t>(nc) * ndf);
+  for (int p = 0; p < nc; ++p)
+    for (int d = 0; d < ndf; ++d) mpos[static_cast<size_t>(p) * ndf + d] = f.dofmap[f.cell_of_pos[p] * ndf + d] - 1;
+  f.map_bytes = sizeof(int) * mpos.size();
+  f.flag_bytes = sizeof(uint32_t) * nc;
+  if (int rc = dmalloc(reinterpret_cast<void **>(&f.d_map), f.map_bytes)) return rc;
+  if (int rc = dmalloc(reinterpret_cast<void **>(&f.d_first), f.flag_bytes)) return rc;
+  if (int rc = dmalloc(reinterpret_cast<void **>(&f.d_last), f.flag_bytes)) return rc;
+  LF_HIP(hipMemcpy(f.d_map, mpos.data(), f.map_bytes, hipMemcpyHostToDevice));
+  LF_HIP(hipMemcpy(f.d_first, first.data(), f.flag_bytes, hipMemcpyHostToDevice));
+  LF_HIP(hipMemcpy(f.d_last, last.data(), f.flag_bytes, hipMemcpyHostToDevice));
+  return LFRIC_OK;
+}
+
+static int space_from_topology(const std::shared_ptr<Topology> &t, int nlayers, int space, int order,
+                               lfric_fs **out) {
+  auto fs = std::make_shared<FunctionSpace>();
+  FunctionSpace &f = *fs;
+  if (int rc = build_dofmap(*t, nlayers, space, order, f.dofmap, f.ndf, f.undf)) return rc;
+  f.ncells = t->ncells;
+  f.nlayers = nlayers;
+  f.space = space;
+  f.order = order;
+  f.topo = t;
+  f.ncolours = 4;
+  f.npc.assign(4, 0);
+  for (int c = 0; c < t->ncells; ++c) f.npc[t->colour_of_cell[c] - 1]++;
+  f.maxcc = *std::max_element(f.npc.begin(), f.npc.end());
+  f.cic.assign(static_cast<size_t>(4) * f.maxcc, 0);
+  std::vector<int> fill(4, 0);
+  for (int c = 1; c <= t->ncells; ++c) {
+    const int col = t->colour_of_cell[c - 1] - 1;
+    f.cic[static_cast<size_t>(col) * f.maxcc + fill[col]++] = c;
+  }
+  if (int rc = finish_space(fs)) return rc;
+  *out = new lfric_fs{fs};
+  return LFRIC_OK;
+}
+
+// ---- solver drivers ---------------------------------------------------------------------------
+struct CgArgs {
+  const lfric_op *A;
+  const void *diag;  // nullptr: identity
+  void *x;
+  const void *b;
+  long long n;
+  int prec;
+  bool x_zero;
+  double r_tol, a_tol;
+  int max_iter, monitor, check_every;
+};
+
+// cg_solve (components/science/source/solver/sci_iterative_solver_mod.f90:495-592) with every
+// vector and scalar on the device.  The host enqueues `check_every` iterations at a time and
+// then reads the device state once; kernels of iterations past convergence return at once.
+static int cg_device(const CgArgs &a, SolverState &out) {
+  Context &c = ctx();
+  const size_t vb = esize(a.prec) * static_cast<size_t>(a.n);
+  void *r, *z, *p;
+  if (int rc = ws_get(0, vb, &r)) return rc;
+  if (int rc = ws_get(1, vb, &z)) return rc;
+  if (int rc = ws_get(2, vb, &p)) return rc;
+  SolverState s{};
+  s.r_tol = a.r_tol;
+  s.a_tol = a.a_tol;
+  s.max_iter = a.max_iter;
+  s.monitor = a.monitor;
+  LF_HIP(hipMemcpyAsync(g_state, &s, sizeof(s), hipMemcpyHostToDevice, c.stream));
+  if (a.x_zero) {
+    if (int rc = launch_copy(a.prec, r, a.b, a.n)) return rc;
+  } else {
+    if (int rc = launch_lma_apply(a.A, a.x, r, nullptr)) return rc;
+    if (int rc = launch_cg_residual(a.prec, r, a.b, a.n)) return rc;
+  }
+  if (int rc = launch_cg_init_norm(a.prec, r, a.n, g_state)) return rc;
+  if (int rc = launch_cg_init_prec(a.prec, r, z, p, a.diag, a.n, g_state)) return rc;
+  const int every = std::max(1, a.check_every);
+  const int max_chunks = a.max_iter / every + 2;
+  for (int ch = 0; ch < max_chunks; ++ch) {
+    for (int i = 0; i < every; ++i) {
+      if (int rc = launch_lma_apply(a.A, p, z, &g_state->done)) return rc;
+      if (int rc = launch_cg_dot_pz(a.prec, p, z, a.n, g_state)) return rc;
+      if (int rc = launch_cg_update(a.prec, a.x, r, z, p, a.diag, a.n, g_state)) return rc;
+      if (int rc = launch_cg_aypx(a.prec, p, z, a.n, g_state)) return rc;
+    }
+    LF_HIP(hipMemcpyAsync(g_state_host, g_state, sizeof(SolverState), hipMemcpyDeviceToHost, c.stream));
+    LF_HIP(hipStreamSynchronize(c.stream));
+    if (g_state_host->done) break;
+  }
+  out = *g_state_host;
+  return LFRIC_OK;
+}
+
+static int true_residual(const lfric_op *A, const void *x, const void *b, long long n, int prec, double *nrm) {
+  void *r;
+  if (int rc = ws_get(3, esize(prec) * static_cast<size_t>(n), &r)) return rc;
+  if (int rc = launch_lma_apply(A, x, r, nullptr)) return rc;
+  if (int rc = launch_cg_residual(prec, r, b, n)) return rc;
+  double *d_out = &g_state->red0;
+  if (int rc = launch_dot(prec, r, r, n, d_out)) return rc;
+  double h = 0;
+  LF_HIP(hipMemcpyAsync(g_state_host, g_state, sizeof(SolverState), hipMemcpyDeviceToHost, ctx().stream));
+  LF_HIP(hipStreamSynchronize(ctx().stream));
+  h = g_state_host->red0;
+  *nrm = std::sqrt(h);
+  return LFRIC_OK;
+}
+
+}  // namespace lfric
+
+using namespace lfric;
+
+// ======================================================================================
+extern "C" {
+
+int lfric_init(int device) {
+  Context &c = ctx();
+  if (c.device >= 0) {
+    if (c.device == device) return LFRIC_OK;
+    return fail(LFRIC_E_STATE, "library already initialised on another device");
+  }
+  int n = 0;
+  LF_HIP(hipGetDeviceCount(&n));
+  LF_CHECK(device >= 0 && device < n, LFRIC_E_ARG, "no such device");
+  LF_HIP(hipSetDevice(device));
+  LF_HIP(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
+  LF_HIP(hipEventCreate(&c.t0));
+  LF_HIP(hipEventCreate(&c.t1));
+  c.device = device;
+  c.partials_cap = static_cast<size_t>(kRedBlocks) * 4;
+  if (int rc = dmalloc(reinterpret_cast<void **>(&c.d_partials), sizeof(double) * c.partials_cap)) return rc;
+  if (int rc = dmalloc(reinterpret_cast<void **>(&c.d_counter), sizeof(unsigned int) * 16)) return rc;
+  LF_HIP(hipMemset(c.d_counter, 0, sizeof(unsigned int) * 16));
+  if (int rc = dmalloc(reinterpret_cast<void **>(&g_state), sizeof(SolverState))) return rc;
+  LF_HIP(hipMemset(g_state, 0, sizeof(SolverState)));
+  LF_HIP(hipHostMalloc(reinterpret_cast<void **>(&g_state_host), sizeof(SolverState), hipHostMallocDefault));
+  return LFRIC_OK;
+}
+
+const char *lfric_last_error(void) { return g_err.c_str(); }
+
+int lfric_synchronize(void) {
+  if (int rc = require_init()) return rc;
+  LF_HIP(hipStreamSynchronize(ctx().stream));
+  return LFRIC_OK;
+}
+
+int lfric_timer_start(void) {
+  if (int rc = require_init()) return rc;
+  LF_HIP(hipEventRecord(ctx().t0, ctx().stream));
+  return LFRIC_OK;
+}
+
+int lfric_timer_stop(double *ms) {
+  if (int rc = require_init()) return rc;
+  LF_HIP(hipEventRecord(ctx().t1, ctx().stream));
+  LF_HIP(hipEventSynchronize(ctx().t1));
+  float f = 0.f;
+  LF_HIP(hipEventElapsedTime(&f, ctx().t0, ctx().t1));
+  *ms = f;
+  return LFRIC_OK;
+}
+
+long long lfric_device_bytes(void) { return ctx().bytes; }
+
+// ---- function spaces ------------------------------------------------------------------------
+int lfric_fs_create(int ncells, int nlayers, int ndf, int undf, const int *dofmap, int ncolours,
+                    const int *ncells_per_colour, const int *cells_in_colour, int maxcc, lfric_fs **out) {
+  if (int rc = require_init()) return rc;
+  LF_CHECK(out && dofmap && ncells_per_colour && cells_in_colour, LFRIC_E_ARG, "null argument");
+  LF_CHECK(ncells > 0 && nlayers > 0 && ndf > 0 && undf > 0 && ncolours > 0 && maxcc > 0, LFRIC_E_ARG,
+           "sizes must be positive");
+  auto fs = std::make_shared<FunctionSpace>();
+  FunctionSpace &f = *fs;
+  f.ncells = ncells;
+  f.nlayers = nlayers;
+  f.ndf = ndf;
+  f.undf = undf;
+  f.ncolours = ncolours;
+  f.maxcc = maxcc;
+  f.dofmap.assign(dofmap, dofmap + static_cast<size_t>(ncells) * ndf);
+  f.npc.assign(ncells_per_colour, ncells_per_colour + ncolours);
+  f.cic.assign(cells_in_colour, cells_in_colour + static_cast<size_t>(ncolours) * maxcc);
+  for (int c = 0; c < ncolours; ++c)
+    LF_CHECK(f.npc[c] >= 0 && f.npc[c] <= maxcc, LFRIC_E_ARG, "ncells_per_colour exceeds maxcc");
+  if (int rc = finish_space(fs)) return rc;
+  *out = new lfric_fs{fs};
+  return LFRIC_OK;
+}
+
+int lfric_fs_create_cubedsphere(int edge_cells, int nlayers, int space, int order, lfric_fs **out) {
+  if (int rc = require_init()) return rc;
+  LF_CHECK(out && nlayers > 0, LFRIC_E_ARG, "bad arguments");
+  auto t = std::make_shared<Topology>();
+  if (int rc = build_cubedsphere_topology(edge_cells, *t)) return rc;
+  return space_from_topology(t, nlayers, space, order, out);
+}
+
+int lfric_fs_create_sibling(const lfric_fs *like, int space, int order, lfric_fs **out) {
+  if (int rc = require_init()) return rc;
+  LF_CHECK(like && out && like->fs->topo, LFRIC_E_ARG, "sibling spaces need a generated mesh");
+  return space_from_topology(like->fs->topo, like->fs->nlayers, space, order, out);
+}
+
+int lfric_fs_info(const lfric_fs *fs, int *ncells, int *nlayers, int *ndf, int *undf, int *ncolours) {
+  LF_CHECK(fs, LFRIC_E_ARG, "null space");
+  const FunctionSpace &f = *fs->fs;
+  if (ncells) *ncells = f.ncells;
+  if (nlayers) *nlayers = f.nlayers;
+  if (ndf) *ndf = f.ndf;
+  if (undf) *undf = f.undf;
+  if (ncolours) *ncolours = f.ncolours;
+  return LFRIC_OK;
+}
+
+int lfric_fs_get_dofmap(const lfric_fs *fs, int *dofmap) {
+  LF_CHECK(fs && dofmap, LFRIC_E_ARG, "null argument");
+  std::copy(fs->fs->dofmap.begin(), fs->fs->dofmap.end(), dofmap);
+  return LFRIC_OK;
+}
+
+int lfric_fs_get_colouring(const lfric_fs *fs, int *npc, int *cic) {
+  LF_CHECK(fs, LFRIC_E_ARG, "null space");
+  const FunctionSpace &f = *fs->fs;
+  if (npc) std::copy(f.npc.begin(), f.npc.end(), npc);
+  if (cic) std::copy(f.cic.begin(), f.cic.end(), cic);
+  return LFRIC_OK;
+}
+
+void lfric_fs_destroy(lfric_fs *fs) { delete fs; }
+
+// ---- vectors ----------------------------------------------------------------------------------
+int lfric_vec_create(const lfric_fs *fs, int precision, int nrhs, lfric_vec **out) {
+  if (int rc = require_init()) return rc;
+  LF_CHECK(fs && out && nrhs >= 1, LFRIC_E_ARG, "bad arguments");
+  LF_CHECK(precision == LFRIC_R64 || precision == LFRIC_R32, LFRIC_E_UNSUPPORTED, "vector precision");
+  auto *v = new lfric_vec;
+  v->fs = fs->fs;
+  v->precision = precision;
+  v->nrhs = nrhs;
+  v->n = static_cast<long long>(fs->fs->undf) * nrhs;
+  v->bytes = esize(precision) * static_cast<size_t>(v->n);
+  if (int rc = dmalloc(&v->d, v->bytes)) {
+    delete v;
+    return rc;
+  }
+  LF_HIP(hipMemsetAsync(v->d, 0, v->bytes, ctx().stream));
+  *out = v;
+  return LFRIC_OK;
+}
+
+void lfric_vec_destroy(lfric_vec *v) {
+  if (!v) return;
+  dfree(v->d, v->bytes);
+  delete v;
+}
+
+static int upload_any(lfric_vec *v, const void *host, int hprec) {
+  if (int rc = require_init()) return rc;
+  LF_CHECK(v && host, LFRIC_E_ARG, "null argument");
+  if (hprec == v->precision) {
+    LF_HIP(hipMemcpyAsync(v->d, host, v->bytes, hipMemcpyHostToDevice, ctx().stream));
+    LF_HIP(hipStreamSynchronize(ctx().stream));
+    return LFRIC_OK;
+  }
+  void *tmp;
+  const size_t hb = esize(hprec) * static_cast<size_t>(v->n);
+  if (int rc = ws_get(7, hb, &tmp)) return rc;
+  LF_HIP(hipMemcpyAsync(tmp, host, hb, hipMemcpyHostToDevice, ctx().stream));
+  if (int rc = launch_convert(v->precision, v->d, hprec, tmp, v->n, 1.0)) return rc;
+  LF_HIP(hipStreamSynchronize(ctx().stream));
+  return LFRIC_OK;
+}
+
+static int download_any(const lfric_vec *v, void *host, int hprec) {
+  if (int rc = require_init()) return rc;
+  LF_CHECK(v && host, LFRIC_E_ARG, "null argument");
+  if (hprec == v->precision) {
+    LF_HIP(hipMemcpyAsync(host, v->d, v->bytes, hipMemcpyDeviceToHost, ctx().stream));
+    LF_HIP(hipStreamSynchronize(ctx().stream));
+    return LFRIC_OK;
+  }
+  void *tmp;
+  const size_t hb = esize(hprec) * static_cast<size_t>(v->n);
+  if (int rc = ws_get(7, hb, &tmp)) return rc;
+  if (int rc = launch_convert(hprec, tmp, v->precision, v->d, v->n, 1.0)) return rc;
+  LF_HIP(hipMemcpyAsync(host, tmp, hb, hipMemcpyDeviceToHost, ctx().stream));
+  LF_HIP(hipStreamSynchronize(ctx().stream));
+  return LFRIC_OK;
+}
+
+int lfric_vec_upload(lfric_vec *v, const double *host) { return upload_any(v, host, LFRIC_R64); }
+int lfric_vec_download(const lfric_vec *v, double *host) { return download_any(v, host, LFRIC_R64); }
+int lfric_vec_upload_r32(lfric_vec *v, const float *host) { return upload_any(v, host, LFRIC_R32); }
+int lfric_vec_download_r32(const lfric_vec *v, float *host) { return download_any(v, host, LFRIC_R32); }
+
+int lfric_vec_size(const lfric_vec *v, int *undf, int *nrhs) {
+  LF_CHECK(v, LFRIC_E_ARG, "null vector");
+  if (undf) *undf = v->fs->undf;
+  if (nrhs) *nrhs = v->nrhs;
+  return LFRIC_OK;
+}
+
+#define SAME_SHAPE(a, b) LF_CHECK((a) && (b) && (a)->n == (b)->n && (a)->precision == (b)->precision, LFRIC_E_ARG, "vector shapes or precisions differ")
+
+int lfric_vec_set_scalar(lfric_vec *v, double s) {
+  if (int rc = require_init()) return rc;
+  LF_CHECK(v, LFRIC_E_ARG, "null vector");
+  return launch_set(v->precision, v->d, v->n, s);
+}
+int lfric_vec_axpy(lfric_vec *y, double a, const lfric_vec *x) {
+  if (int rc = require_init()) return rc;
+  SAME_SHAPE(y, x);
+  return launch_axpy(y->precision, y->d, x->d, y->n, a);
+}
+int lfric_vec_aypx(lfric_vec *y, double a, const lfric_vec *x) {
+  if (int rc = require_init()) return rc;
+  SAME_SHAPE(y, x);
+  return launch_aypx(y->precision, y->d, x->d, y->n, a);
+}
+int lfric_vec_axpby(lfric_vec *z, double a, const lfric_vec *x, double b, const lfric_vec *y) {
+  if (int rc = require_init()) return rc;
+  SAME_SHAPE(z, x);
+  SAME_SHAPE(z, y);
+  return launch_axpby(z->precision, z->d, a, x->d, b, y->d, z->n);
+}
+int lfric_vec_scale(lfric_vec *v, double a) {
+  if (int rc = require_init()) return rc;
+  LF_CHECK(v, LFRIC_E_ARG, "null vector");
+  return launch_scale(v->precision, v->d, v->n, a);
+}
+int lfric_vec_copy(lfric_vec *dst, const lfric_vec *src) {
+  if (int rc = require_init()) return rc;
+  LF_CHECK(dst && src && dst->n == src->n, LFRIC_E_ARG, "vector shapes differ");
+  if (dst->precision == src->precision) return launch_copy(dst->precision, dst->d, src->d, dst->n);
+  return launch_convert(dst->precision, dst->d, src->precision, src->d, dst->n, 1.0);
+}
+int lfric_vec_divide(lfric_vec *z, const lfric_vec *x, const lfric_vec *d) {
+  if (int rc = require_init()) return rc;
+  SAME_SHAPE(z, x);
+  SAME_SHAPE(z, d);
+  return launch_divide(z->precision, z->d, x->d, d->d, z->n);
+}
+int lfric_vec_dot(const lfric_vec *x, const lfric_vec *y, double *out) {
+  if (int rc = require_init()) return rc;
+  SAME_SHAPE(x, y);
+  LF_CHECK(out, LFRIC_E_ARG, "null output");
+  if (int rc = launch_dot(x->precision, x->d, y->d, x->n, &g_state->red0)) return rc;
+  LF_HIP(hipMemcpyAsync(out, &g_state->red0, sizeof(double), hipMemcpyDeviceToHost, ctx().stream));
+  LF_HIP(hipStreamSynchronize(ctx().stream));
+  return LFRIC_OK;
+}
+int lfric_vec_norm(const lfric_vec *x, double *out) {
+  double d = 0;
+  if (int rc = lfric_vec_dot(x, x, &d)) return rc;
+  *out = std::sqrt(d);
+  return LFRIC_OK;
+}
+int lfric_vec_fill_uniform(lfric_vec *v, unsigned long long seed, unsigned long long stream) {
+  if (int rc = require_init()) return rc;
+  LF_CHECK(v, LFRIC_E_ARG, "null vector");
+  return launch_vec_uniform(v, seed, stream);
+}
+
+// ---- operators ------------------------------------------------------------------------------
+int lfric_op_create(const lfric_fs *to, const lfric_fs *from, int precision, lfric_op **out) {
+  if (int rc = require_init()) return rc;
+  LF_CHECK(to && from && out, LFRIC_E_ARG, "null argument");
+  LF_CHECK(precision == LFRIC_R64 || precision == LFRIC_R32, LFRIC_E_UNSUPPORTED, "operator precision");
+  const FunctionSpace &t = *to->fs, &f = *from->fs;
+  LF_CHECK(t.ncells == f.ncells && t.nlayers == f.nlayers && t.cell_of_pos == f.cell_of_pos, LFRIC_E_ARG,
+           "operator spaces must share one mesh and colouring");
+  auto *op = new lfric_op;
+  op->to = to->fs;
+  op->from = from->fs;
+  op->precision = precision;
+  const long long nc3 = static_cast<long long>(t.ncells) * t.nlayers;
+  op->plane_stride = (nc3 + 63) / 64 * 64;
+  op->bytes = esize(precision) * static_cast<size_t>(op->plane_stride) * t.ndf * f.ndf;
+  if (int rc = dmalloc(&op->d, op->bytes)) {
+    delete op;
+    return rc;
+  }
+  *out = op;
+  return LFRIC_OK;
+}
+
+void lfric_op_destroy(lfric_op *op) {
+  if (!op) return;
+  dfree(op->d, op->bytes);
+  delete op;
+}
+
+int lfric_op_upload(lfric_op *op, const double *ls) {
+  if (int rc = require_init()) return rc;
+  LF_CHECK(op && ls, LFRIC_E_ARG, "null argument");
+  const size_t n = static_cast<size_t>(op->to->ncells) * op->to->nlayers * op->to->ndf * op->from->ndf;
+  double *tmp = nullptr;
+  if (int rc = dmalloc(reinterpret_cast<void **>(&tmp), n * sizeof(double))) return rc;
+  LF_HIP(hipMemcpyAsync(tmp, ls, n * sizeof(double), hipMemcpyHostToDevice, ctx().stream));
+  int rc = launch_op_from_reference(op, tmp);
+  dfree(tmp, n * sizeof(double));
+  return rc;
+}
+
+int lfric_op_download(const lfric_op *op, double *ls) {
+  if (int rc = require_init()) return rc;
+  LF_CHECK(op && ls, LFRIC_E_ARG, "null argument");
+  const size_t n = static_cast<size_t>(op->to->ncells) * op->to->nlayers * op->to->ndf * op->from->ndf;
+  double *tmp = nullptr;
+  if (int rc = dmalloc(reinterpret_cast<void **>(&tmp), n * sizeof(double))) return rc;
+  int rc = launch_op_to_reference(op, tmp);
+  if (rc == LFRIC_OK) {
+    hipError_t e = hipMemcpy(ls, tmp, n * sizeof(double), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) rc = fail(LFRIC_E_HIP, hipGetErrorString(e));
+  }
+  dfree(tmp, n * sizeof(double));
+  return rc;
+}
+
+int lfric_op_generate_spd(lfric_op *op, unsigned long long seed, int scaled, double shift) {
+  if (int rc = require_init()) return rc;
+  LF_CHECK(op, LFRIC_E_ARG, "null operator");
+  return launch_gen_spd(op, seed, scaled, shift);
+}
+
+int lfric_op_generate_uniform(lfric_op *op, unsigned long long seed, unsigned long long stream, double lo,
+                              double hi, const double *col_sign) {
+  if (int rc = require_init()) return rc;
+  LF_CHECK(op, LFRIC_E_ARG, "null operator");
+  return launch_gen_uniform(op, seed, stream, lo, hi, col_sign);
+}
+
+int lfric_op_convert(const lfric_op *src, lfric_op *dst) {
+  if (int rc = require_init()) return rc;
+  LF_CHECK(src && dst && src->to == dst->to && src->from == dst->from, LFRIC_E_ARG, "operators differ in shape");
+  return launch_op_convert(src, dst);
+}
+
+static int check_apply(const lfric_op *op, const lfric_vec *x, const lfric_vec *y, bool transpose) {
+  LF_CHECK(op && x && y, LFRIC_E_ARG, "null argument");
+  const auto &xin = transpose ? op->to : op->from;
+  const auto &yout = transpose ? op->from : op->to;
+  LF_CHECK(x->fs->undf == xin->undf && y->fs->undf == yout->undf, LFRIC_E_ARG, "vector does not match operator space");
+  LF_CHECK(x->precision == op->precision && y->precision == op->precision, LFRIC_E_ARG,
+           "vector precision differs from operator precision");
+  LF_CHECK(x->nrhs == 1 && y->nrhs == 1, LFRIC_E_ARG, "single right-hand side expected");
+  LF_CHECK(x->d != y->d, LFRIC_E_ARG, "x and y must differ");
+  return LFRIC_OK;
+}
+
+int lfric_lma_apply(const lfric_op *op, const lfric_vec *x, lfric_vec *y) {
+  if (int rc = require_init()) return rc;
+  if (int rc = check_apply(op, x, y, false)) return rc;
+  return launch_lma_apply(op, x->d, y->d, nullptr);
+}
+
+int lfric_lma_apply_gather(const lfric_op *op, const lfric_vec *x, lfric_vec *y) {
+  if (int rc = require_init()) return rc;
+  if (int rc = check_apply(op, x, y, false)) return rc;
+  return launch_lma_apply_gather(op, x->d, y->d);
+}
+
+int lfric_lma_apply_transpose(const lfric_op *op, const lfric_vec *x, lfric_vec *y) {
+  if (int rc = require_init()) return rc;
+  if (int rc = check_apply(op, x, y, true)) return rc;
+  return launch_lma_transpose(op, x->d, y->d);
+}
+
+int lfric_dg_apply(const lfric_op *op, const lfric_vec *x, lfric_vec *y) {
+  if (int rc = require_init()) return rc;
+  if (int rc = check_apply(op, x, y, false)) return rc;
+  return launch_dg_apply(op, x->d, y->d);
+}
+
+int lfric_mm_diagonal(const lfric_op *op, lfric_vec *diag) {
+  if (int rc = require_init()) return rc;
+  LF_CHECK(op && diag && diag->fs->undf == op->to->undf && diag->precision == op->precision, LFRIC_E_ARG,
+           "diagonal vector does not match operator");
+  return launch_mm_diagonal(op, diag->d);
+}
+
+// ---- preconditioners ----------------------------------------------------------------------------
+int lfric_prec_create_diagonal(const lfric_vec *diag, lfric_prec **out) {
+  LF_CHECK(diag && out, LFRIC_E_ARG, "null argument");
+  *out = new lfric_prec{1, diag};
+  return LFRIC_OK;
+}
+int lfric_prec_create_identity(lfric_prec **out) {
+  LF_CHECK(out, LFRIC_E_ARG, "null argument");
+  *out = new lfric_prec{0, nullptr};
+  return LFRIC_OK;
+}
+int lfric_prec_apply(const lfric_prec *p, const lfric_vec *x, lfric_vec *y) {
+  if (int rc = require_init()) return rc;
+  LF_CHECK(p && x && y && x->n == y->n && x->precision == y->precision, LFRIC_E_ARG, "bad arguments");
+  if (p->kind == 0) return launch_copy(y->precision, y->d, x->d, y->n);
+  LF_CHECK(p->diag->n == x->n && p->diag->precision == x->precision, LFRIC_E_ARG, "diagonal does not match vector");
+  return launch_divide(y->precision, y->d, x->d, p->diag->d, y->n);
+}
+void lfric_prec_destroy(lfric_prec *p) { delete p; }
+
+// ---- solvers ------------------------------------------------------------------------------------
+void lfric_solver_opts_default(lfric_solver_opts *o) {
+  if (!o) return;
+  o->r_tol = 1e-8;
+  o->a_tol = 1e-16;  // sci_mass_matrix_solver_alg_mod.x90:109
+  o->max_iter = 500;
+  o->gcrk = 30;
+  o->monitor_convergence = 1;
+  o->fail_on_non_converged = 0;
+  o->check_every = 8;
+  o->inner_r_tol = 1e-4;
+  o->max_outer = 20;
+}
+
+static int finish_status(const SolverState &s, const lfric_solver_opts &o, lfric_solver_result *res) {
+  int st = LFRIC_OK;
+  if (s.status == LFRIC_NAN || s.status == LFRIC_BREAKDOWN)
+    st = s.status;
+  else if (o.monitor_convergence && !s.converged)
+    st = LFRIC_NOT_CONVERGED;
+  if (res) {
+    res->status = st;
+    res->iterations = s.iter;
+    res->outer_iterations = 0;
+    res->r_nrm_0 = s.r_nrm0;
+    res->r_nrm = s.r_nrm;
+  }
+  if (st == LFRIC_NOT_CONVERGED && !o.fail_on_non_converged) return LFRIC_OK;
+  if (st != LFRIC_OK) set_error(st == LFRIC_NAN ? "NaN in solver residual" : st == LFRIC_BREAKDOWN ? "solver breakdown" : "solver did not converge");
+  return st;
+}
+
+static int prec_diag_ptr(const lfric_prec *P, const lfric_op *A, const void **diag) {
+  *diag = nullptr;
+  if (!P || P->kind == 0) return LFRIC_OK;
+  LF_CHECK(P->diag->fs->undf == A->to->undf && P->diag->precision == A->precision, LFRIC_E_ARG,
+           "preconditioner diagonal does not match the operator");
+  *diag = P->diag->d;
+  return LFRIC_OK;
+}
+
+int lfric_cg_solve(const lfric_op *A, const lfric_prec *P, lfric_vec *x, const lfric_vec *b,
+                   const lfric_solver_opts *opts, lfric_solver_result *res) {
+  if (int rc = require_init()) return rc;
+  LF_CHECK(A && x && b && opts, LFRIC_E_ARG, "null argument");
+  LF_CHECK(A->to == A->from, LFRIC_E_ARG, "CG needs a square operator on one space");
+  if (int rc = check_apply(A, x, b, false)) return rc;
+  const void *diag;
+  if (int rc = prec_diag_ptr(P, A, &diag)) return rc;
+  CgArgs a{A, diag, x->d, b->d, x->n, A->precision, false, opts->r_tol, opts->a_tol, opts->max_iter,
+           opts->monitor_convergence, opts->check_every};
+  SolverState s{};
+  if (int rc = cg_device(a, s)) return rc;
+  int st = finish_status(s, *opts, res);
+  if (res) res->true_rel_res = -1.0;  // not computed here: see lfric_residual_norm
+  return st;
+}
+
+int lfric_residual_norm(const lfric_op *A, const lfric_vec *x, const lfric_vec *b, double *nrm) {
+  if (int rc = require_init()) return rc;
+  if (int rc = check_apply(A, x, b, false)) return rc;
+  LF_CHECK(nrm, LFRIC_E_ARG, "null output");
+  return true_residual(A, x->d, b->d, x->n, x->precision, nrm);
+}
+
+int lfric_ir_cg_solve(const lfric_op *A64, const lfric_op *A32, const lfric_prec *P32, lfric_vec *x,
+                      const lfric_vec *b, const lfric_solver_opts *opts, lfric_solver_result *res) {
+  if (int rc = require_init()) return rc;
+  LF_CHECK(A64 && A32 && x && b && opts, LFRIC_E_ARG, "null argument");
+  LF_CHECK(A64->precision == LFRIC_R64 && A32->precision == LFRIC_R32, LFRIC_E_ARG,
+           "refinement needs an FP64 and an FP32 operator");
+  LF_CHECK(A64->to == A32->to && A64->from == A32->from && A64->to == A64->from, LFRIC_E_ARG,
+           "operators must share one square space");
+  if (int rc = check_apply(A64, x, b, false)) return rc;
+  const void *diag;
+  if (int rc = prec_diag_ptr(P32, A32, &diag)) return rc;
+  const long long n = x->n;
+  void *r64, *b32, *d32;
+  if (int rc = ws_get(4, 8 * static_cast<size_t>(n), &r64)) return rc;
+  if (int rc = ws_get(5, 4 * static_cast<size_t>(n), &b32)) return rc;
+  if (int rc = ws_get(6, 4 * static_cast<size_t>(n), &d32)) return rc;
+  // r = b - A x in FP64 and its norm
+  auto residual = [&](double *nrm) -> int {
+    if (int rc = launch_lma_apply(A64, x->d, r64, nullptr)) return rc;
+    if (int rc = launch_cg_residual(LFRIC_R64, r64, b->d, n)) return rc;
+    if (int rc = launch_dot(LFRIC_R64, r64, r64, n, &g_state->red0)) return rc;
+    LF_HIP(hipMemcpyAsync(g_state_host, g_state, sizeof(SolverState), hipMemcpyDeviceToHost, ctx().stream));
+    LF_HIP(hipStreamSynchronize(ctx().stream));
+    *nrm = std::sqrt(g_state_host->red0);
+    return LFRIC_OK;
+  };
+  double nr0 = 0, nr = 0;
+  if (int rc = residual(&nr0)) return rc;
+  nr = nr0;
+  int total = 0, outer = 0, status = LFRIC_NOT_CONVERGED;
+  if (!(nr0 == nr0)) status = LFRIC_NAN;
+  else if (nr0 < 2.220446049250313e-16) status = LFRIC_OK;
+  while (status == LFRIC_NOT_CONVERGED && outer < opts->max_outer && total < opts->max_iter) {
+    if (nr / nr0 <= opts->r_tol || nr <= opts->a_tol) {
+      status = LFRIC_OK;
+      break;
+    }
+    // inner: A32 d = r / ||r|| in FP32 from d = 0
+    if (int rc = launch_convert(LFRIC_R32, b32, LFRIC_R64, r64, n, 1.0 / nr)) return rc;
+    if (int rc = launch_set(LFRIC_R32, d32, n, 0.0)) return rc;
+    const double want = opts->r_tol * nr0 / nr;  // reduction still needed
+    CgArgs a{A32, diag, d32, b32, n, LFRIC_R32, true, std::max(opts->inner_r_tol, want * 0.5), 0.0,
+             opts->max_iter - total, 1, opts->check_every};
+    SolverState s{};
+    if (int rc = cg_device(a, s)) return rc;
+    total += s.iter;
+    ++outer;
+    if (s.status == LFRIC_NAN) {
+      status = LFRIC_NAN;
+      break;
+    }
+    if (int rc = launch_axpy_mixed(static_cast<double *>(x->d), nr, LFRIC_R32, d32, n)) return rc;
+    if (int rc = residual(&nr)) return rc;
+    if (!(nr == nr)) status = LFRIC_NAN;
+  }
+  if (status == LFRIC_NOT_CONVERGED && (nr / nr0 <= opts->r_tol || nr <= opts->a_tol)) status = LFRIC_OK;
+  if (res) {
+    res->status = status;
+    res->iterations = total;
+    res->outer_iterations = outer;
+    res->r_nrm_0 = nr0;
+    res->r_nrm = nr;
+    res->true_rel_res = nr0 > 0 ? nr / nr0 : 0.0;
+  }
+  if (status == LFRIC_NOT_CONVERGED && !opts->fail_on_non_converged) return LFRIC_OK;
+  if (status != LFRIC_OK) set_error(status == LFRIC_NAN ? "NaN in refinement residual" : "refinement did not converge");
+  return status;
+}
+
+}  // extern "C"

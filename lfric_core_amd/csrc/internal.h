@@ -1,0 +1,110 @@
+// Internal structures of liblfric_hip (not part of the C ABI).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/lfric_hip.h"
+
+namespace lfric {
+
+// ---- errors -----------------------------------------------------------------------------
+void set_error(const std::string &msg);
+int fail(int code, const std::string &msg);
+
+#define LF_HIP(call)                                                                       \
+  do {                                                                                     \
+    hipError_t e_ = (call);                                                                \
+    if (e_ != hipSuccess)                                                                  \
+      return ::lfric::fail(LFRIC_E_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+#define LF_CHECK(cond, code, msg)                   \
+  do {                                              \
+    if (!(cond)) return ::lfric::fail((code), (msg)); \
+  } while (0)
+
+// ---- device context ---------------------------------------------------------------------
+struct Context {
+  int device = -1;
+  hipStream_t stream = nullptr;
+  hipEvent_t t0 = nullptr, t1 = nullptr;
+  // scratch for reductions (partials) and a pinned host mirror for scalars
+  double *d_partials = nullptr;
+  size_t partials_cap = 0;
+  unsigned int *d_counter = nullptr;
+  double *h_pinned = nullptr;
+  long long bytes = 0;
+};
+Context &ctx();
+int require_init();
+int dmalloc(void **p, size_t bytes);
+void dfree(void *p, size_t bytes);
+
+// ---- mesh topology (host) ---------------------------------------------------------------
+// Serial-partition extruded cubed-sphere base-layer connectivity.
+struct Topology {
+  int edge_cells = 0;
+  int ncells = 0;
+  int nverts = 0, nedges = 0;             // unique 2D vertices / edges (= side faces)
+  std::vector<int> cell_next;             // [cell][4] W,S,E,N (1-based)
+  std::vector<int> vert_on_cell;          // [cell][4] SW,SE,NE,NW local vertex ids (1-based)
+  std::vector<int> face_on_cell;          // [cell][6] W,S,E,N,B,T
+  std::vector<int> edge_on_cell;          // [cell][12]
+  std::vector<int> colour_of_cell;        // 1..4
+};
+int build_cubedsphere_topology(int edge_cells, Topology &t);
+int build_dofmap(const Topology &t, int nlayers, int space, int order, std::vector<int> &dofmap,
+                 int &ndf, int &undf);
+
+// ---- function space -------------------------------------------------------------------------
+struct FunctionSpace {
+  int ncells = 0, nlayers = 0, ndf = 0, undf = 0, ncolours = 0, maxcc = 0;
+  int space = -1, order = -1;
+  std::shared_ptr<Topology> topo;          // set for generated cubed-sphere spaces
+  std::vector<int> dofmap;                 // [cell][ndf] 1-based (reference layout)
+  std::vector<int> npc;                    // cells per colour
+  std::vector<int> cic;                    // [colour][maxcc] 1-based cells
+  std::vector<int> colour_start;           // prefix sums of npc, size ncolours+1
+  std::vector<int> cell_of_pos;            // colour-major position -> cell (0-based)
+  // column roles: df is "top" when map(df) = map(bottom)+1 in every cell
+  uint32_t top_mask = 0;
+  std::vector<int> partner;                // bottom df -> top df or -1
+  // device
+  int *d_map = nullptr;                    // [pos][ndf] 0-based column-base dofs
+  uint32_t *d_first = nullptr;             // [pos] bit df: first touch of df's column
+  uint32_t *d_last = nullptr;              // [pos] bit df: last touch of df's column
+  size_t map_bytes = 0, flag_bytes = 0;
+  ~FunctionSpace();
+};
+
+}  // namespace lfric
+
+struct lfric_fs {
+  std::shared_ptr<lfric::FunctionSpace> fs;
+};
+
+struct lfric_vec {
+  std::shared_ptr<lfric::FunctionSpace> fs;
+  int precision = LFRIC_R64;
+  int nrhs = 1;
+  long long n = 0;  // undf * nrhs
+  void *d = nullptr;
+  size_t bytes = 0;
+};
+
+struct lfric_op {
+  std::shared_ptr<lfric::FunctionSpace> to, from;
+  int precision = LFRIC_R64;
+  long long plane_stride = 0;  // elements per (df1, df2) plane, padded
+  void *d = nullptr;           // [df1*ndf2+df2][pos*nl + k]
+  size_t bytes = 0;
+};
+
+struct lfric_prec {
+  int kind = 0;  // 0 identity, 1 diagonal
+  const lfric_vec *diag = nullptr;
+};

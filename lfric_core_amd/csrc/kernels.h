@@ -1,0 +1,64 @@
+// Kernel launchers of liblfric_hip (host side declarations).
+#pragma once
+#include "internal.h"
+
+namespace lfric {
+
+// column roles of a function space's cell dofs, passed by value to kernels
+struct Roles {
+  uint32_t top_mask;
+  int8_t partner[32];
+};
+Roles roles_of(const FunctionSpace &fs);
+
+// device solver state (FP64 scalars stay on the GPU between kernels)
+struct SolverState {
+  double rz, alpha, beta, r_nrm0, r_nrm, r_tol, a_tol, pz;
+  double red0, red1;  // last reduction results
+  int iter, max_iter, done, status, monitor, converged;
+  unsigned int counter;
+  int pad;
+};
+
+constexpr int kRedBlocks = 1024;  // fixed grid of the reduction kernels (deterministic sums)
+
+// generators
+int launch_gen_spd(lfric_op *op, unsigned long long seed, int scaled, double shift);
+int launch_gen_uniform(lfric_op *op, unsigned long long seed, unsigned long long stream,
+                       double lo, double hi, const double *col_sign_host);
+int launch_vec_uniform(lfric_vec *v, unsigned long long seed, unsigned long long stream);
+// layout changes between the reference local_stencil and the device plane layout
+int launch_op_from_reference(lfric_op *op, const double *d_ref);
+int launch_op_to_reference(const lfric_op *op, double *d_ref);
+int launch_op_convert(const lfric_op *src, lfric_op *dst);
+
+// operator kernels
+int launch_lma_apply(const lfric_op *op, const void *x, void *y, const int *d_skip);
+int launch_lma_apply_gather(const lfric_op *op, const void *x, void *y);
+int launch_lma_transpose(const lfric_op *op, const void *x, void *y);
+int launch_dg_apply(const lfric_op *op, const void *x, void *y);
+int launch_mm_diagonal(const lfric_op *op, void *diag);
+
+// vector kernels (precision of the vectors, n elements)
+int launch_set(int prec, void *v, long long n, double s);
+int launch_axpy(int prec, void *y, const void *x, long long n, double a);
+int launch_aypx(int prec, void *y, const void *x, long long n, double a);
+int launch_axpby(int prec, void *z, double a, const void *x, double b, const void *y, long long n);
+int launch_scale(int prec, void *v, long long n, double a);
+int launch_copy(int prec, void *dst, const void *src, long long n);
+int launch_divide(int prec, void *z, const void *x, const void *d, long long n);
+int launch_convert(int pdst, void *dst, int psrc, const void *src, long long n, double scale);
+int launch_axpy_mixed(double *y, double a, int prec_x, const void *x, long long n);
+int launch_dot(int prec, const void *x, const void *y, long long n, double *d_out);
+
+// CG building blocks on a SolverState
+int launch_cg_residual(int prec, void *r, const void *b, long long n);  // r = b - r
+int launch_cg_init_norm(int prec, const void *r, long long n, SolverState *st);
+int launch_cg_init_prec(int prec, const void *r, void *z, void *p, const void *diag, long long n,
+                        SolverState *st);
+int launch_cg_dot_pz(int prec, const void *p, const void *z, long long n, SolverState *st);
+int launch_cg_update(int prec, void *x, void *r, void *z, const void *p, const void *diag,
+                     long long n, SolverState *st);
+int launch_cg_aypx(int prec, void *p, const void *z, long long n, SolverState *st);
+
+}  // namespace lfric

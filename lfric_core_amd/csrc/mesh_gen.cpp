@@ -1,0 +1,380 @@
+// Host-side construction of the serial-partition extruded cubed-sphere function spaces:
+// panel topology, local vertex numbering, 3D entity connectivity, column-contiguous dofmaps
+// and the closed-form 4-colouring.  Runs once at set-up, like the reference's mesh and
+// function-space constructors; the result is uploaded once and stays on the device.
+//
+// The numbering must equal the reference's bit for bit (it fixes which dof is which), so the
+// steps follow the reference algorithms:
+//   panel adjacency and vertex numbering  mesh_tools/source/support/gencube_ps_mod.F90:336-720
+//   panel orientation (PANEL_ROTATIONS)   gencube_ps_mod.F90:69, 1561-1601
+//   local vertex ids (first encounter)    infrastructure/source/mesh/local_mesh_mod.f90:381-406
+//   3D face / edge ids                    infrastructure/source/mesh/mesh_constructor_helper_functions_mod.F90:266-438
+//   dofmap                                infrastructure/source/function_space/function_space_constructor_helper_functions_mod.F90:1715-2300
+//   colouring                             infrastructure/source/mesh/mesh_colouring_mod.F90:151-308, 703-837
+#include <algorithm>
+#include <array>
+#include <cstring>
+
+#include "internal.h"
+
+namespace lfric {
+namespace {
+
+enum Dir { DW = 0, DS = 1, DE = 2, DN = 3 };       // cell_next slots
+enum Corner { CSW = 0, CSE = 1, CNE = 2, CNW = 3 };  // vertex slots
+
+struct Panelled {
+  int n, cpp, nc;
+  std::vector<int> next;  // [cell-1][dir], 1-based cells
+  std::vector<int> vert;  // [cell-1][corner], 1-based global vertex ids
+  int &nb(int cell, int d) { return next[(cell - 1) * 4 + d]; }
+  int &vx(int cell, int c) { return vert[(cell - 1) * 4 + c]; }
+  // first cell of row/column along a panel side, i = 1..n
+  int side_cell(int panel, int side, int i) const {
+    int b = (panel - 1) * cpp;
+    switch (side) {
+      case DW: return b + n * (i - 1) + 1;
+      case DS: return b + cpp - n + i;
+      case DE: return b + n * i;
+      default: return b + i;
+    }
+  }
+};
+
+// Neighbour panel across each side, the side of that panel it meets and whether the running
+// index reverses.  Panels: 1-4 around the equator, 5 on top, 6 below; cells numbered from the
+// NW corner of each panel, row by row.
+struct PanelLink {
+  int panel, side, reversed;
+};
+constexpr PanelLink kLinks[6][4] = {
+    // W              S              E              N
+    {{4, DE, 0}, {6, DW, 1}, {2, DW, 0}, {5, DW, 0}},
+    {{1, DE, 0}, {6, DN, 0}, {3, DW, 0}, {5, DS, 0}},
+    {{2, DE, 0}, {6, DE, 0}, {4, DW, 0}, {5, DE, 1}},
+    {{3, DE, 0}, {6, DS, 1}, {1, DW, 0}, {5, DN, 1}},
+    {{1, DN, 0}, {2, DN, 0}, {3, DN, 1}, {4, DN, 1}},
+    {{1, DS, 1}, {4, DS, 1}, {3, DS, 0}, {2, DS, 0}},
+};
+
+void panel_adjacency(Panelled &P) {
+  const int n = P.n;
+  for (int c = 1; c <= P.nc; ++c) {
+    P.nb(c, DW) = c - 1;
+    P.nb(c, DS) = c + n;
+    P.nb(c, DE) = c + 1;
+    P.nb(c, DN) = c - n;
+  }
+  for (int p = 1; p <= 6; ++p)
+    for (int s = 0; s < 4; ++s) {
+      const PanelLink &L = kLinks[p - 1][s];
+      for (int i = 1; i <= n; ++i)
+        P.nb(P.side_cell(p, s, i), s) = P.side_cell(L.panel, L.side, L.reversed ? n + 1 - i : i);
+    }
+}
+
+void vertex_numbering(Panelled &P) {
+  const int n = P.n, cpp = P.cpp;
+  std::fill(P.vert.begin(), P.vert.end(), 0);
+  auto nb = [&](int c, int d) { return P.nb(c, d); };
+  int id = 1;
+  // equatorial panels: NW corner of every cell takes the cell's id, NE from the E neighbour
+  for (int c = 1; c <= 4 * cpp; ++c) P.vx(c, CNW) = id++;
+  for (int c = 1; c <= 4 * cpp; ++c) P.vx(c, CNE) = P.vx(nb(c, DE), CNW);
+  for (int p = 1; p <= 4; ++p)
+    for (int c = (p - 1) * cpp + 1; c <= p * cpp - n; ++c) {
+      P.vx(c, CSW) = P.vx(nb(c, DS), CNW);
+      P.vx(c, CSE) = P.vx(nb(c, DS), CNE);
+    }
+  // interior vertices of the top panel
+  for (int r = 0; r <= n - 2; ++r)
+    for (int c = 4 * cpp + r * n + 1; c <= 4 * cpp + r * n + n - 1; ++c) {
+      int v = id++;
+      P.vx(c, CSE) = v;
+      P.vx(nb(c, DE), CSW) = v;
+      P.vx(nb(c, DS), CNE) = v;
+      P.vx(nb(nb(c, DS), DE), CNW) = v;
+    }
+  // bottom panel: NW of every cell, then SW below the first rows, then the last row
+  for (int c = 5 * cpp + 1; c <= 6 * cpp; ++c) P.vx(c, CNW) = id++;
+  for (int c = 5 * cpp + 1; c <= 6 * cpp - n; ++c) P.vx(c, CSW) = P.vx(nb(c, DS), CNW);
+  for (int c = 6 * cpp - n + 1; c <= 6 * cpp; ++c) P.vx(c, CSW) = id++;
+  // last row of panel 3 and its SE corner
+  for (int c = 3 * cpp - n + 1; c <= 3 * cpp; ++c) {
+    P.vx(c, CSW) = id;
+    P.vx(nb(c, DW), CSE) = id;
+    ++id;
+  }
+  P.vx(3 * cpp, CSE) = id;
+  P.vx(nb(3 * cpp, DE), CSW) = id;
+  // seams between panels
+  for (int c = cpp - n + 1; c <= cpp; ++c) {
+    P.vx(c, CSW) = P.vx(nb(c, DS), CSW);
+    P.vx(c, CSE) = P.vx(nb(c, DS), CNW);
+  }
+  for (int c = 1; c <= n; ++c) {
+    P.vx(nb(c, DN), CNW) = P.vx(c, CNW);
+    P.vx(nb(c, DN), CSW) = P.vx(c, CNE);
+  }
+  for (int c = 2 * cpp + 1; c <= 2 * cpp + n; ++c) {
+    P.vx(nb(c, DN), CSE) = P.vx(c, CNW);
+    P.vx(nb(c, DN), CNE) = P.vx(c, CNE);
+  }
+  for (int c = 3 * cpp - n + 1; c <= 3 * cpp; ++c) {
+    P.vx(nb(c, DS), CNE) = P.vx(c, CSW);
+    P.vx(nb(c, DS), CSE) = P.vx(c, CSE);
+  }
+  for (int c = 3 * cpp + 1; c <= 3 * cpp + n; ++c) {
+    P.vx(nb(c, DN), CNE) = P.vx(c, CNW);
+    P.vx(nb(c, DN), CNW) = P.vx(c, CNE);
+  }
+  for (int r = 0; r <= n - 1; ++r)
+    for (int c = 5 * cpp + 1 + r * n; c <= 5 * cpp + r * n + n - 1; ++c) {
+      P.vx(c, CNE) = P.vx(nb(c, DE), CNW);
+      P.vx(c, CSE) = P.vx(nb(c, DE), CSW);
+    }
+  for (int c = 6 * cpp - n + 1; c <= 6 * cpp; ++c) {
+    P.vx(nb(c, DS), CSW) = P.vx(c, CSE);
+    P.vx(nb(c, DS), CSE) = P.vx(c, CSW);
+  }
+  for (int c = 2 * cpp - n + 1; c <= 2 * cpp; ++c) {
+    P.vx(c, CSW) = P.vx(nb(c, DS), CNW);
+    P.vx(c, CSE) = P.vx(nb(c, DS), CNE);
+  }
+  for (int c = cpp + 1; c <= cpp + n; ++c) {
+    P.vx(nb(c, DN), CSW) = P.vx(c, CNW);
+    P.vx(nb(c, DN), CSE) = P.vx(c, CNE);
+  }
+}
+
+// Panels 3 and 4 are turned by one slot (cshift +1), panel 5 by one slot the other way.
+void orient_panels(Panelled &P) {
+  constexpr int rot[6] = {0, 0, 1, 1, -1, 0};
+  for (int p = 1; p <= 6; ++p) {
+    if (!rot[p - 1]) continue;
+    for (int c = (p - 1) * P.cpp + 1; c <= p * P.cpp; ++c)
+      for (std::vector<int> *arr : {&P.vert, &P.next}) {
+        int *a = &(*arr)[(c - 1) * 4];
+        std::array<int, 4> t{a[0], a[1], a[2], a[3]};
+        for (int i = 0; i < 4; ++i) a[i] = t[(i + (rot[p - 1] > 0 ? 1 : 3)) % 4];
+      }
+  }
+}
+
+int closed_form_colour(int n, int cell) {
+  // cell_colour_order_per_panel(column,row,panel), even n (mesh_colouring_mod.F90:242-247)
+  static const int even[6][2][2] = {{{1, 3}, {2, 4}}, {{3, 1}, {2, 4}}, {{3, 1}, {4, 2}},
+                                    {{1, 3}, {4, 2}}, {{3, 4}, {2, 1}}, {{2, 1}, {3, 4}}};
+  static const int odd[6][2][2] = {{{1, 3}, {2, 4}}, {{4, 2}, {1, 3}}, {{1, 3}, {2, 4}},
+                                   {{4, 2}, {1, 3}}, {{3, 4}, {2, 1}}, {{3, 4}, {2, 1}}};
+  const int cpp = n * n;
+  int panel = (cell - 1) / cpp;
+  int rel = cell - cpp * panel;
+  int column = 1 - (rel % n) % 2;        // 0-based column slot
+  int row = ((rel - 1) / n) % 2;         // 0-based row slot
+  return (n % 2 == 0 ? even : odd)[panel][column][row];
+}
+
+// No two cells of one colour may share an edge or a vertex (mesh_colouring_mod.F90:765-837,
+// checked here directly on shared vertices).
+bool colouring_valid(const Topology &t) {
+  std::vector<int> seen(static_cast<size_t>(t.nverts + 1) * 4, 0);
+  for (int c = 1; c <= t.ncells; ++c) {
+    int col = t.colour_of_cell[c - 1] - 1;
+    for (int v = 0; v < 4; ++v) {
+      int &s = seen[static_cast<size_t>(t.vert_on_cell[(c - 1) * 4 + v]) * 4 + col];
+      if (s) return false;
+      s = c;
+    }
+  }
+  return true;
+}
+
+}  // namespace
+
+int build_cubedsphere_topology(int n, Topology &t) {
+  if (n < 2) return fail(LFRIC_E_ARG, "cubed sphere needs edge_cells >= 2");
+  Panelled P{n, n * n, 6 * n * n, {}, {}};
+  P.next.assign(static_cast<size_t>(P.nc) * 4, 0);
+  P.vert.assign(static_cast<size_t>(P.nc) * 4, 0);
+  panel_adjacency(P);
+  vertex_numbering(P);
+  orient_panels(P);
+
+  t.edge_cells = n;
+  t.ncells = P.nc;
+  t.cell_next = P.next;
+  // local vertex ids in order of first appearance
+  {
+    int maxg = *std::max_element(P.vert.begin(), P.vert.end());
+    std::vector<int> lid(static_cast<size_t>(maxg) + 1, 0);
+    t.vert_on_cell.resize(P.vert.size());
+    int nv = 0;
+    for (size_t i = 0; i < P.vert.size(); ++i) {
+      int &l = lid[P.vert[i]];
+      if (!l) l = ++nv;
+      t.vert_on_cell[i] = l;
+    }
+    t.nverts = nv;
+  }
+  // 3D connectivity of the base layer: side faces and horizontal edges are numbered when
+  // first met and shared with the neighbour across that side; B/T faces and bottom/top edge
+  // pairs take consecutive ids; vertical edges are shared round their vertex.
+  const int nc = t.ncells;
+  t.face_on_cell.assign(static_cast<size_t>(nc) * 6, 0);
+  t.edge_on_cell.assign(static_cast<size_t>(nc) * 12, 0);
+  auto NX = [&](int c, int s) { return t.cell_next[(c - 1) * 4 + s]; };
+  auto F = [&](int c, int f) -> int & { return t.face_on_cell[(c - 1) * 6 + f]; };
+  auto E = [&](int c, int e) -> int & { return t.edge_on_cell[(c - 1) * 12 + e]; };
+  auto V = [&](int c, int v) { return t.vert_on_cell[(c - 1) * 4 + v]; };
+  int fid = 1, nside = 0;
+  for (int c = 1; c <= nc; ++c) {
+    for (int s = 0; s < 4; ++s) {
+      if (F(c, s)) continue;
+      F(c, s) = fid;
+      int o = NX(c, s);
+      if (o)
+        for (int so = 0; so < 4; ++so)
+          if (NX(o, so) == c) F(o, so) = fid;
+      ++fid;
+      ++nside;
+    }
+    F(c, 4) = fid++;
+    F(c, 5) = fid++;
+  }
+  int eid = 1, nh = 0;
+  for (int c = 1; c <= nc; ++c) {
+    for (int s = 0; s < 4; ++s) {
+      if (E(c, s)) continue;
+      E(c, s) = eid;
+      E(c, s + 8) = eid + 1;
+      int o = NX(c, s);
+      if (o)
+        for (int so = 0; so < 4; ++so)
+          if (NX(o, so) == c) {
+            E(o, so) = eid;
+            E(o, so + 8) = eid + 1;
+          }
+      eid += 2;
+      ++nh;
+    }
+    for (int v = 0; v < 4; ++v) {
+      if (E(c, 4 + v)) continue;
+      E(c, 4 + v) = eid;
+      const int vert = V(c, v);
+      for (int s = 0; s < 4; ++s) {
+        int o = NX(c, s);
+        if (!o) continue;
+        for (int vo = 0; vo < 4; ++vo) {
+          if (V(o, vo) != vert) continue;
+          E(o, 4 + vo) = eid;
+          for (int s2 = 0; s2 < 4; ++s2) {
+            int o2 = NX(o, s2);
+            if (!o2) continue;
+            for (int v2 = 0; v2 < 4; ++v2)
+              if (V(o2, v2) == vert) E(o2, 4 + v2) = eid;
+          }
+        }
+      }
+      ++eid;
+    }
+  }
+  t.nedges = nh;
+  (void)nside;
+  t.colour_of_cell.resize(nc);
+  for (int c = 1; c <= nc; ++c) t.colour_of_cell[c - 1] = closed_form_colour(n, c);
+  if (!colouring_valid(t))
+    return fail(LFRIC_E_UNSUPPORTED,
+                "closed-form cubed-sphere colouring does not hold for edge_cells=" + std::to_string(n));
+  return LFRIC_OK;
+}
+
+// Entity dof counts for the spaces built here (function_space_constructor_helper_functions_mod.F90:
+// 258-266 W0, 284-295 W2, 391-400 W3).
+namespace {
+struct EntityDofs {
+  int vert, edge_h, edge_v, face_h, face_v, vol, cell;
+};
+bool entity_dofs(int space, int k, EntityDofs &d) {
+  switch (space) {
+    case LFRIC_W0: d = {1, k, k, k * k, k * k, k * k * k, (k + 2) * (k + 2) * (k + 2)}; return true;
+    case LFRIC_W2:
+      d = {0, 0, 0, (k + 1) * (k + 1), (k + 1) * (k + 1), 3 * k * (k + 1) * (k + 1),
+           2 * (k + 2) * (k + 1) * (k + 1) + (k + 1) * (k + 1) * (k + 2)};
+      return true;
+    case LFRIC_W3: d = {0, 0, 0, 0, 0, (k + 1) * (k + 1) * (k + 1), (k + 1) * (k + 1) * (k + 1)}; return true;
+  }
+  return false;
+}
+}  // namespace
+
+int build_dofmap(const Topology &t, int nl, int space, int order, std::vector<int> &dofmap,
+                 int &ndf, int &undf) {
+  EntityDofs d{};
+  if (!entity_dofs(space, order, d) || order < 0 || order > 1 || (space != LFRIC_W0 && order != 0))
+    return fail(LFRIC_E_UNSUPPORTED, "function space / element order not supported");
+  const int nc = t.ncells, nv2 = t.nverts, ne2 = t.nedges;
+  // per-entity column bases (0 = unassigned); top entities are bottom + 1
+  std::vector<int> vcol(static_cast<size_t>(2 * nv2 + 1) * std::max(d.vert, 1), 0);
+  std::vector<int> ecol(static_cast<size_t>(2 * ne2 + nv2 + 1) * std::max({d.edge_h, d.edge_v, 1}), 0);
+  std::vector<int> fcol(static_cast<size_t>(ne2 + 2 * nc + 1) * std::max({d.face_h, d.face_v, 1}), 0);
+  std::vector<int> ccol(static_cast<size_t>(nc + 1) * std::max(d.vol, 1), 0);
+  const int sv = std::max(d.vert, 1), se = std::max({d.edge_h, d.edge_v, 1});
+  const int sf = std::max({d.face_h, d.face_v, 1}), sc = std::max(d.vol, 1);
+  auto VC = [&](int e, int i) -> int & { return vcol[static_cast<size_t>(e) * sv + i]; };
+  auto EC = [&](int e, int i) -> int & { return ecol[static_cast<size_t>(e) * se + i]; };
+  auto FC = [&](int e, int i) -> int & { return fcol[static_cast<size_t>(e) * sf + i]; };
+  auto CC = [&](int e, int i) -> int & { return ccol[static_cast<size_t>(e) * sc + i]; };
+  auto face = [&](int c, int f) { return t.face_on_cell[(c - 1) * 6 + f]; };
+  auto edge = [&](int c, int e) { return t.edge_on_cell[(c - 1) * 12 + e]; };
+  auto vert3 = [&](int c, int v) { return v < 4 ? t.vert_on_cell[(c - 1) * 4 + v] : t.vert_on_cell[(c - 1) * 4 + v - 4] + nv2; };
+
+  int next = 1;
+  for (int c = 1; c <= nc; ++c) {
+    for (int i = 0; i < d.vol; ++i) { CC(c, i) = next; next += nl; }
+    for (int f = 0; f < 4; ++f)
+      if (!FC(face(c, f), 0))
+        for (int i = 0; i < d.face_h; ++i) { FC(face(c, f), i) = next; next += nl; }
+    // bottom/top faces share one column of nl+1 dofs per face dof
+    for (int i = 0; i < d.face_v; ++i) {
+      FC(face(c, 4), i) = next;
+      FC(face(c, 5), i) = next + 1;
+      next += nl + 1;
+    }
+    for (int e = 0; e < 4; ++e)
+      if (!EC(edge(c, e), 0))
+        for (int i = 0; i < d.edge_h; ++i) {
+          EC(edge(c, e), i) = next;
+          EC(edge(c, e + 8), i) = next + 1;
+          next += nl + 1;
+        }
+    for (int e = 4; e < 8; ++e)
+      if (!EC(edge(c, e), 0))
+        for (int i = 0; i < d.edge_v; ++i) { EC(edge(c, e), i) = next; next += nl; }
+    for (int v = 0; v < 4; ++v)
+      if (!VC(vert3(c, v), 0))
+        for (int i = 0; i < d.vert; ++i) {
+          VC(vert3(c, v), i) = next;
+          VC(vert3(c, v + 4), i) = next + 1;
+          next += nl + 1;
+        }
+  }
+  undf = next - 1;
+  ndf = d.cell;
+  dofmap.assign(static_cast<size_t>(nc) * ndf, 0);
+  for (int c = 1; c <= nc; ++c) {
+    int *row = &dofmap[static_cast<size_t>(c - 1) * ndf];
+    int k = 0;
+    for (int i = 0; i < d.vol; ++i) row[k++] = CC(c, i);
+    for (int f = 0; f < 6; ++f)
+      for (int i = 0; i < (f < 4 ? d.face_h : d.face_v); ++i) row[k++] = FC(face(c, f), i);
+    for (int e = 0; e < 12; ++e)
+      for (int i = 0; i < ((e < 4 || e >= 8) ? d.edge_h : d.edge_v); ++i) row[k++] = EC(edge(c, e), i);
+    for (int v = 0; v < 8; ++v)
+      for (int i = 0; i < d.vert; ++i) row[k++] = VC(vert3(c, v), i);
+    if (k != ndf) return fail(LFRIC_E_STATE, "dofmap entity count mismatch");
+  }
+  return LFRIC_OK;
+}
+
+}  // namespace lfric

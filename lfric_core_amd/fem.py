@@ -1,0 +1,224 @@
+"""Function spaces, field vectors and LMA operators resident on the GPU."""
+import ctypes
+
+import numpy as np
+
+from ._lib import R32, R64, W0, check, init, lib, ptr
+
+
+class FunctionSpace:
+    """Mesh + function space inputs of the hot path: the column-contiguous dofmap
+    (infrastructure/source/function_space/function_space_mod.F90:733-757) and the colour map
+    (infrastructure/source/mesh/mesh_mod.F90:2202-2245), held on the device."""
+
+    def __init__(self, handle):
+        self._h = ctypes.c_void_p(handle) if not isinstance(handle, ctypes.c_void_p) else handle
+        n = [ctypes.c_int() for _ in range(5)]
+        check(lib().lfric_fs_info(self._h, *[ctypes.byref(v) for v in n]))
+        self.ncells, self.nlayers, self.ndf, self.undf, self.ncolours = [v.value for v in n]
+
+    @classmethod
+    def cubedsphere(cls, edge_cells, nlayers, space=W0, order=0):
+        init()
+        h = ctypes.c_void_p()
+        check(lib().lfric_fs_create_cubedsphere(edge_cells, nlayers, space, order, ctypes.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def from_dofmap(cls, nlayers, undf, dofmap, ncells_per_colour, cells_in_colour):
+        """dofmap (ncells, ndf) 1-based; cells_in_colour (ncolours, maxcc) 1-based."""
+        init()
+        dm = np.ascontiguousarray(dofmap, np.int32)
+        npc = np.ascontiguousarray(ncells_per_colour, np.int32)
+        cic = np.ascontiguousarray(cells_in_colour, np.int32)
+        h = ctypes.c_void_p()
+        check(lib().lfric_fs_create(dm.shape[0], nlayers, dm.shape[1], undf, ptr(dm), cic.shape[0], ptr(npc),
+                                    ptr(cic), cic.shape[1], ctypes.byref(h)))
+        return cls(h)
+
+    def sibling(self, space, order=0):
+        h = ctypes.c_void_p()
+        check(lib().lfric_fs_create_sibling(self._h, space, order, ctypes.byref(h)))
+        return FunctionSpace(h)
+
+    @property
+    def ncell_3d(self):
+        return self.ncells * self.nlayers
+
+    def dofmap(self):
+        dm = np.zeros((self.ncells, self.ndf), np.int32)
+        check(lib().lfric_fs_get_dofmap(self._h, ptr(dm)))
+        return dm
+
+    def colouring(self):
+        npc = np.zeros(self.ncolours, np.int32)
+        check(lib().lfric_fs_get_colouring(self._h, ptr(npc), None))
+        cic = np.zeros((self.ncolours, int(npc.max())), np.int32)
+        check(lib().lfric_fs_get_colouring(self._h, ptr(npc), ptr(cic)))
+        return npc, cic
+
+    def __del__(self):
+        if getattr(self, "_h", None) and lib is not None:
+            try:
+                lib().lfric_fs_destroy(self._h)
+            except Exception:
+                pass
+            self._h = None
+
+
+class FieldVector:
+    """abstract_vector_type on the device (infrastructure/source/field/vector_mod.F90:17-59).
+    precision R64 mirrors field_vector_type, R32 mirrors r_solver_field_vector_type (FP32
+    data, FP64-accumulated dot and norm)."""
+
+    def __init__(self, fs, precision=R64, nrhs=1):
+        self.fs = fs
+        self.precision = precision
+        self.nrhs = nrhs
+        self._h = ctypes.c_void_p()
+        check(lib().lfric_vec_create(fs._h, precision, nrhs, ctypes.byref(self._h)))
+
+    # import_field / export_field (sci_field_vector_mod.x90:294-334)
+    def import_field(self, data):
+        a = np.ascontiguousarray(data, np.float64).ravel()
+        assert a.size == self.fs.undf * self.nrhs
+        check(lib().lfric_vec_upload(self._h, ptr(a)))
+        return self
+
+    def export_field(self):
+        out = np.zeros(self.fs.undf * self.nrhs, np.float64)
+        check(lib().lfric_vec_download(self._h, ptr(out)))
+        return out
+
+    def export_raw(self):
+        if self.precision == R32:
+            out = np.zeros(self.fs.undf * self.nrhs, np.float32)
+            check(lib().lfric_vec_download_r32(self._h, ptr(out)))
+            return out
+        return self.export_field()
+
+    def import_raw(self, data):
+        if self.precision == R32:
+            a = np.ascontiguousarray(data, np.float32).ravel()
+            check(lib().lfric_vec_upload_r32(self._h, ptr(a)))
+            return self
+        return self.import_field(data)
+
+    def set_scalar(self, scalar):
+        check(lib().lfric_vec_set_scalar(self._h, float(scalar)))
+
+    def axpy(self, alpha, x):
+        check(lib().lfric_vec_axpy(self._h, float(alpha), x._h))
+
+    def aypx(self, alpha, x):
+        check(lib().lfric_vec_aypx(self._h, float(alpha), x._h))
+
+    def axpby(self, a, x, b, y):
+        check(lib().lfric_vec_axpby(self._h, float(a), x._h, float(b), y._h))
+
+    def scale(self, alpha):
+        check(lib().lfric_vec_scale(self._h, float(alpha)))
+
+    def copy(self, source):
+        check(lib().lfric_vec_copy(self._h, source._h))
+
+    def divide(self, x, d):
+        check(lib().lfric_vec_divide(self._h, x._h, d._h))
+
+    def dot(self, x):
+        out = ctypes.c_double()
+        check(lib().lfric_vec_dot(self._h, x._h, ctypes.byref(out)))
+        return out.value
+
+    def norm(self):
+        out = ctypes.c_double()
+        check(lib().lfric_vec_norm(self._h, ctypes.byref(out)))
+        return out.value
+
+    def duplicate(self):
+        return FieldVector(self.fs, self.precision, self.nrhs)
+
+    def vector_size(self):
+        return self.fs.undf * self.nrhs
+
+    def fill_uniform(self, seed, stream=3):
+        check(lib().lfric_vec_fill_uniform(self._h, seed, stream))
+        return self
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            try:
+                lib().lfric_vec_destroy(self._h)
+            except Exception:
+                pass
+            self._h = None
+
+
+class LmaOperator:
+    """operator_type (infrastructure/source/operator/operator_mod.t90:31-137): one dense
+    ndf_to x ndf_from element matrix per 3D cell, held on the device; apply() is the invoke of
+    setval_c + matrix_vector_kernel_type (sci_mass_matrix_operator_alg_mod.x90:98-100)."""
+
+    def __init__(self, fs_to, fs_from=None, precision=R64):
+        self.fs_to = fs_to
+        self.fs_from = fs_from or fs_to
+        self.precision = precision
+        self._h = ctypes.c_void_p()
+        check(lib().lfric_op_create(self.fs_to._h, self.fs_from._h, precision, ctypes.byref(self._h)))
+
+    @property
+    def shape(self):
+        return (self.fs_to.ncell_3d, self.fs_to.ndf, self.fs_from.ndf)
+
+    def upload(self, local_stencil):
+        """local_stencil (ncell_3d, ndf_to, ndf_from) as the reference holds it."""
+        a = np.asarray(local_stencil, np.float64)
+        assert a.shape == self.shape
+        flat = np.ascontiguousarray(a.transpose(2, 1, 0)).ravel()  # Fortran order
+        check(lib().lfric_op_upload(self._h, ptr(flat)))
+        return self
+
+    def download(self):
+        n1, n2 = self.fs_to.ndf, self.fs_from.ndf
+        flat = np.zeros(self.fs_to.ncell_3d * n1 * n2, np.float64)
+        check(lib().lfric_op_download(self._h, ptr(flat)))
+        return flat.reshape((n2, n1, self.fs_to.ncell_3d)).transpose(2, 1, 0)
+
+    def generate_spd(self, seed, scaled=False, shift=0.1):
+        check(lib().lfric_op_generate_spd(self._h, seed, int(scaled), float(shift)))
+        return self
+
+    def generate_uniform(self, seed, stream, lo, hi, col_sign=None):
+        s = None if col_sign is None else ptr(np.ascontiguousarray(col_sign, np.float64))
+        check(lib().lfric_op_generate_uniform(self._h, seed, stream, float(lo), float(hi), s))
+        return self
+
+    def convert(self, precision):
+        out = LmaOperator(self.fs_to, self.fs_from, precision)
+        check(lib().lfric_op_convert(self._h, out._h))
+        return out
+
+    def apply(self, x, y):
+        """y = A x (abstract_linear_operator_type%apply)."""
+        check(lib().lfric_lma_apply(self._h, x._h, y._h))
+
+    def apply_gather(self, x, y):
+        check(lib().lfric_lma_apply_gather(self._h, x._h, y._h))
+
+    def apply_transpose(self, x, y):
+        check(lib().lfric_lma_apply_transpose(self._h, x._h, y._h))
+
+    def apply_dg(self, x, y):
+        check(lib().lfric_dg_apply(self._h, x._h, y._h))
+
+    def mm_diagonal(self, diag):
+        check(lib().lfric_mm_diagonal(self._h, diag._h))
+        return diag
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            try:
+                lib().lfric_op_destroy(self._h)
+            except Exception:
+                pass
+            self._h = None

@@ -1,0 +1,127 @@
+"""Preconditioners and iterative solvers over the device-resident vectors and operators
+(components/science/source/solver/sci_iterative_solver_mod.f90,
+sci_preconditioner_mod.f90, components/science/source/algorithm/solver/)."""
+import ctypes
+from dataclasses import dataclass
+
+from ._lib import (LFRIC_NOT_CONVERGED, LFRIC_OK, SolverOpts, SolverRes, check, lib)
+
+
+class NullPreconditioner:
+    """null_preconditioner_type: y = x."""
+
+    def __init__(self):
+        self._h = ctypes.c_void_p()
+        check(lib().lfric_prec_create_identity(ctypes.byref(self._h)))
+
+    def apply(self, x, y):
+        check(lib().lfric_prec_apply(self._h, x._h, y._h))
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().lfric_prec_destroy(self._h)
+            self._h = None
+
+
+class DiagonalPreconditioner:
+    """diagonal_preconditioner_type (sci_diagonal_preconditioner_alg_mod.x90:71-106):
+    y = x / diag, diag from mm_diagonal of the operator."""
+
+    def __init__(self, diag):
+        self.diag = diag  # keep the vector alive
+        self._h = ctypes.c_void_p()
+        check(lib().lfric_prec_create_diagonal(diag._h, ctypes.byref(self._h)))
+
+    @classmethod
+    def from_operator(cls, op):
+        from .fem import FieldVector
+        d = FieldVector(op.fs_to, op.precision)
+        op.mm_diagonal(d)
+        return cls(d)
+
+    def apply(self, x, y):
+        check(lib().lfric_prec_apply(self._h, x._h, y._h))
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().lfric_prec_destroy(self._h)
+            self._h = None
+
+
+@dataclass
+class SolverOptions:
+    r_tol: float = 1e-8
+    a_tol: float = 1e-16
+    max_iter: int = 500
+    gcrk: int = 30
+    monitor_convergence: bool = True
+    fail_on_non_converged: bool = False
+    check_every: int = 8
+    inner_r_tol: float = 1e-4
+    max_outer: int = 20
+
+    def c(self):
+        return SolverOpts(self.r_tol, self.a_tol, self.max_iter, self.gcrk, int(self.monitor_convergence),
+                          int(self.fail_on_non_converged), self.check_every, self.inner_r_tol, self.max_outer)
+
+
+@dataclass
+class SolverResult:
+    status: int
+    iterations: int
+    outer_iterations: int
+    r_nrm_0: float
+    r_nrm: float
+    true_rel_res: float
+
+
+class _Solver:
+    def __init__(self, lin_op, prec, r_tol, a_tol, max_iter, monitor_convergence=True,
+                 fail_on_non_converged=False, **kw):
+        # the reference stores pointers to lin_op and prec, not copies (sci_iterative_solver_mod.f90:478-479)
+        self.lin_op, self.prec = lin_op, prec
+        self.opts = SolverOptions(r_tol=r_tol, a_tol=a_tol, max_iter=max_iter,
+                                  monitor_convergence=monitor_convergence,
+                                  fail_on_non_converged=fail_on_non_converged, **kw)
+        self.result = None
+
+    def _finish(self, rc, res):
+        self.result = SolverResult(res.status, res.iterations, res.outer_iterations, res.r_nrm_0, res.r_nrm,
+                                   res.true_rel_res)
+        check(rc, allow=(LFRIC_OK,) if self.opts.fail_on_non_converged else (LFRIC_OK, LFRIC_NOT_CONVERGED))
+        return self.result
+
+
+class ConjugateGradient(_Solver):
+    """conjugate_gradient_type(lin_op, prec, r_tol, a_tol, max_iter, monitor_convergence,
+    fail_on_non_converged) — apply(x, b) solves A x = b from the initial guess in x."""
+
+    def apply(self, x, b):
+        opts = self.opts.c()
+        res = SolverRes()
+        p = self.prec._h if self.prec is not None else None
+        rc = lib().lfric_cg_solve(self.lin_op._h, p, x._h, b._h, ctypes.byref(opts), ctypes.byref(res))
+        return self._finish(rc, res)
+
+
+class MixedPrecisionCG(_Solver):
+    """FP64 iterative refinement around FP32 (r_solver) CG.  lin_op is the FP64 operator,
+    lin_op32 its FP32 copy, prec a preconditioner on FP32 vectors."""
+
+    def __init__(self, lin_op, lin_op32, prec, r_tol, a_tol, max_iter, inner_r_tol=1e-4, **kw):
+        super().__init__(lin_op, prec, r_tol, a_tol, max_iter, inner_r_tol=inner_r_tol, **kw)
+        self.lin_op32 = lin_op32
+
+    def apply(self, x, b):
+        opts = self.opts.c()
+        res = SolverRes()
+        p = self.prec._h if self.prec is not None else None
+        rc = lib().lfric_ir_cg_solve(self.lin_op._h, self.lin_op32._h, p, x._h, b._h, ctypes.byref(opts),
+                                     ctypes.byref(res))
+        return self._finish(rc, res)
+
+
+def residual_norm(op, x, b):
+    out = ctypes.c_double()
+    check(lib().lfric_residual_norm(op._h, x._h, b._h, ctypes.byref(out)))
+    return out.value

@@ -1,0 +1,194 @@
+"""GPU parity: the HIP path (through the C ABI) against the oracle on the same seeded inputs.
+
+Bars: bit-exact for integer work (dofmaps, colouring) and for the counter-based generator;
+FP64 LMA apply within 1e-13 relative of the oracle (element-wise, relative to |A||x|); FP32
+apply within 2e-6 relative to |A||x| (FP32 accumulation over <= 2*ndf terms); CG solutions
+within the stopping tolerance times the operator's conditioning (stated per test)."""
+import json
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN
+
+pytestmark = pytest.mark.gpu
+
+
+def _mesh_pair(hip, orc, n, nl, space=0, order=0):
+    fs = hip.FunctionSpace.cubedsphere(n, nl, space, order)
+    m = orc.Mesh(n, nl, space, order)
+    return fs, m
+
+
+@pytest.mark.parametrize("n,nl,space,order", [(4, 3, 0, 0), (12, 20, 0, 0), (4, 5, 0, 1), (12, 5, 2, 0),
+                                              (12, 5, 3, 0), (96, 2, 0, 0)])
+def test_dofmap_and_colouring_bit_exact(hip, orc, n, nl, space, order):
+    fs, m = _mesh_pair(hip, orc, n, nl, space, order)
+    assert (fs.ncells, fs.ndf, fs.undf) == (m.ncells, m.ndf, m.undf)
+    np.testing.assert_array_equal(fs.dofmap(), m.dofmap)
+    npc, cic = fs.colouring()
+    np.testing.assert_array_equal(npc, m.ncells_per_colour)
+    np.testing.assert_array_equal(cic, m.cells_in_colour)
+
+
+@pytest.mark.parametrize("ndf_order,scaled", [(0, False), (0, True), (1, True)])
+def test_generator_bit_exact(hip, orc, ndf_order, scaled):
+    fs, m = _mesh_pair(hip, orc, 4, 3, 0, ndf_order)
+    op = hip.LmaOperator(fs).generate_spd(11, scaled=scaled)
+    A = orc.gen_spd(m.ndf, m.ncell_3d, 11, scaled=scaled)
+    np.testing.assert_array_equal(op.download(), A)
+
+
+def test_upload_download_round_trip(hip, orc):
+    fs, m = _mesh_pair(hip, orc, 4, 3)
+    A = np.random.default_rng(0).standard_normal((m.ncell_3d, 8, 8))
+    op = hip.LmaOperator(fs).upload(A)
+    np.testing.assert_array_equal(op.download(), A)
+
+
+def _bound(orc, m, A, x, rel):
+    S = orc.assembled_csr(m, A)
+    return rel * (abs(S) @ np.abs(x))
+
+
+@pytest.mark.parametrize("n,nl,order", [(4, 3, 0), (12, 20, 0), (4, 5, 1), (12, 7, 1)])
+def test_lma_apply_fp64_parity(hip, orc, n, nl, order):
+    fs, m = _mesh_pair(hip, orc, n, nl, 0, order)
+    rng = np.random.default_rng(n + nl)
+    A = rng.standard_normal((m.ncell_3d, m.ndf, m.ndf))
+    x = rng.standard_normal(m.undf)
+    op = hip.LmaOperator(fs).upload(A)
+    xv = hip.FieldVector(fs).import_field(x)
+    yv = hip.FieldVector(fs)
+    yv.set_scalar(12345.0)  # the apply must overwrite, not accumulate
+    op.apply(xv, yv)
+    y = yv.export_field()
+    yo = orc.lma_apply(m, A, x)
+    assert np.all(np.abs(y - yo) <= _bound(orc, m, A, x, 1e-13))
+    # run-to-run bitwise determinism (no atomics)
+    op.apply(xv, yv)
+    np.testing.assert_array_equal(yv.export_field(), y)
+
+
+@pytest.mark.parametrize("order", [0, 1])
+def test_lma_apply_fp32_parity(hip, orc, order):
+    fs, m = _mesh_pair(hip, orc, 12, 10, 0, order)
+    A = orc.gen_spd(m.ndf, m.ncell_3d, 3, scaled=True)
+    x = orc.gen_uniform(m.undf, 0, 3)
+    op64 = hip.LmaOperator(fs).generate_spd(3, scaled=True)
+    op32 = op64.convert(hip.R32)
+    xv = hip.FieldVector(fs, hip.R32).import_field(x)
+    yv = hip.FieldVector(fs, hip.R32)
+    op32.apply(xv, yv)
+    y = yv.export_field()
+    yo = orc.lma_apply(m, A.astype(np.float32), x.astype(np.float32), r32=True).astype(np.float64)
+    assert np.all(np.abs(y - yo) <= _bound(orc, m, A, x, 2e-6))
+
+
+def test_lma_apply_known_answer(hip):
+    k = json.loads((GOLDEN / "kat.json").read_text())["matrix_vector"]
+    colours = dict(ncells_per_colour=[1], cells_in_colour=[[1]])
+    to = hip.FunctionSpace.from_dofmap(1, 4, [k["map1"]], **colours)
+    fr = hip.FunctionSpace.from_dofmap(1, 2, [k["map2"]], **colours)
+    op = hip.LmaOperator(to, fr).upload(np.array(k["d"])[None])
+    x = hip.FieldVector(fr).import_field(k["x"])
+    y = hip.FieldVector(to)
+    op.apply(x, y)
+    np.testing.assert_allclose(y.export_field(), k["answer"], rtol=0, atol=k["tol"])
+
+
+def test_mm_diagonal_parity(hip, orc):
+    fs, m = _mesh_pair(hip, orc, 12, 20)
+    op = hip.LmaOperator(fs).generate_spd(0)
+    d = op.mm_diagonal(hip.FieldVector(fs)).export_field()
+    np.testing.assert_allclose(d, orc.mm_diagonal(m, orc.gen_spd(8, m.ncell_3d, 0)), rtol=1e-14)
+
+
+def test_vector_ops_against_numpy(hip, orc):
+    fs = hip.FunctionSpace.cubedsphere(12, 20)
+    rng = np.random.default_rng(3)
+    a, b = rng.standard_normal(fs.undf), rng.standard_normal(fs.undf)
+    x, y = hip.FieldVector(fs).import_field(a), hip.FieldVector(fs).import_field(b)
+    assert abs(x.dot(y) - a @ b) <= 1e-13 * (np.abs(a) @ np.abs(b))
+    assert abs(x.norm() - np.linalg.norm(a)) <= 1e-13 * np.linalg.norm(a)
+    z = x.duplicate()
+    z.axpby(2.0, x, -3.0, y)
+    np.testing.assert_allclose(z.export_field(), 2 * a - 3 * b, rtol=1e-15, atol=1e-15)
+    z.aypx(0.5, x)
+    np.testing.assert_allclose(z.export_field(), 0.5 * (2 * a - 3 * b) + a, rtol=1e-15, atol=1e-15)
+    z.axpy(-1.5, y)
+    z.scale(2.0)
+    np.testing.assert_allclose(z.export_field(), 2 * (0.5 * (2 * a - 3 * b) + a - 1.5 * b), rtol=1e-14, atol=1e-14)
+    z.copy(x)
+    np.testing.assert_array_equal(z.export_field(), a)
+    z.divide(x, y)
+    np.testing.assert_array_equal(z.export_field(), a / b)
+    # FP32 storage with FP64-accumulated dots (r_solver_field_vector_type)
+    x32 = hip.FieldVector(fs, hip.R32).import_field(a)
+    a32 = a.astype(np.float32).astype(np.float64)
+    assert abs(x32.dot(x32) - a32 @ a32) <= 1e-12 * (a32 @ a32)
+
+
+def _config0(hip, orc, n=12, nl=20, scaled=False):
+    fs, m = _mesh_pair(hip, orc, n, nl)
+    op = hip.LmaOperator(fs).generate_spd(0, scaled=scaled)
+    b = hip.FieldVector(fs).fill_uniform(0, 3)
+    return fs, m, op, b
+
+
+def test_cg_fp64_matches_oracle(hip, orc):
+    fs, m, op, b = _config0(hip, orc)
+    prec = hip.DiagonalPreconditioner.from_operator(op)
+    x = hip.FieldVector(fs)
+    x.set_scalar(0.0)
+    cg = hip.ConjugateGradient(op, prec, 1e-8, 1e-16, 500, check_every=4)
+    res = cg.apply(x, b)
+    A = orc.gen_spd(8, m.ncell_3d, 0)
+    bo = orc.gen_uniform(m.undf, 0, 3)
+    np.testing.assert_array_equal(b.export_field(), bo)
+    xo, reso, hist = orc.cg(m, A, bo, diag=orc.mm_diagonal(m, A))
+    assert res.status == 0 and reso.converged == 1
+    assert res.iterations == reso.iterations
+    np.testing.assert_allclose(res.r_nrm / res.r_nrm_0, reso.r_nrm / reso.r_nrm_0, rtol=1e-6)
+    xg = x.export_field()
+    assert np.linalg.norm(xg - xo) <= 1e-10 * np.linalg.norm(xo)
+    S = orc.assembled_csr(m, A)
+    assert np.linalg.norm(bo - S @ xg) / np.linalg.norm(bo) <= 1.01e-8
+
+
+def test_mixed_precision_refinement_reaches_fp64_accuracy(hip, orc):
+    fs, m, op, b = _config0(hip, orc, 12, 20, scaled=True)
+    op32 = op.convert(hip.R32)
+    prec32 = hip.DiagonalPreconditioner.from_operator(op32)
+    x = hip.FieldVector(fs)
+    solver = hip.MixedPrecisionCG(op, op32, prec32, 1e-8, 0.0, 5000, inner_r_tol=1e-4)
+    res = solver.apply(x, b)
+    assert res.status == 0 and res.true_rel_res <= 1e-8 and res.outer_iterations >= 2
+    A = orc.gen_spd(8, m.ncell_3d, 0, scaled=True)
+    bo = orc.gen_uniform(m.undf, 0, 3)
+    S = orc.assembled_csr(m, A)
+    xg = x.export_field()
+    assert np.linalg.norm(bo - S @ xg) / np.linalg.norm(bo) <= 1.01e-8
+    # against the FP64 oracle solve driven to 1e-12: error bounded by cond * tol
+    xo, reso, _ = orc.cg(m, A, bo, diag=orc.mm_diagonal(m, A), r_tol=1e-12, max_iter=20000)
+    err = np.linalg.norm(xg - xo) / np.linalg.norm(xo)
+    assert err <= 1e-3, err
+
+
+def test_solver_status_codes(hip, orc):
+    fs, m, op, b = _config0(hip, orc, 4, 3)
+    prec = hip.DiagonalPreconditioner.from_operator(op)
+    x = hip.FieldVector(fs)
+    # zero right-hand side: converged in 0 iterations (||r0|| < EPS)
+    zero = hip.FieldVector(fs)
+    res = hip.ConjugateGradient(op, prec, 1e-8, 0.0, 50).apply(x, zero)
+    assert res.iterations == 0 and res.status == 0
+    # max_iter reached with fail_on_non_converged raises NotConverged
+    with pytest.raises(hip.NotConverged):
+        hip.ConjugateGradient(op, prec, 1e-14, 0.0, 1, fail_on_non_converged=True).apply(x, b)
+    # NaN in b is reported as NaN, not as convergence
+    bn = np.full(fs.undf, 1.0)
+    bn[5] = np.nan
+    with pytest.raises(hip.LfricError) as e:
+        hip.ConjugateGradient(op, prec, 1e-8, 0.0, 10).apply(hip.FieldVector(fs), hip.FieldVector(fs).import_field(bn))
+    assert e.value.code == 2

@@ -1,0 +1,244 @@
+"""The oracle pinned against the reference's own fixtures, known-answer tests, the compiled
+reference kernel (oracle/_ref) and closed-form / brute-force checks (CPU only)."""
+import ctypes
+import hashlib
+import json
+
+import numpy as np
+import pytest
+import scipy.sparse.linalg as spla
+
+from conftest import GOLDEN, ROOT
+
+
+def _digest(a):
+    return hashlib.sha256(np.ascontiguousarray(a, dtype="<i4").tobytes()).hexdigest()
+
+
+# ---- topology: the committed UGRID meshes of the reference --------------------------------
+@pytest.mark.parametrize("n", [4, 8, 12, 16, 24, 32])
+def test_cubedsphere_topology_matches_ugrid_fixtures(orc, n):
+    cn, vc = orc.cubedsphere_topology(n)
+    dig = json.loads((GOLDEN / "mesh_digests.json").read_text())
+    assert _digest(vc) == dig[f"C{n}_face_nodes"]
+    assert _digest(cn) == dig[f"C{n}_face_links"]
+    if n <= 12:
+        fx = np.load(GOLDEN / "mesh_fixtures.npz")
+        np.testing.assert_array_equal(vc, fx[f"C{n}_face_nodes"])
+        np.testing.assert_array_equal(cn, fx[f"C{n}_face_links"])
+
+
+def test_c3_counts_match_mesh_generator_summary(orc):
+    kat = json.loads((GOLDEN / "kat.json").read_text())["c3_summary"]
+    cn, vc = orc.cubedsphere_topology(3)
+    lid, nv = orc.local_vert_lids(vc)
+    foc, eoc, nf, ne = orc.mesh_connectivity(cn, lid)
+    assert (nv, ne, cn.shape[0]) == (kat["Nodes"], kat["Edges"], kat["Faces"])
+
+
+def test_topology_invariants(orc):
+    for n in (2, 5, 12):
+        cn, vc = orc.cubedsphere_topology(n)
+        nc = 6 * n * n
+        # adjacency symmetric, every cell has 4 distinct neighbours
+        for c in range(nc):
+            assert len(set(cn[c])) == 4
+            for d in range(4):
+                assert c + 1 in cn[cn[c, d] - 1]
+        lid, nv = orc.local_vert_lids(vc)
+        assert nv == nc + 2
+        counts = np.bincount(lid.ravel())[1:]
+        assert sorted(set(counts)) == [3, 4] and (counts == 3).sum() == 8
+
+
+# ---- connectivity and dofmaps: canned bi-periodic mesh fixtures ---------------------------
+def test_connectivity_and_dofmaps_match_unit_test_fixtures(orc):
+    c = json.loads((GOLDEN / "canned_mesh.json").read_text())
+    cn = np.array(c["cell_next"], np.int32)
+    vc = np.array(c["vert_on_cell"], np.int32)
+    foc, eoc, nf, ne = orc.mesh_connectivity(cn, vc)
+    np.testing.assert_array_equal(foc, c["face_on_cell"])
+    np.testing.assert_array_equal(eoc, c["edge_on_cell"])
+    nv = int(vc.max())
+    for space, key in ((orc.W0, "fs0k0_dofmap"), (orc.W2, "fs2k0_dofmap"), (orc.W3, "fs3k0_dofmap")):
+        dm, undf = orc.dofmap(space, 0, c["nlayers"], vc, foc, eoc, nv, ne)
+        np.testing.assert_array_equal(dm, c[key])
+
+
+@pytest.mark.parametrize("n,nl", [(4, 3), (12, 20)])
+def test_undf_closed_forms_and_column_contiguity(orc, n, nl):
+    nc, ne, nv = 6 * n * n, 12 * n * n, 6 * n * n + 2
+    m0 = orc.Mesh(n, nl, orc.W0, 0)
+    assert m0.undf == nv * (nl + 1)
+    m1 = orc.Mesh(n, nl, orc.W0, 1)
+    assert m1.undf == (nc + ne + nv) * (2 * nl + 1)
+    # W0 order 1: vertex/h-edge/h-face columns nl+1, volume/side faces/v-edges nl
+    assert m1.undf == nv * (nl + 1) + ne * (nl + 1) + nc * (nl + 1) + nc * nl + ne * nl + nv * nl
+    assert orc.Mesh(n, nl, orc.W2, 0).undf == ne * nl + nc * (nl + 1)
+    assert orc.Mesh(n, nl, orc.W3, 0).undf == nc * nl
+    # every W0 vertex column is shared by 4 cells (3 at the 8 cube corners)
+    counts = np.bincount(m0.dofmap[:, :4].ravel())
+    counts = counts[counts > 0]
+    assert (counts == 3).sum() == 8 and set(counts) == {3, 4}
+    np.testing.assert_array_equal(m0.dofmap[:, 4:], m0.dofmap[:, :4] + 1)
+
+
+# ---- colouring ------------------------------------------------------------------------------
+@pytest.mark.parametrize("n", [4, 12, 24])
+def test_colouring_closed_form(orc, n):
+    m = orc.Mesh(n, 2)
+    assert list(m.ncells_per_colour) == [6 * n * n // 4] * 4
+    assert orc.colour_validate(m.cell_next, m.colour_map) == 1
+    # brute force: no two cells of one colour share a vertex
+    for col in range(1, 5):
+        cells = np.where(m.colour_map == col)[0]
+        verts = m.vert_on_cell[cells].ravel()
+        assert len(np.unique(verts)) == verts.size
+    # the validator rejects a planted clash
+    bad = m.colour_map.copy()
+    bad[0] = bad[m.cell_next[0, 2] - 1]
+    assert orc.colour_validate(m.cell_next, bad) == 0
+
+
+# ---- generator ------------------------------------------------------------------------------
+def test_own_log_exp_against_libm(orc):
+    L = orc.lib()
+    xs = np.concatenate([np.linspace(1e-6, 1.0, 2001), [1e-300, 0.5, 0.999999, 1 - 2**-53]])
+    for x in xs:
+        assert abs(L.orc_log(x) - np.log(x)) <= 4 * np.spacing(abs(np.log(x))) + 1e-300
+    for x in np.linspace(-9.3, 9.3, 2001):
+        assert abs(L.orc_exp(x) - np.exp(x)) <= 4 * np.spacing(np.exp(x))
+
+
+def test_generator_moments(orc):
+    L = orc.lib()
+    z = np.array([L.orc_rng_normal(0, 1, i) for i in range(20000)])
+    assert abs(z.mean()) < 0.03 and abs(z.std() - 1) < 0.02
+    u = orc.gen_uniform(20000, 0, 3)
+    assert u.min() > -1 and u.max() < 1 and abs(u.mean()) < 0.02
+
+
+def test_spd_elements_are_symmetric_positive_definite(orc):
+    A = orc.gen_spd(8, 50, 0)
+    assert np.array_equal(A, A.transpose(0, 2, 1))
+    assert np.linalg.eigvalsh(A).min() > 0.0999
+    A27 = orc.gen_spd(27, 4, 1, scaled=True)
+    assert np.array_equal(A27, A27.transpose(0, 2, 1))
+
+
+# ---- LMA apply -------------------------------------------------------------------------------
+class _OneCell:
+    """The one-cell, one-layer mesh of matrix_vector_mod_test.pf."""
+
+    def __init__(self, dofmap, undf):
+        self.ncells, self.nlayers = 1, 1
+        self.dofmap = np.array([dofmap], np.int32)
+        self.ndf, self.undf = len(dofmap), undf
+        self.ncells_per_colour = np.array([1, 0, 0, 0], np.int32)
+        self.cells_in_colour = np.array([[1], [0], [0], [0]], np.int32)
+
+
+def test_lma_apply_known_answer(orc):
+    k = json.loads((GOLDEN / "kat.json").read_text())["matrix_vector"]
+    to, fr = _OneCell(k["map1"], 4), _OneCell(k["map2"], 2)
+    A = np.array(k["d"])[None, :, :]
+    y = orc.lma_apply(fr, A, np.array(k["x"]), to_mesh=to)
+    np.testing.assert_allclose(y, k["answer"], rtol=0, atol=k["tol"])
+    y32 = orc.lma_apply(fr, A, np.array(k["x"]), to_mesh=to, r32=True)
+    np.testing.assert_allclose(y32, k["answer"], rtol=0, atol=1e-6)
+
+
+@pytest.mark.parametrize("n,nl,order", [(4, 3, 0), (4, 3, 1), (12, 20, 0)])
+def test_lma_apply_equals_assembled_matrix(orc, n, nl, order):
+    m = orc.Mesh(n, nl, orc.W0, order)
+    rng = np.random.default_rng(1)
+    A = rng.standard_normal((m.ncell_3d, m.ndf, m.ndf))  # non-symmetric on purpose
+    x = rng.standard_normal(m.undf)
+    y = orc.lma_apply(m, A, x)
+    S = orc.assembled_csr(m, A)
+    bound = 1e-13 * (abs(S) @ np.abs(x))
+    assert np.all(np.abs(y - S @ x) <= bound)
+    # a transposed element matrix is caught
+    yt = orc.lma_apply(m, A.transpose(0, 2, 1), x)
+    assert np.abs(yt - y).max() > 1e-3
+
+
+def test_mm_diagonal_is_assembled_diagonal(orc):
+    m = orc.Mesh(4, 3, orc.W0, 1)
+    A = orc.gen_spd(m.ndf, m.ncell_3d, 3)
+    np.testing.assert_allclose(orc.mm_diagonal(m, A), orc.assembled_csr(m, A).diagonal(), rtol=1e-14)
+
+
+def _ref_lib():
+    p = ROOT / "oracle" / "_ref" / "libref.so"
+    if not p.exists():
+        pytest.skip("oracle/_ref not built (needs the reference tree and flang)")
+    return ctypes.CDLL(str(p))
+
+
+def test_oracle_apply_bitwise_equals_compiled_reference_kernel(orc):
+    R = _ref_lib()
+    P = lambda a: ctypes.c_void_p(a.ctypes.data)
+    I = ctypes.c_int
+    for order in (0, 1):
+        m = orc.Mesh(4, 5, orc.W0, order)
+        A = orc.gen_spd(m.ndf, m.ncell_3d, 7, scaled=True)
+        x = orc.gen_uniform(m.undf, 0, 3)
+        y = orc.lma_apply(m, A, x)
+        yr = np.zeros(m.undf)
+        Af = orc.flat_matrix(A)
+        R.ref_lma_apply_r64(I(m.ncells), I(m.nlayers), I(4), P(m.ncells_per_colour), P(m.cells_in_colour),
+                            I(m.cells_in_colour.shape[1]), I(m.ndf), I(m.undf), P(m.dofmap), I(m.ndf), I(m.undf),
+                            P(m.dofmap), P(Af), P(x), P(yr))
+        np.testing.assert_array_equal(y, yr)
+        d = np.zeros(m.undf)
+        R.ref_mm_diagonal_r64(I(m.ncells), I(m.nlayers), I(4), P(m.ncells_per_colour), P(m.cells_in_colour),
+                              I(m.cells_in_colour.shape[1]), I(m.ndf), I(m.undf), P(m.dofmap), P(Af), P(d))
+        np.testing.assert_allclose(d, orc.mm_diagonal(m, A), rtol=1e-15)
+
+
+# ---- solvers ----------------------------------------------------------------------------------
+def test_cg_solves_small_spd_system(orc):
+    m = orc.Mesh(4, 3)
+    A = orc.gen_spd(8, m.ncell_3d, 0)
+    b = orc.gen_uniform(m.undf, 0, 3)
+    d = orc.mm_diagonal(m, A)
+    x, res, hist = orc.cg(m, A, b, diag=d, r_tol=1e-13, max_iter=m.undf)
+    S = orc.assembled_csr(m, A).toarray()
+    xs = np.linalg.solve(S, b)
+    assert res.converged == 1 and res.iterations < m.undf
+    np.testing.assert_allclose(x, xs, rtol=0, atol=1e-10 * np.abs(xs).max())
+    xsc, info = spla.cg(S, b, rtol=1e-13, maxiter=1000)
+    np.testing.assert_allclose(x, xsc, atol=1e-8 * np.abs(xs).max())
+
+
+def test_cg_without_monitor_runs_max_iter(orc):
+    m = orc.Mesh(4, 3)
+    A = orc.gen_spd(8, m.ncell_3d, 0)
+    b = orc.gen_uniform(m.undf, 0, 3)
+    x, res, _ = orc.cg(m, A, b, diag=orc.mm_diagonal(m, A), max_iter=7, monitor=False)
+    assert res.iterations == 7 and res.converged == 0
+
+
+def test_givens_least_squares_matches_lstsq(orc):
+    rng = np.random.default_rng(5)
+    mdim = 6
+    h = np.triu(rng.standard_normal((mdim + 1, mdim)), -1)
+    g = np.zeros(mdim + 1)
+    g[0] = 2.5
+    u = orc.givens_lsq(h, g)
+    np.testing.assert_allclose(u, np.linalg.lstsq(h, g, rcond=None)[0], rtol=1e-12, atol=1e-13)
+
+
+def test_gmres_solves_nonsymmetric_system(orc):
+    rng = np.random.default_rng(2)
+    n = 40
+    M = np.eye(n) * 4 + rng.standard_normal((n, n)) * 0.3
+    b = rng.standard_normal(n)
+    x, res, hist = orc.gmres_callbacks(n, lambda v: M @ v, lambda v: v / np.diag(M), b, gcrk=10,
+                                       r_tol=1e-12, max_iter=30)
+    assert res.converged == 1
+    np.testing.assert_allclose(x, np.linalg.solve(M, b), rtol=0, atol=1e-10)
+    # with gcrk >= n one restart cycle solves it
+    x1, res1, _ = orc.gmres_callbacks(n, lambda v: M @ v, lambda v: v, b, gcrk=n, r_tol=1e-10, max_iter=1)
+    assert res1.converged == 1 and res1.iterations == 1
