@@ -20,7 +20,7 @@ struct SolverState {
   int pad;
 };
 
-constexpr int kRedBlocks = 1024;  // fixed grid of the reduction kernels (deterministic sums)
+constexpr int kRedBlocks = 2048;  // fixed grid of the reduction kernels (deterministic sums)
 
 // generators
 int launch_gen_spd(lfric_op *op, unsigned long long seed, int scaled, double shift);

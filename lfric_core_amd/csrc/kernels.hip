@@ -68,35 +68,15 @@ __device__ inline void block_sum(double (&v)[NV], double (&out)[NV]) {
   __syncthreads();
 }
 
-// Returns true in the last block, where `tot` then holds the grid-wide sums.
+// Every block of a reduction kernel writes its partial sums; a one-block finishing kernel
+// (k_finish) adds them in block order, so sums are deterministic run to run.
 template <int NV>
-__device__ inline bool reduce_grid(double (&v)[NV], double *partials, unsigned int *counter,
-                                   double (&tot)[NV]) {
-  __shared__ int is_last;
+__device__ inline void block_partials(double (&v)[NV], double *partials) {
   double bs[NV];
   block_sum<NV>(v, bs);
-  if (threadIdx.x == 0) {
+  if (threadIdx.x == 0)
 #pragma unroll
-    for (int q = 0; q < NV; ++q)
-      __hip_atomic_store(&partials[blockIdx.x * NV + q], bs[q], __ATOMIC_RELAXED,
-                         __HIP_MEMORY_SCOPE_AGENT);
-    __threadfence();
-    unsigned int t = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-    is_last = (t == gridDim.x - 1);
-  }
-  __syncthreads();
-  if (!is_last) return false;
-  __threadfence();
-  double s[NV];
-#pragma unroll
-  for (int q = 0; q < NV; ++q) s[q] = 0.0;
-  for (unsigned int b = threadIdx.x; b < gridDim.x; b += blockDim.x)
-#pragma unroll
-    for (int q = 0; q < NV; ++q)
-      s[q] += __hip_atomic_load(&partials[b * NV + q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  block_sum<NV>(s, tot);
-  if (threadIdx.x == 0) __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  return true;
+    for (int q = 0; q < NV; ++q) partials[blockIdx.x * NV + q] = bs[q];
 }
 
 template <typename T>
@@ -225,12 +205,50 @@ __global__ void k_convert(TD *__restrict__ d, const TS *__restrict__ s, long lon
 // ---------------------------------------------------------------------------------------
 // LMA apply, one colour per launch (see file header)
 // ---------------------------------------------------------------------------------------
-template <typename T, int N1, int N2>
+// Column roles of the output space.  Compile-time tables for the spaces the library builds
+// (the dofmap entity order vol, faces W,S,E,N,B,T, edges WB..NB, SW..NW, WT..NT, vertices
+// SWB..NWB, SWT..NWT), so every element-matrix load of a thread can be issued before the
+// first FMA; RolesRuntime carries any other pattern found in a caller's dofmap.
+struct RolesNone {
+  static constexpr bool kStatic = true;
+  __host__ __device__ static constexpr bool top(int) { return false; }
+  __host__ __device__ static constexpr int partner(int) { return -1; }
+};
+struct RolesW0K0 {  // 4 bottom vertices, 4 top vertices
+  static constexpr bool kStatic = true;
+  __host__ __device__ static constexpr bool top(int i) { return i >= 4; }
+  __host__ __device__ static constexpr int partner(int i) { return i < 4 ? i + 4 : -1; }
+};
+struct RolesW2K0 {  // faces W,S,E,N (height nl), B -> T
+  static constexpr bool kStatic = true;
+  __host__ __device__ static constexpr bool top(int i) { return i == 5; }
+  __host__ __device__ static constexpr int partner(int i) { return i == 4 ? 5 : -1; }
+};
+struct RolesW0K1 {  // B->T face, WB..NB -> WT..NT edges, bottom -> top vertices
+  static constexpr bool kStatic = true;
+  __host__ __device__ static constexpr bool top(int i) { return i == 6 || (i >= 15 && i <= 18) || i >= 23; }
+  __host__ __device__ static constexpr int partner(int i) {
+    return i == 5 ? 6 : (i >= 7 && i <= 10) ? i + 8 : (i >= 19 && i <= 22) ? i + 4 : -1;
+  }
+};
+struct RolesRuntime {
+  static constexpr bool kStatic = false;
+  Roles r;
+  __device__ bool top(int i) const { return (r.top_mask >> i) & 1u; }
+  __device__ int partner(int i) const { return r.partner[i]; }
+};
+
+template <typename T>
+__device__ inline T ld_stream(const T *p) {
+  return __builtin_nontemporal_load(p);
+}
+
+template <typename T, int N1, int N2, class R>
 __global__ __launch_bounds__(kBlock) void k_lma_colour(const T *__restrict__ A, size_t S, const T *__restrict__ x,
                                                        T *__restrict__ y, const int *__restrict__ map1,
                                                        const int *__restrict__ map2,
                                                        const uint32_t *__restrict__ first, int pos0, int npos,
-                                                       int nl, Roles roles, const int *__restrict__ skip) {
+                                                       int nl, R roles, const int *__restrict__ skip) {
   if (skip && *skip) return;
   const int H = nl + 1;
   const int t = blockIdx.x * kBlock + threadIdx.x;
@@ -250,25 +268,58 @@ __global__ __launch_bounds__(kBlock) void k_lma_colour(const T *__restrict__ A, 
   }
   const uint32_t ft = first ? first[pos] : 0xffffffffu;
   const size_t cm = static_cast<size_t>(pos) * nl + m;
+  if constexpr (R::kStatic && N1 * N2 <= 64) {
+    // every row of this thread: bottom rows at layer m, top rows at layer m-1
+    T a[N1][N2];
 #pragma unroll
-  for (int i = 0; i < N1; ++i) {
-    if ((roles.top_mask >> i) & 1u) continue;
-    const int pt = roles.partner[i];
-    if (!has_m && pt < 0) continue;
-    T acc = T(0);
-    if (has_m) {
+    for (int i = 0; i < N1; ++i)
 #pragma unroll
-      for (int j = 0; j < N2; ++j) acc = fma(A[static_cast<size_t>(i * N2 + j) * S + cm], xm[j], acc);
+      for (int j = 0; j < N2; ++j) {
+        const bool tp = R::top(i);
+        a[i][j] = (tp ? has_p : has_m) ? ld_stream(A + static_cast<size_t>(i * N2 + j) * S + cm - (tp ? 1 : 0)) : T(0);
+      }
+    T c[N1];
+#pragma unroll
+    for (int i = 0; i < N1; ++i) {
+      T s = T(0);
+#pragma unroll
+      for (int j = 0; j < N2; ++j) s = fma(a[i][j], R::top(i) ? xp[j] : xm[j], s);
+      c[i] = s;
     }
-    if (pt >= 0 && has_p) {
 #pragma unroll
-      for (int j = 0; j < N2; ++j) acc = fma(A[static_cast<size_t>(pt * N2 + j) * S + cm - 1], xp[j], acc);
+    for (int i = 0; i < N1; ++i) {
+      if (R::top(i)) continue;
+      const int pt = R::partner(i);
+      if (!has_m && pt < 0) continue;
+      const T out = pt >= 0 ? c[i] + c[pt < 0 ? 0 : pt] : c[i];
+      const size_t d = static_cast<size_t>(m1[i]) + m;
+      if ((ft >> i) & 1u)
+        y[d] = out;
+      else
+        y[d] += out;
     }
-    const size_t d = static_cast<size_t>(m1[i]) + m;
-    if ((ft >> i) & 1u)
-      y[d] = acc;
-    else
-      y[d] += acc;
+  } else {
+#pragma unroll
+    for (int i = 0; i < N1; ++i) {
+      if (roles.top(i)) continue;
+      const int pt = roles.partner(i);
+      if (!has_m && pt < 0) continue;
+      T acc = T(0);
+      if (has_m) {
+#pragma unroll
+        for (int j = 0; j < N2; ++j) acc = fma(ld_stream(A + static_cast<size_t>(i * N2 + j) * S + cm), xm[j], acc);
+      }
+      if (pt >= 0 && has_p) {
+#pragma unroll
+        for (int j = 0; j < N2; ++j)
+          acc = fma(ld_stream(A + static_cast<size_t>(pt * N2 + j) * S + cm - 1), xp[j], acc);
+      }
+      const size_t d = static_cast<size_t>(m1[i]) + m;
+      if ((ft >> i) & 1u)
+        y[d] = acc;
+      else
+        y[d] += acc;
+    }
   }
 }
 
@@ -384,13 +435,28 @@ template <typename T>
 __global__ void k_axpy_mixed(double *__restrict__ y, double a, const T *__restrict__ x, long long n) {
   GRID_STRIDE(i, n) y[i] = fma(a, to_d(x[i]), y[i]);
 }
+// Streaming loops of the reduction kernels: 4 independent loads per vector in flight per
+// thread before any arithmetic.
+#define UNROLLED_STRIDE(n)                                                                   \
+  const long long s_ = static_cast<long long>(gridDim.x) * blockDim.x;                     \
+  for (long long i0_ = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x; i0_ < (n); i0_ += 4 * s_)
+
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_dot(const T *__restrict__ x, const T *__restrict__ y, long long n,
                                                 double *partials, unsigned int *counter, double *out) {
   double v[1] = {0.0};
-  GRID_STRIDE(i, n) v[0] += to_d(x[i]) * to_d(y[i]);
-  double tot[1];
-  if (reduce_grid<1>(v, partials, counter, tot) && threadIdx.x == 0) *out = tot[0];
+  UNROLLED_STRIDE(n) {
+    T a[4], b[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const long long i = i0_ + u * s_;
+      a[u] = i < n ? x[i] : T(0);
+      b[u] = i < n ? y[i] : T(0);
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) v[0] += to_d(a[u]) * to_d(b[u]);
+  }
+  block_partials<1>(v, partials);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -406,22 +472,17 @@ template <typename T>
 __global__ __launch_bounds__(kBlock) void k_cg_init_norm(const T *__restrict__ r, long long n, SolverState *st,
                                                          double *partials) {
   double v[1] = {0.0};
-  GRID_STRIDE(i, n) v[0] += to_d(r[i]) * to_d(r[i]);
-  double tot[1];
-  if (reduce_grid<1>(v, partials, &st->counter, tot) && threadIdx.x == 0) {
-    const double nrm = sqrt(tot[0]);
-    st->r_nrm0 = nrm;
-    st->r_nrm = nrm;
-    st->iter = 0;
-    if (nrm < 2.220446049250313e-16) {  // ||r0|| < EPS: converged in 0 iterations (:527-532)
-      st->done = 1;
-      st->converged = 1;
+  UNROLLED_STRIDE(n) {
+    T a[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const long long i = i0_ + u * s_;
+      a[u] = i < n ? r[i] : T(0);
     }
-    if (nrm != nrm) {
-      st->done = 1;
-      st->status = LFRIC_NAN;
-    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) v[0] += to_d(a[u]) * to_d(a[u]);
   }
+  block_partials<1>(v, partials);
 }
 
 template <typename T>
@@ -436,8 +497,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_init_prec(const T *__restrict__ r
     p[i] = zi;
     v[0] += to_d(r[i]) * to_d(zi);
   }
-  double tot[1];
-  if (reduce_grid<1>(v, partials, &st->counter, tot) && threadIdx.x == 0) st->rz = tot[0];
+  block_partials<1>(v, partials);
 }
 
 template <typename T>
@@ -445,9 +505,93 @@ __global__ __launch_bounds__(kBlock) void k_cg_dot_pz(const T *__restrict__ p, c
                                                       SolverState *st, double *partials) {
   if (st->done) return;
   double v[1] = {0.0};
-  GRID_STRIDE(i, n) v[0] += to_d(p[i]) * to_d(z[i]);
-  double tot[1];
-  if (reduce_grid<1>(v, partials, &st->counter, tot) && threadIdx.x == 0) {
+  UNROLLED_STRIDE(n) {
+    T a[4], b[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const long long i = i0_ + u * s_;
+      a[u] = i < n ? p[i] : T(0);
+      b[u] = i < n ? z[i] : T(0);
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) v[0] += to_d(a[u]) * to_d(b[u]);
+  }
+  block_partials<1>(v, partials);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_cg_update(T *__restrict__ x, T *__restrict__ r, T *__restrict__ z,
+                                                      const T *__restrict__ p, const T *__restrict__ diag,
+                                                      long long n, SolverState *st, double *partials) {
+  if (st->done) return;
+  const T a = static_cast<T>(st->alpha);
+  const T ma = static_cast<T>(-st->alpha);
+  double v[2] = {0.0, 0.0};
+  UNROLLED_STRIDE(n) {
+    T xv[4], pv[4], rv[4], zv[4], dv[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const long long i = i0_ + u * s_;
+      const bool ok = i < n;
+      xv[u] = ok ? x[i] : T(0);
+      pv[u] = ok ? p[i] : T(0);
+      rv[u] = ok ? r[i] : T(0);
+      zv[u] = ok ? z[i] : T(0);
+      dv[u] = (ok && diag) ? diag[i] : T(1);
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const long long i = i0_ + u * s_;
+      if (i >= n) continue;
+      x[i] = xv[u] + a * pv[u];              // x%axpy(alpha,p)
+      const T ri = rv[u] + ma * zv[u];       // r%axpy(-alpha,z)
+      r[i] = ri;
+      const T zi = diag ? ri / dv[u] : ri;   // prec%apply(r,z)
+      z[i] = zi;
+      v[0] += to_d(ri) * to_d(ri);
+      v[1] += to_d(ri) * to_d(zi);
+    }
+  }
+  block_partials<2>(v, partials);
+}
+
+// Finishing step of every reduction: sum the block partials in block order, then update the
+// solver scalars (cg_solve's scalar arithmetic, sci_iterative_solver_mod.f90:521-575).
+enum Epilogue { kEpiDot = 0, kEpiInitNorm, kEpiInitPrec, kEpiDotPz, kEpiUpdate };
+
+template <int NV, int EPI>
+__global__ __launch_bounds__(kBlock) void k_finish(const double *__restrict__ partials, int nblocks, SolverState *st,
+                                                   double *out) {
+  if (EPI == kEpiInitPrec || EPI == kEpiDotPz || EPI == kEpiUpdate) {
+    if (st->done) return;
+  }
+  double s[NV];
+#pragma unroll
+  for (int q = 0; q < NV; ++q) s[q] = 0.0;
+  for (int b = threadIdx.x; b < nblocks; b += kBlock)
+#pragma unroll
+    for (int q = 0; q < NV; ++q) s[q] += partials[b * NV + q];
+  double tot[NV];
+  block_sum<NV>(s, tot);
+  if (threadIdx.x != 0) return;
+  if (EPI == kEpiDot) {
+    *out = tot[0];
+  } else if (EPI == kEpiInitNorm) {
+    const double nrm = sqrt(tot[0]);
+    st->r_nrm0 = nrm;
+    st->r_nrm = nrm;
+    st->iter = 0;
+    if (nrm < 2.220446049250313e-16) {  // ||r0|| < EPS: converged in 0 iterations (:527-532)
+      st->done = 1;
+      st->converged = 1;
+    }
+    if (nrm != nrm) {
+      st->done = 1;
+      st->status = LFRIC_NAN;
+    }
+  } else if (EPI == kEpiInitPrec) {
+    st->rz = tot[0];
+  } else if (EPI == kEpiDotPz) {
     const double pz = tot[0];
     st->pz = pz;
     if (pz != pz) {
@@ -459,33 +603,12 @@ __global__ __launch_bounds__(kBlock) void k_cg_dot_pz(const T *__restrict__ p, c
     } else {
       st->alpha = st->rz / pz;  // alpha = <r,z> / <p,A.p>
     }
-  }
-}
-
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_cg_update(T *__restrict__ x, T *__restrict__ r, T *__restrict__ z,
-                                                      const T *__restrict__ p, const T *__restrict__ diag,
-                                                      long long n, SolverState *st, double *partials) {
-  if (st->done) return;
-  const T a = static_cast<T>(st->alpha);
-  const T ma = static_cast<T>(-st->alpha);
-  double v[2] = {0.0, 0.0};
-  GRID_STRIDE(i, n) {
-    x[i] = x[i] + a * p[i];           // x%axpy(alpha,p)
-    const T ri = r[i] + ma * z[i];    // r%axpy(-alpha,z)
-    r[i] = ri;
-    const T zi = diag ? ri / diag[i] : ri;  // prec%apply(r,z)
-    z[i] = zi;
-    v[0] += to_d(ri) * to_d(ri);
-    v[1] += to_d(ri) * to_d(zi);
-  }
-  double tot[2];
-  if (reduce_grid<2>(v, partials, &st->counter, tot) && threadIdx.x == 0) {
+  } else if (EPI == kEpiUpdate) {
     st->iter += 1;
     const double rn = sqrt(tot[0]);
     st->r_nrm = rn;
     st->red0 = tot[0];
-    st->red1 = tot[1];
+    st->red1 = tot[NV - 1];
     if (rn != rn) {
       st->status = LFRIC_NAN;
       st->done = 1;
@@ -496,7 +619,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_update(T *__restrict__ x, T *__re
       st->done = 1;
       return;
     }
-    const double rz_new = tot[1];
+    const double rz_new = tot[NV - 1];
     st->beta = rz_new / st->rz;
     st->rz = rz_new;
     if (st->iter >= st->max_iter) st->done = 1;
@@ -663,21 +786,69 @@ int launch_op_convert(const lfric_op *src, lfric_op *dst) {
 }
 
 // ---- operator apply dispatch ---------------------------------------------------------------
+// which compile-time role table (if any) a space's detected roles match
+enum class RoleKind { None, W0K0, W0K1, W2K0, Runtime };
+template <class R>
+static bool roles_match(const FunctionSpace &f) {
+  for (int i = 0; i < f.ndf; ++i) {
+    if (R::top(i) != static_cast<bool>((f.top_mask >> i) & 1u)) return false;
+    if (R::partner(i) != f.partner[i]) return false;
+  }
+  return true;
+}
+static RoleKind role_kind(const FunctionSpace &f) {
+  if (f.ndf == 8 && roles_match<RolesW0K0>(f)) return RoleKind::W0K0;
+  if (f.ndf == 27 && roles_match<RolesW0K1>(f)) return RoleKind::W0K1;
+  if (f.ndf == 6 && roles_match<RolesW2K0>(f)) return RoleKind::W2K0;
+  if (roles_match<RolesNone>(f)) return RoleKind::None;
+  return RoleKind::Runtime;
+}
+
+template <typename T, int A_, int B_>
+static void launch_colour(RoleKind rk, int grid, const lfric_op *op, size_t S, const T *x, T *y, const FunctionSpace &to,
+                          const FunctionSpace &from, const uint32_t *first, int pos0, int npos, const int *d_skip) {
+  const int nl = to.nlayers;
+  const T *Ad = static_cast<const T *>(op->d);
+  switch (rk) {
+    case RoleKind::W0K0:
+      if constexpr (A_ == 8)
+        LAUNCH((k_lma_colour<T, A_, B_, RolesW0K0>), grid, Ad, S, x, y, to.d_map, from.d_map, first, pos0, npos, nl,
+               RolesW0K0{}, d_skip);
+      break;
+    case RoleKind::W0K1:
+      if constexpr (A_ == 27)
+        LAUNCH((k_lma_colour<T, A_, B_, RolesW0K1>), grid, Ad, S, x, y, to.d_map, from.d_map, first, pos0, npos, nl,
+               RolesW0K1{}, d_skip);
+      break;
+    case RoleKind::W2K0:
+      if constexpr (A_ == 6)
+        LAUNCH((k_lma_colour<T, A_, B_, RolesW2K0>), grid, Ad, S, x, y, to.d_map, from.d_map, first, pos0, npos, nl,
+               RolesW2K0{}, d_skip);
+      break;
+    case RoleKind::None:
+      LAUNCH((k_lma_colour<T, A_, B_, RolesNone>), grid, Ad, S, x, y, to.d_map, from.d_map, first, pos0, npos, nl,
+             RolesNone{}, d_skip);
+      break;
+    default:
+      LAUNCH((k_lma_colour<T, A_, B_, RolesRuntime>), grid, Ad, S, x, y, to.d_map, from.d_map, first, pos0, npos, nl,
+             RolesRuntime{roles_of(to)}, d_skip);
+  }
+}
+
 template <typename T>
 static int apply_colours_t(const lfric_op *op, const T *x, T *y, const int *d_skip) {
   const FunctionSpace &to = *op->to, &from = *op->from;
-  const Roles roles = roles_of(to);
+  RoleKind rk = role_kind(to);
   const size_t S = static_cast<size_t>(op->plane_stride);
   const int nl = to.nlayers;
   for (int c = 0; c < to.ncolours; ++c) {
     const int pos0 = to.colour_start[c], npos = to.colour_start[c + 1] - pos0;
     if (npos == 0) continue;
     const int grid = grid_for(static_cast<long long>(npos) * (nl + 1));
-#define LMA_CASE(A, B)                                                                                  \
-  if (to.ndf == A && from.ndf == B) {                                                                  \
-    LAUNCH((k_lma_colour<T, A, B>), grid, static_cast<const T *>(op->d), S, x, y, to.d_map, from.d_map, \
-           to.d_first, pos0, npos, nl, roles, d_skip);                                                 \
-    continue;                                                                                          \
+#define LMA_CASE(A, B)                                                                              \
+  if (to.ndf == A && from.ndf == B) {                                                              \
+    launch_colour<T, A, B>(rk, grid, op, S, x, y, to, from, to.d_first, pos0, npos, d_skip);    \
+    continue;                                                                                       \
   }
     LMA_CASE(8, 8)
     LMA_CASE(27, 27)
@@ -752,8 +923,9 @@ static int dg_t(const lfric_op *op, const T *x, T *y) {
   const int grid = grid_for(static_cast<long long>(npos) * (nl + 1));
 #define DG_CASE(A, B)                                                                                            \
   if (to.ndf == A && from.ndf == B) {                                                                           \
-    LAUNCH((k_lma_colour<T, A, B>), grid, static_cast<const T *>(op->d), S, x, y, to.d_map, from.d_map,          \
-           static_cast<const uint32_t *>(nullptr), 0, npos, nl, roles, static_cast<const int *>(nullptr));     \
+    LAUNCH((k_lma_colour<T, A, B, RolesRuntime>), grid, static_cast<const T *>(op->d), S, x, y, to.d_map,       \
+           from.d_map, static_cast<const uint32_t *>(nullptr), 0, npos, nl, RolesRuntime{roles},                 \
+           static_cast<const int *>(nullptr));                                                                   \
     CHECK_LAUNCH();                                                                                             \
     return LFRIC_OK;                                                                                            \
   }
@@ -908,6 +1080,7 @@ int launch_dot(int prec, const void *x, const void *y, long long n, double *d_ou
            c.d_counter, d_out);
   else
     return fail(LFRIC_E_UNSUPPORTED, "vector precision");
+  LAUNCH((k_finish<1, kEpiDot>), 1, c.d_partials, kRedBlocks, static_cast<SolverState *>(nullptr), d_out);
   CHECK_LAUNCH();
   return LFRIC_OK;
 }
@@ -927,6 +1100,7 @@ int launch_cg_init_norm(int prec, const void *r, long long n, SolverState *st) {
     LAUNCH(k_cg_init_norm<double>, kRedBlocks, static_cast<const double *>(r), n, st, ctx().d_partials);
   else
     LAUNCH(k_cg_init_norm<float>, kRedBlocks, static_cast<const float *>(r), n, st, ctx().d_partials);
+  LAUNCH((k_finish<1, kEpiInitNorm>), 1, ctx().d_partials, kRedBlocks, st, static_cast<double *>(nullptr));
   CHECK_LAUNCH();
   return LFRIC_OK;
 }
@@ -938,6 +1112,7 @@ int launch_cg_init_prec(int prec, const void *r, void *z, void *p, const void *d
   else
     LAUNCH(k_cg_init_prec<float>, kRedBlocks, static_cast<const float *>(r), static_cast<float *>(z),
            static_cast<float *>(p), static_cast<const float *>(diag), n, st, ctx().d_partials);
+  LAUNCH((k_finish<1, kEpiInitPrec>), 1, ctx().d_partials, kRedBlocks, st, static_cast<double *>(nullptr));
   CHECK_LAUNCH();
   return LFRIC_OK;
 }
@@ -949,6 +1124,7 @@ int launch_cg_dot_pz(int prec, const void *p, const void *z, long long n, Solver
   else
     LAUNCH(k_cg_dot_pz<float>, kRedBlocks, static_cast<const float *>(p), static_cast<const float *>(z), n, st,
            ctx().d_partials);
+  LAUNCH((k_finish<1, kEpiDotPz>), 1, ctx().d_partials, kRedBlocks, st, static_cast<double *>(nullptr));
   CHECK_LAUNCH();
   return LFRIC_OK;
 }
@@ -961,6 +1137,7 @@ int launch_cg_update(int prec, void *x, void *r, void *z, const void *p, const v
   else
     LAUNCH(k_cg_update<float>, kRedBlocks, static_cast<float *>(x), static_cast<float *>(r), static_cast<float *>(z),
            static_cast<const float *>(p), static_cast<const float *>(diag), n, st, ctx().d_partials);
+  LAUNCH((k_finish<2, kEpiUpdate>), 1, ctx().d_partials, kRedBlocks, st, static_cast<double *>(nullptr));
   CHECK_LAUNCH();
   return LFRIC_OK;
 }
