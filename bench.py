@@ -26,6 +26,10 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per FP32 apply of C96x70 W0 from the rocprofv3 FETCH_SIZE / WRITE_SIZE runs recorded in
+# profiles/lma_apply_cg_c96x70_w0.md (4 colour launches x (288.4 MB fetched + 16.8 MB written),
+# counters scaled by a plain copy of known size).  Re-measure when the kernel changes.
+TRAFFIC_BYTES_PER_APPLY_FP32 = {(96, 70): 4 * (288.4e6 + 16.8e6)}
 
 
 def apply_bytes(fs, ndf, s_a, s_x):
@@ -95,6 +99,8 @@ def main():
     ap.add_argument("--apply-reps", type=int, default=20)
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--apply-only", action="store_true",
+                    help="time only the LMA applies and a known-size vector copy (for hardware-counter runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only: --gpus {args.gpus} is not supported (the hot path is a "
@@ -113,6 +119,20 @@ def main():
     b = L.FieldVector(fs).fill_uniform(0, 3)
     x = L.FieldVector(fs)
     undf = fs.undf
+
+    if args.apply_only:
+        xs32, ys32 = L.FieldVector(fs, L.R32).fill_uniform(1, 3), L.FieldVector(fs, L.R32)
+        xs64, ys64 = L.FieldVector(fs).fill_uniform(1, 3), L.FieldVector(fs)
+        t32 = time_applies(L, op32, xs32, ys32, args.apply_reps)
+        t64 = time_applies(L, op64, xs64, ys64, args.apply_reps)
+        for _ in range(args.apply_reps):  # plain copies of a known size (undf * 8 bytes each way)
+            ys64.copy(xs64)
+        L.synchronize()
+        print(json.dumps({"apply_only": True, "fp32_ms": t32, "fp64_ms": t64,
+                          "fp32_gbs": apply_bytes(fs, 8, 4, 4) / (t32 * 1e-3) / 1e9,
+                          "fp64_gbs": apply_bytes(fs, 8, 8, 8) / (t64 * 1e-3) / 1e9,
+                          "copy_bytes_each_way": undf * 8}))
+        return
 
     mixed = L.MixedPrecisionCG(op64, op32, prec32, 1e-8, 0.0, 20000, inner_r_tol=1e-4, check_every=16)
     plain = L.ConjugateGradient(op64, prec64, 1e-8, 0.0, 20000, check_every=16)
@@ -184,8 +204,11 @@ def main():
         "lma_gbs": {"fp32": gbs32, "fp64": gbs64, "fp32_ms": t32, "fp64_ms": t64,
                     "fp32_bytes": b32, "fp64_bytes": b64},
         "roofline": {"bound": "hbm", "achieved": gbs32, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": gbs32 / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_lma_colour<float,8,8> (4 colour launches per apply; bytes and time per apply)"},
+                     "frac": gbs32 / HBM_PEAK_GBS,
+                     "traffic": TRAFFIC_BYTES_PER_APPLY_FP32.get((args.edge_cells, args.nlayers)),
+                     "algorithmic_bytes": b32,
+                     "kernel": "k_lma_colour<float,8,8,RolesW0K0>; bytes, time and traffic per apply "
+                               "(4 colour launches)"},
     }
     if not args.no_cpu_baseline:
         bh = b.export_field()

@@ -50,6 +50,7 @@ FunctionSpace::~FunctionSpace() {
   dfree(d_map, map_bytes);
   dfree(d_first, flag_bytes);
   dfree(d_last, flag_bytes);
+  dfree(d_ebase, ebase_bytes);
 }
 
 // Scratch vectors of the solvers, kept between solves (grown on demand).
@@ -169,6 +170,22 @@ static int finish_space(std::shared_ptr<FunctionSpace> &fs) {
   std::vector<int> mpos(static_cast<size_t>(nc) * ndf);
   for (int p = 0; p < nc; ++p)
     for (int d = 0; d < ndf; ++d) mpos[static_cast<size_t>(p) * ndf + d] = f.dofmap[f.cell_of_pos[p] * ndf + d] - 1;
+  // element slots of the operator layout
+  f.ebase.resize(nc);
+  {
+    long long e0 = 0;
+    for (int c = 0; c < f.ncolours; ++c) {
+      for (int p = f.colour_start[c]; p < f.colour_start[c + 1]; ++p)
+        f.ebase[p] = static_cast<int>(e0 + static_cast<long long>(p - f.colour_start[c]) * (f.nlayers + 1));
+      e0 += static_cast<long long>(f.colour_start[c + 1] - f.colour_start[c]) * (f.nlayers + 1);
+      e0 = (e0 + 63) / 64 * 64;
+    }
+    LF_CHECK(e0 < (1LL << 31), LFRIC_E_UNSUPPORTED, "mesh too large for 32-bit element slots");
+    f.nelem_padded = std::max<long long>(e0, 64);
+  }
+  f.ebase_bytes = sizeof(int) * nc;
+  if (int rc = dmalloc(reinterpret_cast<void **>(&f.d_ebase), f.ebase_bytes)) return rc;
+  LF_HIP(hipMemcpy(f.d_ebase, f.ebase.data(), f.ebase_bytes, hipMemcpyHostToDevice));
   f.map_bytes = sizeof(int) * mpos.size();
   f.flag_bytes = sizeof(uint32_t) * nc;
   if (int rc = dmalloc(reinterpret_cast<void **>(&f.d_map), f.map_bytes)) return rc;
@@ -542,13 +559,15 @@ int lfric_op_create(const lfric_fs *to, const lfric_fs *from, int precision, lfr
   op->to = to->fs;
   op->from = from->fs;
   op->precision = precision;
-  const long long nc3 = static_cast<long long>(t.ncells) * t.nlayers;
-  op->plane_stride = (nc3 + 63) / 64 * 64;
-  op->bytes = esize(precision) * static_cast<size_t>(op->plane_stride) * t.ndf * f.ndf;
+  op->gs = static_cast<size_t>(64) * t.ndf * f.ndf;
+  op->ngroups = t.nelem_padded / 64;
+  op->row_top = t.top_mask;
+  op->bytes = esize(precision) * op->gs * static_cast<size_t>(op->ngroups);
   if (int rc = dmalloc(&op->d, op->bytes)) {
     delete op;
     return rc;
   }
+  LF_HIP(hipMemsetAsync(op->d, 0, op->bytes, ctx().stream));
   *out = op;
   return LFRIC_OK;
 }

@@ -73,11 +73,15 @@ struct FunctionSpace {
   // column roles: df is "top" when map(df) = map(bottom)+1 in every cell
   uint32_t top_mask = 0;
   std::vector<int> partner;                // bottom df -> top df or -1
+  // operator element slots: column pos owns ebase[pos] + 0..nlayers; colours start 64-aligned
+  std::vector<int> ebase;
+  long long nelem_padded = 0;
   // device
+  int *d_ebase = nullptr;
   int *d_map = nullptr;                    // [pos][ndf] 0-based column-base dofs
   uint32_t *d_first = nullptr;             // [pos] bit df: first touch of df's column
   uint32_t *d_last = nullptr;              // [pos] bit df: last touch of df's column
-  size_t map_bytes = 0, flag_bytes = 0;
+  size_t map_bytes = 0, flag_bytes = 0, ebase_bytes = 0;
   ~FunctionSpace();
 };
 
@@ -99,8 +103,10 @@ struct lfric_vec {
 struct lfric_op {
   std::shared_ptr<lfric::FunctionSpace> to, from;
   int precision = LFRIC_R64;
-  long long plane_stride = 0;  // elements per (df1, df2) plane, padded
-  void *d = nullptr;           // [df1*ndf2+df2][pos*nl + k]
+  size_t gs = 0;               // values per 64-element group: 64 * ndf_to * ndf_from
+  long long ngroups = 0;       // 64-element groups
+  uint32_t row_top = 0;        // rows stored one level up (top dofs of the to space)
+  void *d = nullptr;           // level-aligned AoSoA layout (see kernels.hip)
   size_t bytes = 0;
 };
 

@@ -2,11 +2,14 @@
 // layout changes, the colour-ordered LMA apply and its siblings, vector updates and
 // deterministic single-pass reductions.
 //
-// Device operator layout: plane-major, A[(df1*ndf2 + df2) * S + pos*nlayers + k], where pos
-// is the cell's position in colour-major order (cells of colour 1, then colour 2, ...) and
-// S >= ncells*nlayers is padded to 64 elements.  Along a plane the layers of one column are
-// contiguous and consecutive columns of one colour follow each other, so a wavefront of
-// (cell, level) threads reads every plane as one contiguous run.
+// Device operator layout ("level-aligned AoSoA").  Cells are taken in colour-major order
+// (pos); column pos owns the elements ebase[pos] + m, m = 0..nlayers (one slot per vertex
+// level), colours start on 64-element boundaries.  Entry (df1,df2) of 3D cell (pos, k) lives
+// at element e = ebase[pos] + k + top(df1), where top(df1) = 1 when df1 is the top dof of a
+// column pair (so both rows that feed output level m sit at the same e), and at address
+// (e/64)*64*nplanes + (df1*ndf2 + df2)*64 + e%64.  One wavefront of (cell, level) threads thus
+// reads every plane of its 64 elements as one 256-byte line, and all planes of a wavefront
+// form one contiguous block.
 //
 // LMA apply thread decomposition.  The reference kernel adds A_k x_k column by column into
 // lhs(map1(df)+k) (components/science/source/kernel/algebra/matrix_vector_kernel_mod.F90:
@@ -84,6 +87,10 @@ __device__ inline double to_d(T v) {
   return static_cast<double>(v);
 }
 
+__device__ inline size_t aidx(size_t gs, long long e, int p) {
+  return static_cast<size_t>(e >> 6) * gs + static_cast<size_t>(p) * 64 + static_cast<size_t>(e & 63);
+}
+
 // ---------------------------------------------------------------------------------------
 // synthetic operators
 // ---------------------------------------------------------------------------------------
@@ -93,20 +100,21 @@ __device__ inline double to_d(T v) {
 // sc(j) = exp((4u-2)*ln10) when scaled, A(i,j) = fma chain over m in order, + shift on the
 // diagonal, mirrored.
 template <typename T, int NDF>
-__global__ __launch_bounds__(kBlock) void k_gen_spd(T *__restrict__ A, size_t S, const int *__restrict__ cell_of_pos,
-                                                    int nl, long long nelem, unsigned long long seed,
-                                                    int scaled, double shift) {
+__global__ __launch_bounds__(kBlock) void k_gen_spd(T *__restrict__ A, size_t gs, const int *__restrict__ ebase,
+                                                    const int *__restrict__ cell_of_pos, int nl, long long nelem,
+                                                    unsigned long long seed, int scaled, double shift, uint32_t row_top) {
   constexpr int kWaves = kBlock / 64;
   __shared__ double B[kWaves][NDF * NDF];
   __shared__ double sc[kWaves][NDF];
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const long long e = static_cast<long long>(blockIdx.x) * kWaves + w;
   const bool active = e < nelem;
-  long long ic = 0;
+  long long ic = 0, el = 0;
   if (active) {
     const long long pos = e / nl;
     const int k = static_cast<int>(e - pos * nl);
     ic = static_cast<long long>(cell_of_pos[pos]) * nl + k;  // reference 3D cell index
+    el = ebase[pos] + k;
   }
   const double ln10 = 0x1.26bb1bbb55516p+1;
   if (active) {
@@ -124,35 +132,36 @@ __global__ __launch_bounds__(kBlock) void k_gen_spd(T *__restrict__ A, size_t S,
   }
   __syncthreads();
   if (active) {
-    const size_t col = static_cast<size_t>(e);
     for (int q = lane; q < NDF * NDF; q += 64) {
       const int i = q / NDF, j = q % NDF;
       if (j < i) continue;
       double acc = 0.0;
       for (int m = 0; m < NDF; ++m) acc = fma(B[w][i * NDF + m], B[w][j * NDF + m], acc);
       if (i == j) acc += shift;
-      A[static_cast<size_t>(i * NDF + j) * S + col] = static_cast<T>(acc);
-      A[static_cast<size_t>(j * NDF + i) * S + col] = static_cast<T>(acc);
+      A[aidx(gs, el + ((row_top >> i) & 1u), i * NDF + j)] = static_cast<T>(acc);
+      A[aidx(gs, el + ((row_top >> j) & 1u), j * NDF + i)] = static_cast<T>(acc);
     }
   }
 }
 
 // Entry (df1,df2) of element (pos,k) = sign(df2) * (lo + (hi-lo)*u), counter = ic*n1*n2 + df1*n2 + df2
 template <typename T>
-__global__ void k_gen_uniform(T *__restrict__ A, size_t S, const int *__restrict__ cell_of_pos, int nl,
-                              long long nelem, int n1, int n2, unsigned long long seed,
-                              unsigned long long stream, double lo, double hi, const double *__restrict__ sign) {
+__global__ void k_gen_uniform(T *__restrict__ A, size_t gs, const int *__restrict__ ebase,
+                              const int *__restrict__ cell_of_pos, int nl, long long nelem, int n1, int n2,
+                              unsigned long long seed, unsigned long long stream, double lo, double hi,
+                              const double *__restrict__ sign, uint32_t row_top) {
   const long long e = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (e >= nelem) return;
   const long long pos = e / nl;
   const int k = static_cast<int>(e - pos * nl);
   const long long ic = static_cast<long long>(cell_of_pos[pos]) * nl + k;
+  const long long el = ebase[pos] + k;
   for (int i = 0; i < n1; ++i)
     for (int j = 0; j < n2; ++j) {
       const double u = rng::uniform(seed, stream, static_cast<uint64_t>(ic) * n1 * n2 + i * n2 + j);
       double v = lo + (hi - lo) * u;
       if (sign) v = v * sign[j];
-      A[static_cast<size_t>(i * n2 + j) * S + e] = static_cast<T>(v);
+      A[aidx(gs, el + ((row_top >> i) & 1u), i * n2 + j)] = static_cast<T>(v);
     }
 }
 
@@ -164,35 +173,36 @@ __global__ void k_vec_uniform(T *__restrict__ v, long long n, unsigned long long
 }
 #pragma clang fp contract(on)
 
-// reference layout matrix(ncell_3d, ndf1, ndf2) <-> device plane layout
+// reference layout matrix(ncell_3d, ndf1, ndf2) <-> device layout
 template <typename T>
-__global__ void k_from_reference(T *__restrict__ A, size_t S, const double *__restrict__ R,
-                                 const int *__restrict__ cell_of_pos, int nl, long long nelem, int nplanes,
-                                 int n1, int n2) {
+__global__ void k_from_reference(T *__restrict__ A, size_t gs, const int *__restrict__ ebase,
+                                 const double *__restrict__ R, const int *__restrict__ cell_of_pos, int nl,
+                                 long long nelem, int n1, int n2, uint32_t row_top) {
   const long long e = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (e >= nelem) return;
   const long long pos = e / nl;
   const int k = static_cast<int>(e - pos * nl);
   const long long ic = static_cast<long long>(cell_of_pos[pos]) * nl + k;
-  for (int p = 0; p < nplanes; ++p) {
-    const int i = p / n2, j = p % n2;
-    A[static_cast<size_t>(p) * S + e] = static_cast<T>(R[ic + nelem * (i + static_cast<long long>(n1) * j)]);
-  }
+  const long long el = ebase[pos] + k;
+  for (int i = 0; i < n1; ++i)
+    for (int j = 0; j < n2; ++j)
+      A[aidx(gs, el + ((row_top >> i) & 1u), i * n2 + j)] =
+          static_cast<T>(R[ic + nelem * (i + static_cast<long long>(n1) * j)]);
 }
 
 template <typename T>
-__global__ void k_to_reference(const T *__restrict__ A, size_t S, double *__restrict__ R,
-                               const int *__restrict__ cell_of_pos, int nl, long long nelem, int nplanes,
-                               int n1, int n2) {
+__global__ void k_to_reference(const T *__restrict__ A, size_t gs, const int *__restrict__ ebase,
+                               double *__restrict__ R, const int *__restrict__ cell_of_pos, int nl, long long nelem,
+                               int n1, int n2, uint32_t row_top) {
   const long long e = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (e >= nelem) return;
   const long long pos = e / nl;
   const int k = static_cast<int>(e - pos * nl);
   const long long ic = static_cast<long long>(cell_of_pos[pos]) * nl + k;
-  for (int p = 0; p < nplanes; ++p) {
-    const int i = p / n2, j = p % n2;
-    R[ic + nelem * (i + static_cast<long long>(n1) * j)] = to_d(A[static_cast<size_t>(p) * S + e]);
-  }
+  const long long el = ebase[pos] + k;
+  for (int i = 0; i < n1; ++i)
+    for (int j = 0; j < n2; ++j)
+      R[ic + nelem * (i + static_cast<long long>(n1) * j)] = to_d(A[aidx(gs, el + ((row_top >> i) & 1u), i * n2 + j)]);
 }
 
 template <typename TD, typename TS>
@@ -238,17 +248,12 @@ struct RolesRuntime {
   __device__ int partner(int i) const { return r.partner[i]; }
 };
 
-template <typename T>
-__device__ inline T ld_stream(const T *p) {
-  return __builtin_nontemporal_load(p);
-}
-
 template <typename T, int N1, int N2, class R>
-__global__ __launch_bounds__(kBlock) void k_lma_colour(const T *__restrict__ A, size_t S, const T *__restrict__ x,
-                                                       T *__restrict__ y, const int *__restrict__ map1,
-                                                       const int *__restrict__ map2,
-                                                       const uint32_t *__restrict__ first, int pos0, int npos,
-                                                       int nl, R roles, const int *__restrict__ skip) {
+__global__ __launch_bounds__(kBlock) void k_lma_colour(const T *__restrict__ A, size_t gs, const int *__restrict__ ebase,
+                                                       const T *__restrict__ x, T *__restrict__ y,
+                                                       const int *__restrict__ map1, const int *__restrict__ map2,
+                                                       const uint32_t *__restrict__ first, int pos0, int npos, int nl,
+                                                       R roles, const int *__restrict__ skip) {
   if (skip && *skip) return;
   const int H = nl + 1;
   const int t = blockIdx.x * kBlock + threadIdx.x;
@@ -259,6 +264,7 @@ __global__ __launch_bounds__(kBlock) void k_lma_colour(const T *__restrict__ A, 
   const int *m1 = map1 + static_cast<size_t>(pos) * N1;
   const int *m2 = map2 + static_cast<size_t>(pos) * N2;
   const bool has_m = m < nl, has_p = m > 0;
+  const long long e = static_cast<long long>(ebase[pos]) + m;
   T xm[N2], xp[N2];
 #pragma unroll
   for (int j = 0; j < N2; ++j) {
@@ -267,17 +273,13 @@ __global__ __launch_bounds__(kBlock) void k_lma_colour(const T *__restrict__ A, 
     xp[j] = has_p ? x[b + m - 1] : T(0);
   }
   const uint32_t ft = first ? first[pos] : 0xffffffffu;
-  const size_t cm = static_cast<size_t>(pos) * nl + m;
   if constexpr (R::kStatic && N1 * N2 <= 64) {
-    // every row of this thread: bottom rows at layer m, top rows at layer m-1
+    // every row of this thread sits at element e: bottom rows of layer m, top rows of layer m-1
     T a[N1][N2];
 #pragma unroll
     for (int i = 0; i < N1; ++i)
 #pragma unroll
-      for (int j = 0; j < N2; ++j) {
-        const bool tp = R::top(i);
-        a[i][j] = (tp ? has_p : has_m) ? ld_stream(A + static_cast<size_t>(i * N2 + j) * S + cm - (tp ? 1 : 0)) : T(0);
-      }
+      for (int j = 0; j < N2; ++j) a[i][j] = (R::top(i) ? has_p : has_m) ? A[aidx(gs, e, i * N2 + j)] : T(0);
     T c[N1];
 #pragma unroll
     for (int i = 0; i < N1; ++i) {
@@ -307,12 +309,11 @@ __global__ __launch_bounds__(kBlock) void k_lma_colour(const T *__restrict__ A, 
       T acc = T(0);
       if (has_m) {
 #pragma unroll
-        for (int j = 0; j < N2; ++j) acc = fma(ld_stream(A + static_cast<size_t>(i * N2 + j) * S + cm), xm[j], acc);
+        for (int j = 0; j < N2; ++j) acc = fma(A[aidx(gs, e, i * N2 + j)], xm[j], acc);
       }
       if (pt >= 0 && has_p) {
 #pragma unroll
-        for (int j = 0; j < N2; ++j)
-          acc = fma(ld_stream(A + static_cast<size_t>(pt * N2 + j) * S + cm - 1), xp[j], acc);
+        for (int j = 0; j < N2; ++j) acc = fma(A[aidx(gs, e, pt * N2 + j)], xp[j], acc);
       }
       const size_t d = static_cast<size_t>(m1[i]) + m;
       if ((ft >> i) & 1u)
@@ -323,14 +324,15 @@ __global__ __launch_bounds__(kBlock) void k_lma_colour(const T *__restrict__ A, 
   }
 }
 
-// y = A^T x: thread (cell, level m) of the `from` space; planes indexed (to_df*NF + from_df)
+// y = A^T x: thread (cell, level m) of the `from` space; entry (to_df i, from_df j) of layer k
+// sits at element ebase + k + top_to(i)
 template <typename T, int NT, int NF>
-__global__ __launch_bounds__(kBlock) void k_transpose_colour(const T *__restrict__ A, size_t S,
-                                                             const T *__restrict__ x, T *__restrict__ y,
-                                                             const int *__restrict__ map_to,
+__global__ __launch_bounds__(kBlock) void k_transpose_colour(const T *__restrict__ A, size_t gs,
+                                                             const int *__restrict__ ebase, const T *__restrict__ x,
+                                                             T *__restrict__ y, const int *__restrict__ map_to,
                                                              const int *__restrict__ map_from,
-                                                             const uint32_t *__restrict__ first, int pos0,
-                                                             int npos, int nl, Roles roles) {
+                                                             const uint32_t *__restrict__ first, int pos0, int npos,
+                                                             int nl, Roles roles, uint32_t row_top) {
   const int H = nl + 1;
   const int t = blockIdx.x * kBlock + threadIdx.x;
   if (t >= npos * H) return;
@@ -340,6 +342,7 @@ __global__ __launch_bounds__(kBlock) void k_transpose_colour(const T *__restrict
   const int *mt = map_to + static_cast<size_t>(pos) * NT;
   const int *mf = map_from + static_cast<size_t>(pos) * NF;
   const bool has_m = m < nl, has_p = m > 0;
+  const long long e = static_cast<long long>(ebase[pos]) + m;
   T xm[NT], xp[NT];
 #pragma unroll
   for (int i = 0; i < NT; ++i) {
@@ -347,7 +350,6 @@ __global__ __launch_bounds__(kBlock) void k_transpose_colour(const T *__restrict
     xp[i] = has_p ? x[mt[i] + m - 1] : T(0);
   }
   const uint32_t ft = first[pos];
-  const size_t cm = static_cast<size_t>(pos) * nl + m;
 #pragma unroll
   for (int j = 0; j < NF; ++j) {
     if ((roles.top_mask >> j) & 1u) continue;
@@ -356,11 +358,11 @@ __global__ __launch_bounds__(kBlock) void k_transpose_colour(const T *__restrict
     T acc = T(0);
     if (has_m) {
 #pragma unroll
-      for (int i = 0; i < NT; ++i) acc = fma(A[static_cast<size_t>(i * NF + j) * S + cm], xm[i], acc);
+      for (int i = 0; i < NT; ++i) acc = fma(A[aidx(gs, e + ((row_top >> i) & 1u), i * NF + j)], xm[i], acc);
     }
     if (pt >= 0 && has_p) {
 #pragma unroll
-      for (int i = 0; i < NT; ++i) acc = fma(A[static_cast<size_t>(i * NF + pt) * S + cm - 1], xp[i], acc);
+      for (int i = 0; i < NT; ++i) acc = fma(A[aidx(gs, e - 1 + ((row_top >> i) & 1u), i * NF + pt)], xp[i], acc);
     }
     const size_t d = static_cast<size_t>(mf[j]) + m;
     if ((ft >> j) & 1u)
@@ -372,9 +374,10 @@ __global__ __launch_bounds__(kBlock) void k_transpose_colour(const T *__restrict
 
 // assembled diagonal: sum over cells of A_k(df,df) (sci_mm_diagonal_kernel_mod.F90:87-92)
 template <typename T, int N>
-__global__ __launch_bounds__(kBlock) void k_diag_colour(const T *__restrict__ A, size_t S, T *__restrict__ d,
-                                                        const int *__restrict__ map, const uint32_t *__restrict__ first,
-                                                        int pos0, int npos, int nl, Roles roles) {
+__global__ __launch_bounds__(kBlock) void k_diag_colour(const T *__restrict__ A, size_t gs, const int *__restrict__ ebase,
+                                                        T *__restrict__ d, const int *__restrict__ map,
+                                                        const uint32_t *__restrict__ first, int pos0, int npos, int nl,
+                                                        Roles roles) {
   const int H = nl + 1;
   const int t = blockIdx.x * kBlock + threadIdx.x;
   if (t >= npos * H) return;
@@ -383,15 +386,15 @@ __global__ __launch_bounds__(kBlock) void k_diag_colour(const T *__restrict__ A,
   const int pos = pos0 + lp;
   const int *mp = map + static_cast<size_t>(pos) * N;
   const uint32_t ft = first[pos];
-  const size_t cm = static_cast<size_t>(pos) * nl + m;
+  const long long e = static_cast<long long>(ebase[pos]) + m;
 #pragma unroll
   for (int i = 0; i < N; ++i) {
     if ((roles.top_mask >> i) & 1u) continue;
     const int pt = roles.partner[i];
     if (m == nl && pt < 0) continue;
     T acc = T(0);
-    if (m < nl) acc += A[static_cast<size_t>(i * N + i) * S + cm];
-    if (pt >= 0 && m > 0) acc += A[static_cast<size_t>(pt * N + pt) * S + cm - 1];
+    if (m < nl) acc += A[aidx(gs, e, i * N + i)];
+    if (pt >= 0 && m > 0) acc += A[aidx(gs, e, pt * N + pt)];
     const size_t o = static_cast<size_t>(mp[i]) + m;
     if ((ft >> i) & 1u)
       d[o] = acc;
@@ -660,8 +663,8 @@ int launch_gen_spd(lfric_op *op, unsigned long long seed, int scaled, double shi
   auto go = [&](auto tag, auto ndfc) {
     using T = decltype(tag);
     constexpr int N = decltype(ndfc)::value;
-    LAUNCH((k_gen_spd<T, N>), grid, static_cast<T *>(op->d), static_cast<size_t>(op->plane_stride), d_cop,
-           f.nlayers, nelem, seed, scaled, shift);
+    LAUNCH((k_gen_spd<T, N>), grid, static_cast<T *>(op->d), op->gs, f.d_ebase, d_cop, f.nlayers, nelem, seed,
+           scaled, shift, op->row_top);
   };
   switch (f.ndf) {
     case 1: op->precision == LFRIC_R64 ? go(double{}, std::integral_constant<int, 1>{}) : go(float{}, std::integral_constant<int, 1>{}); break;
@@ -693,11 +696,11 @@ int launch_gen_uniform(lfric_op *op, unsigned long long seed, unsigned long long
   LF_HIP(hipMemcpyAsync(d_cop, f.cell_of_pos.data(), bytes, hipMemcpyHostToDevice, ctx().stream));
   const int grid = grid_for(nelem);
   if (op->precision == LFRIC_R64)
-    LAUNCH(k_gen_uniform<double>, grid, static_cast<double *>(op->d), static_cast<size_t>(op->plane_stride), d_cop,
-           f.nlayers, nelem, n1, n2, seed, stream, lo, hi, d_sign);
+    LAUNCH(k_gen_uniform<double>, grid, static_cast<double *>(op->d), op->gs, f.d_ebase, d_cop, f.nlayers, nelem, n1,
+           n2, seed, stream, lo, hi, d_sign, op->row_top);
   else
-    LAUNCH(k_gen_uniform<float>, grid, static_cast<float *>(op->d), static_cast<size_t>(op->plane_stride), d_cop,
-           f.nlayers, nelem, n1, n2, seed, stream, lo, hi, d_sign);
+    LAUNCH(k_gen_uniform<float>, grid, static_cast<float *>(op->d), op->gs, f.d_ebase, d_cop, f.nlayers, nelem, n1,
+           n2, seed, stream, lo, hi, d_sign, op->row_top);
   hipError_t e = hipGetLastError();
   LF_HIP(hipStreamSynchronize(ctx().stream));
   dfree(d_cop, bytes);
@@ -731,11 +734,11 @@ int launch_op_from_reference(lfric_op *op, const double *d_ref) {
   const long long nelem = static_cast<long long>(f.ncells) * f.nlayers;
   const int n1 = op->to->ndf, n2 = op->from->ndf;
   if (op->precision == LFRIC_R64)
-    LAUNCH(k_from_reference<double>, grid_for(nelem), static_cast<double *>(op->d), static_cast<size_t>(op->plane_stride),
-           d_ref, d_cop, f.nlayers, nelem, n1 * n2, n1, n2);
+    LAUNCH(k_from_reference<double>, grid_for(nelem), static_cast<double *>(op->d), op->gs, f.d_ebase, d_ref, d_cop,
+           f.nlayers, nelem, n1, n2, op->row_top);
   else
-    LAUNCH(k_from_reference<float>, grid_for(nelem), static_cast<float *>(op->d), static_cast<size_t>(op->plane_stride),
-           d_ref, d_cop, f.nlayers, nelem, n1 * n2, n1, n2);
+    LAUNCH(k_from_reference<float>, grid_for(nelem), static_cast<float *>(op->d), op->gs, f.d_ebase, d_ref, d_cop,
+           f.nlayers, nelem, n1, n2, op->row_top);
   hipError_t e = hipGetLastError();
   LF_HIP(hipStreamSynchronize(ctx().stream));
   dfree(d_cop, bytes);
@@ -751,11 +754,11 @@ int launch_op_to_reference(const lfric_op *op, double *d_ref) {
   const long long nelem = static_cast<long long>(f.ncells) * f.nlayers;
   const int n1 = op->to->ndf, n2 = op->from->ndf;
   if (op->precision == LFRIC_R64)
-    LAUNCH(k_to_reference<double>, grid_for(nelem), static_cast<const double *>(op->d),
-           static_cast<size_t>(op->plane_stride), d_ref, d_cop, f.nlayers, nelem, n1 * n2, n1, n2);
+    LAUNCH(k_to_reference<double>, grid_for(nelem), static_cast<const double *>(op->d), op->gs, f.d_ebase, d_ref,
+           d_cop, f.nlayers, nelem, n1, n2, op->row_top);
   else
-    LAUNCH(k_to_reference<float>, grid_for(nelem), static_cast<const float *>(op->d),
-           static_cast<size_t>(op->plane_stride), d_ref, d_cop, f.nlayers, nelem, n1 * n2, n1, n2);
+    LAUNCH(k_to_reference<float>, grid_for(nelem), static_cast<const float *>(op->d), op->gs, f.d_ebase, d_ref,
+           d_cop, f.nlayers, nelem, n1, n2, op->row_top);
   hipError_t e = hipGetLastError();
   LF_HIP(hipStreamSynchronize(ctx().stream));
   dfree(d_cop, bytes);
@@ -780,8 +783,8 @@ int launch_convert(int pd, void *dst, int ps, const void *src, long long n, doub
 }
 
 int launch_op_convert(const lfric_op *src, lfric_op *dst) {
-  LF_CHECK(src->plane_stride == dst->plane_stride, LFRIC_E_ARG, "operator shapes differ");
-  const long long n = src->plane_stride * static_cast<long long>(src->to->ndf) * src->from->ndf;
+  LF_CHECK(src->gs == dst->gs && src->ngroups == dst->ngroups, LFRIC_E_ARG, "operator shapes differ");
+  const long long n = static_cast<long long>(src->gs) * src->ngroups;
   return launch_convert(dst->precision, dst->d, src->precision, src->d, n, 1.0);
 }
 
@@ -812,25 +815,25 @@ static void launch_colour(RoleKind rk, int grid, const lfric_op *op, size_t S, c
   switch (rk) {
     case RoleKind::W0K0:
       if constexpr (A_ == 8)
-        LAUNCH((k_lma_colour<T, A_, B_, RolesW0K0>), grid, Ad, S, x, y, to.d_map, from.d_map, first, pos0, npos, nl,
+        LAUNCH((k_lma_colour<T, A_, B_, RolesW0K0>), grid, Ad, S, to.d_ebase, x, y, to.d_map, from.d_map, first, pos0, npos, nl,
                RolesW0K0{}, d_skip);
       break;
     case RoleKind::W0K1:
       if constexpr (A_ == 27)
-        LAUNCH((k_lma_colour<T, A_, B_, RolesW0K1>), grid, Ad, S, x, y, to.d_map, from.d_map, first, pos0, npos, nl,
+        LAUNCH((k_lma_colour<T, A_, B_, RolesW0K1>), grid, Ad, S, to.d_ebase, x, y, to.d_map, from.d_map, first, pos0, npos, nl,
                RolesW0K1{}, d_skip);
       break;
     case RoleKind::W2K0:
       if constexpr (A_ == 6)
-        LAUNCH((k_lma_colour<T, A_, B_, RolesW2K0>), grid, Ad, S, x, y, to.d_map, from.d_map, first, pos0, npos, nl,
+        LAUNCH((k_lma_colour<T, A_, B_, RolesW2K0>), grid, Ad, S, to.d_ebase, x, y, to.d_map, from.d_map, first, pos0, npos, nl,
                RolesW2K0{}, d_skip);
       break;
     case RoleKind::None:
-      LAUNCH((k_lma_colour<T, A_, B_, RolesNone>), grid, Ad, S, x, y, to.d_map, from.d_map, first, pos0, npos, nl,
+      LAUNCH((k_lma_colour<T, A_, B_, RolesNone>), grid, Ad, S, to.d_ebase, x, y, to.d_map, from.d_map, first, pos0, npos, nl,
              RolesNone{}, d_skip);
       break;
     default:
-      LAUNCH((k_lma_colour<T, A_, B_, RolesRuntime>), grid, Ad, S, x, y, to.d_map, from.d_map, first, pos0, npos, nl,
+      LAUNCH((k_lma_colour<T, A_, B_, RolesRuntime>), grid, Ad, S, to.d_ebase, x, y, to.d_map, from.d_map, first, pos0, npos, nl,
              RolesRuntime{roles_of(to)}, d_skip);
   }
 }
@@ -839,7 +842,7 @@ template <typename T>
 static int apply_colours_t(const lfric_op *op, const T *x, T *y, const int *d_skip) {
   const FunctionSpace &to = *op->to, &from = *op->from;
   RoleKind rk = role_kind(to);
-  const size_t S = static_cast<size_t>(op->plane_stride);
+  const size_t S = op->gs;
   const int nl = to.nlayers;
   for (int c = 0; c < to.ncolours; ++c) {
     const int pos0 = to.colour_start[c], npos = to.colour_start[c + 1] - pos0;
@@ -882,7 +885,7 @@ template <typename T>
 static int transpose_t(const lfric_op *op, const T *x, T *y) {
   const FunctionSpace &to = *op->to, &from = *op->from;
   const Roles roles = roles_of(from);
-  const size_t S = static_cast<size_t>(op->plane_stride);
+  const size_t S = op->gs;
   const int nl = from.nlayers;
   for (int c = 0; c < from.ncolours; ++c) {
     const int pos0 = from.colour_start[c], npos = from.colour_start[c + 1] - pos0;
@@ -890,8 +893,8 @@ static int transpose_t(const lfric_op *op, const T *x, T *y) {
     const int grid = grid_for(static_cast<long long>(npos) * (nl + 1));
 #define TR_CASE(A, B)                                                                                        \
   if (to.ndf == A && from.ndf == B) {                                                                       \
-    LAUNCH((k_transpose_colour<T, A, B>), grid, static_cast<const T *>(op->d), S, x, y, to.d_map, from.d_map, \
-           from.d_first, pos0, npos, nl, roles);                                                            \
+    LAUNCH((k_transpose_colour<T, A, B>), grid, static_cast<const T *>(op->d), S, from.d_ebase, x, y, to.d_map,   \
+           from.d_map, from.d_first, pos0, npos, nl, roles, op->row_top);                                  \
     continue;                                                                                               \
   }
     TR_CASE(1, 6)
@@ -918,13 +921,14 @@ static int dg_t(const lfric_op *op, const T *x, T *y) {
   const FunctionSpace &to = *op->to, &from = *op->from;
   LF_CHECK(to.top_mask == 0, LFRIC_E_ARG, "dg apply needs a discontinuous output space");
   const Roles roles = roles_of(to);
-  const size_t S = static_cast<size_t>(op->plane_stride);
+  const size_t S = op->gs;
   const int nl = to.nlayers, npos = to.ncells;
   const int grid = grid_for(static_cast<long long>(npos) * (nl + 1));
 #define DG_CASE(A, B)                                                                                            \
   if (to.ndf == A && from.ndf == B) {                                                                           \
-    LAUNCH((k_lma_colour<T, A, B, RolesRuntime>), grid, static_cast<const T *>(op->d), S, x, y, to.d_map,       \
-           from.d_map, static_cast<const uint32_t *>(nullptr), 0, npos, nl, RolesRuntime{roles},                 \
+    LAUNCH((k_lma_colour<T, A, B, RolesRuntime>), grid, static_cast<const T *>(op->d), S, to.d_ebase, x, y,     \
+           to.d_map,                                                                                           \
+           from.d_map, static_cast<const uint32_t *>(nullptr), 0, npos, nl, RolesRuntime{roles},               \
            static_cast<const int *>(nullptr));                                                                   \
     CHECK_LAUNCH();                                                                                             \
     return LFRIC_OK;                                                                                            \
@@ -947,14 +951,15 @@ static int diag_t(const lfric_op *op, T *d) {
   const FunctionSpace &f = *op->to;
   LF_CHECK(op->to == op->from, LFRIC_E_ARG, "mm_diagonal needs a square operator on one space");
   const Roles roles = roles_of(f);
-  const size_t S = static_cast<size_t>(op->plane_stride);
+  const size_t S = op->gs;
   for (int c = 0; c < f.ncolours; ++c) {
     const int pos0 = f.colour_start[c], npos = f.colour_start[c + 1] - pos0;
     if (npos == 0) continue;
     const int grid = grid_for(static_cast<long long>(npos) * (f.nlayers + 1));
 #define DI_CASE(A)                                                                                                    \
   if (f.ndf == A) {                                                                                                  \
-    LAUNCH((k_diag_colour<T, A>), grid, static_cast<const T *>(op->d), S, d, f.d_map, f.d_first, pos0, npos, f.nlayers, \
+    LAUNCH((k_diag_colour<T, A>), grid, static_cast<const T *>(op->d), S, f.d_ebase, d, f.d_map, f.d_first, pos0,     \
+           npos, f.nlayers,                                                                                          \
            roles);                                                                                                   \
     continue;                                                                                                        \
   }
