@@ -47,6 +47,7 @@ typedef struct lfric_fs lfric_fs;     /* mesh + function space: dofmap, colourin
 typedef struct lfric_op lfric_op;     /* LMA operator local_stencil (device) */
 typedef struct lfric_vec lfric_vec;   /* field vector data (device) */
 typedef struct lfric_prec lfric_prec; /* preconditioner */
+typedef struct lfric_linop lfric_linop; /* abstract linear operator (LMA or composite) */
 
 /* ---- library ------------------------------------------------------------------------ */
 /* Select the device and create the library stream (default priority). */
@@ -154,6 +155,29 @@ int lfric_prec_create_identity(lfric_prec **p);
 int lfric_prec_apply(const lfric_prec *p, const lfric_vec *x, lfric_vec *y);
 void lfric_prec_destroy(lfric_prec *p);
 
+/* ---- abstract linear operators (abstract_linear_operator_type, components/science/source/
+ *      solver/sci_linear_operator_mod.f90:18-36) ------------------------------------------- */
+/* y = A x for a square LMA operator (mass_matrix_operator_type,
+ * components/science/source/algorithm/solver/sci_mass_matrix_operator_alg_mod.x90:72-119). */
+int lfric_linop_create_lma(const lfric_op *A, lfric_linop **L);
+/* Helmholtz-like reduction of a W2/W3 saddle system on W3:
+ *   y = M3 x + tau * D (diag(M2)^-1 (D^T x)),
+ * D: W2 -> W3 LMA operator (dg_matrix_vector / transpose_matrix_vector kernels),
+ * m2_diag: assembled diagonal of the W2 mass matrix (mm_diagonal), M3: W3 -> W3 LMA operator.
+ * All three in one precision. */
+int lfric_linop_create_helmholtz(const lfric_op *M3, const lfric_op *D, const lfric_vec *m2_diag,
+                                 double tau, lfric_linop **L);
+int lfric_linop_apply(const lfric_linop *L, const lfric_vec *x, lfric_vec *y);
+void lfric_linop_destroy(lfric_linop *L);
+
+/* Vertical-column block-Jacobi preconditioner: the tridiagonal block of every dof column of the
+ * assembled operator (couplings between levels m and m+1 of one column), factorised in FP32
+ * and stored in `storage` precision (LFRIC_BF16, LFRIC_F16 or LFRIC_R32); applied with FP32
+ * arithmetic to FP32 or FP64 vectors.  Column solves of the reference's
+ * components/science/source/kernel/solver/sci_tri_solve_kernel_mod.f90 form, built from the
+ * element matrices instead of a columnwise operator. */
+int lfric_prec_create_column_jacobi(const lfric_linop *L, int storage, lfric_prec **p);
+
 /* ---- iterative solvers (abstract_iterative_solver_type and its CG / GMRES extensions,
  *      components/science/source/solver/sci_iterative_solver_mod.f90:38-57) ----------- */
 typedef struct lfric_solver_opts {
@@ -193,6 +217,17 @@ int lfric_cg_solve(const lfric_op *A, const lfric_prec *P, lfric_vec *x, const l
 int lfric_ir_cg_solve(const lfric_op *A64, const lfric_op *A32, const lfric_prec *P32,
                       lfric_vec *x, const lfric_vec *b, const lfric_solver_opts *opts,
                       lfric_solver_result *res);
+/* gmres_solve (sci_iterative_solver_mod.f90:830-1003): left-preconditioned restarted
+ * GMRES(gcrk), modified Gram-Schmidt Arnoldi with FP64-accumulated inner products, FP64
+ * Hessenberg / Givens least-squares on the device, true-residual test after each cycle.
+ * max_iter counts restart cycles, as in the reference. */
+int lfric_gmres_solve(const lfric_linop *A, const lfric_prec *P, lfric_vec *x, const lfric_vec *b,
+                      const lfric_solver_opts *opts, lfric_solver_result *res);
+/* FP64 iterative refinement around FP32 GMRES: r = b - A64 x in FP64, inner GMRES on A32 with
+ * FP32 vectors until the residual falls by inner_r_tol, x += d in FP64. */
+int lfric_ir_gmres_solve(const lfric_linop *A64, const lfric_linop *A32, const lfric_prec *P32,
+                         lfric_vec *x, const lfric_vec *b, const lfric_solver_opts *opts,
+                         lfric_solver_result *res);
 
 #ifdef __cplusplus
 }

@@ -15,14 +15,16 @@ Reference interfaces mirrored (paths relative to the lfric_core tree):
   ConjugateGradient      components/science/source/solver/sci_iterative_solver_mod.f90:87-121, 465-592
 """
 from ._lib import (LfricError, NotConverged, lib, init, synchronize, timer_start, timer_stop,
-                   device_bytes, R64, R32, W0, W2, W3)
+                   device_bytes, R64, R32, BF16, F16, W0, W2, W3)
 from .fem import FunctionSpace, FieldVector, LmaOperator
-from .solvers import (NullPreconditioner, DiagonalPreconditioner, ConjugateGradient,
-                      MixedPrecisionCG, SolverResult, SolverOptions)
+from .solvers import (NullPreconditioner, DiagonalPreconditioner, ColumnJacobiPreconditioner,
+                      LinearOperator, ConjugateGradient, MixedPrecisionCG, GMRES, MixedPrecisionGMRES,
+                      SolverResult, SolverOptions)
 
 __all__ = [
     "LfricError", "NotConverged", "lib", "init", "synchronize", "timer_start", "timer_stop",
-    "device_bytes", "R64", "R32", "W0", "W2", "W3", "FunctionSpace", "FieldVector", "LmaOperator",
-    "NullPreconditioner", "DiagonalPreconditioner", "ConjugateGradient", "MixedPrecisionCG",
+    "device_bytes", "R64", "R32", "BF16", "F16", "W0", "W2", "W3", "FunctionSpace", "FieldVector",
+    "LmaOperator", "NullPreconditioner", "DiagonalPreconditioner", "ColumnJacobiPreconditioner",
+    "LinearOperator", "ConjugateGradient", "MixedPrecisionCG", "GMRES", "MixedPrecisionGMRES",
     "SolverResult", "SolverOptions",
 ]

@@ -95,6 +95,13 @@ SIGNATURES = {
     "lfric_prec_create_identity": (_I, [ctypes.POINTER(_VP)]),
     "lfric_prec_apply": (_I, [_VP, _VP, _VP]),
     "lfric_prec_destroy": (None, [_VP]),
+    "lfric_linop_create_lma": (_I, [_VP, ctypes.POINTER(_VP)]),
+    "lfric_linop_create_helmholtz": (_I, [_VP, _VP, _VP, _D, ctypes.POINTER(_VP)]),
+    "lfric_linop_apply": (_I, [_VP, _VP, _VP]),
+    "lfric_linop_destroy": (None, [_VP]),
+    "lfric_prec_create_column_jacobi": (_I, [_VP, _I, ctypes.POINTER(_VP)]),
+    "lfric_gmres_solve": (_I, [_VP, _VP, _VP, _VP, ctypes.POINTER(SolverOpts), ctypes.POINTER(SolverRes)]),
+    "lfric_ir_gmres_solve": (_I, [_VP, _VP, _VP, _VP, _VP, ctypes.POINTER(SolverOpts), ctypes.POINTER(SolverRes)]),
     "lfric_solver_opts_default": (None, [ctypes.POINTER(SolverOpts)]),
     "lfric_residual_norm": (_I, [_VP, _VP, _VP, _DP]),
     "lfric_cg_solve": (_I, [_VP, _VP, _VP, _VP, ctypes.POINTER(SolverOpts), ctypes.POINTER(SolverRes)]),

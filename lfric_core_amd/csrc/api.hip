@@ -670,20 +670,21 @@ int lfric_mm_diagonal(const lfric_op *op, lfric_vec *diag) {
 // ---- preconditioners ----------------------------------------------------------------------------
 int lfric_prec_create_diagonal(const lfric_vec *diag, lfric_prec **out) {
   LF_CHECK(diag && out, LFRIC_E_ARG, "null argument");
-  *out = new lfric_prec{1, diag};
+  auto *p = new lfric_prec;
+  p->kind = 1;
+  p->diag = diag;
+  *out = p;
   return LFRIC_OK;
 }
 int lfric_prec_create_identity(lfric_prec **out) {
   LF_CHECK(out, LFRIC_E_ARG, "null argument");
-  *out = new lfric_prec{0, nullptr};
+  *out = new lfric_prec;
   return LFRIC_OK;
 }
 int lfric_prec_apply(const lfric_prec *p, const lfric_vec *x, lfric_vec *y) {
   if (int rc = require_init()) return rc;
-  LF_CHECK(p && x && y && x->n == y->n && x->precision == y->precision, LFRIC_E_ARG, "bad arguments");
-  if (p->kind == 0) return launch_copy(y->precision, y->d, x->d, y->n);
-  LF_CHECK(p->diag->n == x->n && p->diag->precision == x->precision, LFRIC_E_ARG, "diagonal does not match vector");
-  return launch_divide(y->precision, y->d, x->d, p->diag->d, y->n);
+  LF_CHECK(p && x && y && x->n == y->n && x->precision == y->precision && x->d != y->d, LFRIC_E_ARG, "bad arguments");
+  return prec_apply_any(p, x->precision, x->d, y->d, x->n);
 }
 void lfric_prec_destroy(lfric_prec *p) { delete p; }
 
@@ -722,6 +723,7 @@ static int finish_status(const SolverState &s, const lfric_solver_opts &o, lfric
 static int prec_diag_ptr(const lfric_prec *P, const lfric_op *A, const void **diag) {
   *diag = nullptr;
   if (!P || P->kind == 0) return LFRIC_OK;
+  LF_CHECK(P->kind == 1, LFRIC_E_UNSUPPORTED, "the fused CG takes an identity or diagonal preconditioner");
   LF_CHECK(P->diag->fs->undf == A->to->undf && P->diag->precision == A->precision, LFRIC_E_ARG,
            "preconditioner diagonal does not match the operator");
   *diag = P->diag->d;

@@ -111,6 +111,27 @@ struct lfric_op {
 };
 
 struct lfric_prec {
-  int kind = 0;  // 0 identity, 1 diagonal
+  int kind = 0;  // 0 identity, 1 diagonal, 2 vertical-column block-Jacobi
   const lfric_vec *diag = nullptr;
+  // column block-Jacobi: columns (base dof, height) and per-dof Thomas factors
+  int storage = LFRIC_R32;
+  long long undf = 0;
+  int ncols = 0;
+  int *d_cols = nullptr;   // [ncols][2] base (0-based), height
+  void *d_fac = nullptr;   // [3][undf]: lower multiplier, 1/pivot, super-diagonal
+  size_t cols_bytes = 0, fac_bytes = 0;
+  ~lfric_prec();
+};
+
+struct lfric_linop {
+  int kind = 0;  // 0 LMA, 1 Helmholtz composite
+  int precision = LFRIC_R64;
+  const lfric_op *A = nullptr;
+  const lfric_op *M3 = nullptr, *D = nullptr;
+  const lfric_vec *m2 = nullptr;
+  double tau = 0.0;
+  std::shared_ptr<lfric::FunctionSpace> space;  // domain and range
+  void *t_w2 = nullptr, *t_w3 = nullptr;      // scratch of the composite apply
+  size_t t_w2_bytes = 0, t_w3_bytes = 0;
+  ~lfric_linop();
 };

@@ -61,4 +61,8 @@ int launch_cg_update(int prec, void *x, void *r, void *z, const void *p, const v
                      long long n, SolverState *st);
 int launch_cg_aypx(int prec, void *p, const void *z, long long n, SolverState *st);
 
+// abstract operators and preconditioners (krylov.hip)
+int linop_apply(const lfric_linop *L, const void *x, void *y);
+int prec_apply_any(const lfric_prec *P, int prec, const void *x, void *y, long long n);
+
 }  // namespace lfric

@@ -4,7 +4,54 @@ sci_preconditioner_mod.f90, components/science/source/algorithm/solver/)."""
 import ctypes
 from dataclasses import dataclass
 
-from ._lib import (LFRIC_NOT_CONVERGED, LFRIC_OK, SolverOpts, SolverRes, check, lib)
+from ._lib import (BF16, LFRIC_NOT_CONVERGED, LFRIC_OK, SolverOpts, SolverRes, check, lib)
+
+
+class LinearOperator:
+    """abstract_linear_operator_type (components/science/source/solver/
+    sci_linear_operator_mod.f90:18-36): apply(x, y) computes y = A x on the device."""
+
+    def __init__(self, handle, space, precision, keep=()):
+        self._h, self.space, self.precision, self._keep = handle, space, precision, keep
+
+    @classmethod
+    def from_lma(cls, op):
+        h = ctypes.c_void_p()
+        check(lib().lfric_linop_create_lma(op._h, ctypes.byref(h)))
+        return cls(h, op.fs_to, op.precision, (op,))
+
+    @classmethod
+    def helmholtz(cls, M3, D, m2_diag, tau):
+        """y = M3 x + tau D diag(M2)^-1 D^T x on W3."""
+        h = ctypes.c_void_p()
+        check(lib().lfric_linop_create_helmholtz(M3._h, D._h, m2_diag._h, float(tau), ctypes.byref(h)))
+        return cls(h, M3.fs_to, M3.precision, (M3, D, m2_diag))
+
+    def apply(self, x, y):
+        check(lib().lfric_linop_apply(self._h, x._h, y._h))
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().lfric_linop_destroy(self._h)
+            self._h = None
+
+
+class ColumnJacobiPreconditioner:
+    """Vertical-column block-Jacobi: tridiagonal column blocks of the assembled operator,
+    factorised in FP32 and stored in `storage` precision (BF16 by default), applied in FP32."""
+
+    def __init__(self, linop, storage=BF16):
+        self.linop = linop
+        self._h = ctypes.c_void_p()
+        check(lib().lfric_prec_create_column_jacobi(linop._h, storage, ctypes.byref(self._h)))
+
+    def apply(self, x, y):
+        check(lib().lfric_prec_apply(self._h, x._h, y._h))
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().lfric_prec_destroy(self._h)
+            self._h = None
 
 
 class NullPreconditioner:
@@ -125,3 +172,37 @@ def residual_norm(op, x, b):
     out = ctypes.c_double()
     check(lib().lfric_residual_norm(op._h, x._h, b._h, ctypes.byref(out)))
     return out.value
+
+
+class GMRES(_Solver):
+    """gmres_type(lin_op, prec, gcrk, r_tol, a_tol, max_iter, monitor_convergence,
+    fail_on_non_converged) (sci_iterative_solver_mod.f90:161-196, 799-1003); max_iter counts
+    restart cycles.  lin_op is a LinearOperator."""
+
+    def __init__(self, lin_op, prec, gcrk, r_tol, a_tol, max_iter, monitor_convergence=True,
+                 fail_on_non_converged=False, **kw):
+        super().__init__(lin_op, prec, r_tol, a_tol, max_iter, monitor_convergence, fail_on_non_converged,
+                         gcrk=gcrk, **kw)
+
+    def apply(self, x, b):
+        opts = self.opts.c()
+        res = SolverRes()
+        p = self.prec._h if self.prec is not None else None
+        rc = lib().lfric_gmres_solve(self.lin_op._h, p, x._h, b._h, ctypes.byref(opts), ctypes.byref(res))
+        return self._finish(rc, res)
+
+
+class MixedPrecisionGMRES(_Solver):
+    """FP64 iterative refinement around FP32 GMRES(gcrk); prec acts on FP32 vectors."""
+
+    def __init__(self, lin_op, lin_op32, prec, gcrk, r_tol, a_tol, max_iter, inner_r_tol=1e-4, **kw):
+        super().__init__(lin_op, prec, r_tol, a_tol, max_iter, gcrk=gcrk, inner_r_tol=inner_r_tol, **kw)
+        self.lin_op32 = lin_op32
+
+    def apply(self, x, b):
+        opts = self.opts.c()
+        res = SolverRes()
+        p = self.prec._h if self.prec is not None else None
+        rc = lib().lfric_ir_gmres_solve(self.lin_op._h, self.lin_op32._h, p, x._h, b._h, ctypes.byref(opts),
+                                        ctypes.byref(res))
+        return self._finish(rc, res)

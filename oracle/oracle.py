@@ -66,6 +66,8 @@ def lib():
             "orc_exp": (_D, [_D]),
             "orc_gen_spd": (None, [_I, _I64, _U64, _I, _VP]),
             "orc_gen_uniform": (None, [_I64, _U64, _U64, _VP]),
+            "orc_gen_spd_shift": (None, [_I, _I64, _U64, _I, _D, _VP]),
+            "orc_gen_op_uniform": (None, [_I, _I, _I64, _U64, _U64, _D, _D, _VP, _VP]),
             "orc_lma_apply": (None, [_I, _I, _I, _VP, _VP, _I, _I, _I, _VP, _I, _I, _VP, _VP, _VP, _VP]),
             "orc_lma_apply_r32": (None, [_I, _I, _I, _VP, _VP, _I, _I, _I, _VP, _I, _I, _VP, _VP, _VP, _VP]),
             "orc_transpose_apply": (None, [_I, _I, _I, _VP, _VP, _I, _I, _I, _VP, _I, _I, _VP, _VP, _VP, _VP]),
@@ -162,11 +164,18 @@ class Mesh:
         self.ncell_3d = self.ncells * nlayers
 
 
-def gen_spd(ndf, ncell_3d, seed, scaled=False):
+def gen_spd(ndf, ncell_3d, seed, scaled=False, shift=0.1):
     """Element matrices in the reference layout matrix(ncell_3d, ndf, ndf) (Fortran order)."""
     A = np.zeros(ndf * ndf * ncell_3d, np.float64)
-    lib().orc_gen_spd(ndf, ncell_3d, seed, int(scaled), _p(A))
+    lib().orc_gen_spd_shift(ndf, ncell_3d, seed, int(scaled), float(shift), _p(A))
     return A.reshape((ndf, ndf, ncell_3d)).transpose(2, 1, 0)  # view as (ncell_3d, ndf, ndf)
+
+
+def gen_op_uniform(n1, n2, ncell_3d, seed, stream, lo, hi, sign=None):
+    A = np.zeros(n1 * n2 * ncell_3d, np.float64)
+    sg = None if sign is None else np.ascontiguousarray(sign, np.float64)
+    lib().orc_gen_op_uniform(n1, n2, ncell_3d, seed, stream, float(lo), float(hi), _p(sg), _p(A))
+    return A.reshape((n2, n1, ncell_3d)).transpose(2, 1, 0)
 
 
 def flat_matrix(A):
@@ -292,3 +301,50 @@ def assembled_csr(mesh, A, to_mesh=None):
                 vals.append(A[ik, i, j])
     return sp.csr_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))),
                          shape=(to.undf, mesh.undf))
+
+
+def transpose_apply(to_mesh, from_mesh, A, x):
+    """y (on to_mesh) = A^T x, A of shape (ncell_3d, from_ndf, to_ndf) as the reference holds it
+    for transpose_matrix_vector_kernel (lhs on ANY_SPACE_1 = to_mesh)."""
+    y = np.zeros(to_mesh.undf)
+    Af = flat_matrix(A)
+    lib().orc_transpose_apply(to_mesh.ncells, to_mesh.nlayers, 4, _p(to_mesh.ncells_per_colour),
+                              _p(to_mesh.cells_in_colour), to_mesh.cells_in_colour.shape[1], to_mesh.ndf,
+                              to_mesh.undf, _p(to_mesh.dofmap), from_mesh.ndf, from_mesh.undf,
+                              _p(from_mesh.dofmap), _p(Af), _p(np.ascontiguousarray(x, np.float64)), _p(y))
+    return y
+
+
+def dg_apply(to_mesh, from_mesh, A, x):
+    y = np.zeros(to_mesh.undf)
+    lib().orc_dg_apply(to_mesh.ncells, to_mesh.nlayers, to_mesh.ndf, to_mesh.undf, _p(to_mesh.dofmap),
+                       from_mesh.ndf, from_mesh.undf, _p(from_mesh.dofmap), _p(flat_matrix(A)),
+                       _p(np.ascontiguousarray(x, np.float64)), _p(y))
+    return y
+
+
+D_SIGNS = np.array([-1.0, -1.0, 1.0, 1.0, -1.0, 1.0])  # outward normals of faces W,S,E,N,B,T
+
+
+class Helmholtz:
+    """H = M3 + tau D diag(M2)^-1 D^T on W3 with the synthetic element matrices of the mixed
+    W2/W3 configuration: M2_e = C C^T + I (6x6, C ~ N(0,1)/8), D_e = sign(face) U(0.5,1.5)
+    (1x6), M3_e = c^2 + 1 (c ~ N(0,1)/8)."""
+
+    def __init__(self, n, nl, seed=0, tau=0.5):
+        self.w2 = Mesh(n, nl, W2, 0)
+        self.w3 = Mesh(n, nl, W3, 0)
+        nc3 = self.w3.ncell_3d
+        self.M2 = gen_spd(6, nc3, seed, shift=1.0)
+        self.D = gen_op_uniform(1, 6, nc3, seed, 5, 0.5, 1.5, D_SIGNS)
+        self.M3 = gen_spd(1, nc3, seed + 1, shift=1.0)
+        self.m2d = mm_diagonal(self.w2, self.M2)
+        self.tau = tau
+
+    def apply(self, x):
+        t = transpose_apply(self.w2, self.w3, self.D, x) / self.m2d
+        return dg_apply(self.w3, self.w3, self.M3, x) + self.tau * dg_apply(self.w3, self.w2, self.D, t)
+
+    def dense(self):
+        n = self.w3.undf
+        return np.column_stack([self.apply(np.eye(1, n, i).ravel()) for i in range(n)])

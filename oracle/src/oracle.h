@@ -55,7 +55,11 @@ double orc_rng_normal(uint64_t seed, uint64_t stream, uint64_t ctr);
 double orc_log(double x);
 double orc_exp(double x);
 void orc_gen_spd(int ndf, int64_t ncell_3d, uint64_t seed, int scaled, double *matrix);
+void orc_gen_spd_shift(int ndf, int64_t ncell_3d, uint64_t seed, int scaled, double shift,
+                       double *matrix);
 void orc_gen_uniform(int64_t n, uint64_t seed, uint64_t stream, double *v);
+void orc_gen_op_uniform(int n1, int n2, int64_t ncell_3d, uint64_t seed, uint64_t stream, double lo,
+                        double hi, const double *sign, double *matrix);
 
 /* ---- LMA kernels ---------------------------------------------------------------------- */
 void orc_lma_apply(int ncells, int nlayers, int ncolours, const int *ncells_per_colour,

@@ -108,6 +108,10 @@ double orc_rng_normal(uint64_t seed, uint64_t stream, uint64_t ctr) {
 enum { STREAM_B = 1, STREAM_SCALE = 2 };
 
 void orc_gen_spd(int ndf, int64_t nc3, uint64_t seed, int scaled, double *A) {
+  orc_gen_spd_shift(ndf, nc3, seed, scaled, 0.1, A);
+}
+
+void orc_gen_spd_shift(int ndf, int64_t nc3, uint64_t seed, int scaled, double shift, double *A) {
   double B[27 * 27];
   double sc[27];
   const double ln10 = 0x1.26bb1bbb55516p+1;
@@ -125,7 +129,7 @@ void orc_gen_spd(int ndf, int64_t nc3, uint64_t seed, int scaled, double *A) {
       for (int j = i; j < ndf; ++j) {
         double acc = 0.0;
         for (int m = 0; m < ndf; ++m) acc = fma(B[i * ndf + m], B[j * ndf + m], acc);
-        if (i == j) acc += 0.1;
+        if (i == j) acc += shift;
         A[ic + nc3 * (i + (int64_t)ndf * j)] = acc;
         A[ic + nc3 * (j + (int64_t)ndf * i)] = acc;
       }
@@ -134,4 +138,17 @@ void orc_gen_spd(int ndf, int64_t nc3, uint64_t seed, int scaled, double *A) {
 
 void orc_gen_uniform(int64_t n, uint64_t seed, uint64_t stream, double *v) {
   for (int64_t i = 0; i < n; ++i) v[i] = 2.0 * orc_rng_uniform(seed, stream, (uint64_t)i) - 1.0;
+}
+
+/* general element entries sign(j) * (lo + (hi-lo) u), counter ic*n1*n2 + i*n2 + j */
+void orc_gen_op_uniform(int n1, int n2, int64_t nc3, uint64_t seed, uint64_t stream, double lo, double hi,
+                        const double *sign, double *A) {
+  for (int64_t ic = 0; ic < nc3; ++ic)
+    for (int i = 0; i < n1; ++i)
+      for (int j = 0; j < n2; ++j) {
+        double u = orc_rng_uniform(seed, stream, (uint64_t)ic * n1 * n2 + (uint64_t)i * n2 + j);
+        double v = lo + (hi - lo) * u;
+        if (sign) v = v * sign[j];
+        A[ic + nc3 * (i + (int64_t)n1 * j)] = v;
+      }
 }

@@ -1,0 +1,145 @@
+"""GPU parity of the mixed W2/W3 path: transpose and DG applies, the Helmholtz-like composite
+operator, the vertical-column block-Jacobi preconditioner and GMRES.
+
+Bars: FP64 operator applies within 1e-13 (norm-wise relative) of the oracle composition;
+column block-Jacobi within 2e-5 (FP32 factors) / 3e-2 (BF16 factors) of exact FP64 tridiagonal
+column solves; FP64 GMRES with no preconditioner matches the oracle's restart count and
+solution to 1e-9; mixed-precision GMRES reaches a true FP64 relative residual <= 1e-8."""
+import numpy as np
+import pytest
+import scipy.linalg as sla
+
+pytestmark = pytest.mark.gpu
+
+
+def _helmholtz(hip, orc, n, nl, precision=0, tau=0.5, seed=0):
+    w2 = hip.FunctionSpace.cubedsphere(n, nl, hip.W2, 0)
+    w3 = w2.sibling(hip.W3, 0)
+    M2 = hip.LmaOperator(w2).generate_spd(seed, shift=1.0)
+    D = hip.LmaOperator(w3, w2).generate_uniform(seed, 5, 0.5, 1.5, orc.D_SIGNS)
+    M3 = hip.LmaOperator(w3).generate_spd(seed + 1, shift=1.0)
+    if precision != 0:
+        M2, D, M3 = M2.convert(precision), D.convert(precision), M3.convert(precision)
+    m2d = M2.mm_diagonal(hip.FieldVector(w2, precision))
+    H = hip.LinearOperator.helmholtz(M3, D, m2d, tau)
+    return w2, w3, H, (M2, D, M3, m2d)
+
+
+def test_generated_mixed_operators_match_oracle(hip, orc):
+    w2, w3, H, (M2, D, M3, m2d) = _helmholtz(hip, orc, 4, 3)
+    o = orc.Helmholtz(4, 3)
+    np.testing.assert_array_equal(D.download(), o.D)
+    np.testing.assert_array_equal(M2.download(), o.M2)
+    np.testing.assert_array_equal(M3.download(), o.M3)
+    np.testing.assert_allclose(m2d.export_field(), o.m2d, rtol=1e-15)
+
+
+def test_transpose_and_dg_apply_parity(hip, orc):
+    w2, w3, H, (M2, D, M3, m2d) = _helmholtz(hip, orc, 12, 7)
+    o = orc.Helmholtz(12, 7)
+    rng = np.random.default_rng(0)
+    x3 = rng.standard_normal(w3.undf)
+    x2 = rng.standard_normal(w2.undf)
+    y2 = hip.FieldVector(w2)
+    D.apply_transpose(hip.FieldVector(w3).import_field(x3), y2)
+    ref = orc.transpose_apply(o.w2, o.w3, o.D, x3)
+    assert np.linalg.norm(y2.export_field() - ref) <= 1e-13 * np.linalg.norm(ref)
+    y3 = hip.FieldVector(w3)
+    D.apply_dg(hip.FieldVector(w2).import_field(x2), y3)
+    ref = orc.dg_apply(o.w3, o.w2, o.D, x2)
+    assert np.linalg.norm(y3.export_field() - ref) <= 1e-13 * np.linalg.norm(ref)
+    # W0 transpose of a non-symmetric operator = apply of its transpose
+    fs = hip.FunctionSpace.cubedsphere(4, 3)
+    m = orc.Mesh(4, 3)
+    A = rng.standard_normal((m.ncell_3d, 8, 8))
+    op = hip.LmaOperator(fs).upload(A)
+    x = rng.standard_normal(m.undf)
+    y = hip.FieldVector(fs)
+    op.apply_transpose(hip.FieldVector(fs).import_field(x), y)
+    ref = orc.lma_apply(m, A.transpose(0, 2, 1), x)
+    assert np.linalg.norm(y.export_field() - ref) <= 1e-13 * np.linalg.norm(ref)
+
+
+@pytest.mark.parametrize("n,nl", [(4, 3), (12, 10)])
+def test_helmholtz_apply_parity(hip, orc, n, nl):
+    w2, w3, H, _ = _helmholtz(hip, orc, n, nl)
+    o = orc.Helmholtz(n, nl)
+    x = np.random.default_rng(1).standard_normal(w3.undf)
+    y = hip.FieldVector(w3)
+    H.apply(hip.FieldVector(w3).import_field(x), y)
+    ref = o.apply(x)
+    assert np.linalg.norm(y.export_field() - ref) <= 1e-13 * np.linalg.norm(ref)
+
+
+def _column_solves(Hd, columns, r):
+    z = np.zeros_like(r)
+    for base, h in columns:
+        idx = np.arange(base, base + h)
+        T = Hd[np.ix_(idx, idx)]
+        ab = np.zeros((3, h))
+        ab[0, 1:] = np.diag(T, 1)
+        ab[1] = np.diag(T)
+        ab[2, :-1] = np.diag(T, -1)
+        z[idx] = sla.solve_banded((1, 1), ab, r[idx])
+    return z
+
+
+@pytest.mark.parametrize("storage,tol", [(1, 2e-5), (2, 3e-2), (3, 5e-3)])
+def test_column_jacobi_helmholtz(hip, orc, storage, tol):
+    w2, w3, H, _ = _helmholtz(hip, orc, 4, 5)
+    o = orc.Helmholtz(4, 5)
+    Hd = o.dense()
+    cols = [(o.w3.dofmap[c, 0] - 1, 5) for c in range(o.w3.ncells)]
+    P = hip.ColumnJacobiPreconditioner(H, storage)
+    r = np.random.default_rng(2).standard_normal(w3.undf)
+    z = hip.FieldVector(w3)
+    P.apply(hip.FieldVector(w3).import_field(r), z)
+    ref = _column_solves(Hd, cols, r)
+    assert np.linalg.norm(z.export_field() - ref) <= tol * np.linalg.norm(ref)
+
+
+def test_column_jacobi_w0_lma(hip, orc):
+    fs = hip.FunctionSpace.cubedsphere(4, 6)
+    m = orc.Mesh(4, 6)
+    op = hip.LmaOperator(fs).generate_spd(3)
+    A = orc.gen_spd(8, m.ncell_3d, 3)
+    S = orc.assembled_csr(m, A).toarray()
+    bases = sorted(set(m.dofmap[:, :4].ravel() - 1))
+    P = hip.ColumnJacobiPreconditioner(hip.LinearOperator.from_lma(op), hip.R32)
+    r = np.random.default_rng(3).standard_normal(m.undf)
+    z = hip.FieldVector(fs)
+    P.apply(hip.FieldVector(fs).import_field(r), z)
+    ref = _column_solves(S, [(b, 7) for b in bases], r)
+    assert np.linalg.norm(z.export_field() - ref) <= 2e-5 * np.linalg.norm(ref)
+
+
+def test_gmres_fp64_matches_oracle(hip, orc):
+    w2, w3, H, _ = _helmholtz(hip, orc, 4, 5)
+    o = orc.Helmholtz(4, 5)
+    b = np.random.default_rng(4).uniform(-1, 1, w3.undf)
+    x = hip.FieldVector(w3)
+    solver = hip.GMRES(H, hip.NullPreconditioner(), 10, 1e-10, 0.0, 40)
+    res = solver.apply(x, hip.FieldVector(w3).import_field(b))
+    xo, reso, hist = orc.gmres_callbacks(w3.undf, o.apply, lambda v: v, b, gcrk=10, r_tol=1e-10, a_tol=0.0,
+                                         max_iter=40)
+    assert res.status == 0 and reso.converged == 1
+    assert res.outer_iterations == reso.iterations
+    assert np.linalg.norm(x.export_field() - xo) <= 1e-9 * np.linalg.norm(xo)
+
+
+def test_mixed_gmres_reaches_fp64_accuracy(hip, orc):
+    w2, w3, H64, (M2, D, M3, m2d) = _helmholtz(hip, orc, 12, 10)
+    M2s, Ds, M3s = M2.convert(hip.R32), D.convert(hip.R32), M3.convert(hip.R32)
+    m2ds = M2s.mm_diagonal(hip.FieldVector(w2, hip.R32))
+    H32 = hip.LinearOperator.helmholtz(M3s, Ds, m2ds, 0.5)
+    o = orc.Helmholtz(12, 10)
+    P = hip.ColumnJacobiPreconditioner(H32, hip.BF16)
+    b = np.random.default_rng(5).uniform(-1, 1, w3.undf)
+    x = hip.FieldVector(w3)
+    solver = hip.MixedPrecisionGMRES(H64, H32, P, 30, 1e-8, 0.0, 50, inner_r_tol=1e-4)
+    res = solver.apply(x, hip.FieldVector(w3).import_field(b))
+    xg = x.export_field()
+    rel = np.linalg.norm(b - o.apply(xg)) / np.linalg.norm(b)
+    assert res.status == 0 and rel <= 1.01e-8, (res, rel)
+    xo, reso, _ = orc.gmres_callbacks(w3.undf, o.apply, lambda v: v, b, gcrk=30, r_tol=1e-12, max_iter=200)
+    assert np.linalg.norm(xg - xo) <= 1e-6 * np.linalg.norm(xo)

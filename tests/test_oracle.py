@@ -242,3 +242,18 @@ def test_gmres_solves_nonsymmetric_system(orc):
     # with gcrk >= n one restart cycle solves it
     x1, res1, _ = orc.gmres_callbacks(n, lambda v: M @ v, lambda v: v, b, gcrk=n, r_tol=1e-10, max_iter=1)
     assert res1.converged == 1 and res1.iterations == 1
+
+
+# ---- mixed W2/W3 reduction -------------------------------------------------------------------
+def test_helmholtz_composition_matches_assembled_sparse_operators(orc):
+    H = orc.Helmholtz(4, 3)
+    D = orc.assembled_csr(H.w2, H.D, to_mesh=H.w3)          # W3 x W2
+    M3 = orc.assembled_csr(H.w3, H.M3)
+    import scipy.sparse as sp
+    Hs = (M3 + H.tau * D @ sp.diags(1.0 / H.m2d) @ D.T).toarray()
+    Hd = H.dense()
+    np.testing.assert_allclose(Hd, Hs, rtol=1e-13, atol=1e-13 * np.abs(Hs).max())
+    assert np.allclose(Hd, Hd.T, rtol=1e-12, atol=1e-13)
+    assert np.linalg.eigvalsh(Hd).min() > 0.9  # M3 >= 1 on the diagonal, D-part PSD
+    # W2 divergence signs W,S,E,N,B,T = -,-,+,+,-,+
+    assert np.all(np.sign(H.D[:, 0, :]) == orc.D_SIGNS)
