@@ -36,6 +36,35 @@ def build(verbose=False):
     return LIB
 
 
+FORTRAN = PKG / "fortran"
+FORTRAN_SOURCES = [
+    "standalone/constants_mod.f90", "standalone/log_mod.f90", "standalone/vector_mod.f90",
+    "standalone/sci_linear_operator_mod.f90", "standalone/sci_preconditioner_mod.f90",
+    "standalone/sci_iterative_solver_mod.f90", "lfric_hip_capi_mod.f90", "lfric_hip_solver_api_mod.f90",
+]
+FORTRAN_TEST = FORTRAN / "_build" / "drop_in_test"
+
+
+def build_fortran(verbose=False):
+    """Compile the ISO_C_BINDING shim with the standalone solver-API modules and the drop-in
+    test program (flang), linked against liblfric_hip.so."""
+    fc = os.environ.get("FC", "/opt/rocm/llvm/bin/flang")
+    out = FORTRAN / "_build"
+    out.mkdir(exist_ok=True)
+    objs = []
+    for src in FORTRAN_SOURCES + ["tests/drop_in_test.f90"]:
+        obj = out / (Path(src).stem + ".o")
+        cmd = [fc, "-O2", "-fPIC", "-module-dir", str(out), "-I", str(out), "-c", str(FORTRAN / src), "-o", str(obj)]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        subprocess.run(cmd, check=True)
+        objs.append(str(obj))
+    subprocess.run([fc, "-o", str(FORTRAN_TEST), *objs, "-L", str(PKG), "-llfric_hip",
+                    "-Wl,-rpath," + str(PKG), "-Wl,-rpath,$ORIGIN/../.."], check=True)
+    return FORTRAN_TEST
+
+
 if __name__ == "__main__":
     build(verbose="-v" in sys.argv)
     print("built", LIB)
+    print("built", build_fortran(verbose="-v" in sys.argv))

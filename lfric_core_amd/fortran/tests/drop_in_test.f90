@@ -1,0 +1,63 @@
+!> Drop-in check of the HIP solver-API types: the stock conjugate_gradient_type and gmres_type
+!> (written against the abstract types) run unmodified on hip_field_vector_type /
+!> hip_lma_operator_type / hip_diagonal_preconditioner_type, and the device-resident
+!> hip_conjugate_gradient_type gives the same solve.  Prints one JSON line; stops with a
+!> non-zero code when the paths disagree.
+program drop_in_test
+  use iso_c_binding
+  use constants_mod,            only : r_def, i_def
+  use lfric_hip_capi_mod
+  use lfric_hip_solver_api_mod
+  use sci_iterative_solver_mod, only : conjugate_gradient_type, gmres_type
+  implicit none
+  type(hip_function_space_type) :: fs
+  type(c_ptr) :: op
+  type(hip_field_vector_type) :: b, diag, x_cg, x_hip, x_gm
+  type(hip_lma_operator_type) :: A
+  type(hip_diagonal_preconditioner_type) :: P
+  type(conjugate_gradient_type) :: cg
+  type(gmres_type) :: gm
+  type(hip_conjugate_gradient_type) :: hcg
+  real(r_def), allocatable :: a1(:), a2(:), a3(:)
+  real(r_def) :: d_cg, d_gm
+  integer :: n, nl
+
+  n = 12
+  nl = 20
+  call hip_check(lfric_init(0_c_int), "lfric_init")
+  call fs%init_cubedsphere(n, nl, int(LFRIC_W0, i_def), 0)
+  call hip_check(lfric_op_create(fs%h, fs%h, LFRIC_R64, op), "lfric_op_create")
+  call hip_check(lfric_op_generate_spd(op, 0_c_long_long, 0_c_int, 0.1_c_double), "generate")
+  A = hip_lma_operator(op)
+  call diag%initialise(fs)
+  call hip_check(lfric_mm_diagonal(op, diag%h), "mm_diagonal")
+  P = hip_diagonal_preconditioner(diag)
+  call b%initialise(fs)
+  call hip_check(lfric_vec_fill_uniform(b%h, 0_c_long_long, 3_c_long_long), "rhs")
+
+  call x_cg%initialise(fs)
+  call x_cg%set_scalar(0.0_r_def)
+  cg = conjugate_gradient_type(A, P, 1.0e-8_r_def, 1.0e-16_r_def, 500, .true., .true.)
+  call cg%apply(x_cg, b)
+
+  call x_hip%initialise(fs)
+  call x_hip%set_scalar(0.0_r_def)
+  hcg = hip_conjugate_gradient(A, P, 1.0e-8_r_def, 1.0e-16_r_def, 500, .true., .true.)
+  call hcg%apply(x_hip, b)
+
+  call x_gm%initialise(fs)
+  call x_gm%set_scalar(0.0_r_def)
+  gm = gmres_type(A, P, 10, 1.0e-10_r_def, 0.0_r_def, 50, .true., .true.)
+  call gm%apply(x_gm, b)
+
+  allocate(a1(fs%undf), a2(fs%undf), a3(fs%undf))
+  call x_cg%export_field(a1)
+  call x_hip%export_field(a2)
+  call x_gm%export_field(a3)
+  d_cg = norm2(a1 - a2) / norm2(a2)
+  d_gm = norm2(a3 - a2) / norm2(a2)
+  write(*, '(A,I0,A,I0,A,I0,A,ES12.5,A,ES12.5,A,I0,A)') '{"undf": ', fs%undf, ', "stock_cg_iterations": ', &
+      cg%iterations, ', "hip_cg_iterations": ', hcg%result%iterations, ', "stock_vs_hip_cg_rel_diff": ', &
+      d_cg, ', "stock_gmres_vs_hip_cg_rel_diff": ', d_gm, ', "gmres_cycles": ', gm%cycles, '}'
+  if (cg%iterations /= hcg%result%iterations .or. d_cg > 1.0e-10_r_def .or. d_gm > 1.0e-7_r_def) error stop 2
+end program drop_in_test

@@ -1,0 +1,28 @@
+"""The Fortran drop-in layer on the GPU: the stock conjugate_gradient_type / gmres_type run on
+the HIP vector, operator and preconditioner types, agree with the device-resident solve and
+with the oracle's iteration count (program lfric_core_amd/fortran/tests/drop_in_test.f90,
+built by __graft_entry__.build())."""
+import json
+import subprocess
+
+import pytest
+
+from conftest import ROOT
+
+pytestmark = pytest.mark.gpu
+
+
+def test_fortran_drop_in(orc):
+    exe = ROOT / "lfric_core_amd" / "fortran" / "_build" / "drop_in_test"
+    assert exe.exists(), "Fortran drop-in test not built"
+    out = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stdout + out.stderr
+    line = [l for l in out.stdout.splitlines() if l.startswith("{")][-1]
+    r = json.loads(line)
+    m = orc.Mesh(12, 20)
+    A = orc.gen_spd(8, m.ncell_3d, 0)
+    _, reso, _ = orc.cg(m, A, orc.gen_uniform(m.undf, 0, 3), diag=orc.mm_diagonal(m, A))
+    assert r["undf"] == m.undf
+    assert r["stock_cg_iterations"] == r["hip_cg_iterations"] == reso.iterations
+    assert r["stock_vs_hip_cg_rel_diff"] <= 1e-10
+    assert r["stock_gmres_vs_hip_cg_rel_diff"] <= 1e-7
