@@ -120,6 +120,23 @@ def main(ref):
             "d": d.tolist(), "x": x.tolist(), "answer": [-2.5, -0.75, 2.05, 10.725],
             "map1": [1, 2, 3, 4], "map2": [1, 2], "tol": 1e-14,
         },
+        # transpose_matrix_vector_mod_test.pf:33-67: y(2) = D(4x2)^T x(4), answer = matmul
+        "transpose_matrix_vector": {
+            "d": d.tolist(), "x": [0.25, 1.5, -0.35, 5.0],
+            "answer": (d.T @ np.array([0.25, 1.5, -0.35, 5.0])).tolist(),
+            "map1": [1, 2], "map2": [1, 2, 3, 4], "tol": 1e-14,
+        },
+        # dg_matrix_vector_mod_test.pf:38-68: y(4) = D(4x2) x(2), answer = matmul
+        "dg_matrix_vector": {
+            "d": d.tolist(), "x": x.tolist(), "answer": (d @ x).tolist(),
+            "map1": [1, 2, 3, 4], "map2": [1, 2], "tol": 1e-14,
+        },
+        # mm_diagonal_kernel_mod_test.pf:42-84: two cells sharing dof 3, literal answer
+        "mm_diagonal": {
+            "mm": [[4.0, 1.0, 1.0], [2.0, 6.0, 2.0], [1.0, 1.0, 4.0]],  # mm(cell, i, j), same per cell
+            "map": [[1, 2, 3], [3, 4, 5]], "undf": 5,
+            "answer": [4.0, 6.0, 8.0, 6.0, 4.0], "tol": 1e-12,
+        },
         "c3_summary": counts,
     }
     (HERE / "kat.json").write_text(json.dumps(kat, indent=1) + "\n")

@@ -197,6 +197,56 @@ def test_oracle_apply_bitwise_equals_compiled_reference_kernel(orc):
         np.testing.assert_allclose(d, orc.mm_diagonal(m, A), rtol=1e-15)
 
 
+def test_transpose_and_dg_bitwise_equal_compiled_reference_kernels(orc):
+    R = _ref_lib()
+    P = lambda a: ctypes.c_void_p(a.ctypes.data)
+    I = ctypes.c_int
+    H = orc.Helmholtz(4, 3)
+    w2, w3 = H.w2, H.w3
+    x3 = orc.gen_uniform(w3.undf, 0, 3)
+    y2 = orc.transpose_apply(w2, w3, H.D, x3)                # D^T: W3 -> W2, D is (c3, 1, 6)
+    yr = np.zeros(w2.undf)
+    R.ref_transpose_apply_r64(I(w2.ncells), I(w2.nlayers), I(4), P(w2.ncells_per_colour),
+                              P(w2.cells_in_colour), I(w2.cells_in_colour.shape[1]), I(w2.ndf),
+                              I(w2.undf), P(w2.dofmap), I(w3.ndf), I(w3.undf), P(w3.dofmap),
+                              P(orc.flat_matrix(H.D)), P(x3), P(yr))
+    np.testing.assert_array_equal(y2, yr)
+    x2 = orc.gen_uniform(w2.undf, 1, 3)
+    y3 = orc.dg_apply(w3, w2, H.D, x2)                       # D: W2 -> W3
+    yr3 = np.zeros(w3.undf)
+    R.ref_dg_apply_r64(I(w3.ncells), I(w3.nlayers), I(w3.ndf), I(w3.undf), P(w3.dofmap), I(w2.ndf),
+                       I(w2.undf), P(w2.dofmap), P(orc.flat_matrix(H.D)), P(x2), P(yr3))
+    np.testing.assert_array_equal(y3, yr3)
+
+
+def test_transpose_known_answer(orc):
+    k = json.loads((GOLDEN / "kat.json").read_text())["transpose_matrix_vector"]
+    to, fr = _OneCell(k["map1"], 2), _OneCell(k["map2"], 4)
+    y = orc.transpose_apply(to, fr, np.array(k["d"])[None, :, :], np.array(k["x"]))
+    np.testing.assert_allclose(y, k["answer"], rtol=0, atol=k["tol"])
+    # a plausible mistake, applying D instead of D^T, does not even have the right length
+    assert len(k["answer"]) == 2 and not np.allclose(y, np.array(k["d"])[:2, :] @ np.array(k["x"])[:2])
+
+
+def test_dg_known_answer(orc):
+    k = json.loads((GOLDEN / "kat.json").read_text())["dg_matrix_vector"]
+    to, fr = _OneCell(k["map1"], 4), _OneCell(k["map2"], 2)
+    y = orc.dg_apply(to, fr, np.array(k["d"])[None, :, :], np.array(k["x"]))
+    np.testing.assert_allclose(y, k["answer"], rtol=0, atol=k["tol"])
+
+
+def test_mm_diagonal_known_answer(orc):
+    k = json.loads((GOLDEN / "kat.json").read_text())["mm_diagonal"]
+
+    class TwoCells:
+        ncells, nlayers, ndf, undf = 2, 1, 3, k["undf"]
+        dofmap = np.array(k["map"], np.int32)
+
+    A = np.array([k["mm"], k["mm"]])
+    d = orc.mm_diagonal(TwoCells, A)
+    np.testing.assert_allclose(d, k["answer"], rtol=0, atol=k["tol"])
+
+
 # ---- solvers ----------------------------------------------------------------------------------
 def test_cg_solves_small_spd_system(orc):
     m = orc.Mesh(4, 3)
