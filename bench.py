@@ -37,12 +37,13 @@ def apply_bytes(fs, ndf, s_a, s_x):
     return fs.ncell_3d * ndf * ndf * s_a + 2 * fs.undf * s_x + 4 * fs.ncells * ndf
 
 
-def time_applies(L, op, x, y, reps):
-    op.apply(x, y)
+def time_applies(L, op, x, y, reps, method="apply"):
+    fn = getattr(op, method)
+    fn(x, y)
     L.synchronize()
     L.timer_start()
     for _ in range(reps):
-        op.apply(x, y)
+        fn(x, y)
     return L.timer_stop() / reps
 
 
@@ -123,15 +124,18 @@ def main():
     if args.apply_only:
         xs32, ys32 = L.FieldVector(fs, L.R32).fill_uniform(1, 3), L.FieldVector(fs, L.R32)
         xs64, ys64 = L.FieldVector(fs).fill_uniform(1, 3), L.FieldVector(fs)
-        t32 = time_applies(L, op32, xs32, ys32, args.apply_reps)
-        t64 = time_applies(L, op64, xs64, ys64, args.apply_reps)
+        out = {"apply_only": True}
+        for method in ("apply", "apply_gather"):
+            t32 = time_applies(L, op32, xs32, ys32, args.apply_reps, method)
+            t64 = time_applies(L, op64, xs64, ys64, args.apply_reps, method)
+            out[method] = {"fp32_ms": t32, "fp64_ms": t64,
+                           "fp32_gbs": apply_bytes(fs, 8, 4, 4) / (t32 * 1e-3) / 1e9,
+                           "fp64_gbs": apply_bytes(fs, 8, 8, 8) / (t64 * 1e-3) / 1e9}
         for _ in range(args.apply_reps):  # plain copies of a known size (undf * 8 bytes each way)
             ys64.copy(xs64)
         L.synchronize()
-        print(json.dumps({"apply_only": True, "fp32_ms": t32, "fp64_ms": t64,
-                          "fp32_gbs": apply_bytes(fs, 8, 4, 4) / (t32 * 1e-3) / 1e9,
-                          "fp64_gbs": apply_bytes(fs, 8, 8, 8) / (t64 * 1e-3) / 1e9,
-                          "copy_bytes_each_way": undf * 8}))
+        out["copy_bytes_each_way"] = undf * 8
+        print(json.dumps(out))
         return
 
     mixed = L.MixedPrecisionCG(op64, op32, prec32, 1e-8, 0.0, 20000, inner_r_tol=1e-4, check_every=16)

@@ -132,8 +132,9 @@ void lfric_op_destroy(lfric_op *op);
  * components/science/source/kernel/algebra/matrix_vector_kernel_mod.F90:68-140 as called by
  * components/science/source/algorithm/solver/sci_mass_matrix_operator_alg_mod.x90:98-100. */
 int lfric_lma_apply(const lfric_op *op, const lfric_vec *x, lfric_vec *y);
-/* Same product through the colour-ordered gather kernel (one launch, same summation order
- * per dof, bit-identical to lfric_lma_apply). */
+/* Same product in gather form: one launch; every output dof is written once by the thread of
+ * its column, which adds the contributions of the column's cells in colour order, top-partner
+ * row of layer k-1 before bottom row of layer k (the order of the reference's coloured loop). */
 int lfric_lma_apply_gather(const lfric_op *op, const lfric_vec *x, lfric_vec *y);
 /* transpose_matrix_vector_kernel (components/science/source/kernel/algebra/
  * transpose_matrix_vector_kernel_mod.F90): y = A^T x, A maps `from` -> `to`, y on `from`. */

@@ -51,6 +51,9 @@ FunctionSpace::~FunctionSpace() {
   dfree(d_first, flag_bytes);
   dfree(d_last, flag_bytes);
   dfree(d_ebase, ebase_bytes);
+  dfree(d_gcol, gcol_bytes);
+  dfree(d_goff, goff_bytes);
+  dfree(d_gent, gent_bytes);
 }
 
 // Scratch vectors of the solvers, kept between solves (grown on demand).
@@ -559,7 +562,9 @@ int lfric_op_create(const lfric_fs *to, const lfric_fs *from, int precision, lfr
   op->to = to->fs;
   op->from = from->fs;
   op->precision = precision;
-  op->gs = static_cast<size_t>(64) * t.ndf * f.ndf;
+  // 64 elements per group, each element's entries padded to whole 16-byte chunks
+  const size_t V = 16 / esize(precision);
+  op->gs = 64 * ((static_cast<size_t>(t.ndf) * f.ndf + V - 1) / V) * V;
   op->ngroups = t.nelem_padded / 64;
   op->row_top = t.top_mask;
   op->bytes = esize(precision) * op->gs * static_cast<size_t>(op->ngroups);

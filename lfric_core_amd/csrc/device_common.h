@@ -63,8 +63,25 @@ __device__ inline double to_d(T v) {
   return static_cast<double>(v);
 }
 
+// Entries of an operator element per 16-byte chunk (4 FP32, 2 FP64).
+template <typename T>
+constexpr int chunk_entries() {
+  return 16 / static_cast<int>(sizeof(T));
+}
+// Group stride of an operator whose elements hold n1*n2 entries: 64 elements per group,
+// entries padded to whole chunks.
+template <typename T>
+constexpr size_t group_stride(int n1, int n2) {
+  return static_cast<size_t>(64) * ((n1 * n2 + chunk_entries<T>() - 1) / chunk_entries<T>()) * chunk_entries<T>();
+}
+// Address of entry p of element e: group e/64, then chunk p/V, then element e%64, then p%V,
+// so that one lane's V consecutive entries form one 16-byte load and a wavefront's 64 loads of
+// one chunk form one contiguous 1 KiB block.
+template <typename T>
 __device__ inline size_t aidx(size_t gs, long long e, int p) {
-  return static_cast<size_t>(e >> 6) * gs + static_cast<size_t>(p) * 64 + static_cast<size_t>(e & 63);
+  constexpr int V = chunk_entries<T>();
+  return static_cast<size_t>(e >> 6) * gs + (static_cast<size_t>(p / V) * 64 + static_cast<size_t>(e & 63)) * V +
+         static_cast<size_t>(p % V);
 }
 
 }  // namespace

@@ -82,6 +82,14 @@ struct FunctionSpace {
   uint32_t *d_first = nullptr;             // [pos] bit df: first touch of df's column
   uint32_t *d_last = nullptr;              // [pos] bit df: last touch of df's column
   size_t map_bytes = 0, flag_bytes = 0, ebase_bytes = 0;
+  // gather form of an apply into this space (built on first use): one list per dof column
+  // (sorted by base dof) of the (cell position, bottom-role df) pairs that write it, in colour
+  // order
+  int gather_ncols = 0;
+  int2 *d_gcol = nullptr;                  // [col] base dof (0-based), 1 when the column is nl+1 high
+  int *d_goff = nullptr;                   // [col+1] offsets into d_gent
+  int *d_gent = nullptr;                   // pos * 32 + df
+  size_t gcol_bytes = 0, goff_bytes = 0, gent_bytes = 0;
   ~FunctionSpace();
 };
 
@@ -103,7 +111,7 @@ struct lfric_vec {
 struct lfric_op {
   std::shared_ptr<lfric::FunctionSpace> to, from;
   int precision = LFRIC_R64;
-  size_t gs = 0;               // values per 64-element group: 64 * ndf_to * ndf_from
+  size_t gs = 0;               // values per 64-element group: 64 * (ndf_to*ndf_from padded to 16-byte chunks)
   long long ngroups = 0;       // 64-element groups
   uint32_t row_top = 0;        // rows stored one level up (top dofs of the to space)
   void *d = nullptr;           // level-aligned AoSoA layout (see kernels.hip)

@@ -6,10 +6,11 @@
 // (pos); column pos owns the elements ebase[pos] + m, m = 0..nlayers (one slot per vertex
 // level), colours start on 64-element boundaries.  Entry (df1,df2) of 3D cell (pos, k) lives
 // at element e = ebase[pos] + k + top(df1), where top(df1) = 1 when df1 is the top dof of a
-// column pair (so both rows that feed output level m sit at the same e), and at address
-// (e/64)*64*nplanes + (df1*ndf2 + df2)*64 + e%64.  One wavefront of (cell, level) threads thus
-// reads every plane of its 64 elements as one 256-byte line, and all planes of a wavefront
-// form one contiguous block.
+// column pair (so both rows that feed output level m sit at the same e).  With p = df1*ndf2 +
+// df2 and V = 16 bytes / sizeof(entry), the entry sits at (e/64)*gs + ((p/V)*64 + e%64)*V +
+// p%V, gs = 64 * (ndf1*ndf2 rounded up to V).  One wavefront of (cell, level) threads thus
+// reads each 16-byte chunk of its 64 elements as one contiguous 1 KiB block, and all chunks of
+// a wavefront form one contiguous block.
 //
 // LMA apply thread decomposition.  The reference kernel adds A_k x_k column by column into
 // lhs(map1(df)+k) (components/science/source/kernel/algebra/matrix_vector_kernel_mod.F90:
@@ -22,8 +23,10 @@
 // colours add to it.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <type_traits>
+#include <vector>
 
 #include "device_common.h"
 #include "kernels.h"
@@ -80,8 +83,8 @@ __global__ __launch_bounds__(kBlock) void k_gen_spd(T *__restrict__ A, size_t gs
       double acc = 0.0;
       for (int m = 0; m < NDF; ++m) acc = fma(B[w][i * NDF + m], B[w][j * NDF + m], acc);
       if (i == j) acc += shift;
-      A[aidx(gs, el + ((row_top >> i) & 1u), i * NDF + j)] = static_cast<T>(acc);
-      A[aidx(gs, el + ((row_top >> j) & 1u), j * NDF + i)] = static_cast<T>(acc);
+      A[aidx<T>(gs, el + ((row_top >> i) & 1u), i * NDF + j)] = static_cast<T>(acc);
+      A[aidx<T>(gs, el + ((row_top >> j) & 1u), j * NDF + i)] = static_cast<T>(acc);
     }
   }
 }
@@ -103,7 +106,7 @@ __global__ void k_gen_uniform(T *__restrict__ A, size_t gs, const int *__restric
       const double u = rng::uniform(seed, stream, static_cast<uint64_t>(ic) * n1 * n2 + i * n2 + j);
       double v = lo + (hi - lo) * u;
       if (sign) v = v * sign[j];
-      A[aidx(gs, el + ((row_top >> i) & 1u), i * n2 + j)] = static_cast<T>(v);
+      A[aidx<T>(gs, el + ((row_top >> i) & 1u), i * n2 + j)] = static_cast<T>(v);
     }
 }
 
@@ -128,7 +131,7 @@ __global__ void k_from_reference(T *__restrict__ A, size_t gs, const int *__rest
   const long long el = ebase[pos] + k;
   for (int i = 0; i < n1; ++i)
     for (int j = 0; j < n2; ++j)
-      A[aidx(gs, el + ((row_top >> i) & 1u), i * n2 + j)] =
+      A[aidx<T>(gs, el + ((row_top >> i) & 1u), i * n2 + j)] =
           static_cast<T>(R[ic + nelem * (i + static_cast<long long>(n1) * j)]);
 }
 
@@ -144,7 +147,19 @@ __global__ void k_to_reference(const T *__restrict__ A, size_t gs, const int *__
   const long long el = ebase[pos] + k;
   for (int i = 0; i < n1; ++i)
     for (int j = 0; j < n2; ++j)
-      R[ic + nelem * (i + static_cast<long long>(n1) * j)] = to_d(A[aidx(gs, el + ((row_top >> i) & 1u), i * n2 + j)]);
+      R[ic + nelem * (i + static_cast<long long>(n1) * j)] = to_d(A[aidx<T>(gs, el + ((row_top >> i) & 1u), i * n2 + j)]);
+}
+
+// operator precision change; the chunk width differs between precisions, so entries move
+// between layouts (padding entries of dst stay as they are: zero from creation)
+template <typename TD, typename TS>
+__global__ void k_op_convert(TD *__restrict__ d, size_t gsd, const TS *__restrict__ s, size_t gss, long long nelem,
+                             int np) {
+  const long long t = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t >= nelem * np) return;
+  const int p = static_cast<int>(t / nelem);
+  const long long e = t - static_cast<long long>(p) * nelem;
+  d[aidx<TD>(gsd, e, p)] = static_cast<TD>(s[aidx<TS>(gss, e, p)]);
 }
 
 template <typename TD, typename TS>
@@ -215,20 +230,54 @@ __global__ __launch_bounds__(kBlock) void k_lma_colour(const T *__restrict__ A, 
     xp[j] = has_p ? x[b + m - 1] : T(0);
   }
   const uint32_t ft = first ? first[pos] : 0xffffffffu;
-  if constexpr (R::kStatic && N1 * N2 <= 64) {
-    // every row of this thread sits at element e: bottom rows of layer m, top rows of layer m-1
-    T a[N1][N2];
-#pragma unroll
-    for (int i = 0; i < N1; ++i)
-#pragma unroll
-      for (int j = 0; j < N2; ++j) a[i][j] = (R::top(i) ? has_p : has_m) ? A[aidx(gs, e, i * N2 + j)] : T(0);
+  if constexpr (R::kStatic) {
+    // every row of this thread sits at element e: bottom rows of layer m, top rows of layer m-1.
+    // The element is read as 16-byte chunks in entry order (one dwordx4 per lane, 1 KiB per
+    // wavefront); row i accumulates over df2 in order in c[i].
+    constexpr int V = chunk_entries<T>();
+    constexpr int NP = N1 * N2, NC = (NP + V - 1) / V;
+    typedef T VecT __attribute__((ext_vector_type(V)));
+    const T *Ae = A + static_cast<size_t>(e >> 6) * gs + static_cast<size_t>(e & 63) * V;
     T c[N1];
 #pragma unroll
-    for (int i = 0; i < N1; ++i) {
-      T s = T(0);
+    for (int i = 0; i < N1; ++i) c[i] = T(0);
+    if constexpr (NP <= 64) {
+      // small elements: every chunk in flight at once
 #pragma unroll
-      for (int j = 0; j < N2; ++j) s = fma(a[i][j], R::top(i) ? xp[j] : xm[j], s);
-      c[i] = s;
+      for (int ch = 0; ch < NC; ++ch) {
+        bool need = false;
+#pragma unroll
+        for (int q = 0; q < V; ++q) {
+          const int p = ch * V + q;
+          if (p < NP) need = need || (R::top(p / N2) ? has_p : has_m);
+        }
+        VecT v = {};
+        if (need) v = *reinterpret_cast<const VecT *>(Ae + static_cast<size_t>(ch) * 64 * V);
+#pragma unroll
+        for (int q = 0; q < V; ++q) {
+          const int p = ch * V + q;
+          if (p < NP) c[p / N2] = fma(v[q], R::top(p / N2) ? xp[p % N2] : xm[p % N2], c[p / N2]);
+        }
+      }
+    } else {
+      // large elements (order 1): one row at a time, loading the chunks that cover it; a chunk
+      // shared by two rows is read twice, the second time from cache
+#pragma unroll
+      for (int i = 0; i < N1; ++i) {
+        if (!(R::top(i) ? has_p : has_m)) continue;
+        const int c0 = (i * N2) / V, c1 = (i * N2 + N2 - 1) / V;
+        T s = T(0);
+#pragma unroll
+        for (int ch = c0; ch <= c1; ++ch) {
+          const VecT v = *reinterpret_cast<const VecT *>(Ae + static_cast<size_t>(ch) * 64 * V);
+#pragma unroll
+          for (int q = 0; q < V; ++q) {
+            const int p = ch * V + q;
+            if (p >= i * N2 && p < i * N2 + N2) s = fma(v[q], R::top(i) ? xp[p - i * N2] : xm[p - i * N2], s);
+          }
+        }
+        c[i] = s;
+      }
     }
 #pragma unroll
     for (int i = 0; i < N1; ++i) {
@@ -251,11 +300,11 @@ __global__ __launch_bounds__(kBlock) void k_lma_colour(const T *__restrict__ A, 
       T acc = T(0);
       if (has_m) {
 #pragma unroll
-        for (int j = 0; j < N2; ++j) acc = fma(A[aidx(gs, e, i * N2 + j)], xm[j], acc);
+        for (int j = 0; j < N2; ++j) acc = fma(A[aidx<T>(gs, e, i * N2 + j)], xm[j], acc);
       }
       if (pt >= 0 && has_p) {
 #pragma unroll
-        for (int j = 0; j < N2; ++j) acc = fma(A[aidx(gs, e, pt * N2 + j)], xp[j], acc);
+        for (int j = 0; j < N2; ++j) acc = fma(A[aidx<T>(gs, e, pt * N2 + j)], xp[j], acc);
       }
       const size_t d = static_cast<size_t>(m1[i]) + m;
       if ((ft >> i) & 1u)
@@ -264,6 +313,63 @@ __global__ __launch_bounds__(kBlock) void k_lma_colour(const T *__restrict__ A, 
         y[d] += acc;
     }
   }
+}
+
+// Row r of element e times x at levels b[j] + lev, accumulated onto acc in df2 order.
+template <typename T, int N2>
+__device__ inline T row_dot(const T *__restrict__ A, size_t gs, long long e, int r, const T *__restrict__ x,
+                            const int (&b)[N2], int lev, T acc) {
+  constexpr int V = chunk_entries<T>();
+  if constexpr (N2 % V == 0) {
+    typedef T VecT __attribute__((ext_vector_type(V)));
+    const T *Ae = A + static_cast<size_t>(e >> 6) * gs + static_cast<size_t>(e & 63) * V;
+    const int ch0 = r * (N2 / V);
+#pragma unroll
+    for (int c = 0; c < N2 / V; ++c) {
+      const VecT v = *reinterpret_cast<const VecT *>(Ae + static_cast<size_t>(ch0 + c) * 64 * V);
+#pragma unroll
+      for (int q = 0; q < V; ++q) acc = fma(v[q], x[b[c * V + q] + lev], acc);
+    }
+  } else {
+#pragma unroll
+    for (int j = 0; j < N2; ++j) acc = fma(A[aidx<T>(gs, e, r * N2 + j)], x[b[j] + lev], acc);
+  }
+  return acc;
+}
+
+// LMA apply in gather form, one launch for all colours.  Thread (column, level m) of the
+// output space sums, over the cells of its column in colour order, the top-partner row of
+// layer m-1 and then the bottom row of layer m: the order in which the reference's coloured
+// loop adds into lhs(map1(df)+k) (matrix_vector_kernel_mod.F90:130-138, k outer).  Every
+// output dof is written once, with no read-modify-write and no zero fill.
+template <typename T, int N2, class R>
+__global__ __launch_bounds__(kBlock) void k_lma_gather(const T *__restrict__ A, size_t gs, const int *__restrict__ ebase,
+                                                       const T *__restrict__ x, T *__restrict__ y,
+                                                       const int *__restrict__ map2, const int2 *__restrict__ cols,
+                                                       const int *__restrict__ off, const int *__restrict__ ent,
+                                                       int ncols, int nl, R roles) {
+  const int H = nl + 1;
+  const long long t = static_cast<long long>(blockIdx.x) * kBlock + threadIdx.x;
+  if (t >= static_cast<long long>(ncols) * H) return;
+  const int col = static_cast<int>(t / H);
+  const int m = static_cast<int>(t - static_cast<long long>(col) * H);
+  const int2 ci = cols[col];
+  if (m == nl && !ci.y) return;
+  const bool has_m = m < nl, has_p = m > 0;
+  T acc = T(0);
+  const int q1 = off[col + 1];
+  for (int q = off[col]; q < q1; ++q) {
+    const int en = ent[q];
+    const int pos = en >> 5, i = en & 31;
+    const int pt = roles.partner(i);
+    const long long e = static_cast<long long>(ebase[pos]) + m;
+    int b[N2];
+#pragma unroll
+    for (int j = 0; j < N2; ++j) b[j] = map2[static_cast<size_t>(pos) * N2 + j];
+    if (pt >= 0 && has_p) acc = row_dot<T, N2>(A, gs, e, pt, x, b, m - 1, acc);
+    if (has_m) acc = row_dot<T, N2>(A, gs, e, i, x, b, m, acc);
+  }
+  y[static_cast<size_t>(ci.x) + m] = acc;
 }
 
 // y = A^T x: thread (cell, level m) of the `from` space; entry (to_df i, from_df j) of layer k
@@ -300,11 +406,11 @@ __global__ __launch_bounds__(kBlock) void k_transpose_colour(const T *__restrict
     T acc = T(0);
     if (has_m) {
 #pragma unroll
-      for (int i = 0; i < NT; ++i) acc = fma(A[aidx(gs, e + ((row_top >> i) & 1u), i * NF + j)], xm[i], acc);
+      for (int i = 0; i < NT; ++i) acc = fma(A[aidx<T>(gs, e + ((row_top >> i) & 1u), i * NF + j)], xm[i], acc);
     }
     if (pt >= 0 && has_p) {
 #pragma unroll
-      for (int i = 0; i < NT; ++i) acc = fma(A[aidx(gs, e - 1 + ((row_top >> i) & 1u), i * NF + pt)], xp[i], acc);
+      for (int i = 0; i < NT; ++i) acc = fma(A[aidx<T>(gs, e - 1 + ((row_top >> i) & 1u), i * NF + pt)], xp[i], acc);
     }
     const size_t d = static_cast<size_t>(mf[j]) + m;
     if ((ft >> j) & 1u)
@@ -335,8 +441,8 @@ __global__ __launch_bounds__(kBlock) void k_diag_colour(const T *__restrict__ A,
     const int pt = roles.partner[i];
     if (m == nl && pt < 0) continue;
     T acc = T(0);
-    if (m < nl) acc += A[aidx(gs, e, i * N + i)];
-    if (pt >= 0 && m > 0) acc += A[aidx(gs, e, pt * N + pt)];
+    if (m < nl) acc += A[aidx<T>(gs, e, i * N + i)];
+    if (pt >= 0 && m > 0) acc += A[aidx<T>(gs, e, pt * N + pt)];
     const size_t o = static_cast<size_t>(mp[i]) + m;
     if ((ft >> i) & 1u)
       d[o] = acc;
@@ -725,9 +831,25 @@ int launch_convert(int pd, void *dst, int ps, const void *src, long long n, doub
 }
 
 int launch_op_convert(const lfric_op *src, lfric_op *dst) {
-  LF_CHECK(src->gs == dst->gs && src->ngroups == dst->ngroups, LFRIC_E_ARG, "operator shapes differ");
-  const long long n = static_cast<long long>(src->gs) * src->ngroups;
-  return launch_convert(dst->precision, dst->d, src->precision, src->d, n, 1.0);
+  LF_CHECK(src->ngroups == dst->ngroups, LFRIC_E_ARG, "operator shapes differ");
+  const long long nelem = static_cast<long long>(src->ngroups) * 64;
+  const int np = src->to->ndf * src->from->ndf;
+  const int grid = grid_for(nelem * np);
+  const bool d64 = dst->precision == LFRIC_R64, s64 = src->precision == LFRIC_R64;
+  if (d64 && s64)
+    LAUNCH((k_op_convert<double, double>), grid, static_cast<double *>(dst->d), dst->gs,
+           static_cast<const double *>(src->d), src->gs, nelem, np);
+  else if (!d64 && s64)
+    LAUNCH((k_op_convert<float, double>), grid, static_cast<float *>(dst->d), dst->gs,
+           static_cast<const double *>(src->d), src->gs, nelem, np);
+  else if (d64 && !s64)
+    LAUNCH((k_op_convert<double, float>), grid, static_cast<double *>(dst->d), dst->gs,
+           static_cast<const float *>(src->d), src->gs, nelem, np);
+  else
+    LAUNCH((k_op_convert<float, float>), grid, static_cast<float *>(dst->d), dst->gs,
+           static_cast<const float *>(src->d), src->gs, nelem, np);
+  CHECK_LAUNCH();
+  return LFRIC_OK;
 }
 
 // ---- operator apply dispatch ---------------------------------------------------------------
@@ -819,8 +941,90 @@ int launch_lma_apply(const lfric_op *op, const void *x, void *y, const int *d_sk
   return fail(LFRIC_E_UNSUPPORTED, "lma_apply: operator precision");
 }
 
+// column lists of the gather form (host build, once per space)
+static int build_gather(FunctionSpace &f) {
+  if (f.d_gcol) return LFRIC_OK;
+  const int nc = f.ncells, ndf = f.ndf;
+  LF_CHECK(static_cast<long long>(nc) * 32 < (1LL << 31), LFRIC_E_UNSUPPORTED, "mesh too large for the gather form");
+  struct Item {
+    int base, pos, df;
+  };
+  std::vector<Item> items;
+  items.reserve(static_cast<size_t>(nc) * ndf);
+  for (int p = 0; p < nc; ++p) {
+    const int cell = f.cell_of_pos[p];
+    for (int d = 0; d < ndf; ++d)
+      if (!((f.top_mask >> d) & 1u)) items.push_back({f.dofmap[static_cast<size_t>(cell) * ndf + d] - 1, p, d});
+  }
+  std::stable_sort(items.begin(), items.end(), [](const Item &a, const Item &b) { return a.base < b.base; });
+  std::vector<int2> cols;
+  std::vector<int> off, ent;
+  ent.reserve(items.size());
+  for (size_t q = 0; q < items.size(); ++q) {
+    if (q == 0 || items[q].base != items[q - 1].base) {
+      cols.push_back(make_int2(items[q].base, f.partner[items[q].df] >= 0 ? 1 : 0));
+      off.push_back(static_cast<int>(ent.size()));
+    } else {
+      LF_CHECK((f.partner[items[q].df] >= 0) == (cols.back().y == 1), LFRIC_E_UNSUPPORTED,
+               "cells disagree on the height of a dof column");
+    }
+    ent.push_back(items[q].pos * 32 + items[q].df);
+  }
+  off.push_back(static_cast<int>(ent.size()));
+  f.gather_ncols = static_cast<int>(cols.size());
+  f.gcol_bytes = sizeof(int2) * cols.size();
+  f.goff_bytes = sizeof(int) * off.size();
+  f.gent_bytes = sizeof(int) * ent.size();
+  if (int rc = dmalloc(reinterpret_cast<void **>(&f.d_gcol), f.gcol_bytes)) return rc;
+  if (int rc = dmalloc(reinterpret_cast<void **>(&f.d_goff), f.goff_bytes)) return rc;
+  if (int rc = dmalloc(reinterpret_cast<void **>(&f.d_gent), f.gent_bytes)) return rc;
+  LF_HIP(hipMemcpy(f.d_gcol, cols.data(), f.gcol_bytes, hipMemcpyHostToDevice));
+  LF_HIP(hipMemcpy(f.d_goff, off.data(), f.goff_bytes, hipMemcpyHostToDevice));
+  LF_HIP(hipMemcpy(f.d_gent, ent.data(), f.gent_bytes, hipMemcpyHostToDevice));
+  return LFRIC_OK;
+}
+
+template <typename T, int N2, class R>
+static void launch_gather_r(const lfric_op *op, const T *x, T *y, R roles) {
+  const FunctionSpace &to = *op->to, &from = *op->from;
+  const int grid = grid_for(static_cast<long long>(to.gather_ncols) * (to.nlayers + 1));
+  LAUNCH((k_lma_gather<T, N2, R>), grid, static_cast<const T *>(op->d), op->gs, to.d_ebase, x, y, from.d_map,
+         to.d_gcol, to.d_goff, to.d_gent, to.gather_ncols, to.nlayers, roles);
+}
+
+template <typename T, int N2>
+static void launch_gather_n(RoleKind rk, const lfric_op *op, const T *x, T *y) {
+  switch (rk) {
+    case RoleKind::W0K0: launch_gather_r<T, N2>(op, x, y, RolesW0K0{}); break;
+    case RoleKind::W0K1: launch_gather_r<T, N2>(op, x, y, RolesW0K1{}); break;
+    case RoleKind::W2K0: launch_gather_r<T, N2>(op, x, y, RolesW2K0{}); break;
+    case RoleKind::None: launch_gather_r<T, N2>(op, x, y, RolesNone{}); break;
+    default: launch_gather_r<T, N2>(op, x, y, RolesRuntime{roles_of(*op->to)});
+  }
+}
+
+template <typename T>
+static int gather_t(const lfric_op *op, const T *x, T *y) {
+  if (int rc = build_gather(*op->to)) return rc;
+  const RoleKind rk = role_kind(*op->to);
+  switch (op->from->ndf) {
+    case 1: launch_gather_n<T, 1>(rk, op, x, y); break;
+    case 2: launch_gather_n<T, 2>(rk, op, x, y); break;
+    case 6: launch_gather_n<T, 6>(rk, op, x, y); break;
+    case 8: launch_gather_n<T, 8>(rk, op, x, y); break;
+    case 27: launch_gather_n<T, 27>(rk, op, x, y); break;
+    default: return fail(LFRIC_E_UNSUPPORTED, "lma_apply_gather: unsupported ndf_from");
+  }
+  CHECK_LAUNCH();
+  return LFRIC_OK;
+}
+
 int launch_lma_apply_gather(const lfric_op *op, const void *x, void *y) {
-  return launch_lma_apply(op, x, y, nullptr);
+  if (op->precision == LFRIC_R64)
+    return gather_t<double>(op, static_cast<const double *>(x), static_cast<double *>(y));
+  if (op->precision == LFRIC_R32)
+    return gather_t<float>(op, static_cast<const float *>(x), static_cast<float *>(y));
+  return fail(LFRIC_E_UNSUPPORTED, "lma_apply_gather: operator precision");
 }
 
 template <typename T>

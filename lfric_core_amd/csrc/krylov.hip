@@ -76,9 +76,9 @@ __global__ __launch_bounds__(kBlock) void k_coltri_lma(const T *__restrict__ A, 
     const int pt = roles.partner[i];
     if (m == nl && pt < 0) continue;
     double dg = 0.0, up = 0.0;
-    if (m < nl) dg += to_d(A[aidx(gs, e, i * N + i)]);
-    if (pt >= 0 && m > 0) dg += to_d(A[aidx(gs, e, pt * N + pt)]);
-    if (pt >= 0 && m < nl) up = to_d(A[aidx(gs, e, i * N + pt)]);  // bottom(m) - top(m+1) of layer m
+    if (m < nl) dg += to_d(A[aidx<T>(gs, e, i * N + i)]);
+    if (pt >= 0 && m > 0) dg += to_d(A[aidx<T>(gs, e, pt * N + pt)]);
+    if (pt >= 0 && m < nl) up = to_d(A[aidx<T>(gs, e, i * N + pt)]);  // bottom(m) - top(m+1) of layer m
     const size_t o = static_cast<size_t>(mp[i]) + m;
     if ((ft >> i) & 1u) {
       diag[o] = dg;
@@ -102,15 +102,15 @@ __global__ void k_coltri_helmholtz(const T *__restrict__ M3, size_t gs3, const T
   const int pos = static_cast<int>(t / nl), k = static_cast<int>(t - static_cast<long long>(pos) * nl);
   const long long e = static_cast<long long>(ebase[pos]) + k;
   const int *f = map2 + static_cast<size_t>(pos) * 6;
-  double dg = to_d(M3[aidx(gs3, e, 0)]);
+  double dg = to_d(M3[aidx<T>(gs3, e, 0)]);
   for (int q = 0; q < 6; ++q) {
-    const double dq = to_d(D[aidx(gsd, e, q)]);
+    const double dq = to_d(D[aidx<T>(gsd, e, q)]);
     dg += tau * dq * dq / to_d(m2[f[q] + k]);
   }
   double up = 0.0;
   if (k < nl - 1) {
-    const double dT = to_d(D[aidx(gsd, e, 5)]);
-    const double dB = to_d(D[aidx(gsd, e + 1, 4)]);
+    const double dT = to_d(D[aidx<T>(gsd, e, 5)]);
+    const double dB = to_d(D[aidx<T>(gsd, e + 1, 4)]);
     up = tau * dT * dB / to_d(m2[f[5] + k]);
   }
   const size_t o = static_cast<size_t>(map3[pos]) + k;

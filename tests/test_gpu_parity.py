@@ -44,6 +44,16 @@ def test_upload_download_round_trip(hip, orc):
     A = np.random.default_rng(0).standard_normal((m.ncell_3d, 8, 8))
     op = hip.LmaOperator(fs).upload(A)
     np.testing.assert_array_equal(op.download(), A)
+    # precision changes move entries between the FP64 and FP32 chunk layouts
+    op32 = op.convert(hip.R32)
+    np.testing.assert_array_equal(op32.download(), A.astype(np.float32).astype(np.float64))
+    np.testing.assert_array_equal(op32.convert(hip.R64).download(), A.astype(np.float32).astype(np.float64))
+    # order 1 (27x27 entries, not a whole number of chunks per row)
+    fs1, m1 = _mesh_pair(hip, orc, 4, 2, 0, 1)
+    A1 = np.random.default_rng(1).standard_normal((m1.ncell_3d, 27, 27))
+    op1 = hip.LmaOperator(fs1).upload(A1)
+    np.testing.assert_array_equal(op1.download(), A1)
+    np.testing.assert_array_equal(op1.convert(hip.R32).download(), A1.astype(np.float32).astype(np.float64))
 
 
 def _bound(orc, m, A, x, rel):
@@ -51,9 +61,11 @@ def _bound(orc, m, A, x, rel):
     return rel * (abs(S) @ np.abs(x))
 
 
-@pytest.mark.parametrize("n,nl,order", [(4, 3, 0), (12, 20, 0), (4, 5, 1), (12, 7, 1)])
-def test_lma_apply_fp64_parity(hip, orc, n, nl, order):
-    fs, m = _mesh_pair(hip, orc, n, nl, 0, order)
+@pytest.mark.parametrize("method", ["apply", "apply_gather"])
+@pytest.mark.parametrize("n,nl,space,order", [(4, 3, 0, 0), (12, 20, 0, 0), (4, 5, 0, 1), (12, 7, 0, 1),
+                                              (6, 4, 2, 0), (5, 3, 0, 0)])
+def test_lma_apply_fp64_parity(hip, orc, n, nl, space, order, method):
+    fs, m = _mesh_pair(hip, orc, n, nl, space, order)
     rng = np.random.default_rng(n + nl)
     A = rng.standard_normal((m.ncell_3d, m.ndf, m.ndf))
     x = rng.standard_normal(m.undf)
@@ -61,17 +73,19 @@ def test_lma_apply_fp64_parity(hip, orc, n, nl, order):
     xv = hip.FieldVector(fs).import_field(x)
     yv = hip.FieldVector(fs)
     yv.set_scalar(12345.0)  # the apply must overwrite, not accumulate
-    op.apply(xv, yv)
+    getattr(op, method)(xv, yv)
     y = yv.export_field()
     yo = orc.lma_apply(m, A, x)
     assert np.all(np.abs(y - yo) <= _bound(orc, m, A, x, 1e-13))
     # run-to-run bitwise determinism (no atomics)
-    op.apply(xv, yv)
+    yv.set_scalar(-1.0)
+    getattr(op, method)(xv, yv)
     np.testing.assert_array_equal(yv.export_field(), y)
 
 
+@pytest.mark.parametrize("method", ["apply", "apply_gather"])
 @pytest.mark.parametrize("order", [0, 1])
-def test_lma_apply_fp32_parity(hip, orc, order):
+def test_lma_apply_fp32_parity(hip, orc, order, method):
     fs, m = _mesh_pair(hip, orc, 12, 10, 0, order)
     A = orc.gen_spd(m.ndf, m.ncell_3d, 3, scaled=True)
     x = orc.gen_uniform(m.undf, 0, 3)
@@ -79,7 +93,7 @@ def test_lma_apply_fp32_parity(hip, orc, order):
     op32 = op64.convert(hip.R32)
     xv = hip.FieldVector(fs, hip.R32).import_field(x)
     yv = hip.FieldVector(fs, hip.R32)
-    op32.apply(xv, yv)
+    getattr(op32, method)(xv, yv)
     y = yv.export_field()
     yo = orc.lma_apply(m, A.astype(np.float32), x.astype(np.float32), r32=True).astype(np.float64)
     assert np.all(np.abs(y - yo) <= _bound(orc, m, A, x, 2e-6))
@@ -93,8 +107,10 @@ def test_lma_apply_known_answer(hip):
     op = hip.LmaOperator(to, fr).upload(np.array(k["d"])[None])
     x = hip.FieldVector(fr).import_field(k["x"])
     y = hip.FieldVector(to)
-    op.apply(x, y)
-    np.testing.assert_allclose(y.export_field(), k["answer"], rtol=0, atol=k["tol"])
+    for method in ("apply", "apply_gather"):
+        y.set_scalar(7.0)
+        getattr(op, method)(x, y)
+        np.testing.assert_allclose(y.export_field(), k["answer"], rtol=0, atol=k["tol"])
 
 
 def test_mm_diagonal_parity(hip, orc):
