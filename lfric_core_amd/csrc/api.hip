@@ -210,12 +210,12 @@ static int space_from_topology(const std::shared_ptr<Topology> &t, int nlayers, 
   f.space = space;
   f.order = order;
   f.topo = t;
-  f.ncolours = 4;
-  f.npc.assign(4, 0);
+  f.ncolours = t->ncolours;
+  f.npc.assign(f.ncolours, 0);
   for (int c = 0; c < t->ncells; ++c) f.npc[t->colour_of_cell[c] - 1]++;
   f.maxcc = *std::max_element(f.npc.begin(), f.npc.end());
-  f.cic.assign(static_cast<size_t>(4) * f.maxcc, 0);
-  std::vector<int> fill(4, 0);
+  f.cic.assign(static_cast<size_t>(f.ncolours) * f.maxcc, 0);
+  std::vector<int> fill(f.ncolours, 0);
   for (int c = 1; c <= t->ncells; ++c) {
     const int col = t->colour_of_cell[c - 1] - 1;
     f.cic[static_cast<size_t>(col) * f.maxcc + fill[col]++] = c;

@@ -54,7 +54,8 @@ struct Topology {
   std::vector<int> vert_on_cell;          // [cell][4] SW,SE,NE,NW local vertex ids (1-based)
   std::vector<int> face_on_cell;          // [cell][6] W,S,E,N,B,T
   std::vector<int> edge_on_cell;          // [cell][12]
-  std::vector<int> colour_of_cell;        // 1..4
+  std::vector<int> colour_of_cell;        // 1..ncolours
+  int ncolours = 0;
 };
 int build_cubedsphere_topology(int edge_cells, Topology &t);
 int build_dofmap(const Topology &t, int nlayers, int space, int order, std::vector<int> &dofmap,

@@ -178,16 +178,68 @@ int closed_form_colour(int n, int cell) {
 // No two cells of one colour may share an edge or a vertex (mesh_colouring_mod.F90:765-837,
 // checked here directly on shared vertices).
 bool colouring_valid(const Topology &t) {
-  std::vector<int> seen(static_cast<size_t>(t.nverts + 1) * 4, 0);
+  const int K = t.ncolours;
+  std::vector<int> seen(static_cast<size_t>(t.nverts + 1) * K, 0);
   for (int c = 1; c <= t.ncells; ++c) {
     int col = t.colour_of_cell[c - 1] - 1;
+    if (col < 0 || col >= K) return false;
     for (int v = 0; v < 4; ++v) {
-      int &s = seen[static_cast<size_t>(t.vert_on_cell[(c - 1) * 4 + v]) * 4 + col];
+      int &s = seen[static_cast<size_t>(t.vert_on_cell[(c - 1) * 4 + v]) * K + col];
       if (s) return false;
       s = c;
     }
   }
   return true;
+}
+
+// set_colours_generic (mesh_colouring_mod.F90:514-650) with choose_colour_greedy (:848-864),
+// the reference's fallback when the closed form does not validate (:100-113): cells in id order
+// take the lowest colour (of at most 50) not yet held by a cell sharing an edge, or a vertex as
+// found through the N and S neighbours.  Returns the number of colours, 0 on failure.
+int greedy_colour(Topology &t) {
+  constexpr int kMaxColours = 50;
+  enum { kW = 0, kS = 1, kE = 2, kN = 3 };
+  const int nc = t.ncells;
+  auto nx = [&](int d, int c) { return c > 0 ? t.cell_next[static_cast<size_t>(c - 1) * 4 + d] : 0; };
+  auto cm = [&](int c) { return c > 0 ? t.colour_of_cell[c - 1] : 0; };
+  std::fill(t.colour_of_cell.begin(), t.colour_of_cell.end(), 0);
+  std::vector<int> count(kMaxColours + 1, 0);
+  for (int c = 1; c <= nc; ++c) {
+    bool used[kMaxColours + 1] = {};
+    auto mark = [&](int cell) { used[cm(cell)] = true; };
+    // N and S neighbours, then the two cells across each of them that share only a vertex;
+    // which two depends on the neighbour's orientation (its S or N leads back to c)
+    for (int d : {kN, kS}) {
+      const int u = nx(d, c);
+      if (u <= 0) continue;
+      mark(u);
+      if (c == nx(d == kN ? kS : kN, u)) {
+        mark(nx(kE, u));
+        mark(nx(kW, u));
+      } else {
+        mark(nx(kN, u));
+        mark(nx(kS, u));
+      }
+    }
+    mark(nx(kE, c));
+    mark(nx(kW, c));
+    int colour = 0;
+    for (int i = 1; i <= kMaxColours; ++i)
+      if (!used[i]) {
+        colour = i;
+        break;
+      }
+    if (colour == 0) return 0;
+    t.colour_of_cell[c - 1] = colour;
+    count[colour]++;
+  }
+  int ncol = kMaxColours;
+  for (int i = 1; i <= kMaxColours; ++i)
+    if (count[i] == 0) {
+      ncol = i - 1;
+      break;
+    }
+  return ncol;
 }
 
 }  // namespace
@@ -283,9 +335,12 @@ int build_cubedsphere_topology(int n, Topology &t) {
   (void)nside;
   t.colour_of_cell.resize(nc);
   for (int c = 1; c <= nc; ++c) t.colour_of_cell[c - 1] = closed_form_colour(n, c);
-  if (!colouring_valid(t))
-    return fail(LFRIC_E_UNSUPPORTED,
-                "closed-form cubed-sphere colouring does not hold for edge_cells=" + std::to_string(n));
+  t.ncolours = 4;
+  if (!colouring_valid(t)) {
+    t.ncolours = greedy_colour(t);
+    if (t.ncolours == 0 || !colouring_valid(t))
+      return fail(LFRIC_E_UNSUPPORTED, "no valid colouring found for edge_cells=" + std::to_string(n));
+  }
   return LFRIC_OK;
 }
 

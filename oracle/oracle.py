@@ -57,7 +57,9 @@ def lib():
             "orc_mesh_connectivity": (_I, [_I, _VP, _VP, _VP, _VP, _VP, _VP]),
             "orc_ndf": (_I, [_I, _I]),
             "orc_dofmap": (_I, [_I, _I, _I, _I, _I, _I, _VP, _VP, _VP, _VP, _VP]),
-            "orc_colour_cubedsphere": (_I, [_I, _VP, _VP, _VP, _VP]),
+            "orc_colour_cubedsphere": (_I, [_I, _VP, _VP]),
+            "orc_colour_generic": (_I, [_I, _VP, _VP]),
+            "orc_colour_lookup": (_I, [_I, _I, _VP, _VP, _VP, _I]),
             "orc_colour_validate": (_I, [_I, _VP, _VP]),
             "orc_rng_bits": (_U64, [_U64, _U64, _U64]),
             "orc_rng_uniform": (_D, [_U64, _U64, _U64]),
@@ -132,13 +134,20 @@ def dofmap(space, order, nlayers, vert_on_cell, face_on_cell, edge_on_cell, nver
 
 
 def colour_cubedsphere(n, cell_next):
+    """set_colours for a 6-panel mesh: the closed form, or the generic greedy colouring when the
+    closed form does not validate.  Returns (colour_map, ncells_per_colour, cells_in_colour)."""
     nc = 6 * n * n
+    cn = _p(np.ascontiguousarray(cell_next, np.int32))
     cm = np.zeros(nc, np.int32)
-    npc = np.zeros(4, np.int32)
-    maxc = (nc + 3) // 4
-    cic = np.zeros((4, maxc), np.int32)
-    rc = lib().orc_colour_cubedsphere(n, _p(np.ascontiguousarray(cell_next, np.int32)), _p(cm), _p(npc), _p(cic))
-    return rc, cm, npc, cic
+    ncol = 4
+    if lib().orc_colour_cubedsphere(n, cn, _p(cm)) != 0:
+        ncol = lib().orc_colour_generic(nc, cn, _p(cm))
+        assert ncol > 0, "generic colouring did not validate"
+    npc = np.zeros(ncol, np.int32)
+    maxcc = int(np.bincount(cm, minlength=ncol + 1)[1:].max())
+    cic = np.zeros((ncol, maxcc), np.int32)
+    lib().orc_colour_lookup(nc, ncol, _p(cm), _p(npc), _p(cic), maxcc)
+    return cm, npc, cic
 
 
 def colour_validate(cell_next, colour_map):
@@ -157,8 +166,8 @@ class Mesh:
             self.cell_next, self.vert_on_cell)
         self.dofmap, self.undf = dofmap(space, order, nlayers, self.vert_on_cell, self.face_on_cell,
                                         self.edge_on_cell, self.nverts, self.nedges_2d)
-        rc, self.colour_map, self.ncells_per_colour, self.cells_in_colour = colour_cubedsphere(n, self.cell_next)
-        assert rc == 0, "closed-form colouring did not validate"
+        self.colour_map, self.ncells_per_colour, self.cells_in_colour = colour_cubedsphere(n, self.cell_next)
+        self.ncolours = len(self.ncells_per_colour)
         self.ncells = 6 * n * n
         self.ndf = self.dofmap.shape[1]
         self.ncell_3d = self.ncells * nlayers
@@ -201,7 +210,7 @@ def lma_apply(mesh, A, x, to_mesh=None, r32=False):
         x = np.ascontiguousarray(x, np.float64)
         y = np.zeros(to.undf, np.float64)
         fn = lib().orc_lma_apply
-    fn(mesh.ncells, mesh.nlayers, 4, _p(to.ncells_per_colour), _p(to.cells_in_colour),
+    fn(mesh.ncells, mesh.nlayers, len(to.ncells_per_colour), _p(to.ncells_per_colour), _p(to.cells_in_colour),
        to.cells_in_colour.shape[1], to.ndf, to.undf, _p(to.dofmap), mesh.ndf, mesh.undf,
        _p(mesh.dofmap), _p(Af), _p(x), _p(y))
     return y
@@ -217,7 +226,7 @@ def mm_diagonal(mesh, A):
 def make_problem(mesh, A, diag=None):
     Af = flat_matrix(A)
     keep = [Af, diag]
-    prob = OrcProblem(mesh.ncells, mesh.nlayers, 4, mesh.cells_in_colour.shape[1], mesh.ndf,
+    prob = OrcProblem(mesh.ncells, mesh.nlayers, len(mesh.ncells_per_colour), mesh.cells_in_colour.shape[1], mesh.ndf,
                       mesh.undf, _p(mesh.dofmap).value, _p(mesh.ncells_per_colour).value,
                       _p(mesh.cells_in_colour).value, _p(Af).value,
                       _p(diag).value if diag is not None else None, None, None, None)
@@ -308,7 +317,7 @@ def transpose_apply(to_mesh, from_mesh, A, x):
     for transpose_matrix_vector_kernel (lhs on ANY_SPACE_1 = to_mesh)."""
     y = np.zeros(to_mesh.undf)
     Af = flat_matrix(A)
-    lib().orc_transpose_apply(to_mesh.ncells, to_mesh.nlayers, 4, _p(to_mesh.ncells_per_colour),
+    lib().orc_transpose_apply(to_mesh.ncells, to_mesh.nlayers, len(to_mesh.ncells_per_colour), _p(to_mesh.ncells_per_colour),
                               _p(to_mesh.cells_in_colour), to_mesh.cells_in_colour.shape[1], to_mesh.ndf,
                               to_mesh.undf, _p(to_mesh.dofmap), from_mesh.ndf, from_mesh.undf,
                               _p(from_mesh.dofmap), _p(Af), _p(np.ascontiguousarray(x, np.float64)), _p(y))

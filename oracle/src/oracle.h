@@ -44,8 +44,9 @@ int orc_dofmap(int space, int order, int ncells, int nlayers, int nverts_2d, int
                const int *vert_on_cell, const int *face_on_cell, const int *edge_on_cell,
                int *dofmap, int *undf);
 
-int orc_colour_cubedsphere(int n, const int *cell_next, int *colour_map,
-                           int *ncells_per_colour, int *cells_in_colour);
+int orc_colour_cubedsphere(int n, const int *cell_next, int *colour_map);
+int orc_colour_generic(int ncells, const int *cell_next, int *colour_map);
+int orc_colour_lookup(int ncells, int ncolours, const int *colour_map, int *npc, int *cic, int maxcc);
 int orc_colour_validate(int ncells, const int *cell_next, const int *colour_map);
 
 /* ---- counter-based synthetic inputs -------------------------------------------------- */

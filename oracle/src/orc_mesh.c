@@ -394,12 +394,11 @@ int orc_dofmap(int space, int order, int nc, int nl, int nv2, int ne2, const int
 #undef VID
 }
 
-/* Closed-form 4-colouring of a cubed sphere with an even number of cells per panel edge
- * (infrastructure/source/mesh/mesh_colouring_mod.F90:151-308), its validator (:765-837)
- * and the colour-major lookup (:703-752).  cells_in_colour is [4][ncells/4 rounded up],
- * row-major by colour; returns 0, or 1 when the closed form does not validate. */
-int orc_colour_cubedsphere(int n, const int *cell_next, int *colour_map, int *npc,
-                           int *cic) {
+/* Closed-form 4-colouring of a cubed sphere (infrastructure/source/mesh/mesh_colouring_mod.F90:
+ * 151-308, tables :242-247 even n and :263-268 odd n) and its validator (:765-837).
+ * Fills colour_map; returns 0, or 1 when the closed form does not validate (the reference then
+ * falls back to set_colours_generic, :100-113). */
+int orc_colour_cubedsphere(int n, const int *cell_next, int *colour_map) {
   const int cpp = n * n, nc = 6 * cpp;
   /* cell_colour_order_per_panel(column,row,panel), reshape((/a,b,c,d/),(/2,2/)) ->
    * (1,1)=a (2,1)=b (1,2)=c (2,2)=d */
@@ -415,15 +414,84 @@ int orc_colour_cubedsphere(int n, const int *cell_next, int *colour_map, int *np
     int row = ((rel - 1) / n) % 2 + 1;
     colour_map[cell - 1] = tab[panel][(column - 1) + 2 * (row - 1)];
   }
-  if (!orc_colour_validate(nc, cell_next, colour_map)) return 1;
-  int maxc = (nc + 3) / 4;
-  for (int c = 0; c < 4; ++c) npc[c] = 0;
+  return orc_colour_validate(nc, cell_next, colour_map) ? 0 : 1;
+}
+
+/* set_colours_generic (mesh_colouring_mod.F90:514-650) with choose_colour_greedy (:848-864):
+ * cells in id order take the lowest colour (of at most 50) not yet held by a cell that shares
+ * an edge or, through the N and S neighbours, a vertex.  Fills colour_map; returns the number
+ * of colours (the first colour with no cells, minus one), or 0 when the result does not
+ * validate. */
+int orc_colour_generic(int nc, const int *cell_next, int *colour_map) {
+  enum { MAXCOLS = 50 };
+  int count[MAXCOLS + 1] = {0};
+#define NX(d, c) cell_next[((c)-1) * 4 + (d)-1]
+#define CM(c) ((c) > 0 ? colour_map[(c)-1] : 0)
+  for (int cell = 1; cell <= nc; ++cell) colour_map[cell - 1] = 0;
   for (int cell = 1; cell <= nc; ++cell) {
-    int col = colour_map[cell - 1] - 1;
-    if (npc[col] >= maxc) return 2;
-    cic[col * maxc + npc[col]++] = cell;
+    int used[MAXCOLS + 1] = {0};
+    int t = NX(N, cell);
+    if (t > 0) {
+      used[CM(t)] = 1;
+      if (cell == NX(S, t)) {
+        used[CM(NX(E, t))] = 1;
+        used[CM(NX(W, t))] = 1;
+      } else {
+        used[CM(NX(N, t))] = 1;
+        used[CM(NX(S, t))] = 1;
+      }
+    }
+    t = NX(E, cell);
+    if (t > 0) used[CM(t)] = 1;
+    t = NX(W, cell);
+    if (t > 0) used[CM(t)] = 1;
+    t = NX(S, cell);
+    if (t > 0) {
+      used[CM(t)] = 1;
+      if (cell == NX(N, t)) {
+        used[CM(NX(E, t))] = 1;
+        used[CM(NX(W, t))] = 1;
+      } else {
+        used[CM(NX(N, t))] = 1;
+        used[CM(NX(S, t))] = 1;
+      }
+    }
+    int colour = 0;
+    for (int i = 1; i <= MAXCOLS; ++i)
+      if (!used[i]) {
+        colour = i;
+        break;
+      }
+    colour_map[cell - 1] = colour;
+    count[colour]++;
   }
-  return 0;
+#undef NX
+#undef CM
+  int ncolours = MAXCOLS;
+  for (int i = 1; i <= MAXCOLS; ++i)
+    if (count[i] == 0) {
+      ncolours = i - 1;
+      break;
+    }
+  return orc_colour_validate(nc, cell_next, colour_map) ? ncolours : 0;
+}
+
+/* create_colour_map_lookup (mesh_colouring_mod.F90:703-752): cells of each colour in id
+ * order.  cic is [ncolours][maxcc] row-major by colour, zero-padded; maxcc must be at least
+ * the largest colour count.  Returns that count. */
+int orc_colour_lookup(int nc, int ncolours, const int *colour_map, int *npc, int *cic, int maxcc) {
+  int mx = 0;
+  for (int c = 0; c < ncolours; ++c) npc[c] = 0;
+  for (int cell = 1; cell <= nc; ++cell) npc[colour_map[cell - 1] - 1]++;
+  for (int c = 0; c < ncolours; ++c) mx = npc[c] > mx ? npc[c] : mx;
+  if (mx > maxcc) return mx;
+  for (int i = 0; i < ncolours * maxcc; ++i) cic[i] = 0;
+  for (int c = 0; c < ncolours; ++c) {
+    int i = 0;
+    for (int cell = 1; cell <= nc; ++cell)
+      if (colour_map[cell - 1] == c + 1) cic[c * maxcc + i++] = cell;
+  }
+  return mx;
 }
 
 /* test_set_colours (infrastructure/source/mesh/mesh_colouring_mod.F90:765-837).

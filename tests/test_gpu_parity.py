@@ -21,7 +21,8 @@ def _mesh_pair(hip, orc, n, nl, space=0, order=0):
 
 
 @pytest.mark.parametrize("n,nl,space,order", [(4, 3, 0, 0), (12, 20, 0, 0), (4, 5, 0, 1), (12, 5, 2, 0),
-                                              (12, 5, 3, 0), (96, 2, 0, 0)])
+                                              (12, 5, 3, 0), (96, 2, 0, 0), (3, 2, 0, 0), (5, 3, 2, 0),
+                                              (7, 2, 0, 1)])
 def test_dofmap_and_colouring_bit_exact(hip, orc, n, nl, space, order):
     fs, m = _mesh_pair(hip, orc, n, nl, space, order)
     assert (fs.ncells, fs.ndf, fs.undf) == (m.ncells, m.ndf, m.undf)

@@ -100,6 +100,30 @@ def test_colouring_closed_form(orc, n):
     assert orc.colour_validate(m.cell_next, bad) == 0
 
 
+@pytest.mark.parametrize("n", [3, 5, 7])
+def test_odd_colouring_falls_back_to_greedy(orc, n):
+    """Odd panels: the closed form does not validate, so set_colours takes the generic greedy
+    colouring.  Checked independently from the vertex lists: each cell holds the lowest colour
+    not held by a lower-numbered cell sharing a vertex with it, and no colour class shares a
+    vertex."""
+    m = orc.Mesh(n, 2)
+    cm0 = np.zeros(m.ncells, np.int32)
+    assert orc.lib().orc_colour_cubedsphere(n, orc._p(np.ascontiguousarray(m.cell_next, np.int32)),
+                                            orc._p(cm0)) == 1
+    cells_of_vert = {}
+    for c in range(m.ncells):
+        for v in m.vert_on_cell[c]:
+            cells_of_vert.setdefault(v, []).append(c)
+    for c in range(m.ncells):
+        earlier = {m.colour_map[o] for v in m.vert_on_cell[c] for o in cells_of_vert[v] if o < c}
+        assert m.colour_map[c] == min(set(range(1, 51)) - earlier)
+    for col in range(1, m.ncolours + 1):
+        verts = m.vert_on_cell[m.colour_map == col].ravel()
+        assert len(np.unique(verts)) == verts.size
+    assert m.ncells_per_colour.sum() == m.ncells and m.cells_in_colour.shape == (m.ncolours,
+                                                                                  m.ncells_per_colour.max())
+
+
 # ---- generator ------------------------------------------------------------------------------
 def test_own_log_exp_against_libm(orc):
     L = orc.lib()
