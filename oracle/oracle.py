@@ -77,6 +77,13 @@ def lib():
             "orc_mm_diagonal": (None, [_I, _I, _I, _I, _VP, _VP, _VP]),
             "orc_cg": (_I, [ctypes.POINTER(OrcProblem), _VP, _VP, _D, _D, _I, _I, ctypes.POINTER(OrcResult), _VP]),
             "orc_gmres": (_I, [ctypes.POINTER(OrcProblem), _VP, _VP, _I, _D, _D, _I, _I, ctypes.POINTER(OrcResult), _VP]),
+            "orc_bicgstab": (_I, [ctypes.POINTER(OrcProblem), _VP, _VP, _D, _D, _I, _I, ctypes.POINTER(OrcResult), _VP]),
+            "orc_fgmres": (_I, [ctypes.POINTER(OrcProblem), _VP, _VP, _I, _D, _D, _I, _I, ctypes.POINTER(OrcResult), _VP]),
+            "orc_gcr": (_I, [ctypes.POINTER(OrcProblem), _VP, _VP, _I, _D, _D, _I, _I, ctypes.POINTER(OrcResult), _VP]),
+            "orc_jacobi": (_I, [ctypes.POINTER(OrcProblem), _VP, _VP, _D, _D, _I, _I, _D, ctypes.POINTER(OrcResult),
+                                _VP]),
+            "orc_chebyshev": (_I, [ctypes.POINTER(OrcProblem), _VP, _VP, _I, _D, _D, ctypes.POINTER(OrcResult)]),
+            "orc_prec_only": (_I, [ctypes.POINTER(OrcProblem), _VP, _VP, ctypes.POINTER(OrcResult)]),
             "orc_givens_lsq": (_I, [_I, _VP, _VP, _VP]),
         }
         for name, (res, args) in sig.items():
@@ -357,3 +364,47 @@ class Helmholtz:
     def dense(self):
         n = self.w3.undf
         return np.column_stack([self.apply(np.eye(1, n, i).ravel()) for i in range(n)])
+
+
+METHODS = ("cg", "bicgstab", "gmres", "fgmres", "gcr", "prec_only", "jacobi", "chebyshev")
+
+
+def solve_callbacks(method, n, apply, prec, b, x0=None, gcrk=30, r_tol=1e-8, a_tol=1e-16, max_iter=500,
+                    monitor=True, rho_relax=1.0, lmin=0.124, lmax=3.377):
+    """Any solver of the mass-matrix solver factory (sci_mass_matrix_solver_alg_mod.x90:164-181)
+    with python-callable operator and preconditioner.  Returns (x, result, history)."""
+
+    def wrap(f):
+        def cb(ctx, xp, yp):
+            xv = np.ctypeslib.as_array(xp, shape=(n,))
+            yv = np.ctypeslib.as_array(yp, shape=(n,))
+            yv[:] = f(xv.copy())
+        return CALLBACK(cb)
+
+    ca, cp = wrap(apply), wrap(prec)
+    prob = OrcProblem(0, 0, 0, 0, 0, n, None, None, None, None, None,
+                      ctypes.cast(ca, _VP).value, ctypes.cast(cp, _VP).value, None)
+    x = np.zeros(n) if x0 is None else np.array(x0, np.float64)
+    b = np.ascontiguousarray(b, np.float64)
+    res = OrcResult()
+    hist = np.zeros(max_iter * max(gcrk, 1) + 2)
+    P, R, L = ctypes.byref(prob), ctypes.byref(res), lib()
+    if method == "cg":
+        L.orc_cg(P, _p(b), _p(x), r_tol, a_tol, max_iter, int(monitor), R, _p(hist))
+    elif method == "bicgstab":
+        L.orc_bicgstab(P, _p(b), _p(x), r_tol, a_tol, max_iter, int(monitor), R, _p(hist))
+    elif method == "gmres":
+        L.orc_gmres(P, _p(b), _p(x), gcrk, r_tol, a_tol, max_iter, int(monitor), R, _p(hist))
+    elif method == "fgmres":
+        L.orc_fgmres(P, _p(b), _p(x), gcrk, r_tol, a_tol, max_iter, int(monitor), R, _p(hist))
+    elif method == "gcr":
+        L.orc_gcr(P, _p(b), _p(x), gcrk, r_tol, a_tol, max_iter, int(monitor), R, _p(hist))
+    elif method == "jacobi":
+        L.orc_jacobi(P, _p(b), _p(x), r_tol, a_tol, max_iter, int(monitor), rho_relax, R, _p(hist))
+    elif method == "chebyshev":
+        L.orc_chebyshev(P, _p(b), _p(x), max_iter, lmin, lmax, R)
+    elif method == "prec_only":
+        L.orc_prec_only(P, _p(b), _p(x), R)
+    else:
+        raise ValueError(method)
+    return x, res, hist

@@ -21,6 +21,17 @@
  *                             Fortran kernel compiled from source) bit-level comparison
  *   orc_cg / orc_gmres        numpy.linalg.solve on tiny SPD/nonsymmetric systems; scipy cg
  *                             solutions; Givens least-squares vs numpy.linalg.lstsq
+ *   orc_colour_generic        greedy rule checked from vertex lists (independent neighbour sets)
+ *   orc_transpose_apply,      known-answer tests of transpose/dg/mm_diagonal unit tests;
+ *   orc_dg_apply,             bit-level comparison with the compiled reference kernels
+ *   orc_mm_diagonal
+ *   orc_bicgstab              numpy.linalg.solve on a tiny nonsymmetric system
+ *   orc_fgmres / orc_gcr      one cycle = minimal residual over the right-preconditioned Krylov
+ *                             space (numpy lstsq on an explicit basis); GCR(m) == FGMRES(m+1)
+ *   orc_jacobi                closed form x_k = x* + (I - rho P^-1 A)^k (x_0 - x*)
+ *   orc_chebyshev             convergence to numpy.linalg.solve with exact eigenvalue bounds;
+ *                             linearity in b
+ *   orc_prec_only             x = b / diag exactly
  */
 #ifndef LFRIC_ORACLE_H
 #define LFRIC_ORACLE_H
@@ -106,6 +117,17 @@ int orc_cg(const orc_problem *p, const double *b, double *x, double r_tol, doubl
 int orc_gmres(const orc_problem *p, const double *b, double *x, int gcrk, double r_tol,
               double a_tol, int max_iter, int monitor, orc_result *res, double *history);
 int orc_givens_lsq(int m, const double *h_in, const double *g_in, double *u);
+int orc_bicgstab(const orc_problem *p, const double *b, double *x, double r_tol, double a_tol,
+                 int max_iter, int monitor, orc_result *res, double *history);
+int orc_fgmres(const orc_problem *p, const double *b, double *x, int gcrk, double r_tol,
+               double a_tol, int max_iter, int monitor, orc_result *res, double *history);
+int orc_gcr(const orc_problem *p, const double *b, double *x, int gcrk, double r_tol, double a_tol,
+            int max_iter, int monitor, orc_result *res, double *history);
+int orc_jacobi(const orc_problem *p, const double *b, double *x, double r_tol, double a_tol,
+               int max_iter, int monitor, double rho_relax, orc_result *res, double *history);
+int orc_chebyshev(const orc_problem *p, const double *b, double *x, int max_iter, double lmin,
+                  double lmax, orc_result *res);
+int orc_prec_only(const orc_problem *p, const double *b, double *x, orc_result *res);
 
 #ifdef __cplusplus
 }

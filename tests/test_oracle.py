@@ -331,3 +331,85 @@ def test_helmholtz_composition_matches_assembled_sparse_operators(orc):
     assert np.linalg.eigvalsh(Hd).min() > 0.9  # M3 >= 1 on the diagonal, D-part PSD
     # W2 divergence signs W,S,E,N,B,T = -,-,+,+,-,+
     assert np.all(np.sign(H.D[:, 0, :]) == orc.D_SIGNS)
+
+
+# ---- the remaining solvers of the mass-matrix solver factory ----------------------------------
+def _small_system(seed=7, n=30, sym=False):
+    rng = np.random.default_rng(seed)
+    M = rng.standard_normal((n, n)) * 0.3
+    M = M @ M.T + n * 0.25 * np.eye(n) if sym else np.eye(n) * 4 + M
+    d = np.diag(M).copy()
+    b = rng.standard_normal(n)
+    return M, d, b
+
+
+def test_bicgstab_solves_nonsymmetric_system(orc):
+    M, d, b = _small_system()
+    x, res, _ = orc.solve_callbacks("bicgstab", len(b), lambda v: M @ v, lambda v: v / d, b, r_tol=1e-12,
+                                    max_iter=200)
+    assert res.converged == 1
+    np.testing.assert_allclose(x, np.linalg.solve(M, b), rtol=0, atol=1e-9)
+
+
+def _right_prec_krylov_min(M, d, b, k, x0=None):
+    """Minimal-residual x over x0 + span{P^-1 r0, P^-1 (A P^-1) r0, ..., P^-1 (A P^-1)^(k-1) r0},
+    r0 = b - A x0 (P = diag d), by least squares on an explicit basis."""
+    x0 = np.zeros_like(b) if x0 is None else x0
+    r0 = b - M @ x0
+    K = [r0 / d]
+    for _ in range(k - 1):
+        K.append((M @ K[-1]) / d)
+    B = np.array(K).T
+    c = np.linalg.lstsq(M @ B, r0, rcond=None)[0]
+    return x0 + B @ c
+
+
+@pytest.mark.parametrize("m", [3, 6])
+def test_fgmres_and_gcr_one_cycle_are_minimal_residual(orc, m):
+    M, d, b = _small_system(n=25)
+    P = lambda v: v / d
+    xf, rf, _ = orc.solve_callbacks("fgmres", len(b), lambda v: M @ v, P, b, gcrk=m, max_iter=1, r_tol=1e-30)
+    np.testing.assert_allclose(xf, _right_prec_krylov_min(M, d, b, m), rtol=0, atol=1e-10)
+    # GCR(m) starts from x0 = P^-1 b and minimises over x0 + P^-1 K_m(A P^-1, b - A x0)
+    xg, rg, _ = orc.solve_callbacks("gcr", len(b), lambda v: M @ v, P, b, gcrk=m, max_iter=1, r_tol=1e-30)
+    np.testing.assert_allclose(xg, _right_prec_krylov_min(M, d, b, m, x0=b / d), rtol=0, atol=1e-10)
+    # restarted to convergence both reach the solution
+    for meth in ("fgmres", "gcr"):
+        x, r, _ = orc.solve_callbacks(meth, len(b), lambda v: M @ v, P, b, gcrk=m, max_iter=200, r_tol=1e-12)
+        assert r.converged == 1
+        np.testing.assert_allclose(x, np.linalg.solve(M, b), rtol=0, atol=1e-9)
+
+
+def test_jacobi_matches_closed_form(orc):
+    M, d, b = _small_system(sym=True)
+    xs = np.linalg.solve(M, b)
+    x0 = np.random.default_rng(1).standard_normal(len(b))
+    rho, k = 0.7, 12
+    x, res, _ = orc.solve_callbacks("jacobi", len(b), lambda v: M @ v, lambda v: v / d, b, x0=x0, max_iter=k,
+                                    monitor=False, rho_relax=rho)
+    G = np.eye(len(b)) - rho * (M / d[:, None])
+    np.testing.assert_allclose(x, xs + np.linalg.matrix_power(G, k) @ (x0 - xs), rtol=0, atol=1e-12)
+    x, res, _ = orc.solve_callbacks("jacobi", len(b), lambda v: M @ v, lambda v: v / d, b, x0=x0, max_iter=2000,
+                                    r_tol=1e-10, rho_relax=rho)
+    assert res.converged == 1 and res.r_nrm <= 1e-10 * res.r_nrm_0
+
+
+def test_chebyshev_converges_with_exact_bounds_and_is_linear(orc):
+    M, d, b = _small_system(sym=True)
+    Dh = 1 / np.sqrt(d)
+    ev = np.linalg.eigvalsh(Dh[:, None] * M * Dh[None, :])
+    kw = dict(lmin=ev.min(), lmax=ev.max(), max_iter=80)
+    A, P = (lambda v: M @ v), (lambda v: v / d)
+    x, res, _ = orc.solve_callbacks("chebyshev", len(b), A, P, b, **kw)
+    np.testing.assert_allclose(x, np.linalg.solve(M, b), rtol=0, atol=1e-8)
+    b2 = np.random.default_rng(9).standard_normal(len(b))
+    x2, _, _ = orc.solve_callbacks("chebyshev", len(b), A, P, b2, **kw)
+    x12, _, _ = orc.solve_callbacks("chebyshev", len(b), A, P, b + 2 * b2, **kw)
+    np.testing.assert_allclose(x12, x + 2 * x2, rtol=0, atol=1e-12)
+
+
+def test_prec_only_is_one_division(orc):
+    M, d, b = _small_system()
+    x, res, _ = orc.solve_callbacks("prec_only", len(b), lambda v: M @ v, lambda v: v / d, b)
+    np.testing.assert_array_equal(x, b / d)
+    assert abs(res.r_nrm - np.linalg.norm(b - M @ x)) <= 1e-13 * np.linalg.norm(b)
