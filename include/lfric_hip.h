@@ -191,6 +191,8 @@ typedef struct lfric_solver_opts {
   int check_every;           /* host polls the device convergence flag every n iterations */
   double inner_r_tol;        /* mixed precision: inner (FP32) relative reduction per cycle */
   int max_outer;             /* mixed precision: maximum refinement cycles */
+  double rho_relax;          /* Jacobi relaxation (solver_config jacobi_relaxation) */
+  double lmin, lmax;         /* Chebyshev eigenvalue bounds of P^-1 A (fs_eigenvalues_mod) */
 } lfric_solver_opts;
 
 typedef struct lfric_solver_result {
@@ -229,6 +231,46 @@ int lfric_gmres_solve(const lfric_linop *A, const lfric_prec *P, lfric_vec *x, c
 int lfric_ir_gmres_solve(const lfric_linop *A64, const lfric_linop *A32, const lfric_prec *P32,
                          lfric_vec *x, const lfric_vec *b, const lfric_solver_opts *opts,
                          lfric_solver_result *res);
+
+/* fgmres_solve (sci_iterative_solver_mod.f90:1055-1237): flexible restarted GMRES with a
+ * right preconditioner; as in the reference the residual starts from b (zero initial guess
+ * assumed) and x += dx at the end.  max_iter counts restart cycles. */
+int lfric_fgmres_solve(const lfric_linop *A, const lfric_prec *P, lfric_vec *x, const lfric_vec *b,
+                       const lfric_solver_opts *opts, lfric_solver_result *res);
+/* bicgstab_solve (sci_iterative_solver_mod.f90:645-779), device resident; the preconditioner
+ * acts on the search directions. */
+int lfric_bicgstab_solve(const lfric_linop *A, const lfric_prec *P, lfric_vec *x, const lfric_vec *b,
+                         const lfric_solver_opts *opts, lfric_solver_result *res);
+/* gcr_solve (sci_iterative_solver_mod.f90:1288-1408): restarted GCR(gcrk), right
+ * preconditioner, dx from P^-1 b, relative to ||b||; max_iter counts restart cycles. */
+int lfric_gcr_solve(const lfric_linop *A, const lfric_prec *P, lfric_vec *x, const lfric_vec *b,
+                    const lfric_solver_opts *opts, lfric_solver_result *res);
+/* jacobi_solve (sci_iterative_solver_mod.f90, jacobi_smod): x -= rho_relax P^-1 (A x - b). */
+int lfric_jacobi_solve(const lfric_linop *A, const lfric_prec *P, lfric_vec *x, const lfric_vec *b,
+                       const lfric_solver_opts *opts, lfric_solver_result *res);
+/* chebyshev_solve (sci_iterative_solver_mod.f90, chebyshev_smod): max_iter steps from x = 0
+ * for eigenvalue bounds [lmin, lmax]; r_nrm = ||A x - b||, r_nrm_0 = max(1, ||b||). */
+int lfric_chebyshev_solve(const lfric_linop *A, const lfric_prec *P, lfric_vec *x, const lfric_vec *b,
+                          const lfric_solver_opts *opts, lfric_solver_result *res);
+/* precondition_only_solve (sci_iterative_solver_mod.f90, precondition_only_smod): x = P^-1 b. */
+int lfric_prec_only_solve(const lfric_linop *A, const lfric_prec *P, lfric_vec *x, const lfric_vec *b,
+                          const lfric_solver_opts *opts, lfric_solver_result *res);
+
+/* Solver methods of the mass-matrix solve factory (solver_config method_*). */
+#define LFRIC_METHOD_CG 1
+#define LFRIC_METHOD_BICGSTAB 2
+#define LFRIC_METHOD_GMRES 3
+#define LFRIC_METHOD_FGMRES 4
+#define LFRIC_METHOD_GCR 5
+#define LFRIC_METHOD_PREC_ONLY 6
+#define LFRIC_METHOD_JACOBI 7
+#define LFRIC_METHOD_CHEBYSHEV 8
+/* select case(method) of mass_matrix_solver_alg
+ * (components/science/source/algorithm/solver/sci_mass_matrix_solver_alg_mod.x90:164-181):
+ * one entry point for every method, on an operator and a preconditioner (identity or
+ * diagonal in the reference's factory). */
+int lfric_solve(int method, const lfric_linop *A, const lfric_prec *P, lfric_vec *x, const lfric_vec *b,
+                const lfric_solver_opts *opts, lfric_solver_result *res);
 
 #ifdef __cplusplus
 }

@@ -13,12 +13,17 @@ Reference interfaces mirrored (paths relative to the lfric_core tree):
   DiagonalPreconditioner sci_diagonal_preconditioner_alg_mod.x90:71-106
   NullPreconditioner     sci_null_preconditioner_alg_mod.x90
   ConjugateGradient      components/science/source/solver/sci_iterative_solver_mod.f90:87-121, 465-592
+  GMRES, FGMRES, GCR,    the other abstract_iterative_solver_type extensions of
+  BiCGStab, Jacobi,      sci_iterative_solver_mod.f90
+  Chebyshev, PreconditionOnly
+  solve                  the method factory of sci_mass_matrix_solver_alg_mod.x90:164-181
 """
 from ._lib import (LfricError, NotConverged, lib, init, synchronize, timer_start, timer_stop,
                    device_bytes, R64, R32, BF16, F16, W0, W2, W3)
 from .fem import FunctionSpace, FieldVector, LmaOperator
 from .solvers import (NullPreconditioner, DiagonalPreconditioner, ColumnJacobiPreconditioner,
                       LinearOperator, ConjugateGradient, MixedPrecisionCG, GMRES, MixedPrecisionGMRES,
+                      FGMRES, GCR, BiCGStab, Jacobi, Chebyshev, PreconditionOnly, solve, METHODS,
                       SolverResult, SolverOptions)
 
 __all__ = [
@@ -26,5 +31,6 @@ __all__ = [
     "device_bytes", "R64", "R32", "BF16", "F16", "W0", "W2", "W3", "FunctionSpace", "FieldVector",
     "LmaOperator", "NullPreconditioner", "DiagonalPreconditioner", "ColumnJacobiPreconditioner",
     "LinearOperator", "ConjugateGradient", "MixedPrecisionCG", "GMRES", "MixedPrecisionGMRES",
+    "FGMRES", "GCR", "BiCGStab", "Jacobi", "Chebyshev", "PreconditionOnly", "solve", "METHODS",
     "SolverResult", "SolverOptions",
 ]

@@ -39,7 +39,7 @@ _DP = ctypes.POINTER(ctypes.c_double)
 class SolverOpts(ctypes.Structure):
     _fields_ = [("r_tol", _D), ("a_tol", _D), ("max_iter", _I), ("gcrk", _I),
                 ("monitor_convergence", _I), ("fail_on_non_converged", _I), ("check_every", _I),
-                ("inner_r_tol", _D), ("max_outer", _I)]
+                ("inner_r_tol", _D), ("max_outer", _I), ("rho_relax", _D), ("lmin", _D), ("lmax", _D)]
 
 
 class SolverRes(ctypes.Structure):
@@ -106,6 +106,13 @@ SIGNATURES = {
     "lfric_residual_norm": (_I, [_VP, _VP, _VP, _DP]),
     "lfric_cg_solve": (_I, [_VP, _VP, _VP, _VP, ctypes.POINTER(SolverOpts), ctypes.POINTER(SolverRes)]),
     "lfric_ir_cg_solve": (_I, [_VP, _VP, _VP, _VP, _VP, ctypes.POINTER(SolverOpts), ctypes.POINTER(SolverRes)]),
+    "lfric_fgmres_solve": (_I, [_VP, _VP, _VP, _VP, ctypes.POINTER(SolverOpts), ctypes.POINTER(SolverRes)]),
+    "lfric_bicgstab_solve": (_I, [_VP, _VP, _VP, _VP, ctypes.POINTER(SolverOpts), ctypes.POINTER(SolverRes)]),
+    "lfric_gcr_solve": (_I, [_VP, _VP, _VP, _VP, ctypes.POINTER(SolverOpts), ctypes.POINTER(SolverRes)]),
+    "lfric_jacobi_solve": (_I, [_VP, _VP, _VP, _VP, ctypes.POINTER(SolverOpts), ctypes.POINTER(SolverRes)]),
+    "lfric_chebyshev_solve": (_I, [_VP, _VP, _VP, _VP, ctypes.POINTER(SolverOpts), ctypes.POINTER(SolverRes)]),
+    "lfric_prec_only_solve": (_I, [_VP, _VP, _VP, _VP, ctypes.POINTER(SolverOpts), ctypes.POINTER(SolverRes)]),
+    "lfric_solve": (_I, [_I, _VP, _VP, _VP, _VP, ctypes.POINTER(SolverOpts), ctypes.POINTER(SolverRes)]),
 }
 
 _lib = None

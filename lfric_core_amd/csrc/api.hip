@@ -705,6 +705,9 @@ void lfric_solver_opts_default(lfric_solver_opts *o) {
   o->check_every = 8;
   o->inner_r_tol = 1e-4;
   o->max_outer = 20;
+  o->rho_relax = 1.0;
+  o->lmin = 0.124;  // W0 bounds of fs_eigenvalues_mod (w0_lmin, w0_lmax)
+  o->lmax = 3.377;
 }
 
 static int finish_status(const SolverState &s, const lfric_solver_opts &o, lfric_solver_result *res) {

@@ -434,6 +434,85 @@ static int gmres_t(const lfric_linop *A, const lfric_prec *P, T *x, const T *b, 
   return LFRIC_OK;
 }
 
+// fgmres_solve (sci_iterative_solver_mod.f90:1055-1237): flexible (right-preconditioned)
+// restarted GMRES.  As in the reference the residual starts from b (a zero initial guess is
+// assumed, :1080), the correction dx is built from the preconditioned basis Pv, and x += dx at
+// the end.  One host synchronisation per restart cycle.
+template <typename T>
+static int fgmres_t(const lfric_linop *A, const lfric_prec *P, T *x, const T *b, long long n, int m, double r_tol,
+                    double a_tol, int max_cycles, int monitor, GmresOut &out) {
+  if (int rc = gm_init()) return rc;
+  LF_CHECK(m >= 1 && m <= kMaxKrylov, LFRIC_E_ARG, "gcrk must be in [1, 64]");
+  const int prec = sizeof(T) == 8 ? LFRIC_R64 : LFRIC_R32;
+  void *vV, *vPV, *vw, *vr, *vdx;
+  if (int rc = gm_ws(0, sizeof(T) * static_cast<size_t>(n) * m, &vV)) return rc;
+  if (int rc = gm_ws(5, sizeof(T) * static_cast<size_t>(n) * m, &vPV)) return rc;
+  if (int rc = gm_ws(1, sizeof(T) * static_cast<size_t>(n), &vw)) return rc;
+  if (int rc = gm_ws(3, sizeof(T) * static_cast<size_t>(n), &vr)) return rc;
+  if (int rc = gm_ws(4, sizeof(T) * static_cast<size_t>(n), &vdx)) return rc;
+  T *V = static_cast<T *>(vV), *PV = static_cast<T *>(vPV), *w = static_cast<T *>(vw);
+  T *r = static_cast<T *>(vr), *dx = static_cast<T *>(vdx);
+  GmresState *st = g_gstate;
+  const int vg = grid_for(n, kBlock, 8192);
+  if (int rc = launch_set(prec, dx, n, 0.0)) return rc;
+  if (int rc = launch_copy(prec, r, b, n)) return rc;  // res = b
+  if (int rc = dot_into<T>(r, r, n, &st->rnorm2, true)) return rc;
+  double sc_err;
+  if (int rc = read_scalar(&st->rnorm2, &sc_err)) return rc;
+  out.r0 = out.rn = sc_err;
+  if (!(sc_err == sc_err)) {
+    out.status = LFRIC_NAN;
+    return LFRIC_OK;
+  }
+  if (sc_err < 2.220446049250313e-16) {
+    out.status = LFRIC_OK;
+    return LFRIC_OK;
+  }
+  if (monitor) sc_err = std::max(sc_err, a_tol);
+  // beta = ||s||, v1 = s / beta (s = res)
+  if (int rc = dot_into<T>(r, r, n, &st->beta, true)) return rc;
+  LAUNCHK(k_scale_by_inv<T>, vg, kBlock, V, r, n, &st->beta);
+  out.status = monitor ? LFRIC_NOT_CONVERGED : LFRIC_OK;
+  for (int cyc = 1; cyc <= max_cycles; ++cyc) {
+    for (int iv = 0; iv < m; ++iv) {
+      T *viv = V + static_cast<size_t>(iv) * n, *pviv = PV + static_cast<size_t>(iv) * n;
+      if (int rc = prec_apply_any(P, prec, viv, pviv, n)) return rc;  // Pv = P^-1 v
+      if (int rc = linop_apply(A, pviv, w)) return rc;               // w = s = A Pv
+      for (int j = 0; j <= iv + 1; ++j) {
+        const T *vprev = j > 0 ? V + static_cast<size_t>(j - 1) * n : nullptr;
+        const T *vj = j <= iv ? V + static_cast<size_t>(j) * n : nullptr;
+        const double *hprev = j > 0 ? &st->h[iv * (m + 1) + (j - 1)] : nullptr;
+        LAUNCHK(k_mgs_step<T>, kGmBlocks, kBlock, w, vprev, vj, n, hprev, g_gpart);
+        LAUNCHK(k_finish_to, 1, kBlock, g_gpart, kGmBlocks, &st->h[iv * (m + 1) + j], j == iv + 1 ? 1 : 0);
+      }
+      if (iv < m - 1)
+        LAUNCHK(k_scale_by_inv<T>, vg, kBlock, V + static_cast<size_t>(iv + 1) * n, w, n,
+                &st->h[iv * (m + 1) + iv + 1]);
+    }
+    LAUNCHK(k_givens, 1, 64, st, m);
+    LAUNCHK(k_gmres_update<T>, vg, kBlock, dx, PV, n, m, st->u);  // dx += sum u_iv Pv_iv
+    LF_HIP(hipGetLastError());
+    if (int rc = linop_apply(A, dx, w)) return rc;                 // Ax = A dx
+    if (int rc = launch_copy(prec, r, w, n)) return rc;
+    if (int rc = launch_cg_residual(prec, r, b, n)) return rc;     // res%aypx(-1, b)
+    if (int rc = dot_into<T>(r, r, n, &st->beta, true)) return rc;
+    double rn;
+    if (int rc = read_scalar(&st->beta, &rn)) return rc;
+    out.cycles = cyc;
+    out.rn = rn;
+    if (!(rn == rn)) {
+      out.status = LFRIC_NAN;
+      break;
+    }
+    if (monitor && rn / sc_err < r_tol) {
+      out.status = LFRIC_OK;
+      break;
+    }
+    LAUNCHK(k_scale_by_inv<T>, vg, kBlock, V, r, n, &st->beta);  // v1 = res / beta
+  }
+  return launch_axpy(prec, x, dx, n, 1.0);  // x = x + dx
+}
+
 }  // namespace lfric
 
 using namespace lfric;
@@ -628,6 +707,33 @@ int lfric_gmres_solve(const lfric_linop *A, const lfric_prec *P, lfric_vec *x, c
   }
   if (g.status == LFRIC_NOT_CONVERGED && !opts->fail_on_non_converged) return LFRIC_OK;
   if (g.status != LFRIC_OK) set_error(g.status == LFRIC_NAN ? "NaN in GMRES residual" : "GMRES did not converge");
+  return g.status;
+}
+
+int lfric_fgmres_solve(const lfric_linop *A, const lfric_prec *P, lfric_vec *x, const lfric_vec *b,
+                       const lfric_solver_opts *opts, lfric_solver_result *res) {
+  if (int rc = require_init()) return rc;
+  LF_CHECK(A && x && b && opts, LFRIC_E_ARG, "null argument");
+  LF_CHECK(x->fs == A->space && b->fs == A->space && x->precision == A->precision && b->precision == A->precision,
+           LFRIC_E_ARG, "vectors must match the operator space and precision");
+  LF_CHECK(x->d != b->d, LFRIC_E_ARG, "x and b must differ");
+  GmresOut g;
+  int rc = A->precision == LFRIC_R64
+               ? fgmres_t<double>(A, P, static_cast<double *>(x->d), static_cast<const double *>(b->d), x->n,
+                                  opts->gcrk, opts->r_tol, opts->a_tol, opts->max_iter, opts->monitor_convergence, g)
+               : fgmres_t<float>(A, P, static_cast<float *>(x->d), static_cast<const float *>(b->d), x->n,
+                                 opts->gcrk, opts->r_tol, opts->a_tol, opts->max_iter, opts->monitor_convergence, g);
+  if (rc) return rc;
+  if (res) {
+    res->status = g.status;
+    res->iterations = g.cycles * opts->gcrk;
+    res->outer_iterations = g.cycles;
+    res->r_nrm_0 = g.r0;
+    res->r_nrm = g.rn;
+    res->true_rel_res = g.r0 > 0 ? g.rn / g.r0 : 0.0;
+  }
+  if (g.status == LFRIC_NOT_CONVERGED && !opts->fail_on_non_converged) return LFRIC_OK;
+  if (g.status != LFRIC_OK) set_error(g.status == LFRIC_NAN ? "NaN in FGMRES residual" : "FGMRES did not converge");
   return g.status;
 }
 

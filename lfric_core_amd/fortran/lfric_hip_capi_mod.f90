@@ -22,6 +22,9 @@ module lfric_hip_capi_mod
     integer(c_int) :: check_every
     real(c_double) :: inner_r_tol
     integer(c_int) :: max_outer
+    real(c_double) :: rho_relax
+    real(c_double) :: lmin
+    real(c_double) :: lmax
   end type lfric_solver_opts
 
   type, bind(c), public :: lfric_solver_result

@@ -106,10 +106,14 @@ class SolverOptions:
     check_every: int = 8
     inner_r_tol: float = 1e-4
     max_outer: int = 20
+    rho_relax: float = 1.0
+    lmin: float = 0.124
+    lmax: float = 3.377
 
     def c(self):
         return SolverOpts(self.r_tol, self.a_tol, self.max_iter, self.gcrk, int(self.monitor_convergence),
-                          int(self.fail_on_non_converged), self.check_every, self.inner_r_tol, self.max_outer)
+                          int(self.fail_on_non_converged), self.check_every, self.inner_r_tol, self.max_outer,
+                          self.rho_relax, self.lmin, self.lmax)
 
 
 @dataclass
@@ -206,3 +210,88 @@ class MixedPrecisionGMRES(_Solver):
         rc = lib().lfric_ir_gmres_solve(self.lin_op._h, self.lin_op32._h, p, x._h, b._h, ctypes.byref(opts),
                                         ctypes.byref(res))
         return self._finish(rc, res)
+
+
+METHOD_CG, METHOD_BICGSTAB, METHOD_GMRES, METHOD_FGMRES, METHOD_GCR, METHOD_PREC_ONLY, METHOD_JACOBI, \
+    METHOD_CHEBYSHEV = range(1, 9)
+METHODS = {"cg": METHOD_CG, "bicgstab": METHOD_BICGSTAB, "gmres": METHOD_GMRES, "fgmres": METHOD_FGMRES,
+           "gcr": METHOD_GCR, "prec_only": METHOD_PREC_ONLY, "jacobi": METHOD_JACOBI, "chebyshev": METHOD_CHEBYSHEV}
+
+
+class _LinopSolver(_Solver):
+    _entry = None
+
+    def apply(self, x, b):
+        opts = self.opts.c()
+        res = SolverRes()
+        p = self.prec._h if self.prec is not None else None
+        rc = getattr(lib(), self._entry)(self.lin_op._h, p, x._h, b._h, ctypes.byref(opts), ctypes.byref(res))
+        return self._finish(rc, res)
+
+
+class BiCGStab(_LinopSolver):
+    """bicgstab_type(lin_op, prec, r_tol, a_tol, max_iter, monitor_convergence,
+    fail_on_non_converged) (sci_iterative_solver_mod.f90:123-156, 645-779)."""
+    _entry = "lfric_bicgstab_solve"
+
+
+class FGMRES(_LinopSolver):
+    """fgmres_type(lin_op, prec, gcrk, r_tol, a_tol, max_iter, ...) (sci_iterative_solver_mod.f90:
+    199-233, 1055-1237); the residual starts from b (zero initial guess), max_iter counts
+    restart cycles."""
+    _entry = "lfric_fgmres_solve"
+
+    def __init__(self, lin_op, prec, gcrk, r_tol, a_tol, max_iter, monitor_convergence=True,
+                 fail_on_non_converged=False, **kw):
+        super().__init__(lin_op, prec, r_tol, a_tol, max_iter, monitor_convergence, fail_on_non_converged,
+                         gcrk=gcrk, **kw)
+
+
+class GCR(FGMRES):
+    """gcr_type(lin_op, prec, gcrk, r_tol, a_tol, max_iter, ...) (sci_iterative_solver_mod.f90:
+    237-271, 1288-1408)."""
+    _entry = "lfric_gcr_solve"
+
+
+class Jacobi(_LinopSolver):
+    """jacobi_type(lin_op, prec, r_tol, a_tol, max_iter, monitor_convergence,
+    fail_on_non_converged, rho_relax) (sci_iterative_solver_mod.f90:350-387, jacobi_smod)."""
+    _entry = "lfric_jacobi_solve"
+
+    def __init__(self, lin_op, prec, r_tol, a_tol, max_iter, monitor_convergence=True,
+                 fail_on_non_converged=False, rho_relax=1.0, **kw):
+        super().__init__(lin_op, prec, r_tol, a_tol, max_iter, monitor_convergence, fail_on_non_converged,
+                         rho_relax=rho_relax, **kw)
+
+
+class Chebyshev(_LinopSolver):
+    """chebyshev_type(lin_op, prec, r_tol, a_tol, max_iter, monitor_convergence,
+    fail_on_non_converged, lmin, lmax) (sci_iterative_solver_mod.f90:391-428, chebyshev_smod)."""
+    _entry = "lfric_chebyshev_solve"
+
+    def __init__(self, lin_op, prec, r_tol, a_tol, max_iter, monitor_convergence=True,
+                 fail_on_non_converged=False, lmin=0.124, lmax=3.377, **kw):
+        super().__init__(lin_op, prec, r_tol, a_tol, max_iter, monitor_convergence, fail_on_non_converged,
+                         lmin=lmin, lmax=lmax, **kw)
+
+
+class PreconditionOnly(_LinopSolver):
+    """precondition_only_type(lin_op, prec, monitor_convergence) (sci_iterative_solver_mod.f90:
+    315-346, precondition_only_smod): x = P^-1 b."""
+    _entry = "lfric_prec_only_solve"
+
+    def __init__(self, lin_op, prec, monitor_convergence=True):
+        super().__init__(lin_op, prec, 0.0, 0.0, 1, monitor_convergence)
+
+
+def solve(method, lin_op, prec, x, b, **opts):
+    """The solver factory of mass_matrix_solver_alg (sci_mass_matrix_solver_alg_mod.x90:164-181):
+    method is one of METHODS' names; opts are SolverOptions fields."""
+    o = SolverOptions(**opts).c()
+    res = SolverRes()
+    p = prec._h if prec is not None else None
+    rc = lib().lfric_solve(METHODS[method], lin_op._h, p, x._h, b._h, ctypes.byref(o), ctypes.byref(res))
+    out = SolverResult(res.status, res.iterations, res.outer_iterations, res.r_nrm_0, res.r_nrm,
+                       res.true_rel_res)
+    check(rc, allow=(LFRIC_OK,) if opts.get("fail_on_non_converged") else (LFRIC_OK, LFRIC_NOT_CONVERGED))
+    return out
