@@ -136,6 +136,14 @@ int lfric_lma_apply(const lfric_op *op, const lfric_vec *x, lfric_vec *y);
  * its column, which adds the contributions of the column's cells in colour order, top-partner
  * row of layer k-1 before bottom row of layer k (the order of the reference's coloured loop). */
 int lfric_lma_apply_gather(const lfric_op *op, const lfric_vec *x, lfric_vec *y);
+/* Several right-hand sides at once: x and y hold nrhs (1, 2, 4 or 8) values per dof, RHS
+ * fastest (entry (dof, r) at dof*nrhs + r).  Each element matrix is read once for all
+ * right-hand sides; column r of y equals the one-RHS apply of column r bit for bit for the
+ * same variant.  No reference counterpart (the reference applies one field at a time). */
+#define LFRIC_MRHS_AUTO 0   /* the library's choice */
+#define LFRIC_MRHS_GATHER 1 /* one launch, gather form */
+#define LFRIC_MRHS_COLOUR 2 /* colour-ordered scatter, one launch per colour */
+int lfric_lma_apply_mrhs(const lfric_op *op, const lfric_vec *x, lfric_vec *y, int variant);
 /* transpose_matrix_vector_kernel (components/science/source/kernel/algebra/
  * transpose_matrix_vector_kernel_mod.F90): y = A^T x, A maps `from` -> `to`, y on `from`. */
 int lfric_lma_apply_transpose(const lfric_op *op, const lfric_vec *x, lfric_vec *y);

@@ -647,6 +647,18 @@ int lfric_lma_apply(const lfric_op *op, const lfric_vec *x, lfric_vec *y) {
   return launch_lma_apply(op, x->d, y->d, nullptr);
 }
 
+int lfric_lma_apply_mrhs(const lfric_op *op, const lfric_vec *x, lfric_vec *y, int variant) {
+  if (int rc = require_init()) return rc;
+  LF_CHECK(op && x && y, LFRIC_E_ARG, "null argument");
+  LF_CHECK(x->fs->undf == op->from->undf && y->fs->undf == op->to->undf, LFRIC_E_ARG,
+           "vector does not match operator space");
+  LF_CHECK(x->precision == op->precision && y->precision == op->precision, LFRIC_E_ARG,
+           "vector precision differs from operator precision");
+  LF_CHECK(x->nrhs == y->nrhs, LFRIC_E_ARG, "x and y hold different numbers of right-hand sides");
+  LF_CHECK(x->d != y->d, LFRIC_E_ARG, "x and y must differ");
+  return launch_lma_apply_mrhs(op, x->d, y->d, x->nrhs, variant);
+}
+
 int lfric_lma_apply_gather(const lfric_op *op, const lfric_vec *x, lfric_vec *y) {
   if (int rc = require_init()) return rc;
   if (int rc = check_apply(op, x, y, false)) return rc;

@@ -35,6 +35,7 @@ int launch_op_convert(const lfric_op *src, lfric_op *dst);
 // operator kernels
 int launch_lma_apply(const lfric_op *op, const void *x, void *y, const int *d_skip);
 int launch_lma_apply_gather(const lfric_op *op, const void *x, void *y);
+int launch_lma_apply_mrhs(const lfric_op *op, const void *x, void *y, int nrhs, int variant);
 int launch_lma_transpose(const lfric_op *op, const void *x, void *y);
 int launch_dg_apply(const lfric_op *op, const void *x, void *y);
 int launch_mm_diagonal(const lfric_op *op, void *diag);

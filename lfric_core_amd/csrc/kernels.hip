@@ -372,6 +372,119 @@ __global__ __launch_bounds__(kBlock) void k_lma_gather(const T *__restrict__ A, 
   y[static_cast<size_t>(ci.x) + m] = acc;
 }
 
+// ---- several right-hand sides ------------------------------------------------------------
+// Multi-RHS vectors hold NR values per dof, RHS fastest: (dof, r) at dof*NR + r.  Every RHS
+// is accumulated exactly as the one-RHS kernels do (same fma order), so column r of a
+// multi-RHS apply equals the one-RHS apply of column r bit for bit.
+template <typename T, int NR>
+struct RVec {
+  typedef T type __attribute__((ext_vector_type(NR)));
+};
+
+template <typename T, int N2, int NR>
+__device__ inline void row_dot_mrhs(const T *__restrict__ A, size_t gs, long long e, int r, const T *__restrict__ x,
+                                    const int (&b)[N2], int lev, typename RVec<T, NR>::type &acc) {
+  typedef typename RVec<T, NR>::type VR;
+  constexpr int V = chunk_entries<T>();
+  auto xrow = [&](int j) { return *reinterpret_cast<const VR *>(x + (static_cast<size_t>(b[j]) + lev) * NR); };
+  if constexpr (N2 % V == 0) {
+    typedef T VecT __attribute__((ext_vector_type(V)));
+    const T *Ae = A + static_cast<size_t>(e >> 6) * gs + static_cast<size_t>(e & 63) * V;
+    const int ch0 = r * (N2 / V);
+#pragma unroll
+    for (int c = 0; c < N2 / V; ++c) {
+      const VecT v = *reinterpret_cast<const VecT *>(Ae + static_cast<size_t>(ch0 + c) * 64 * V);
+#pragma unroll
+      for (int q = 0; q < V; ++q) {
+        const VR xv = xrow(c * V + q);
+#pragma unroll
+        for (int k = 0; k < NR; ++k) acc[k] = fma(v[q], xv[k], acc[k]);
+      }
+    }
+  } else {
+#pragma unroll
+    for (int j = 0; j < N2; ++j) {
+      const T a = A[aidx<T>(gs, e, r * N2 + j)];
+      const VR xv = xrow(j);
+#pragma unroll
+      for (int k = 0; k < NR; ++k) acc[k] = fma(a, xv[k], acc[k]);
+    }
+  }
+}
+
+// gather form, NR right-hand sides per thread (see k_lma_gather)
+template <typename T, int N2, int NR, class R>
+__global__ __launch_bounds__(kBlock) void k_lma_gather_mrhs(const T *__restrict__ A, size_t gs,
+                                                            const int *__restrict__ ebase, const T *__restrict__ x,
+                                                            T *__restrict__ y, const int *__restrict__ map2,
+                                                            const int2 *__restrict__ cols, const int *__restrict__ off,
+                                                            const int *__restrict__ ent, int ncols, int nl, R roles) {
+  typedef typename RVec<T, NR>::type VR;
+  const int H = nl + 1;
+  const long long t = static_cast<long long>(blockIdx.x) * kBlock + threadIdx.x;
+  if (t >= static_cast<long long>(ncols) * H) return;
+  const int col = static_cast<int>(t / H);
+  const int m = static_cast<int>(t - static_cast<long long>(col) * H);
+  const int2 ci = cols[col];
+  if (m == nl && !ci.y) return;
+  const bool has_m = m < nl, has_p = m > 0;
+  VR acc = {};
+  const int q1 = off[col + 1];
+  for (int q = off[col]; q < q1; ++q) {
+    const int en = ent[q];
+    const int pos = en >> 5, i = en & 31;
+    const int pt = roles.partner(i);
+    const long long e = static_cast<long long>(ebase[pos]) + m;
+    int b[N2];
+#pragma unroll
+    for (int j = 0; j < N2; ++j) b[j] = map2[static_cast<size_t>(pos) * N2 + j];
+    if (pt >= 0 && has_p) row_dot_mrhs<T, N2, NR>(A, gs, e, pt, x, b, m - 1, acc);
+    if (has_m) row_dot_mrhs<T, N2, NR>(A, gs, e, i, x, b, m, acc);
+  }
+  *reinterpret_cast<VR *>(y + (static_cast<size_t>(ci.x) + m) * NR) = acc;
+}
+
+// colour-ordered scatter form, NR right-hand sides per thread (see k_lma_colour): rows of
+// element e accumulate in c[i], then out = c[i] + c[partner(i)] is written or added
+template <typename T, int N1, int N2, int NR, class R>
+__global__ __launch_bounds__(kBlock) void k_lma_colour_mrhs(const T *__restrict__ A, size_t gs,
+                                                            const int *__restrict__ ebase, const T *__restrict__ x,
+                                                            T *__restrict__ y, const int *__restrict__ map1,
+                                                            const int *__restrict__ map2,
+                                                            const uint32_t *__restrict__ first, int pos0, int npos,
+                                                            int nl, R roles) {
+  typedef typename RVec<T, NR>::type VR;
+  const int H = nl + 1;
+  const int t = blockIdx.x * kBlock + threadIdx.x;
+  if (t >= npos * H) return;
+  const int lp = t / H;
+  const int m = t - lp * H;
+  const int pos = pos0 + lp;
+  const bool has_m = m < nl, has_p = m > 0;
+  const long long e = static_cast<long long>(ebase[pos]) + m;
+  int b[N2];
+#pragma unroll
+  for (int j = 0; j < N2; ++j) b[j] = map2[static_cast<size_t>(pos) * N2 + j];
+  const uint32_t ft = first[pos];
+#pragma unroll
+  for (int i = 0; i < N1; ++i) {
+    if (roles.top(i)) continue;
+    const int pt = roles.partner(i);
+    if (!has_m && pt < 0) continue;
+    VR c = {}, cp = {};
+    if (has_m) row_dot_mrhs<T, N2, NR>(A, gs, e, i, x, b, m, c);
+    if (pt >= 0 && has_p) row_dot_mrhs<T, N2, NR>(A, gs, e, pt, x, b, m - 1, cp);
+    VR out;
+#pragma unroll
+    for (int k = 0; k < NR; ++k) out[k] = pt >= 0 ? c[k] + cp[k] : c[k];
+    VR *yd = reinterpret_cast<VR *>(y + (static_cast<size_t>(map1[static_cast<size_t>(pos) * N1 + i]) + m) * NR);
+    if ((ft >> i) & 1u)
+      *yd = out;
+    else
+      *yd = *yd + out;
+  }
+}
+
 // y = A^T x: thread (cell, level m) of the `from` space; entry (to_df i, from_df j) of layer k
 // sits at element ebase + k + top_to(i)
 template <typename T, int NT, int NF>
@@ -1017,6 +1130,89 @@ static int gather_t(const lfric_op *op, const T *x, T *y) {
   }
   CHECK_LAUNCH();
   return LFRIC_OK;
+}
+
+// ---- multi-RHS dispatch ---------------------------------------------------------------------
+template <typename T, int N2, int NR, class R>
+static void launch_gather_mrhs_r(const lfric_op *op, const T *x, T *y, R roles) {
+  const FunctionSpace &to = *op->to, &from = *op->from;
+  const int grid = grid_for(static_cast<long long>(to.gather_ncols) * (to.nlayers + 1));
+  LAUNCH((k_lma_gather_mrhs<T, N2, NR, R>), grid, static_cast<const T *>(op->d), op->gs, to.d_ebase, x, y,
+         from.d_map, to.d_gcol, to.d_goff, to.d_gent, to.gather_ncols, to.nlayers, roles);
+}
+
+template <typename T, int N1, int N2, int NR, class R>
+static void launch_colour_mrhs_r(const lfric_op *op, const T *x, T *y, R roles) {
+  const FunctionSpace &to = *op->to, &from = *op->from;
+  for (int c = 0; c < to.ncolours; ++c) {
+    const int pos0 = to.colour_start[c], npos = to.colour_start[c + 1] - pos0;
+    if (npos == 0) continue;
+    LAUNCH((k_lma_colour_mrhs<T, N1, N2, NR, R>), grid_for(static_cast<long long>(npos) * (to.nlayers + 1)),
+           static_cast<const T *>(op->d), op->gs, to.d_ebase, x, y, to.d_map, from.d_map, to.d_first, pos0, npos,
+           to.nlayers, roles);
+  }
+}
+
+template <typename T, int N1, int N2, int NR>
+static void launch_mrhs_n(int variant, RoleKind rk, const lfric_op *op, const T *x, T *y) {
+  const Roles rr = roles_of(*op->to);
+  if (variant == LFRIC_MRHS_COLOUR) {
+    switch (rk) {
+      case RoleKind::W0K0: launch_colour_mrhs_r<T, N1, N2, NR>(op, x, y, RolesW0K0{}); break;
+      case RoleKind::W0K1: launch_colour_mrhs_r<T, N1, N2, NR>(op, x, y, RolesW0K1{}); break;
+      case RoleKind::W2K0: launch_colour_mrhs_r<T, N1, N2, NR>(op, x, y, RolesW2K0{}); break;
+      case RoleKind::None: launch_colour_mrhs_r<T, N1, N2, NR>(op, x, y, RolesNone{}); break;
+      default: launch_colour_mrhs_r<T, N1, N2, NR>(op, x, y, RolesRuntime{rr});
+    }
+    return;
+  }
+  switch (rk) {
+    case RoleKind::W0K0: launch_gather_mrhs_r<T, N2, NR>(op, x, y, RolesW0K0{}); break;
+    case RoleKind::W0K1: launch_gather_mrhs_r<T, N2, NR>(op, x, y, RolesW0K1{}); break;
+    case RoleKind::W2K0: launch_gather_mrhs_r<T, N2, NR>(op, x, y, RolesW2K0{}); break;
+    case RoleKind::None: launch_gather_mrhs_r<T, N2, NR>(op, x, y, RolesNone{}); break;
+    default: launch_gather_mrhs_r<T, N2, NR>(op, x, y, RolesRuntime{rr});
+  }
+}
+
+template <typename T, int NR>
+static int mrhs_nr(int variant, const lfric_op *op, const T *x, T *y) {
+  const RoleKind rk = role_kind(*op->to);
+  const int n1 = op->to->ndf, n2 = op->from->ndf;
+#define MRHS_CASE(A, B)                                      \
+  if (n1 == A && n2 == B) {                                  \
+    launch_mrhs_n<T, A, B, NR>(variant, rk, op, x, y);       \
+    CHECK_LAUNCH();                                          \
+    return LFRIC_OK;                                         \
+  }
+  MRHS_CASE(8, 8)
+  MRHS_CASE(27, 27)
+  MRHS_CASE(6, 6)
+  MRHS_CASE(1, 1)
+#undef MRHS_CASE
+  return fail(LFRIC_E_UNSUPPORTED, "lma_apply_mrhs: unsupported (ndf_to, ndf_from)");
+}
+
+template <typename T>
+static int mrhs_t(int variant, int nrhs, const lfric_op *op, const T *x, T *y) {
+  if (variant == LFRIC_MRHS_GATHER)
+    if (int rc = build_gather(*op->to)) return rc;
+  switch (nrhs) {
+    case 2: return mrhs_nr<T, 2>(variant, op, x, y);
+    case 4: return mrhs_nr<T, 4>(variant, op, x, y);
+    case 8: return mrhs_nr<T, 8>(variant, op, x, y);
+  }
+  return fail(LFRIC_E_UNSUPPORTED, "lma_apply_mrhs: nrhs must be 1, 2, 4 or 8");
+}
+
+int launch_lma_apply_mrhs(const lfric_op *op, const void *x, void *y, int nrhs, int variant) {
+  if (variant == LFRIC_MRHS_AUTO) variant = LFRIC_MRHS_GATHER;
+  LF_CHECK(variant == LFRIC_MRHS_GATHER || variant == LFRIC_MRHS_COLOUR, LFRIC_E_ARG, "unknown multi-RHS variant");
+  if (nrhs == 1)
+    return variant == LFRIC_MRHS_GATHER ? launch_lma_apply_gather(op, x, y) : launch_lma_apply(op, x, y, nullptr);
+  if (op->precision == LFRIC_R64)
+    return mrhs_t<double>(variant, nrhs, op, static_cast<const double *>(x), static_cast<double *>(y));
+  return mrhs_t<float>(variant, nrhs, op, static_cast<const float *>(x), static_cast<float *>(y));
 }
 
 int launch_lma_apply_gather(const lfric_op *op, const void *x, void *y) {

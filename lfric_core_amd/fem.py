@@ -202,6 +202,12 @@ class LmaOperator:
         """y = A x (abstract_linear_operator_type%apply)."""
         check(lib().lfric_lma_apply(self._h, x._h, y._h))
 
+    def apply_mrhs(self, x, y, variant=0):
+        """y = A x for every right-hand side of x (x.nrhs == y.nrhs); variant 0 auto,
+        1 gather form, 2 colour-ordered scatter."""
+        check(lib().lfric_lma_apply_mrhs(self._h, x._h, y._h, int(variant)))
+        return y
+
     def apply_gather(self, x, y):
         check(lib().lfric_lma_apply_gather(self._h, x._h, y._h))
 

@@ -100,6 +100,34 @@ def test_lma_apply_fp32_parity(hip, orc, order, method):
     assert np.all(np.abs(y - yo) <= _bound(orc, m, A, x, 2e-6))
 
 
+@pytest.mark.parametrize("prec", [0, 1])
+@pytest.mark.parametrize("n,nl,order,nrhs", [(4, 3, 0, 4), (12, 7, 0, 2), (4, 3, 1, 4), (5, 4, 0, 8)])
+def test_lma_apply_multi_rhs(hip, orc, n, nl, order, nrhs, prec):
+    """Column r of the multi-RHS apply equals the one-RHS apply of column r bit for bit (same
+    variant), and the oracle within the FP64 / FP32 bars."""
+    fs, m = _mesh_pair(hip, orc, n, nl, 0, order)
+    op64 = hip.LmaOperator(fs).generate_spd(2, scaled=True)
+    op = op64 if prec == 0 else op64.convert(hip.R32)
+    P = hip.R64 if prec == 0 else hip.R32
+    X = np.random.default_rng(nrhs).uniform(-1, 1, (m.undf, nrhs))
+    if prec:
+        X = X.astype(np.float32).astype(np.float64)
+    xv = hip.FieldVector(fs, P, nrhs).import_field(X.ravel())
+    yv = hip.FieldVector(fs, P, nrhs)
+    A = orc.gen_spd(m.ndf, m.ncell_3d, 2, scaled=True)
+    for variant, single in ((1, "apply_gather"), (2, "apply")):
+        yv.set_scalar(3.0)
+        op.apply_mrhs(xv, yv, variant)
+        Y = yv.export_field().reshape(m.undf, nrhs)
+        for r in range(nrhs):
+            x1 = hip.FieldVector(fs, P).import_field(X[:, r])
+            y1 = hip.FieldVector(fs, P)
+            getattr(op, single)(x1, y1)
+            np.testing.assert_array_equal(Y[:, r], y1.export_field())
+            yo = orc.lma_apply(m, A, X[:, r])
+            assert np.all(np.abs(Y[:, r] - yo) <= _bound(orc, m, A, X[:, r], 1e-13 if prec == 0 else 2e-6))
+
+
 def test_lma_apply_known_answer(hip):
     k = json.loads((GOLDEN / "kat.json").read_text())["matrix_vector"]
     colours = dict(ncells_per_colour=[1], cells_in_colour=[[1]])
