@@ -280,6 +280,37 @@ int lfric_prec_only_solve(const lfric_linop *A, const lfric_prec *P, lfric_vec *
 int lfric_solve(int method, const lfric_linop *A, const lfric_prec *P, lfric_vec *x, const lfric_vec *b,
                 const lfric_solver_opts *opts, lfric_solver_result *res);
 
+/* ---- batched solves (no reference counterpart: many independent mass-matrix solves at
+ *      once) ------------------------------------------------------------------------------
+ * Each problem is A x = b with FP64 and FP32 copies of one square LMA operator (ndf 8 or 27),
+ * a Jacobi diagonal (FP32) and x, b with 1, 2, 4 or 8 right-hand sides.  Every (problem,
+ * RHS) runs FP32 CG inside FP64 iterative refinement (as lfric_ir_cg_solve) with the
+ * options given at creation; all problems advance together, one persistent apply kernel and
+ * a handful of vector kernels per inner iteration, replayed from a HIP graph.  Problems may
+ * be added between runs (continuous batching) and removed once finished.  x is read as the
+ * initial guess and holds the solution; b, the operators and the diagonal must stay alive
+ * until the problem is removed.  A batch uses the library's stream; calls on one batch must
+ * not overlap. */
+typedef struct lfric_batch lfric_batch;
+int lfric_batch_create(const lfric_solver_opts *opts, lfric_batch **b);
+int lfric_batch_add(lfric_batch *b, const lfric_op *A64, const lfric_op *A32, const lfric_vec *diag32,
+                    lfric_vec *x, const lfric_vec *rhs, int *id);
+/* Advance by up to `rounds` rounds (check_every inner iterations each, then the refinement
+ * steps due); *nactive = problems not yet finished. */
+int lfric_batch_run(lfric_batch *b, int rounds, int *nactive);
+int lfric_batch_solve(lfric_batch *b); /* run until every problem has finished */
+int lfric_batch_finished(const lfric_batch *b, int id, int *finished);
+/* One result per RHS (res[nrhs]): status, inner iterations, refinement steps, FP64 norms. */
+int lfric_batch_result(const lfric_batch *b, int id, lfric_solver_result *res);
+int lfric_batch_remove(lfric_batch *b, int id);
+/* Autotuner: time the persistent apply of the active problems for each block and grid size
+ * and keep the fastest (*best_ms: its time per apply). */
+int lfric_batch_tune(lfric_batch *b, double *best_ms);
+/* use_graph 0/1 (-1 keeps), block 64/128/256 (0 keeps), grid_mult > 0 (0 keeps). */
+int lfric_batch_config(lfric_batch *b, int use_graph, int block, int grid_mult);
+int lfric_batch_stats(const lfric_batch *b, long long *iterations, int *block, int *grid_mult, int *ntiles);
+void lfric_batch_destroy(lfric_batch *b);
+
 #ifdef __cplusplus
 }
 #endif

@@ -114,6 +114,18 @@ SIGNATURES = {
     "lfric_chebyshev_solve": (_I, [_VP, _VP, _VP, _VP, ctypes.POINTER(SolverOpts), ctypes.POINTER(SolverRes)]),
     "lfric_prec_only_solve": (_I, [_VP, _VP, _VP, _VP, ctypes.POINTER(SolverOpts), ctypes.POINTER(SolverRes)]),
     "lfric_solve": (_I, [_I, _VP, _VP, _VP, _VP, ctypes.POINTER(SolverOpts), ctypes.POINTER(SolverRes)]),
+    "lfric_batch_create": (_I, [ctypes.POINTER(SolverOpts), ctypes.POINTER(_VP)]),
+    "lfric_batch_add": (_I, [_VP, _VP, _VP, _VP, _VP, _VP, ctypes.POINTER(_I)]),
+    "lfric_batch_run": (_I, [_VP, _I, ctypes.POINTER(_I)]),
+    "lfric_batch_solve": (_I, [_VP]),
+    "lfric_batch_finished": (_I, [_VP, _I, ctypes.POINTER(_I)]),
+    "lfric_batch_result": (_I, [_VP, _I, ctypes.POINTER(SolverRes)]),
+    "lfric_batch_remove": (_I, [_VP, _I]),
+    "lfric_batch_tune": (_I, [_VP, _DP]),
+    "lfric_batch_config": (_I, [_VP, _I, _I, _I]),
+    "lfric_batch_stats": (_I, [_VP, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(_I), ctypes.POINTER(_I),
+                               ctypes.POINTER(_I)]),
+    "lfric_batch_destroy": (None, [_VP]),
 }
 
 _lib = None

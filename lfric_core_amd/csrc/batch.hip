@@ -1,0 +1,897 @@
+// Batched solves of many independent LMA systems, each with several right-hand sides:
+// FP32 (r_solver) CG inside FP64 iterative refinement for every (problem, RHS) at once.
+//
+// One inner CG iteration of the whole batch is a fixed chain of kernels whatever the number
+// of problems: a persistent apply kernel that takes (problem, column block) tiles from a
+// work counter (gather form of the LMA apply, so tiles of one problem have no colour order
+// between them), then tile-flat vector kernels whose partial sums are finished per problem
+// and RHS in tile order (deterministic).  The chain is captured once per active set in a HIP
+// graph and replayed; the host looks at the per-RHS flags every round (check_every
+// replays), runs the FP64 refinement step of the problems whose inner solve has converged,
+// retires finished problems and admits new ones (continuous batching).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <numeric>
+#include <vector>
+
+#include "device_common.h"
+#include "internal.h"
+#include "kernels.h"
+
+namespace lfric {
+
+namespace {
+
+constexpr int kMaxR = 8;     // right-hand sides per problem (1, 2, 4 or 8)
+constexpr int kVecTile = 4096;  // flat entries per vector tile
+constexpr int kVecBlock = 256;
+
+// device descriptor of one active problem
+struct ProbDev {
+  const float *A;
+  unsigned long long gs;
+  const int *ebase, *map2, *goff, *gent;
+  const int2 *gcol;
+  const float *diag;
+  float *d, *r, *z, *p, *q;
+  long long n;      // undf * nrhs
+  int ncols, nl, ndf, nrhs;
+  int apply_tile0, vec_tile0, vec_ntiles, scal0;
+  int8_t partner[32];
+};
+
+// per (problem, RHS) inner CG state
+struct RhsDev {
+  double rz, alpha, beta, r0, rn, tol;
+  int done, iters;
+};
+
+__device__ inline int find_problem(const int *tile0, int nprob, int tile) {
+  int lo = 0, hi = nprob;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (tile0[mid] <= tile)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+// thread (column, level) of one problem, NR right-hand sides (see k_lma_gather_mrhs)
+template <int N2, int NR>
+__device__ inline void gather_point(const ProbDev &d, long long t) {
+  const int H = d.nl + 1;
+  if (t >= static_cast<long long>(d.ncols) * H) return;
+  const int col = static_cast<int>(t / H);
+  const int m = static_cast<int>(t - static_cast<long long>(col) * H);
+  const int2 ci = d.gcol[col];
+  if (m == d.nl && !ci.y) return;
+  const bool has_m = m < d.nl, has_p = m > 0;
+  typedef float VR __attribute__((ext_vector_type(NR)));
+  typedef float V4 __attribute__((ext_vector_type(4)));
+  VR acc = {};
+  auto row = [&](long long e, int r, const int (&b)[N2], int lev) {
+    const float *Ae = d.A + static_cast<size_t>(e >> 6) * d.gs + static_cast<size_t>(e & 63) * 4;
+    if constexpr (N2 % 4 == 0) {
+#pragma unroll
+      for (int c = 0; c < N2 / 4; ++c) {
+        const V4 v = *reinterpret_cast<const V4 *>(Ae + static_cast<size_t>(r * (N2 / 4) + c) * 256);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const VR xv = *reinterpret_cast<const VR *>(d.p + (static_cast<size_t>(b[c * 4 + q]) + lev) * NR);
+#pragma unroll
+          for (int k = 0; k < NR; ++k) acc[k] = fma(v[q], xv[k], acc[k]);
+        }
+      }
+    } else {
+#pragma unroll
+      for (int j = 0; j < N2; ++j) {
+        const int p = r * N2 + j;
+        const float a = Ae[static_cast<size_t>(p >> 2) * 256 + (p & 3)];
+        const VR xv = *reinterpret_cast<const VR *>(d.p + (static_cast<size_t>(b[j]) + lev) * NR);
+#pragma unroll
+        for (int k = 0; k < NR; ++k) acc[k] = fma(a, xv[k], acc[k]);
+      }
+    }
+  };
+  const int q1 = d.goff[col + 1];
+  for (int q = d.goff[col]; q < q1; ++q) {
+    const int en = d.gent[q];
+    const int pos = en >> 5, i = en & 31;
+    const int pt = d.partner[i];
+    const long long e = static_cast<long long>(d.ebase[pos]) + m;
+    int b[N2];
+#pragma unroll
+    for (int j = 0; j < N2; ++j) b[j] = d.map2[static_cast<size_t>(pos) * N2 + j];
+    if (pt >= 0 && has_p) row(e, pt, b, m - 1);
+    if (has_m) row(e, i, b, m);
+  }
+  *reinterpret_cast<VR *>(d.q + (static_cast<size_t>(ci.x) + m) * NR) = acc;
+}
+
+template <int N2>
+__device__ inline void gather_nr(const ProbDev &d, long long t) {
+  switch (d.nrhs) {
+    case 1: gather_point<N2, 1>(d, t); break;
+    case 2: gather_point<N2, 2>(d, t); break;
+    case 4: gather_point<N2, 4>(d, t); break;
+    default: gather_point<N2, 8>(d, t); break;
+  }
+}
+
+// q = A32 p for the active problems with N2 dofs per cell, whose tiles are [t_begin, t_end);
+// persistent: blocks take tiles from *counter until none is left, so every wavefront exits
+template <int BS, int N2>
+__global__ __launch_bounds__(BS) void k_batch_apply(const ProbDev *__restrict__ P, const int *__restrict__ tile0,
+                                                    int nprob, int t_begin, int t_end, unsigned int *counter) {
+  __shared__ int s_tile;
+  for (;;) {
+    if (threadIdx.x == 0) s_tile = t_begin + static_cast<int>(atomicAdd(counter, 1u));
+    __syncthreads();
+    const int tile = s_tile;
+    __syncthreads();
+    if (tile >= t_end) return;
+    const ProbDev &d = P[find_problem(tile0, nprob, tile)];
+    gather_nr<N2>(d, static_cast<long long>(tile - d.apply_tile0) * BS + threadIdx.x);
+  }
+}
+
+__global__ void k_zero_counter(unsigned int *counter) {
+  if (threadIdx.x < 2 && blockIdx.x == 0) counter[threadIdx.x] = 0u;
+}
+
+// per vector tile and RHS: sums of a.b (and c.d) over the tile's entries; thread t of the
+// block always holds RHS t % nrhs, and the per-RHS sums are added in thread order
+template <int NV>
+__device__ inline void tile_sums(double (&v)[NV], int nrhs, double *out /* [NV][kMaxR] */) {
+  __shared__ double sh[NV][kVecBlock];
+#pragma unroll
+  for (int k = 0; k < NV; ++k) sh[k][threadIdx.x] = v[k];
+  __syncthreads();
+  if (threadIdx.x < nrhs) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+      double s = 0.0;
+      for (int u = threadIdx.x; u < kVecBlock; u += nrhs) s += sh[k][u];
+      out[k * kMaxR + threadIdx.x] = s;
+    }
+  }
+}
+
+__device__ inline void vec_tile(const ProbDev *P, const int *vtile0, int nprob, const ProbDev *&d, long long &f0,
+                                long long &f1) {
+  const int tile = blockIdx.x;
+  d = &P[find_problem(vtile0, nprob, tile)];
+  f0 = static_cast<long long>(tile - d->vec_tile0) * kVecTile;
+  f1 = std::min<long long>(f0 + kVecTile, d->n);
+}
+
+// partials of p.q
+__global__ __launch_bounds__(kVecBlock) void k_batch_pq(const ProbDev *__restrict__ P, const int *__restrict__ vtile0,
+                                                        int nprob, double *__restrict__ partials) {
+  const ProbDev *d;
+  long long f0, f1;
+  vec_tile(P, vtile0, nprob, d, f0, f1);
+  double v[1] = {0.0};
+  for (long long f = f0 + threadIdx.x; f < f1; f += kVecBlock) v[0] += static_cast<double>(d->p[f]) * d->q[f];
+  tile_sums<1>(v, d->nrhs, partials + static_cast<size_t>(blockIdx.x) * 2 * kMaxR);
+}
+
+// alpha = rz / p.q per (problem, RHS), from the tile partials in order
+__global__ void k_batch_alpha(const ProbDev *__restrict__ P, int nprob, const double *__restrict__ partials,
+                              RhsDev *__restrict__ S) {
+  const int pi = blockIdx.x, r = threadIdx.x;
+  if (pi >= nprob) return;
+  const ProbDev &d = P[pi];
+  if (r >= d.nrhs) return;
+  RhsDev &s = S[d.scal0 + r];
+  if (s.done) return;
+  double pq = 0.0;
+  for (int t = 0; t < d.vec_ntiles; ++t) pq += partials[static_cast<size_t>(d.vec_tile0 + t) * 2 * kMaxR + r];
+  if (pq != pq || pq == 0.0) {
+    s.done = 2;  // NaN or breakdown: stop this RHS (the host reports it)
+    return;
+  }
+  s.alpha = s.rz / pq;
+}
+
+// d += alpha p; r -= alpha q; z = r / diag; partials of r.r and r.z
+__global__ __launch_bounds__(kVecBlock) void k_batch_update(const ProbDev *__restrict__ P,
+                                                            const int *__restrict__ vtile0, int nprob,
+                                                            const RhsDev *__restrict__ S, double *__restrict__ partials) {
+  const ProbDev *d;
+  long long f0, f1;
+  vec_tile(P, vtile0, nprob, d, f0, f1);
+  const int nr = d->nrhs;
+  const int r = threadIdx.x % nr;
+  const RhsDev &s = S[d->scal0 + r];
+  const bool live = !s.done;
+  const float a = static_cast<float>(s.alpha), ma = static_cast<float>(-s.alpha);
+  double v[2] = {0.0, 0.0};
+  for (long long f = f0 + threadIdx.x; f < f1; f += kVecBlock) {
+    float ri = d->r[f];
+    if (live) {
+      d->d[f] = d->d[f] + a * d->p[f];
+      ri = ri + ma * d->q[f];
+      d->r[f] = ri;
+      const float zi = ri / d->diag[f / nr];
+      d->z[f] = zi;
+      v[0] += static_cast<double>(ri) * ri;
+      v[1] += static_cast<double>(ri) * zi;
+    }
+  }
+  tile_sums<2>(v, nr, partials + static_cast<size_t>(blockIdx.x) * 2 * kMaxR);
+}
+
+// ||r||, convergence test and beta per (problem, RHS)
+__global__ void k_batch_beta(const ProbDev *__restrict__ P, int nprob, const double *__restrict__ partials,
+                             RhsDev *__restrict__ S, double a_tol, int max_iter) {
+  const int pi = blockIdx.x, r = threadIdx.x;
+  if (pi >= nprob) return;
+  const ProbDev &d = P[pi];
+  if (r >= d.nrhs) return;
+  RhsDev &s = S[d.scal0 + r];
+  if (s.done) return;
+  double rr = 0.0, rz = 0.0;
+  for (int t = 0; t < d.vec_ntiles; ++t) {
+    rr += partials[static_cast<size_t>(d.vec_tile0 + t) * 2 * kMaxR + r];
+    rz += partials[static_cast<size_t>(d.vec_tile0 + t) * 2 * kMaxR + kMaxR + r];
+  }
+  s.iters += 1;
+  const double rn = sqrt(rr);
+  s.rn = rn;
+  if (rn != rn) {
+    s.done = 2;
+    return;
+  }
+  if (rn / s.r0 <= s.tol || rn <= a_tol || s.iters >= max_iter) {
+    s.done = 1;
+    return;
+  }
+  s.beta = rz / s.rz;
+  s.rz = rz;
+}
+
+// p = z + beta p for the live RHS
+__global__ __launch_bounds__(kVecBlock) void k_batch_aypx(const ProbDev *__restrict__ P, const int *__restrict__ vtile0,
+                                                          int nprob, const RhsDev *__restrict__ S) {
+  const ProbDev *d;
+  long long f0, f1;
+  vec_tile(P, vtile0, nprob, d, f0, f1);
+  const int r = threadIdx.x % d->nrhs;
+  const RhsDev &s = S[d->scal0 + r];
+  if (s.done) return;
+  const float b = static_cast<float>(s.beta);
+  for (long long f = f0 + threadIdx.x; f < f1; f += kVecBlock) d->p[f] = b * d->p[f] + d->z[f];
+}
+
+// ---- refinement step kernels (one problem at a time) ---------------------------------------
+// x64 += scale[r] * d32 (the scaled inner solution)
+__global__ void k_ir_accumulate(double *__restrict__ x, const float *__restrict__ d, const double *__restrict__ scale,
+                                long long n, int nr) {
+  for (long long f = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x; f < n;
+       f += static_cast<long long>(gridDim.x) * blockDim.x)
+    x[f] = x[f] + scale[f % nr] * static_cast<double>(d[f]);
+}
+
+// r = b - r (in place), per-block partial sums of r.r per RHS
+__global__ __launch_bounds__(kVecBlock) void k_ir_residual(double *__restrict__ r, const double *__restrict__ b,
+                                                           long long n, int nr, double *__restrict__ partials) {
+  double v[1] = {0.0};
+  for (long long f = static_cast<long long>(blockIdx.x) * kVecBlock + threadIdx.x; f < n;
+       f += static_cast<long long>(gridDim.x) * kVecBlock) {
+    const double ri = b[f] - r[f];
+    r[f] = ri;
+    v[0] += ri * ri;
+  }
+  tile_sums<1>(v, nr, partials + static_cast<size_t>(blockIdx.x) * 2 * kMaxR);
+}
+
+__global__ void k_ir_norms(const double *__restrict__ partials, int nblocks, int nr, double *__restrict__ out) {
+  const int r = threadIdx.x;
+  if (r >= nr) return;
+  double s = 0.0;
+  for (int b = 0; b < nblocks; ++b) s += partials[static_cast<size_t>(b) * 2 * kMaxR + r];
+  out[r] = sqrt(s);
+}
+
+// start a new inner solve on rhs = r64 / scale[r]: d = 0, r = rhs, z = r / diag, p = z,
+// partials of r.r and r.z per RHS (done RHS get zeros)
+__global__ __launch_bounds__(kVecBlock) void k_ir_start(const double *__restrict__ r64, const double *__restrict__ inv,
+                                                        float *__restrict__ d, float *__restrict__ r,
+                                                        float *__restrict__ z, float *__restrict__ p,
+                                                        const float *__restrict__ diag, long long n, int nr,
+                                                        double *__restrict__ partials) {
+  double v[2] = {0.0, 0.0};
+  for (long long f = static_cast<long long>(blockIdx.x) * kVecBlock + threadIdx.x; f < n;
+       f += static_cast<long long>(gridDim.x) * kVecBlock) {
+    const float ri = static_cast<float>(r64[f] * inv[f % nr]);
+    const float zi = ri / diag[f / nr];
+    d[f] = 0.f;
+    r[f] = ri;
+    z[f] = zi;
+    p[f] = zi;
+    v[0] += static_cast<double>(ri) * ri;
+    v[1] += static_cast<double>(ri) * zi;
+  }
+  tile_sums<2>(v, nr, partials + static_cast<size_t>(blockIdx.x) * 2 * kMaxR);
+}
+
+__global__ void k_ir_start_fin(const double *__restrict__ partials, int nblocks, int nr, RhsDev *__restrict__ S,
+                               const double *__restrict__ tol, const int *__restrict__ live) {
+  const int r = threadIdx.x;
+  if (r >= nr) return;
+  double rr = 0.0, rz = 0.0;
+  for (int b = 0; b < nblocks; ++b) {
+    rr += partials[static_cast<size_t>(b) * 2 * kMaxR + r];
+    rz += partials[static_cast<size_t>(b) * 2 * kMaxR + kMaxR + r];
+  }
+  RhsDev &s = S[r];
+  s.rz = rz;
+  s.r0 = sqrt(rr);
+  s.rn = s.r0;
+  s.tol = tol[r];
+  s.alpha = s.beta = 0.0;
+  s.iters = 0;
+  s.done = (live[r] && s.r0 > 0.0) ? 0 : 1;
+}
+
+constexpr int kIrBlocks = 512;
+
+// ---- host side ---------------------------------------------------------------------------------
+struct Problem {
+  int id = -1;
+  const lfric_op *A64 = nullptr, *A32 = nullptr;
+  const lfric_vec *diag = nullptr, *b = nullptr;
+  lfric_vec *x = nullptr;
+  int nrhs = 1;
+  long long n = 0;
+  int scal0 = 0;  // first of this problem's nrhs entries of the batch's RhsDev array
+  float *d = nullptr, *r = nullptr, *z = nullptr, *p = nullptr, *q = nullptr;
+  double *r64 = nullptr;
+  size_t bytes32 = 0, bytes64 = 0;
+  // refinement state per RHS
+  std::vector<double> nr0, nr;
+  std::vector<int> conv, inner_iters;
+  int outer = 0, status = LFRIC_NOT_CONVERGED;
+  bool started = false, finished = false, inner_running = false;
+};
+
+}  // namespace
+}  // namespace lfric
+
+struct lfric_batch {
+  lfric_solver_opts opts{};
+  std::vector<std::unique_ptr<lfric::Problem>> probs;  // every problem not yet removed
+  int next_id = 0;
+  // per-(problem, RHS) state; ranges of freed problems are reused by size
+  lfric::RhsDev *d_state = nullptr, *h_state = nullptr;
+  size_t cap_state = 0, used_state = 0;
+  std::vector<std::pair<int, int>> free_ranges;  // (scal0, nrhs)
+  // tables of the active set
+  lfric::ProbDev *d_desc = nullptr;
+  int *d_tile0 = nullptr, *d_vtile0 = nullptr;
+  size_t cap_desc = 0, cap_tile0 = 0, cap_vtile0 = 0, cap_partials = 0;
+  double *d_partials = nullptr, *d_scratch = nullptr, *h_scratch = nullptr;
+  unsigned int *d_counter = nullptr;
+  std::vector<lfric::Problem *> active;
+  int ntiles = 0, nvtiles = 0, ntiles27 = 0;  // order-1 problems come first: tiles [0, ntiles27)
+  bool dirty = true;
+  // graph of one inner iteration of the active set
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  int use_graph = 1;
+  // launch shape of the apply (autotuned)
+  int block = 256, grid_mult = 2, ncu = 256;
+  long long replays = 0, iterations = 0;
+};
+
+namespace lfric {
+namespace {
+
+#define LAUNCHB(kernel, grid, block, ...) \
+  hipLaunchKernelGGL((kernel), dim3(grid), dim3(block), 0, ctx().stream, __VA_ARGS__)
+
+constexpr double kEps = 2.220446049250313e-16;
+
+int ensure(void **p, size_t &cap, size_t need, size_t elem) {
+  if (need <= cap && *p) return LFRIC_OK;
+  dfree(*p, cap * elem);
+  *p = nullptr;
+  cap = 0;
+  const size_t c = std::max<size_t>(need, 16) * 2;
+  if (int rc = dmalloc(p, c * elem)) return rc;
+  cap = c;
+  return LFRIC_OK;
+}
+
+void drop_graph(lfric_batch &B) {
+  if (B.exec) (void)hipGraphExecDestroy(B.exec);
+  if (B.graph) (void)hipGraphDestroy(B.graph);
+  B.exec = nullptr;
+  B.graph = nullptr;
+}
+
+int apply_tiles(const Problem &p, int block) {
+  const long long pts = static_cast<long long>(p.A32->to->gather_ncols) * (p.A32->to->nlayers + 1);
+  return static_cast<int>((pts + block - 1) / block);
+}
+
+// state entries for a new problem (reusing a freed range of the same size), growing the array
+// with a device copy when needed
+int alloc_state(lfric_batch &B, int nr, int *scal0) {
+  for (size_t i = 0; i < B.free_ranges.size(); ++i)
+    if (B.free_ranges[i].second == nr) {
+      *scal0 = B.free_ranges[i].first;
+      B.free_ranges.erase(B.free_ranges.begin() + static_cast<long>(i));
+      return LFRIC_OK;
+    }
+  const size_t need = B.used_state + nr;
+  if (need > B.cap_state) {
+    const size_t cap = std::max<size_t>(need * 2, 64);
+    RhsDev *d = nullptr, *h = nullptr;
+    if (int rc = dmalloc(reinterpret_cast<void **>(&d), cap * sizeof(RhsDev))) return rc;
+    if (hipHostMalloc(reinterpret_cast<void **>(&h), cap * sizeof(RhsDev), hipHostMallocDefault) != hipSuccess) {
+      dfree(d, cap * sizeof(RhsDev));
+      return fail(LFRIC_E_NOMEM, "pinned host state");
+    }
+    if (B.used_state) LF_HIP(hipMemcpy(d, B.d_state, B.used_state * sizeof(RhsDev), hipMemcpyDeviceToDevice));
+    dfree(B.d_state, B.cap_state * sizeof(RhsDev));
+    if (B.h_state) (void)hipHostFree(B.h_state);
+    B.d_state = d;
+    B.h_state = h;
+    B.cap_state = cap;
+    B.dirty = true;  // kernels hold the array's address
+  }
+  *scal0 = static_cast<int>(B.used_state);
+  B.used_state = need;
+  return LFRIC_OK;
+}
+
+// device tables of the problems whose inner solve runs; the graph is rebuilt on next use
+int rebuild(lfric_batch &B) {
+  B.active.clear();
+  for (auto &u : B.probs)
+    if (u->inner_running) B.active.push_back(u.get());
+  // order-1 problems first (their own apply kernel), heaviest tiles first within each kind
+  std::stable_sort(B.active.begin(), B.active.end(), [](const Problem *a, const Problem *b) {
+    const int ka = a->A32->from->ndf, kb = b->A32->from->ndf;
+    if (ka != kb) return ka > kb;
+    if (a->nrhs != b->nrhs) return a->nrhs > b->nrhs;
+    return a->n > b->n;
+  });
+  const int np = static_cast<int>(B.active.size());
+  std::vector<ProbDev> desc(np);
+  std::vector<int> tile0(np + 1, 0), vtile0(np + 1, 0);
+  for (int i = 0; i < np; ++i) {
+    const Problem &p = *B.active[i];
+    const FunctionSpace &to = *p.A32->to, &fr = *p.A32->from;
+    ProbDev &d = desc[i];
+    std::memset(&d, 0, sizeof(d));
+    d.A = static_cast<const float *>(p.A32->d);
+    d.gs = p.A32->gs;
+    d.ebase = to.d_ebase;
+    d.map2 = fr.d_map;
+    d.goff = to.d_goff;
+    d.gent = to.d_gent;
+    d.gcol = to.d_gcol;
+    d.diag = static_cast<const float *>(p.diag->d);
+    d.d = p.d;
+    d.r = p.r;
+    d.z = p.z;
+    d.p = p.p;
+    d.q = p.q;
+    d.n = p.n;
+    d.ncols = to.gather_ncols;
+    d.nl = to.nlayers;
+    d.ndf = fr.ndf;
+    d.nrhs = p.nrhs;
+    for (int k = 0; k < 32; ++k) d.partner[k] = static_cast<int8_t>(k < to.ndf ? to.partner[k] : -1);
+    d.apply_tile0 = tile0[i];
+    tile0[i + 1] = tile0[i] + apply_tiles(p, B.block);
+    d.vec_tile0 = vtile0[i];
+    d.vec_ntiles = static_cast<int>((p.n + kVecTile - 1) / kVecTile);
+    vtile0[i + 1] = vtile0[i] + d.vec_ntiles;
+    d.scal0 = p.scal0;
+  }
+  B.ntiles = tile0[np];
+  B.nvtiles = vtile0[np];
+  B.ntiles27 = 0;
+  for (int i = 0; i < np; ++i)
+    if (desc[i].ndf == 27) B.ntiles27 = tile0[i + 1];
+  if (int rc = ensure(reinterpret_cast<void **>(&B.d_desc), B.cap_desc, np + 1, sizeof(ProbDev))) return rc;
+  if (int rc = ensure(reinterpret_cast<void **>(&B.d_tile0), B.cap_tile0, np + 1, sizeof(int))) return rc;
+  if (int rc = ensure(reinterpret_cast<void **>(&B.d_vtile0), B.cap_vtile0, np + 1, sizeof(int))) return rc;
+  if (int rc = ensure(reinterpret_cast<void **>(&B.d_partials), B.cap_partials,
+                      static_cast<size_t>(std::max(B.nvtiles, kIrBlocks)) * 2 * kMaxR, sizeof(double)))
+    return rc;
+  if (np) {
+    LF_HIP(hipMemcpyAsync(B.d_desc, desc.data(), sizeof(ProbDev) * np, hipMemcpyHostToDevice, ctx().stream));
+    LF_HIP(hipMemcpyAsync(B.d_tile0, tile0.data(), sizeof(int) * (np + 1), hipMemcpyHostToDevice, ctx().stream));
+    LF_HIP(hipMemcpyAsync(B.d_vtile0, vtile0.data(), sizeof(int) * (np + 1), hipMemcpyHostToDevice, ctx().stream));
+  }
+  LF_HIP(hipStreamSynchronize(ctx().stream));
+  drop_graph(B);
+  B.dirty = false;
+  return LFRIC_OK;
+}
+
+template <int BS>
+void launch_apply_bs(lfric_batch &B, int grid_mult) {
+  const int np = static_cast<int>(B.active.size());
+  const int cap = grid_mult * B.ncu * (1024 / BS);
+  if (B.ntiles27 > 0)
+    LAUNCHB((k_batch_apply<BS, 27>), std::min(B.ntiles27, cap), BS, B.d_desc, B.d_tile0, np, 0, B.ntiles27,
+            B.d_counter);
+  if (B.ntiles > B.ntiles27)
+    LAUNCHB((k_batch_apply<BS, 8>), std::min(B.ntiles - B.ntiles27, cap), BS, B.d_desc, B.d_tile0, np, B.ntiles27,
+            B.ntiles, B.d_counter + 1);
+}
+
+void launch_apply(lfric_batch &B, int block, int grid_mult) {
+  LAUNCHB(k_zero_counter, 1, 64, B.d_counter);
+  switch (block) {
+    case 64: launch_apply_bs<64>(B, grid_mult); break;
+    case 128: launch_apply_bs<128>(B, grid_mult); break;
+    default: launch_apply_bs<256>(B, grid_mult); break;
+  }
+}
+
+// one inner CG iteration of every active problem (the reference's cg_solve loop body,
+// sci_iterative_solver_mod.f90:545-575, per RHS)
+int enqueue_iteration(lfric_batch &B) {
+  const int np = static_cast<int>(B.active.size());
+  launch_apply(B, B.block, B.grid_mult);                                                       // q = A p
+  LAUNCHB(k_batch_pq, B.nvtiles, kVecBlock, B.d_desc, B.d_vtile0, np, B.d_partials);           // p.q
+  LAUNCHB(k_batch_alpha, np, 64, B.d_desc, np, B.d_partials, B.d_state);                      // alpha
+  LAUNCHB(k_batch_update, B.nvtiles, kVecBlock, B.d_desc, B.d_vtile0, np, B.d_state, B.d_partials);
+  LAUNCHB(k_batch_beta, np, 64, B.d_desc, np, B.d_partials, B.d_state, 0.0, B.opts.max_iter);  // test, beta
+  LAUNCHB(k_batch_aypx, B.nvtiles, kVecBlock, B.d_desc, B.d_vtile0, np, B.d_state);            // p = z + beta p
+  LF_HIP(hipGetLastError());
+  return LFRIC_OK;
+}
+
+int run_iterations(lfric_batch &B, int k) {
+  if (B.active.empty()) return LFRIC_OK;
+  if (B.use_graph) {
+    if (!B.exec) {
+      LF_HIP(hipStreamBeginCapture(ctx().stream, hipStreamCaptureModeThreadLocal));
+      const int rc = enqueue_iteration(B);
+      hipGraph_t g = nullptr;
+      const hipError_t e = hipStreamEndCapture(ctx().stream, &g);
+      if (rc) return rc;
+      if (e != hipSuccess) return fail(LFRIC_E_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+      B.graph = g;
+      LF_HIP(hipGraphInstantiate(&B.exec, B.graph, nullptr, nullptr, 0));
+    }
+    for (int i = 0; i < k; ++i) LF_HIP(hipGraphLaunch(B.exec, ctx().stream));
+    B.replays += k;
+  } else {
+    for (int i = 0; i < k; ++i)
+      if (int rc = enqueue_iteration(B)) return rc;
+  }
+  B.iterations += k;
+  return LFRIC_OK;
+}
+
+// FP64 refinement step of one problem (the outer loop of the mixed-precision CG): fold the
+// scaled inner solution into x, take b - A64 x in FP64, test each RHS, and start the next
+// inner solve on r / ||r|| for the RHS that still need one
+int refine(lfric_batch &B, Problem &p) {
+  const int nr = p.nrhs;
+  const long long n = p.n;
+  const int vg = grid_for(n, kBlock, 8192);
+  double *scale = B.d_scratch, *inv = B.d_scratch + kMaxR, *tol = B.d_scratch + 2 * kMaxR;
+  int *live = reinterpret_cast<int *>(B.d_scratch + 3 * kMaxR);
+  double *nrm = B.d_scratch + 4 * kMaxR;
+  if (p.started) {
+    for (int r = 0; r < nr; ++r) B.h_scratch[r] = p.conv[r] ? 0.0 : p.nr[r];
+    LF_HIP(hipMemcpyAsync(scale, B.h_scratch, sizeof(double) * nr, hipMemcpyHostToDevice, ctx().stream));
+    LAUNCHB(k_ir_accumulate, vg, kBlock, static_cast<double *>(p.x->d), p.d, scale, n, nr);
+  }
+  if (int rc = launch_lma_apply_mrhs(p.A64, p.x->d, p.r64, nr, LFRIC_MRHS_GATHER)) return rc;
+  LAUNCHB(k_ir_residual, kIrBlocks, kVecBlock, p.r64, static_cast<const double *>(p.b->d), n, nr, B.d_partials);
+  LAUNCHB(k_ir_norms, 1, 64, B.d_partials, kIrBlocks, nr, nrm);
+  LF_HIP(hipMemcpyAsync(B.h_scratch + 4 * kMaxR, nrm, sizeof(double) * nr, hipMemcpyDeviceToHost, ctx().stream));
+  LF_HIP(hipStreamSynchronize(ctx().stream));
+  bool all = true, nan = false;
+  for (int r = 0; r < nr; ++r) {
+    const double v = B.h_scratch[4 * kMaxR + r];
+    if (!p.started) p.nr0[r] = v;
+    if (!p.conv[r]) p.nr[r] = v;
+    if (v != v) nan = true;
+    if (!p.conv[r] && (v <= B.opts.r_tol * p.nr0[r] || v <= B.opts.a_tol || p.nr0[r] < kEps)) p.conv[r] = 1;
+    all = all && p.conv[r];
+  }
+  if (p.started) ++p.outer;
+  p.started = true;
+  if (nan || all || p.outer >= B.opts.max_outer) {
+    p.status = nan ? LFRIC_NAN : all ? LFRIC_OK : LFRIC_NOT_CONVERGED;
+    p.finished = true;
+    if (p.inner_running) B.dirty = true;
+    p.inner_running = false;
+    return LFRIC_OK;
+  }
+  double hv[4 * kMaxR] = {};
+  for (int r = 0; r < nr; ++r) {
+    hv[kMaxR + r] = p.conv[r] ? 0.0 : 1.0 / p.nr[r];
+    hv[2 * kMaxR + r] = std::max(B.opts.inner_r_tol, 0.5 * B.opts.r_tol * p.nr0[r] / p.nr[r]);
+    reinterpret_cast<int *>(&hv[3 * kMaxR])[r] = p.conv[r] ? 0 : 1;
+  }
+  std::memcpy(B.h_scratch, hv, sizeof(hv));
+  LF_HIP(hipMemcpyAsync(B.d_scratch, B.h_scratch, sizeof(hv), hipMemcpyHostToDevice, ctx().stream));
+  LAUNCHB(k_ir_start, kIrBlocks, kVecBlock, p.r64, inv, p.d, p.r, p.z, p.p, static_cast<const float *>(p.diag->d), n,
+          nr, B.d_partials);
+  LAUNCHB(k_ir_start_fin, 1, 64, B.d_partials, kIrBlocks, nr, B.d_state + p.scal0, tol, live);
+  LF_HIP(hipGetLastError());
+  LF_HIP(hipStreamSynchronize(ctx().stream));
+  if (!p.inner_running) B.dirty = true;
+  p.inner_running = true;
+  return LFRIC_OK;
+}
+
+void free_problem(lfric_batch &B, Problem &p) {
+  dfree(p.d, p.bytes32);
+  dfree(p.r64, p.bytes64);
+  p.d = nullptr;
+  p.r64 = nullptr;
+  if (p.scal0 >= 0) B.free_ranges.push_back({p.scal0, p.nrhs});
+  p.scal0 = -1;
+}
+
+Problem *find(lfric_batch &B, int id) {
+  for (auto &u : B.probs)
+    if (u->id == id) return u.get();
+  return nullptr;
+}
+
+int run_rounds(lfric_batch &B, int rounds, int *nactive) {
+  const int k = std::max(1, B.opts.check_every);
+  for (int round = 0; round < rounds; ++round) {
+    for (auto &u : B.probs)  // problems added since the last round: first residual
+      if (!u->started && !u->finished)
+        if (int rc = refine(B, *u)) return rc;
+    if (B.dirty)
+      if (int rc = rebuild(B)) return rc;
+    if (B.active.empty()) break;
+    if (int rc = run_iterations(B, k)) return rc;
+    LF_HIP(hipMemcpyAsync(B.h_state, B.d_state, sizeof(RhsDev) * B.used_state, hipMemcpyDeviceToHost,
+                          ctx().stream));
+    LF_HIP(hipStreamSynchronize(ctx().stream));
+    for (Problem *p : std::vector<Problem *>(B.active)) {
+      bool ended = true;
+      for (int r = 0; r < p->nrhs; ++r)
+        if (!p->conv[r] && !B.h_state[p->scal0 + r].done) ended = false;
+      if (!ended) continue;
+      for (int r = 0; r < p->nrhs; ++r)
+        if (!p->conv[r]) p->inner_iters[r] += B.h_state[p->scal0 + r].iters;
+      if (int rc = refine(B, *p)) return rc;
+    }
+  }
+  if (nactive) {
+    int a = 0;
+    for (auto &u : B.probs) a += !u->finished;
+    *nactive = a;
+  }
+  return LFRIC_OK;
+}
+
+}  // namespace
+}  // namespace lfric
+
+using namespace lfric;
+
+extern "C" {
+
+int lfric_batch_create(const lfric_solver_opts *opts, lfric_batch **out) {
+  if (int rc = require_init()) return rc;
+  LF_CHECK(opts && out, LFRIC_E_ARG, "null argument");
+  auto *B = new lfric_batch;
+  B->opts = *opts;
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, ctx().device) == hipSuccess) B->ncu = prop.multiProcessorCount;
+  int rc = dmalloc(reinterpret_cast<void **>(&B->d_counter), 256);
+  if (!rc) rc = dmalloc(reinterpret_cast<void **>(&B->d_scratch), sizeof(double) * 8 * kMaxR);
+  if (!rc && hipHostMalloc(reinterpret_cast<void **>(&B->h_scratch), sizeof(double) * 8 * kMaxR,
+                           hipHostMallocDefault) != hipSuccess)
+    rc = fail(LFRIC_E_NOMEM, "pinned host scratch");
+  if (rc) {
+    lfric_batch_destroy(B);
+    return rc;
+  }
+  *out = B;
+  return LFRIC_OK;
+}
+
+int lfric_batch_add(lfric_batch *B, const lfric_op *A64, const lfric_op *A32, const lfric_vec *diag32, lfric_vec *x,
+                    const lfric_vec *b, int *id) {
+  if (int rc = require_init()) return rc;
+  LF_CHECK(B && A64 && A32 && diag32 && x && b && id, LFRIC_E_ARG, "null argument");
+  LF_CHECK(A64->precision == LFRIC_R64 && A32->precision == LFRIC_R32 && A64->to == A32->to &&
+               A64->to == A64->from && A32->to == A32->from,
+           LFRIC_E_ARG, "a batch problem needs square FP64 and FP32 operators on one space");
+  LF_CHECK(A32->from->ndf == 8 || A32->from->ndf == 27, LFRIC_E_UNSUPPORTED, "batched solves take ndf 8 or 27");
+  LF_CHECK(x->precision == LFRIC_R64 && b->precision == LFRIC_R64 && x->nrhs == b->nrhs &&
+               x->fs->undf == A64->to->undf && b->fs->undf == A64->to->undf && x->d != b->d,
+           LFRIC_E_ARG, "x and b must be distinct FP64 vectors of the operator space with one number of RHS");
+  LF_CHECK(diag32->precision == LFRIC_R32 && diag32->nrhs == 1 && diag32->fs->undf == A64->to->undf, LFRIC_E_ARG,
+           "the Jacobi diagonal must be a one-RHS FP32 vector of the space");
+  const int nr = x->nrhs;
+  LF_CHECK(nr == 1 || nr == 2 || nr == 4 || nr == 8, LFRIC_E_UNSUPPORTED, "nrhs must be 1, 2, 4 or 8");
+  auto p = std::make_unique<Problem>();
+  p->A64 = A64;
+  p->A32 = A32;
+  p->diag = diag32;
+  p->x = x;
+  p->b = b;
+  p->nrhs = nr;
+  p->n = x->n;
+  p->bytes32 = sizeof(float) * static_cast<size_t>(p->n) * 5;
+  p->bytes64 = sizeof(double) * static_cast<size_t>(p->n);
+  p->scal0 = -1;
+  if (int rc = dmalloc(reinterpret_cast<void **>(&p->d), p->bytes32)) return rc;
+  if (int rc = dmalloc(reinterpret_cast<void **>(&p->r64), p->bytes64)) {
+    free_problem(*B, *p);
+    return rc;
+  }
+  if (int rc = alloc_state(*B, nr, &p->scal0)) {
+    free_problem(*B, *p);
+    return rc;
+  }
+  p->r = p->d + p->n;
+  p->z = p->r + p->n;
+  p->p = p->z + p->n;
+  p->q = p->p + p->n;
+  p->nr0.assign(nr, 0.0);
+  p->nr.assign(nr, 0.0);
+  p->conv.assign(nr, 0);
+  p->inner_iters.assign(nr, 0);
+  // the gather lists of the space (built once per space)
+  if (!A32->to->d_gcol)
+    if (int rc = launch_lma_apply_mrhs(A64, x->d, p->r64, nr, LFRIC_MRHS_GATHER)) {
+      free_problem(*B, *p);
+      return rc;
+    }
+  p->id = B->next_id++;
+  *id = p->id;
+  B->probs.push_back(std::move(p));
+  return LFRIC_OK;
+}
+
+int lfric_batch_run(lfric_batch *B, int rounds, int *nactive) {
+  if (int rc = require_init()) return rc;
+  LF_CHECK(B && rounds >= 0, LFRIC_E_ARG, "bad arguments");
+  return run_rounds(*B, rounds, nactive);
+}
+
+int lfric_batch_solve(lfric_batch *B) {
+  if (int rc = require_init()) return rc;
+  LF_CHECK(B, LFRIC_E_ARG, "null argument");
+  int nactive = 1;
+  while (nactive > 0)
+    if (int rc = run_rounds(*B, 1, &nactive)) return rc;
+  return LFRIC_OK;
+}
+
+int lfric_batch_finished(const lfric_batch *B, int id, int *finished) {
+  LF_CHECK(B && finished, LFRIC_E_ARG, "null argument");
+  for (auto &u : B->probs)
+    if (u->id == id) {
+      *finished = u->finished ? 1 : 0;
+      return LFRIC_OK;
+    }
+  return fail(LFRIC_E_ARG, "no such problem in the batch");
+}
+
+int lfric_batch_result(const lfric_batch *B, int id, lfric_solver_result *res) {
+  LF_CHECK(B && res, LFRIC_E_ARG, "null argument");
+  for (auto &u : B->probs)
+    if (u->id == id) {
+      const Problem &p = *u;
+      for (int r = 0; r < p.nrhs; ++r) {
+        res[r].status = p.finished ? (p.conv[r] ? LFRIC_OK : p.status == LFRIC_NAN ? LFRIC_NAN : LFRIC_NOT_CONVERGED)
+                                   : LFRIC_NOT_CONVERGED;
+        res[r].iterations = p.inner_iters[r];
+        res[r].outer_iterations = p.outer;
+        res[r].r_nrm_0 = p.nr0[r];
+        res[r].r_nrm = p.nr[r];
+        res[r].true_rel_res = p.nr0[r] > 0 ? p.nr[r] / p.nr0[r] : 0.0;
+      }
+      return LFRIC_OK;
+    }
+  return fail(LFRIC_E_ARG, "no such problem in the batch");
+}
+
+int lfric_batch_remove(lfric_batch *B, int id) {
+  if (int rc = require_init()) return rc;
+  LF_CHECK(B, LFRIC_E_ARG, "null argument");
+  for (size_t i = 0; i < B->probs.size(); ++i)
+    if (B->probs[i]->id == id) {
+      if (B->probs[i]->inner_running) B->dirty = true;
+      LF_HIP(hipStreamSynchronize(ctx().stream));
+      free_problem(*B, *B->probs[i]);
+      B->probs.erase(B->probs.begin() + static_cast<long>(i));
+      return LFRIC_OK;
+    }
+  return fail(LFRIC_E_ARG, "no such problem in the batch");
+}
+
+// Autotuner: time the persistent apply of the current active set for each block size and
+// grid size, keep the fastest.  Needs a round to have run (so the active set exists).
+int lfric_batch_tune(lfric_batch *B, double *best_ms) {
+  if (int rc = require_init()) return rc;
+  LF_CHECK(B, LFRIC_E_ARG, "null argument");
+  if (B->dirty)
+    if (int rc = rebuild(*B)) return rc;
+  if (B->active.empty()) return fail(LFRIC_E_STATE, "no active problem to tune on");
+  const int blocks[] = {64, 128, 256};
+  const int mults[] = {1, 2, 4, 8};
+  double best = 1e30;
+  int bb = B->block, bm = B->grid_mult;
+  for (int blk : blocks) {
+    B->block = blk;
+    if (int rc = rebuild(*B)) return rc;  // tiles follow the block size
+    for (int gm : mults) {
+      launch_apply(*B, blk, gm);  // warm
+      LF_HIP(hipEventRecord(ctx().t0, ctx().stream));
+      for (int rep = 0; rep < 5; ++rep) launch_apply(*B, blk, gm);
+      LF_HIP(hipEventRecord(ctx().t1, ctx().stream));
+      LF_HIP(hipEventSynchronize(ctx().t1));
+      float ms = 0.f;
+      LF_HIP(hipEventElapsedTime(&ms, ctx().t0, ctx().t1));
+      if (ms / 5 < best) {
+        best = ms / 5;
+        bb = blk;
+        bm = gm;
+      }
+    }
+  }
+  B->block = bb;
+  B->grid_mult = bm;
+  if (int rc = rebuild(*B)) return rc;
+  if (best_ms) *best_ms = best;
+  return LFRIC_OK;
+}
+
+int lfric_batch_config(lfric_batch *B, int use_graph, int block, int grid_mult) {
+  LF_CHECK(B, LFRIC_E_ARG, "null argument");
+  LF_CHECK(block == 0 || block == 64 || block == 128 || block == 256, LFRIC_E_ARG, "block must be 64, 128 or 256");
+  if (use_graph >= 0) B->use_graph = use_graph;
+  if (block) B->block = block;
+  if (grid_mult > 0) B->grid_mult = grid_mult;
+  B->dirty = true;
+  return LFRIC_OK;
+}
+
+int lfric_batch_stats(const lfric_batch *B, long long *iterations, int *block, int *grid_mult, int *nactive_tiles) {
+  LF_CHECK(B, LFRIC_E_ARG, "null argument");
+  if (iterations) *iterations = B->iterations;
+  if (block) *block = B->block;
+  if (grid_mult) *grid_mult = B->grid_mult;
+  if (nactive_tiles) *nactive_tiles = B->ntiles;
+  return LFRIC_OK;
+}
+
+void lfric_batch_destroy(lfric_batch *B) {
+  if (!B) return;
+  (void)hipStreamSynchronize(ctx().stream);
+  for (auto &u : B->probs) free_problem(*B, *u);
+  drop_graph(*B);
+  dfree(B->d_state, B->cap_state * sizeof(RhsDev));
+  if (B->h_state) (void)hipHostFree(B->h_state);
+  dfree(B->d_desc, B->cap_desc * sizeof(ProbDev));
+  dfree(B->d_tile0, B->cap_tile0 * sizeof(int));
+  dfree(B->d_vtile0, B->cap_vtile0 * sizeof(int));
+  dfree(B->d_partials, B->cap_partials * sizeof(double));
+  dfree(B->d_scratch, sizeof(double) * 8 * kMaxR);
+  dfree(B->d_counter, 256);
+  if (B->h_scratch) (void)hipHostFree(B->h_scratch);
+  delete B;
+}
+
+}  // extern "C"
