@@ -440,7 +440,10 @@ int alloc_state(lfric_batch &B, int nr, int *scal0) {
       dfree(d, cap * sizeof(RhsDev));
       return fail(LFRIC_E_NOMEM, "pinned host state");
     }
-    if (B.used_state) LF_HIP(hipMemcpy(d, B.d_state, B.used_state * sizeof(RhsDev), hipMemcpyDeviceToDevice));
+    if (B.used_state) {
+      LF_HIP(hipMemcpyAsync(d, B.d_state, B.used_state * sizeof(RhsDev), hipMemcpyDeviceToDevice, ctx().stream));
+      LF_HIP(hipStreamSynchronize(ctx().stream));
+    }
     dfree(B.d_state, B.cap_state * sizeof(RhsDev));
     if (B.h_state) (void)hipHostFree(B.h_state);
     B.d_state = d;
@@ -697,6 +700,10 @@ int lfric_batch_create(const lfric_solver_opts *opts, lfric_batch **out) {
   if (hipGetDeviceProperties(&prop, ctx().device) == hipSuccess) B->ncu = prop.multiProcessorCount;
   int rc = dmalloc(reinterpret_cast<void **>(&B->d_counter), 256);
   if (!rc) rc = dmalloc(reinterpret_cast<void **>(&B->d_scratch), sizeof(double) * 8 * kMaxR);
+  // the refinement kernels use kIrBlocks partials before any table exists
+  if (!rc)
+    rc = ensure(reinterpret_cast<void **>(&B->d_partials), B->cap_partials, static_cast<size_t>(kIrBlocks) * 2 * kMaxR,
+                sizeof(double));
   if (!rc && hipHostMalloc(reinterpret_cast<void **>(&B->h_scratch), sizeof(double) * 8 * kMaxR,
                            hipHostMallocDefault) != hipSuccess)
     rc = fail(LFRIC_E_NOMEM, "pinned host scratch");
