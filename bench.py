@@ -102,11 +102,23 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--apply-only", action="store_true",
                     help="time only the LMA applies and a known-size vector copy (for hardware-counter runs)")
+    ap.add_argument("--workload", choices=("single", "batched", "served"), default="single",
+                    help="single: C96x70 mixed vs FP64 solve (default); batched: 64 uneven multi-RHS problems "
+                         "in one batch vs sequential FP64 solves; served: Poisson request stream with "
+                         "continuous batching vs sequential FP64 solves")
+    ap.add_argument("--problems", type=int, default=64)
+    ap.add_argument("--serve-window", type=float, default=20.0, help="seconds of arrivals")
+    ap.add_argument("--serve-rate", type=float, default=200.0, help="requests per second")
+    ap.add_argument("--serve-limit", type=float, default=30.0, help="wall seconds of admission per server")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only: --gpus {args.gpus} is not supported (the hot path is a "
               "single mesh partition; see the sharding design for more than one GPU)", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "batched":
+        return bench_batched(args)
+    if args.workload == "served":
+        return bench_served(args)
 
     import lfric_core_amd as L
     from lfric_core_amd.solvers import residual_norm
@@ -217,6 +229,177 @@ def main():
     if not args.no_cpu_baseline:
         bh = b.export_field()
         line["cpu_baseline"] = cpu_baseline(fs, op64, bh, pres.iterations, args.cpu_budget)
+    print(json.dumps(line))
+
+
+def _common(args, metric, value, unit, ms, config, dtype):
+    return {"metric": metric, "value": value, "unit": unit, "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": dtype, "data": "synthetic (counter-based generator)",
+            "config": config}
+
+
+def batched_mix(n_problems, seed=0):
+    """64 W0 SPD problems: mesh C12/C24/C48, 10/30/70 levels, element order 0 or 1 (p = 0.5),
+    4 right-hand sides, scaled element matrices."""
+    rng = np.random.default_rng(seed)
+    return [(int(rng.choice((12, 24, 48))), int(rng.choice((10, 30, 70))), int(rng.integers(0, 2)), 4,
+             int(rng.integers(0, 2**31))) for _ in range(n_problems)]
+
+
+def bench_batched(args):
+    import lfric_core_amd as L
+    from lfric_core_amd.batch import Batch
+
+    L.init(0)
+    mix = batched_mix(args.problems)
+    probs = []
+    for (n, nl, order, nrhs, seed) in mix:
+        fs = L.FunctionSpace.cubedsphere(n, nl, L.W0, order)
+        op64 = L.LmaOperator(fs).generate_spd(seed, scaled=True)
+        op32 = op64.convert(L.R32)
+        d32 = op32.mm_diagonal(L.FieldVector(fs, L.R32))
+        b = L.FieldVector(fs, L.R64, nrhs).fill_uniform(seed, 3)
+        x = L.FieldVector(fs, L.R64, nrhs)
+        probs.append(dict(fs=fs, op64=op64, op32=op32, d32=d32, b=b, x=x, nrhs=nrhs, key=(n, nl, order)))
+    total_dofs = sum(p["fs"].undf * p["nrhs"] for p in probs)
+
+    def run_batch(tune):
+        bt = Batch(r_tol=1e-8, a_tol=0.0, max_iter=20000, check_every=8)
+        ids = []
+        for p in probs:
+            p["x"].set_scalar(0.0)
+            ids.append(bt.add(p["op64"], p["op32"], p["d32"], p["x"], p["b"]))
+        L.synchronize()
+        t0 = time.perf_counter()
+        done_at = {}
+        tuned = None
+        while True:
+            na = bt.run(1)
+            if tune and tuned is None:
+                tuned = bt.tune()
+            now = time.perf_counter() - t0
+            for i, pid in enumerate(ids):
+                if pid not in done_at and bt.finished(pid):
+                    done_at[pid] = now
+            if na == 0:
+                break
+        wall = time.perf_counter() - t0
+        res = [bt.result(pid) for pid in ids]
+        st = bt.stats()
+        bt.close()
+        return wall, np.array(sorted(done_at.values())), res, st, tuned
+
+    run_batch(tune=True)  # warm-up, and the autotuner's choice for the timed runs
+    times = []
+    for _ in range(max(1, args.steps)):
+        wall, done, res, st, _ = run_batch(tune=False)
+        times.append(wall)
+    worst = max(r.true_rel_res for rr in res for r in rr)
+    ok = all(r.status == 0 for rr in res for r in rr)
+    # sequential plain FP64 solves of the same systems, one RHS at a time
+    seq_t = 0.0
+    seq_it = 0
+    for p in probs:
+        B = p["b"].export_field().reshape(p["fs"].undf, p["nrhs"])
+        prec = L.DiagonalPreconditioner.from_operator(p["op64"])
+        cg = L.ConjugateGradient(p["op64"], prec, 1e-8, 0.0, 20000, check_every=16)
+        for r in range(p["nrhs"]):
+            bv = L.FieldVector(p["fs"]).import_field(B[:, r])
+            xv = L.FieldVector(p["fs"])
+            xv.set_scalar(0.0)
+            L.synchronize()
+            t0 = time.perf_counter()
+            out = cg.apply(xv, bv)
+            L.synchronize()
+            seq_t += time.perf_counter() - t0
+            seq_it += out.iterations
+    wall = float(np.median(times))
+    line = _common(args, "solved dofs/s at rel-res 1e-8 (batched mixed precision vs sequential FP64), p50/p99 "
+                         "solve ms", total_dofs / wall, "solved dofs/s", wall * 1e3,
+                   {"workload": f"{len(probs)} uneven W0 SPD problems (C12/C24/C48, 10/30/70 levels, order 0/1), "
+                                "4 RHS each, Jacobi CG to 1e-8 in FP64 refinement, one batch",
+                    "problems": len(probs), "rhs_total": sum(p["nrhs"] for p in probs),
+                    "dofs_total": total_dofs, "parallelism": "one GPU, one batch"},
+                   "f32 inner CG / f64 refinement")
+    line.update({
+        "solves_per_s": len(probs) / wall,
+        "rhs_solves_per_s": sum(p["nrhs"] for p in probs) / wall,
+        "solve_ms": {"batch_wall": wall * 1e3, "problem_done_p50": float(np.percentile(done, 50) * 1e3),
+                     "problem_done_p99": float(np.percentile(done, 99) * 1e3)},
+        "sequential_fp64": {"wall_ms": seq_t * 1e3, "dofs_per_s": total_dofs / seq_t,
+                            "speedup_of_batch": seq_t / wall, "iterations": seq_it},
+        "verification": {"all_converged": ok, "worst_true_rel_res_fp64": worst},
+        "autotuned": {"block": st["block"], "grid_mult": st["grid_mult"]},
+        "batch_iterations": st["iterations"],
+    })
+    print(json.dumps(line))
+
+
+def bench_served(args):
+    import lfric_core_amd as L
+    from lfric_core_amd import serving
+
+    L.init(0)
+    reqs = serving.poisson_stream(args.serve_rate, args.serve_window, seed=0)
+    import ctypes
+    free, total = ctypes.c_longlong(), ctypes.c_longlong()
+    L.lib().lfric_device_memory(ctypes.byref(free), ctypes.byref(total))
+    # the 288 GB of HBM, less what the runtime already holds and 8 GB of head-room
+    budget = min(288 * 10**9, free.value) - 8 * 10**9
+    srv = serving.SolveServer(budget, starve_s=2.0, check_every=8)
+    rep = srv.serve(reqs, wall_limit_s=args.serve_limit)
+    srv.close()
+    done = [r for r in reqs if r.status == 0]
+    worst = max((r.true_rel_res / r.tol for r in done), default=float("nan"))
+    # the same stream served one request at a time with plain FP64 CG per RHS, in arrival order
+    seq = serving.poisson_stream(args.serve_rate, args.serve_window, seed=0)
+    t0 = time.perf_counter()
+    seq_lat, seq_done, seq_rhs = [], 0, 0
+    for r in seq:
+        now = time.perf_counter() - t0
+        if now >= args.serve_limit:
+            break
+        if r.t_arrive > now:
+            time.sleep(r.t_arrive - now)
+        fs = L.FunctionSpace.cubedsphere(r.n, r.nl, L.W0, r.order)
+        op64 = L.LmaOperator(fs).generate_spd(r.seed, scaled=True)
+        prec = L.DiagonalPreconditioner.from_operator(op64)
+        bm = L.FieldVector(fs, L.R64, serving.padded_rhs(r.nrhs)).fill_uniform(r.seed, 3)
+        B = bm.export_field().reshape(fs.undf, -1)
+        cg = L.ConjugateGradient(op64, prec, r.tol, 0.0, 20000, check_every=16)
+        for k in range(r.nrhs):
+            xv = L.FieldVector(fs)
+            xv.set_scalar(0.0)
+            cg.apply(xv, L.FieldVector(fs).import_field(B[:, k]))
+        L.synchronize()
+        seq_lat.append((time.perf_counter() - t0 - r.t_arrive) * 1e3)
+        seq_done += 1
+        seq_rhs += r.nrhs
+        del op64, prec, bm
+    seq_wall = time.perf_counter() - t0
+    line = _common(args, "served solves/s with p50/p99 latency (continuous batching, mixed precision, vs "
+                         "sequential plain FP64)", rep.solves_per_s, "solves/s", rep.wall_s * 1e3,
+                   {"workload": f"Poisson {args.serve_rate:g} req/s for {args.serve_window:g} s (seed 0), C24/C48/C96/"
+                                "C192 (0.4/0.3/0.2/0.1), 30/70 levels, order 0/1, 1-8 RHS, tol 1e-6/1e-8, scaled "
+                                "SPD elements, 288 GB HBM budget",
+                    "requests": len(reqs), "admission_wall_s": args.serve_limit,
+                    "parallelism": "one GPU, continuous batching"},
+                   "f32 inner CG / f64 refinement")
+    line.update({
+        "completed": rep.completed, "unfinished": rep.unfinished, "rejected_too_large": rep.rejected,
+        "rhs_per_s": rep.rhs_per_s, "dofs_per_s": rep.dofs_per_s,
+        "latency_ms": {"p50": rep.p50_ms, "p99": rep.p99_ms, "queue_p50": rep.queue_p50_ms,
+                       "queue_p99": rep.queue_p99_ms},
+        "peak_admitted_bytes": rep.peak_bytes,
+        "verification": {"worst_true_rel_res_over_tol": worst},
+        "sequential_fp64": {"completed": seq_done, "solves_per_s": seq_done / seq_wall, "rhs_per_s": seq_rhs / seq_wall,
+                            "latency_ms_p50": float(np.percentile(seq_lat, 50)) if seq_lat else None,
+                            "latency_ms_p99": float(np.percentile(seq_lat, 99)) if seq_lat else None},
+        "admission": "arrival order with backfill; a request waiting over 2 s blocks later admissions; requests "
+                     "larger than the budget are rejected; admission stops at the wall limit and admitted "
+                     "requests drain",
+    })
     print(json.dumps(line))
 
 

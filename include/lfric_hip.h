@@ -61,6 +61,8 @@ int lfric_timer_start(void);
 int lfric_timer_stop(double *ms);
 /* Bytes of device memory held by live handles. */
 long long lfric_device_bytes(void);
+/* Free and total device memory (hipMemGetInfo) of the library's device. */
+int lfric_device_memory(long long *free_bytes, long long *total_bytes);
 
 /* ---- function spaces (mesh_type + function_space_type inputs of the hot path) ---------- */
 /* Wrap a caller-held dofmap and colouring.  Replaces the reads of

@@ -55,6 +55,7 @@ SIGNATURES = {
     "lfric_timer_start": (_I, []),
     "lfric_timer_stop": (_I, [_DP]),
     "lfric_device_bytes": (ctypes.c_longlong, []),
+    "lfric_device_memory": (_I, [ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong)]),
     "lfric_fs_create": (_I, [_I, _I, _I, _I, _VP, _I, _VP, _VP, _I, ctypes.POINTER(_VP)]),
     "lfric_fs_create_cubedsphere": (_I, [_I, _I, _I, _I, ctypes.POINTER(_VP)]),
     "lfric_fs_create_sibling": (_I, [_VP, _I, _I, ctypes.POINTER(_VP)]),

@@ -351,6 +351,15 @@ int lfric_timer_stop(double *ms) {
 
 long long lfric_device_bytes(void) { return ctx().bytes; }
 
+int lfric_device_memory(long long *free_bytes, long long *total_bytes) {
+  if (int rc = require_init()) return rc;
+  size_t f = 0, t = 0;
+  LF_HIP(hipMemGetInfo(&f, &t));
+  if (free_bytes) *free_bytes = static_cast<long long>(f);
+  if (total_bytes) *total_bytes = static_cast<long long>(t);
+  return LFRIC_OK;
+}
+
 // ---- function spaces ------------------------------------------------------------------------
 int lfric_fs_create(int ncells, int nlayers, int ndf, int undf, const int *dofmap, int ncolours,
                     const int *ncells_per_colour, const int *cells_in_colour, int maxcc, lfric_fs **out) {
