@@ -107,6 +107,8 @@ def main():
                          "in one batch vs sequential FP64 solves; served: Poisson request stream with "
                          "continuous batching vs sequential FP64 solves")
     ap.add_argument("--problems", type=int, default=64)
+    ap.add_argument("--apply-form", type=int, default=0, choices=(0, 1),
+                    help="LMA apply inside the solvers: 0 colour-ordered scatter, 1 gather with fused p.Ap")
     ap.add_argument("--serve-window", type=float, default=20.0, help="seconds of arrivals")
     ap.add_argument("--serve-rate", type=float, default=200.0, help="requests per second")
     ap.add_argument("--serve-limit", type=float, default=30.0, help="wall seconds of admission per server")
@@ -124,6 +126,7 @@ def main():
     from lfric_core_amd.solvers import residual_norm
 
     L.init(0)
+    L.set_apply_form(args.apply_form)
     fs = L.FunctionSpace.cubedsphere(args.edge_cells, args.nlayers)
     op64 = L.LmaOperator(fs).generate_spd(0, scaled=True)
     op32 = op64.convert(L.R32)

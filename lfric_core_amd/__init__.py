@@ -19,7 +19,7 @@ Reference interfaces mirrored (paths relative to the lfric_core tree):
   solve                  the method factory of sci_mass_matrix_solver_alg_mod.x90:164-181
 """
 from ._lib import (LfricError, NotConverged, lib, init, synchronize, timer_start, timer_stop,
-                   device_bytes, R64, R32, BF16, F16, W0, W2, W3)
+                   device_bytes, set_apply_form, R64, R32, BF16, F16, W0, W2, W3)
 from .fem import FunctionSpace, FieldVector, LmaOperator
 from .solvers import (NullPreconditioner, DiagonalPreconditioner, ColumnJacobiPreconditioner,
                       LinearOperator, ConjugateGradient, MixedPrecisionCG, GMRES, MixedPrecisionGMRES,
@@ -28,7 +28,7 @@ from .solvers import (NullPreconditioner, DiagonalPreconditioner, ColumnJacobiPr
 
 __all__ = [
     "LfricError", "NotConverged", "lib", "init", "synchronize", "timer_start", "timer_stop",
-    "device_bytes", "R64", "R32", "BF16", "F16", "W0", "W2", "W3", "FunctionSpace", "FieldVector",
+    "device_bytes", "set_apply_form", "R64", "R32", "BF16", "F16", "W0", "W2", "W3", "FunctionSpace", "FieldVector",
     "LmaOperator", "NullPreconditioner", "DiagonalPreconditioner", "ColumnJacobiPreconditioner",
     "LinearOperator", "ConjugateGradient", "MixedPrecisionCG", "GMRES", "MixedPrecisionGMRES",
     "FGMRES", "GCR", "BiCGStab", "Jacobi", "Chebyshev", "PreconditionOnly", "solve", "METHODS",

@@ -55,6 +55,7 @@ SIGNATURES = {
     "lfric_timer_start": (_I, []),
     "lfric_timer_stop": (_I, [_DP]),
     "lfric_device_bytes": (ctypes.c_longlong, []),
+    "lfric_set_apply_form": (_I, [_I]),
     "lfric_device_memory": (_I, [ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong)]),
     "lfric_fs_create": (_I, [_I, _I, _I, _I, _VP, _I, _VP, _VP, _I, ctypes.POINTER(_VP)]),
     "lfric_fs_create_cubedsphere": (_I, [_I, _I, _I, _I, ctypes.POINTER(_VP)]),
@@ -187,3 +188,8 @@ def device_bytes():
 
 def ptr(a):
     return ctypes.c_void_p(a.ctypes.data)
+
+
+def set_apply_form(form):
+    """LMA apply used inside the solvers: 0 colour-ordered scatter, 1 gather (fused p.Ap in CG)."""
+    check(lib().lfric_set_apply_form(int(form)))

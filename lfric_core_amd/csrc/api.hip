@@ -257,17 +257,32 @@ static int cg_device(const CgArgs &a, SolverState &out) {
   if (a.x_zero) {
     if (int rc = launch_copy(a.prec, r, a.b, a.n)) return rc;
   } else {
-    if (int rc = launch_lma_apply(a.A, a.x, r, nullptr)) return rc;
+    if (int rc = launch_lma_apply_solver(a.A, a.x, r, nullptr)) return rc;
     if (int rc = launch_cg_residual(a.prec, r, a.b, a.n)) return rc;
   }
   if (int rc = launch_cg_init_norm(a.prec, r, a.n, g_state)) return rc;
   if (int rc = launch_cg_init_prec(a.prec, r, z, p, a.diag, a.n, g_state)) return rc;
   const int every = std::max(1, a.check_every);
   const int max_chunks = a.max_iter / every + 2;
+  // gather form: z = A p and the partial sums of p.z in one kernel
+  const bool fused = c.apply_form == 1 && a.A->to == a.A->from;
+  double *pz_partials = nullptr;
+  int pz_blocks = 0;
+  if (fused) {
+    if (int rc = launch_gather_blocks(a.A, &pz_blocks)) return rc;
+    void *w;
+    if (int rc = ws_get(4, sizeof(double) * static_cast<size_t>(pz_blocks), &w)) return rc;
+    pz_partials = static_cast<double *>(w);
+  }
   for (int ch = 0; ch < max_chunks; ++ch) {
     for (int i = 0; i < every; ++i) {
-      if (int rc = launch_lma_apply(a.A, p, z, &g_state->done)) return rc;
-      if (int rc = launch_cg_dot_pz(a.prec, p, z, a.n, g_state)) return rc;
+      if (fused) {
+        if (int rc = launch_lma_apply_gather_dot(a.A, p, z, pz_partials, &g_state->done)) return rc;
+        if (int rc = launch_cg_finish_pz(pz_partials, pz_blocks, g_state)) return rc;
+      } else {
+        if (int rc = launch_lma_apply_solver(a.A, p, z, &g_state->done)) return rc;
+        if (int rc = launch_cg_dot_pz(a.prec, p, z, a.n, g_state)) return rc;
+      }
       if (int rc = launch_cg_update(a.prec, a.x, r, z, p, a.diag, a.n, g_state)) return rc;
       if (int rc = launch_cg_aypx(a.prec, p, z, a.n, g_state)) return rc;
     }
@@ -282,7 +297,7 @@ static int cg_device(const CgArgs &a, SolverState &out) {
 static int true_residual(const lfric_op *A, const void *x, const void *b, long long n, int prec, double *nrm) {
   void *r;
   if (int rc = ws_get(3, esize(prec) * static_cast<size_t>(n), &r)) return rc;
-  if (int rc = launch_lma_apply(A, x, r, nullptr)) return rc;
+  if (int rc = launch_lma_apply_solver(A, x, r, nullptr)) return rc;
   if (int rc = launch_cg_residual(prec, r, b, n)) return rc;
   double *d_out = &g_state->red0;
   if (int rc = launch_dot(prec, r, r, n, d_out)) return rc;
@@ -350,6 +365,13 @@ int lfric_timer_stop(double *ms) {
 }
 
 long long lfric_device_bytes(void) { return ctx().bytes; }
+
+int lfric_set_apply_form(int form) {
+  if (int rc = require_init()) return rc;
+  LF_CHECK(form == 0 || form == 1, LFRIC_E_ARG, "apply form must be 0 (colour-ordered scatter) or 1 (gather)");
+  ctx().apply_form = form;
+  return LFRIC_OK;
+}
 
 int lfric_device_memory(long long *free_bytes, long long *total_bytes) {
   if (int rc = require_init()) return rc;
@@ -801,7 +823,7 @@ int lfric_ir_cg_solve(const lfric_op *A64, const lfric_op *A32, const lfric_prec
   if (int rc = ws_get(6, 4 * static_cast<size_t>(n), &d32)) return rc;
   // r = b - A x in FP64 and its norm
   auto residual = [&](double *nrm) -> int {
-    if (int rc = launch_lma_apply(A64, x->d, r64, nullptr)) return rc;
+    if (int rc = launch_lma_apply_solver(A64, x->d, r64, nullptr)) return rc;
     if (int rc = launch_cg_residual(LFRIC_R64, r64, b->d, n)) return rc;
     if (int rc = launch_dot(LFRIC_R64, r64, r64, n, &g_state->red0)) return rc;
     LF_HIP(hipMemcpyAsync(g_state_host, g_state, sizeof(SolverState), hipMemcpyDeviceToHost, ctx().stream));

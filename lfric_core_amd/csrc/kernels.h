@@ -35,6 +35,12 @@ int launch_op_convert(const lfric_op *src, lfric_op *dst);
 // operator kernels
 int launch_lma_apply(const lfric_op *op, const void *x, void *y, const int *d_skip);
 int launch_lma_apply_gather(const lfric_op *op, const void *x, void *y);
+// gather apply that also leaves one x.y partial sum per block in `partials` (square
+// operators); stops at once when *skip is set
+int launch_lma_apply_gather_dot(const lfric_op *op, const void *x, void *y, double *partials, const int *skip);
+int launch_gather_blocks(const lfric_op *op, int *nblocks);
+// the apply form the solvers use (lfric_set_apply_form)
+int launch_lma_apply_solver(const lfric_op *op, const void *x, void *y, const int *skip);
 int launch_lma_apply_mrhs(const lfric_op *op, const void *x, void *y, int nrhs, int variant);
 int launch_lma_transpose(const lfric_op *op, const void *x, void *y);
 int launch_dg_apply(const lfric_op *op, const void *x, void *y);
@@ -61,6 +67,7 @@ int launch_cg_dot_pz(int prec, const void *p, const void *z, long long n, Solver
 int launch_cg_update(int prec, void *x, void *r, void *z, const void *p, const void *diag,
                      long long n, SolverState *st);
 int launch_cg_aypx(int prec, void *p, const void *z, long long n, SolverState *st);
+int launch_cg_finish_pz(const double *partials, int nblocks, SolverState *st);
 
 // abstract operators and preconditioners (krylov.hip)
 int linop_apply(const lfric_linop *L, const void *x, void *y);

@@ -341,35 +341,48 @@ __device__ inline T row_dot(const T *__restrict__ A, size_t gs, long long e, int
 // output space sums, over the cells of its column in colour order, the top-partner row of
 // layer m-1 and then the bottom row of layer m: the order in which the reference's coloured
 // loop adds into lhs(map1(df)+k) (matrix_vector_kernel_mod.F90:130-138, k outer).  Every
-// output dof is written once, with no read-modify-write and no zero fill.
-template <typename T, int N2, class R>
+// output dof is written once, with no read-modify-write and no zero fill.  With DOT the
+// kernel also leaves one partial sum of x.y per block (square operators: the CG's p.(A p)),
+// which a one-block kernel adds in block order.  `skip` (a solver's done flag) stops it.
+template <typename T, int N2, class R, bool DOT>
 __global__ __launch_bounds__(kBlock) void k_lma_gather(const T *__restrict__ A, size_t gs, const int *__restrict__ ebase,
                                                        const T *__restrict__ x, T *__restrict__ y,
                                                        const int *__restrict__ map2, const int2 *__restrict__ cols,
                                                        const int *__restrict__ off, const int *__restrict__ ent,
-                                                       int ncols, int nl, R roles) {
+                                                       int ncols, int nl, R roles, double *partials,
+                                                       const int *skip) {
+  if (skip && *skip) return;
   const int H = nl + 1;
   const long long t = static_cast<long long>(blockIdx.x) * kBlock + threadIdx.x;
-  if (t >= static_cast<long long>(ncols) * H) return;
-  const int col = static_cast<int>(t / H);
-  const int m = static_cast<int>(t - static_cast<long long>(col) * H);
-  const int2 ci = cols[col];
-  if (m == nl && !ci.y) return;
-  const bool has_m = m < nl, has_p = m > 0;
-  T acc = T(0);
-  const int q1 = off[col + 1];
-  for (int q = off[col]; q < q1; ++q) {
-    const int en = ent[q];
-    const int pos = en >> 5, i = en & 31;
-    const int pt = roles.partner(i);
-    const long long e = static_cast<long long>(ebase[pos]) + m;
-    int b[N2];
-#pragma unroll
-    for (int j = 0; j < N2; ++j) b[j] = map2[static_cast<size_t>(pos) * N2 + j];
-    if (pt >= 0 && has_p) acc = row_dot<T, N2>(A, gs, e, pt, x, b, m - 1, acc);
-    if (has_m) acc = row_dot<T, N2>(A, gs, e, i, x, b, m, acc);
+  bool valid = t < static_cast<long long>(ncols) * H;
+  const int col = valid ? static_cast<int>(t / H) : 0;
+  const int m = valid ? static_cast<int>(t - static_cast<long long>(col) * H) : 0;
+  int2 ci = make_int2(0, 0);
+  if (valid) {
+    ci = cols[col];
+    if (m == nl && !ci.y) valid = false;
   }
-  y[static_cast<size_t>(ci.x) + m] = acc;
+  T acc = T(0);
+  if (valid) {
+    const bool has_m = m < nl, has_p = m > 0;
+    const int q1 = off[col + 1];
+    for (int q = off[col]; q < q1; ++q) {
+      const int en = ent[q];
+      const int pos = en >> 5, i = en & 31;
+      const int pt = roles.partner(i);
+      const long long e = static_cast<long long>(ebase[pos]) + m;
+      int b[N2];
+#pragma unroll
+      for (int j = 0; j < N2; ++j) b[j] = map2[static_cast<size_t>(pos) * N2 + j];
+      if (pt >= 0 && has_p) acc = row_dot<T, N2>(A, gs, e, pt, x, b, m - 1, acc);
+      if (has_m) acc = row_dot<T, N2>(A, gs, e, i, x, b, m, acc);
+    }
+    y[static_cast<size_t>(ci.x) + m] = acc;
+  }
+  if constexpr (DOT) {
+    double v[1] = {valid ? to_d(acc) * to_d(x[static_cast<size_t>(ci.x) + m]) : 0.0};
+    block_partials<1>(v, partials);
+  }
 }
 
 // ---- several right-hand sides ------------------------------------------------------------
@@ -1097,39 +1110,61 @@ static int build_gather(FunctionSpace &f) {
   return LFRIC_OK;
 }
 
+int gather_blocks(const lfric_op *op) {
+  const FunctionSpace &to = *op->to;
+  return grid_for(static_cast<long long>(to.gather_ncols) * (to.nlayers + 1));
+}
+
+struct GatherExtra {
+  double *partials = nullptr;  // non-null: fused x.y partial sums, one per block
+  const int *skip = nullptr;
+};
+
 template <typename T, int N2, class R>
-static void launch_gather_r(const lfric_op *op, const T *x, T *y, R roles) {
+static void launch_gather_r(const lfric_op *op, const T *x, T *y, R roles, GatherExtra ex) {
   const FunctionSpace &to = *op->to, &from = *op->from;
-  const int grid = grid_for(static_cast<long long>(to.gather_ncols) * (to.nlayers + 1));
-  LAUNCH((k_lma_gather<T, N2, R>), grid, static_cast<const T *>(op->d), op->gs, to.d_ebase, x, y, from.d_map,
-         to.d_gcol, to.d_goff, to.d_gent, to.gather_ncols, to.nlayers, roles);
+  const int grid = gather_blocks(op);
+  if (ex.partials)
+    LAUNCH((k_lma_gather<T, N2, R, true>), grid, static_cast<const T *>(op->d), op->gs, to.d_ebase, x, y, from.d_map,
+           to.d_gcol, to.d_goff, to.d_gent, to.gather_ncols, to.nlayers, roles, ex.partials, ex.skip);
+  else
+    LAUNCH((k_lma_gather<T, N2, R, false>), grid, static_cast<const T *>(op->d), op->gs, to.d_ebase, x, y,
+           from.d_map, to.d_gcol, to.d_goff, to.d_gent, to.gather_ncols, to.nlayers, roles, ex.partials, ex.skip);
 }
 
 template <typename T, int N2>
-static void launch_gather_n(RoleKind rk, const lfric_op *op, const T *x, T *y) {
+static void launch_gather_n(RoleKind rk, const lfric_op *op, const T *x, T *y, GatherExtra ex) {
   switch (rk) {
-    case RoleKind::W0K0: launch_gather_r<T, N2>(op, x, y, RolesW0K0{}); break;
-    case RoleKind::W0K1: launch_gather_r<T, N2>(op, x, y, RolesW0K1{}); break;
-    case RoleKind::W2K0: launch_gather_r<T, N2>(op, x, y, RolesW2K0{}); break;
-    case RoleKind::None: launch_gather_r<T, N2>(op, x, y, RolesNone{}); break;
-    default: launch_gather_r<T, N2>(op, x, y, RolesRuntime{roles_of(*op->to)});
+    case RoleKind::W0K0: launch_gather_r<T, N2>(op, x, y, RolesW0K0{}, ex); break;
+    case RoleKind::W0K1: launch_gather_r<T, N2>(op, x, y, RolesW0K1{}, ex); break;
+    case RoleKind::W2K0: launch_gather_r<T, N2>(op, x, y, RolesW2K0{}, ex); break;
+    case RoleKind::None: launch_gather_r<T, N2>(op, x, y, RolesNone{}, ex); break;
+    default: launch_gather_r<T, N2>(op, x, y, RolesRuntime{roles_of(*op->to)}, ex);
   }
 }
 
 template <typename T>
-static int gather_t(const lfric_op *op, const T *x, T *y) {
+static int gather_t(const lfric_op *op, const T *x, T *y, GatherExtra ex) {
   if (int rc = build_gather(*op->to)) return rc;
   const RoleKind rk = role_kind(*op->to);
   switch (op->from->ndf) {
-    case 1: launch_gather_n<T, 1>(rk, op, x, y); break;
-    case 2: launch_gather_n<T, 2>(rk, op, x, y); break;
-    case 6: launch_gather_n<T, 6>(rk, op, x, y); break;
-    case 8: launch_gather_n<T, 8>(rk, op, x, y); break;
-    case 27: launch_gather_n<T, 27>(rk, op, x, y); break;
+    case 1: launch_gather_n<T, 1>(rk, op, x, y, ex); break;
+    case 2: launch_gather_n<T, 2>(rk, op, x, y, ex); break;
+    case 6: launch_gather_n<T, 6>(rk, op, x, y, ex); break;
+    case 8: launch_gather_n<T, 8>(rk, op, x, y, ex); break;
+    case 27: launch_gather_n<T, 27>(rk, op, x, y, ex); break;
     default: return fail(LFRIC_E_UNSUPPORTED, "lma_apply_gather: unsupported ndf_from");
   }
   CHECK_LAUNCH();
   return LFRIC_OK;
+}
+
+static int gather_any(const lfric_op *op, const void *x, void *y, GatherExtra ex) {
+  if (op->precision == LFRIC_R64)
+    return gather_t<double>(op, static_cast<const double *>(x), static_cast<double *>(y), ex);
+  if (op->precision == LFRIC_R32)
+    return gather_t<float>(op, static_cast<const float *>(x), static_cast<float *>(y), ex);
+  return fail(LFRIC_E_UNSUPPORTED, "lma_apply_gather: operator precision");
 }
 
 // ---- multi-RHS dispatch ---------------------------------------------------------------------
@@ -1215,12 +1250,20 @@ int launch_lma_apply_mrhs(const lfric_op *op, const void *x, void *y, int nrhs, 
   return mrhs_t<float>(variant, nrhs, op, static_cast<const float *>(x), static_cast<float *>(y));
 }
 
-int launch_lma_apply_gather(const lfric_op *op, const void *x, void *y) {
-  if (op->precision == LFRIC_R64)
-    return gather_t<double>(op, static_cast<const double *>(x), static_cast<double *>(y));
-  if (op->precision == LFRIC_R32)
-    return gather_t<float>(op, static_cast<const float *>(x), static_cast<float *>(y));
-  return fail(LFRIC_E_UNSUPPORTED, "lma_apply_gather: operator precision");
+int launch_lma_apply_gather(const lfric_op *op, const void *x, void *y) { return gather_any(op, x, y, GatherExtra{}); }
+
+int launch_lma_apply_gather_dot(const lfric_op *op, const void *x, void *y, double *partials, const int *skip) {
+  return gather_any(op, x, y, GatherExtra{partials, skip});
+}
+
+int launch_lma_apply_solver(const lfric_op *op, const void *x, void *y, const int *skip) {
+  return ctx().apply_form == 1 ? gather_any(op, x, y, GatherExtra{nullptr, skip}) : launch_lma_apply(op, x, y, skip);
+}
+
+int launch_gather_blocks(const lfric_op *op, int *nblocks) {
+  if (int rc = build_gather(*op->to)) return rc;
+  *nblocks = gather_blocks(op);
+  return LFRIC_OK;
 }
 
 template <typename T>
@@ -1472,6 +1515,13 @@ int launch_cg_dot_pz(int prec, const void *p, const void *z, long long n, Solver
     LAUNCH(k_cg_dot_pz<float>, kRedBlocks, static_cast<const float *>(p), static_cast<const float *>(z), n, st,
            ctx().d_partials);
   LAUNCH((k_finish<1, kEpiDotPz>), 1, ctx().d_partials, kRedBlocks, st, static_cast<double *>(nullptr));
+  CHECK_LAUNCH();
+  return LFRIC_OK;
+}
+
+// alpha from partial sums of p.(A p) left by the fused gather apply
+int launch_cg_finish_pz(const double *partials, int nblocks, SolverState *st) {
+  LAUNCH((k_finish<1, kEpiDotPz>), 1, partials, nblocks, st, static_cast<double *>(nullptr));
   CHECK_LAUNCH();
   return LFRIC_OK;
 }

@@ -262,7 +262,7 @@ static size_t esz(int p) { return p == LFRIC_R64 ? 8 : p == LFRIC_R32 ? 4 : 2; }
 #define LAUNCHK(kernel, grid, block, ...) hipLaunchKernelGGL((kernel), dim3(grid), dim3(block), 0, ctx().stream, __VA_ARGS__)
 
 int linop_apply(const lfric_linop *L, const void *x, void *y) {
-  if (L->kind == 0) return launch_lma_apply(L->A, x, y, nullptr);
+  if (L->kind == 0) return launch_lma_apply_solver(L->A, x, y, nullptr);
   // y = M3 x + tau * D (D^T x / m2)
   const long long n2 = L->D->from->undf, n3 = L->space->undf;
   if (int rc = launch_lma_transpose(L->D, x, L->t_w2)) return rc;

@@ -181,13 +181,18 @@ def _config0(hip, orc, n=12, nl=20, scaled=False):
     return fs, m, op, b
 
 
-def test_cg_fp64_matches_oracle(hip, orc):
+@pytest.mark.parametrize("form", [0, 1])
+def test_cg_fp64_matches_oracle(hip, orc, form):
     fs, m, op, b = _config0(hip, orc)
     prec = hip.DiagonalPreconditioner.from_operator(op)
     x = hip.FieldVector(fs)
     x.set_scalar(0.0)
     cg = hip.ConjugateGradient(op, prec, 1e-8, 1e-16, 500, check_every=4)
-    res = cg.apply(x, b)
+    hip.set_apply_form(form)  # 1: gather apply with the fused p.(A p)
+    try:
+        res = cg.apply(x, b)
+    finally:
+        hip.set_apply_form(0)
     A = orc.gen_spd(8, m.ncell_3d, 0)
     bo = orc.gen_uniform(m.undf, 0, 3)
     np.testing.assert_array_equal(b.export_field(), bo)
@@ -201,13 +206,18 @@ def test_cg_fp64_matches_oracle(hip, orc):
     assert np.linalg.norm(bo - S @ xg) / np.linalg.norm(bo) <= 1.01e-8
 
 
-def test_mixed_precision_refinement_reaches_fp64_accuracy(hip, orc):
+@pytest.mark.parametrize("form", [0, 1])
+def test_mixed_precision_refinement_reaches_fp64_accuracy(hip, orc, form):
     fs, m, op, b = _config0(hip, orc, 12, 20, scaled=True)
     op32 = op.convert(hip.R32)
     prec32 = hip.DiagonalPreconditioner.from_operator(op32)
     x = hip.FieldVector(fs)
     solver = hip.MixedPrecisionCG(op, op32, prec32, 1e-8, 0.0, 5000, inner_r_tol=1e-4)
-    res = solver.apply(x, b)
+    hip.set_apply_form(form)
+    try:
+        res = solver.apply(x, b)
+    finally:
+        hip.set_apply_form(0)
     assert res.status == 0 and res.true_rel_res <= 1e-8 and res.outer_iterations >= 2
     A = orc.gen_spd(8, m.ncell_3d, 0, scaled=True)
     bo = orc.gen_uniform(m.undf, 0, 3)
