@@ -11,6 +11,11 @@ module lfric_hip_capi_mod
                                        LFRIC_BREAKDOWN = 3
   integer(c_int), parameter, public :: LFRIC_R64 = 0, LFRIC_R32 = 1, LFRIC_BF16 = 2, LFRIC_F16 = 3
   integer(c_int), parameter, public :: LFRIC_W0 = 0, LFRIC_W2 = 2, LFRIC_W3 = 3
+  ! solver methods of lfric_solve (the method enumeration of the mass-matrix solver factory)
+  integer(c_int), parameter, public :: LFRIC_METHOD_CG = 1, LFRIC_METHOD_BICGSTAB = 2, &
+                                       LFRIC_METHOD_GMRES = 3, LFRIC_METHOD_FGMRES = 4, &
+                                       LFRIC_METHOD_GCR = 5, LFRIC_METHOD_PREC_ONLY = 6, &
+                                       LFRIC_METHOD_JACOBI = 7, LFRIC_METHOD_CHEBYSHEV = 8
 
   type, bind(c), public :: lfric_solver_opts
     real(c_double) :: r_tol
@@ -48,7 +53,8 @@ module lfric_hip_capi_mod
   public :: lfric_prec_create_diagonal, lfric_prec_create_identity, lfric_prec_apply, &
             lfric_prec_destroy
   public :: lfric_linop_create_lma, lfric_linop_destroy, lfric_prec_create_column_jacobi
-  public :: lfric_solver_opts_default, lfric_cg_solve, lfric_ir_cg_solve, lfric_gmres_solve
+  public :: lfric_solver_opts_default, lfric_cg_solve, lfric_ir_cg_solve, lfric_gmres_solve, &
+            lfric_solve
 
   interface
     integer(c_int) function lfric_init(device) bind(c, name="lfric_init")
@@ -253,6 +259,13 @@ module lfric_hip_capi_mod
     end function
     integer(c_int) function lfric_gmres_solve(a, p, x, b, o, r) bind(c, name="lfric_gmres_solve")
       import :: c_int, c_ptr, lfric_solver_opts, lfric_solver_result
+      type(c_ptr), value :: a, p, x, b
+      type(lfric_solver_opts), intent(in) :: o
+      type(lfric_solver_result), intent(out) :: r
+    end function
+    integer(c_int) function lfric_solve(method, a, p, x, b, o, r) bind(c, name="lfric_solve")
+      import :: c_int, c_ptr, lfric_solver_opts, lfric_solver_result
+      integer(c_int), value :: method
       type(c_ptr), value :: a, p, x, b
       type(lfric_solver_opts), intent(in) :: o
       type(lfric_solver_result), intent(out) :: r

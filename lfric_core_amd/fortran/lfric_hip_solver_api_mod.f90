@@ -101,7 +101,18 @@ module lfric_hip_solver_api_mod
     procedure :: apply => hip_mixed_cg_apply
   end type hip_mixed_cg_type
 
-  public :: hip_lma_operator, hip_diagonal_preconditioner, hip_conjugate_gradient, hip_mixed_cg
+  ! any method of the mass-matrix solver factory, run by the library (lfric_solve)
+  type, public, extends(abstract_iterative_solver_type) :: hip_factory_solver_type
+    integer(c_int) :: method = LFRIC_METHOD_CG
+    type(c_ptr) :: op_h = c_null_ptr, prec_h = c_null_ptr
+    type(lfric_solver_opts) :: opts
+    type(lfric_solver_result) :: result
+  contains
+    procedure :: apply => hip_factory_apply
+  end type hip_factory_solver_type
+
+  public :: hip_lma_operator, hip_diagonal_preconditioner, hip_conjugate_gradient, hip_mixed_cg, &
+            hip_factory_solver
 
 contains
 
@@ -416,5 +427,54 @@ contains
       call log_event("hip_mixed_cg_type: x is not a hip_field_vector_type", LOG_LEVEL_ERROR)
     end select
   end subroutine hip_mixed_cg_apply
+
+  !> The solver of sci_mass_matrix_solver_alg_mod.x90:164-283 for `method` (LFRIC_METHOD_*),
+  !> with the options of its constructor there; gcrk for GMRES / FGMRES / GCR, rho_relax for
+  !> Jacobi, lmin / lmax for Chebyshev.
+  function hip_factory_solver(method, op, prec, r_tol, a_tol, max_iter, monitor_convergence, &
+                              fail_on_non_converged, gcrk, rho_relax, lmin, lmax) result(self)
+    integer(c_int), intent(in) :: method
+    type(hip_lma_operator_type), intent(in) :: op
+    type(hip_diagonal_preconditioner_type), intent(in) :: prec
+    real(r_def), intent(in) :: r_tol, a_tol
+    integer(i_def), intent(in) :: max_iter
+    logical(l_def), intent(in) :: monitor_convergence, fail_on_non_converged
+    integer(i_def), intent(in), optional :: gcrk
+    real(r_def), intent(in), optional :: rho_relax, lmin, lmax
+    type(hip_factory_solver_type) :: self
+    self%method = method
+    self%op_h = op%op
+    self%prec_h = prec%p
+    call lfric_solver_opts_default(self%opts)
+    self%opts%r_tol = r_tol
+    self%opts%a_tol = a_tol
+    self%opts%max_iter = max_iter
+    self%opts%monitor_convergence = merge(1, 0, monitor_convergence)
+    self%opts%fail_on_non_converged = merge(1, 0, fail_on_non_converged)
+    if (present(gcrk)) self%opts%gcrk = gcrk
+    if (present(rho_relax)) self%opts%rho_relax = rho_relax
+    if (present(lmin)) self%opts%lmin = lmin
+    if (present(lmax)) self%opts%lmax = lmax
+  end function hip_factory_solver
+
+  subroutine hip_factory_apply(self, x, b)
+    class(hip_factory_solver_type), intent(inout) :: self
+    class(abstract_vector_type), intent(inout) :: x
+    class(abstract_vector_type), intent(inout) :: b
+    integer(c_int) :: rc
+    type(c_ptr) :: linop
+    select type (x)
+    type is (hip_field_vector_type)
+      call x%own()
+      ! the abstract linear operator over the LMA operator lives for this solve only, so copies
+      ! of the solver object own nothing
+      call hip_check(lfric_linop_create_lma(self%op_h, linop), "linop")
+      rc = lfric_solve(self%method, linop, self%prec_h, x%h, handle_of(b, "solve"), self%opts, self%result)
+      call lfric_linop_destroy(linop)
+      call report(rc, self%result, "hip solver")
+    class default
+      call log_event("hip_factory_solver_type: x is not a hip_field_vector_type", LOG_LEVEL_ERROR)
+    end select
+  end subroutine hip_factory_apply
 
 end module lfric_hip_solver_api_mod

@@ -1,7 +1,8 @@
 !> abstract_iterative_solver_type of components/science/source/solver/
 !> sci_iterative_solver_mod.f90:38-57 and, for building the shim without LFRic, its
-!> conjugate_gradient_type (:87-121, cg_solve :495-592) and gmres_type (:161-196, gmres_solve
-!> :830-1003) in the same operation order, written against the abstract types only.
+!> conjugate_gradient_type (:87-121, cg_solve :495-592), bicgstab_type (:123-156,
+!> bicgstab_solve :645-779) and gmres_type (:161-196, gmres_solve :830-1003) in the same
+!> operation order, written against the abstract types only.
 module sci_iterative_solver_mod
   use constants_mod,          only : r_def, i_def, l_def, EPS
   use vector_mod,             only : abstract_vector_type
@@ -48,6 +49,17 @@ module sci_iterative_solver_mod
   type :: array_abstract_vector_type
     class(abstract_vector_type), allocatable :: vt
   end type array_abstract_vector_type
+
+  type, public, extends(abstract_iterative_solver_type) :: bicgstab_type
+    private
+    integer(kind=i_def), public :: iterations = 0
+  contains
+    procedure :: apply => bicgstab_solve
+  end type bicgstab_type
+
+  interface bicgstab_type
+    module procedure bicgstab_constructor
+  end interface
 
   type, public, extends(abstract_iterative_solver_type) :: gmres_type
     private
@@ -133,6 +145,91 @@ contains
       end if
     end if
   end subroutine cg_solve
+
+  function bicgstab_constructor(lin_op, prec, r_tol, a_tol, max_iter, monitor_convergence, &
+                                fail_on_non_converged) result(self)
+    class(abstract_linear_operator_type), target, intent(in) :: lin_op
+    class(abstract_preconditioner_type),  target, intent(in) :: prec
+    real(kind=r_def),    intent(in) :: r_tol, a_tol
+    integer(kind=i_def), intent(in) :: max_iter
+    logical(kind=l_def), intent(in) :: monitor_convergence, fail_on_non_converged
+    type(bicgstab_type) :: self
+    self%lin_op => lin_op
+    self%prec => prec
+    self%r_tol = r_tol
+    self%a_tol = a_tol
+    self%max_iter = max_iter
+    self%monitor_convergence = monitor_convergence
+    self%fail_on_non_converged = fail_on_non_converged
+  end function bicgstab_constructor
+
+  subroutine bicgstab_solve(self, x, b)
+    class(bicgstab_type),        intent(inout) :: self
+    class(abstract_vector_type), intent(inout) :: x
+    class(abstract_vector_type), intent(inout) :: b
+    class(abstract_vector_type), allocatable :: r, r0, p, v, t, s, y, z
+    real(kind=r_def) :: alpha, beta, rho, omega, rho_old, err, sc_err, tt, ts
+    integer(kind=i_def) :: iter
+
+    call x%duplicate(r)
+    call r%copy(b)
+    call x%duplicate(v)
+    call v%set_scalar(0.0_r_def)
+    call self%lin_op%apply(x, v)
+    call r%axpy(-1.0_r_def, v)
+    call r%duplicate(r0)
+    call r0%copy(r)
+    sc_err = r%norm()
+    self%iterations = 0
+    if (sc_err < EPS) return
+    sc_err = max(sc_err, self%a_tol)
+    alpha = 1.0_r_def
+    omega = 1.0_r_def
+    rho_old = 1.0_r_def
+    call x%duplicate(p)
+    call x%duplicate(s)
+    call x%duplicate(t)
+    call v%set_scalar(0.0_r_def)
+    call p%set_scalar(0.0_r_def)
+    call x%duplicate(y)
+    call x%duplicate(z)
+    err = huge(1.0_r_def)
+    do iter = 1, self%max_iter
+      rho = r%dot(r0)
+      beta = (rho / rho_old) * (alpha / omega)
+      call t%copy(r)
+      call t%axpy(-beta * omega, v)
+      call self%prec%apply(t, y)
+      call p%aypx(beta, y)
+      call self%lin_op%apply(p, v)
+      alpha = rho / r0%dot(v)
+      call s%copy(r)
+      call s%axpy(-alpha, v)
+      call self%prec%apply(s, z)
+      call self%lin_op%apply(z, t)
+      tt = t%dot(t)
+      ts = t%dot(s)
+      omega = ts / tt
+      call x%axpy(omega, z)
+      call x%axpy(alpha, p)
+      call r%copy(s)
+      call r%axpy(-omega, t)
+      rho_old = rho
+      self%iterations = iter
+      if (self%monitor_convergence) then
+        err = r%norm() / sc_err
+        if (err < self%r_tol) exit
+      end if
+    end do
+    if (self%monitor_convergence .and. err >= self%r_tol) then
+      write(log_scratch_space, '(A,I4,A,E15.8)') "bicgstab: NOT converged in", self%max_iter, " iters, Res=", err
+      if (self%fail_on_non_converged) then
+        call log_event(log_scratch_space, LOG_LEVEL_ERROR)
+      else
+        call log_event(log_scratch_space, LOG_LEVEL_INFO)
+      end if
+    end if
+  end subroutine bicgstab_solve
 
   function gmres_constructor(lin_op, prec, gcrk, r_tol, a_tol, max_iter, monitor_convergence, &
                              fail_on_non_converged) result(self)

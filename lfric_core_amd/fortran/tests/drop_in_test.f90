@@ -1,25 +1,28 @@
 !> Drop-in check of the HIP solver-API types: the stock conjugate_gradient_type and gmres_type
 !> (written against the abstract types) run unmodified on hip_field_vector_type /
 !> hip_lma_operator_type / hip_diagonal_preconditioner_type, and the device-resident
-!> hip_conjugate_gradient_type gives the same solve.  Prints one JSON line; stops with a
-!> non-zero code when the paths disagree.
+!> hip_conjugate_gradient_type gives the same solve; the stock bicgstab_type and the library's
+!> BiCGStab behind hip_factory_solver_type (lfric_solve) agree.  Prints one JSON line; stops
+!> with a non-zero code when the paths disagree.
 program drop_in_test
   use iso_c_binding
   use constants_mod,            only : r_def, i_def
   use lfric_hip_capi_mod
   use lfric_hip_solver_api_mod
-  use sci_iterative_solver_mod, only : conjugate_gradient_type, gmres_type
+  use sci_iterative_solver_mod, only : conjugate_gradient_type, gmres_type, bicgstab_type
   implicit none
   type(hip_function_space_type) :: fs
   type(c_ptr) :: op
-  type(hip_field_vector_type) :: b, diag, x_cg, x_hip, x_gm
+  type(hip_field_vector_type) :: b, diag, x_cg, x_hip, x_gm, x_bs, x_fbs
   type(hip_lma_operator_type) :: A
   type(hip_diagonal_preconditioner_type) :: P
   type(conjugate_gradient_type) :: cg
   type(gmres_type) :: gm
   type(hip_conjugate_gradient_type) :: hcg
-  real(r_def), allocatable :: a1(:), a2(:), a3(:)
-  real(r_def) :: d_cg, d_gm
+  type(bicgstab_type) :: bs
+  type(hip_factory_solver_type) :: fbs
+  real(r_def), allocatable :: a1(:), a2(:), a3(:), a4(:), a5(:)
+  real(r_def) :: d_cg, d_gm, d_bs
   integer :: n, nl
 
   n = 12
@@ -50,14 +53,30 @@ program drop_in_test
   gm = gmres_type(A, P, 10, 1.0e-10_r_def, 0.0_r_def, 50, .true., .true.)
   call gm%apply(x_gm, b)
 
-  allocate(a1(fs%undf), a2(fs%undf), a3(fs%undf))
+  call x_bs%initialise(fs)
+  call x_bs%set_scalar(0.0_r_def)
+  bs = bicgstab_type(A, P, 1.0e-10_r_def, 1.0e-16_r_def, 300, .true., .true.)
+  call bs%apply(x_bs, b)
+
+  call x_fbs%initialise(fs)
+  call x_fbs%set_scalar(0.0_r_def)
+  fbs = hip_factory_solver(LFRIC_METHOD_BICGSTAB, A, P, 1.0e-10_r_def, 1.0e-16_r_def, 300, .true., .true.)
+  call fbs%apply(x_fbs, b)
+
+  allocate(a1(fs%undf), a2(fs%undf), a3(fs%undf), a4(fs%undf), a5(fs%undf))
   call x_cg%export_field(a1)
   call x_hip%export_field(a2)
   call x_gm%export_field(a3)
+  call x_bs%export_field(a4)
+  call x_fbs%export_field(a5)
   d_cg = norm2(a1 - a2) / norm2(a2)
   d_gm = norm2(a3 - a2) / norm2(a2)
-  write(*, '(A,I0,A,I0,A,I0,A,ES12.5,A,ES12.5,A,I0,A)') '{"undf": ', fs%undf, ', "stock_cg_iterations": ', &
-      cg%iterations, ', "hip_cg_iterations": ', hcg%result%iterations, ', "stock_vs_hip_cg_rel_diff": ', &
-      d_cg, ', "stock_gmres_vs_hip_cg_rel_diff": ', d_gm, ', "gmres_cycles": ', gm%cycles, '}'
+  d_bs = norm2(a4 - a5) / norm2(a5)
+  write(*, '(A,I0,A,I0,A,I0,A,ES12.5,A,ES12.5,A,I0,A,I0,A,I0,A,ES12.5,A)') '{"undf": ', fs%undf, &
+      ', "stock_cg_iterations": ', cg%iterations, ', "hip_cg_iterations": ', hcg%result%iterations, &
+      ', "stock_vs_hip_cg_rel_diff": ', d_cg, ', "stock_gmres_vs_hip_cg_rel_diff": ', d_gm, &
+      ', "gmres_cycles": ', gm%cycles, ', "stock_bicgstab_iterations": ', bs%iterations, &
+      ', "hip_bicgstab_iterations": ', fbs%result%iterations, ', "stock_vs_hip_bicgstab_rel_diff": ', d_bs, '}'
   if (cg%iterations /= hcg%result%iterations .or. d_cg > 1.0e-10_r_def .or. d_gm > 1.0e-7_r_def) error stop 2
+  if (bs%iterations /= fbs%result%iterations .or. d_bs > 1.0e-9_r_def) error stop 3
 end program drop_in_test

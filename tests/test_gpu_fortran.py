@@ -26,3 +26,8 @@ def test_fortran_drop_in(orc):
     assert r["stock_cg_iterations"] == r["hip_cg_iterations"] == reso.iterations
     assert r["stock_vs_hip_cg_rel_diff"] <= 1e-10
     assert r["stock_gmres_vs_hip_cg_rel_diff"] <= 1e-7
+    xb, resb, _ = orc.solve_callbacks("bicgstab", m.undf, lambda v: orc.lma_apply(m, A, v),
+                                      lambda v: v / orc.mm_diagonal(m, A), orc.gen_uniform(m.undf, 0, 3),
+                                      r_tol=1e-10, max_iter=300)
+    assert r["stock_bicgstab_iterations"] == r["hip_bicgstab_iterations"] == resb.iterations
+    assert r["stock_vs_hip_bicgstab_rel_diff"] <= 1e-9
