@@ -27,9 +27,9 @@ sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per FP32 apply of C96x70 W0 from the rocprofv3 FETCH_SIZE / WRITE_SIZE runs recorded in
-# profiles/lma_apply_cg_c96x70_w0.md (4 colour launches x (288.4 MB fetched + 16.8 MB written),
-# counters scaled by a plain copy of known size).  Re-measure when the kernel changes.
-TRAFFIC_BYTES_PER_APPLY_FP32 = {(96, 70): 4 * (288.4e6 + 16.8e6)}
+# profiles/ (counters scaled by a plain copy of known size), per apply form: the colour-ordered
+# scatter (4 launches) and the gather kernel.  Re-measure when a kernel changes.
+TRAFFIC_BYTES_PER_APPLY_FP32 = {("colour", 96, 70): 4 * (288.4e6 + 16.8e6)}
 
 
 def apply_bytes(fs, ndf, s_a, s_x):
@@ -107,7 +107,7 @@ def main():
                          "in one batch vs sequential FP64 solves; served: Poisson request stream with "
                          "continuous batching vs sequential FP64 solves")
     ap.add_argument("--problems", type=int, default=64)
-    ap.add_argument("--apply-form", type=int, default=0, choices=(0, 1),
+    ap.add_argument("--apply-form", type=int, default=1, choices=(0, 1),
                     help="LMA apply inside the solvers: 0 colour-ordered scatter, 1 gather with fused p.Ap")
     ap.add_argument("--serve-window", type=float, default=20.0, help="seconds of arrivals")
     ap.add_argument("--serve-rate", type=float, default=200.0, help="requests per second")
@@ -185,8 +185,9 @@ def main():
     # LMA apply bandwidth (HIP events on the library stream)
     xs32, ys32 = L.FieldVector(fs, L.R32).fill_uniform(1, 3), L.FieldVector(fs, L.R32)
     xs64, ys64 = L.FieldVector(fs).fill_uniform(1, 3), L.FieldVector(fs)
-    t32 = time_applies(L, op32, xs32, ys32, args.apply_reps)
-    t64 = time_applies(L, op64, xs64, ys64, args.apply_reps)
+    method = "apply_gather" if args.apply_form == 1 else "apply"   # the form the solvers used
+    t32 = time_applies(L, op32, xs32, ys32, args.apply_reps, method)
+    t64 = time_applies(L, op64, xs64, ys64, args.apply_reps, method)
     b32 = apply_bytes(fs, 8, 4, 4)
     b64 = apply_bytes(fs, 8, 8, 8)
     gbs32 = b32 / (t32 * 1e-3) / 1e9
@@ -224,10 +225,12 @@ def main():
                     "fp32_bytes": b32, "fp64_bytes": b64},
         "roofline": {"bound": "hbm", "achieved": gbs32, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": gbs32 / HBM_PEAK_GBS,
-                     "traffic": TRAFFIC_BYTES_PER_APPLY_FP32.get((args.edge_cells, args.nlayers)),
+                     "traffic": TRAFFIC_BYTES_PER_APPLY_FP32.get(
+                         ("gather" if args.apply_form == 1 else "colour", args.edge_cells, args.nlayers)),
                      "algorithmic_bytes": b32,
-                     "kernel": "k_lma_colour<float,8,8,RolesW0K0>; bytes, time and traffic per apply "
-                               "(4 colour launches)"},
+                     "kernel": ("k_lma_gather<float,8,RolesW0K0> (one launch per apply)" if args.apply_form == 1
+                                else "k_lma_colour<float,8,8,RolesW0K0> (4 colour launches per apply)")
+                               + "; bytes, time and traffic per apply"},
     }
     if not args.no_cpu_baseline:
         bh = b.export_field()
@@ -257,6 +260,8 @@ def bench_batched(args):
     L.init(0)
     mix = batched_mix(args.problems)
     probs = []
+    say = lambda msg: print(f"[batched] {msg}", file=sys.stderr, flush=True)
+    say(f"building {len(mix)} problems")
     for (n, nl, order, nrhs, seed) in mix:
         fs = L.FunctionSpace.cubedsphere(n, nl, L.W0, order)
         op64 = L.LmaOperator(fs).generate_spd(seed, scaled=True)
@@ -267,8 +272,10 @@ def bench_batched(args):
         probs.append(dict(fs=fs, op64=op64, op32=op32, d32=d32, b=b, x=x, nrhs=nrhs, key=(n, nl, order)))
     total_dofs = sum(p["fs"].undf * p["nrhs"] for p in probs)
 
+    say(f"built: {total_dofs} dofs x RHS in total")
+
     def run_batch(tune):
-        bt = Batch(r_tol=1e-8, a_tol=0.0, max_iter=20000, check_every=8)
+        bt = Batch(r_tol=1e-8, a_tol=0.0, max_iter=4000, check_every=8)
         ids = []
         for p in probs:
             p["x"].set_scalar(0.0)
@@ -277,11 +284,15 @@ def bench_batched(args):
         t0 = time.perf_counter()
         done_at = {}
         tuned = None
+        last = 0.0
         while True:
             na = bt.run(1)
             if tune and tuned is None:
                 tuned = bt.tune()
             now = time.perf_counter() - t0
+            if now - last > 5.0:
+                last = now
+                say(f"{now:.1f} s: {na} problems active, {bt.stats()['iterations']} batch iterations")
             for i, pid in enumerate(ids):
                 if pid not in done_at and bt.finished(pid):
                     done_at[pid] = now
@@ -293,11 +304,13 @@ def bench_batched(args):
         bt.close()
         return wall, np.array(sorted(done_at.values())), res, st, tuned
 
-    run_batch(tune=True)  # warm-up, and the autotuner's choice for the timed runs
+    w, _, _, st, tuned = run_batch(tune=True)  # warm-up, and the autotuner's choice for the timed runs
+    say(f"warm-up batch {w:.2f} s, autotuned apply {tuned:.3f} ms: {st}")
     times = []
     for _ in range(max(1, args.steps)):
         wall, done, res, st, _ = run_batch(tune=False)
         times.append(wall)
+        say(f"timed batch {wall:.2f} s")
     worst = max(r.true_rel_res for rr in res for r in rr)
     ok = all(r.status == 0 for rr in res for r in rr)
     # sequential plain FP64 solves of the same systems, one RHS at a time
@@ -307,6 +320,7 @@ def bench_batched(args):
         B = p["b"].export_field().reshape(p["fs"].undf, p["nrhs"])
         prec = L.DiagonalPreconditioner.from_operator(p["op64"])
         cg = L.ConjugateGradient(p["op64"], prec, 1e-8, 0.0, 20000, check_every=16)
+        say(f"sequential FP64: C{p['key'][0]}x{p['key'][1]} order {p['key'][2]}, {seq_t:.1f} s so far")
         for r in range(p["nrhs"]):
             bv = L.FieldVector(p["fs"]).import_field(B[:, r])
             xv = L.FieldVector(p["fs"])

@@ -62,7 +62,7 @@ int lfric_timer_stop(double *ms);
 /* Bytes of device memory held by live handles. */
 long long lfric_device_bytes(void);
 /* LMA apply used inside the solvers: 0 the colour-ordered scatter (4 launches per apply),
- * 1 the gather form (one launch; in CG it also forms p.(A p)).  Default 0. */
+ * 1 the gather form (one launch; in CG it also forms p.(A p)).  Default 1. */
 int lfric_set_apply_form(int form);
 /* Free and total device memory (hipMemGetInfo) of the library's device. */
 int lfric_device_memory(long long *free_bytes, long long *total_bytes);

@@ -38,7 +38,7 @@ struct Context {
   unsigned int *d_counter = nullptr;
   double *h_pinned = nullptr;
   long long bytes = 0;
-  int apply_form = 0;  // LMA apply inside the solvers: 0 colour-ordered scatter, 1 gather (+ fused dot)
+  int apply_form = 1;  // LMA apply inside the solvers: 0 colour-ordered scatter, 1 gather (+ fused dot)
 };
 Context &ctx();
 int require_init();

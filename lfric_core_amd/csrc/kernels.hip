@@ -365,17 +365,43 @@ __global__ __launch_bounds__(kBlock) void k_lma_gather(const T *__restrict__ A, 
   T acc = T(0);
   if (valid) {
     const bool has_m = m < nl, has_p = m > 0;
-    const int q1 = off[col + 1];
-    for (int q = off[col]; q < q1; ++q) {
-      const int en = ent[q];
-      const int pos = en >> 5, i = en & 31;
-      const int pt = roles.partner(i);
-      const long long e = static_cast<long long>(ebase[pos]) + m;
-      int b[N2];
+    const int q0 = off[col], q1 = off[col + 1];
+    constexpr int KMAX = 4;  // cells around a vertex column of a quadrilateral mesh
+    if (N2 <= 8 && q1 - q0 <= KMAX) {
+      // the list's loads are issued together (three dependent rounds for the whole column,
+      // not three per cell), then the rows are added in list order
+      const int cnt = q1 - q0;
+      int en[KMAX], eb[KMAX], b[KMAX][N2];
 #pragma unroll
-      for (int j = 0; j < N2; ++j) b[j] = map2[static_cast<size_t>(pos) * N2 + j];
-      if (pt >= 0 && has_p) acc = row_dot<T, N2>(A, gs, e, pt, x, b, m - 1, acc);
-      if (has_m) acc = row_dot<T, N2>(A, gs, e, i, x, b, m, acc);
+      for (int k = 0; k < KMAX; ++k) en[k] = k < cnt ? ent[q0 + k] : 0;
+#pragma unroll
+      for (int k = 0; k < KMAX; ++k) {
+        const int pos = en[k] >> 5;
+        eb[k] = k < cnt ? ebase[pos] : 0;
+#pragma unroll
+        for (int j = 0; j < N2; ++j) b[k][j] = k < cnt ? map2[static_cast<size_t>(pos) * N2 + j] : 0;
+      }
+#pragma unroll
+      for (int k = 0; k < KMAX; ++k) {
+        if (k >= cnt) break;
+        const int i = en[k] & 31;
+        const int pt = roles.partner(i);
+        const long long e = static_cast<long long>(eb[k]) + m;
+        if (pt >= 0 && has_p) acc = row_dot<T, N2>(A, gs, e, pt, x, b[k], m - 1, acc);
+        if (has_m) acc = row_dot<T, N2>(A, gs, e, i, x, b[k], m, acc);
+      }
+    } else {
+      for (int q = q0; q < q1; ++q) {
+        const int en = ent[q];
+        const int pos = en >> 5, i = en & 31;
+        const int pt = roles.partner(i);
+        const long long e = static_cast<long long>(ebase[pos]) + m;
+        int b[N2];
+#pragma unroll
+        for (int j = 0; j < N2; ++j) b[j] = map2[static_cast<size_t>(pos) * N2 + j];
+        if (pt >= 0 && has_p) acc = row_dot<T, N2>(A, gs, e, pt, x, b, m - 1, acc);
+        if (has_m) acc = row_dot<T, N2>(A, gs, e, i, x, b, m, acc);
+      }
     }
     y[static_cast<size_t>(ci.x) + m] = acc;
   }

@@ -192,7 +192,7 @@ def test_cg_fp64_matches_oracle(hip, orc, form):
     try:
         res = cg.apply(x, b)
     finally:
-        hip.set_apply_form(0)
+        hip.set_apply_form(1)  # the library default
     A = orc.gen_spd(8, m.ncell_3d, 0)
     bo = orc.gen_uniform(m.undf, 0, 3)
     np.testing.assert_array_equal(b.export_field(), bo)
@@ -217,7 +217,7 @@ def test_mixed_precision_refinement_reaches_fp64_accuracy(hip, orc, form):
     try:
         res = solver.apply(x, b)
     finally:
-        hip.set_apply_form(0)
+        hip.set_apply_form(1)  # the library default
     assert res.status == 0 and res.true_rel_res <= 1e-8 and res.outer_iterations >= 2
     A = orc.gen_spd(8, m.ncell_3d, 0, scaled=True)
     bo = orc.gen_uniform(m.undf, 0, 3)
