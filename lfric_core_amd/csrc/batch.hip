@@ -73,31 +73,13 @@ __device__ inline void gather_point(const ProbDev &d, long long t) {
   if (m == d.nl && !ci.y) return;
   const bool has_m = m < d.nl, has_p = m > 0;
   typedef float VR __attribute__((ext_vector_type(NR)));
-  typedef float V4 __attribute__((ext_vector_type(4)));
   VR acc = {};
   auto row = [&](long long e, int r, const int (&b)[N2], int lev) {
-    const float *Ae = d.A + static_cast<size_t>(e >> 6) * d.gs + static_cast<size_t>(e & 63) * 4;
-    if constexpr (N2 % 4 == 0) {
+    for_row<float, N2>(d.A, d.gs, e, r, [&](int j, float a) {
+      const VR xv = *reinterpret_cast<const VR *>(d.p + (static_cast<size_t>(b[j]) + lev) * NR);
 #pragma unroll
-      for (int c = 0; c < N2 / 4; ++c) {
-        const V4 v = *reinterpret_cast<const V4 *>(Ae + static_cast<size_t>(r * (N2 / 4) + c) * 256);
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          const VR xv = *reinterpret_cast<const VR *>(d.p + (static_cast<size_t>(b[c * 4 + q]) + lev) * NR);
-#pragma unroll
-          for (int k = 0; k < NR; ++k) acc[k] = fma(v[q], xv[k], acc[k]);
-        }
-      }
-    } else {
-#pragma unroll
-      for (int j = 0; j < N2; ++j) {
-        const int p = r * N2 + j;
-        const float a = Ae[static_cast<size_t>(p >> 2) * 256 + (p & 3)];
-        const VR xv = *reinterpret_cast<const VR *>(d.p + (static_cast<size_t>(b[j]) + lev) * NR);
-#pragma unroll
-        for (int k = 0; k < NR; ++k) acc[k] = fma(a, xv[k], acc[k]);
-      }
-    }
+      for (int k = 0; k < NR; ++k) acc[k] = fma(a, xv[k], acc[k]);
+    });
   };
   const int q1 = d.goff[col + 1];
   for (int q = d.goff[col]; q < q1; ++q) {
@@ -114,35 +96,30 @@ __device__ inline void gather_point(const ProbDev &d, long long t) {
   *reinterpret_cast<VR *>(d.q + (static_cast<size_t>(ci.x) + m) * NR) = acc;
 }
 
-template <int N2>
-__device__ inline void gather_nr(const ProbDev &d, long long t) {
-  switch (d.nrhs) {
-    case 1: gather_point<N2, 1>(d, t); break;
-    case 2: gather_point<N2, 2>(d, t); break;
-    case 4: gather_point<N2, 4>(d, t); break;
-    default: gather_point<N2, 8>(d, t); break;
-  }
-}
-
-// q = A32 p for the active problems with N2 dofs per cell, whose tiles are [t_begin, t_end);
-// persistent: blocks take tiles from *counter until none is left, so every wavefront exits
-template <int BS, int N2>
+// q = A32 p for the active problems with N2 dofs per cell and NR right-hand sides, whose tiles
+// are [t_begin, t_end); persistent: blocks take tiles from *counter until none is left, so
+// every wavefront exits.  One kernel per (N2, NR) group keeps each group's registers (and so
+// its occupancy) its own.
+template <int BS, int N2, int NR>
 __global__ __launch_bounds__(BS) void k_batch_apply(const ProbDev *__restrict__ P, const int *__restrict__ tile0,
                                                     int nprob, int t_begin, int t_end, unsigned int *counter) {
   __shared__ int s_tile;
   for (;;) {
     if (threadIdx.x == 0) s_tile = t_begin + static_cast<int>(atomicAdd(counter, 1u));
     __syncthreads();
-    const int tile = s_tile;
+    // the tile is the same for the whole block: keep it (and the descriptor) in scalar registers
+    const int tile = __builtin_amdgcn_readfirstlane(s_tile);
     __syncthreads();
     if (tile >= t_end) return;
     const ProbDev &d = P[find_problem(tile0, nprob, tile)];
-    gather_nr<N2>(d, static_cast<long long>(tile - d.apply_tile0) * BS + threadIdx.x);
+    gather_point<N2, NR>(d, static_cast<long long>(tile - d.apply_tile0) * BS + threadIdx.x);
   }
 }
 
+constexpr int kMaxGroups = 8;  // (ndf 8 or 27) x (nrhs 1, 2, 4 or 8)
+
 __global__ void k_zero_counter(unsigned int *counter) {
-  if (threadIdx.x < 2 && blockIdx.x == 0) counter[threadIdx.x] = 0u;
+  if (threadIdx.x < kMaxGroups && blockIdx.x == 0) counter[threadIdx.x] = 0u;
 }
 
 // per vector tile and RHS: sums of a.b (and c.d) over the tile's entries; thread t of the
@@ -380,7 +357,12 @@ struct lfric_batch {
   double *d_partials = nullptr, *d_scratch = nullptr, *h_scratch = nullptr;
   unsigned int *d_counter = nullptr;
   std::vector<lfric::Problem *> active;
-  int ntiles = 0, nvtiles = 0, ntiles27 = 0;  // order-1 problems come first: tiles [0, ntiles27)
+  int ntiles = 0, nvtiles = 0;
+  // apply groups of equal (ndf, nrhs): tiles [begin, end) each
+  struct Group {
+    int ndf, nrhs, begin, end;
+  };
+  std::vector<Group> groups;
   bool dirty = true;
   // graph of one inner iteration of the active set
   hipGraph_t graph = nullptr;
@@ -504,9 +486,12 @@ int rebuild(lfric_batch &B) {
   }
   B.ntiles = tile0[np];
   B.nvtiles = vtile0[np];
-  B.ntiles27 = 0;
-  for (int i = 0; i < np; ++i)
-    if (desc[i].ndf == 27) B.ntiles27 = tile0[i + 1];
+  B.groups.clear();
+  for (int i = 0; i < np; ++i) {
+    if (B.groups.empty() || B.groups.back().ndf != desc[i].ndf || B.groups.back().nrhs != desc[i].nrhs)
+      B.groups.push_back({desc[i].ndf, desc[i].nrhs, tile0[i], tile0[i]});
+    B.groups.back().end = tile0[i + 1];
+  }
   if (int rc = ensure(reinterpret_cast<void **>(&B.d_desc), B.cap_desc, np + 1, sizeof(ProbDev))) return rc;
   if (int rc = ensure(reinterpret_cast<void **>(&B.d_tile0), B.cap_tile0, np + 1, sizeof(int))) return rc;
   if (int rc = ensure(reinterpret_cast<void **>(&B.d_vtile0), B.cap_vtile0, np + 1, sizeof(int))) return rc;
@@ -524,20 +509,34 @@ int rebuild(lfric_batch &B) {
   return LFRIC_OK;
 }
 
+template <int BS, int N2>
+void launch_group(lfric_batch &B, int g, int cap) {
+  const auto &G = B.groups[g];
+  const int np = static_cast<int>(B.active.size());
+  const int grid = std::min(G.end - G.begin, cap);
+  unsigned int *cnt = B.d_counter + g;
+  switch (G.nrhs) {
+    case 1: LAUNCHB((k_batch_apply<BS, N2, 1>), grid, BS, B.d_desc, B.d_tile0, np, G.begin, G.end, cnt); break;
+    case 2: LAUNCHB((k_batch_apply<BS, N2, 2>), grid, BS, B.d_desc, B.d_tile0, np, G.begin, G.end, cnt); break;
+    case 4: LAUNCHB((k_batch_apply<BS, N2, 4>), grid, BS, B.d_desc, B.d_tile0, np, G.begin, G.end, cnt); break;
+    default: LAUNCHB((k_batch_apply<BS, N2, 8>), grid, BS, B.d_desc, B.d_tile0, np, G.begin, G.end, cnt); break;
+  }
+}
+
 template <int BS>
 void launch_apply_bs(lfric_batch &B, int grid_mult) {
-  const int np = static_cast<int>(B.active.size());
   const int cap = grid_mult * B.ncu * (1024 / BS);
-  if (B.ntiles27 > 0)
-    LAUNCHB((k_batch_apply<BS, 27>), std::min(B.ntiles27, cap), BS, B.d_desc, B.d_tile0, np, 0, B.ntiles27,
-            B.d_counter);
-  if (B.ntiles > B.ntiles27)
-    LAUNCHB((k_batch_apply<BS, 8>), std::min(B.ntiles - B.ntiles27, cap), BS, B.d_desc, B.d_tile0, np, B.ntiles27,
-            B.ntiles, B.d_counter + 1);
+  for (int g = 0; g < static_cast<int>(B.groups.size()); ++g) {
+    if (B.groups[g].end == B.groups[g].begin) continue;
+    if (B.groups[g].ndf == 27)
+      launch_group<BS, 27>(B, g, cap);
+    else
+      launch_group<BS, 8>(B, g, cap);
+  }
 }
 
 void launch_apply(lfric_batch &B, int block, int grid_mult) {
-  LAUNCHB(k_zero_counter, 1, 64, B.d_counter);
+  LAUNCHB(k_zero_counter, 1, 64, B.d_counter);  // kMaxGroups counters
   switch (block) {
     case 64: launch_apply_bs<64>(B, grid_mult); break;
     case 128: launch_apply_bs<128>(B, grid_mult); break;

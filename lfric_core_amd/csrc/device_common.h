@@ -84,5 +84,48 @@ __device__ inline size_t aidx(size_t gs, long long e, int p) {
          static_cast<size_t>(p % V);
 }
 
+
+// Visit the entries of row r of element e in df2 order, f(j, a_rj), reading the row as whole
+// 16-byte chunks.  A row that does not start on a chunk boundary (ndf2 not a multiple of the
+// chunk) starts S entries into its first chunk; S is made a compile-time constant by a switch
+// so that every index stays static.
+template <typename T, int N2, int S, class F>
+__device__ inline void for_row_s(const T *__restrict__ Ae, int c0, F &&f) {
+  constexpr int V = chunk_entries<T>();
+  constexpr int NCH = (N2 + S + V - 1) / V;
+  typedef T VecT __attribute__((ext_vector_type(V)));
+#pragma unroll
+  for (int c = 0; c < NCH; ++c) {
+    const VecT v = *reinterpret_cast<const VecT *>(Ae + static_cast<size_t>(c0 + c) * 64 * V);
+#pragma unroll
+    for (int q = 0; q < V; ++q) {
+      const int j = c * V + q - S;
+      if (j >= 0 && j < N2) f(j, v[q]);
+    }
+  }
+}
+
+template <typename T, int N2, class F>
+__device__ inline void for_row(const T *__restrict__ A, size_t gs, long long e, int r, F &&f) {
+  constexpr int V = chunk_entries<T>();
+  const T *Ae = A + static_cast<size_t>(e >> 6) * gs + static_cast<size_t>(e & 63) * V;
+  const int p0 = r * N2;
+  if constexpr (N2 % V == 0) {
+    for_row_s<T, N2, 0>(Ae, p0 / V, f);
+  } else if constexpr (V == 2) {
+    if (p0 & 1)
+      for_row_s<T, N2, 1>(Ae, p0 / V, f);
+    else
+      for_row_s<T, N2, 0>(Ae, p0 / V, f);
+  } else {
+    switch (p0 & 3) {
+      case 0: for_row_s<T, N2, 0>(Ae, p0 / V, f); break;
+      case 1: for_row_s<T, N2, 1>(Ae, p0 / V, f); break;
+      case 2: for_row_s<T, N2, 2>(Ae, p0 / V, f); break;
+      default: for_row_s<T, N2, 3>(Ae, p0 / V, f); break;
+    }
+  }
+}
+
 }  // namespace
 }  // namespace lfric

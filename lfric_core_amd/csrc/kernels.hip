@@ -319,21 +319,7 @@ __global__ __launch_bounds__(kBlock) void k_lma_colour(const T *__restrict__ A, 
 template <typename T, int N2>
 __device__ inline T row_dot(const T *__restrict__ A, size_t gs, long long e, int r, const T *__restrict__ x,
                             const int (&b)[N2], int lev, T acc) {
-  constexpr int V = chunk_entries<T>();
-  if constexpr (N2 % V == 0) {
-    typedef T VecT __attribute__((ext_vector_type(V)));
-    const T *Ae = A + static_cast<size_t>(e >> 6) * gs + static_cast<size_t>(e & 63) * V;
-    const int ch0 = r * (N2 / V);
-#pragma unroll
-    for (int c = 0; c < N2 / V; ++c) {
-      const VecT v = *reinterpret_cast<const VecT *>(Ae + static_cast<size_t>(ch0 + c) * 64 * V);
-#pragma unroll
-      for (int q = 0; q < V; ++q) acc = fma(v[q], x[b[c * V + q] + lev], acc);
-    }
-  } else {
-#pragma unroll
-    for (int j = 0; j < N2; ++j) acc = fma(A[aidx<T>(gs, e, r * N2 + j)], x[b[j] + lev], acc);
-  }
+  for_row<T, N2>(A, gs, e, r, [&](int j, T a) { acc = fma(a, x[b[j] + lev], acc); });
   return acc;
 }
 
@@ -367,9 +353,10 @@ __global__ __launch_bounds__(kBlock) void k_lma_gather(const T *__restrict__ A, 
     const bool has_m = m < nl, has_p = m > 0;
     const int q0 = off[col], q1 = off[col + 1];
     constexpr int KMAX = 4;  // cells around a vertex column of a quadrilateral mesh
-    if (N2 <= 8 && q1 - q0 <= KMAX) {
-      // the list's loads are issued together (three dependent rounds for the whole column,
-      // not three per cell), then the rows are added in list order
+    if (sizeof(T) == 8 && N2 <= 8 && q1 - q0 <= KMAX) {
+      // FP64: the list's loads are issued together (three dependent rounds for the whole
+      // column, not three per cell), then the rows are added in list order.  (For FP32 the
+      // extra registers cost more occupancy than the shorter chain gains.)
       const int cnt = q1 - q0;
       int en[KMAX], eb[KMAX], b[KMAX][N2];
 #pragma unroll
@@ -424,31 +411,11 @@ template <typename T, int N2, int NR>
 __device__ inline void row_dot_mrhs(const T *__restrict__ A, size_t gs, long long e, int r, const T *__restrict__ x,
                                     const int (&b)[N2], int lev, typename RVec<T, NR>::type &acc) {
   typedef typename RVec<T, NR>::type VR;
-  constexpr int V = chunk_entries<T>();
-  auto xrow = [&](int j) { return *reinterpret_cast<const VR *>(x + (static_cast<size_t>(b[j]) + lev) * NR); };
-  if constexpr (N2 % V == 0) {
-    typedef T VecT __attribute__((ext_vector_type(V)));
-    const T *Ae = A + static_cast<size_t>(e >> 6) * gs + static_cast<size_t>(e & 63) * V;
-    const int ch0 = r * (N2 / V);
+  for_row<T, N2>(A, gs, e, r, [&](int j, T a) {
+    const VR xv = *reinterpret_cast<const VR *>(x + (static_cast<size_t>(b[j]) + lev) * NR);
 #pragma unroll
-    for (int c = 0; c < N2 / V; ++c) {
-      const VecT v = *reinterpret_cast<const VecT *>(Ae + static_cast<size_t>(ch0 + c) * 64 * V);
-#pragma unroll
-      for (int q = 0; q < V; ++q) {
-        const VR xv = xrow(c * V + q);
-#pragma unroll
-        for (int k = 0; k < NR; ++k) acc[k] = fma(v[q], xv[k], acc[k]);
-      }
-    }
-  } else {
-#pragma unroll
-    for (int j = 0; j < N2; ++j) {
-      const T a = A[aidx<T>(gs, e, r * N2 + j)];
-      const VR xv = xrow(j);
-#pragma unroll
-      for (int k = 0; k < NR; ++k) acc[k] = fma(a, xv[k], acc[k]);
-    }
-  }
+    for (int k = 0; k < NR; ++k) acc[k] = fma(a, xv[k], acc[k]);
+  });
 }
 
 // gather form, NR right-hand sides per thread (see k_lma_gather)
