@@ -63,6 +63,17 @@ __device__ inline double to_d(T v) {
   return static_cast<double>(v);
 }
 
+// Workgroups are handed to the 8 XCDs (each with its own L2) in turn by workgroup id.  This
+// maps the workgroup id to a logical block so that each XCD gets one contiguous 1/8 of the
+// blocks: neighbouring blocks, which read the same x entries, then share an L2.  A bijection
+// on [0, G).
+__device__ inline int xcd_block(int b, int G) {
+  constexpr int kXcd = 8;
+  const int xcd = b % kXcd, idx = b / kXcd;
+  const int q = G / kXcd, r = G % kXcd;  // the first r XCDs get q + 1 blocks
+  return xcd < r ? xcd * (q + 1) + idx : r * (q + 1) + (xcd - r) * q + idx;
+}
+
 // Entries of an operator element per 16-byte chunk (4 FP32, 2 FP64).
 template <typename T>
 constexpr int chunk_entries() {

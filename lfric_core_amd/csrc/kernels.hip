@@ -213,7 +213,7 @@ __global__ __launch_bounds__(kBlock) void k_lma_colour(const T *__restrict__ A, 
                                                        R roles, const int *__restrict__ skip) {
   if (skip && *skip) return;
   const int H = nl + 1;
-  const int t = blockIdx.x * kBlock + threadIdx.x;
+  const int t = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
   if (t >= npos * H) return;
   const int lp = t / H;
   const int m = t - lp * H;
@@ -339,7 +339,7 @@ __global__ __launch_bounds__(kBlock) void k_lma_gather(const T *__restrict__ A, 
                                                        const int *skip) {
   if (skip && *skip) return;
   const int H = nl + 1;
-  const long long t = static_cast<long long>(blockIdx.x) * kBlock + threadIdx.x;
+  const long long t = static_cast<long long>(xcd_block(blockIdx.x, gridDim.x)) * kBlock + threadIdx.x;
   bool valid = t < static_cast<long long>(ncols) * H;
   const int col = valid ? static_cast<int>(t / H) : 0;
   const int m = valid ? static_cast<int>(t - static_cast<long long>(col) * H) : 0;
@@ -427,7 +427,7 @@ __global__ __launch_bounds__(kBlock) void k_lma_gather_mrhs(const T *__restrict_
                                                             const int *__restrict__ ent, int ncols, int nl, R roles) {
   typedef typename RVec<T, NR>::type VR;
   const int H = nl + 1;
-  const long long t = static_cast<long long>(blockIdx.x) * kBlock + threadIdx.x;
+  const long long t = static_cast<long long>(xcd_block(blockIdx.x, gridDim.x)) * kBlock + threadIdx.x;
   if (t >= static_cast<long long>(ncols) * H) return;
   const int col = static_cast<int>(t / H);
   const int m = static_cast<int>(t - static_cast<long long>(col) * H);
@@ -461,7 +461,7 @@ __global__ __launch_bounds__(kBlock) void k_lma_colour_mrhs(const T *__restrict_
                                                             int nl, R roles) {
   typedef typename RVec<T, NR>::type VR;
   const int H = nl + 1;
-  const int t = blockIdx.x * kBlock + threadIdx.x;
+  const int t = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
   if (t >= npos * H) return;
   const int lp = t / H;
   const int m = t - lp * H;
