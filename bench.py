@@ -140,6 +140,16 @@ def main():
         xs32, ys32 = L.FieldVector(fs, L.R32).fill_uniform(1, 3), L.FieldVector(fs, L.R32)
         xs64, ys64 = L.FieldVector(fs).fill_uniform(1, 3), L.FieldVector(fs)
         out = {"apply_only": True}
+        # block-order windows of the apply (the autotuner's search space), FP32 and FP64 gather
+        out["windows"] = {}
+        for wdw in (0, 1, 4, 16, 64, 256):
+            op32.set_window(wdw)
+            op64.set_window(wdw)
+            out["windows"][wdw] = {"fp32_gather_ms": time_applies(L, op32, xs32, ys32, args.apply_reps, "apply_gather"),
+                                   "fp64_gather_ms": time_applies(L, op64, xs64, ys64, args.apply_reps, "apply_gather"),
+                                   "fp32_colour_ms": time_applies(L, op32, xs32, ys32, args.apply_reps, "apply")}
+        op32.set_window(0)
+        op64.set_window(0)
         for method in ("apply", "apply_gather"):
             t32 = time_applies(L, op32, xs32, ys32, args.apply_reps, method)
             t64 = time_applies(L, op64, xs64, ys64, args.apply_reps, method)
@@ -153,6 +163,8 @@ def main():
         print(json.dumps(out))
         return
 
+    # the apply autotuner picks each operator's XCD block-order window (once per operator)
+    tuned = {"fp32": op32.tune(args.apply_form), "fp64": op64.tune(args.apply_form)}
     mixed = L.MixedPrecisionCG(op64, op32, prec32, 1e-8, 0.0, 20000, inner_r_tol=1e-4, check_every=16)
     plain = L.ConjugateGradient(op64, prec64, 1e-8, 0.0, 20000, check_every=16)
 
@@ -222,7 +234,8 @@ def main():
         "verification": {"mixed_true_rel_res_fp64": mixed_true, "fp64_true_rel_res": plain_true,
                          "mixed_vs_fp64_solution_rel_err": err_mixed_vs_fp64},
         "lma_gbs": {"fp32": gbs32, "fp64": gbs64, "fp32_ms": t32, "fp64_ms": t64,
-                    "fp32_bytes": b32, "fp64_bytes": b64},
+                    "fp32_bytes": b32, "fp64_bytes": b64,
+                    "autotuned_window": {"fp32": tuned["fp32"][0], "fp64": tuned["fp64"][0]}},
         "roofline": {"bound": "hbm", "achieved": gbs32, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": gbs32 / HBM_PEAK_GBS,
                      "traffic": TRAFFIC_BYTES_PER_APPLY_FP32.get(

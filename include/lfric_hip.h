@@ -128,6 +128,13 @@ int lfric_op_generate_spd(lfric_op *op, unsigned long long seed, int scaled, dou
 /* General element entries scale*U(lo,hi) (optionally times a per-column sign table). */
 int lfric_op_generate_uniform(lfric_op *op, unsigned long long seed, unsigned long long stream,
                               double lo, double hi, const double *col_sign);
+/* Autotuner of the apply kernels' block order: the 8 XCDs of the GPU (each with its own L2)
+ * take workgroups in turn; a window K gives each XCD K consecutive blocks of every 8K.  Times
+ * the apply of `form` (0 colour-ordered scatter, 1 gather) for K in {0 (dispatch order), 1,
+ * 2, ..., 256} and keeps the fastest in the operator (*best_window, *best_ms per apply).
+ * No reference counterpart. */
+int lfric_op_tune(lfric_op *op, int form, int *best_window, double *best_ms);
+int lfric_op_set_window(lfric_op *op, int window);
 /* Round an operator into another precision (e.g. the r_solver FP32 copy of an FP64 master). */
 int lfric_op_convert(const lfric_op *src, lfric_op *dst);
 void lfric_op_destroy(lfric_op *op);

@@ -678,6 +678,63 @@ int lfric_lma_apply(const lfric_op *op, const lfric_vec *x, lfric_vec *y) {
   return launch_lma_apply(op, x->d, y->d, nullptr);
 }
 
+// Autotuner of the apply's block order: time the apply (form 0 colour-ordered scatter, 1 gather)
+// for each XCD window and keep the fastest in the operator.
+int lfric_op_tune(lfric_op *op, int form, int *best_window, double *best_ms) {
+  if (int rc = require_init()) return rc;
+  LF_CHECK(op && (form == 0 || form == 1), LFRIC_E_ARG, "bad arguments");
+  Context &c = ctx();
+  const size_t es = esize(op->precision);
+  void *x = nullptr, *y = nullptr;
+  if (int rc = dmalloc(&x, es * static_cast<size_t>(op->from->undf))) return rc;
+  if (int rc = dmalloc(&y, es * static_cast<size_t>(op->to->undf))) {
+    dfree(x, es * static_cast<size_t>(op->from->undf));
+    return rc;
+  }
+  lfric_vec xv;
+  xv.fs = op->from;
+  xv.precision = op->precision;
+  xv.n = op->from->undf;
+  xv.d = x;
+  int rc = launch_vec_uniform(&xv, 1, 3);
+  const int windows[] = {0, 1, 2, 4, 8, 16, 32, 64, 256};
+  double best = 1e30;
+  int bw = op->xwin;
+  for (int w : windows) {
+    if (rc) break;
+    op->xwin = w;
+    auto go = [&]() { return form == 1 ? launch_lma_apply_gather(op, x, y) : launch_lma_apply(op, x, y, nullptr); };
+    if ((rc = go())) break;
+    if (hipEventRecord(c.t0, c.stream) != hipSuccess) { rc = fail(LFRIC_E_HIP, "event"); break; }
+    for (int r = 0; r < 5 && !rc; ++r) rc = go();
+    if (rc) break;
+    float ms = 0.f;
+    if (hipEventRecord(c.t1, c.stream) != hipSuccess || hipEventSynchronize(c.t1) != hipSuccess ||
+        hipEventElapsedTime(&ms, c.t0, c.t1) != hipSuccess) {
+      rc = fail(LFRIC_E_HIP, "event timing");
+      break;
+    }
+    if (ms / 5 < best) {
+      best = ms / 5;
+      bw = w;
+    }
+  }
+  op->xwin = bw;
+  (void)hipStreamSynchronize(c.stream);
+  dfree(x, es * static_cast<size_t>(op->from->undf));
+  dfree(y, es * static_cast<size_t>(op->to->undf));
+  if (rc) return rc;
+  if (best_window) *best_window = bw;
+  if (best_ms) *best_ms = best;
+  return LFRIC_OK;
+}
+
+int lfric_op_set_window(lfric_op *op, int window) {
+  LF_CHECK(op && window >= 0, LFRIC_E_ARG, "bad arguments");
+  op->xwin = window;
+  return LFRIC_OK;
+}
+
 int lfric_lma_apply_mrhs(const lfric_op *op, const lfric_vec *x, lfric_vec *y, int variant) {
   if (int rc = require_init()) return rc;
   LF_CHECK(op && x && y, LFRIC_E_ARG, "null argument");

@@ -64,14 +64,18 @@ __device__ inline double to_d(T v) {
 }
 
 // Workgroups are handed to the 8 XCDs (each with its own L2) in turn by workgroup id.  This
-// maps the workgroup id to a logical block so that each XCD gets one contiguous 1/8 of the
-// blocks: neighbouring blocks, which read the same x entries, then share an L2.  A bijection
-// on [0, G).
-__device__ inline int xcd_block(int b, int G) {
+// maps the workgroup id to a logical block so that, within each window of 8K blocks, each XCD
+// gets K consecutive blocks: neighbouring blocks, which read the same x entries, then share an
+// L2, while all XCDs still stream nearby addresses.  K <= 0 keeps dispatch order; the last
+// partial window keeps dispatch order.  A bijection on [0, G).
+__device__ inline int xcd_block(int b, int G, int K) {
   constexpr int kXcd = 8;
-  const int xcd = b % kXcd, idx = b / kXcd;
-  const int q = G / kXcd, r = G % kXcd;  // the first r XCDs get q + 1 blocks
-  return xcd < r ? xcd * (q + 1) + idx : r * (q + 1) + (xcd - r) * q + idx;
+  if (K <= 0) return b;
+  const int W = kXcd * K;
+  const int base = (b / W) * W;
+  if (base + W > G) return b;
+  const int r = b - base;
+  return base + (r % kXcd) * K + r / kXcd;
 }
 
 // Entries of an operator element per 16-byte chunk (4 FP32, 2 FP64).

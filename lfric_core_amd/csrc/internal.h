@@ -116,6 +116,7 @@ struct lfric_op {
   size_t gs = 0;               // values per 64-element group: 64 * (ndf_to*ndf_from padded to 16-byte chunks)
   long long ngroups = 0;       // 64-element groups
   uint32_t row_top = 0;        // rows stored one level up (top dofs of the to space)
+  int xwin = 0;                // XCD block-order window of the apply kernels (lfric_op_tune)
   void *d = nullptr;           // level-aligned AoSoA layout (see kernels.hip)
   size_t bytes = 0;
 };

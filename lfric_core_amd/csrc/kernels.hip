@@ -210,10 +210,10 @@ __global__ __launch_bounds__(kBlock) void k_lma_colour(const T *__restrict__ A, 
                                                        const T *__restrict__ x, T *__restrict__ y,
                                                        const int *__restrict__ map1, const int *__restrict__ map2,
                                                        const uint32_t *__restrict__ first, int pos0, int npos, int nl,
-                                                       R roles, const int *__restrict__ skip) {
+                                                       R roles, const int *__restrict__ skip, int xwin) {
   if (skip && *skip) return;
   const int H = nl + 1;
-  const int t = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+  const int t = xcd_block(blockIdx.x, gridDim.x, xwin) * kBlock + threadIdx.x;
   if (t >= npos * H) return;
   const int lp = t / H;
   const int m = t - lp * H;
@@ -336,10 +336,10 @@ __global__ __launch_bounds__(kBlock) void k_lma_gather(const T *__restrict__ A, 
                                                        const int *__restrict__ map2, const int2 *__restrict__ cols,
                                                        const int *__restrict__ off, const int *__restrict__ ent,
                                                        int ncols, int nl, R roles, double *partials,
-                                                       const int *skip) {
+                                                       const int *skip, int xwin) {
   if (skip && *skip) return;
   const int H = nl + 1;
-  const long long t = static_cast<long long>(xcd_block(blockIdx.x, gridDim.x)) * kBlock + threadIdx.x;
+  const long long t = static_cast<long long>(xcd_block(blockIdx.x, gridDim.x, xwin)) * kBlock + threadIdx.x;
   bool valid = t < static_cast<long long>(ncols) * H;
   const int col = valid ? static_cast<int>(t / H) : 0;
   const int m = valid ? static_cast<int>(t - static_cast<long long>(col) * H) : 0;
@@ -424,10 +424,11 @@ __global__ __launch_bounds__(kBlock) void k_lma_gather_mrhs(const T *__restrict_
                                                             const int *__restrict__ ebase, const T *__restrict__ x,
                                                             T *__restrict__ y, const int *__restrict__ map2,
                                                             const int2 *__restrict__ cols, const int *__restrict__ off,
-                                                            const int *__restrict__ ent, int ncols, int nl, R roles) {
+                                                            const int *__restrict__ ent, int ncols, int nl, R roles,
+                                                            int xwin) {
   typedef typename RVec<T, NR>::type VR;
   const int H = nl + 1;
-  const long long t = static_cast<long long>(xcd_block(blockIdx.x, gridDim.x)) * kBlock + threadIdx.x;
+  const long long t = static_cast<long long>(xcd_block(blockIdx.x, gridDim.x, xwin)) * kBlock + threadIdx.x;
   if (t >= static_cast<long long>(ncols) * H) return;
   const int col = static_cast<int>(t / H);
   const int m = static_cast<int>(t - static_cast<long long>(col) * H);
@@ -458,10 +459,10 @@ __global__ __launch_bounds__(kBlock) void k_lma_colour_mrhs(const T *__restrict_
                                                             T *__restrict__ y, const int *__restrict__ map1,
                                                             const int *__restrict__ map2,
                                                             const uint32_t *__restrict__ first, int pos0, int npos,
-                                                            int nl, R roles) {
+                                                            int nl, R roles, int xwin) {
   typedef typename RVec<T, NR>::type VR;
   const int H = nl + 1;
-  const int t = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+  const int t = xcd_block(blockIdx.x, gridDim.x, xwin) * kBlock + threadIdx.x;
   if (t >= npos * H) return;
   const int lp = t / H;
   const int m = t - lp * H;
@@ -999,25 +1000,25 @@ static void launch_colour(RoleKind rk, int grid, const lfric_op *op, size_t S, c
     case RoleKind::W0K0:
       if constexpr (A_ == 8)
         LAUNCH((k_lma_colour<T, A_, B_, RolesW0K0>), grid, Ad, S, to.d_ebase, x, y, to.d_map, from.d_map, first, pos0, npos, nl,
-               RolesW0K0{}, d_skip);
+               RolesW0K0{}, d_skip, op->xwin);
       break;
     case RoleKind::W0K1:
       if constexpr (A_ == 27)
         LAUNCH((k_lma_colour<T, A_, B_, RolesW0K1>), grid, Ad, S, to.d_ebase, x, y, to.d_map, from.d_map, first, pos0, npos, nl,
-               RolesW0K1{}, d_skip);
+               RolesW0K1{}, d_skip, op->xwin);
       break;
     case RoleKind::W2K0:
       if constexpr (A_ == 6)
         LAUNCH((k_lma_colour<T, A_, B_, RolesW2K0>), grid, Ad, S, to.d_ebase, x, y, to.d_map, from.d_map, first, pos0, npos, nl,
-               RolesW2K0{}, d_skip);
+               RolesW2K0{}, d_skip, op->xwin);
       break;
     case RoleKind::None:
       LAUNCH((k_lma_colour<T, A_, B_, RolesNone>), grid, Ad, S, to.d_ebase, x, y, to.d_map, from.d_map, first, pos0, npos, nl,
-             RolesNone{}, d_skip);
+             RolesNone{}, d_skip, op->xwin);
       break;
     default:
       LAUNCH((k_lma_colour<T, A_, B_, RolesRuntime>), grid, Ad, S, to.d_ebase, x, y, to.d_map, from.d_map, first, pos0, npos, nl,
-             RolesRuntime{roles_of(to)}, d_skip);
+             RolesRuntime{roles_of(to)}, d_skip, op->xwin);
   }
 }
 
@@ -1119,10 +1120,10 @@ static void launch_gather_r(const lfric_op *op, const T *x, T *y, R roles, Gathe
   const int grid = gather_blocks(op);
   if (ex.partials)
     LAUNCH((k_lma_gather<T, N2, R, true>), grid, static_cast<const T *>(op->d), op->gs, to.d_ebase, x, y, from.d_map,
-           to.d_gcol, to.d_goff, to.d_gent, to.gather_ncols, to.nlayers, roles, ex.partials, ex.skip);
+           to.d_gcol, to.d_goff, to.d_gent, to.gather_ncols, to.nlayers, roles, ex.partials, ex.skip, op->xwin);
   else
     LAUNCH((k_lma_gather<T, N2, R, false>), grid, static_cast<const T *>(op->d), op->gs, to.d_ebase, x, y,
-           from.d_map, to.d_gcol, to.d_goff, to.d_gent, to.gather_ncols, to.nlayers, roles, ex.partials, ex.skip);
+           from.d_map, to.d_gcol, to.d_goff, to.d_gent, to.gather_ncols, to.nlayers, roles, ex.partials, ex.skip, op->xwin);
 }
 
 template <typename T, int N2>
@@ -1166,7 +1167,7 @@ static void launch_gather_mrhs_r(const lfric_op *op, const T *x, T *y, R roles) 
   const FunctionSpace &to = *op->to, &from = *op->from;
   const int grid = grid_for(static_cast<long long>(to.gather_ncols) * (to.nlayers + 1));
   LAUNCH((k_lma_gather_mrhs<T, N2, NR, R>), grid, static_cast<const T *>(op->d), op->gs, to.d_ebase, x, y,
-         from.d_map, to.d_gcol, to.d_goff, to.d_gent, to.gather_ncols, to.nlayers, roles);
+         from.d_map, to.d_gcol, to.d_goff, to.d_gent, to.gather_ncols, to.nlayers, roles, op->xwin);
 }
 
 template <typename T, int N1, int N2, int NR, class R>
@@ -1177,7 +1178,7 @@ static void launch_colour_mrhs_r(const lfric_op *op, const T *x, T *y, R roles) 
     if (npos == 0) continue;
     LAUNCH((k_lma_colour_mrhs<T, N1, N2, NR, R>), grid_for(static_cast<long long>(npos) * (to.nlayers + 1)),
            static_cast<const T *>(op->d), op->gs, to.d_ebase, x, y, to.d_map, from.d_map, to.d_first, pos0, npos,
-           to.nlayers, roles);
+           to.nlayers, roles, op->xwin);
   }
 }
 
@@ -1307,7 +1308,7 @@ static int dg_t(const lfric_op *op, const T *x, T *y) {
     LAUNCH((k_lma_colour<T, A, B, RolesRuntime>), grid, static_cast<const T *>(op->d), S, to.d_ebase, x, y,     \
            to.d_map,                                                                                           \
            from.d_map, static_cast<const uint32_t *>(nullptr), 0, npos, nl, RolesRuntime{roles},               \
-           static_cast<const int *>(nullptr));                                                                   \
+           static_cast<const int *>(nullptr), op->xwin);                                                         \
     CHECK_LAUNCH();                                                                                             \
     return LFRIC_OK;                                                                                            \
   }

@@ -202,6 +202,16 @@ class LmaOperator:
         """y = A x (abstract_linear_operator_type%apply)."""
         check(lib().lfric_lma_apply(self._h, x._h, y._h))
 
+    def tune(self, form=1):
+        """Autotune the apply's XCD block order for this operator; returns (window, ms)."""
+        w, ms = ctypes.c_int(), ctypes.c_double()
+        check(lib().lfric_op_tune(self._h, int(form), ctypes.byref(w), ctypes.byref(ms)))
+        return w.value, ms.value
+
+    def set_window(self, window):
+        check(lib().lfric_op_set_window(self._h, int(window)))
+        return self
+
     def apply_mrhs(self, x, y, variant=0):
         """y = A x for every right-hand side of x (x.nrhs == y.nrhs); variant 0 auto,
         1 gather form, 2 colour-ordered scatter."""
