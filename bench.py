@@ -140,6 +140,10 @@ def main():
         xs32, ys32 = L.FieldVector(fs, L.R32).fill_uniform(1, 3), L.FieldVector(fs, L.R32)
         xs64, ys64 = L.FieldVector(fs).fill_uniform(1, 3), L.FieldVector(fs)
         out = {"apply_only": True}
+        out["gather_cell_layout"] = {"fp32_ms": time_applies(L, op32, xs32, ys32, args.apply_reps, "apply_gather"),
+                                     "fp64_ms": time_applies(L, op64, xs64, ys64, args.apply_reps, "apply_gather")}
+        op32.prepare_rows()
+        op64.prepare_rows()
         # block-order windows of the apply (the autotuner's search space), FP32 and FP64 gather
         out["windows"] = {}
         for wdw in (0, 1, 4, 16, 64, 256):
@@ -163,6 +167,9 @@ def main():
         print(json.dumps(out))
         return
 
+    if args.apply_form == 1:  # row-gathered copies for the gather apply (set-up, like the operators)
+        op32.prepare_rows()
+        op64.prepare_rows()
     # the apply autotuner picks each operator's XCD block-order window (once per operator)
     tuned = {"fp32": op32.tune(args.apply_form), "fp64": op64.tune(args.apply_form)}
     mixed = L.MixedPrecisionCG(op64, op32, prec32, 1e-8, 0.0, 20000, inner_r_tol=1e-4, check_every=16)
@@ -241,7 +248,8 @@ def main():
                      "traffic": TRAFFIC_BYTES_PER_APPLY_FP32.get(
                          ("gather" if args.apply_form == 1 else "colour", args.edge_cells, args.nlayers)),
                      "algorithmic_bytes": b32,
-                     "kernel": ("k_lma_gather<float,8,RolesW0K0> (one launch per apply)" if args.apply_form == 1
+                     "kernel": ("k_lma_gather_rows<float,8,RolesW0K0> (one launch per apply, row-gathered copy)"
+                                if args.apply_form == 1
                                 else "k_lma_colour<float,8,8,RolesW0K0> (4 colour launches per apply)")
                                + "; bytes, time and traffic per apply"},
     }

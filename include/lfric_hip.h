@@ -135,6 +135,11 @@ int lfric_op_generate_uniform(lfric_op *op, unsigned long long seed, unsigned lo
  * No reference counterpart. */
 int lfric_op_tune(lfric_op *op, int form, int *best_window, double *best_ms);
 int lfric_op_set_window(lfric_op *op, int window);
+/* Add a row-gathered copy of a square lowest-order W0 operator: the rows that feed each
+ * output dof stored contiguously, so the gather apply streams the matrix in order.  Costs a
+ * second copy of the operator's memory; the gather apply uses it from then on (same sums in
+ * the same order).  The copy is taken now: call again after changing the operator. */
+int lfric_op_prepare_rows(lfric_op *op);
 /* Round an operator into another precision (e.g. the r_solver FP32 copy of an FP64 master). */
 int lfric_op_convert(const lfric_op *src, lfric_op *dst);
 void lfric_op_destroy(lfric_op *op);

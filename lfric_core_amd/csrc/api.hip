@@ -611,6 +611,7 @@ int lfric_op_create(const lfric_fs *to, const lfric_fs *from, int precision, lfr
 void lfric_op_destroy(lfric_op *op) {
   if (!op) return;
   dfree(op->d, op->bytes);
+  dfree(op->rows, op->rows_bytes);
   delete op;
 }
 
@@ -706,7 +707,7 @@ int lfric_op_tune(lfric_op *op, int form, int *best_window, double *best_ms) {
     auto go = [&]() { return form == 1 ? launch_lma_apply_gather(op, x, y) : launch_lma_apply(op, x, y, nullptr); };
     if ((rc = go())) break;
     if (hipEventRecord(c.t0, c.stream) != hipSuccess) { rc = fail(LFRIC_E_HIP, "event"); break; }
-    for (int r = 0; r < 5 && !rc; ++r) rc = go();
+    for (int r = 0; r < 20 && !rc; ++r) rc = go();
     if (rc) break;
     float ms = 0.f;
     if (hipEventRecord(c.t1, c.stream) != hipSuccess || hipEventSynchronize(c.t1) != hipSuccess ||
@@ -714,8 +715,8 @@ int lfric_op_tune(lfric_op *op, int form, int *best_window, double *best_ms) {
       rc = fail(LFRIC_E_HIP, "event timing");
       break;
     }
-    if (ms / 5 < best) {
-      best = ms / 5;
+    if (ms / 20 < best) {
+      best = ms / 20;
       bw = w;
     }
   }
@@ -726,6 +727,14 @@ int lfric_op_tune(lfric_op *op, int form, int *best_window, double *best_ms) {
   if (rc) return rc;
   if (best_window) *best_window = bw;
   if (best_ms) *best_ms = best;
+  return LFRIC_OK;
+}
+
+int lfric_op_prepare_rows(lfric_op *op) {
+  if (int rc = require_init()) return rc;
+  LF_CHECK(op, LFRIC_E_ARG, "null argument");
+  if (int rc = launch_build_rows(op)) return rc;
+  LF_HIP(hipStreamSynchronize(ctx().stream));
   return LFRIC_OK;
 }
 

@@ -117,6 +117,10 @@ struct lfric_op {
   long long ngroups = 0;       // 64-element groups
   uint32_t row_top = 0;        // rows stored one level up (top dofs of the to space)
   int xwin = 0;                // XCD block-order window of the apply kernels (lfric_op_tune)
+  // optional row-gathered copy for the gather apply (lfric_op_prepare_rows): the matrix rows
+  // that feed each output dof, stored contiguously per thread of the gather kernel
+  void *rows = nullptr;
+  size_t rows_bytes = 0, gsr = 0;
   void *d = nullptr;           // level-aligned AoSoA layout (see kernels.hip)
   size_t bytes = 0;
 };

@@ -398,6 +398,112 @@ __global__ __launch_bounds__(kBlock) void k_lma_gather(const T *__restrict__ A, 
   }
 }
 
+// ---- row-gathered operator copy --------------------------------------------------------------
+// For the gather apply of a square operator whose dof columns each take rows from at most
+// KMAX cells, the rows that thread t = (column, level m) reads are stored together: slot
+// s = 2k + h holds, for the k-th cell of the column's list, its top-partner row of layer m-1
+// (h = 0) or its bottom row of layer m (h = 1), each padded to whole chunks (RS entries).
+// Address of entry j of slot s: (t/64)*gsr + ((s*RS + j)/V*64 + t%64)*V + (s*RS + j)%V, so a
+// wavefront reads every chunk of its 64 threads as one contiguous block and the whole copy is
+// streamed once in order, as the cell layout is by the colour-ordered kernel.  Rows that do
+// not exist (m = 0 top, m = nlayers bottom, k beyond the list) hold zeros.
+constexpr int kRowsKmax = 4;
+template <typename T, int N2>
+constexpr int rows_rs() {
+  return (N2 + chunk_entries<T>() - 1) / chunk_entries<T>() * chunk_entries<T>();
+}
+
+template <typename T, int N2, class R>
+__global__ __launch_bounds__(kBlock) void k_build_rows(const T *__restrict__ A, size_t gs, const int *__restrict__ ebase,
+                                                       T *__restrict__ rows, size_t gsr, const int2 *__restrict__ cols,
+                                                       const int *__restrict__ off, const int *__restrict__ ent,
+                                                       int ncols, int nl, R roles) {
+  constexpr int V = chunk_entries<T>(), RS = rows_rs<T, N2>();
+  const int H = nl + 1;
+  const long long t = static_cast<long long>(blockIdx.x) * kBlock + threadIdx.x;
+  if (t >= static_cast<long long>(ncols) * H) return;
+  const int col = static_cast<int>(t / H), m = static_cast<int>(t - static_cast<long long>(col) * H);
+  const int q0 = off[col], cnt = off[col + 1] - q0;
+  T *Rt = rows + static_cast<size_t>(t >> 6) * gsr + static_cast<size_t>(t & 63) * V;
+  for (int k = 0; k < kRowsKmax; ++k) {
+    int pos = 0, i = 0, pt = -1;
+    if (k < cnt) {
+      pos = ent[q0 + k] >> 5;
+      i = ent[q0 + k] & 31;
+      pt = roles.partner(i);
+    }
+    const long long e = static_cast<long long>(ebase[pos]) + m;
+    for (int h = 0; h < 2; ++h) {
+      const int r = h == 0 ? pt : i;
+      const bool live = k < cnt && (h == 0 ? (pt >= 0 && m > 0) : m < nl);
+      for (int j = 0; j < RS; ++j) {
+        const int p = (2 * k + h) * RS + j;
+        const T v = (live && j < N2) ? A[aidx<T>(gs, e, r * N2 + j)] : T(0);
+        Rt[static_cast<size_t>(p / V) * 64 * V + (p % V)] = v;
+      }
+    }
+  }
+}
+
+// Gather apply over the row-gathered copy: the same sums in the same order as k_lma_gather
+// (per cell of the list: top-partner row of layer m-1, then bottom row of layer m).
+template <typename T, int N2, class R, bool DOT>
+__global__ __launch_bounds__(kBlock) void k_lma_gather_rows(const T *__restrict__ rows, size_t gsr,
+                                                            const T *__restrict__ x, T *__restrict__ y,
+                                                            const int *__restrict__ map2, const int2 *__restrict__ cols,
+                                                            const int *__restrict__ off, const int *__restrict__ ent,
+                                                            int ncols, int nl, R roles, double *partials,
+                                                            const int *skip, int xwin) {
+  if (skip && *skip) return;
+  constexpr int V = chunk_entries<T>(), RS = rows_rs<T, N2>();
+  typedef T VecT __attribute__((ext_vector_type(V)));
+  const int H = nl + 1;
+  const long long t = static_cast<long long>(xcd_block(blockIdx.x, gridDim.x, xwin)) * kBlock + threadIdx.x;
+  bool valid = t < static_cast<long long>(ncols) * H;
+  const int col = valid ? static_cast<int>(t / H) : 0;
+  const int m = valid ? static_cast<int>(t - static_cast<long long>(col) * H) : 0;
+  int2 ci = make_int2(0, 0);
+  if (valid) {
+    ci = cols[col];
+    if (m == nl && !ci.y) valid = false;
+  }
+  T acc = T(0);
+  if (valid) {
+    const bool has_m = m < nl, has_p = m > 0;
+    const int q0 = off[col], cnt = off[col + 1] - q0;
+    const T *Rt = rows + static_cast<size_t>(t >> 6) * gsr + static_cast<size_t>(t & 63) * V;
+#pragma unroll
+    for (int k = 0; k < kRowsKmax; ++k) {
+      if (k >= cnt) break;
+      const int en = ent[q0 + k];
+      const int pos = en >> 5, i = en & 31;
+      const int pt = roles.partner(i);
+      int b[N2];
+#pragma unroll
+      for (int j = 0; j < N2; ++j) b[j] = map2[static_cast<size_t>(pos) * N2 + j];
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        if (h == 0 ? !(pt >= 0 && has_p) : !has_m) continue;
+        const int lev = h == 0 ? m - 1 : m;
+#pragma unroll
+        for (int c = 0; c < RS / V; ++c) {
+          const VecT v = *reinterpret_cast<const VecT *>(Rt + static_cast<size_t>((2 * k + h) * (RS / V) + c) * 64 * V);
+#pragma unroll
+          for (int q = 0; q < V; ++q) {
+            const int j = c * V + q;
+            if (j < N2) acc = fma(v[q], x[b[j] + lev], acc);
+          }
+        }
+      }
+    }
+    y[static_cast<size_t>(ci.x) + m] = acc;
+  }
+  if constexpr (DOT) {
+    double v[1] = {valid ? to_d(acc) * to_d(x[static_cast<size_t>(ci.x) + m]) : 0.0};
+    block_partials<1>(v, partials);
+  }
+}
+
 // ---- several right-hand sides ------------------------------------------------------------
 // Multi-RHS vectors hold NR values per dof, RHS fastest: (dof, r) at dof*NR + r.  Every RHS
 // is accumulated exactly as the one-RHS kernels do (same fma order), so column r of a
@@ -1118,6 +1224,16 @@ template <typename T, int N2, class R>
 static void launch_gather_r(const lfric_op *op, const T *x, T *y, R roles, GatherExtra ex) {
   const FunctionSpace &to = *op->to, &from = *op->from;
   const int grid = gather_blocks(op);
+  if (op->rows) {
+    const T *rw = static_cast<const T *>(op->rows);
+    if (ex.partials)
+      LAUNCH((k_lma_gather_rows<T, N2, R, true>), grid, rw, op->gsr, x, y, from.d_map, to.d_gcol, to.d_goff,
+             to.d_gent, to.gather_ncols, to.nlayers, roles, ex.partials, ex.skip, op->xwin);
+    else
+      LAUNCH((k_lma_gather_rows<T, N2, R, false>), grid, rw, op->gsr, x, y, from.d_map, to.d_gcol, to.d_goff,
+             to.d_gent, to.gather_ncols, to.nlayers, roles, ex.partials, ex.skip, op->xwin);
+    return;
+  }
   if (ex.partials)
     LAUNCH((k_lma_gather<T, N2, R, true>), grid, static_cast<const T *>(op->d), op->gs, to.d_ebase, x, y, from.d_map,
            to.d_gcol, to.d_goff, to.d_gent, to.gather_ncols, to.nlayers, roles, ex.partials, ex.skip, op->xwin);
@@ -1245,6 +1361,44 @@ int launch_lma_apply_mrhs(const lfric_op *op, const void *x, void *y, int nrhs, 
 }
 
 int launch_lma_apply_gather(const lfric_op *op, const void *x, void *y) { return gather_any(op, x, y, GatherExtra{}); }
+
+// build the row-gathered copy of a square operator (see k_build_rows)
+template <typename T, int N2, class R>
+static int build_rows_r(lfric_op *op, R roles) {
+  const FunctionSpace &to = *op->to;
+  constexpr int RS = rows_rs<T, N2>();
+  const long long threads = static_cast<long long>(to.gather_ncols) * (to.nlayers + 1);
+  const size_t gsr = static_cast<size_t>(64) * 2 * kRowsKmax * RS;
+  const size_t bytes = sizeof(T) * gsr * static_cast<size_t>((threads + 63) / 64);
+  void *rows = nullptr;
+  if (int rc = dmalloc(&rows, bytes)) return rc;
+  LAUNCH((k_build_rows<T, N2, R>), grid_for(threads), static_cast<const T *>(op->d), op->gs, to.d_ebase,
+         static_cast<T *>(rows), gsr, to.d_gcol, to.d_goff, to.d_gent, to.gather_ncols, to.nlayers, roles);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    dfree(rows, bytes);
+    return fail(LFRIC_E_HIP, hipGetErrorString(e));
+  }
+  dfree(op->rows, op->rows_bytes);
+  op->rows = rows;
+  op->rows_bytes = bytes;
+  op->gsr = gsr;
+  return LFRIC_OK;
+}
+
+int launch_build_rows(lfric_op *op) {
+  FunctionSpace &to = *op->to;
+  LF_CHECK(op->to == op->from, LFRIC_E_ARG, "the row-gathered copy is for square operators");
+  if (int rc = build_gather(to)) return rc;
+  std::vector<int> off(static_cast<size_t>(to.gather_ncols) + 1);
+  LF_HIP(hipMemcpy(off.data(), to.d_goff, sizeof(int) * off.size(), hipMemcpyDeviceToHost));
+  for (int c = 0; c < to.gather_ncols; ++c)
+    LF_CHECK(off[c + 1] - off[c] <= kRowsKmax, LFRIC_E_UNSUPPORTED, "a dof column takes rows from more than 4 cells");
+  const RoleKind rk = role_kind(to);
+  LF_CHECK(rk == RoleKind::W0K0 && to.ndf == 8, LFRIC_E_UNSUPPORTED,
+           "the row-gathered copy is built for lowest-order W0 operators");
+  return op->precision == LFRIC_R64 ? build_rows_r<double, 8>(op, RolesW0K0{}) : build_rows_r<float, 8>(op, RolesW0K0{});
+}
 
 int launch_lma_apply_gather_dot(const lfric_op *op, const void *x, void *y, double *partials, const int *skip) {
   return gather_any(op, x, y, GatherExtra{partials, skip});

@@ -208,6 +208,11 @@ class LmaOperator:
         check(lib().lfric_op_tune(self._h, int(form), ctypes.byref(w), ctypes.byref(ms)))
         return w.value, ms.value
 
+    def prepare_rows(self):
+        """Row-gathered copy for the gather apply (square lowest-order W0 operators)."""
+        check(lib().lfric_op_prepare_rows(self._h))
+        return self
+
     def set_window(self, window):
         check(lib().lfric_op_set_window(self._h, int(window)))
         return self
