@@ -28,8 +28,10 @@ sys.path.insert(0, str(ROOT))
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per FP32 apply of C96x70 W0 from the rocprofv3 FETCH_SIZE / WRITE_SIZE runs recorded in
 # profiles/ (counters scaled by a plain copy of known size), per apply form: the colour-ordered
-# scatter (4 launches) and the gather kernel.  Re-measure when a kernel changes.
-TRAFFIC_BYTES_PER_APPLY_FP32 = {("colour", 96, 70): 4 * (288.4e6 + 16.8e6)}
+# scatter (4 launches) and the gather kernel over the row-gathered copy (one launch).  Re-measure
+# when a kernel changes.
+TRAFFIC_BYTES_PER_APPLY_FP32 = {("colour", 96, 70): 4 * (288.4e6 + 16.8e6),
+                               ("gather", 96, 70): 1099.46e6 + 15.70e6}
 
 
 def apply_bytes(fs, ndf, s_a, s_x):
