@@ -31,7 +31,7 @@ HBM_PEAK_GBS = 8000.0
 # scatter (4 launches) and the gather kernel over the row-gathered copy (one launch).  Re-measure
 # when a kernel changes.
 TRAFFIC_BYTES_PER_APPLY_FP32 = {("colour", 96, 70): 4 * (288.4e6 + 16.8e6),
-                               ("gather", 96, 70): 1099.46e6 + 15.70e6}
+                               ("gather", 96, 70): 1074.05e6 + 15.70e6}
 
 
 def apply_bytes(fs, ndf, s_a, s_x):
@@ -250,7 +250,7 @@ def main():
                      "traffic": TRAFFIC_BYTES_PER_APPLY_FP32.get(
                          ("gather" if args.apply_form == 1 else "colour", args.edge_cells, args.nlayers)),
                      "algorithmic_bytes": b32,
-                     "kernel": ("k_lma_gather_rows<float,8,RolesW0K0> (one launch per apply, row-gathered copy)"
+                     "kernel": ("k_lma_gather_rows_w0<float> (one launch per apply, row-gathered copy)"
                                 if args.apply_form == 1
                                 else "k_lma_colour<float,8,8,RolesW0K0> (4 colour launches per apply)")
                                + "; bytes, time and traffic per apply"},

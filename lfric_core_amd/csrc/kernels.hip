@@ -445,17 +445,21 @@ __global__ __launch_bounds__(kBlock) void k_build_rows(const T *__restrict__ A, 
   }
 }
 
-// Gather apply over the row-gathered copy: the same sums in the same order as k_lma_gather
-// (per cell of the list: top-partner row of layer m-1, then bottom row of layer m).
-template <typename T, int N2, class R, bool DOT>
-__global__ __launch_bounds__(kBlock) void k_lma_gather_rows(const T *__restrict__ rows, size_t gsr,
-                                                            const T *__restrict__ x, T *__restrict__ y,
-                                                            const int *__restrict__ map2, const int2 *__restrict__ cols,
-                                                            const int *__restrict__ off, const int *__restrict__ ent,
-                                                            int ncols, int nl, R roles, double *partials,
-                                                            const int *skip, int xwin) {
+// Gather apply over the row-gathered copy of a lowest-order W0 operator: the same sums in the
+// same order as k_lma_gather (per cell of the list: top-partner row of layer m-1, then bottom
+// row of layer m).  A W0 cell's top vertex k+4 lies one level above its bottom vertex k in the
+// same dof column (the copy is only built when the dofmap says so), so the x entries of a
+// cell's two rows are its 4 columns at levels m-1, m and m+1, and each is loaded once.  The
+// copy is read once per apply: its loads are nontemporal, so the stream does not evict x.
+template <typename T, bool DOT>
+__global__ __launch_bounds__(kBlock) void k_lma_gather_rows_w0(const T *__restrict__ rows, size_t gsr,
+                                                               const T *__restrict__ x, T *__restrict__ y,
+                                                               const int *__restrict__ map2, const int2 *__restrict__ cols,
+                                                               const int *__restrict__ off, const int *__restrict__ ent,
+                                                               int ncols, int nl, double *partials, const int *skip,
+                                                               int xwin) {
   if (skip && *skip) return;
-  constexpr int V = chunk_entries<T>(), RS = rows_rs<T, N2>();
+  constexpr int V = chunk_entries<T>(), RS = rows_rs<T, 8>();
   typedef T VecT __attribute__((ext_vector_type(V)));
   const int H = nl + 1;
   const long long t = static_cast<long long>(xcd_block(blockIdx.x, gridDim.x, xwin)) * kBlock + threadIdx.x;
@@ -475,23 +479,28 @@ __global__ __launch_bounds__(kBlock) void k_lma_gather_rows(const T *__restrict_
 #pragma unroll
     for (int k = 0; k < kRowsKmax; ++k) {
       if (k >= cnt) break;
-      const int en = ent[q0 + k];
-      const int pos = en >> 5, i = en & 31;
-      const int pt = roles.partner(i);
-      int b[N2];
+      const int pos = ent[q0 + k] >> 5;
+      int b[4];
 #pragma unroll
-      for (int j = 0; j < N2; ++j) b[j] = map2[static_cast<size_t>(pos) * N2 + j];
+      for (int j = 0; j < 4; ++j) b[j] = map2[static_cast<size_t>(pos) * 8 + j];
+      T xl[3][4];  // x of the cell's 4 columns at levels m-1, m, m+1
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        xl[0][j] = has_p ? x[b[j] + m - 1] : T(0);
+        xl[1][j] = x[b[j] + m];
+        xl[2][j] = has_m ? x[b[j] + m + 1] : T(0);
+      }
 #pragma unroll
       for (int h = 0; h < 2; ++h) {
-        if (h == 0 ? !(pt >= 0 && has_p) : !has_m) continue;
-        const int lev = h == 0 ? m - 1 : m;
+        if (h == 0 ? !has_p : !has_m) continue;
 #pragma unroll
         for (int c = 0; c < RS / V; ++c) {
-          const VecT v = *reinterpret_cast<const VecT *>(Rt + static_cast<size_t>((2 * k + h) * (RS / V) + c) * 64 * V);
+          const VecT v = __builtin_nontemporal_load(
+              reinterpret_cast<const VecT *>(Rt + static_cast<size_t>((2 * k + h) * (RS / V) + c) * 64 * V));
 #pragma unroll
           for (int q = 0; q < V; ++q) {
             const int j = c * V + q;
-            if (j < N2) acc = fma(v[q], x[b[j] + lev], acc);
+            acc = fma(v[q], j < 4 ? xl[h][j] : xl[h + 1][j - 4], acc);
           }
         }
       }
@@ -1224,15 +1233,17 @@ template <typename T, int N2, class R>
 static void launch_gather_r(const lfric_op *op, const T *x, T *y, R roles, GatherExtra ex) {
   const FunctionSpace &to = *op->to, &from = *op->from;
   const int grid = gather_blocks(op);
-  if (op->rows) {
-    const T *rw = static_cast<const T *>(op->rows);
-    if (ex.partials)
-      LAUNCH((k_lma_gather_rows<T, N2, R, true>), grid, rw, op->gsr, x, y, from.d_map, to.d_gcol, to.d_goff,
-             to.d_gent, to.gather_ncols, to.nlayers, roles, ex.partials, ex.skip, op->xwin);
-    else
-      LAUNCH((k_lma_gather_rows<T, N2, R, false>), grid, rw, op->gsr, x, y, from.d_map, to.d_gcol, to.d_goff,
-             to.d_gent, to.gather_ncols, to.nlayers, roles, ex.partials, ex.skip, op->xwin);
-    return;
+  if constexpr (N2 == 8 && std::is_same<R, RolesW0K0>::value) {
+    if (op->rows) {  // built only for square lowest-order W0 operators (launch_build_rows)
+      const T *rw = static_cast<const T *>(op->rows);
+      if (ex.partials)
+        LAUNCH((k_lma_gather_rows_w0<T, true>), grid, rw, op->gsr, x, y, from.d_map, to.d_gcol, to.d_goff,
+               to.d_gent, to.gather_ncols, to.nlayers, ex.partials, ex.skip, op->xwin);
+      else
+        LAUNCH((k_lma_gather_rows_w0<T, false>), grid, rw, op->gsr, x, y, from.d_map, to.d_gcol, to.d_goff,
+               to.d_gent, to.gather_ncols, to.nlayers, ex.partials, ex.skip, op->xwin);
+      return;
+    }
   }
   if (ex.partials)
     LAUNCH((k_lma_gather<T, N2, R, true>), grid, static_cast<const T *>(op->d), op->gs, to.d_ebase, x, y, from.d_map,
@@ -1397,6 +1408,10 @@ int launch_build_rows(lfric_op *op) {
   const RoleKind rk = role_kind(to);
   LF_CHECK(rk == RoleKind::W0K0 && to.ndf == 8, LFRIC_E_UNSUPPORTED,
            "the row-gathered copy is built for lowest-order W0 operators");
+  for (int c = 0; c < to.ncells; ++c)
+    for (int j = 0; j < 4; ++j)
+      LF_CHECK(to.dofmap[static_cast<size_t>(c) * 8 + j + 4] == to.dofmap[static_cast<size_t>(c) * 8 + j] + 1,
+               LFRIC_E_UNSUPPORTED, "the row-gathered copy needs each top vertex one level above its bottom vertex");
   return op->precision == LFRIC_R64 ? build_rows_r<double, 8>(op, RolesW0K0{}) : build_rows_r<float, 8>(op, RolesW0K0{});
 }
 
