@@ -109,6 +109,9 @@ def main():
                          "in one batch vs sequential FP64 solves; served: Poisson request stream with "
                          "continuous batching vs sequential FP64 solves")
     ap.add_argument("--problems", type=int, default=64)
+    ap.add_argument("--batch-mfma", type=int, default=1, choices=(0, 1),
+                    help="batched: multi-RHS apply of the lowest-order problems on the matrix cores (1) or in "
+                         "gather form (0)")
     ap.add_argument("--apply-form", type=int, default=1, choices=(0, 1),
                     help="LMA apply inside the solvers: 0 colour-ordered scatter, 1 gather with fused p.Ap")
     ap.add_argument("--serve-window", type=float, default=20.0, help="seconds of arrivals")
@@ -162,6 +165,17 @@ def main():
             out[method] = {"fp32_ms": t32, "fp64_ms": t64,
                            "fp32_gbs": apply_bytes(fs, 8, 4, 4) / (t32 * 1e-3) / 1e9,
                            "fp64_gbs": apply_bytes(fs, 8, 8, 8) / (t64 * 1e-3) / 1e9}
+        # 4 right-hand sides at once (FP32): gather form, colour-ordered FMA, colour-ordered MFMA
+        xm, ym = L.FieldVector(fs, L.R32, 4).fill_uniform(5, 3), L.FieldVector(fs, L.R32, 4)
+        out["multi_rhs4_fp32"] = {}
+        for name, variant in (("gather", 1), ("colour", 2), ("mfma", 3)):
+            op32.apply_mrhs(xm, ym, variant)
+            L.synchronize()
+            L.timer_start()
+            for _ in range(args.apply_reps):
+                op32.apply_mrhs(xm, ym, variant)
+            ms = L.timer_stop() / args.apply_reps
+            out["multi_rhs4_fp32"][name] = {"ms": ms, "gbs": apply_bytes(fs, 8, 4, 16) / (ms * 1e-3) / 1e9}
         for _ in range(args.apply_reps):  # plain copies of a known size (undf * 8 bytes each way)
             ys64.copy(xs64)
         L.synchronize()
@@ -270,7 +284,7 @@ def _common(args, metric, value, unit, ms, config, dtype):
 
 def batched_mix(n_problems, seed=0):
     """64 W0 SPD problems: mesh C12/C24/C48, 10/30/70 levels, element order 0 or 1 (p = 0.5),
-    4 right-hand sides, scaled element matrices."""
+    4 right-hand sides; element matrices B B^T + 0.1 I with B entries N(0,1)/8 (unscaled)."""
     rng = np.random.default_rng(seed)
     return [(int(rng.choice((12, 24, 48))), int(rng.choice((10, 30, 70))), int(rng.integers(0, 2)), 4,
              int(rng.integers(0, 2**31))) for _ in range(n_problems)]
@@ -287,7 +301,7 @@ def bench_batched(args):
     say(f"building {len(mix)} problems")
     for (n, nl, order, nrhs, seed) in mix:
         fs = L.FunctionSpace.cubedsphere(n, nl, L.W0, order)
-        op64 = L.LmaOperator(fs).generate_spd(seed, scaled=True)
+        op64 = L.LmaOperator(fs).generate_spd(seed, scaled=False)
         op32 = op64.convert(L.R32)
         d32 = op32.mm_diagonal(L.FieldVector(fs, L.R32))
         b = L.FieldVector(fs, L.R64, nrhs).fill_uniform(seed, 3)
@@ -299,6 +313,7 @@ def bench_batched(args):
 
     def run_batch(tune):
         bt = Batch(r_tol=1e-8, a_tol=0.0, max_iter=4000, check_every=8)
+        bt.config(mfma=bool(args.batch_mfma))
         ids = []
         for p in probs:
             p["x"].set_scalar(0.0)
@@ -329,6 +344,18 @@ def bench_batched(args):
 
     w, _, _, st, tuned = run_batch(tune=True)  # warm-up, and the autotuner's choice for the timed runs
     say(f"warm-up batch {w:.2f} s, autotuned apply {tuned:.3f} ms: {st}")
+    # the batch's multi-RHS apply of all 64 problems at once, matrix-core and gather forms
+    apply_ms = {}
+    for form, on in (("mfma", True), ("gather", False)):
+        bt = Batch(r_tol=1e-8, a_tol=0.0, max_iter=4000, check_every=8)
+        bt.config(mfma=on)
+        for p in probs:
+            p["x"].set_scalar(0.0)
+            bt.add(p["op64"], p["op32"], p["d32"], p["x"], p["b"])
+        bt.run(1)
+        apply_ms[form] = bt.tune()
+        bt.close()
+    say(f"batch apply (autotuned): {apply_ms}")
     times = []
     for _ in range(max(1, args.steps)):
         wall, done, res, st, _ = run_batch(tune=False)
@@ -358,7 +385,10 @@ def bench_batched(args):
     line = _common(args, "solved dofs/s at rel-res 1e-8 (batched mixed precision vs sequential FP64), p50/p99 "
                          "solve ms", total_dofs / wall, "solved dofs/s", wall * 1e3,
                    {"workload": f"{len(probs)} uneven W0 SPD problems (C12/C24/C48, 10/30/70 levels, order 0/1), "
-                                "4 RHS each, Jacobi CG to 1e-8 in FP64 refinement, one batch",
+                                "element matrices B B^T + 0.1 I with B ~ N(0,1)/8, 4 RHS each, Jacobi CG to 1e-8 "
+                                "in FP64 refinement, one batch",
+                    "multi_rhs_apply": "matrix cores (order 0) / gather form (order 1)" if args.batch_mfma
+                    else "gather form",
                     "problems": len(probs), "rhs_total": sum(p["nrhs"] for p in probs),
                     "dofs_total": total_dofs, "parallelism": "one GPU, one batch"},
                    "f32 inner CG / f64 refinement")
@@ -371,6 +401,7 @@ def bench_batched(args):
                             "speedup_of_batch": seq_t / wall, "iterations": seq_it},
         "verification": {"all_converged": ok, "worst_true_rel_res_fp64": worst},
         "autotuned": {"block": st["block"], "grid_mult": st["grid_mult"]},
+        "batch_apply_ms": apply_ms,
         "batch_iterations": st["iterations"],
     })
     print(json.dumps(line))

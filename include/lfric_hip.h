@@ -142,6 +142,12 @@ int lfric_op_set_window(lfric_op *op, int window);
 int lfric_op_prepare_rows(lfric_op *op);
 /* Round an operator into another precision (e.g. the r_solver FP32 copy of an FP64 master). */
 int lfric_op_convert(const lfric_op *src, lfric_op *dst);
+/* invert_local_operator_code (components/science/source/kernel/algebra/
+ * sci_invert_local_operator_kernel_mod.F90:51-77, LU of infrastructure/source/utilities/
+ * matrix_invert_mod.F90:62-216): Ainv = per-cell inverse of every element matrix of A, an
+ * operator from a space to itself; Ainv on the same spaces and precision, not A itself.  The
+ * factorization does not pivot, as the reference's does not. */
+int lfric_op_invert_local(const lfric_op *A, lfric_op *Ainv);
 void lfric_op_destroy(lfric_op *op);
 
 /* invoke(setval_c(y,0), matrix_vector_kernel_type(y, x, A)): y = A x over the coloured
@@ -160,6 +166,9 @@ int lfric_lma_apply_gather(const lfric_op *op, const lfric_vec *x, lfric_vec *y)
 #define LFRIC_MRHS_AUTO 0   /* the library's choice */
 #define LFRIC_MRHS_GATHER 1 /* one launch, gather form */
 #define LFRIC_MRHS_COLOUR 2 /* colour-ordered scatter, one launch per colour */
+/* colour-ordered scatter on the matrix cores (v_mfma_f32_4x4x1f32), square lowest-order W0
+ * FP32 operators only; bit-identical to LFRIC_MRHS_COLOUR */
+#define LFRIC_MRHS_MFMA 3
 int lfric_lma_apply_mrhs(const lfric_op *op, const lfric_vec *x, lfric_vec *y, int variant);
 /* transpose_matrix_vector_kernel (components/science/source/kernel/algebra/
  * transpose_matrix_vector_kernel_mod.F90): y = A^T x, A maps `from` -> `to`, y on `from`. */
@@ -325,6 +334,9 @@ int lfric_batch_remove(lfric_batch *b, int id);
 int lfric_batch_tune(lfric_batch *b, double *best_ms);
 /* use_graph 0/1 (-1 keeps), block 64/128/256 (0 keeps), grid_mult > 0 (0 keeps). */
 int lfric_batch_config(lfric_batch *b, int use_graph, int block, int grid_mult);
+/* Multi-RHS apply of lowest-order W0 problems (with at most 8 colours): 1 (default) colour-
+ * ordered on the matrix cores (LFRIC_MRHS_MFMA), 0 gather form like the other problems. */
+int lfric_batch_set_mfma(lfric_batch *b, int on);
 int lfric_batch_stats(const lfric_batch *b, long long *iterations, int *block, int *grid_mult, int *ntiles);
 void lfric_batch_destroy(lfric_batch *b);
 

@@ -87,6 +87,7 @@ SIGNATURES = {
     "lfric_op_generate_spd": (_I, [_VP, _ULL, _I, _D]),
     "lfric_op_generate_uniform": (_I, [_VP, _ULL, _ULL, _D, _D, _VP]),
     "lfric_op_convert": (_I, [_VP, _VP]),
+    "lfric_op_invert_local": (_I, [_VP, _VP]),
     "lfric_op_destroy": (None, [_VP]),
     "lfric_lma_apply": (_I, [_VP, _VP, _VP]),
     "lfric_lma_apply_gather": (_I, [_VP, _VP, _VP]),
@@ -128,6 +129,7 @@ SIGNATURES = {
     "lfric_batch_remove": (_I, [_VP, _I]),
     "lfric_batch_tune": (_I, [_VP, _DP]),
     "lfric_batch_config": (_I, [_VP, _I, _I, _I]),
+    "lfric_batch_set_mfma": (_I, [_VP, _I]),
     "lfric_batch_stats": (_I, [_VP, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(_I), ctypes.POINTER(_I),
                                ctypes.POINTER(_I)]),
     "lfric_batch_destroy": (None, [_VP]),

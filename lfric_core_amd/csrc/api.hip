@@ -615,6 +615,12 @@ void lfric_op_destroy(lfric_op *op) {
   delete op;
 }
 
+// an operator whose entries change keeps its row-gathered copy (if it has one) in step
+static int refresh_rows(lfric_op *op, int rc) {
+  if (rc != LFRIC_OK || !op->rows) return rc;
+  return launch_build_rows(op);
+}
+
 int lfric_op_upload(lfric_op *op, const double *ls) {
   if (int rc = require_init()) return rc;
   LF_CHECK(op && ls, LFRIC_E_ARG, "null argument");
@@ -624,7 +630,7 @@ int lfric_op_upload(lfric_op *op, const double *ls) {
   LF_HIP(hipMemcpyAsync(tmp, ls, n * sizeof(double), hipMemcpyHostToDevice, ctx().stream));
   int rc = launch_op_from_reference(op, tmp);
   dfree(tmp, n * sizeof(double));
-  return rc;
+  return refresh_rows(op, rc);
 }
 
 int lfric_op_download(const lfric_op *op, double *ls) {
@@ -645,20 +651,26 @@ int lfric_op_download(const lfric_op *op, double *ls) {
 int lfric_op_generate_spd(lfric_op *op, unsigned long long seed, int scaled, double shift) {
   if (int rc = require_init()) return rc;
   LF_CHECK(op, LFRIC_E_ARG, "null operator");
-  return launch_gen_spd(op, seed, scaled, shift);
+  return refresh_rows(op, launch_gen_spd(op, seed, scaled, shift));
 }
 
 int lfric_op_generate_uniform(lfric_op *op, unsigned long long seed, unsigned long long stream, double lo,
                               double hi, const double *col_sign) {
   if (int rc = require_init()) return rc;
   LF_CHECK(op, LFRIC_E_ARG, "null operator");
-  return launch_gen_uniform(op, seed, stream, lo, hi, col_sign);
+  return refresh_rows(op, launch_gen_uniform(op, seed, stream, lo, hi, col_sign));
 }
 
 int lfric_op_convert(const lfric_op *src, lfric_op *dst) {
   if (int rc = require_init()) return rc;
   LF_CHECK(src && dst && src->to == dst->to && src->from == dst->from, LFRIC_E_ARG, "operators differ in shape");
-  return launch_op_convert(src, dst);
+  return refresh_rows(dst, launch_op_convert(src, dst));
+}
+
+int lfric_op_invert_local(const lfric_op *A, lfric_op *Ainv) {
+  if (int rc = require_init()) return rc;
+  LF_CHECK(A && Ainv && A != Ainv, LFRIC_E_ARG, "null or aliased operators");
+  return refresh_rows(Ainv, launch_op_invert_local(A, Ainv));
 }
 
 static int check_apply(const lfric_op *op, const lfric_vec *x, const lfric_vec *y, bool transpose) {

@@ -21,12 +21,14 @@
 #include "device_common.h"
 #include "internal.h"
 #include "kernels.h"
+#include "mfma_apply.h"
 
 namespace lfric {
 
 namespace {
 
 constexpr int kMaxR = 8;     // right-hand sides per problem (1, 2, 4 or 8)
+constexpr int kMaxC = 8;     // colours of a problem whose apply runs on the matrix cores
 constexpr int kVecTile = 4096;  // flat entries per vector tile
 constexpr int kVecBlock = 256;
 
@@ -42,6 +44,11 @@ struct ProbDev {
   int ncols, nl, ndf, nrhs;
   int apply_tile0, vec_tile0, vec_ntiles, scal0;
   int8_t partner[32];
+  // colour-ordered matrix-core apply (lowest-order W0): per colour, first tile, first cell
+  // position and number of cell positions
+  const int *map1;
+  const uint32_t *first;
+  int ctile0[kMaxC], cpos0[kMaxC], cnpos[kMaxC];
 };
 
 // per (problem, RHS) inner CG state
@@ -116,10 +123,39 @@ __global__ __launch_bounds__(BS) void k_batch_apply(const ProbDev *__restrict__ 
   }
 }
 
-constexpr int kMaxGroups = 8;  // (ndf 8 or 27) x (nrhs 1, 2, 4 or 8)
+// q = A32 p in colour-ordered form on the matrix cores for one colour of the active lowest-order
+// W0 problems of one nrhs, whose tiles (64 (cell, level) units of the colour each) are
+// [t_begin, t_end); persistent like k_batch_apply.  The colours run as consecutive launches,
+// so a colour's stores see the earlier colours' (no atomics).
+template <int NR>
+__global__ __launch_bounds__(kBlock) void k_batch_apply_mfma(const ProbDev *__restrict__ P,
+                                                             const int *__restrict__ ctile0, int nprob, int colour,
+                                                             int t_begin, int t_end, unsigned int *counter) {
+  __shared__ int s_tile;
+  for (;;) {
+    if (threadIdx.x == 0) s_tile = t_begin + static_cast<int>(atomicAdd(counter, 1u));
+    __syncthreads();
+    const int tile = __builtin_amdgcn_readfirstlane(s_tile);
+    __syncthreads();
+    if (tile >= t_end) return;
+    const ProbDev &d = P[find_problem(ctile0, nprob, tile)];
+    const int H = d.nl + 1;
+    const long long u = static_cast<long long>(tile - d.ctile0[colour]) * (kBlock / 4) + (threadIdx.x >> 2);
+    const bool valid = u < static_cast<long long>(d.cnpos[colour]) * H;
+    const int lp = valid ? static_cast<int>(u / H) : 0;
+    const int m = valid ? static_cast<int>(u - static_cast<long long>(lp) * H) : 0;
+    const int pos = d.cpos0[colour] + lp;
+    mfma_unit_w0<NR>(d.A, d.gs, static_cast<long long>(d.ebase[pos]) + m, d.map1 + static_cast<size_t>(pos) * 8,
+                     d.map2 + static_cast<size_t>(pos) * 8, d.first[pos], d.p, d.q, m, d.nl, valid);
+  }
+}
+
+// work counters: one per (apply group, colour)
+constexpr int kMaxGroups = 16;
+constexpr int kCounters = kMaxGroups * kMaxC;
 
 __global__ void k_zero_counter(unsigned int *counter) {
-  if (threadIdx.x < kMaxGroups && blockIdx.x == 0) counter[threadIdx.x] = 0u;
+  if (blockIdx.x == 0 && threadIdx.x < kCounters) counter[threadIdx.x] = 0u;
 }
 
 // per vector tile and RHS: sums of a.b (and c.d) over the tile's entries; thread t of the
@@ -352,22 +388,26 @@ struct lfric_batch {
   std::vector<std::pair<int, int>> free_ranges;  // (scal0, nrhs)
   // tables of the active set
   lfric::ProbDev *d_desc = nullptr;
-  int *d_tile0 = nullptr, *d_vtile0 = nullptr;
-  size_t cap_desc = 0, cap_tile0 = 0, cap_vtile0 = 0, cap_partials = 0;
+  int *d_tile0 = nullptr, *d_vtile0 = nullptr, *d_ctile0 = nullptr;
+  size_t cap_desc = 0, cap_tile0 = 0, cap_vtile0 = 0, cap_ctile0 = 0, cap_partials = 0;
   double *d_partials = nullptr, *d_scratch = nullptr, *h_scratch = nullptr;
   unsigned int *d_counter = nullptr;
   std::vector<lfric::Problem *> active;
   int ntiles = 0, nvtiles = 0;
-  // apply groups of equal (ndf, nrhs): tiles [begin, end) each
+  // apply groups of equal (ndf, nrhs, form): gather tiles [begin, end), or for the
+  // matrix-core form tiles [cbegin[c], cend[c]) of each colour c
   struct Group {
-    int ndf, nrhs, begin, end;
+    int ndf, nrhs, mfma, begin, end;
+    int cbegin[lfric::kMaxC], cend[lfric::kMaxC];
   };
+  int np_tables = 0;  // problems in the tables (row length of the colour tile tables - 1)
   std::vector<Group> groups;
   bool dirty = true;
   // graph of one inner iteration of the active set
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   int use_graph = 1;
+  int use_mfma = 1;  // lowest-order W0 problems apply on the matrix cores
   // launch shape of the apply (autotuned)
   int block = 256, grid_mult = 2, ncu = 256;
   long long replays = 0, iterations = 0;
@@ -402,6 +442,11 @@ void drop_graph(lfric_batch &B) {
 int apply_tiles(const Problem &p, int block) {
   const long long pts = static_cast<long long>(p.A32->to->gather_ncols) * (p.A32->to->nlayers + 1);
   return static_cast<int>((pts + block - 1) / block);
+}
+
+// whether the problem's apply runs in colour-ordered form on the matrix cores
+bool uses_mfma(const lfric_batch &B, const Problem &p) {
+  return B.use_mfma && mfma_apply_supported(p.A32) && p.A32->to->ncolours <= kMaxC;
 }
 
 // state entries for a new problem (reusing a freed range of the same size), growing the array
@@ -443,16 +488,20 @@ int rebuild(lfric_batch &B) {
   B.active.clear();
   for (auto &u : B.probs)
     if (u->inner_running) B.active.push_back(u.get());
-  // order-1 problems first (their own apply kernel), heaviest tiles first within each kind
-  std::stable_sort(B.active.begin(), B.active.end(), [](const Problem *a, const Problem *b) {
+  // order-1 problems first (their own apply kernel), then the gather-form and matrix-core groups,
+  // heaviest tiles first within each kind
+  std::stable_sort(B.active.begin(), B.active.end(), [&B](const Problem *a, const Problem *b) {
     const int ka = a->A32->from->ndf, kb = b->A32->from->ndf;
     if (ka != kb) return ka > kb;
+    const bool ma = uses_mfma(B, *a), mb = uses_mfma(B, *b);
+    if (ma != mb) return mb;
     if (a->nrhs != b->nrhs) return a->nrhs > b->nrhs;
     return a->n > b->n;
   });
   const int np = static_cast<int>(B.active.size());
   std::vector<ProbDev> desc(np);
-  std::vector<int> tile0(np + 1, 0), vtile0(np + 1, 0);
+  std::vector<int> tile0(np + 1, 0), vtile0(np + 1, 0), ctile0(static_cast<size_t>(kMaxC) * (np + 1), 0);
+  std::vector<char> mf(np, 0);
   for (int i = 0; i < np; ++i) {
     const Problem &p = *B.active[i];
     const FunctionSpace &to = *p.A32->to, &fr = *p.A32->from;
@@ -477,8 +526,20 @@ int rebuild(lfric_batch &B) {
     d.ndf = fr.ndf;
     d.nrhs = p.nrhs;
     for (int k = 0; k < 32; ++k) d.partner[k] = static_cast<int8_t>(k < to.ndf ? to.partner[k] : -1);
+    mf[i] = uses_mfma(B, p);
+    d.map1 = to.d_map;
+    d.first = to.d_first;
+    for (int c = 0; c < kMaxC; ++c) {
+      int *row = ctile0.data() + static_cast<size_t>(c) * (np + 1);
+      const bool has = mf[i] && c < to.ncolours;
+      d.cpos0[c] = has ? to.colour_start[c] : 0;
+      d.cnpos[c] = has ? to.colour_start[c + 1] - to.colour_start[c] : 0;
+      d.ctile0[c] = row[i];
+      const long long units = static_cast<long long>(d.cnpos[c]) * (to.nlayers + 1);
+      row[i + 1] = row[i] + static_cast<int>((units + kBlock / 4 - 1) / (kBlock / 4));
+    }
     d.apply_tile0 = tile0[i];
-    tile0[i + 1] = tile0[i] + apply_tiles(p, B.block);
+    tile0[i + 1] = tile0[i] + (mf[i] ? 0 : apply_tiles(p, B.block));
     d.vec_tile0 = vtile0[i];
     d.vec_ntiles = static_cast<int>((p.n + kVecTile - 1) / kVecTile);
     vtile0[i + 1] = vtile0[i] + d.vec_ntiles;
@@ -488,13 +549,25 @@ int rebuild(lfric_batch &B) {
   B.nvtiles = vtile0[np];
   B.groups.clear();
   for (int i = 0; i < np; ++i) {
-    if (B.groups.empty() || B.groups.back().ndf != desc[i].ndf || B.groups.back().nrhs != desc[i].nrhs)
-      B.groups.push_back({desc[i].ndf, desc[i].nrhs, tile0[i], tile0[i]});
+    if (B.groups.empty() || B.groups.back().ndf != desc[i].ndf || B.groups.back().nrhs != desc[i].nrhs ||
+        B.groups.back().mfma != mf[i]) {
+      lfric_batch::Group g{};
+      g.ndf = desc[i].ndf;
+      g.nrhs = desc[i].nrhs;
+      g.mfma = mf[i];
+      g.begin = g.end = tile0[i];
+      for (int c = 0; c < kMaxC; ++c) g.cbegin[c] = g.cend[c] = ctile0[static_cast<size_t>(c) * (np + 1) + i];
+      B.groups.push_back(g);
+    }
     B.groups.back().end = tile0[i + 1];
+    for (int c = 0; c < kMaxC; ++c) B.groups.back().cend[c] = ctile0[static_cast<size_t>(c) * (np + 1) + i + 1];
   }
+  LF_CHECK(static_cast<int>(B.groups.size()) <= kMaxGroups, LFRIC_E_UNSUPPORTED, "too many apply groups");
+  B.np_tables = np;
   if (int rc = ensure(reinterpret_cast<void **>(&B.d_desc), B.cap_desc, np + 1, sizeof(ProbDev))) return rc;
   if (int rc = ensure(reinterpret_cast<void **>(&B.d_tile0), B.cap_tile0, np + 1, sizeof(int))) return rc;
   if (int rc = ensure(reinterpret_cast<void **>(&B.d_vtile0), B.cap_vtile0, np + 1, sizeof(int))) return rc;
+  if (int rc = ensure(reinterpret_cast<void **>(&B.d_ctile0), B.cap_ctile0, ctile0.size(), sizeof(int))) return rc;
   if (int rc = ensure(reinterpret_cast<void **>(&B.d_partials), B.cap_partials,
                       static_cast<size_t>(std::max(B.nvtiles, kIrBlocks)) * 2 * kMaxR, sizeof(double)))
     return rc;
@@ -502,6 +575,7 @@ int rebuild(lfric_batch &B) {
     LF_HIP(hipMemcpyAsync(B.d_desc, desc.data(), sizeof(ProbDev) * np, hipMemcpyHostToDevice, ctx().stream));
     LF_HIP(hipMemcpyAsync(B.d_tile0, tile0.data(), sizeof(int) * (np + 1), hipMemcpyHostToDevice, ctx().stream));
     LF_HIP(hipMemcpyAsync(B.d_vtile0, vtile0.data(), sizeof(int) * (np + 1), hipMemcpyHostToDevice, ctx().stream));
+    LF_HIP(hipMemcpyAsync(B.d_ctile0, ctile0.data(), sizeof(int) * ctile0.size(), hipMemcpyHostToDevice, ctx().stream));
   }
   LF_HIP(hipStreamSynchronize(ctx().stream));
   drop_graph(B);
@@ -514,7 +588,7 @@ void launch_group(lfric_batch &B, int g, int cap) {
   const auto &G = B.groups[g];
   const int np = static_cast<int>(B.active.size());
   const int grid = std::min(G.end - G.begin, cap);
-  unsigned int *cnt = B.d_counter + g;
+  unsigned int *cnt = B.d_counter + g * kMaxC;
   switch (G.nrhs) {
     case 1: LAUNCHB((k_batch_apply<BS, N2, 1>), grid, BS, B.d_desc, B.d_tile0, np, G.begin, G.end, cnt); break;
     case 2: LAUNCHB((k_batch_apply<BS, N2, 2>), grid, BS, B.d_desc, B.d_tile0, np, G.begin, G.end, cnt); break;
@@ -523,10 +597,33 @@ void launch_group(lfric_batch &B, int g, int cap) {
   }
 }
 
+// the colours of a matrix-core group, one persistent launch each
+void launch_group_mfma(lfric_batch &B, int g, int grid_mult) {
+  const auto &G = B.groups[g];
+  const int np = B.np_tables;
+  const int cap = grid_mult * B.ncu * (1024 / kBlock);
+  for (int c = 0; c < kMaxC; ++c) {
+    if (G.cend[c] == G.cbegin[c]) continue;
+    const int grid = std::min(G.cend[c] - G.cbegin[c], cap);
+    const int *tab = B.d_ctile0 + static_cast<size_t>(c) * (np + 1);
+    unsigned int *cnt = B.d_counter + g * kMaxC + c;
+    switch (G.nrhs) {
+      case 1: LAUNCHB((k_batch_apply_mfma<1>), grid, kBlock, B.d_desc, tab, np, c, G.cbegin[c], G.cend[c], cnt); break;
+      case 2: LAUNCHB((k_batch_apply_mfma<2>), grid, kBlock, B.d_desc, tab, np, c, G.cbegin[c], G.cend[c], cnt); break;
+      case 4: LAUNCHB((k_batch_apply_mfma<4>), grid, kBlock, B.d_desc, tab, np, c, G.cbegin[c], G.cend[c], cnt); break;
+      default: LAUNCHB((k_batch_apply_mfma<8>), grid, kBlock, B.d_desc, tab, np, c, G.cbegin[c], G.cend[c], cnt); break;
+    }
+  }
+}
+
 template <int BS>
 void launch_apply_bs(lfric_batch &B, int grid_mult) {
   const int cap = grid_mult * B.ncu * (1024 / BS);
   for (int g = 0; g < static_cast<int>(B.groups.size()); ++g) {
+    if (B.groups[g].mfma) {
+      launch_group_mfma(B, g, grid_mult);
+      continue;
+    }
     if (B.groups[g].end == B.groups[g].begin) continue;
     if (B.groups[g].ndf == 27)
       launch_group<BS, 27>(B, g, cap);
@@ -536,7 +633,7 @@ void launch_apply_bs(lfric_batch &B, int grid_mult) {
 }
 
 void launch_apply(lfric_batch &B, int block, int grid_mult) {
-  LAUNCHB(k_zero_counter, 1, 64, B.d_counter);  // kMaxGroups counters
+  LAUNCHB(k_zero_counter, 1, kCounters, B.d_counter);
   switch (block) {
     case 64: launch_apply_bs<64>(B, grid_mult); break;
     case 128: launch_apply_bs<128>(B, grid_mult); break;
@@ -697,7 +794,7 @@ int lfric_batch_create(const lfric_solver_opts *opts, lfric_batch **out) {
   B->opts = *opts;
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, ctx().device) == hipSuccess) B->ncu = prop.multiProcessorCount;
-  int rc = dmalloc(reinterpret_cast<void **>(&B->d_counter), 256);
+  int rc = dmalloc(reinterpret_cast<void **>(&B->d_counter), sizeof(unsigned int) * kCounters);
   if (!rc) rc = dmalloc(reinterpret_cast<void **>(&B->d_scratch), sizeof(double) * 8 * kMaxR);
   // the refinement kernels use kIrBlocks partials before any table exists
   if (!rc)
@@ -874,6 +971,13 @@ int lfric_batch_config(lfric_batch *B, int use_graph, int block, int grid_mult) 
   return LFRIC_OK;
 }
 
+int lfric_batch_set_mfma(lfric_batch *B, int on) {
+  LF_CHECK(B, LFRIC_E_ARG, "null argument");
+  B->use_mfma = on ? 1 : 0;
+  B->dirty = true;
+  return LFRIC_OK;
+}
+
 int lfric_batch_stats(const lfric_batch *B, long long *iterations, int *block, int *grid_mult, int *nactive_tiles) {
   LF_CHECK(B, LFRIC_E_ARG, "null argument");
   if (iterations) *iterations = B->iterations;
@@ -895,7 +999,8 @@ void lfric_batch_destroy(lfric_batch *B) {
   dfree(B->d_vtile0, B->cap_vtile0 * sizeof(int));
   dfree(B->d_partials, B->cap_partials * sizeof(double));
   dfree(B->d_scratch, sizeof(double) * 8 * kMaxR);
-  dfree(B->d_counter, 256);
+  dfree(B->d_counter, sizeof(unsigned int) * kCounters);
+  dfree(B->d_ctile0, B->cap_ctile0 * sizeof(int));
   if (B->h_scratch) (void)hipHostFree(B->h_scratch);
   delete B;
 }

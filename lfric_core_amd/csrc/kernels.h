@@ -43,6 +43,9 @@ int launch_build_rows(lfric_op *op);
 // the apply form the solvers use (lfric_set_apply_form)
 int launch_lma_apply_solver(const lfric_op *op, const void *x, void *y, const int *skip);
 int launch_lma_apply_mrhs(const lfric_op *op, const void *x, void *y, int nrhs, int variant);
+int launch_op_invert_local(const lfric_op *A, lfric_op *Ainv);
+// whether the MFMA multi-RHS apply (LFRIC_MRHS_MFMA) handles this operator
+bool mfma_apply_supported(const lfric_op *op);
 int launch_lma_transpose(const lfric_op *op, const void *x, void *y);
 int launch_dg_apply(const lfric_op *op, const void *x, void *y);
 int launch_mm_diagonal(const lfric_op *op, void *diag);

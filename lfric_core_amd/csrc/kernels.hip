@@ -30,6 +30,7 @@
 
 #include "device_common.h"
 #include "kernels.h"
+#include "mfma_apply.h"
 #include "rng.h"
 
 namespace lfric {
@@ -116,6 +117,93 @@ __global__ void k_vec_uniform(T *__restrict__ v, long long n, unsigned long long
        i += static_cast<long long>(gridDim.x) * blockDim.x)
     v[i] = static_cast<T>(2.0 * rng::uniform(seed, stream, static_cast<uint64_t>(i)) - 1.0);
 }
+// Per-cell dense inverse of a square operator: invert_local_operator_code
+// (components/science/source/kernel/algebra/sci_invert_local_operator_kernel_mod.F90:51-77)
+// with matrix_invert_lu (infrastructure/source/utilities/matrix_invert_mod.F90:62-216):
+// Doolittle LU without pivoting (l below the diagonal, u on and above, in one array), then
+// column m of the inverse by forward substitution on e_m and back substitution, in the
+// reference's loop order and without contraction, so each entry is the reference's to the
+// bit.  One thread per 3D cell.  Up to 8 dofs the factors live in registers; larger elements
+// keep them in a workspace laid out [entry][element] so that a wavefront's accesses coalesce.
+template <typename T, int N>
+struct InvRegs {
+  T c[N][N], y[N], x[N];
+  __device__ T &C(int i, int j) { return c[i][j]; }
+  __device__ T &Y(int i) { return y[i]; }
+  __device__ T &X(int i) { return x[i]; }
+};
+template <typename T>
+struct InvWork {
+  T *w;
+  long long stride, e;
+  int n;
+  __device__ T &C(int i, int j) { return w[static_cast<size_t>(i * n + j) * stride + e]; }
+  __device__ T &Y(int i) { return w[static_cast<size_t>(n * n + i) * stride + e]; }
+  __device__ T &X(int i) { return w[static_cast<size_t>(n * n + n + i) * stride + e]; }
+};
+
+template <typename T, class S>
+__device__ inline void invert_lu(S &s, int n) {
+  for (int m = 0; m < n - 1; ++m)
+    for (int i = m + 1; i < n; ++i) {
+      const T coeff = -(s.C(i, m) / s.C(m, m));
+      s.C(i, m) = -coeff;  // l(i,m)
+      for (int j = m + 1; j < n; ++j) s.C(i, j) = s.C(i, j) + coeff * s.C(m, j);
+    }
+}
+
+template <typename T, class S, class Out>
+__device__ inline void invert_columns(S &s, int n, Out &&out) {
+  for (int m = 0; m < n; ++m) {
+    s.Y(0) = m == 0 ? T(1) : T(0);
+    for (int i = 1; i < n; ++i) {
+      s.Y(i) = i == m ? T(1) : T(0);
+      for (int j = 0; j < i; ++j) s.Y(i) = s.Y(i) - s.Y(j) * s.C(i, j);
+    }
+    s.X(n - 1) = s.Y(n - 1) / s.C(n - 1, n - 1);
+    for (int i = n - 2; i >= 0; --i) {
+      s.X(i) = s.Y(i);
+      for (int j = n - 1; j >= i + 1; --j) s.X(i) = s.X(i) - s.X(j) * s.C(i, j);
+      s.X(i) = s.X(i) / s.C(i, i);
+    }
+    for (int i = 0; i < n; ++i) out(i, m, s.X(i));
+  }
+}
+
+template <typename T, int N>
+__global__ __launch_bounds__(kBlock) void k_invert_local(const T *__restrict__ A, T *__restrict__ B, size_t gs,
+                                                         const int *__restrict__ ebase, int nl, long long nelem,
+                                                         uint32_t row_top) {
+  const long long e = static_cast<long long>(blockIdx.x) * kBlock + threadIdx.x;
+  if (e >= nelem) return;
+  const long long pos = e / nl;
+  const long long el = ebase[pos] + (e - pos * nl);
+  InvRegs<T, N> s;
+#pragma unroll
+  for (int i = 0; i < N; ++i)
+#pragma unroll
+    for (int j = 0; j < N; ++j) s.c[i][j] = A[aidx<T>(gs, el + ((row_top >> i) & 1u), i * N + j)];
+  invert_lu<T>(s, N);
+  invert_columns<T>(s, N, [&](int i, int m, T v) { B[aidx<T>(gs, el + ((row_top >> i) & 1u), i * N + m)] = v; });
+}
+
+// elements [e0, e0 + count) with the factors in a workspace of (n*n + 2n) x count entries
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_invert_local_ws(const T *__restrict__ A, T *__restrict__ B, size_t gs,
+                                                            const int *__restrict__ ebase, int nl, long long e0,
+                                                            long long count, int n, uint32_t row_top, T *work) {
+  const long long t = static_cast<long long>(blockIdx.x) * kBlock + threadIdx.x;
+  if (t >= count) return;
+  const long long e = e0 + t;
+  const long long pos = e / nl;
+  const long long el = ebase[pos] + (e - pos * nl);
+  InvWork<T> s{work, count, t, n};
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < n; ++j) s.C(i, j) = A[aidx<T>(gs, el + ((row_top >> i) & 1u), i * n + j)];
+  invert_lu<T>(s, n);
+  invert_columns<T>(s, n, [&](int i, int m, T v) { B[aidx<T>(gs, el + ((row_top >> i) & 1u), i * n + m)] = v; });
+}
+
 #pragma clang fp contract(on)
 
 // reference layout matrix(ncell_3d, ndf1, ndf2) <-> device layout
@@ -607,6 +695,26 @@ __global__ __launch_bounds__(kBlock) void k_lma_colour_mrhs(const T *__restrict_
   }
 }
 
+// Colour-ordered multi-RHS apply of a lowest-order W0 FP32 operator on the matrix cores (see
+// mfma_apply.h): 64 (cell, level) units per 256-thread block, 4 lanes per unit.
+template <int NR>
+__global__ __launch_bounds__(kBlock) void k_lma_colour_mrhs_mfma(const float *__restrict__ A, size_t gs,
+                                                                 const int *__restrict__ ebase,
+                                                                 const float *__restrict__ x, float *__restrict__ y,
+                                                                 const int *__restrict__ map1,
+                                                                 const int *__restrict__ map2,
+                                                                 const uint32_t *__restrict__ first, int pos0,
+                                                                 int npos, int nl, int xwin) {
+  const int H = nl + 1;
+  const long long u = static_cast<long long>(xcd_block(blockIdx.x, gridDim.x, xwin)) * (kBlock / 4) + (threadIdx.x >> 2);
+  const bool valid = u < static_cast<long long>(npos) * H;
+  const int lp = valid ? static_cast<int>(u / H) : 0;
+  const int m = valid ? static_cast<int>(u - static_cast<long long>(lp) * H) : 0;
+  const int pos = pos0 + lp;
+  mfma_unit_w0<NR>(A, gs, static_cast<long long>(ebase[pos]) + m, map1 + static_cast<size_t>(pos) * 8,
+                   map2 + static_cast<size_t>(pos) * 8, first[pos], x, y, m, nl, valid);
+}
+
 // y = A^T x: thread (cell, level m) of the `from` space; entry (to_df i, from_df j) of layer k
 // sits at element ebase + k + top_to(i)
 template <typename T, int NT, int NF>
@@ -1049,6 +1157,50 @@ int launch_op_to_reference(const lfric_op *op, double *d_ref) {
   return LFRIC_OK;
 }
 
+template <typename T>
+static int invert_local_t(const lfric_op *A, lfric_op *Ainv) {
+  const FunctionSpace &f = *A->to;
+  const long long nelem = static_cast<long long>(f.ncells) * f.nlayers;
+  const int n = f.ndf;
+  const T *a = static_cast<const T *>(A->d);
+  T *b = static_cast<T *>(Ainv->d);
+  switch (n) {
+    case 1: LAUNCH((k_invert_local<T, 1>), grid_for(nelem), a, b, A->gs, f.d_ebase, f.nlayers, nelem, A->row_top); break;
+    case 2: LAUNCH((k_invert_local<T, 2>), grid_for(nelem), a, b, A->gs, f.d_ebase, f.nlayers, nelem, A->row_top); break;
+    case 6: LAUNCH((k_invert_local<T, 6>), grid_for(nelem), a, b, A->gs, f.d_ebase, f.nlayers, nelem, A->row_top); break;
+    case 8: LAUNCH((k_invert_local<T, 8>), grid_for(nelem), a, b, A->gs, f.d_ebase, f.nlayers, nelem, A->row_top); break;
+    default: {
+      // workspace for at most 2^20 elements at a time
+      const long long chunk = std::min<long long>(nelem, 1LL << 20);
+      const size_t bytes = sizeof(T) * static_cast<size_t>(n * n + 2 * n) * static_cast<size_t>(chunk);
+      void *work = nullptr;
+      if (int rc = dmalloc(&work, bytes)) return rc;
+      for (long long e0 = 0; e0 < nelem; e0 += chunk) {
+        const long long cnt = std::min(chunk, nelem - e0);
+        LAUNCH(k_invert_local_ws<T>, grid_for(cnt), a, b, A->gs, f.d_ebase, f.nlayers, e0, cnt, n, A->row_top,
+               static_cast<T *>(work));
+      }
+      const hipError_t e = hipGetLastError();
+      const hipError_t es = hipStreamSynchronize(ctx().stream);
+      dfree(work, bytes);
+      if (e != hipSuccess) return fail(LFRIC_E_HIP, hipGetErrorString(e));
+      if (es != hipSuccess) return fail(LFRIC_E_HIP, hipGetErrorString(es));
+      return LFRIC_OK;
+    }
+  }
+  CHECK_LAUNCH();
+  return LFRIC_OK;
+}
+
+int launch_op_invert_local(const lfric_op *A, lfric_op *Ainv) {
+  LF_CHECK(A->to == A->from, LFRIC_E_ARG, "invert_local: the operator must map a space to itself");
+  LF_CHECK(Ainv->to == A->to && Ainv->from == A->from, LFRIC_E_ARG, "invert_local: output on other spaces");
+  LF_CHECK(Ainv->precision == A->precision, LFRIC_E_ARG, "invert_local: output of another precision");
+  LF_CHECK(A->to->ndf <= 32, LFRIC_E_UNSUPPORTED, "invert_local: more than 32 dofs per cell");
+  if (A->precision == LFRIC_R64) return invert_local_t<double>(A, Ainv);
+  return invert_local_t<float>(A, Ainv);
+}
+
 int launch_convert(int pd, void *dst, int ps, const void *src, long long n, double scale) {
   const int grid = grid_for(n, kBlock, 65536);
   if (pd == LFRIC_R64 && ps == LFRIC_R64)
@@ -1361,9 +1513,43 @@ static int mrhs_t(int variant, int nrhs, const lfric_op *op, const T *x, T *y) {
   return fail(LFRIC_E_UNSUPPORTED, "lma_apply_mrhs: nrhs must be 1, 2, 4 or 8");
 }
 
+template <int NR>
+static void launch_mrhs_mfma_nr(const lfric_op *op, const float *x, float *y) {
+  const FunctionSpace &to = *op->to, &from = *op->from;
+  for (int c = 0; c < to.ncolours; ++c) {
+    const int pos0 = to.colour_start[c], npos = to.colour_start[c + 1] - pos0;
+    if (npos == 0) continue;
+    LAUNCH((k_lma_colour_mrhs_mfma<NR>), grid_for(static_cast<long long>(npos) * (to.nlayers + 1), kBlock / 4),
+           static_cast<const float *>(op->d), op->gs, to.d_ebase, x, y, to.d_map, from.d_map, to.d_first, pos0, npos,
+           to.nlayers, op->xwin);
+  }
+}
+
+bool mfma_apply_supported(const lfric_op *op) {
+  return op->precision == LFRIC_R32 && op->to == op->from && op->to->ndf == 8 && role_kind(*op->to) == RoleKind::W0K0;
+}
+
+static int mrhs_mfma(const lfric_op *op, const void *xv, void *yv, int nrhs) {
+  LF_CHECK(mfma_apply_supported(op), LFRIC_E_UNSUPPORTED,
+           "lma_apply_mrhs: the MFMA variant is for square lowest-order W0 FP32 operators");
+  const float *x = static_cast<const float *>(xv);
+  float *y = static_cast<float *>(yv);
+  switch (nrhs) {
+    case 1: launch_mrhs_mfma_nr<1>(op, x, y); break;
+    case 2: launch_mrhs_mfma_nr<2>(op, x, y); break;
+    case 4: launch_mrhs_mfma_nr<4>(op, x, y); break;
+    case 8: launch_mrhs_mfma_nr<8>(op, x, y); break;
+    default: return fail(LFRIC_E_UNSUPPORTED, "lma_apply_mrhs: nrhs must be 1, 2, 4 or 8");
+  }
+  CHECK_LAUNCH();
+  return LFRIC_OK;
+}
+
 int launch_lma_apply_mrhs(const lfric_op *op, const void *x, void *y, int nrhs, int variant) {
   if (variant == LFRIC_MRHS_AUTO) variant = LFRIC_MRHS_GATHER;
-  LF_CHECK(variant == LFRIC_MRHS_GATHER || variant == LFRIC_MRHS_COLOUR, LFRIC_E_ARG, "unknown multi-RHS variant");
+  LF_CHECK(variant == LFRIC_MRHS_GATHER || variant == LFRIC_MRHS_COLOUR || variant == LFRIC_MRHS_MFMA, LFRIC_E_ARG,
+           "unknown multi-RHS variant");
+  if (variant == LFRIC_MRHS_MFMA) return mrhs_mfma(op, x, y, nrhs);
   if (nrhs == 1)
     return variant == LFRIC_MRHS_GATHER ? launch_lma_apply_gather(op, x, y) : launch_lma_apply(op, x, y, nullptr);
   if (op->precision == LFRIC_R64)

@@ -198,6 +198,14 @@ class LmaOperator:
         check(lib().lfric_op_convert(self._h, out._h))
         return out
 
+    def invert_local(self):
+        """Per-cell inverse of every element matrix (invert_local_operator_code,
+        components/science/source/kernel/algebra/sci_invert_local_operator_kernel_mod.F90:51-77),
+        as a new operator on the same space."""
+        out = LmaOperator(self.fs_to, self.fs_from, self.precision)
+        check(lib().lfric_op_invert_local(self._h, out._h))
+        return out
+
     def apply(self, x, y):
         """y = A x (abstract_linear_operator_type%apply)."""
         check(lib().lfric_lma_apply(self._h, x._h, y._h))
@@ -219,7 +227,8 @@ class LmaOperator:
 
     def apply_mrhs(self, x, y, variant=0):
         """y = A x for every right-hand side of x (x.nrhs == y.nrhs); variant 0 auto,
-        1 gather form, 2 colour-ordered scatter."""
+        1 gather form, 2 colour-ordered scatter, 3 colour-ordered scatter on the matrix cores
+        (square lowest-order W0 FP32 operators; bit-identical to 2)."""
         check(lib().lfric_lma_apply_mrhs(self._h, x._h, y._h, int(variant)))
         return y
 

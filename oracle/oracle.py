@@ -75,6 +75,8 @@ def lib():
             "orc_transpose_apply": (None, [_I, _I, _I, _VP, _VP, _I, _I, _I, _VP, _I, _I, _VP, _VP, _VP, _VP]),
             "orc_dg_apply": (None, [_I, _I, _I, _I, _VP, _I, _I, _VP, _VP, _VP, _VP]),
             "orc_mm_diagonal": (None, [_I, _I, _I, _I, _VP, _VP, _VP]),
+            "orc_invert_local": (_I, [_I64, _I, _VP, _VP]),
+            "orc_invert_local_r32": (_I, [_I64, _I, _VP, _VP]),
             "orc_cg": (_I, [ctypes.POINTER(OrcProblem), _VP, _VP, _D, _D, _I, _I, ctypes.POINTER(OrcResult), _VP]),
             "orc_gmres": (_I, [ctypes.POINTER(OrcProblem), _VP, _VP, _I, _D, _D, _I, _I, ctypes.POINTER(OrcResult), _VP]),
             "orc_bicgstab": (_I, [ctypes.POINTER(OrcProblem), _VP, _VP, _D, _D, _I, _I, ctypes.POINTER(OrcResult), _VP]),
@@ -228,6 +230,20 @@ def mm_diagonal(mesh, A):
     lib().orc_mm_diagonal(mesh.ncells, mesh.nlayers, mesh.ndf, mesh.undf, _p(mesh.dofmap),
                           _p(flat_matrix(A)), _p(d))
     return d
+
+
+def invert_local(A, r32=False):
+    """Per-cell inverse of a (ncell_3d, ndf, ndf) operator (invert_local_operator_code)."""
+    A = np.asarray(A)
+    nc3, n, n2 = A.shape
+    assert n == n2
+    dt = np.float32 if r32 else np.float64
+    Af = np.ascontiguousarray(A.astype(dt).transpose(2, 1, 0)).ravel()
+    out = np.zeros_like(Af)
+    fn = lib().orc_invert_local_r32 if r32 else lib().orc_invert_local
+    if fn(nc3, n, _p(Af), _p(out)) != 0:
+        raise ValueError("invert_local: ndf out of range")
+    return out.reshape((n, n, nc3)).transpose(2, 1, 0)
 
 
 def make_problem(mesh, A, diag=None):

@@ -25,6 +25,10 @@
  *   orc_transpose_apply,      known-answer tests of transpose/dg/mm_diagonal unit tests;
  *   orc_dg_apply,             bit-level comparison with the compiled reference kernels
  *   orc_mm_diagonal
+ *   orc_invert_local          known answers of invert_local_operator_kernel_mod_test.pf (1/5) and
+ *                             matrix_invert_mod_test.pf (a a^-1 = I); numpy.linalg.inv on
+ *                             nonsymmetric matrices; bit-level comparison with the reference's
+ *                             matrix_invert_lu compiled from source
  *   orc_bicgstab              numpy.linalg.solve on a tiny nonsymmetric system
  *   orc_fgmres / orc_gcr      one cycle = minimal residual over the right-preconditioned Krylov
  *                             space (numpy lstsq on an explicit basis); GCR(m) == FGMRES(m+1)
@@ -91,6 +95,10 @@ void orc_dg_apply(int ncells, int nlayers, int ndf1, int undf1, const int *map1,
                   double *y);
 void orc_mm_diagonal(int ncells, int nlayers, int ndf, int undf, const int *map,
                      const double *matrix, double *diag);
+/* per-cell dense inverse of a square operator (reference layout), FP64 and FP32 arithmetic */
+#define ORC_MAX_NDF 32
+int orc_invert_local(int64_t ncell3d, int ndf, const double *matrix, double *inv);
+int orc_invert_local_r32(int64_t ncell3d, int ndf, const float *matrix, float *inv);
 
 /* ---- Krylov solvers ------------------------------------------------------------------- */
 typedef struct orc_problem {

@@ -124,3 +124,49 @@ void orc_mm_diagonal(int ncells, int nl, int ndf, int undf, const int *map,
     }
   }
 }
+
+/* invert_local_operator_code (components/science/source/kernel/algebra/
+ * sci_invert_local_operator_kernel_mod.F90:51-77) over every 3D cell, each element inverted by
+ * matrix_invert_lu (infrastructure/source/utilities/matrix_invert_mod.F90:62-138 single,
+ * :140-216 double): Doolittle LU without pivoting into c (l below the diagonal, u on and above),
+ * then column m of the inverse by forward substitution on e_m and back substitution, in the
+ * reference's loop order.  Matrices in the reference layout matrix(ncell3d, ndf, ndf). */
+#define ORC_INVERT_BODY(T)                                                                     \
+  T c[ORC_MAX_NDF][ORC_MAX_NDF], y[ORC_MAX_NDF], x[ORC_MAX_NDF];                               \
+  for (int64_t ik = 0; ik < nc3; ++ik) {                                                       \
+    for (int i = 0; i < n; ++i)                                                                \
+      for (int j = 0; j < n; ++j) c[i][j] = matrix[ik + nc3 * (i + (int64_t)n * j)];           \
+    for (int m = 0; m < n - 1; ++m)                                                            \
+      for (int i = m + 1; i < n; ++i) {                                                        \
+        const T coeff = -(c[i][m] / c[m][m]);                                                  \
+        c[i][m] = -coeff; /* l(i,m) */                                                         \
+        for (int j = m + 1; j < n; ++j) c[i][j] = c[i][j] + coeff * c[m][j];                   \
+      }                                                                                        \
+    for (int m = 0; m < n; ++m) {                                                              \
+      y[0] = (m == 0) ? (T)1 : (T)0;                                                           \
+      for (int i = 1; i < n; ++i) {                                                            \
+        y[i] = (i == m) ? (T)1 : (T)0;                                                         \
+        for (int j = 0; j < i; ++j) y[i] = y[i] - y[j] * c[i][j];                              \
+      }                                                                                        \
+      x[n - 1] = y[n - 1] / c[n - 1][n - 1];                                                   \
+      for (int i = n - 2; i >= 0; --i) {                                                       \
+        x[i] = y[i];                                                                           \
+        for (int j = n - 1; j >= i + 1; --j) x[i] = x[i] - x[j] * c[i][j];                     \
+        x[i] = x[i] / c[i][i];                                                                 \
+      }                                                                                        \
+      for (int i = 0; i < n; ++i) inv[ik + nc3 * (i + (int64_t)n * m)] = x[i];                 \
+    }                                                                                          \
+  }
+
+int orc_invert_local(int64_t nc3, int n, const double *matrix, double *inv) {
+  if (n < 1 || n > ORC_MAX_NDF) return 1;
+  ORC_INVERT_BODY(double)
+  return 0;
+}
+
+int orc_invert_local_r32(int64_t nc3, int n, const float *matrix, float *inv) {
+  if (n < 1 || n > ORC_MAX_NDF) return 1;
+  ORC_INVERT_BODY(float)
+  return 0;
+}
+#undef ORC_INVERT_BODY

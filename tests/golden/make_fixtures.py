@@ -16,7 +16,10 @@ Everything written is data (inputs and expected outputs); no reference source te
                       that infrastructure/unit-test/support/fs_test_data_mod.F90 expects on
                       it (fs0k0_dofmap :24-33, fs2k0_dofmap :1026-1033, fs3k0_dofmap :1824-1826).
 * kat.json            the matrix_vector known-answer test
-                      (components/science/unit-test/kernel/algebra/matrix_vector_mod_test.pf:57-89)
+                      (components/science/unit-test/kernel/algebra/matrix_vector_mod_test.pf:57-89),
+                      the transpose / dg / mm_diagonal and invert_local_operator kernel tests,
+                      the matrix_invert_lu test
+                      (infrastructure/unit-test/utilities/matrix_invert_mod_test.pf:25,39-46)
                       and the C3 mesh-generator summary counts
                       (rose-stem/app/check_summary/file/summary_cubedsphere_c3_ugrid.kgo.txt:8-10).
 """
@@ -136,6 +139,15 @@ def main(ref):
             "mm": [[4.0, 1.0, 1.0], [2.0, 6.0, 2.0], [1.0, 1.0, 4.0]],  # mm(cell, i, j), same per cell
             "map": [[1, 2, 3], [3, 4, 5]], "undf": 5,
             "answer": [4.0, 6.0, 8.0, 6.0, 4.0], "tol": 1e-12,
+        },
+        # invert_local_operator_kernel_mod_test.pf:87-98: W3 (1 dof), 3 layers, every entry 5,
+        # inverse entry (1,1,1) = 1/5
+        "invert_local_operator": {"ndf": 1, "nlayers": 3, "value": 5.0, "answer": 0.2, "tol": 1e-6},
+        # matrix_invert_mod_test.pf:25,39-46: matrix_invert_lu of a 4x4 matrix, a a^-1 = I within tol
+        "matrix_invert": {
+            "a": (np.array([[4.0, 2.0, 2.0, 1.0], [2.0, 4.0, 1.0, 2.0], [2.0, 1.0, 4.0, 2.0],
+                            [1.0, 2.0, 2.0, 4.0]]) / 9.0).tolist(),
+            "tol": 1e-7,
         },
         "c3_summary": counts,
     }

@@ -128,6 +128,31 @@ def test_lma_apply_multi_rhs(hip, orc, n, nl, order, nrhs, prec):
             assert np.all(np.abs(Y[:, r] - yo) <= _bound(orc, m, A, X[:, r], 1e-13 if prec == 0 else 2e-6))
 
 
+@pytest.mark.parametrize("n,nl,nrhs", [(4, 3, 1), (4, 3, 4), (12, 7, 2), (5, 4, 8), (6, 2, 4)])
+def test_multi_rhs_mfma_is_bit_identical(hip, orc, n, nl, nrhs):
+    """The matrix-core multi-RHS apply (chains of v_mfma_f32_4x4x1f32) adds the same products in
+    the same order as the colour-ordered FMA kernel, so the two agree bit for bit, and both lie
+    within the FP32 bar of the oracle.  Operators it does not cover are refused."""
+    fs, m = _mesh_pair(hip, orc, n, nl)
+    op = hip.LmaOperator(fs).generate_spd(3, scaled=True).convert(hip.R32)
+    X = np.random.default_rng(n * 10 + nrhs).uniform(-1, 1, (m.undf, nrhs)).astype(np.float32).astype(np.float64)
+    xv = hip.FieldVector(fs, hip.R32, nrhs).import_field(X.ravel())
+    y2, y3 = hip.FieldVector(fs, hip.R32, nrhs), hip.FieldVector(fs, hip.R32, nrhs)
+    y2.set_scalar(1.0)
+    y3.set_scalar(-2.0)
+    op.apply_mrhs(xv, y2, 2)
+    op.apply_mrhs(xv, y3, 3)
+    np.testing.assert_array_equal(y3.export_raw(), y2.export_raw())
+    A = orc.gen_spd(8, m.ncell_3d, 3, scaled=True)
+    Y = y3.export_field().reshape(m.undf, nrhs)
+    for r in range(nrhs):
+        yo = orc.lma_apply(m, A.astype(np.float32), X[:, r].astype(np.float32), r32=True).astype(np.float64)
+        assert np.all(np.abs(Y[:, r] - yo) <= _bound(orc, m, A, X[:, r], 2e-6))
+    op64 = hip.LmaOperator(fs).generate_spd(3)
+    with pytest.raises(hip.LfricError):
+        op64.apply_mrhs(hip.FieldVector(fs, hip.R64, nrhs), hip.FieldVector(fs, hip.R64, nrhs), 3)
+
+
 @pytest.mark.parametrize("prec", [0, 1])
 def test_row_gathered_copy_is_bit_identical(hip, orc, prec):
     """The gather apply over the row-gathered copy adds the same products in the same order as
@@ -280,3 +305,46 @@ def test_solver_status_codes(hip, orc):
     with pytest.raises(hip.LfricError) as e:
         hip.ConjugateGradient(op, prec, 1e-8, 0.0, 10).apply(hip.FieldVector(fs), hip.FieldVector(fs).import_field(bn))
     assert e.value.code == 2
+
+
+@pytest.mark.parametrize("space,order,prec", [(3, 0, 0), (2, 0, 0), (0, 0, 0), (0, 1, 0), (3, 0, 1),
+                                              (2, 0, 1), (0, 0, 1), (0, 1, 1)])
+def test_invert_local_bitwise_equals_oracle(hip, orc, space, order, prec):
+    """Per-cell dense inverse (invert_local_operator_code with matrix_invert_lu): the device
+    repeats the reference's operation order without contraction, so every entry equals the
+    oracle's (itself bit-equal to the reference's compiled matrix_invert_lu) for FP64 and FP32;
+    spaces with and without top partners, register (<= 8 dofs) and workspace (27 dofs) forms."""
+    fs, m = _mesh_pair(hip, orc, 4, 3, space, order)
+    rng = np.random.default_rng(100 * space + 10 * order + prec)
+    A = rng.uniform(-1, 1, (m.ncell_3d, m.ndf, m.ndf)) + m.ndf * np.eye(m.ndf)
+    op = hip.LmaOperator(fs).upload(A)
+    if prec:
+        op = op.convert(hip.R32)
+        A = A.astype(np.float32).astype(np.float64)
+    inv = op.invert_local().download()
+    np.testing.assert_array_equal(inv, orc.invert_local(A, r32=bool(prec)).astype(np.float64))
+
+
+def test_invert_local_known_answer_and_refusals(hip, orc):
+    k = json.loads((GOLDEN / "kat.json").read_text())["invert_local_operator"]
+    fs, m = _mesh_pair(hip, orc, 3, k["nlayers"], 3, 0)
+    op = hip.LmaOperator(fs).upload(np.full((m.ncell_3d, 1, 1), k["value"]))
+    inv = op.invert_local().download()
+    assert np.all(np.abs(inv - k["answer"]) <= k["tol"])
+    w0 = hip.FunctionSpace.cubedsphere(3, k["nlayers"], 0, 0)
+    with pytest.raises(hip.LfricError):  # an operator between two spaces has no local inverse
+        hip.LmaOperator(fs, w0).invert_local()
+
+
+def test_row_copy_follows_operator_changes(hip, orc):
+    """An operator rewritten after its row-gathered copy was built applies its new entries."""
+    fs, m = _mesh_pair(hip, orc, 6, 4)
+    op = hip.LmaOperator(fs).generate_spd(1).convert(hip.R32).prepare_rows()
+    fresh = hip.LmaOperator(fs).generate_spd(2).convert(hip.R32)
+    x = hip.FieldVector(fs, hip.R32).fill_uniform(3)
+    y0, y1 = hip.FieldVector(fs, hip.R32), hip.FieldVector(fs, hip.R32)
+    hip.LmaOperator(fs).generate_spd(2).convert(hip.R32)  # unrelated operator: no effect
+    op.upload(fresh.download())
+    op.apply_gather(x, y0)
+    fresh.apply_gather(x, y1)
+    np.testing.assert_array_equal(y0.export_raw(), y1.export_raw())

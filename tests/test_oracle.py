@@ -413,3 +413,42 @@ def test_prec_only_is_one_division(orc):
     x, res, _ = orc.solve_callbacks("prec_only", len(b), lambda v: M @ v, lambda v: v / d, b)
     np.testing.assert_array_equal(x, b / d)
     assert abs(res.r_nrm - np.linalg.norm(b - M @ x)) <= 1e-13 * np.linalg.norm(b)
+
+
+# ---- per-cell dense inverse (invert_local_operator_code / matrix_invert_lu) -------------------
+def test_invert_local_known_answers(orc):
+    k = json.loads((GOLDEN / "kat.json").read_text())
+    t = k["invert_local_operator"]
+    A = np.full((6 * t["nlayers"], t["ndf"], t["ndf"]), t["value"])
+    inv = orc.invert_local(A)
+    assert abs(inv[0, 0, 0] - t["answer"]) <= t["tol"]
+    mi = k["matrix_invert"]
+    a = np.array(mi["a"])
+    ai = orc.invert_local(a[None])[0]
+    np.testing.assert_allclose(a @ ai, np.eye(4), rtol=0, atol=mi["tol"])
+
+
+@pytest.mark.parametrize("n", [1, 2, 6, 8, 27])
+def test_invert_local_matches_numpy_inverse(orc, n):
+    """Nonsymmetric, diagonally dominant elements (LU without pivoting is stable on them): the
+    restatement agrees with numpy.linalg.inv, so a transposed or mis-indexed result fails."""
+    rng = np.random.default_rng(n)
+    A = rng.uniform(-1, 1, (5, n, n)) + n * np.eye(n)
+    inv = orc.invert_local(A)
+    for e in range(5):
+        np.testing.assert_allclose(inv[e], np.linalg.inv(A[e]), rtol=1e-12, atol=1e-13)
+    inv32 = orc.invert_local(A, r32=True)
+    np.testing.assert_allclose(inv32, inv, rtol=2e-5, atol=2e-6)
+
+
+def test_invert_local_bitwise_equals_compiled_reference(orc):
+    R = _ref_lib()
+    P = lambda a: ctypes.c_void_p(a.ctypes.data)
+    for n in (1, 6, 8, 27):
+        A = np.random.default_rng(10 + n).uniform(-1, 1, (7, n, n)) + n * np.eye(n)
+        for r32, fn, dt in ((False, R.ref_invert_local_r64, np.float64), (True, R.ref_invert_local_r32, np.float32)):
+            Af = np.ascontiguousarray(A.astype(dt).transpose(2, 1, 0)).ravel()
+            out = np.zeros_like(Af)
+            fn(ctypes.c_int64(7), ctypes.c_int(n), P(Af), P(out))
+            ref = out.reshape((n, n, 7)).transpose(2, 1, 0)
+            np.testing.assert_array_equal(orc.invert_local(A, r32=r32), ref)
