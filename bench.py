@@ -109,9 +109,9 @@ def main():
                          "in one batch vs sequential FP64 solves; served: Poisson request stream with "
                          "continuous batching vs sequential FP64 solves")
     ap.add_argument("--problems", type=int, default=64)
-    ap.add_argument("--batch-mfma", type=int, default=1, choices=(0, 1),
-                    help="batched: multi-RHS apply of the lowest-order problems on the matrix cores (1) or in "
-                         "gather form (0)")
+    ap.add_argument("--batch-colour", type=int, default=1, choices=(0, 1),
+                    help="batched: colour-ordered multi-RHS apply (1: lowest order on the matrix cores, order 1 "
+                         "with register-staged x) or gather form for every problem (0)")
     ap.add_argument("--apply-form", type=int, default=1, choices=(0, 1),
                     help="LMA apply inside the solvers: 0 colour-ordered scatter, 1 gather with fused p.Ap")
     ap.add_argument("--serve-window", type=float, default=20.0, help="seconds of arrivals")
@@ -313,7 +313,7 @@ def bench_batched(args):
 
     def run_batch(tune):
         bt = Batch(r_tol=1e-8, a_tol=0.0, max_iter=4000, check_every=8)
-        bt.config(mfma=bool(args.batch_mfma))
+        bt.config(colour=bool(args.batch_colour))
         ids = []
         for p in probs:
             p["x"].set_scalar(0.0)
@@ -346,9 +346,9 @@ def bench_batched(args):
     say(f"warm-up batch {w:.2f} s, autotuned apply {tuned:.3f} ms: {st}")
     # the batch's multi-RHS apply of all 64 problems at once, matrix-core and gather forms
     apply_ms = {}
-    for form, on in (("mfma", True), ("gather", False)):
+    for form, on in (("colour_mfma", True), ("gather", False)):
         bt = Batch(r_tol=1e-8, a_tol=0.0, max_iter=4000, check_every=8)
-        bt.config(mfma=on)
+        bt.config(colour=on)
         for p in probs:
             p["x"].set_scalar(0.0)
             bt.add(p["op64"], p["op32"], p["d32"], p["x"], p["b"])
@@ -387,8 +387,8 @@ def bench_batched(args):
                    {"workload": f"{len(probs)} uneven W0 SPD problems (C12/C24/C48, 10/30/70 levels, order 0/1), "
                                 "element matrices B B^T + 0.1 I with B ~ N(0,1)/8, 4 RHS each, Jacobi CG to 1e-8 "
                                 "in FP64 refinement, one batch",
-                    "multi_rhs_apply": "matrix cores (order 0) / gather form (order 1)" if args.batch_mfma
-                    else "gather form",
+                    "multi_rhs_apply": "colour-ordered: matrix cores (order 0), register-staged FMA (order 1)"
+                    if args.batch_colour else "gather form",
                     "problems": len(probs), "rhs_total": sum(p["nrhs"] for p in probs),
                     "dofs_total": total_dofs, "parallelism": "one GPU, one batch"},
                    "f32 inner CG / f64 refinement")

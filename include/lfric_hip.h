@@ -334,9 +334,11 @@ int lfric_batch_remove(lfric_batch *b, int id);
 int lfric_batch_tune(lfric_batch *b, double *best_ms);
 /* use_graph 0/1 (-1 keeps), block 64/128/256 (0 keeps), grid_mult > 0 (0 keeps). */
 int lfric_batch_config(lfric_batch *b, int use_graph, int block, int grid_mult);
-/* Multi-RHS apply of lowest-order W0 problems (with at most 8 colours): 1 (default) colour-
- * ordered on the matrix cores (LFRIC_MRHS_MFMA), 0 gather form like the other problems. */
-int lfric_batch_set_mfma(lfric_batch *b, int on);
+/* Multi-RHS apply of the batch: 1 (default) colour-ordered where supported (FP32 W0 problems
+ * with at most 8 colours: lowest order on the matrix cores as LFRIC_MRHS_MFMA, order 1 one
+ * (cell, level) unit per thread as LFRIC_MRHS_COLOUR), gather form for the rest; 0 gather form
+ * for every problem. */
+int lfric_batch_set_apply(lfric_batch *b, int form);
 int lfric_batch_stats(const lfric_batch *b, long long *iterations, int *block, int *grid_mult, int *ntiles);
 void lfric_batch_destroy(lfric_batch *b);
 

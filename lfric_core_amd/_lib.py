@@ -129,7 +129,7 @@ SIGNATURES = {
     "lfric_batch_remove": (_I, [_VP, _I]),
     "lfric_batch_tune": (_I, [_VP, _DP]),
     "lfric_batch_config": (_I, [_VP, _I, _I, _I]),
-    "lfric_batch_set_mfma": (_I, [_VP, _I]),
+    "lfric_batch_set_apply": (_I, [_VP, _I]),
     "lfric_batch_stats": (_I, [_VP, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(_I), ctypes.POINTER(_I),
                                ctypes.POINTER(_I)]),
     "lfric_batch_destroy": (None, [_VP]),

@@ -65,12 +65,13 @@ class Batch:
         check(lib().lfric_batch_tune(self._h, ctypes.byref(ms)))
         return ms.value
 
-    def config(self, use_graph=-1, block=0, grid_mult=0, mfma=None):
-        """use_graph 0/1, block 64/128/256, grid_mult > 0 (-1/0 keep); mfma True/False: the
-        lowest-order W0 problems apply on the matrix cores (default) or in gather form."""
+    def config(self, use_graph=-1, block=0, grid_mult=0, colour=None):
+        """use_graph 0/1, block 64/128/256, grid_mult > 0 (-1/0 keep); colour True/False: the
+        W0 problems apply in colour-ordered form (lowest order on the matrix cores; default) or
+        every problem in gather form."""
         check(lib().lfric_batch_config(self._h, int(use_graph), int(block), int(grid_mult)))
-        if mfma is not None:
-            check(lib().lfric_batch_set_mfma(self._h, int(bool(mfma))))
+        if colour is not None:
+            check(lib().lfric_batch_set_apply(self._h, int(bool(colour))))
 
     def stats(self):
         it, blk, gm, nt = ctypes.c_longlong(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int()

@@ -21,6 +21,7 @@
 #include "device_common.h"
 #include "internal.h"
 #include "kernels.h"
+#include "colour_apply.h"
 #include "mfma_apply.h"
 
 namespace lfric {
@@ -147,6 +148,33 @@ __global__ __launch_bounds__(kBlock) void k_batch_apply_mfma(const ProbDev *__re
     const int pos = d.cpos0[colour] + lp;
     mfma_unit_w0<NR>(d.A, d.gs, static_cast<long long>(d.ebase[pos]) + m, d.map1 + static_cast<size_t>(pos) * 8,
                      d.map2 + static_cast<size_t>(pos) * 8, d.first[pos], d.p, d.q, m, d.nl, valid);
+  }
+}
+
+// q = A32 p in colour-ordered form, one (cell, level) unit per thread, for one colour of the
+// active order-1 W0 problems of one nrhs (tiles of 256 units); persistent like k_batch_apply.
+template <int NR>
+__global__ __launch_bounds__(kBlock) void k_batch_apply_colour27(const ProbDev *__restrict__ P,
+                                                                 const int *__restrict__ ctile0, int nprob, int colour,
+                                                                 int t_begin, int t_end, unsigned int *counter) {
+  __shared__ int s_tile;
+  for (;;) {
+    if (threadIdx.x == 0) s_tile = t_begin + static_cast<int>(atomicAdd(counter, 1u));
+    __syncthreads();
+    const int tile = __builtin_amdgcn_readfirstlane(s_tile);
+    __syncthreads();
+    if (tile >= t_end) return;
+    const ProbDev &d = P[find_problem(ctile0, nprob, tile)];
+    const int H = d.nl + 1;
+    const long long u = static_cast<long long>(tile - d.ctile0[colour]) * kBlock + threadIdx.x;
+    if (u < static_cast<long long>(d.cnpos[colour]) * H) {
+      const int lp = static_cast<int>(u / H);
+      const int m = static_cast<int>(u - static_cast<long long>(lp) * H);
+      const int pos = d.cpos0[colour] + lp;
+      colour_unit_mrhs<float, 27, 27, NR>(d.A, d.gs, static_cast<long long>(d.ebase[pos]) + m,
+                                          d.map1 + static_cast<size_t>(pos) * 27, d.map2 + static_cast<size_t>(pos) * 27,
+                                          d.first[pos], d.p, d.q, m, d.nl, RolesW0K1{});
+    }
   }
 }
 
@@ -394,10 +422,11 @@ struct lfric_batch {
   unsigned int *d_counter = nullptr;
   std::vector<lfric::Problem *> active;
   int ntiles = 0, nvtiles = 0;
-  // apply groups of equal (ndf, nrhs, form): gather tiles [begin, end), or for the
-  // matrix-core form tiles [cbegin[c], cend[c]) of each colour c
+  // apply groups of equal (ndf, nrhs, form): gather tiles [begin, end), or for the colour-
+  // ordered forms (1 matrix cores, 2 register-staged FMA) tiles [cbegin[c], cend[c]) of each
+  // colour c
   struct Group {
-    int ndf, nrhs, mfma, begin, end;
+    int ndf, nrhs, form, begin, end;
     int cbegin[lfric::kMaxC], cend[lfric::kMaxC];
   };
   int np_tables = 0;  // problems in the tables (row length of the colour tile tables - 1)
@@ -407,7 +436,7 @@ struct lfric_batch {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   int use_graph = 1;
-  int use_mfma = 1;  // lowest-order W0 problems apply on the matrix cores
+  int use_colour = 1;  // colour-ordered apply where supported (batch_colour_form)
   // launch shape of the apply (autotuned)
   int block = 256, grid_mult = 2, ncu = 256;
   long long replays = 0, iterations = 0;
@@ -444,10 +473,15 @@ int apply_tiles(const Problem &p, int block) {
   return static_cast<int>((pts + block - 1) / block);
 }
 
-// whether the problem's apply runs in colour-ordered form on the matrix cores
-bool uses_mfma(const lfric_batch &B, const Problem &p) {
-  return B.use_mfma && mfma_apply_supported(p.A32) && p.A32->to->ncolours <= kMaxC;
+// the problem's apply form in the batch: 0 gather, 1 colour-ordered on the matrix cores, 2
+// colour-ordered with register-staged x
+int apply_form(const lfric_batch &B, const Problem &p) {
+  if (!B.use_colour || p.A32->to->ncolours > kMaxC) return 0;
+  return batch_colour_form(p.A32);
 }
+
+// (cell, level) units per tile of a colour-ordered form
+int units_per_tile(int form) { return form == 1 ? kBlock / 4 : kBlock; }
 
 // state entries for a new problem (reusing a freed range of the same size), growing the array
 // with a device copy when needed
@@ -493,15 +527,15 @@ int rebuild(lfric_batch &B) {
   std::stable_sort(B.active.begin(), B.active.end(), [&B](const Problem *a, const Problem *b) {
     const int ka = a->A32->from->ndf, kb = b->A32->from->ndf;
     if (ka != kb) return ka > kb;
-    const bool ma = uses_mfma(B, *a), mb = uses_mfma(B, *b);
-    if (ma != mb) return mb;
+    const int fa = apply_form(B, *a), fb = apply_form(B, *b);
+    if (fa != fb) return fa < fb;
     if (a->nrhs != b->nrhs) return a->nrhs > b->nrhs;
     return a->n > b->n;
   });
   const int np = static_cast<int>(B.active.size());
   std::vector<ProbDev> desc(np);
   std::vector<int> tile0(np + 1, 0), vtile0(np + 1, 0), ctile0(static_cast<size_t>(kMaxC) * (np + 1), 0);
-  std::vector<char> mf(np, 0);
+  std::vector<int> mf(np, 0);
   for (int i = 0; i < np; ++i) {
     const Problem &p = *B.active[i];
     const FunctionSpace &to = *p.A32->to, &fr = *p.A32->from;
@@ -526,7 +560,7 @@ int rebuild(lfric_batch &B) {
     d.ndf = fr.ndf;
     d.nrhs = p.nrhs;
     for (int k = 0; k < 32; ++k) d.partner[k] = static_cast<int8_t>(k < to.ndf ? to.partner[k] : -1);
-    mf[i] = uses_mfma(B, p);
+    mf[i] = apply_form(B, p);
     d.map1 = to.d_map;
     d.first = to.d_first;
     for (int c = 0; c < kMaxC; ++c) {
@@ -536,7 +570,8 @@ int rebuild(lfric_batch &B) {
       d.cnpos[c] = has ? to.colour_start[c + 1] - to.colour_start[c] : 0;
       d.ctile0[c] = row[i];
       const long long units = static_cast<long long>(d.cnpos[c]) * (to.nlayers + 1);
-      row[i + 1] = row[i] + static_cast<int>((units + kBlock / 4 - 1) / (kBlock / 4));
+      const int upt = units_per_tile(mf[i]);
+      row[i + 1] = row[i] + static_cast<int>((units + upt - 1) / upt);
     }
     d.apply_tile0 = tile0[i];
     tile0[i + 1] = tile0[i] + (mf[i] ? 0 : apply_tiles(p, B.block));
@@ -550,11 +585,11 @@ int rebuild(lfric_batch &B) {
   B.groups.clear();
   for (int i = 0; i < np; ++i) {
     if (B.groups.empty() || B.groups.back().ndf != desc[i].ndf || B.groups.back().nrhs != desc[i].nrhs ||
-        B.groups.back().mfma != mf[i]) {
+        B.groups.back().form != mf[i]) {
       lfric_batch::Group g{};
       g.ndf = desc[i].ndf;
       g.nrhs = desc[i].nrhs;
-      g.mfma = mf[i];
+      g.form = mf[i];
       g.begin = g.end = tile0[i];
       for (int c = 0; c < kMaxC; ++c) g.cbegin[c] = g.cend[c] = ctile0[static_cast<size_t>(c) * (np + 1) + i];
       B.groups.push_back(g);
@@ -597,8 +632,8 @@ void launch_group(lfric_batch &B, int g, int cap) {
   }
 }
 
-// the colours of a matrix-core group, one persistent launch each
-void launch_group_mfma(lfric_batch &B, int g, int grid_mult) {
+// the colours of a colour-ordered group, one persistent launch each
+void launch_group_colour(lfric_batch &B, int g, int grid_mult) {
   const auto &G = B.groups[g];
   const int np = B.np_tables;
   const int cap = grid_mult * B.ncu * (1024 / kBlock);
@@ -607,12 +642,23 @@ void launch_group_mfma(lfric_batch &B, int g, int grid_mult) {
     const int grid = std::min(G.cend[c] - G.cbegin[c], cap);
     const int *tab = B.d_ctile0 + static_cast<size_t>(c) * (np + 1);
     unsigned int *cnt = B.d_counter + g * kMaxC + c;
-    switch (G.nrhs) {
-      case 1: LAUNCHB((k_batch_apply_mfma<1>), grid, kBlock, B.d_desc, tab, np, c, G.cbegin[c], G.cend[c], cnt); break;
-      case 2: LAUNCHB((k_batch_apply_mfma<2>), grid, kBlock, B.d_desc, tab, np, c, G.cbegin[c], G.cend[c], cnt); break;
-      case 4: LAUNCHB((k_batch_apply_mfma<4>), grid, kBlock, B.d_desc, tab, np, c, G.cbegin[c], G.cend[c], cnt); break;
-      default: LAUNCHB((k_batch_apply_mfma<8>), grid, kBlock, B.d_desc, tab, np, c, G.cbegin[c], G.cend[c], cnt); break;
+#define COLOUR_LAUNCH(K) LAUNCHB(K, grid, kBlock, B.d_desc, tab, np, c, G.cbegin[c], G.cend[c], cnt)
+    if (G.form == 1) {
+      switch (G.nrhs) {
+        case 1: COLOUR_LAUNCH(k_batch_apply_mfma<1>); break;
+        case 2: COLOUR_LAUNCH(k_batch_apply_mfma<2>); break;
+        case 4: COLOUR_LAUNCH(k_batch_apply_mfma<4>); break;
+        default: COLOUR_LAUNCH(k_batch_apply_mfma<8>); break;
+      }
+    } else {
+      switch (G.nrhs) {
+        case 1: COLOUR_LAUNCH(k_batch_apply_colour27<1>); break;
+        case 2: COLOUR_LAUNCH(k_batch_apply_colour27<2>); break;
+        case 4: COLOUR_LAUNCH(k_batch_apply_colour27<4>); break;
+        default: COLOUR_LAUNCH(k_batch_apply_colour27<8>); break;
+      }
     }
+#undef COLOUR_LAUNCH
   }
 }
 
@@ -620,8 +666,8 @@ template <int BS>
 void launch_apply_bs(lfric_batch &B, int grid_mult) {
   const int cap = grid_mult * B.ncu * (1024 / BS);
   for (int g = 0; g < static_cast<int>(B.groups.size()); ++g) {
-    if (B.groups[g].mfma) {
-      launch_group_mfma(B, g, grid_mult);
+    if (B.groups[g].form) {
+      launch_group_colour(B, g, grid_mult);
       continue;
     }
     if (B.groups[g].end == B.groups[g].begin) continue;
@@ -971,9 +1017,9 @@ int lfric_batch_config(lfric_batch *B, int use_graph, int block, int grid_mult) 
   return LFRIC_OK;
 }
 
-int lfric_batch_set_mfma(lfric_batch *B, int on) {
-  LF_CHECK(B, LFRIC_E_ARG, "null argument");
-  B->use_mfma = on ? 1 : 0;
+int lfric_batch_set_apply(lfric_batch *B, int form) {
+  LF_CHECK(B && (form == 0 || form == 1), LFRIC_E_ARG, "bad arguments");
+  B->use_colour = form;
   B->dirty = true;
   return LFRIC_OK;
 }

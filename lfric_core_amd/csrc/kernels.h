@@ -46,6 +46,9 @@ int launch_lma_apply_mrhs(const lfric_op *op, const void *x, void *y, int nrhs, 
 int launch_op_invert_local(const lfric_op *A, lfric_op *Ainv);
 // whether the MFMA multi-RHS apply (LFRIC_MRHS_MFMA) handles this operator
 bool mfma_apply_supported(const lfric_op *op);
+// colour-ordered form of a batch problem's apply: 1 matrix cores (lowest-order W0), 2 one unit
+// per thread with register-staged x (order-1 W0), 0 none (gather form)
+int batch_colour_form(const lfric_op *op);
 int launch_lma_transpose(const lfric_op *op, const void *x, void *y);
 int launch_dg_apply(const lfric_op *op, const void *x, void *y);
 int launch_mm_diagonal(const lfric_op *op, void *diag);

@@ -30,6 +30,7 @@
 
 #include "device_common.h"
 #include "kernels.h"
+#include "colour_apply.h"
 #include "mfma_apply.h"
 #include "rng.h"
 
@@ -264,28 +265,6 @@ __global__ void k_convert(TD *__restrict__ d, const TS *__restrict__ s, long lon
 // (the dofmap entity order vol, faces W,S,E,N,B,T, edges WB..NB, SW..NW, WT..NT, vertices
 // SWB..NWB, SWT..NWT), so every element-matrix load of a thread can be issued before the
 // first FMA; RolesRuntime carries any other pattern found in a caller's dofmap.
-struct RolesNone {
-  static constexpr bool kStatic = true;
-  __host__ __device__ static constexpr bool top(int) { return false; }
-  __host__ __device__ static constexpr int partner(int) { return -1; }
-};
-struct RolesW0K0 {  // 4 bottom vertices, 4 top vertices
-  static constexpr bool kStatic = true;
-  __host__ __device__ static constexpr bool top(int i) { return i >= 4; }
-  __host__ __device__ static constexpr int partner(int i) { return i < 4 ? i + 4 : -1; }
-};
-struct RolesW2K0 {  // faces W,S,E,N (height nl), B -> T
-  static constexpr bool kStatic = true;
-  __host__ __device__ static constexpr bool top(int i) { return i == 5; }
-  __host__ __device__ static constexpr int partner(int i) { return i == 4 ? 5 : -1; }
-};
-struct RolesW0K1 {  // B->T face, WB..NB -> WT..NT edges, bottom -> top vertices
-  static constexpr bool kStatic = true;
-  __host__ __device__ static constexpr bool top(int i) { return i == 6 || (i >= 15 && i <= 18) || i >= 23; }
-  __host__ __device__ static constexpr int partner(int i) {
-    return i == 5 ? 6 : (i >= 7 && i <= 10) ? i + 8 : (i >= 19 && i <= 22) ? i + 4 : -1;
-  }
-};
 struct RolesRuntime {
   static constexpr bool kStatic = false;
   Roles r;
@@ -605,21 +584,6 @@ __global__ __launch_bounds__(kBlock) void k_lma_gather_rows_w0(const T *__restri
 // Multi-RHS vectors hold NR values per dof, RHS fastest: (dof, r) at dof*NR + r.  Every RHS
 // is accumulated exactly as the one-RHS kernels do (same fma order), so column r of a
 // multi-RHS apply equals the one-RHS apply of column r bit for bit.
-template <typename T, int NR>
-struct RVec {
-  typedef T type __attribute__((ext_vector_type(NR)));
-};
-
-template <typename T, int N2, int NR>
-__device__ inline void row_dot_mrhs(const T *__restrict__ A, size_t gs, long long e, int r, const T *__restrict__ x,
-                                    const int (&b)[N2], int lev, typename RVec<T, NR>::type &acc) {
-  typedef typename RVec<T, NR>::type VR;
-  for_row<T, N2>(A, gs, e, r, [&](int j, T a) {
-    const VR xv = *reinterpret_cast<const VR *>(x + (static_cast<size_t>(b[j]) + lev) * NR);
-#pragma unroll
-    for (int k = 0; k < NR; ++k) acc[k] = fma(a, xv[k], acc[k]);
-  });
-}
 
 // gather form, NR right-hand sides per thread (see k_lma_gather)
 template <typename T, int N2, int NR, class R>
@@ -654,8 +618,8 @@ __global__ __launch_bounds__(kBlock) void k_lma_gather_mrhs(const T *__restrict_
   *reinterpret_cast<VR *>(y + (static_cast<size_t>(ci.x) + m) * NR) = acc;
 }
 
-// colour-ordered scatter form, NR right-hand sides per thread (see k_lma_colour): rows of
-// element e accumulate in c[i], then out = c[i] + c[partner(i)] is written or added
+// colour-ordered scatter form, NR right-hand sides per thread, one (cell, level) unit per thread
+// (see colour_apply.h)
 template <typename T, int N1, int N2, int NR, class R>
 __global__ __launch_bounds__(kBlock) void k_lma_colour_mrhs(const T *__restrict__ A, size_t gs,
                                                             const int *__restrict__ ebase, const T *__restrict__ x,
@@ -663,36 +627,14 @@ __global__ __launch_bounds__(kBlock) void k_lma_colour_mrhs(const T *__restrict_
                                                             const int *__restrict__ map2,
                                                             const uint32_t *__restrict__ first, int pos0, int npos,
                                                             int nl, R roles, int xwin) {
-  typedef typename RVec<T, NR>::type VR;
   const int H = nl + 1;
   const int t = xcd_block(blockIdx.x, gridDim.x, xwin) * kBlock + threadIdx.x;
   if (t >= npos * H) return;
   const int lp = t / H;
   const int m = t - lp * H;
   const int pos = pos0 + lp;
-  const bool has_m = m < nl, has_p = m > 0;
-  const long long e = static_cast<long long>(ebase[pos]) + m;
-  int b[N2];
-#pragma unroll
-  for (int j = 0; j < N2; ++j) b[j] = map2[static_cast<size_t>(pos) * N2 + j];
-  const uint32_t ft = first[pos];
-#pragma unroll
-  for (int i = 0; i < N1; ++i) {
-    if (roles.top(i)) continue;
-    const int pt = roles.partner(i);
-    if (!has_m && pt < 0) continue;
-    VR c = {}, cp = {};
-    if (has_m) row_dot_mrhs<T, N2, NR>(A, gs, e, i, x, b, m, c);
-    if (pt >= 0 && has_p) row_dot_mrhs<T, N2, NR>(A, gs, e, pt, x, b, m - 1, cp);
-    VR out;
-#pragma unroll
-    for (int k = 0; k < NR; ++k) out[k] = pt >= 0 ? c[k] + cp[k] : c[k];
-    VR *yd = reinterpret_cast<VR *>(y + (static_cast<size_t>(map1[static_cast<size_t>(pos) * N1 + i]) + m) * NR);
-    if ((ft >> i) & 1u)
-      *yd = out;
-    else
-      *yd = *yd + out;
-  }
+  colour_unit_mrhs<T, N1, N2, NR>(A, gs, static_cast<long long>(ebase[pos]) + m, map1 + static_cast<size_t>(pos) * N1,
+                                  map2 + static_cast<size_t>(pos) * N2, first[pos], x, y, m, nl, roles);
 }
 
 // Colour-ordered multi-RHS apply of a lowest-order W0 FP32 operator on the matrix cores (see
@@ -1527,6 +1469,13 @@ static void launch_mrhs_mfma_nr(const lfric_op *op, const float *x, float *y) {
 
 bool mfma_apply_supported(const lfric_op *op) {
   return op->precision == LFRIC_R32 && op->to == op->from && op->to->ndf == 8 && role_kind(*op->to) == RoleKind::W0K0;
+}
+
+int batch_colour_form(const lfric_op *op) {
+  if (mfma_apply_supported(op)) return 1;
+  if (op->precision == LFRIC_R32 && op->to == op->from && op->to->ndf == 27 && role_kind(*op->to) == RoleKind::W0K1)
+    return 2;
+  return 0;
 }
 
 static int mrhs_mfma(const lfric_op *op, const void *xv, void *yv, int nrhs) {
