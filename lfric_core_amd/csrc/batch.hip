@@ -223,22 +223,43 @@ __global__ __launch_bounds__(kVecBlock) void k_batch_pq(const ProbDev *__restric
   tile_sums<1>(v, d->nrhs, partials + static_cast<size_t>(blockIdx.x) * 2 * kMaxR);
 }
 
-// alpha = rz / p.q per (problem, RHS), from the tile partials in order
-__global__ void k_batch_alpha(const ProbDev *__restrict__ P, int nprob, const double *__restrict__ partials,
-                              RhsDev *__restrict__ S) {
-  const int pi = blockIdx.x, r = threadIdx.x;
+// Total of one partial-sum column (entry idx of rows t0 .. t0+nt-1, rows of 2*kMaxR doubles)
+// by one block of kFinBlock threads in a fixed order: thread k adds rows k, k+kFinBlock, ...
+// in turn, then a fixed pairwise tree over the threads.  The same for every launch, so a
+// problem's sums do not depend on the other problems of the batch.
+constexpr int kFinBlock = 256;
+__device__ inline double fixed_total(const double *__restrict__ partials, int t0, int nt, int idx) {
+  __shared__ double sh[kFinBlock];
+  double s = 0.0;
+  for (int t = threadIdx.x; t < nt; t += kFinBlock) s += partials[static_cast<size_t>(t0 + t) * 2 * kMaxR + idx];
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = kFinBlock / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+    __syncthreads();
+  }
+  const double tot = sh[0];
+  __syncthreads();
+  return tot;
+}
+
+// alpha = rz / p.q per (problem, RHS), from the tile partials; one block per problem
+__global__ __launch_bounds__(kFinBlock) void k_batch_alpha(const ProbDev *__restrict__ P, int nprob,
+                                                           const double *__restrict__ partials, RhsDev *__restrict__ S) {
+  const int pi = blockIdx.x;
   if (pi >= nprob) return;
   const ProbDev &d = P[pi];
-  if (r >= d.nrhs) return;
-  RhsDev &s = S[d.scal0 + r];
-  if (s.done) return;
-  double pq = 0.0;
-  for (int t = 0; t < d.vec_ntiles; ++t) pq += partials[static_cast<size_t>(d.vec_tile0 + t) * 2 * kMaxR + r];
-  if (pq != pq || pq == 0.0) {
-    s.done = 2;  // NaN or breakdown: stop this RHS (the host reports it)
-    return;
+  for (int r = 0; r < d.nrhs; ++r) {
+    RhsDev &s = S[d.scal0 + r];
+    if (s.done) continue;  // the same for every thread of the block
+    const double pq = fixed_total(partials, d.vec_tile0, d.vec_ntiles, r);
+    if (threadIdx.x == 0) {
+      if (pq != pq || pq == 0.0)
+        s.done = 2;  // NaN or breakdown: stop this RHS (the host reports it)
+      else
+        s.alpha = s.rz / pq;
+    }
   }
-  s.alpha = s.rz / pq;
 }
 
 // d += alpha p; r -= alpha q; z = r / diag; partials of r.r and r.z
@@ -269,20 +290,8 @@ __global__ __launch_bounds__(kVecBlock) void k_batch_update(const ProbDev *__res
   tile_sums<2>(v, nr, partials + static_cast<size_t>(blockIdx.x) * 2 * kMaxR);
 }
 
-// ||r||, convergence test and beta per (problem, RHS)
-__global__ void k_batch_beta(const ProbDev *__restrict__ P, int nprob, const double *__restrict__ partials,
-                             RhsDev *__restrict__ S, double a_tol, int max_iter) {
-  const int pi = blockIdx.x, r = threadIdx.x;
-  if (pi >= nprob) return;
-  const ProbDev &d = P[pi];
-  if (r >= d.nrhs) return;
-  RhsDev &s = S[d.scal0 + r];
-  if (s.done) return;
-  double rr = 0.0, rz = 0.0;
-  for (int t = 0; t < d.vec_ntiles; ++t) {
-    rr += partials[static_cast<size_t>(d.vec_tile0 + t) * 2 * kMaxR + r];
-    rz += partials[static_cast<size_t>(d.vec_tile0 + t) * 2 * kMaxR + kMaxR + r];
-  }
+// convergence test and beta of one (problem, RHS) from its r.r and r.z
+__device__ inline void beta_step(RhsDev &s, double rr, double rz, double a_tol, int max_iter) {
   s.iters += 1;
   const double rn = sqrt(rr);
   s.rn = rn;
@@ -297,6 +306,22 @@ __global__ void k_batch_beta(const ProbDev *__restrict__ P, int nprob, const dou
   s.beta = rz / s.rz;
   s.rz = rz;
 }
+
+// ||r||, convergence test and beta per (problem, RHS); one block per problem
+__global__ __launch_bounds__(kFinBlock) void k_batch_beta(const ProbDev *__restrict__ P, int nprob,
+                                                          const double *__restrict__ partials, RhsDev *__restrict__ S,
+                                                          double a_tol, int max_iter) {
+  const int pi = blockIdx.x;
+  if (pi >= nprob) return;
+  const ProbDev &d = P[pi];
+  for (int r = 0; r < d.nrhs; ++r) {
+    if (S[d.scal0 + r].done) continue;  // the same for every thread of the block
+    const double rr = fixed_total(partials, d.vec_tile0, d.vec_ntiles, r);
+    const double rz = fixed_total(partials, d.vec_tile0, d.vec_ntiles, kMaxR + r);
+    if (threadIdx.x == 0) beta_step(S[d.scal0 + r], rr, rz, a_tol, max_iter);
+  }
+}
+
 
 // p = z + beta p for the live RHS
 __global__ __launch_bounds__(kVecBlock) void k_batch_aypx(const ProbDev *__restrict__ P, const int *__restrict__ vtile0,
@@ -333,12 +358,12 @@ __global__ __launch_bounds__(kVecBlock) void k_ir_residual(double *__restrict__ 
   tile_sums<1>(v, nr, partials + static_cast<size_t>(blockIdx.x) * 2 * kMaxR);
 }
 
-__global__ void k_ir_norms(const double *__restrict__ partials, int nblocks, int nr, double *__restrict__ out) {
-  const int r = threadIdx.x;
-  if (r >= nr) return;
-  double s = 0.0;
-  for (int b = 0; b < nblocks; ++b) s += partials[static_cast<size_t>(b) * 2 * kMaxR + r];
-  out[r] = sqrt(s);
+__global__ __launch_bounds__(kFinBlock) void k_ir_norms(const double *__restrict__ partials, int nblocks, int nr,
+                                                        double *__restrict__ out) {
+  for (int r = 0; r < nr; ++r) {
+    const double s = fixed_total(partials, 0, nblocks, r);
+    if (threadIdx.x == 0) out[r] = sqrt(s);
+  }
 }
 
 // start a new inner solve on rhs = r64 / scale[r]: d = 0, r = rhs, z = r / diag, p = z,
@@ -363,23 +388,24 @@ __global__ __launch_bounds__(kVecBlock) void k_ir_start(const double *__restrict
   tile_sums<2>(v, nr, partials + static_cast<size_t>(blockIdx.x) * 2 * kMaxR);
 }
 
-__global__ void k_ir_start_fin(const double *__restrict__ partials, int nblocks, int nr, RhsDev *__restrict__ S,
-                               const double *__restrict__ tol, const int *__restrict__ live) {
-  const int r = threadIdx.x;
-  if (r >= nr) return;
-  double rr = 0.0, rz = 0.0;
-  for (int b = 0; b < nblocks; ++b) {
-    rr += partials[static_cast<size_t>(b) * 2 * kMaxR + r];
-    rz += partials[static_cast<size_t>(b) * 2 * kMaxR + kMaxR + r];
-  }
-  RhsDev &s = S[r];
+__device__ inline void start_rhs(RhsDev &s, double rr, double rz, double tol, int live) {
   s.rz = rz;
   s.r0 = sqrt(rr);
   s.rn = s.r0;
-  s.tol = tol[r];
+  s.tol = tol;
   s.alpha = s.beta = 0.0;
   s.iters = 0;
-  s.done = (live[r] && s.r0 > 0.0) ? 0 : 1;
+  s.done = (live && s.r0 > 0.0) ? 0 : 1;
+}
+
+__global__ __launch_bounds__(kFinBlock) void k_ir_start_fin(const double *__restrict__ partials, int nblocks, int nr,
+                                                            RhsDev *__restrict__ S, const double *__restrict__ tol,
+                                                            const int *__restrict__ live) {
+  for (int r = 0; r < nr; ++r) {
+    const double rr = fixed_total(partials, 0, nblocks, r);
+    const double rz = fixed_total(partials, 0, nblocks, kMaxR + r);
+    if (threadIdx.x == 0) start_rhs(S[r], rr, rz, tol[r], live[r]);
+  }
 }
 
 constexpr int kIrBlocks = 512;
@@ -693,9 +719,9 @@ int enqueue_iteration(lfric_batch &B) {
   const int np = static_cast<int>(B.active.size());
   launch_apply(B, B.block, B.grid_mult);                                                       // q = A p
   LAUNCHB(k_batch_pq, B.nvtiles, kVecBlock, B.d_desc, B.d_vtile0, np, B.d_partials);           // p.q
-  LAUNCHB(k_batch_alpha, np, 64, B.d_desc, np, B.d_partials, B.d_state);                      // alpha
+  LAUNCHB(k_batch_alpha, np, kFinBlock, B.d_desc, np, B.d_partials, B.d_state);               // alpha
   LAUNCHB(k_batch_update, B.nvtiles, kVecBlock, B.d_desc, B.d_vtile0, np, B.d_state, B.d_partials);
-  LAUNCHB(k_batch_beta, np, 64, B.d_desc, np, B.d_partials, B.d_state, 0.0, B.opts.max_iter);  // test, beta
+  LAUNCHB(k_batch_beta, np, kFinBlock, B.d_desc, np, B.d_partials, B.d_state, 0.0, B.opts.max_iter);  // test, beta
   LAUNCHB(k_batch_aypx, B.nvtiles, kVecBlock, B.d_desc, B.d_vtile0, np, B.d_state);            // p = z + beta p
   LF_HIP(hipGetLastError());
   return LFRIC_OK;
@@ -741,7 +767,7 @@ int refine(lfric_batch &B, Problem &p) {
   }
   if (int rc = launch_lma_apply_mrhs(p.A64, p.x->d, p.r64, nr, LFRIC_MRHS_GATHER)) return rc;
   LAUNCHB(k_ir_residual, kIrBlocks, kVecBlock, p.r64, static_cast<const double *>(p.b->d), n, nr, B.d_partials);
-  LAUNCHB(k_ir_norms, 1, 64, B.d_partials, kIrBlocks, nr, nrm);
+  LAUNCHB(k_ir_norms, 1, kFinBlock, B.d_partials, kIrBlocks, nr, nrm);
   LF_HIP(hipMemcpyAsync(B.h_scratch + 4 * kMaxR, nrm, sizeof(double) * nr, hipMemcpyDeviceToHost, ctx().stream));
   LF_HIP(hipStreamSynchronize(ctx().stream));
   bool all = true, nan = false;
@@ -772,7 +798,7 @@ int refine(lfric_batch &B, Problem &p) {
   LF_HIP(hipMemcpyAsync(B.d_scratch, B.h_scratch, sizeof(hv), hipMemcpyHostToDevice, ctx().stream));
   LAUNCHB(k_ir_start, kIrBlocks, kVecBlock, p.r64, inv, p.d, p.r, p.z, p.p, static_cast<const float *>(p.diag->d), n,
           nr, B.d_partials);
-  LAUNCHB(k_ir_start_fin, 1, 64, B.d_partials, kIrBlocks, nr, B.d_state + p.scal0, tol, live);
+  LAUNCHB(k_ir_start_fin, 1, kFinBlock, B.d_partials, kIrBlocks, nr, B.d_state + p.scal0, tol, live);
   LF_HIP(hipGetLastError());
   LF_HIP(hipStreamSynchronize(ctx().stream));
   if (!p.inner_running) B.dirty = true;
