@@ -163,7 +163,7 @@ int lfric_lma_apply_gather(const lfric_op *op, const lfric_vec *x, lfric_vec *y)
  * fastest (entry (dof, r) at dof*nrhs + r).  Each element matrix is read once for all
  * right-hand sides; column r of y equals the one-RHS apply of column r bit for bit for the
  * same variant.  No reference counterpart (the reference applies one field at a time). */
-#define LFRIC_MRHS_AUTO 0   /* the library's choice */
+#define LFRIC_MRHS_AUTO 0   /* the library's choice (by measured speed: colour from ~200k cells, else gather) */
 #define LFRIC_MRHS_GATHER 1 /* one launch, gather form */
 #define LFRIC_MRHS_COLOUR 2 /* colour-ordered scatter, one launch per colour */
 /* colour-ordered scatter on the matrix cores (v_mfma_f32_4x4x1f32), square lowest-order W0
