@@ -765,7 +765,7 @@ int refine(lfric_batch &B, Problem &p) {
     LF_HIP(hipMemcpyAsync(scale, B.h_scratch, sizeof(double) * nr, hipMemcpyHostToDevice, ctx().stream));
     LAUNCHB(k_ir_accumulate, vg, kBlock, static_cast<double *>(p.x->d), p.d, scale, n, nr);
   }
-  if (int rc = launch_lma_apply_mrhs(p.A64, p.x->d, p.r64, nr, LFRIC_MRHS_GATHER)) return rc;
+  if (int rc = launch_lma_apply_mrhs(p.A64, p.x->d, p.r64, nr, LFRIC_MRHS_AUTO)) return rc;
   LAUNCHB(k_ir_residual, kIrBlocks, kVecBlock, p.r64, static_cast<const double *>(p.b->d), n, nr, B.d_partials);
   LAUNCHB(k_ir_norms, 1, kFinBlock, B.d_partials, kIrBlocks, nr, nrm);
   LF_HIP(hipMemcpyAsync(B.h_scratch + 4 * kMaxR, nrm, sizeof(double) * nr, hipMemcpyDeviceToHost, ctx().stream));
