@@ -49,7 +49,8 @@ module lfric_hip_capi_mod
             lfric_vec_scale, lfric_vec_copy, lfric_vec_dot, lfric_vec_norm, lfric_vec_size, &
             lfric_vec_fill_uniform
   public :: lfric_op_create, lfric_op_upload, lfric_op_generate_spd, lfric_op_convert, &
-            lfric_op_destroy, lfric_lma_apply, lfric_mm_diagonal
+            lfric_op_destroy, lfric_lma_apply, lfric_mm_diagonal, lfric_op_invert_local, &
+            lfric_op_prepare_rows, lfric_op_tune
   public :: lfric_prec_create_diagonal, lfric_prec_create_identity, lfric_prec_apply, &
             lfric_prec_destroy
   public :: lfric_linop_create_lma, lfric_linop_destroy, lfric_prec_create_column_jacobi
@@ -193,6 +194,24 @@ module lfric_hip_capi_mod
     integer(c_int) function lfric_op_convert(src, dst) bind(c, name="lfric_op_convert")
       import :: c_int, c_ptr
       type(c_ptr), value :: src, dst
+    end function
+    ! invert_local_operator_code (sci_invert_local_operator_kernel_mod.F90:51-77)
+    integer(c_int) function lfric_op_invert_local(a, ainv) bind(c, name="lfric_op_invert_local")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: a, ainv
+    end function
+    ! row-gathered copy for the one-launch W0 apply (set-up, once per operator)
+    integer(c_int) function lfric_op_prepare_rows(op) bind(c, name="lfric_op_prepare_rows")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: op
+    end function
+    ! apply autotuner (XCD block-order window), form 0 colour / 1 gather
+    integer(c_int) function lfric_op_tune(op, form, best_window, best_ms) bind(c, name="lfric_op_tune")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: op
+      integer(c_int), value :: form
+      integer(c_int) :: best_window
+      real(c_double) :: best_ms
     end function
     subroutine lfric_op_destroy(op) bind(c, name="lfric_op_destroy")
       import :: c_ptr
