@@ -1,0 +1,78 @@
+"""GPU checks at the full size of the headline workload (C96 cubed sphere, 70 levels, W0,
+3,926,158 dofs, 3,870,720 cells), where the oracle would take minutes: size-independent
+properties instead of oracle values.
+
+* the gather apply over the row-gathered copy equals the cell-layout gather bit for bit
+  (FP32 and FP64), and the multi-RHS colour / matrix-core forms equal their one-RHS colour
+  apply column by column;
+* the operator is symmetric (x.Ay = y.Ax to FP64 rounding), as B B^T + 0.1 I element matrices
+  must make it;
+* the mixed-precision solve reaches an FP64 true relative residual <= 1e-8 and agrees with the
+  FP64 solve to the conditioning of the system.
+Small-size parity of every form against the oracle is in test_gpu_parity.py."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def c96(hip):
+    fs = hip.FunctionSpace.cubedsphere(96, 70)
+    assert (fs.undf, fs.ncell_3d) == (3926158, 3870720)
+    op64 = hip.LmaOperator(fs).generate_spd(0, scaled=True)
+    op32 = op64.convert(hip.R32)
+    return fs, op64, op32
+
+
+def test_fullsize_apply_forms_agree_bitwise(hip, c96):
+    fs, op64, op32 = c96
+    for op, P in ((op32, hip.R32), (op64, hip.R64)):
+        x = hip.FieldVector(fs, P).fill_uniform(7)
+        y0, y1 = hip.FieldVector(fs, P), hip.FieldVector(fs, P)
+        op.apply_gather(x, y0)
+        ref = y0.export_raw()
+        op.prepare_rows()
+        y1.set_scalar(-3.0)
+        op.apply_gather(x, y1)
+        np.testing.assert_array_equal(y1.export_raw(), ref)
+    # 4 right-hand sides: colour-ordered FMA and matrix-core forms against the one-RHS colour apply
+    X = np.random.default_rng(3).uniform(-1, 1, (fs.undf, 4)).astype(np.float32)
+    xm = hip.FieldVector(fs, hip.R32, 4).import_raw(X.ravel())
+    yc, ym = hip.FieldVector(fs, hip.R32, 4), hip.FieldVector(fs, hip.R32, 4)
+    op32.apply_mrhs(xm, yc, 2)
+    op32.apply_mrhs(xm, ym, 3)
+    Yc = yc.export_raw().reshape(fs.undf, 4)
+    np.testing.assert_array_equal(ym.export_raw().reshape(fs.undf, 4), Yc)
+    for r in (0, 3):
+        x1, y1 = hip.FieldVector(fs, hip.R32).import_raw(X[:, r]), hip.FieldVector(fs, hip.R32)
+        op32.apply(x1, y1)
+        np.testing.assert_array_equal(Yc[:, r], y1.export_raw())
+
+
+def test_fullsize_operator_is_symmetric(hip, c96):
+    fs, op64, _ = c96
+    x = hip.FieldVector(fs).fill_uniform(11)
+    y = hip.FieldVector(fs).fill_uniform(12)
+    ax, ay = hip.FieldVector(fs), hip.FieldVector(fs)
+    op64.apply(x, ax)
+    op64.apply(y, ay)
+    a, b = y.dot(ax), x.dot(ay)
+    assert abs(a - b) <= 1e-12 * max(abs(a), abs(b)), (a, b)
+
+
+def test_fullsize_mixed_solve_reaches_fp64_tolerance(hip, c96):
+    from lfric_core_amd.solvers import residual_norm
+    fs, op64, op32 = c96
+    b = hip.FieldVector(fs).fill_uniform(0, 3)
+    x_mixed, x_fp64 = hip.FieldVector(fs), hip.FieldVector(fs)
+    prec32 = hip.DiagonalPreconditioner.from_operator(op32)
+    prec64 = hip.DiagonalPreconditioner.from_operator(op64)
+    res = hip.MixedPrecisionCG(op64, op32, prec32, 1e-8, 0.0, 20000, inner_r_tol=1e-4, check_every=16).apply(x_mixed, b)
+    assert res.status == 0 and res.true_rel_res <= 1e-8
+    assert residual_norm(op64, x_mixed, b) / res.r_nrm_0 <= 1e-8
+    res64 = hip.ConjugateGradient(op64, prec64, 1e-8, 0.0, 20000, check_every=16).apply(x_fp64, b)
+    assert res64.status == 0
+    xm, x6 = x_mixed.export_field(), x_fp64.export_field()
+    # both residuals <= 1e-8 ||b||; the solutions differ by at most cond(A) x 2e-8 relative
+    assert np.linalg.norm(xm - x6) / np.linalg.norm(x6) <= 1e-5
