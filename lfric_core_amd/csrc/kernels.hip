@@ -580,6 +580,65 @@ __global__ __launch_bounds__(kBlock) void k_lma_gather_rows_w0(const T *__restri
   }
 }
 
+// Several right-hand sides over the row-gathered copy: each RHS accumulates exactly as
+// k_lma_gather_rows_w0 (and so as k_lma_gather_mrhs), x and y hold NR values per dof, RHS
+// fastest.  The matrix streams once for all right-hand sides.
+template <typename T, int NR>
+__global__ __launch_bounds__(kBlock) void k_lma_gather_rows_w0_mrhs(const T *__restrict__ rows, size_t gsr,
+                                                                    const T *__restrict__ x, T *__restrict__ y,
+                                                                    const int *__restrict__ map2,
+                                                                    const int2 *__restrict__ cols,
+                                                                    const int *__restrict__ off,
+                                                                    const int *__restrict__ ent, int ncols, int nl,
+                                                                    int xwin) {
+  typedef typename RVec<T, NR>::type VR;
+  constexpr int V = chunk_entries<T>(), RS = rows_rs<T, 8>();
+  typedef T VecT __attribute__((ext_vector_type(V)));
+  const int H = nl + 1;
+  const long long t = static_cast<long long>(xcd_block(blockIdx.x, gridDim.x, xwin)) * kBlock + threadIdx.x;
+  if (t >= static_cast<long long>(ncols) * H) return;
+  const int col = static_cast<int>(t / H);
+  const int m = static_cast<int>(t - static_cast<long long>(col) * H);
+  const int2 ci = cols[col];
+  if (m == nl && !ci.y) return;
+  const bool has_m = m < nl, has_p = m > 0;
+  const int q0 = off[col], cnt = off[col + 1] - q0;
+  const T *Rt = rows + static_cast<size_t>(t >> 6) * gsr + static_cast<size_t>(t & 63) * V;
+  VR acc = {};
+#pragma unroll
+  for (int k = 0; k < kRowsKmax; ++k) {
+    if (k >= cnt) break;
+    const int pos = ent[q0 + k] >> 5;
+    int b[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) b[j] = map2[static_cast<size_t>(pos) * 8 + j];
+    VR xl[3][4];  // x of the cell's 4 columns at levels m-1, m, m+1
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      xl[0][j] = has_p ? *reinterpret_cast<const VR *>(x + (static_cast<size_t>(b[j]) + m - 1) * NR) : VR{};
+      xl[1][j] = *reinterpret_cast<const VR *>(x + (static_cast<size_t>(b[j]) + m) * NR);
+      xl[2][j] = has_m ? *reinterpret_cast<const VR *>(x + (static_cast<size_t>(b[j]) + m + 1) * NR) : VR{};
+    }
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      if (h == 0 ? !has_p : !has_m) continue;
+#pragma unroll
+      for (int c = 0; c < RS / V; ++c) {
+        const VecT v = __builtin_nontemporal_load(
+            reinterpret_cast<const VecT *>(Rt + static_cast<size_t>((2 * k + h) * (RS / V) + c) * 64 * V));
+#pragma unroll
+        for (int q = 0; q < V; ++q) {
+          const int j = c * V + q;
+          const VR &xv = j < 4 ? xl[h][j] : xl[h + 1][j - 4];
+#pragma unroll
+          for (int r = 0; r < NR; ++r) acc[r] = fma(v[q], xv[r], acc[r]);
+        }
+      }
+    }
+  }
+  *reinterpret_cast<VR *>(y + (static_cast<size_t>(ci.x) + m) * NR) = acc;
+}
+
 // ---- several right-hand sides ------------------------------------------------------------
 // Multi-RHS vectors hold NR values per dof, RHS fastest: (dof, r) at dof*NR + r.  Every RHS
 // is accumulated exactly as the one-RHS kernels do (same fma order), so column r of a
@@ -1407,6 +1466,13 @@ template <typename T, int N2, int NR, class R>
 static void launch_gather_mrhs_r(const lfric_op *op, const T *x, T *y, R roles) {
   const FunctionSpace &to = *op->to, &from = *op->from;
   const int grid = grid_for(static_cast<long long>(to.gather_ncols) * (to.nlayers + 1));
+  if constexpr (N2 == 8 && std::is_same<R, RolesW0K0>::value) {
+    if (op->rows) {  // the row-gathered copy (lowest-order W0, launch_build_rows)
+      LAUNCH((k_lma_gather_rows_w0_mrhs<T, NR>), grid, static_cast<const T *>(op->rows), op->gsr, x, y, from.d_map,
+             to.d_gcol, to.d_goff, to.d_gent, to.gather_ncols, to.nlayers, op->xwin);
+      return;
+    }
+  }
   LAUNCH((k_lma_gather_mrhs<T, N2, NR, R>), grid, static_cast<const T *>(op->d), op->gs, to.d_ebase, x, y,
          from.d_map, to.d_gcol, to.d_goff, to.d_gent, to.gather_ncols, to.nlayers, roles, op->xwin);
 }
@@ -1516,11 +1582,12 @@ static int mrhs_mfma(const lfric_op *op, const void *xv, void *yv, int nrhs) {
 
 int launch_lma_apply_mrhs(const lfric_op *op, const void *x, void *y, int nrhs, int variant) {
   if (variant == LFRIC_MRHS_AUTO) {
-    // measured: the colour-ordered form with register-staged x is the fastest from a few
+    // measured: an operator with a row-gathered copy is fastest in the gather form over the
+    // copy; otherwise the colour-ordered form with register-staged x is the fastest from a few
     // hundred thousand cells up (C96x70 W0, 4 FP32 RHS: 0.310 ms vs gather 0.354); on small
     // meshes its one launch per colour leaves the GPU idle and the gather form wins
     const long long nc3 = static_cast<long long>(op->to->ncells) * op->to->nlayers;
-    variant = (nrhs > 1 && nc3 >= 200000) ? LFRIC_MRHS_COLOUR : LFRIC_MRHS_GATHER;
+    variant = (nrhs > 1 && nc3 >= 200000 && !op->rows) ? LFRIC_MRHS_COLOUR : LFRIC_MRHS_GATHER;
   }
   LF_CHECK(variant == LFRIC_MRHS_GATHER || variant == LFRIC_MRHS_COLOUR || variant == LFRIC_MRHS_MFMA, LFRIC_E_ARG,
            "unknown multi-RHS variant");

@@ -156,7 +156,8 @@ def test_multi_rhs_mfma_is_bit_identical(hip, orc, n, nl, nrhs):
 @pytest.mark.parametrize("prec", [0, 1])
 def test_row_gathered_copy_is_bit_identical(hip, orc, prec):
     """The gather apply over the row-gathered copy adds the same products in the same order as
-    over the cell layout; CG with the fused p.(A p) then takes the same iterations."""
+    over the cell layout, for one and for several right-hand sides; CG with the fused p.(A p)
+    then takes the same iterations."""
     fs, m = _mesh_pair(hip, orc, 12, 9)
     op64 = hip.LmaOperator(fs).generate_spd(4)
     op = op64 if prec == 0 else op64.convert(hip.R32)
@@ -165,7 +166,17 @@ def test_row_gathered_copy_is_bit_identical(hip, orc, prec):
     y0, y1 = hip.FieldVector(fs, P), hip.FieldVector(fs, P)
     op.apply_gather(xv, y0)
     before = y0.export_field()
+    xm = {nr: hip.FieldVector(fs, P, nr).fill_uniform(5 + nr, 3) for nr in (2, 4, 8)}
+    ym = {nr: hip.FieldVector(fs, P, nr) for nr in (2, 4, 8)}
+    mbefore = {}
+    for nr in (2, 4, 8):
+        op.apply_mrhs(xm[nr], ym[nr], 1)
+        mbefore[nr] = ym[nr].export_field()
     op.prepare_rows()
+    for nr in (2, 4, 8):
+        ym[nr].set_scalar(7.0)
+        op.apply_mrhs(xm[nr], ym[nr], 1)
+        np.testing.assert_array_equal(ym[nr].export_field(), mbefore[nr])
     for w in (0, 4):
         op.set_window(w)
         y1.set_scalar(5.0)
