@@ -84,6 +84,22 @@ def poisson_stream(rate, window_s, seed=0, meshes=(24, 48, 96, 192), weights=(0.
                            int(rng.integers(0, 2**31))))
 
 
+def admissions(waiting, used, budget, now, starve_s):
+    """The requests of `waiting` (in arrival order) to admit at time `now` with `used` of
+    `budget` bytes taken: arrival order with backfill - a request that does not fit is passed
+    over by later ones that do - until the head of the queue has waited more than `starve_s`;
+    then nothing behind a request that does not fit is admitted."""
+    out = []
+    head_starved = bool(waiting) and now - waiting[0].t_arrive > starve_s
+    for r in waiting:
+        if used + r.footprint <= budget:
+            out.append(r)
+            used += r.footprint
+        elif head_starved:
+            break
+    return out
+
+
 class _Live:
     """Device objects of one admitted request."""
 
@@ -157,19 +173,15 @@ class SolveServer:
                 waiting.append(r)
             # admission: arrival order with backfill, unless the head has starved
             if now < wall_limit_s:
-                head_starved = bool(waiting) and now - waiting[0].t_arrive > self.starve_s
-                for r in list(waiting):
-                    if used + r.footprint <= self.budget:
-                        waiting.remove(r)
-                        r.t_admit = clock() - t0
-                        lv = _Live(r)
-                        bt = self._batch(r.tol)
-                        pid = bt.add(lv.op64, lv.op32, lv.diag32, lv.x, lv.b)
-                        live[(r.tol, pid)] = lv
-                        used += r.footprint
-                        peak = max(peak, used)
-                    elif head_starved:
-                        break
+                for r in admissions(waiting, used, self.budget, now, self.starve_s):
+                    waiting.remove(r)
+                    r.t_admit = clock() - t0
+                    lv = _Live(r)
+                    bt = self._batch(r.tol)
+                    pid = bt.add(lv.op64, lv.op32, lv.diag32, lv.x, lv.b)
+                    live[(r.tol, pid)] = lv
+                    used += r.footprint
+                    peak = max(peak, used)
             if not live:
                 if not pending and not waiting:
                     break
