@@ -77,6 +77,8 @@ def lib():
             "orc_mm_diagonal": (None, [_I, _I, _I, _I, _VP, _VP, _VP]),
             "orc_invert_local": (_I, [_I64, _I, _VP, _VP]),
             "orc_invert_local_r32": (_I, [_I64, _I, _VP, _VP]),
+            "orc_gauss_rule": (_I, [_I, _VP, _VP]),
+            "orc_mass_matrix_w0": (_I, [_I, _I, _VP, _VP, _VP, _VP, _I, _VP]),
             "orc_cg": (_I, [ctypes.POINTER(OrcProblem), _VP, _VP, _D, _D, _I, _I, ctypes.POINTER(OrcResult), _VP]),
             "orc_gmres": (_I, [ctypes.POINTER(OrcProblem), _VP, _VP, _I, _D, _D, _I, _I, ctypes.POINTER(OrcResult), _VP]),
             "orc_bicgstab": (_I, [ctypes.POINTER(OrcProblem), _VP, _VP, _D, _D, _I, _I, ctypes.POINTER(OrcResult), _VP]),
@@ -244,6 +246,23 @@ def invert_local(A, r32=False):
     if fn(nc3, n, _p(Af), _p(out)) != 0:
         raise ValueError("invert_local: ndf out of range")
     return out.reshape((n, n, nc3)).transpose(2, 1, 0)
+
+
+def gauss_rule(n):
+    p, w = np.zeros(n), np.zeros(n)
+    if lib().orc_gauss_rule(n, _p(p), _p(w)) != 0:
+        raise ValueError("gauss_rule: n out of range")
+    return p, w
+
+
+def mass_matrix_w0(ncells, nlayers, dofmap, chi1, chi2, chi3, nqp=3):
+    """Lowest-order W0 mass matrices (ncells*nlayers, 8, 8) from Cartesian chi fields."""
+    dm = np.ascontiguousarray(dofmap, np.int32)
+    c = [np.ascontiguousarray(v, np.float64) for v in (chi1, chi2, chi3)]
+    out = np.zeros(ncells * nlayers * 64)
+    if lib().orc_mass_matrix_w0(ncells, nlayers, _p(dm), _p(c[0]), _p(c[1]), _p(c[2]), nqp, _p(out)) != 0:
+        raise ValueError("mass_matrix_w0: nqp out of range")
+    return out.reshape((8, 8, ncells * nlayers)).transpose(2, 1, 0)
 
 
 def make_problem(mesh, A, diag=None):

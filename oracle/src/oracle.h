@@ -25,6 +25,12 @@
  *   orc_transpose_apply,      known-answer tests of transpose/dg/mm_diagonal unit tests;
  *   orc_dg_apply,             bit-level comparison with the compiled reference kernels
  *   orc_mm_diagonal
+ *   orc_gauss_rule            numpy.polynomial.legendre.leggauss mapped to [0, 1]; the 3-point
+ *                             weights of get_unit_test_q3x3x3_quadrature_mod.f90
+ *   orc_mass_matrix_w0        known answer of compute_mass_matrix_kernel_w0_mod_test.pf (a
+ *                             6000 x 1000 x 2000 box); closed forms V (M1 x M1 x M1) for boxes
+ *                             and det(J) M_ref for sheared parallelepipeds; sum of entries =
+ *                             cell volume
  *   orc_invert_local          known answers of invert_local_operator_kernel_mod_test.pf (1/5) and
  *                             matrix_invert_mod_test.pf (a a^-1 = I); numpy.linalg.inv on
  *                             nonsymmetric matrices; bit-level comparison with the reference's
@@ -99,6 +105,11 @@ void orc_mm_diagonal(int ncells, int nlayers, int ndf, int undf, const int *map,
 #define ORC_MAX_NDF 32
 int orc_invert_local(int64_t ncell3d, int ndf, const double *matrix, double *inv);
 int orc_invert_local_r32(int64_t ncell3d, int ndf, const float *matrix, float *inv);
+/* Gauss-Legendre points and weights on [0, 1]; lowest-order W0 mass matrix from Cartesian
+ * vertex coordinates (chi1..3 W0 fields), nqp Gauss points per direction */
+int orc_gauss_rule(int n, double *pts, double *wts);
+int orc_mass_matrix_w0(int ncells, int nlayers, const int *map, const double *chi1, const double *chi2,
+                       const double *chi3, int nqp, double *mm);
 
 /* ---- Krylov solvers ------------------------------------------------------------------- */
 typedef struct orc_problem {

@@ -8,6 +8,7 @@
  * the reference's vector statements.
  */
 #include "oracle.h"
+#include <math.h>
 #include <string.h>
 
 #define MAT(ik, df, df2, nc3, ndf1) matrix[(ik) + (int64_t)(nc3) * ((df) + (int64_t)(ndf1) * (df2))]
@@ -170,3 +171,110 @@ int orc_invert_local_r32(int64_t nc3, int n, const float *matrix, float *inv) {
   return 0;
 }
 #undef ORC_INVERT_BODY
+
+/* Gauss-Legendre rule of n points on [0, 1]: quadrature_rule of
+ * infrastructure/source/quadrature/quadrature_rule_gaussian_mod.F90:35-94 (Newton iteration on
+ * the Legendre recurrence from cos(pi (i - 1/4) / (n + 1/2)) until |z - z1| <= epsilon, z1
+ * carried over from the previous root, then the map from [-1, 1]).  The restatement caps the
+ * iteration at 100 steps. */
+int orc_gauss_rule(int n, double *pts, double *wts) {
+  if (n < 1 || n > 16) return 1;
+  const double pi = 4.0 * atan(1.0), eps = 2.220446049250313e-16;
+  const int m = (n + 1) / 2;
+  double z1 = 0.0;  // set once: each root's iteration starts from the previous root
+  for (int i = 1; i <= m; ++i) {
+    double z = cos(pi * (i - 0.25) / (n + 0.5)), pp = 1.0;
+    for (int it = 0; it < 100 && fabs(z - z1) > eps; ++it) {
+      double p1 = 1.0, p2 = 0.0, p3;
+      for (int j = 1; j <= n; ++j) {
+        p3 = p2;
+        p2 = p1;
+        p1 = ((2.0 * j - 1.0) * z * p2 - (j - 1.0) * p3) / j;
+      }
+      pp = n * (z * p1 - p2) / (z * z - 1.0);
+      z1 = z;
+      z = z1 - p1 / pp;
+    }
+    pts[i - 1] = -z;
+    pts[n - i] = +z;
+    wts[i - 1] = 2.0 / ((1.0 - z * z) * pp * pp);
+    wts[n - i] = wts[i - 1];
+  }
+  for (int i = 0; i < n; ++i) {
+    pts[i] = 0.5 * (pts[i] + 1.0);
+    wts[i] = 0.5 * wts[i];
+  }
+  return 0;
+}
+
+/* Lowest-order W0 mass matrix from Cartesian vertex coordinates: compute_mass_matrix_w_scalar_code
+ * (components/science/source/kernel/fem/sci_compute_mass_matrix_kernel_w_scalar_mod.F90:290-384,
+ * r_double) with the coordinate Jacobian of coordinate_jacobian for coord_system_xyz
+ * (components/science/source/kernel/geometry/sci_coordinate_jacobian_mod.F90:289-445: jac(i,dir)
+ * = sum_df chi_i(df) d_dir N_df, dj by cofactors).  Quadrature xyoz: the horizontal points pair
+ * x_i with y_j, y fastest (infrastructure/source/quadrature/quadrature_xyoz_mod.F90:235-243).
+ * Trilinear basis on the reference cube, vertices SWB, SEB, NEB, NWB, SWT, SET, NET, NWT
+ * (reference_element_mod.F90:1244-1251): N = lx*ly*lz with l = t or 1 - t, derivatives with
+ * +-1 in place of the differentiated factor.  Entries df >= df2 are summed over (qp_v outer,
+ * qp_h inner) of wh*wv*N_df*N_df2*dj and mirrored.  chi1..3 are W0 fields (undf); map is the
+ * W0 dofmap (1-based, [cell][8]); mm in the reference layout (ncell_3d, 8, 8). */
+int orc_mass_matrix_w0(int ncells, int nl, const int *map, const double *chi1, const double *chi2,
+                       const double *chi3, int nqp, double *mm) {
+  enum { NQMAX = 8 };
+  if (nqp < 1 || nqp > NQMAX) return 1;
+  double p[NQMAX], w[NQMAX];
+  orc_gauss_rule(nqp, p, w);
+  const int nh = nqp * nqp, nv = nqp;
+  static const int vx[8] = {0, 1, 1, 0, 0, 1, 1, 0}, vy[8] = {0, 0, 1, 1, 0, 0, 1, 1},
+                   vz[8] = {0, 0, 0, 0, 1, 1, 1, 1};
+  double b[8][NQMAX * NQMAX][NQMAX], db[3][8][NQMAX * NQMAX][NQMAX], wh[NQMAX * NQMAX];
+  for (int i = 0; i < nqp; ++i)
+    for (int j = 0; j < nqp; ++j) {
+      const int ic = i * nqp + j;
+      wh[ic] = w[i] * w[j];
+      for (int kz = 0; kz < nv; ++kz)
+        for (int df = 0; df < 8; ++df) {
+          const double x = p[i], y = p[j], z = p[kz];
+          const double lx = vx[df] ? x : 1.0 - x, ly = vy[df] ? y : 1.0 - y, lz = vz[df] ? z : 1.0 - z;
+          const double dx = vx[df] ? 1.0 : -1.0, dy = vy[df] ? 1.0 : -1.0, dz = vz[df] ? 1.0 : -1.0;
+          b[df][ic][kz] = lx * ly * lz;
+          db[0][df][ic][kz] = dx * ly * lz;
+          db[1][df][ic][kz] = lx * dy * lz;
+          db[2][df][ic][kz] = lx * ly * dz;
+        }
+    }
+  const int64_t nc3 = (int64_t)ncells * nl;
+  for (int cell = 0; cell < ncells; ++cell) {
+    const int *mc = &map[(size_t)cell * 8];
+    for (int k = 0; k < nl; ++k) {
+      const int64_t ik = (int64_t)cell * nl + k;
+      double c[3][8], dj[NQMAX * NQMAX][NQMAX];
+      for (int df = 0; df < 8; ++df) {
+        c[0][df] = chi1[mc[df] - 1 + k];
+        c[1][df] = chi2[mc[df] - 1 + k];
+        c[2][df] = chi3[mc[df] - 1 + k];
+      }
+      for (int kz = 0; kz < nv; ++kz)
+        for (int ic = 0; ic < nh; ++ic) {
+          double J[3][3] = {{0}};
+          for (int df = 0; df < 8; ++df)
+            for (int dir = 0; dir < 3; ++dir)
+              for (int r = 0; r < 3; ++r) J[r][dir] = J[r][dir] + c[r][df] * db[dir][df][ic][kz];
+          dj[ic][kz] = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) -
+                       J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
+                       J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
+        }
+      for (int df2 = 0; df2 < 8; ++df2) {
+        for (int df = df2; df < 8; ++df) {
+          double s = 0.0;
+          for (int kz = 0; kz < nv; ++kz)
+            for (int ic = 0; ic < nh; ++ic)
+              s = s + wh[ic] * w[kz] * b[df][ic][kz] * b[df2][ic][kz] * dj[ic][kz];
+          mm[ik + nc3 * (df + 8 * (int64_t)df2)] = s;
+        }
+        for (int df = df2 - 1; df >= 0; --df) mm[ik + nc3 * (df + 8 * (int64_t)df2)] = mm[ik + nc3 * (df2 + 8 * (int64_t)df)];
+      }
+    }
+  }
+  return 0;
+}

@@ -19,7 +19,9 @@ Everything written is data (inputs and expected outputs); no reference source te
                       (components/science/unit-test/kernel/algebra/matrix_vector_mod_test.pf:57-89),
                       the transpose / dg / mm_diagonal and invert_local_operator kernel tests,
                       the matrix_invert_lu test
-                      (infrastructure/unit-test/utilities/matrix_invert_mod_test.pf:25,39-46)
+                      (infrastructure/unit-test/utilities/matrix_invert_mod_test.pf:25,39-46) and
+                      the W0 mass-matrix test
+                      (components/science/unit-test/kernel/fem/compute_mass_matrix_kernel_w0_mod_test.pf)
                       and the C3 mesh-generator summary counts
                       (rose-stem/app/check_summary/file/summary_cubedsphere_c3_ugrid.kgo.txt:8-10).
 """
@@ -114,6 +116,20 @@ def main(ref):
     }
     (HERE / "canned_mesh.json").write_text(json.dumps(canned) + "\n")
 
+    # compute_mass_matrix_kernel_w0_mod_test.pf:326-469: cell 1 of the 3x3 mesh of
+    # get_unit_test_m3x3_dofmap_mod.f90:37-69, coordinates of get_unit_test_3x3x3_chi_mod.f90:25-62
+    # (box dx x dy x dz), answer mass_matrix_w0_r64 (:522-594) normalised by dx*dy*dz
+    mm_text = (ref / "components/science/unit-test/kernel/fem/compute_mass_matrix_kernel_w0_mod_test.pf").read_text()
+    body = mm_text[mm_text.index("subroutine mass_matrix_w0_r64"):]
+    body = body[:body.index("end subroutine mass_matrix_w0_r64")]
+    vals = [float(v) for v in re.findall(r"([0-9.]+E[+-][0-9]+)_r_double", body)]
+    assert len(vals) == 64
+    dims = {k: float(re.search(k + r"\s*=\s*([0-9.]+)_r_def", mm_text).group(1)) for k in ("dx", "dy", "dz")}
+    dm_text = (ref / "components/science/unit-test/support/get_unit_test_m3x3_dofmap_mod.f90").read_text()
+    dm_body = dm_text[dm_text.index("subroutine get_w0_m3x3_dofmap"):]
+    dm_body = dm_body[dm_body.index("reshape"):dm_body.index("[8,9]")]
+    table = [int(v) for v in re.findall(r"\b(\d+)\b", dm_body)]
+    assert len(table) == 72
     d = np.array([[2.0, -2.0], [3.0, -1.0], [0.4, 1.3], [-4.5, 7.9]])
     x = np.array([0.25, 1.5])
     summary = (ref / "rose-stem/app/check_summary/file/summary_cubedsphere_c3_ugrid.kgo.txt").read_text()
@@ -148,6 +164,11 @@ def main(ref):
             "a": (np.array([[4.0, 2.0, 2.0, 1.0], [2.0, 4.0, 1.0, 2.0], [2.0, 1.0, 4.0, 2.0],
                             [1.0, 2.0, 2.0, 4.0]]) / 9.0).tolist(),
             "tol": 1e-7,
+        },
+        "mass_matrix_w0": {
+            "vertices_of_cell": [table[8 * c: 8 * c + 4] for c in range(9)], "nlayers": 3,
+            "dx": dims["dx"], "dy": dims["dy"], "dz": dims["dz"],
+            "answer_column_major": vals, "tol": 1e-6,
         },
         "c3_summary": counts,
     }

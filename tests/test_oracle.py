@@ -452,3 +452,95 @@ def test_invert_local_bitwise_equals_compiled_reference(orc):
             fn(ctypes.c_int64(7), ctypes.c_int(n), P(Af), P(out))
             ref = out.reshape((n, n, 7)).transpose(2, 1, 0)
             np.testing.assert_array_equal(orc.invert_local(A, r32=r32), ref)
+
+
+# ---- lowest-order W0 mass matrix from coordinates (compute_mass_matrix_w_scalar_code) ---------
+_VX = np.array([0, 1, 1, 0, 0, 1, 1, 0])
+_VY = np.array([0, 0, 1, 1, 0, 0, 1, 1])
+_VZ = np.array([0, 0, 0, 0, 1, 1, 1, 1])
+
+
+def _m_ref():
+    m1 = np.array([[1 / 3, 1 / 6], [1 / 6, 1 / 3]])
+    return m1[_VX][:, _VX] * m1[_VY][:, _VY] * m1[_VZ][:, _VZ]
+
+
+def _one_cell(X):
+    """One cell, one layer: vertex columns 1..4 with bottom/top dofs (1,2), (3,4), ..."""
+    dm = np.array([[1, 3, 5, 7, 2, 4, 6, 8]], np.int32)
+    chi = np.zeros((3, 8))
+    for v in range(8):
+        chi[:, dm[0, v] - 1] = X[v]
+    return dm, chi
+
+
+def test_gauss_rule_matches_numpy_and_the_reference_weights(orc):
+    for n in range(1, 9):
+        p, w = orc.gauss_rule(n)
+        x, wl = np.polynomial.legendre.leggauss(n)
+        np.testing.assert_allclose(p, 0.5 * (x + 1), rtol=0, atol=4e-16)
+        np.testing.assert_allclose(w, 0.5 * wl, rtol=0, atol=2e-15)  # weights from the last Newton step
+    # get_unit_test_q3x3x3_quadrature_mod.f90: 3x3 horizontal weights, 15 digits
+    p, w = orc.gauss_rule(3)
+    np.testing.assert_allclose(np.outer(w, w).ravel()[:3], [0.077160493827160, 0.123456790123456, 0.077160493827160],
+                               atol=1e-14)
+
+
+def test_mass_matrix_w0_known_answer(orc):
+    k = json.loads((GOLDEN / "kat.json").read_text())["mass_matrix_w0"]
+    nl, dx, dy, dz = k["nlayers"], k["dx"], k["dy"], k["dz"]
+    verts = np.array(k["vertices_of_cell"])
+    dm = np.zeros((9, 8), np.int32)
+    dm[:, :4] = verts * (nl + 1) + 1
+    dm[:, 4:] = verts * (nl + 1) + 2
+    undf = 9 * (nl + 1)
+    chi = np.zeros((3, undf))
+    cell = 0
+    for j in range(3):
+        for i in range(3):
+            for kk in range(nl + 1):
+                chi[:, dm[cell, 0] - 1 + kk] = (i * dx, j * dy, kk * dz)
+            cell += 1
+    mm = orc.mass_matrix_w0(9, nl, dm, chi[0], chi[1], chi[2], nqp=3)
+    ans = np.array(k["answer_column_major"]).reshape((8, 8), order="F")
+    vol = dx * dy * dz
+    np.testing.assert_allclose(mm[0] / vol, ans / vol, rtol=0, atol=k["tol"])
+    np.testing.assert_allclose(mm[0], vol * _m_ref(), rtol=1e-14)
+
+
+def test_mass_matrix_w0_closed_forms(orc):
+    rng = np.random.default_rng(3)
+    # sheared parallelepiped: constant Jacobian [a b c], M = det * M_ref
+    a, b, c = rng.uniform(-1, 1, (3, 3)) + np.diag([3.0, 2.0, 4.0])
+    O = rng.uniform(-5, 5, 3)
+    X = [O + _VX[v] * a + _VY[v] * b + _VZ[v] * c for v in range(8)]
+    dm, chi = _one_cell(X)
+    det = np.linalg.det(np.column_stack([a, b, c]))
+    for nqp in (2, 3, 5):
+        mm = orc.mass_matrix_w0(1, 1, dm, chi[0], chi[1], chi[2], nqp=nqp)[0]
+        np.testing.assert_allclose(mm, det * _m_ref(), rtol=1e-13)
+    # a general (non-affine) hexahedron: symmetric, positive definite, entries add up to the
+    # volume, and 3 Gauss points per direction are already exact
+    Xg = [x + rng.uniform(-0.3, 0.3, 3) for x in X]
+    dm, chi = _one_cell(Xg)
+    m3 = orc.mass_matrix_w0(1, 1, dm, chi[0], chi[1], chi[2], nqp=3)[0]
+    m6 = orc.mass_matrix_w0(1, 1, dm, chi[0], chi[1], chi[2], nqp=6)[0]
+    np.testing.assert_array_equal(m3, m3.T)
+    assert np.all(np.linalg.eigvalsh(m3) > 0)
+    np.testing.assert_allclose(m3, m6, rtol=1e-12)
+    # the volume from a fine midpoint rule on the trilinear map's Jacobian (independent of the
+    # Gauss rule and of the basis ordering)
+    n = 40
+    t = (np.arange(n) + 0.5) / n
+    xs, ys, zs = np.meshgrid(t, t, t, indexing="ij")
+    P = np.array(Xg)
+    J = np.zeros(xs.shape + (3, 3))
+    for v in range(8):
+        lx = np.where(_VX[v], xs, 1 - xs); ly = np.where(_VY[v], ys, 1 - ys); lz = np.where(_VZ[v], zs, 1 - zs)
+        sx, sy, sz = (1 if _VX[v] else -1), (1 if _VY[v] else -1), (1 if _VZ[v] else -1)
+        for r in range(3):
+            J[..., r, 0] += P[v, r] * sx * ly * lz
+            J[..., r, 1] += P[v, r] * lx * sy * lz
+            J[..., r, 2] += P[v, r] * lx * ly * sz
+    vol = np.linalg.det(J).mean()
+    assert abs(m3.sum() - vol) <= 1e-3 * abs(vol)
