@@ -148,6 +148,13 @@ int lfric_op_convert(const lfric_op *src, lfric_op *dst);
  * operator from a space to itself; Ainv on the same spaces and precision, not A itself.  The
  * factorization does not pivot, as the reference's does not. */
 int lfric_op_invert_local(const lfric_op *A, lfric_op *Ainv);
+/* compute_mass_matrix_w_scalar_code (components/science/source/kernel/fem/
+ * sci_compute_mass_matrix_kernel_w_scalar_mod.F90:290-384) for a square lowest-order W0 FP64
+ * operator on the device: every cell's mass matrix from the Cartesian coordinate fields chi1..3
+ * (FP64 W0 fields on the operator's space, coord_system_xyz, trilinear coordinates), with
+ * nqp (1..4) Gauss points per direction (3 integrates a trilinear cell exactly). */
+int lfric_op_assemble_mass_w0(lfric_op *op, const lfric_vec *chi1, const lfric_vec *chi2, const lfric_vec *chi3,
+                              int nqp);
 void lfric_op_destroy(lfric_op *op);
 
 /* invoke(setval_c(y,0), matrix_vector_kernel_type(y, x, A)): y = A x over the coloured

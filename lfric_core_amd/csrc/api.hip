@@ -667,6 +667,18 @@ int lfric_op_convert(const lfric_op *src, lfric_op *dst) {
   return refresh_rows(dst, launch_op_convert(src, dst));
 }
 
+int lfric_op_assemble_mass_w0(lfric_op *op, const lfric_vec *chi1, const lfric_vec *chi2, const lfric_vec *chi3,
+                              int nqp) {
+  if (int rc = require_init()) return rc;
+  LF_CHECK(op && chi1 && chi2 && chi3, LFRIC_E_ARG, "null argument");
+  for (const lfric_vec *c : {chi1, chi2, chi3})
+    LF_CHECK(c->precision == LFRIC_R64 && c->nrhs == 1 && c->fs == op->to, LFRIC_E_ARG,
+             "assemble_mass_w0: chi must be FP64 fields on the operator's space");
+  return refresh_rows(op, launch_assemble_mass_w0(op, static_cast<const double *>(chi1->d),
+                                                  static_cast<const double *>(chi2->d),
+                                                  static_cast<const double *>(chi3->d), nqp));
+}
+
 int lfric_op_invert_local(const lfric_op *A, lfric_op *Ainv) {
   if (int rc = require_init()) return rc;
   LF_CHECK(A && Ainv && A != Ainv, LFRIC_E_ARG, "null or aliased operators");

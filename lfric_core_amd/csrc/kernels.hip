@@ -205,6 +205,81 @@ __global__ __launch_bounds__(kBlock) void k_invert_local_ws(const T *__restrict_
   invert_columns<T>(s, n, [&](int i, int m, T v) { B[aidx<T>(gs, el + ((row_top >> i) & 1u), i * n + m)] = v; });
 }
 
+// Lowest-order W0 mass matrix of every cell from Cartesian vertex coordinates:
+// compute_mass_matrix_w_scalar_code (components/science/source/kernel/fem/
+// sci_compute_mass_matrix_kernel_w_scalar_mod.F90:290-384) with coordinate_jacobian for
+// coord_system_xyz (components/science/source/kernel/geometry/sci_coordinate_jacobian_mod.F90:
+// 289-445).  One thread per 3D cell: the Jacobian jac(i,dir) = sum_df chi_i(df) d_dir N_df and
+// its cofactor determinant at each quadrature point, then the entries df >= df2 as sums over
+// (vertical point outer, horizontal point inner) of wh*wv*N_df*N_df2*dj, mirrored.  The basis
+// values and weights are tables built on the host ([8][nh*nv] values, [3][8][nh*nv] derivatives,
+// [nh] and [nv] weights) and staged in LDS; no contraction, so the arithmetic is the reference's.
+template <int NQ>
+__global__ __launch_bounds__(kBlock) void k_assemble_mass_w0(double *__restrict__ A, size_t gs,
+                                                             const int *__restrict__ ebase,
+                                                             const int *__restrict__ map,
+                                                             const double *__restrict__ chi1,
+                                                             const double *__restrict__ chi2,
+                                                             const double *__restrict__ chi3,
+                                                             const double *__restrict__ tables, int nl,
+                                                             long long nelem, uint32_t row_top) {
+  constexpr int NH = NQ * NQ, NP = NH * NQ;
+  constexpr int TB = 8 * NP + 24 * NP + NH + NQ;  // values, derivatives, weights
+  __shared__ double tab[TB];
+  for (int i = threadIdx.x; i < TB; i += kBlock) tab[i] = tables[i];
+  __syncthreads();
+  const double *b = tab, *db = tab + 8 * NP, *wh = tab + 32 * NP, *wv = wh + NH;
+  const long long e = static_cast<long long>(blockIdx.x) * kBlock + threadIdx.x;
+  if (e >= nelem) return;
+  const long long pos = e / nl;
+  const int k = static_cast<int>(e - pos * nl);
+  const long long el = ebase[pos] + k;
+  double c[3][8];
+#pragma unroll
+  for (int df = 0; df < 8; ++df) {
+    const size_t d = static_cast<size_t>(map[pos * 8 + df]) + k;
+    c[0][df] = chi1[d];
+    c[1][df] = chi2[d];
+    c[2][df] = chi3[d];
+  }
+  // points in the reference's order (vertical outer, horizontal inner); every entry's sum runs
+  // over them in that order, so interleaving the 36 sums leaves each one the reference's
+  double s[36];
+#pragma unroll
+  for (int t = 0; t < 36; ++t) s[t] = 0.0;
+  for (int kz = 0; kz < NQ; ++kz)
+    for (int ic = 0; ic < NH; ++ic) {
+      const int q = kz * NH + ic;
+      double J[3][3] = {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}};
+#pragma unroll
+      for (int df = 0; df < 8; ++df)
+#pragma unroll
+        for (int dir = 0; dir < 3; ++dir)
+#pragma unroll
+          for (int r = 0; r < 3; ++r) J[r][dir] = J[r][dir] + c[r][df] * db[(dir * 8 + df) * NP + q];
+      const double dj = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) -
+                        J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
+                        J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
+      const double w = wh[ic] * wv[kz];
+      double bq[8];
+#pragma unroll
+      for (int df = 0; df < 8; ++df) bq[df] = b[df * NP + q];
+      int t = 0;
+#pragma unroll
+      for (int df2 = 0; df2 < 8; ++df2)
+#pragma unroll
+        for (int df = df2; df < 8; ++df, ++t) s[t] = s[t] + w * bq[df] * bq[df2] * dj;
+    }
+  int t = 0;
+#pragma unroll
+  for (int df2 = 0; df2 < 8; ++df2)
+#pragma unroll
+    for (int df = df2; df < 8; ++df, ++t) {
+      A[aidx<double>(gs, el + ((row_top >> df) & 1u), df * 8 + df2)] = s[t];
+      A[aidx<double>(gs, el + ((row_top >> df2) & 1u), df2 * 8 + df)] = s[t];
+    }
+}
+
 #pragma clang fp contract(on)
 
 // reference layout matrix(ncell_3d, ndf1, ndf2) <-> device layout
@@ -1210,6 +1285,100 @@ static int invert_local_t(const lfric_op *A, lfric_op *Ainv) {
     }
   }
   CHECK_LAUNCH();
+  return LFRIC_OK;
+}
+
+// Gauss-Legendre points and weights on [0, 1]: the Newton iteration of quadrature_rule in
+// infrastructure/source/quadrature/quadrature_rule_gaussian_mod.F90:35-94 (each root's
+// iteration starts from the previous root; capped here at 100 steps), mapped from [-1, 1].
+static void gauss_rule(int n, double *pts, double *wts) {
+  const double pi = 4.0 * std::atan(1.0), eps = 2.220446049250313e-16;
+  double z1 = 0.0;
+  for (int i = 1; i <= (n + 1) / 2; ++i) {
+    double z = std::cos(pi * (i - 0.25) / (n + 0.5)), pp = 1.0;
+    for (int it = 0; it < 100 && std::fabs(z - z1) > eps; ++it) {
+      double p1 = 1.0, p2 = 0.0, p3;
+      for (int j = 1; j <= n; ++j) {
+        p3 = p2;
+        p2 = p1;
+        p1 = ((2.0 * j - 1.0) * z * p2 - (j - 1.0) * p3) / j;
+      }
+      pp = n * (z * p1 - p2) / (z * z - 1.0);
+      z1 = z;
+      z = z1 - p1 / pp;
+    }
+    pts[i - 1] = -z;
+    pts[n - i] = z;
+    wts[i - 1] = 2.0 / ((1.0 - z * z) * pp * pp);
+    wts[n - i] = wts[i - 1];
+  }
+  for (int i = 0; i < n; ++i) {
+    pts[i] = 0.5 * (pts[i] + 1.0);
+    wts[i] = 0.5 * wts[i];
+  }
+}
+
+// Quadrature tables of the W0 trilinear basis (reference-cube vertices SWB, SEB, NEB, NWB, SWT,
+// SET, NET, NWT, components of the reference element in reference_element_mod.F90:1244-1251):
+// values [8][NP], derivatives [3][8][NP], horizontal weights [nh] (x_i with y_j, y fastest, as
+// quadrature_xyoz_mod.F90:235-243), vertical weights [nq]; point q = kz*nh + i*nq + j.
+static std::vector<double> w0_tables(int nq) {
+  std::vector<double> p(nq), w(nq);
+  gauss_rule(nq, p.data(), w.data());
+  const int nh = nq * nq, np = nh * nq;
+  static const int vx[8] = {0, 1, 1, 0, 0, 1, 1, 0}, vy[8] = {0, 0, 1, 1, 0, 0, 1, 1}, vz[8] = {0, 0, 0, 0, 1, 1, 1, 1};
+  std::vector<double> t(static_cast<size_t>(32 * np + nh + nq));
+  for (int kz = 0; kz < nq; ++kz)
+    for (int i = 0; i < nq; ++i)
+      for (int j = 0; j < nq; ++j) {
+        const int q = kz * nh + i * nq + j;
+        const double x = p[i], y = p[j], z = p[kz];
+        for (int df = 0; df < 8; ++df) {
+          const double lx = vx[df] ? x : 1.0 - x, ly = vy[df] ? y : 1.0 - y, lz = vz[df] ? z : 1.0 - z;
+          const double dx = vx[df] ? 1.0 : -1.0, dy = vy[df] ? 1.0 : -1.0, dz = vz[df] ? 1.0 : -1.0;
+          t[static_cast<size_t>(df) * np + q] = lx * ly * lz;
+          t[static_cast<size_t>(8 * np) + static_cast<size_t>(0 * 8 + df) * np + q] = dx * ly * lz;
+          t[static_cast<size_t>(8 * np) + static_cast<size_t>(1 * 8 + df) * np + q] = lx * dy * lz;
+          t[static_cast<size_t>(8 * np) + static_cast<size_t>(2 * 8 + df) * np + q] = lx * ly * dz;
+        }
+      }
+  for (int i = 0; i < nq; ++i)
+    for (int j = 0; j < nq; ++j) t[static_cast<size_t>(32 * np) + i * nq + j] = w[i] * w[j];
+  for (int kz = 0; kz < nq; ++kz) t[static_cast<size_t>(32 * np + nh) + kz] = w[kz];
+  return t;
+}
+
+int launch_assemble_mass_w0(lfric_op *op, const double *chi1, const double *chi2, const double *chi3, int nqp) {
+  const FunctionSpace &f = *op->to;
+  bool w0 = op->to == op->from && f.ndf == 8 && f.top_mask == 0xF0u;
+  for (int i = 0; w0 && i < 4; ++i) w0 = f.partner[i] == i + 4;
+  LF_CHECK(w0, LFRIC_E_UNSUPPORTED, "assemble_mass_w0: a square lowest-order W0 operator is needed");
+  LF_CHECK(op->precision == LFRIC_R64, LFRIC_E_UNSUPPORTED, "assemble_mass_w0: FP64 operator (convert for FP32)");
+  LF_CHECK(nqp >= 1 && nqp <= 4, LFRIC_E_ARG, "assemble_mass_w0: 1 to 4 quadrature points per direction");
+  const std::vector<double> t = w0_tables(nqp);
+  void *dt = nullptr;
+  if (int rc = dmalloc(&dt, sizeof(double) * t.size())) return rc;
+  LF_HIP(hipMemcpyAsync(dt, t.data(), sizeof(double) * t.size(), hipMemcpyHostToDevice, ctx().stream));
+  const long long nelem = static_cast<long long>(f.ncells) * f.nlayers;
+  double *A = static_cast<double *>(op->d);
+  const double *tb = static_cast<const double *>(dt);
+#define ASM_CASE(N)                                                                                             \
+  case N:                                                                                                       \
+    LAUNCH(k_assemble_mass_w0<N>, grid_for(nelem), A, op->gs, f.d_ebase, f.d_map, chi1, chi2, chi3, tb, f.nlayers, \
+           nelem, op->row_top);                                                                                 \
+    break;
+  switch (nqp) {
+    ASM_CASE(1)
+    ASM_CASE(2)
+    ASM_CASE(3)
+    ASM_CASE(4)
+  }
+#undef ASM_CASE
+  const hipError_t e = hipGetLastError();
+  const hipError_t es = hipStreamSynchronize(ctx().stream);
+  dfree(dt, sizeof(double) * t.size());
+  if (e != hipSuccess) return fail(LFRIC_E_HIP, hipGetErrorString(e));
+  if (es != hipSuccess) return fail(LFRIC_E_HIP, hipGetErrorString(es));
   return LFRIC_OK;
 }
 

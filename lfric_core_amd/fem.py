@@ -198,6 +198,13 @@ class LmaOperator:
         check(lib().lfric_op_convert(self._h, out._h))
         return out
 
+    def assemble_mass_w0(self, chi1, chi2, chi3, nqp=3):
+        """Every cell's lowest-order W0 mass matrix from the Cartesian coordinate fields chi1..3
+        (compute_mass_matrix_w_scalar_code, components/science/source/kernel/fem/
+        sci_compute_mass_matrix_kernel_w_scalar_mod.F90:290-384); FP64 operator."""
+        check(lib().lfric_op_assemble_mass_w0(self._h, chi1._h, chi2._h, chi3._h, int(nqp)))
+        return self
+
     def invert_local(self):
         """Per-cell inverse of every element matrix (invert_local_operator_code,
         components/science/source/kernel/algebra/sci_invert_local_operator_kernel_mod.F90:51-77),

@@ -50,7 +50,7 @@ module lfric_hip_capi_mod
             lfric_vec_fill_uniform
   public :: lfric_op_create, lfric_op_upload, lfric_op_generate_spd, lfric_op_convert, &
             lfric_op_destroy, lfric_lma_apply, lfric_mm_diagonal, lfric_op_invert_local, &
-            lfric_op_prepare_rows, lfric_op_tune
+            lfric_op_prepare_rows, lfric_op_tune, lfric_op_assemble_mass_w0
   public :: lfric_prec_create_diagonal, lfric_prec_create_identity, lfric_prec_apply, &
             lfric_prec_destroy
   public :: lfric_linop_create_lma, lfric_linop_destroy, lfric_prec_create_column_jacobi
@@ -199,6 +199,13 @@ module lfric_hip_capi_mod
     integer(c_int) function lfric_op_invert_local(a, ainv) bind(c, name="lfric_op_invert_local")
       import :: c_int, c_ptr
       type(c_ptr), value :: a, ainv
+    end function
+    ! compute_mass_matrix_w_scalar_code for lowest-order W0 from Cartesian chi fields
+    integer(c_int) function lfric_op_assemble_mass_w0(op, chi1, chi2, chi3, nqp) &
+        bind(c, name="lfric_op_assemble_mass_w0")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: op, chi1, chi2, chi3
+      integer(c_int), value :: nqp
     end function
     ! row-gathered copy for the one-launch W0 apply (set-up, once per operator)
     integer(c_int) function lfric_op_prepare_rows(op) bind(c, name="lfric_op_prepare_rows")
