@@ -76,3 +76,30 @@ def test_fullsize_mixed_solve_reaches_fp64_tolerance(hip, c96):
     xm, x6 = x_mixed.export_field(), x_fp64.export_field()
     # both residuals <= 1e-8 ||b||; the solutions differ by at most cond(A) x 2e-8 relative
     assert np.linalg.norm(xm - x6) / np.linalg.norm(x6) <= 1e-5
+
+
+def test_fullsize_helmholtz_gmres_bf16_block_jacobi(hip):
+    """The W2/W3 Helmholtz-like system at its full size (C48 cubed sphere, 50 levels: W2
+    2,087,424 dofs, W3 691,200): GMRES(30) with FP32 applies and the BF16 vertical-column
+    block-Jacobi preconditioner inside FP64 refinement reaches an FP64 true relative residual
+    <= 1e-8, and agrees with the plain FP64 GMRES solution to the system's conditioning."""
+    w2 = hip.FunctionSpace.cubedsphere(48, 50, hip.W2, 0)
+    w3 = w2.sibling(hip.W3, 0)
+    assert (w2.undf, w3.undf) == (2087424, 691200)
+    from oracle.oracle import D_SIGNS
+    M2 = hip.LmaOperator(w2).generate_spd(0, shift=1.0)
+    D = hip.LmaOperator(w3, w2).generate_uniform(0, 5, 0.5, 1.5, D_SIGNS)
+    M3 = hip.LmaOperator(w3).generate_spd(1, shift=1.0)
+    H64 = hip.LinearOperator.helmholtz(M3, D, M2.mm_diagonal(hip.FieldVector(w2)), 0.5)
+    M2s, Ds, M3s = M2.convert(hip.R32), D.convert(hip.R32), M3.convert(hip.R32)
+    H32 = hip.LinearOperator.helmholtz(M3s, Ds, M2s.mm_diagonal(hip.FieldVector(w2, hip.R32)), 0.5)
+    P = hip.ColumnJacobiPreconditioner(H32, hip.BF16)
+    b = hip.FieldVector(w3).fill_uniform(9, 3)
+    x = hip.FieldVector(w3)
+    res = hip.MixedPrecisionGMRES(H64, H32, P, 30, 1e-8, 0.0, 50, inner_r_tol=1e-4).apply(x, b)
+    assert res.status == 0 and res.true_rel_res <= 1e-8, res
+    x64 = hip.FieldVector(w3)
+    res64 = hip.GMRES(H64, hip.NullPreconditioner(), 30, 1e-10, 0.0, 200).apply(x64, b)
+    assert res64.status == 0
+    xm, xf = x.export_field(), x64.export_field()
+    assert np.linalg.norm(xm - xf) <= 1e-6 * np.linalg.norm(xf)
