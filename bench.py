@@ -39,6 +39,12 @@ def apply_bytes(fs, ndf, s_a, s_x):
     return fs.ncell_3d * ndf * ndf * s_a + 2 * fs.undf * s_x + 4 * fs.ncells * ndf
 
 
+def sym_bytes(fs, s_a, s_x):
+    """Algorithmic bytes of one symmetric-packed apply: the upper triangle of each element
+    (36 of 64 entries) once, x once, y once, dofmap once."""
+    return fs.ncell_3d * 36 * s_a + 2 * fs.undf * s_x + 4 * fs.ncells * 8
+
+
 def solve_roofline(fs, mres, pres, t_mixed, t_fp64):
     """Whole-solve HBM rate: sum over iterations of the fused PCG iteration's algorithmic bytes
     (apply + 13 vector passes of undf entries) plus, for the mixed solve, each FP64 refinement
@@ -190,6 +196,18 @@ def main():
                 op32.apply_mrhs(xm, ym, variant)
             ms = L.timer_stop() / args.apply_reps
             out["multi_rhs4_fp32"][name] = {"ms": ms, "gbs": apply_bytes(fs, 8, 4, 16) / (ms * 1e-3) / 1e9}
+        # symmetric-packed copies (upper triangles; two-phase apply), on operators of their own
+        s32, s64 = op64.convert(L.R32), L.LmaOperator(fs).generate_spd(0, scaled=True)
+        out["symmetric_packed"] = {}
+        for seg in (2, 4, 8):
+            s32.prepare_symmetric(seg)
+            s64.prepare_symmetric(seg)
+            t32 = time_applies(L, s32, xs32, ys32, args.apply_reps, "apply_gather")
+            t64 = time_applies(L, s64, xs64, ys64, args.apply_reps, "apply_gather")
+            out["symmetric_packed"][seg] = {"fp32_ms": t32, "fp64_ms": t64,
+                                            "fp32_tri_gbs": sym_bytes(fs, 4, 4) / (t32 * 1e-3) / 1e9,
+                                            "fp64_tri_gbs": sym_bytes(fs, 8, 8) / (t64 * 1e-3) / 1e9}
+        del s32, s64
         for _ in range(args.apply_reps):  # plain copies of a known size (undf * 8 bytes each way)
             ys64.copy(xs64)
         L.synchronize()

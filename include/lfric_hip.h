@@ -138,8 +138,17 @@ int lfric_op_set_window(lfric_op *op, int window);
 /* Add a row-gathered copy of a square lowest-order W0 operator: the rows that feed each
  * output dof stored contiguously, so the gather apply streams the matrix in order.  Costs a
  * second copy of the operator's memory; the gather apply uses it from then on (same sums in
- * the same order).  The copy is taken now: call again after changing the operator. */
+ * the same order).  Writing the operator (upload, generate, convert, invert, assemble)
+ * rebuilds the copy. */
 int lfric_op_prepare_rows(lfric_op *op);
+/* Add a symmetric-packed copy of a square lowest-order W0 operator whose every element matrix
+ * is symmetric bit for bit (refused otherwise): the upper triangle of each element (36 of 64
+ * entries), grouped per cell column in segments of levels_per_thread levels (0: 4, the
+ * fastest measured).  From then on the solvers' apply reads it in two phases (per-cell outputs,
+ * then a colour-ordered gather) whose result equals the colour-ordered scatter (lfric_lma_apply)
+ * bit for bit, reading ~70 % of the matrix bytes.  Writing the operator rebuilds the copy, or drops it if the new entries are
+ * not symmetric.  No reference counterpart (the reference stores full element matrices). */
+int lfric_op_prepare_symmetric(lfric_op *op, int levels_per_thread);
 /* Round an operator into another precision (e.g. the r_solver FP32 copy of an FP64 master). */
 int lfric_op_convert(const lfric_op *src, lfric_op *dst);
 /* invert_local_operator_code (components/science/source/kernel/algebra/

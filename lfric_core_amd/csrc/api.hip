@@ -612,13 +612,29 @@ void lfric_op_destroy(lfric_op *op) {
   if (!op) return;
   dfree(op->d, op->bytes);
   dfree(op->rows, op->rows_bytes);
+  dfree(op->sym, op->sym_bytes);
+  dfree(op->symz, op->symz_bytes);
   delete op;
 }
 
-// an operator whose entries change keeps its row-gathered copy (if it has one) in step
+// an operator whose entries change keeps its row-gathered and symmetric-packed copies (those it
+// has) in step; new entries that are not symmetric drop the symmetric copy
 static int refresh_rows(lfric_op *op, int rc) {
-  if (rc != LFRIC_OK || !op->rows) return rc;
-  return launch_build_rows(op);
+  if (rc != LFRIC_OK) return rc;
+  if (op->rows)
+    if (int r = launch_build_rows(op)) return r;
+  if (op->sym) {
+    const int r = launch_build_sym(op, op->sym_seg);
+    if (r == LFRIC_E_UNSUPPORTED) {
+      dfree(op->sym, op->sym_bytes);
+      dfree(op->symz, op->symz_bytes);
+      op->sym = op->symz = nullptr;
+      op->sym_bytes = op->symz_bytes = 0;
+    } else if (r) {
+      return r;
+    }
+  }
+  return LFRIC_OK;
 }
 
 int lfric_op_upload(lfric_op *op, const double *ls) {
@@ -758,6 +774,14 @@ int lfric_op_prepare_rows(lfric_op *op) {
   if (int rc = require_init()) return rc;
   LF_CHECK(op, LFRIC_E_ARG, "null argument");
   if (int rc = launch_build_rows(op)) return rc;
+  LF_HIP(hipStreamSynchronize(ctx().stream));
+  return LFRIC_OK;
+}
+
+int lfric_op_prepare_symmetric(lfric_op *op, int levels_per_thread) {
+  if (int rc = require_init()) return rc;
+  LF_CHECK(op, LFRIC_E_ARG, "null argument");
+  if (int rc = launch_build_sym(op, levels_per_thread > 0 ? levels_per_thread : 4)) return rc;
   LF_HIP(hipStreamSynchronize(ctx().stream));
   return LFRIC_OK;
 }

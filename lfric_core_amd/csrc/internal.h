@@ -121,6 +121,11 @@ struct lfric_op {
   // that feed each output dof, stored contiguously per thread of the gather kernel
   void *rows = nullptr;
   size_t rows_bytes = 0, gsr = 0;
+  // symmetric-packed copy (lfric_op_prepare_symmetric): upper triangles per (cell, level
+  // segment); per-unit outputs in symz between the two phases of its apply
+  void *sym = nullptr, *symz = nullptr;
+  size_t sym_bytes = 0, symz_bytes = 0, gss = 0;
+  int sym_seg = 0;
   void *d = nullptr;           // level-aligned AoSoA layout (see kernels.hip)
   size_t bytes = 0;
 };

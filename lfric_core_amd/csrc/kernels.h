@@ -44,6 +44,7 @@ int launch_build_rows(lfric_op *op);
 int launch_lma_apply_solver(const lfric_op *op, const void *x, void *y, const int *skip);
 int launch_lma_apply_mrhs(const lfric_op *op, const void *x, void *y, int nrhs, int variant);
 int launch_op_invert_local(const lfric_op *A, lfric_op *Ainv);
+int launch_build_sym(lfric_op *op, int levels_per_thread);
 int launch_assemble_mass_w0(lfric_op *op, const double *chi1, const double *chi2, const double *chi3, int nqp);
 // whether the MFMA multi-RHS apply (LFRIC_MRHS_MFMA) handles this operator
 bool mfma_apply_supported(const lfric_op *op);

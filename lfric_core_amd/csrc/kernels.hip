@@ -655,6 +655,214 @@ __global__ __launch_bounds__(kBlock) void k_lma_gather_rows_w0(const T *__restri
   }
 }
 
+// ---- symmetric-packed apply of lowest-order W0 operators ------------------------------------
+// For operators whose every element matrix is symmetric (bit for bit), each element's upper
+// triangle T(i, j), i <= j (36 entries, row-major) is enough.  Thread t = (cell position,
+// segment of S levels) owns units m = seg*S .. seg*S+S-1 (m <= nlayers) of its cell: unit m needs
+// the bottom rows of the layer-m element and the top rows of the layer-(m-1) element, so the
+// thread walks up its segment keeping the previous triangle in registers.  Its copy holds S+1
+// triangles: slot s is layer seg*S - 1 + s (slot 0 repeats the previous segment's last one).
+// Entry k of slot s of thread t sits at (t/64)*gss + ((s*NCH + k/V)*64 + t%64)*V + k%V, NCH =
+// 36/V chunks, so each wavefront load is one contiguous 1 KiB block.
+__host__ __device__ constexpr int tri_index(int i, int j) { return i * 8 - i * (i - 1) / 2 + (j - i); }
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_build_sym(const T *__restrict__ A, size_t gs, const int *__restrict__ ebase,
+                                                      T *__restrict__ sym, size_t gss, int ncells, int nl, int S,
+                                                      int nseg, uint32_t row_top, int *__restrict__ asym) {
+  constexpr int V = chunk_entries<T>(), NCH = 36 / V;
+  const long long t = static_cast<long long>(blockIdx.x) * kBlock + threadIdx.x;
+  if (t >= static_cast<long long>(ncells) * nseg) return;
+  const int pos = static_cast<int>(t / nseg), seg = static_cast<int>(t - static_cast<long long>(pos) * nseg);
+  T *St = sym + static_cast<size_t>(t >> 6) * gss + static_cast<size_t>(t & 63) * V;
+  bool mismatch = false;
+  for (int s = 0; s <= S; ++s) {
+    const int L = seg * S - 1 + s;
+    const bool live = L >= 0 && L < nl;
+    const long long e = static_cast<long long>(ebase[pos]) + (live ? L : 0);
+    for (int i = 0; i < 8; ++i)
+      for (int j = i; j < 8; ++j) {
+        T v = T(0);
+        if (live) {
+          v = A[aidx<T>(gs, e + ((row_top >> i) & 1u), i * 8 + j)];
+          const T w = A[aidx<T>(gs, e + ((row_top >> j) & 1u), j * 8 + i)];
+          if constexpr (sizeof(T) == 8)
+            mismatch = mismatch || (__double_as_longlong(v) != __double_as_longlong(w));
+          else
+            mismatch = mismatch || (__float_as_uint(v) != __float_as_uint(w));
+        }
+        const int k = tri_index(i, j);
+        St[(static_cast<size_t>(s) * NCH + k / V) * 64 * V + k % V] = v;
+      }
+  }
+  if (mismatch) *asym = 1;  // any thread; every writer stores the same value
+}
+
+// phase 1: the 4 outputs of every unit, z[(pos*4 + i)*(nl+1) + m] = c_i + cp_i with c_i the fma
+// chain of bottom row i (layer m) and cp_i that of top row i+4 (layer m-1), exactly as
+// k_lma_colour forms them
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_sym_units(const T *__restrict__ sym, size_t gss, const T *__restrict__ x,
+                                                      const int *__restrict__ map, T *__restrict__ z, int ncells,
+                                                      int nl, int S, int nseg, const int *skip) {
+  if (skip && *skip) return;
+  constexpr int V = chunk_entries<T>(), NCH = 36 / V;
+  typedef T VecT __attribute__((ext_vector_type(V)));
+  const long long t = static_cast<long long>(blockIdx.x) * kBlock + threadIdx.x;
+  if (t >= static_cast<long long>(ncells) * nseg) return;
+  const int pos = static_cast<int>(t / nseg), seg = static_cast<int>(t - static_cast<long long>(pos) * nseg);
+  const int H = nl + 1, m0 = seg * S;
+  const T *St = sym + static_cast<size_t>(t >> 6) * gss + static_cast<size_t>(t & 63) * V;
+  auto load_tri = [&](int s, T (&tri)[36]) {
+#pragma unroll
+    for (int c = 0; c < NCH; ++c) {
+      const VecT v = __builtin_nontemporal_load(reinterpret_cast<const VecT *>(St + (static_cast<size_t>(s) * NCH + c) * 64 * V));
+#pragma unroll
+      for (int q = 0; q < V; ++q) tri[c * V + q] = v[q];
+    }
+  };
+  int b[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) b[j] = map[static_cast<size_t>(pos) * 8 + j];
+  auto xat = [&](int j, int lev) { return (lev >= 0 && lev <= nl) ? x[b[j] + lev] : T(0); };
+  // triangles of layers m-1 and m, and the next one loaded a step ahead; x at levels m-1, m,
+  // m+1 and the next level, also a step ahead (the loads of step u+1 are in flight while step u
+  // computes)
+  T prev[36], cur[36], nxt[36], xw[3][4], xn[4];
+  load_tri(0, prev);
+  if (m0 < nl) load_tri(1, nxt);
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    xw[0][j] = xat(j, m0 - 1);
+    xw[1][j] = xat(j, m0);
+    xw[2][j] = xat(j, m0 + 1);
+    xn[j] = xat(j, m0 + 2);
+  }
+  T *zp = z + static_cast<size_t>(pos) * 4 * H;
+  for (int u = 0; u < S; ++u) {
+    const int m = m0 + u;
+    if (m > nl) break;
+    const bool has_m = m < nl, has_p = m > 0;
+#pragma unroll
+    for (int k = 0; k < 36; ++k) cur[k] = nxt[k];
+    if (u + 1 < S && m + 1 < nl) load_tri(u + 2, nxt);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      T c = T(0), cp = T(0);
+      if (has_m) {
+#pragma unroll
+        for (int j = 0; j < 8; ++j)
+          c = fma(cur[j >= i ? tri_index(i, j) : tri_index(j, i)], j < 4 ? xw[1][j] : xw[2][j - 4], c);
+      }
+      if (has_p) {
+#pragma unroll
+        for (int j = 0; j < 8; ++j)
+          cp = fma(prev[j >= i + 4 ? tri_index(i + 4, j) : tri_index(j, i + 4)], j < 4 ? xw[0][j] : xw[1][j - 4], cp);
+      }
+      zp[static_cast<size_t>(i) * H + m] = c + cp;
+    }
+#pragma unroll
+    for (int k = 0; k < 36; ++k) prev[k] = cur[k];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      xw[0][j] = xw[1][j];
+      xw[1][j] = xw[2][j];
+      xw[2][j] = xn[j];
+      xn[j] = xat(j, m + 3);
+    }
+  }
+}
+
+// phase 1 with a compile-time segment: every triangle of the segment is loaded before any
+// arithmetic, so all of a thread's matrix loads are in flight together (same sums as above)
+template <typename T, int S>
+__global__ __launch_bounds__(kBlock) void k_sym_units_s(const T *__restrict__ sym, size_t gss, const T *__restrict__ x,
+                                                        const int *__restrict__ map, T *__restrict__ z, int ncells,
+                                                        int nl, int nseg, const int *skip) {
+  if (skip && *skip) return;
+  constexpr int V = chunk_entries<T>(), NCH = 36 / V;
+  typedef T VecT __attribute__((ext_vector_type(V)));
+  const long long t = static_cast<long long>(blockIdx.x) * kBlock + threadIdx.x;
+  if (t >= static_cast<long long>(ncells) * nseg) return;
+  const int pos = static_cast<int>(t / nseg), seg = static_cast<int>(t - static_cast<long long>(pos) * nseg);
+  const int H = nl + 1, m0 = seg * S;
+  const T *St = sym + static_cast<size_t>(t >> 6) * gss + static_cast<size_t>(t & 63) * V;
+  VecT tri[S + 1][NCH];
+#pragma unroll
+  for (int s = 0; s <= S; ++s)
+#pragma unroll
+    for (int c = 0; c < NCH; ++c)
+      tri[s][c] = __builtin_nontemporal_load(reinterpret_cast<const VecT *>(St + (static_cast<size_t>(s) * NCH + c) * 64 * V));
+  int b[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) b[j] = map[static_cast<size_t>(pos) * 8 + j];
+  T xs[S + 2][4];  // x at levels m0-1 .. m0+S
+#pragma unroll
+  for (int l = 0; l < S + 2; ++l)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int lev = m0 - 1 + l;
+      xs[l][j] = (lev >= 0 && lev <= nl) ? x[b[j] + lev] : T(0);
+    }
+  auto at = [&](int s, int k) { return tri[s][k / V][k % V]; };
+  T *zp = z + static_cast<size_t>(pos) * 4 * H;
+#pragma unroll
+  for (int u = 0; u < S; ++u) {
+    const int m = m0 + u;
+    if (m > nl) break;
+    const bool has_m = m < nl, has_p = m > 0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      T c = T(0), cp = T(0);
+      if (has_m) {
+#pragma unroll
+        for (int j = 0; j < 8; ++j)
+          c = fma(at(u + 1, j >= i ? tri_index(i, j) : tri_index(j, i)), j < 4 ? xs[u + 1][j] : xs[u + 2][j - 4], c);
+      }
+      if (has_p) {
+#pragma unroll
+        for (int j = 0; j < 8; ++j)
+          cp = fma(at(u, j >= i + 4 ? tri_index(i + 4, j) : tri_index(j, i + 4)), j < 4 ? xs[u][j] : xs[u + 1][j - 4], cp);
+      }
+      zp[static_cast<size_t>(i) * H + m] = c + cp;
+    }
+  }
+}
+
+// phase 2: y at (column, level) = the units' outputs of the column's cells in colour order, the
+// first written, the rest added (the colour kernel's first-touch write and += order)
+template <typename T, bool DOT>
+__global__ __launch_bounds__(kBlock) void k_sym_gather(const T *__restrict__ z, const T *__restrict__ x,
+                                                       T *__restrict__ y, const int2 *__restrict__ cols,
+                                                       const int *__restrict__ off, const int *__restrict__ ent,
+                                                       int ncols, int nl, double *partials, const int *skip, int xwin) {
+  if (skip && *skip) return;
+  const int H = nl + 1;
+  const long long t = static_cast<long long>(xcd_block(blockIdx.x, gridDim.x, xwin)) * kBlock + threadIdx.x;
+  bool valid = t < static_cast<long long>(ncols) * H;
+  const int col = valid ? static_cast<int>(t / H) : 0;
+  const int m = valid ? static_cast<int>(t - static_cast<long long>(col) * H) : 0;
+  int2 ci = make_int2(0, 0);
+  if (valid) {
+    ci = cols[col];
+    if (m == nl && !ci.y) valid = false;
+  }
+  T acc = T(0);
+  if (valid) {
+    const int q0 = off[col], q1 = off[col + 1];
+    for (int q = q0; q < q1; ++q) {
+      const int en = ent[q];
+      const T v = z[(static_cast<size_t>(en >> 5) * 4 + (en & 31)) * H + m];
+      acc = q == q0 ? v : acc + v;
+    }
+    y[static_cast<size_t>(ci.x) + m] = acc;
+  }
+  if constexpr (DOT) {
+    double v[1] = {valid ? to_d(acc) * to_d(x[static_cast<size_t>(ci.x) + m]) : 0.0};
+    block_partials<1>(v, partials);
+  }
+}
+
 // Several right-hand sides over the row-gathered copy: each RHS accumulates exactly as
 // k_lma_gather_rows_w0 (and so as k_lma_gather_mrhs), x and y hold NR values per dof, RHS
 // fastest.  The matrix streams once for all right-hand sides.
@@ -1576,6 +1784,27 @@ static void launch_gather_r(const lfric_op *op, const T *x, T *y, R roles, Gathe
   const FunctionSpace &to = *op->to, &from = *op->from;
   const int grid = gather_blocks(op);
   if constexpr (N2 == 8 && std::is_same<R, RolesW0K0>::value) {
+    if (op->sym) {  // symmetric-packed copy (launch_build_sym): two phases
+      const int nseg = (to.nlayers + 1 + op->sym_seg - 1) / op->sym_seg;
+      if (op->sym_seg == 4)
+        LAUNCH((k_sym_units_s<T, 4>), grid_for(static_cast<long long>(to.ncells) * nseg),
+               static_cast<const T *>(op->sym), op->gss, x, from.d_map, static_cast<T *>(op->symz), to.ncells,
+               to.nlayers, nseg, ex.skip);
+      else if (op->sym_seg == 2)
+        LAUNCH((k_sym_units_s<T, 2>), grid_for(static_cast<long long>(to.ncells) * nseg),
+               static_cast<const T *>(op->sym), op->gss, x, from.d_map, static_cast<T *>(op->symz), to.ncells,
+               to.nlayers, nseg, ex.skip);
+      else
+        LAUNCH((k_sym_units<T>), grid_for(static_cast<long long>(to.ncells) * nseg), static_cast<const T *>(op->sym),
+               op->gss, x, from.d_map, static_cast<T *>(op->symz), to.ncells, to.nlayers, op->sym_seg, nseg, ex.skip);
+      if (ex.partials)
+        LAUNCH((k_sym_gather<T, true>), grid, static_cast<const T *>(op->symz), x, y, to.d_gcol, to.d_goff, to.d_gent,
+               to.gather_ncols, to.nlayers, ex.partials, ex.skip, op->xwin);
+      else
+        LAUNCH((k_sym_gather<T, false>), grid, static_cast<const T *>(op->symz), x, y, to.d_gcol, to.d_goff,
+               to.d_gent, to.gather_ncols, to.nlayers, ex.partials, ex.skip, op->xwin);
+      return;
+    }
     if (op->rows) {  // built only for square lowest-order W0 operators (launch_build_rows)
       const T *rw = static_cast<const T *>(op->rows);
       if (ex.partials)
@@ -1810,6 +2039,69 @@ int launch_build_rows(lfric_op *op) {
       LF_CHECK(to.dofmap[static_cast<size_t>(c) * 8 + j + 4] == to.dofmap[static_cast<size_t>(c) * 8 + j] + 1,
                LFRIC_E_UNSUPPORTED, "the row-gathered copy needs each top vertex one level above its bottom vertex");
   return op->precision == LFRIC_R64 ? build_rows_r<double, 8>(op, RolesW0K0{}) : build_rows_r<float, 8>(op, RolesW0K0{});
+}
+
+template <typename T>
+static int build_sym_t(lfric_op *op, int S) {
+  const FunctionSpace &f = *op->to;
+  constexpr int V = chunk_entries<T>(), NCH = 36 / V;
+  const int nseg = (f.nlayers + 1 + S - 1) / S;
+  const long long threads = static_cast<long long>(f.ncells) * nseg;
+  const size_t gss = static_cast<size_t>(64) * (S + 1) * NCH * V;
+  const size_t bytes = sizeof(T) * gss * static_cast<size_t>((threads + 63) / 64);
+  const size_t zbytes = sizeof(T) * static_cast<size_t>(f.ncells) * 4 * (f.nlayers + 1);
+  void *sym = nullptr, *z = nullptr;
+  int *flag = nullptr;
+  if (int rc = dmalloc(&sym, bytes)) return rc;
+  if (int rc = dmalloc(&z, zbytes)) {
+    dfree(sym, bytes);
+    return rc;
+  }
+  if (int rc = dmalloc(reinterpret_cast<void **>(&flag), sizeof(int))) {
+    dfree(sym, bytes);
+    dfree(z, zbytes);
+    return rc;
+  }
+  int h_flag = 0;
+  hipError_t e = hipMemcpyAsync(flag, &h_flag, sizeof(int), hipMemcpyHostToDevice, ctx().stream);
+  if (e == hipSuccess) {
+    LAUNCH((k_build_sym<T>), grid_for(threads), static_cast<const T *>(op->d), op->gs, f.d_ebase,
+           static_cast<T *>(sym), gss, f.ncells, f.nlayers, S, nseg, op->row_top, flag);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipMemcpyAsync(&h_flag, flag, sizeof(int), hipMemcpyDeviceToHost, ctx().stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx().stream);
+  dfree(flag, sizeof(int));
+  if (e != hipSuccess || h_flag) {
+    dfree(sym, bytes);
+    dfree(z, zbytes);
+    if (e != hipSuccess) return fail(LFRIC_E_HIP, hipGetErrorString(e));
+    return fail(LFRIC_E_UNSUPPORTED, "the operator's element matrices are not symmetric");
+  }
+  dfree(op->sym, op->sym_bytes);
+  dfree(op->symz, op->symz_bytes);
+  op->sym = sym;
+  op->sym_bytes = bytes;
+  op->symz = z;
+  op->symz_bytes = zbytes;
+  op->gss = gss;
+  op->sym_seg = S;
+  return LFRIC_OK;
+}
+
+int launch_build_sym(lfric_op *op, int levels_per_thread) {
+  FunctionSpace &f = *op->to;
+  LF_CHECK(op->to == op->from, LFRIC_E_ARG, "the symmetric-packed copy is for square operators");
+  LF_CHECK(role_kind(f) == RoleKind::W0K0 && f.ndf == 8, LFRIC_E_UNSUPPORTED,
+           "the symmetric-packed copy is built for lowest-order W0 operators");
+  LF_CHECK(levels_per_thread >= 1 && levels_per_thread <= 64, LFRIC_E_ARG, "levels per thread must be 1..64");
+  for (int c = 0; c < f.ncells; ++c)
+    for (int j = 0; j < 4; ++j)
+      LF_CHECK(f.dofmap[static_cast<size_t>(c) * 8 + j + 4] == f.dofmap[static_cast<size_t>(c) * 8 + j] + 1,
+               LFRIC_E_UNSUPPORTED, "the symmetric-packed copy needs each top vertex one level above its bottom vertex");
+  if (int rc = build_gather(f)) return rc;
+  return op->precision == LFRIC_R64 ? build_sym_t<double>(op, levels_per_thread)
+                                    : build_sym_t<float>(op, levels_per_thread);
 }
 
 int launch_lma_apply_gather_dot(const lfric_op *op, const void *x, void *y, double *partials, const int *skip) {

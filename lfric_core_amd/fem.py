@@ -228,6 +228,13 @@ class LmaOperator:
         check(lib().lfric_op_prepare_rows(self._h))
         return self
 
+    def prepare_symmetric(self, levels_per_thread=0):
+        """Symmetric-packed copy (square lowest-order W0 operators with symmetric element
+        matrices): the solvers' apply then reads ~70 % of the matrix bytes, same result as the
+        colour-ordered apply bit for bit; levels_per_thread 0 picks 4."""
+        check(lib().lfric_op_prepare_symmetric(self._h, int(levels_per_thread)))
+        return self
+
     def set_window(self, window):
         check(lib().lfric_op_set_window(self._h, int(window)))
         return self

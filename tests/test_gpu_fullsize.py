@@ -48,6 +48,13 @@ def test_fullsize_apply_forms_agree_bitwise(hip, c96):
         x1, y1 = hip.FieldVector(fs, hip.R32).import_raw(X[:, r]), hip.FieldVector(fs, hip.R32)
         op32.apply(x1, y1)
         np.testing.assert_array_equal(Yc[:, r], y1.export_raw())
+    # the symmetric-packed apply of a copy of the FP32 operator equals its colour-ordered apply
+    ops = op64.convert(hip.R32).prepare_symmetric()
+    x = hip.FieldVector(fs, hip.R32).fill_uniform(7)
+    yc, ys = hip.FieldVector(fs, hip.R32), hip.FieldVector(fs, hip.R32)
+    ops.apply(x, yc)
+    ops.apply_gather(x, ys)
+    np.testing.assert_array_equal(ys.export_raw(), yc.export_raw())
 
 
 def test_fullsize_operator_is_symmetric(hip, c96):

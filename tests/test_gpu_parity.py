@@ -359,3 +359,43 @@ def test_row_copy_follows_operator_changes(hip, orc):
     op.apply_gather(x, y0)
     fresh.apply_gather(x, y1)
     np.testing.assert_array_equal(y0.export_raw(), y1.export_raw())
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+@pytest.mark.parametrize("n,nl,seg", [(12, 9, 8), (5, 4, 3), (6, 7, 1), (4, 3, 64)])
+def test_symmetric_packed_apply_equals_colour_apply(hip, orc, n, nl, seg, prec):
+    """The symmetric-packed apply (upper triangles, per-cell outputs, colour-ordered gather) forms
+    the colour-ordered scatter's sums in its order: equal bit for bit, for any segment length;
+    CG over it takes the oracle's iteration count.  Non-symmetric operators are refused, and an
+    operator rewritten with non-symmetric entries drops the copy."""
+    fs, m = _mesh_pair(hip, orc, n, nl)
+    op64 = hip.LmaOperator(fs).generate_spd(6, scaled=True)
+    op = op64 if prec == 0 else op64.convert(hip.R32)
+    P = hip.R64 if prec == 0 else hip.R32
+    x = hip.FieldVector(fs, P).fill_uniform(4, 3)
+    yc, ys = hip.FieldVector(fs, P), hip.FieldVector(fs, P)
+    op.apply(x, yc)
+    op.prepare_symmetric(seg)
+    ys.set_scalar(9.0)
+    op.apply_gather(x, ys)
+    np.testing.assert_array_equal(ys.export_raw(), yc.export_raw())
+    if prec == 0 and seg == 8:
+        b = hip.FieldVector(fs).fill_uniform(0, 3)
+        xs = hip.FieldVector(fs)
+        xs.set_scalar(0.0)
+        res = hip.ConjugateGradient(op, hip.DiagonalPreconditioner.from_operator(op), 1e-8, 1e-16, 2000,
+                                    check_every=4).apply(xs, b)
+        A = orc.gen_spd(8, m.ncell_3d, 6, scaled=True)
+        xo, ro, _ = orc.cg(m, A, b.export_field(), diag=orc.mm_diagonal(m, A), max_iter=2000)
+        assert res.iterations == ro.iterations
+        assert np.linalg.norm(xs.export_field() - xo) <= 1e-8 * np.linalg.norm(xo)
+    nonsym = hip.LmaOperator(fs, precision=P).generate_uniform(1, 5, -1.0, 1.0)
+    with pytest.raises(hip.LfricError):
+        nonsym.prepare_symmetric(seg)
+    # rewriting with non-symmetric entries drops the copy: the gather apply is the plain one again
+    A_ns = nonsym.download()
+    op.upload(A_ns)
+    y1, y2 = hip.FieldVector(fs, P), hip.FieldVector(fs, P)
+    op.apply_gather(x, y1)
+    nonsym.apply_gather(x, y2)
+    np.testing.assert_array_equal(y1.export_raw(), y2.export_raw())
