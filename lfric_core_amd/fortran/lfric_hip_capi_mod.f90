@@ -50,7 +50,7 @@ module lfric_hip_capi_mod
             lfric_vec_fill_uniform
   public :: lfric_op_create, lfric_op_upload, lfric_op_generate_spd, lfric_op_convert, &
             lfric_op_destroy, lfric_lma_apply, lfric_mm_diagonal, lfric_op_invert_local, &
-            lfric_op_prepare_rows, lfric_op_tune, lfric_op_assemble_mass_w0
+            lfric_op_prepare_rows, lfric_op_tune, lfric_op_assemble_mass_w0, lfric_op_prepare_symmetric
   public :: lfric_prec_create_diagonal, lfric_prec_create_identity, lfric_prec_apply, &
             lfric_prec_destroy
   public :: lfric_linop_create_lma, lfric_linop_destroy, lfric_prec_create_column_jacobi
@@ -211,6 +211,13 @@ module lfric_hip_capi_mod
     integer(c_int) function lfric_op_prepare_rows(op) bind(c, name="lfric_op_prepare_rows")
       import :: c_int, c_ptr
       type(c_ptr), value :: op
+    end function
+    ! symmetric-packed copy of a symmetric lowest-order W0 operator (0: 4 levels per thread)
+    integer(c_int) function lfric_op_prepare_symmetric(op, levels_per_thread) &
+        bind(c, name="lfric_op_prepare_symmetric")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: op
+      integer(c_int), value :: levels_per_thread
     end function
     ! apply autotuner (XCD block-order window), form 0 colour / 1 gather
     integer(c_int) function lfric_op_tune(op, form, best_window, best_ms) bind(c, name="lfric_op_tune")
