@@ -164,6 +164,23 @@ int lfric_op_invert_local(const lfric_op *A, lfric_op *Ainv);
  * nqp (1..4) Gauss points per direction (3 integrates a trilinear cell exactly). */
 int lfric_op_assemble_mass_w0(lfric_op *op, const lfric_vec *chi1, const lfric_vec *chi2, const lfric_vec *chi3,
                               int nqp);
+/* The lowest-order mass matrix of the operator's space, FP64, on the device, from Cartesian
+ * coordinate fields chi1..3 (FP64 fields on one lowest-order W0 space of the same mesh and
+ * colouring, coord_system_xyz), nqp (1..4) Gauss points per direction:
+ *   W0 (8 dofs)  as lfric_op_assemble_mass_w0 (chi on a space with the operator's dofmap);
+ *   W2 (6 dofs)  compute_mass_matrix_w2_code (components/science/source/kernel/fem/
+ *                sci_compute_mass_matrix_kernel_w2_mod.F90:83-164), Piola-mapped face basis;
+ *   W3 (1 dof)   compute_mass_matrix_w3_code (sci_compute_mass_matrix_kernel_w3_mod.F90:270-356),
+ *                rehabilitate selecting dj (nonzero, the reference's default) or 1/dj.
+ * The space is recognised from its dofmap's column roles; anything else is refused. */
+int lfric_op_assemble_mass(lfric_op *op, const lfric_vec *chi1, const lfric_vec *chi2, const lfric_vec *chi3, int nqp,
+                           int rehabilitate);
+/* compute_div_operator_code (components/science/source/kernel/fem/
+ * sci_compute_div_operator_kernel_mod.F90:120-150): the weak divergence of an FP64 operator
+ * from a lowest-order W2 space to a W3 space of the same mesh, from the same coordinate fields;
+ * rehabilitate as above. */
+int lfric_op_assemble_div(lfric_op *op, const lfric_vec *chi1, const lfric_vec *chi2, const lfric_vec *chi3, int nqp,
+                          int rehabilitate);
 void lfric_op_destroy(lfric_op *op);
 
 /* invoke(setval_c(y,0), matrix_vector_kernel_type(y, x, A)): y = A x over the coloured

@@ -89,6 +89,8 @@ SIGNATURES = {
     "lfric_op_convert": (_I, [_VP, _VP]),
     "lfric_op_invert_local": (_I, [_VP, _VP]),
     "lfric_op_assemble_mass_w0": (_I, [_VP, _VP, _VP, _VP, _I]),
+    "lfric_op_assemble_mass": (_I, [_VP, _VP, _VP, _VP, _I, _I]),
+    "lfric_op_assemble_div": (_I, [_VP, _VP, _VP, _VP, _I, _I]),
     "lfric_op_prepare_symmetric": (_I, [_VP, _I]),
     "lfric_op_destroy": (None, [_VP]),
     "lfric_lma_apply": (_I, [_VP, _VP, _VP]),

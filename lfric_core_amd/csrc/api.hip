@@ -695,6 +695,41 @@ int lfric_op_assemble_mass_w0(lfric_op *op, const lfric_vec *chi1, const lfric_v
                                                   static_cast<const double *>(chi3->d), nqp));
 }
 
+static int check_chi(const lfric_vec *chi1, const lfric_vec *chi2, const lfric_vec *chi3) {
+  LF_CHECK(chi1 && chi2 && chi3, LFRIC_E_ARG, "null argument");
+  for (const lfric_vec *c : {chi1, chi2, chi3})
+    LF_CHECK(c->precision == LFRIC_R64 && c->nrhs == 1 && c->fs == chi1->fs, LFRIC_E_ARG,
+             "assemble: chi must be FP64 fields on one coordinate space");
+  return LFRIC_OK;
+}
+
+int lfric_op_assemble_mass(lfric_op *op, const lfric_vec *chi1, const lfric_vec *chi2, const lfric_vec *chi3, int nqp,
+                           int rehabilitate) {
+  if (int rc = require_init()) return rc;
+  LF_CHECK(op, LFRIC_E_ARG, "null argument");
+  if (int rc = check_chi(chi1, chi2, chi3)) return rc;
+  const auto *c1 = static_cast<const double *>(chi1->d), *c2 = static_cast<const double *>(chi2->d),
+             *c3 = static_cast<const double *>(chi3->d);
+  const FunctionSpace &f = *op->to;
+  if (f.ndf == 8) {
+    LF_CHECK(chi1->fs == op->to || chi1->fs->dofmap == f.dofmap, LFRIC_E_ARG,
+             "assemble: W0 coordinates must share the operator's dofmap");
+    return refresh_rows(op, launch_assemble_mass_w0(op, c1, c2, c3, nqp));
+  }
+  const int kind = f.ndf == 1 ? 0 : 1;  // W3, else W2 (the launcher checks the roles)
+  return refresh_rows(op, launch_assemble_fem(op, kind, *chi1->fs, c1, c2, c3, nqp, rehabilitate != 0));
+}
+
+int lfric_op_assemble_div(lfric_op *op, const lfric_vec *chi1, const lfric_vec *chi2, const lfric_vec *chi3, int nqp,
+                          int rehabilitate) {
+  if (int rc = require_init()) return rc;
+  LF_CHECK(op, LFRIC_E_ARG, "null argument");
+  if (int rc = check_chi(chi1, chi2, chi3)) return rc;
+  return refresh_rows(op, launch_assemble_fem(op, 2, *chi1->fs, static_cast<const double *>(chi1->d),
+                                              static_cast<const double *>(chi2->d),
+                                              static_cast<const double *>(chi3->d), nqp, rehabilitate != 0));
+}
+
 int lfric_op_invert_local(const lfric_op *A, lfric_op *Ainv) {
   if (int rc = require_init()) return rc;
   LF_CHECK(A && Ainv && A != Ainv, LFRIC_E_ARG, "null or aliased operators");

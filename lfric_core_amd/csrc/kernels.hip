@@ -280,6 +280,117 @@ __global__ __launch_bounds__(kBlock) void k_assemble_mass_w0(double *__restrict_
     }
 }
 
+// Lowest-order W2 dofs W, S, E, N, B, T: the one nonzero component of each basis vector, and
+// its divergence (basis_setup of function_space_constructor_helper_functions_mod.F90:973-1070
+// with the face normals +i, -j, +i, -j, +k, +k of reference_element_mod.F90:1436-1441).
+__device__ constexpr int kW2Comp[6] = {0, 1, 0, 1, 2, 2};
+__device__ constexpr double kW2Div[6] = {-1.0, 1.0, 1.0, -1.0, -1.0, 1.0};
+
+enum FemKind { kFemMassW3 = 0, kFemMassW2 = 1, kFemDiv = 2 };
+
+// W3 and W2 mass matrices and the weak divergence W2 -> W3 of every cell from Cartesian
+// coordinate fields on a W0 space of the same mesh (map: that space's [pos][8] column-base
+// dofs):
+//   kFemMassW3  compute_mass_matrix_w3_code (components/science/source/kernel/fem/
+//               sci_compute_mass_matrix_kernel_w3_mod.F90:270-356): sum of wh*wv*1*1*dj
+//               (rehabilitated) or wh*wv*1*1/dj;
+//   kFemMassW2  compute_mass_matrix_w2_code (sci_compute_mass_matrix_kernel_w2_mod.F90:83-164):
+//               entries df >= df2 summed from wh*wv*dot_product(J b_df, J b_df2)/dj, mirrored;
+//               J b is v*J(:, c) for a basis vector whose one nonzero component v sits in c;
+//   kFemDiv     compute_div_operator_code (sci_compute_div_operator_kernel_mod.F90:120-150):
+//               sums of wh*wv*1*div(b_df2), over dj without rehabilitation.
+// Quadrature points in the reference's order (vertical outer, horizontal inner), one thread per
+// 3D cell, no contraction: each entry is the reference's sum in the reference's order.  Tables:
+// coordinate-basis derivatives [3][8][NP], weights [NH] and [NQ], W2 values [6][NP].
+template <int NQ, int KIND>
+__global__ __launch_bounds__(kBlock) void k_assemble_fem(double *__restrict__ A, size_t gs,
+                                                         const int *__restrict__ ebase,
+                                                         const int *__restrict__ map,
+                                                         const double *__restrict__ chi1,
+                                                         const double *__restrict__ chi2,
+                                                         const double *__restrict__ chi3,
+                                                         const double *__restrict__ tables, int nl,
+                                                         long long nelem, uint32_t row_top, int rehab) {
+  constexpr int NH = NQ * NQ, NP = NH * NQ;
+  constexpr int TB = 24 * NP + NH + NQ + 6 * NP;
+  __shared__ double tab[TB];
+  for (int i = threadIdx.x; i < TB; i += kBlock) tab[i] = tables[i];
+  __syncthreads();
+  const double *db = tab, *wh = tab + 24 * NP, *wv = wh + NH, *w2 = wv + NQ;
+  const long long e = static_cast<long long>(blockIdx.x) * kBlock + threadIdx.x;
+  if (e >= nelem) return;
+  const long long pos = e / nl;
+  const int k = static_cast<int>(e - pos * nl);
+  const long long el = ebase[pos] + k;
+  double c[3][8];
+#pragma unroll
+  for (int df = 0; df < 8; ++df) {
+    const size_t d = static_cast<size_t>(map[pos * 8 + df]) + k;
+    c[0][df] = chi1[d];
+    c[1][df] = chi2[d];
+    c[2][df] = chi3[d];
+  }
+  constexpr int NS = KIND == kFemMassW3 ? 1 : KIND == kFemMassW2 ? 21 : 6;
+  double s[NS];
+#pragma unroll
+  for (int t = 0; t < NS; ++t) s[t] = 0.0;
+  for (int kz = 0; kz < NQ; ++kz)
+    for (int ic = 0; ic < NH; ++ic) {
+      const int q = kz * NH + ic;
+      double J[3][3] = {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}};
+#pragma unroll
+      for (int df = 0; df < 8; ++df)
+#pragma unroll
+        for (int dir = 0; dir < 3; ++dir)
+#pragma unroll
+          for (int r = 0; r < 3; ++r) J[r][dir] = J[r][dir] + c[r][df] * db[(dir * 8 + df) * NP + q];
+      const double dj = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) -
+                        J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
+                        J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
+      const double w = wh[ic] * wv[kz];
+      if constexpr (KIND == kFemMassW3) {
+        s[0] = s[0] + (rehab ? w * dj : w / dj);
+      } else if constexpr (KIND == kFemDiv) {
+#pragma unroll
+        for (int d = 0; d < 6; ++d) {
+          const double v = w * kW2Div[d];
+          s[d] = s[d] + (rehab ? v : v / dj);
+        }
+      } else {
+        double jb[6][3];
+#pragma unroll
+        for (int d = 0; d < 6; ++d) {
+          const double v = w2[d * NP + q];
+#pragma unroll
+          for (int r = 0; r < 3; ++r) jb[d][r] = J[r][kW2Comp[d]] * v;
+        }
+        int t = 0;
+#pragma unroll
+        for (int df2 = 0; df2 < 6; ++df2)
+#pragma unroll
+          for (int df = df2; df < 6; ++df, ++t) {
+            const double dot = jb[df][0] * jb[df2][0] + jb[df][1] * jb[df2][1] + jb[df][2] * jb[df2][2];
+            s[t] = s[t] + w * dot / dj;
+          }
+      }
+    }
+  if constexpr (KIND == kFemMassW3) {
+    A[aidx<double>(gs, el, 0)] = s[0];
+  } else if constexpr (KIND == kFemDiv) {
+#pragma unroll
+    for (int d = 0; d < 6; ++d) A[aidx<double>(gs, el, d)] = s[d];
+  } else {
+    int t = 0;
+#pragma unroll
+    for (int df2 = 0; df2 < 6; ++df2)
+#pragma unroll
+      for (int df = df2; df < 6; ++df, ++t) {
+        A[aidx<double>(gs, el + ((row_top >> df) & 1u), df * 6 + df2)] = s[t];
+        A[aidx<double>(gs, el + ((row_top >> df2) & 1u), df2 * 6 + df)] = s[t];
+      }
+  }
+}
+
 #pragma clang fp contract(on)
 
 // reference layout matrix(ncell_3d, ndf1, ndf2) <-> device layout
@@ -1582,6 +1693,89 @@ int launch_assemble_mass_w0(lfric_op *op, const double *chi1, const double *chi2
     ASM_CASE(4)
   }
 #undef ASM_CASE
+  const hipError_t e = hipGetLastError();
+  const hipError_t es = hipStreamSynchronize(ctx().stream);
+  dfree(dt, sizeof(double) * t.size());
+  if (e != hipSuccess) return fail(LFRIC_E_HIP, hipGetErrorString(e));
+  if (es != hipSuccess) return fail(LFRIC_E_HIP, hipGetErrorString(es));
+  return LFRIC_OK;
+}
+
+// Tables of k_assemble_fem: coordinate-basis derivatives [3][8][NP] and weights [nh], [nq] as
+// in w0_tables, then the W2 basis values [6][NP]: sign * (1 - t) on W, S, B and sign * t on E,
+// N, T, with t the dof's coordinate and sign that of its reference normal (-1 for S and N).
+static std::vector<double> fem_tables(int nq) {
+  const std::vector<double> t0 = w0_tables(nq);
+  std::vector<double> p(nq), w(nq);
+  gauss_rule(nq, p.data(), w.data());
+  const int nh = nq * nq, np = nh * nq;
+  std::vector<double> t(static_cast<size_t>(30 * np + nh + nq));
+  std::copy(t0.begin() + 8 * np, t0.end(), t.begin());  // derivatives, wh, wv
+  static const int comp[6] = {0, 1, 0, 1, 2, 2}, far[6] = {0, 0, 1, 1, 0, 1};
+  static const double sign[6] = {1.0, -1.0, 1.0, -1.0, 1.0, 1.0};
+  double *w2 = t.data() + 24 * np + nh + nq;
+  for (int kz = 0; kz < nq; ++kz)
+    for (int i = 0; i < nq; ++i)
+      for (int j = 0; j < nq; ++j) {
+        const int q = kz * nh + i * nq + j;
+        const double xyz[3] = {p[i], p[j], p[kz]};
+        for (int d = 0; d < 6; ++d) {
+          const double v = xyz[comp[d]];
+          w2[static_cast<size_t>(d) * np + q] = sign[d] * (far[d] ? v : 1.0 - v);
+        }
+      }
+  return t;
+}
+
+// Roles of a lowest-order space from its dofmap: W3 has one dof, W2 six with only the top face
+// (dof 6) on the next level's bottom (dof 5).
+static bool is_w3_space(const FunctionSpace &f) { return f.ndf == 1 && f.top_mask == 0u; }
+static bool is_w2_space(const FunctionSpace &f) {
+  return f.ndf == 6 && f.top_mask == (1u << 5) && f.partner[4] == 5;
+}
+static bool is_w0_space(const FunctionSpace &f) {
+  bool w0 = f.ndf == 8 && f.top_mask == 0xF0u;
+  for (int i = 0; w0 && i < 4; ++i) w0 = f.partner[i] == i + 4;
+  return w0;
+}
+
+int launch_assemble_fem(lfric_op *op, int kind, const FunctionSpace &chi_fs, const double *chi1, const double *chi2,
+                        const double *chi3, int nqp, int rehab) {
+  const FunctionSpace &f = *op->to;
+  LF_CHECK(op->precision == LFRIC_R64, LFRIC_E_UNSUPPORTED, "assemble: FP64 operator (convert for FP32)");
+  LF_CHECK(nqp >= 1 && nqp <= 4, LFRIC_E_ARG, "assemble: 1 to 4 quadrature points per direction");
+  LF_CHECK(is_w0_space(chi_fs), LFRIC_E_ARG, "assemble: the coordinate fields must be on a lowest-order W0 space");
+  LF_CHECK(chi_fs.ncells == f.ncells && chi_fs.nlayers == f.nlayers && chi_fs.cell_of_pos == f.cell_of_pos,
+           LFRIC_E_ARG, "assemble: the coordinate space is on another mesh or colouring");
+  if (kind == kFemMassW3)
+    LF_CHECK(op->to == op->from && is_w3_space(f), LFRIC_E_UNSUPPORTED, "assemble: a square lowest-order W3 operator is needed");
+  else if (kind == kFemMassW2)
+    LF_CHECK(op->to == op->from && is_w2_space(f), LFRIC_E_UNSUPPORTED, "assemble: a square lowest-order W2 operator is needed");
+  else
+    LF_CHECK(is_w3_space(f) && is_w2_space(*op->from) && op->from->cell_of_pos == f.cell_of_pos, LFRIC_E_UNSUPPORTED,
+             "assemble: the divergence maps a lowest-order W2 space to a W3 space of the same mesh");
+  const std::vector<double> t = fem_tables(nqp);
+  void *dt = nullptr;
+  if (int rc = dmalloc(&dt, sizeof(double) * t.size())) return rc;
+  LF_HIP(hipMemcpyAsync(dt, t.data(), sizeof(double) * t.size(), hipMemcpyHostToDevice, ctx().stream));
+  const long long nelem = static_cast<long long>(f.ncells) * f.nlayers;
+  double *A = static_cast<double *>(op->d);
+  const double *tb = static_cast<const double *>(dt);
+#define FEM_CASE(N, K)                                                                                           \
+  case N * 4 + K:                                                                                                \
+    LAUNCH((k_assemble_fem<N, K>), grid_for(nelem), A, op->gs, f.d_ebase, chi_fs.d_map, chi1, chi2, chi3, tb,      \
+           f.nlayers, nelem, op->row_top, rehab);                                                               \
+    break;
+  switch (nqp * 4 + kind) {
+    FEM_CASE(1, 0) FEM_CASE(1, 1) FEM_CASE(1, 2)
+    FEM_CASE(2, 0) FEM_CASE(2, 1) FEM_CASE(2, 2)
+    FEM_CASE(3, 0) FEM_CASE(3, 1) FEM_CASE(3, 2)
+    FEM_CASE(4, 0) FEM_CASE(4, 1) FEM_CASE(4, 2)
+    default:
+      dfree(dt, sizeof(double) * t.size());
+      return fail(LFRIC_E_ARG, "assemble: unknown operator kind");
+  }
+#undef FEM_CASE
   const hipError_t e = hipGetLastError();
   const hipError_t es = hipStreamSynchronize(ctx().stream);
   dfree(dt, sizeof(double) * t.size());

@@ -205,6 +205,22 @@ class LmaOperator:
         check(lib().lfric_op_assemble_mass_w0(self._h, chi1._h, chi2._h, chi3._h, int(nqp)))
         return self
 
+    def assemble_mass(self, chi1, chi2, chi3, nqp=3, rehabilitate=True):
+        """Every cell's lowest-order mass matrix of the operator's space (W0, W2 or W3, told
+        from its dofmap) from Cartesian coordinate fields on a W0 space of the same mesh:
+        compute_mass_matrix_w_scalar_code / compute_mass_matrix_w2_code /
+        compute_mass_matrix_w3_code (components/science/source/kernel/fem/
+        sci_compute_mass_matrix_kernel_{w_scalar,w2,w3}_mod.F90); FP64 operator."""
+        check(lib().lfric_op_assemble_mass(self._h, chi1._h, chi2._h, chi3._h, int(nqp), int(bool(rehabilitate))))
+        return self
+
+    def assemble_div(self, chi1, chi2, chi3, nqp=3, rehabilitate=True):
+        """The weak divergence W2 -> W3 of every cell (compute_div_operator_code,
+        components/science/source/kernel/fem/sci_compute_div_operator_kernel_mod.F90:120-150);
+        FP64 operator from a lowest-order W2 space to a W3 space."""
+        check(lib().lfric_op_assemble_div(self._h, chi1._h, chi2._h, chi3._h, int(nqp), int(bool(rehabilitate))))
+        return self
+
     def invert_local(self):
         """Per-cell inverse of every element matrix (invert_local_operator_code,
         components/science/source/kernel/algebra/sci_invert_local_operator_kernel_mod.F90:51-77),

@@ -50,7 +50,8 @@ module lfric_hip_capi_mod
             lfric_vec_fill_uniform
   public :: lfric_op_create, lfric_op_upload, lfric_op_generate_spd, lfric_op_convert, &
             lfric_op_destroy, lfric_lma_apply, lfric_mm_diagonal, lfric_op_invert_local, &
-            lfric_op_prepare_rows, lfric_op_tune, lfric_op_assemble_mass_w0, lfric_op_prepare_symmetric
+            lfric_op_prepare_rows, lfric_op_tune, lfric_op_assemble_mass_w0, lfric_op_prepare_symmetric, &
+            lfric_op_assemble_mass, lfric_op_assemble_div
   public :: lfric_prec_create_diagonal, lfric_prec_create_identity, lfric_prec_apply, &
             lfric_prec_destroy
   public :: lfric_linop_create_lma, lfric_linop_destroy, lfric_prec_create_column_jacobi
@@ -206,6 +207,21 @@ module lfric_hip_capi_mod
       import :: c_int, c_ptr
       type(c_ptr), value :: op, chi1, chi2, chi3
       integer(c_int), value :: nqp
+    end function
+    ! compute_mass_matrix_w2_code / _w3_code (or the W0 one) for the operator's lowest-order
+    ! space, from Cartesian chi fields on a W0 space of the same mesh
+    integer(c_int) function lfric_op_assemble_mass(op, chi1, chi2, chi3, nqp, rehabilitate) &
+        bind(c, name="lfric_op_assemble_mass")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: op, chi1, chi2, chi3
+      integer(c_int), value :: nqp, rehabilitate
+    end function
+    ! compute_div_operator_code: weak divergence W2 -> W3
+    integer(c_int) function lfric_op_assemble_div(op, chi1, chi2, chi3, nqp, rehabilitate) &
+        bind(c, name="lfric_op_assemble_div")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: op, chi1, chi2, chi3
+      integer(c_int), value :: nqp, rehabilitate
     end function
     ! row-gathered copy for the one-launch W0 apply (set-up, once per operator)
     integer(c_int) function lfric_op_prepare_rows(op) bind(c, name="lfric_op_prepare_rows")

@@ -79,6 +79,10 @@ def lib():
             "orc_invert_local_r32": (_I, [_I64, _I, _VP, _VP]),
             "orc_gauss_rule": (_I, [_I, _VP, _VP]),
             "orc_mass_matrix_w0": (_I, [_I, _I, _VP, _VP, _VP, _VP, _I, _VP]),
+            "orc_w2_basis": (_I, [_I, _VP, _VP]),
+            "orc_mass_matrix_w2": (_I, [_I, _I, _VP, _VP, _VP, _VP, _I, _VP]),
+            "orc_mass_matrix_w3": (_I, [_I, _I, _VP, _VP, _VP, _VP, _I, _I, _VP]),
+            "orc_div_operator": (_I, [_I, _I, _VP, _VP, _VP, _VP, _I, _I, _VP]),
             "orc_cg": (_I, [ctypes.POINTER(OrcProblem), _VP, _VP, _D, _D, _I, _I, ctypes.POINTER(OrcResult), _VP]),
             "orc_gmres": (_I, [ctypes.POINTER(OrcProblem), _VP, _VP, _I, _D, _D, _I, _I, ctypes.POINTER(OrcResult), _VP]),
             "orc_bicgstab": (_I, [ctypes.POINTER(OrcProblem), _VP, _VP, _D, _D, _I, _I, ctypes.POINTER(OrcResult), _VP]),
@@ -263,6 +267,50 @@ def mass_matrix_w0(ncells, nlayers, dofmap, chi1, chi2, chi3, nqp=3):
     if lib().orc_mass_matrix_w0(ncells, nlayers, _p(dm), _p(c[0]), _p(c[1]), _p(c[2]), nqp, _p(out)) != 0:
         raise ValueError("mass_matrix_w0: nqp out of range")
     return out.reshape((8, 8, ncells * nlayers)).transpose(2, 1, 0)
+
+
+def _chi_args(dofmap, chi1, chi2, chi3):
+    dm = np.ascontiguousarray(dofmap, np.int32)
+    c = [np.ascontiguousarray(v, np.float64) for v in (chi1, chi2, chi3)]
+    return dm, c
+
+
+def w2_basis(nqp):
+    """Lowest-order W2 basis (3, 6, nqp_h, nqp_v) and divergence (1, 6, nqp_h, nqp_v) tables."""
+    nh = nqp * nqp
+    b, d = np.zeros(3 * 6 * nh * nqp), np.zeros(6 * nh * nqp)
+    if lib().orc_w2_basis(nqp, _p(b), _p(d)) != 0:
+        raise ValueError("w2_basis: nqp out of range")
+    return b.reshape((3, 6, nh, nqp), order="F"), d.reshape((1, 6, nh, nqp), order="F")
+
+
+def mass_matrix_w2(ncells, nlayers, dofmap, chi1, chi2, chi3, nqp=3):
+    """Lowest-order W2 mass matrices (ncells*nlayers, 6, 6); dofmap is the coordinate (W0) map."""
+    dm, c = _chi_args(dofmap, chi1, chi2, chi3)
+    out = np.zeros(ncells * nlayers * 36)
+    if lib().orc_mass_matrix_w2(ncells, nlayers, _p(dm), _p(c[0]), _p(c[1]), _p(c[2]), nqp, _p(out)) != 0:
+        raise ValueError("mass_matrix_w2: nqp out of range")
+    return out.reshape((6, 6, ncells * nlayers)).transpose(2, 1, 0)
+
+
+def mass_matrix_w3(ncells, nlayers, dofmap, chi1, chi2, chi3, nqp=3, rehabilitate=True):
+    """Lowest-order W3 mass matrices (ncells*nlayers, 1, 1)."""
+    dm, c = _chi_args(dofmap, chi1, chi2, chi3)
+    out = np.zeros(ncells * nlayers)
+    if lib().orc_mass_matrix_w3(ncells, nlayers, _p(dm), _p(c[0]), _p(c[1]), _p(c[2]), nqp, int(bool(rehabilitate)),
+                                _p(out)) != 0:
+        raise ValueError("mass_matrix_w3: nqp out of range")
+    return out.reshape((-1, 1, 1))
+
+
+def div_operator(ncells, nlayers, dofmap, chi1, chi2, chi3, nqp=3, rehabilitate=True):
+    """Weak divergence W2 -> W3 element matrices (ncells*nlayers, 1, 6)."""
+    dm, c = _chi_args(dofmap, chi1, chi2, chi3)
+    out = np.zeros(ncells * nlayers * 6)
+    if lib().orc_div_operator(ncells, nlayers, _p(dm), _p(c[0]), _p(c[1]), _p(c[2]), nqp, int(bool(rehabilitate)),
+                              _p(out)) != 0:
+        raise ValueError("div_operator: nqp out of range")
+    return out.reshape((6, 1, ncells * nlayers)).transpose(2, 1, 0)
 
 
 def make_problem(mesh, A, diag=None):

@@ -31,6 +31,16 @@
  *                             6000 x 1000 x 2000 box); closed forms V (M1 x M1 x M1) for boxes
  *                             and det(J) M_ref for sheared parallelepipeds; sum of entries =
  *                             cell volume
+ *   orc_w2_basis              basis and divergence tables of get_unit_test_q3x3x3_basis_mod.f90
+ *   orc_mass_matrix_w2        known answer of compute_mass_matrix_kernel_w2_mod_test.pf; closed
+ *                             form G(c_i, c_j) I_ij / det(A) with G = A^T A for sheared
+ *                             parallelepipeds x = A xi + o
+ *   orc_mass_matrix_w3        known answer of compute_mass_matrix_kernel_w3_mod_test.pf (volume);
+ *                             equals the sum of the cell's W0 mass entries; independent of nqp
+ *                             >= 2 on trilinear cells; 1/det(A) without rehabilitation
+ *   orc_div_operator          known answer of compute_div_operator_kernel_mod_test.pf
+ *                             (-1, 1, 1, -1, -1, 1); without rehabilitation = divergence sign x
+ *                             the W3 entry without rehabilitation, bit for bit
  *   orc_invert_local          known answers of invert_local_operator_kernel_mod_test.pf (1/5) and
  *                             matrix_invert_mod_test.pf (a a^-1 = I); numpy.linalg.inv on
  *                             nonsymmetric matrices; bit-level comparison with the reference's
@@ -110,6 +120,15 @@ int orc_invert_local_r32(int64_t ncell3d, int ndf, const float *matrix, float *i
 int orc_gauss_rule(int n, double *pts, double *wts);
 int orc_mass_matrix_w0(int ncells, int nlayers, const int *map, const double *chi1, const double *chi2,
                        const double *chi3, int nqp, double *mm);
+/* Lowest-order W2 basis tables, W2 and W3 mass matrices and the weak divergence W2 -> W3 from
+ * the same coordinate fields (map is the W0 dofmap of the coordinate space) */
+int orc_w2_basis(int nqp, double *basis, double *div);
+int orc_mass_matrix_w2(int ncells, int nlayers, const int *map, const double *chi1, const double *chi2,
+                       const double *chi3, int nqp, double *mm);
+int orc_mass_matrix_w3(int ncells, int nlayers, const int *map, const double *chi1, const double *chi2,
+                       const double *chi3, int nqp, int rehabilitate, double *mm);
+int orc_div_operator(int ncells, int nlayers, const int *map, const double *chi1, const double *chi2,
+                     const double *chi3, int nqp, int rehabilitate, double *div);
 
 /* ---- Krylov solvers ------------------------------------------------------------------- */
 typedef struct orc_problem {

@@ -207,74 +207,222 @@ int orc_gauss_rule(int n, double *pts, double *wts) {
   return 0;
 }
 
-/* Lowest-order W0 mass matrix from Cartesian vertex coordinates: compute_mass_matrix_w_scalar_code
- * (components/science/source/kernel/fem/sci_compute_mass_matrix_kernel_w_scalar_mod.F90:290-384,
- * r_double) with the coordinate Jacobian of coordinate_jacobian for coord_system_xyz
- * (components/science/source/kernel/geometry/sci_coordinate_jacobian_mod.F90:289-445: jac(i,dir)
- * = sum_df chi_i(df) d_dir N_df, dj by cofactors).  Quadrature xyoz: the horizontal points pair
- * x_i with y_j, y fastest (infrastructure/source/quadrature/quadrature_xyoz_mod.F90:235-243).
- * Trilinear basis on the reference cube, vertices SWB, SEB, NEB, NWB, SWT, SET, NET, NWT
- * (reference_element_mod.F90:1244-1251): N = lx*ly*lz with l = t or 1 - t, derivatives with
- * +-1 in place of the differentiated factor.  Entries df >= df2 are summed over (qp_v outer,
- * qp_h inner) of wh*wv*N_df*N_df2*dj and mirrored.  chi1..3 are W0 fields (undf); map is the
- * W0 dofmap (1-based, [cell][8]); mm in the reference layout (ncell_3d, 8, 8). */
-int orc_mass_matrix_w0(int ncells, int nl, const int *map, const double *chi1, const double *chi2,
-                       const double *chi3, int nqp, double *mm) {
-  enum { NQMAX = 8 };
-  if (nqp < 1 || nqp > NQMAX) return 1;
-  double p[NQMAX], w[NQMAX];
-  orc_gauss_rule(nqp, p, w);
-  const int nh = nqp * nqp, nv = nqp;
-  static const int vx[8] = {0, 1, 1, 0, 0, 1, 1, 0}, vy[8] = {0, 0, 1, 1, 0, 0, 1, 1},
-                   vz[8] = {0, 0, 0, 0, 1, 1, 1, 1};
-  double b[8][NQMAX * NQMAX][NQMAX], db[3][8][NQMAX * NQMAX][NQMAX], wh[NQMAX * NQMAX];
+/* Quadrature and W0 (coordinate) basis tables shared by the assembly restatements.  Gauss
+ * points from orc_gauss_rule; horizontal points pair x_i with y_j, y fastest
+ * (infrastructure/source/quadrature/quadrature_xyoz_mod.F90:235-243).  Trilinear basis on the
+ * reference cube, vertices SWB, SEB, NEB, NWB, SWT, SET, NET, NWT (reference_element_mod.F90:
+ * 1244-1251): N = lx*ly*lz with l = t or 1 - t, derivatives with +-1 in place of the
+ * differentiated factor. */
+enum { ORC_NQMAX = 8 };
+typedef struct {
+  int nq, nh;
+  double p[ORC_NQMAX], w[ORC_NQMAX], wh[ORC_NQMAX * ORC_NQMAX];
+  double b[8][ORC_NQMAX * ORC_NQMAX][ORC_NQMAX], db[3][8][ORC_NQMAX * ORC_NQMAX][ORC_NQMAX];
+} orc_tab;
+
+static const int orc_vx[8] = {0, 1, 1, 0, 0, 1, 1, 0}, orc_vy[8] = {0, 0, 1, 1, 0, 0, 1, 1},
+                 orc_vz[8] = {0, 0, 0, 0, 1, 1, 1, 1};
+
+static int orc_tab_init(orc_tab *t, int nqp) {
+  if (nqp < 1 || nqp > ORC_NQMAX) return 1;
+  t->nq = nqp;
+  t->nh = nqp * nqp;
+  orc_gauss_rule(nqp, t->p, t->w);
   for (int i = 0; i < nqp; ++i)
     for (int j = 0; j < nqp; ++j) {
       const int ic = i * nqp + j;
-      wh[ic] = w[i] * w[j];
-      for (int kz = 0; kz < nv; ++kz)
+      t->wh[ic] = t->w[i] * t->w[j];
+      for (int kz = 0; kz < nqp; ++kz)
         for (int df = 0; df < 8; ++df) {
-          const double x = p[i], y = p[j], z = p[kz];
-          const double lx = vx[df] ? x : 1.0 - x, ly = vy[df] ? y : 1.0 - y, lz = vz[df] ? z : 1.0 - z;
-          const double dx = vx[df] ? 1.0 : -1.0, dy = vy[df] ? 1.0 : -1.0, dz = vz[df] ? 1.0 : -1.0;
-          b[df][ic][kz] = lx * ly * lz;
-          db[0][df][ic][kz] = dx * ly * lz;
-          db[1][df][ic][kz] = lx * dy * lz;
-          db[2][df][ic][kz] = lx * ly * dz;
+          const double x = t->p[i], y = t->p[j], z = t->p[kz];
+          const double lx = orc_vx[df] ? x : 1.0 - x, ly = orc_vy[df] ? y : 1.0 - y, lz = orc_vz[df] ? z : 1.0 - z;
+          const double dx = orc_vx[df] ? 1.0 : -1.0, dy = orc_vy[df] ? 1.0 : -1.0, dz = orc_vz[df] ? 1.0 : -1.0;
+          t->b[df][ic][kz] = lx * ly * lz;
+          t->db[0][df][ic][kz] = dx * ly * lz;
+          t->db[1][df][ic][kz] = lx * dy * lz;
+          t->db[2][df][ic][kz] = lx * ly * dz;
         }
     }
+  return 0;
+}
+
+/* coordinate_jacobian for coord_system_xyz
+ * (components/science/source/kernel/geometry/sci_coordinate_jacobian_mod.F90:289-445): jac(i,dir)
+ * = sum_df chi_i(df) d_dir N_df in df order, dj by cofactors along the first row.  c[3][8] are
+ * the cell's vertex coordinates. */
+static double orc_jacobian(const orc_tab *t, const double c[3][8], int ic, int kz, double J[3][3]) {
+  for (int r = 0; r < 3; ++r)
+    for (int dir = 0; dir < 3; ++dir) J[r][dir] = 0.0;
+  for (int df = 0; df < 8; ++df)
+    for (int dir = 0; dir < 3; ++dir)
+      for (int r = 0; r < 3; ++r) J[r][dir] = J[r][dir] + c[r][df] * t->db[dir][df][ic][kz];
+  return J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) - J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
+         J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
+}
+
+static void orc_cell_coords(const int *mc, int k, const double *chi1, const double *chi2, const double *chi3,
+                            double c[3][8]) {
+  for (int df = 0; df < 8; ++df) {
+    c[0][df] = chi1[mc[df] - 1 + k];
+    c[1][df] = chi2[mc[df] - 1 + k];
+    c[2][df] = chi3[mc[df] - 1 + k];
+  }
+}
+
+/* Lowest-order W0 mass matrix from Cartesian vertex coordinates: compute_mass_matrix_w_scalar_code
+ * (components/science/source/kernel/fem/sci_compute_mass_matrix_kernel_w_scalar_mod.F90:290-384,
+ * r_double) with the Jacobian above.  Entries df >= df2 are summed over (qp_v outer, qp_h inner)
+ * of wh*wv*N_df*N_df2*dj and mirrored.  chi1..3 are W0 fields (undf); map is the W0 dofmap
+ * (1-based, [cell][8]); mm in the reference layout (ncell_3d, 8, 8). */
+int orc_mass_matrix_w0(int ncells, int nl, const int *map, const double *chi1, const double *chi2,
+                       const double *chi3, int nqp, double *mm) {
+  static orc_tab t;
+  if (orc_tab_init(&t, nqp)) return 1;
+  const int nh = t.nh, nv = nqp;
   const int64_t nc3 = (int64_t)ncells * nl;
   for (int cell = 0; cell < ncells; ++cell) {
-    const int *mc = &map[(size_t)cell * 8];
     for (int k = 0; k < nl; ++k) {
       const int64_t ik = (int64_t)cell * nl + k;
-      double c[3][8], dj[NQMAX * NQMAX][NQMAX];
-      for (int df = 0; df < 8; ++df) {
-        c[0][df] = chi1[mc[df] - 1 + k];
-        c[1][df] = chi2[mc[df] - 1 + k];
-        c[2][df] = chi3[mc[df] - 1 + k];
-      }
+      double c[3][8], dj[ORC_NQMAX * ORC_NQMAX][ORC_NQMAX], J[3][3];
+      orc_cell_coords(&map[(size_t)cell * 8], k, chi1, chi2, chi3, c);
       for (int kz = 0; kz < nv; ++kz)
-        for (int ic = 0; ic < nh; ++ic) {
-          double J[3][3] = {{0}};
-          for (int df = 0; df < 8; ++df)
-            for (int dir = 0; dir < 3; ++dir)
-              for (int r = 0; r < 3; ++r) J[r][dir] = J[r][dir] + c[r][df] * db[dir][df][ic][kz];
-          dj[ic][kz] = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) -
-                       J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
-                       J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
-        }
+        for (int ic = 0; ic < nh; ++ic) dj[ic][kz] = orc_jacobian(&t, c, ic, kz, J);
       for (int df2 = 0; df2 < 8; ++df2) {
         for (int df = df2; df < 8; ++df) {
           double s = 0.0;
           for (int kz = 0; kz < nv; ++kz)
             for (int ic = 0; ic < nh; ++ic)
-              s = s + wh[ic] * w[kz] * b[df][ic][kz] * b[df2][ic][kz] * dj[ic][kz];
+              s = s + t.wh[ic] * t.w[kz] * t.b[df][ic][kz] * t.b[df2][ic][kz] * dj[ic][kz];
           mm[ik + nc3 * (df + 8 * (int64_t)df2)] = s;
         }
         for (int df = df2 - 1; df >= 0; --df) mm[ik + nc3 * (df + 8 * (int64_t)df2)] = mm[ik + nc3 * (df2 + 8 * (int64_t)df)];
       }
     }
   }
+  return 0;
+}
+
+/* Lowest-order W2 basis (Raviart-Thomas on the reference cube) as basis_setup builds it
+ * (infrastructure/source/function_space/function_space_constructor_helper_functions_mod.F90:
+ * 973-1070 with k_h = k_v = 0: one dof per face W, S, E, N, B, T, unit vector the face's
+ * normals_to_faces entry, reference_element_mod.F90:1436-1441, i.e. +i, -j, +i, -j, +k, +k,
+ * times the linear Lagrange factor that is 1 on that face): each dof has one nonzero
+ * component, comp[d], of value sign[d] * (1 - t) on W, S, B and sign[d] * t on E, N, T, where t
+ * is x, y or z; its divergence is the constant ddiv[d]. */
+static const int orc_w2_comp[6] = {0, 1, 0, 1, 2, 2};
+static const double orc_w2_sign[6] = {1.0, -1.0, 1.0, -1.0, 1.0, 1.0};
+static const int orc_w2_far[6] = {0, 0, 1, 1, 0, 1};  /* factor t (1) or 1 - t (0) */
+static const double orc_w2_div[6] = {-1.0, 1.0, 1.0, -1.0, -1.0, 1.0};
+
+static double orc_w2_value(const orc_tab *t, int d, int ic, int kz) {
+  const int i = ic / t->nq, j = ic % t->nq;
+  const double xyz[3] = {t->p[i], t->p[j], t->p[kz]};
+  const double s = xyz[orc_w2_comp[d]];
+  return orc_w2_sign[d] * (orc_w2_far[d] ? s : 1.0 - s);
+}
+
+/* basis_w2(3, 6, nqp_h, nqp_v) and diff_basis_w2(1, 6, nqp_h, nqp_v) in the reference layout. */
+int orc_w2_basis(int nqp, double *basis, double *div) {
+  static orc_tab t;
+  if (orc_tab_init(&t, nqp)) return 1;
+  for (int kz = 0; kz < nqp; ++kz)
+    for (int ic = 0; ic < t.nh; ++ic)
+      for (int d = 0; d < 6; ++d) {
+        const size_t q = (size_t)ic + (size_t)t.nh * kz;
+        for (int r = 0; r < 3; ++r) basis[r + 3 * (d + 6 * q)] = 0.0;
+        basis[orc_w2_comp[d] + 3 * (d + 6 * q)] = orc_w2_value(&t, d, ic, kz);
+        div[d + 6 * q] = orc_w2_div[d];
+      }
+  return 0;
+}
+
+/* Lowest-order W2 mass matrix: compute_mass_matrix_w2_code
+ * (components/science/source/kernel/fem/sci_compute_mass_matrix_kernel_w2_mod.F90:83-164).
+ * Entries df >= df2 are sums over (qp_v outer, qp_h inner) of
+ * wh*wv*dot_product(matmul(jac, b_df), matmul(jac, b_df2))/dj, mirrored.  matmul of jac with a
+ * basis vector whose one nonzero component is v in column c is v*jac(:, c) (the other two
+ * products are zeros, which leave every nonzero sum unchanged); dot_product sums the three
+ * component products in order.  mm in the reference layout (ncell_3d, 6, 6). */
+int orc_mass_matrix_w2(int ncells, int nl, const int *map, const double *chi1, const double *chi2,
+                       const double *chi3, int nqp, double *mm) {
+  static orc_tab t;
+  if (orc_tab_init(&t, nqp)) return 1;
+  const int nh = t.nh, nv = nqp;
+  const int64_t nc3 = (int64_t)ncells * nl;
+  for (int cell = 0; cell < ncells; ++cell) {
+    for (int k = 0; k < nl; ++k) {
+      const int64_t ik = (int64_t)cell * nl + k;
+      double c[3][8], s[6][6] = {{0}};
+      orc_cell_coords(&map[(size_t)cell * 8], k, chi1, chi2, chi3, c);
+      for (int kz = 0; kz < nv; ++kz)
+        for (int ic = 0; ic < nh; ++ic) {
+          double J[3][3], jb[6][3];
+          const double dj = orc_jacobian(&t, c, ic, kz, J);
+          for (int d = 0; d < 6; ++d) {
+            const double v = orc_w2_value(&t, d, ic, kz);
+            for (int r = 0; r < 3; ++r) jb[d][r] = J[r][orc_w2_comp[d]] * v;
+          }
+          for (int df2 = 0; df2 < 6; ++df2)
+            for (int df = df2; df < 6; ++df) {
+              const double dot = jb[df][0] * jb[df2][0] + jb[df][1] * jb[df2][1] + jb[df][2] * jb[df2][2];
+              s[df][df2] = s[df][df2] + t.wh[ic] * t.w[kz] * dot / dj;
+            }
+        }
+      for (int df2 = 0; df2 < 6; ++df2) {
+        for (int df = df2; df < 6; ++df) mm[ik + nc3 * (df + 6 * (int64_t)df2)] = s[df][df2];
+        for (int df = df2 - 1; df >= 0; --df) mm[ik + nc3 * (df + 6 * (int64_t)df2)] = s[df2][df];
+      }
+    }
+  }
+  return 0;
+}
+
+/* Lowest-order W3 mass matrix: compute_mass_matrix_w3_code_r_double
+ * (components/science/source/kernel/fem/sci_compute_mass_matrix_kernel_w3_mod.F90:270-356): the
+ * one W3 basis function is 1, so the entry is the sum over (qp_v outer, qp_h inner) of
+ * wh*wv*1*1*dj with rehabilitation, wh*wv*1*1/dj without.  mm has ncell_3d entries. */
+int orc_mass_matrix_w3(int ncells, int nl, const int *map, const double *chi1, const double *chi2,
+                       const double *chi3, int nqp, int rehabilitate, double *mm) {
+  static orc_tab t;
+  if (orc_tab_init(&t, nqp)) return 1;
+  for (int cell = 0; cell < ncells; ++cell)
+    for (int k = 0; k < nl; ++k) {
+      double c[3][8], J[3][3], s = 0.0;
+      orc_cell_coords(&map[(size_t)cell * 8], k, chi1, chi2, chi3, c);
+      for (int kz = 0; kz < nqp; ++kz)
+        for (int ic = 0; ic < t.nh; ++ic) {
+          const double dj = orc_jacobian(&t, c, ic, kz, J);
+          const double w = t.wh[ic] * t.w[kz] * 1.0 * 1.0;
+          s = s + (rehabilitate ? w * dj : w / dj);
+        }
+      mm[(int64_t)cell * nl + k] = s;
+    }
+  return 0;
+}
+
+/* Weak divergence W2 -> W3: compute_div_operator_code
+ * (components/science/source/kernel/fem/sci_compute_div_operator_kernel_mod.F90:120-150): entry
+ * (1, df2) is the sum over (qp_v outer, qp_h inner) of wh*wv*1*div(b_df2) with rehabilitation,
+ * that over dj without.  div in the reference layout (ncell_3d, 1, 6). */
+int orc_div_operator(int ncells, int nl, const int *map, const double *chi1, const double *chi2,
+                     const double *chi3, int nqp, int rehabilitate, double *div) {
+  static orc_tab t;
+  if (orc_tab_init(&t, nqp)) return 1;
+  const int64_t nc3 = (int64_t)ncells * nl;
+  for (int cell = 0; cell < ncells; ++cell)
+    for (int k = 0; k < nl; ++k) {
+      const int64_t ik = (int64_t)cell * nl + k;
+      double c[3][8], J[3][3], s[6] = {0};
+      orc_cell_coords(&map[(size_t)cell * 8], k, chi1, chi2, chi3, c);
+      for (int kz = 0; kz < nqp; ++kz)
+        for (int ic = 0; ic < t.nh; ++ic) {
+          const double dj = orc_jacobian(&t, c, ic, kz, J);
+          for (int d = 0; d < 6; ++d) {
+            const double v = t.wh[ic] * t.w[kz] * 1.0 * orc_w2_div[d];
+            s[d] = s[d] + (rehabilitate ? v : v / dj);
+          }
+        }
+      for (int d = 0; d < 6; ++d) div[ik + nc3 * d] = s[d];
+    }
   return 0;
 }

@@ -20,8 +20,10 @@ Everything written is data (inputs and expected outputs); no reference source te
                       the transpose / dg / mm_diagonal and invert_local_operator kernel tests,
                       the matrix_invert_lu test
                       (infrastructure/unit-test/utilities/matrix_invert_mod_test.pf:25,39-46) and
-                      the W0 mass-matrix test
-                      (components/science/unit-test/kernel/fem/compute_mass_matrix_kernel_w0_mod_test.pf)
+                      the W0, W2 and W3 mass-matrix and weak-divergence tests
+                      (components/science/unit-test/kernel/fem/compute_mass_matrix_kernel_w{0,2,3}_mod_test.pf,
+                      compute_div_operator_kernel_mod_test.pf), the W2 basis tables at the
+                      3x3x3 Gauss points (components/science/unit-test/support/get_unit_test_q3x3x3_basis_mod.f90)
                       and the C3 mesh-generator summary counts
                       (rose-stem/app/check_summary/file/summary_cubedsphere_c3_ugrid.kgo.txt:8-10).
 """
@@ -132,6 +134,35 @@ def main(ref):
     assert len(table) == 72
     d = np.array([[2.0, -2.0], [3.0, -1.0], [0.4, 1.3], [-4.5, 7.9]])
     x = np.array([0.25, 1.5])
+    # compute_mass_matrix_kernel_w2_mod_test.pf:88-183 (answer mass_matrix_w2 :197-), the W3
+    # mass test (compute_mass_matrix_kernel_w3_mod_test.pf:215-216, answer 1 after dividing by
+    # dx*dy*dz) and compute_div_operator_kernel_mod_test.pf:183-184 (answer div_operator), all on
+    # the same 6000 x 1000 x 2000 box cell with the 3x3x3 Gauss rule
+    fem = ref / "components/science/unit-test/kernel/fem"
+    w2_text = (fem / "compute_mass_matrix_kernel_w2_mod_test.pf").read_text()
+    w2_body = w2_text[w2_text.index("subroutine mass_matrix_w2( mm )"):]
+    w2_body = w2_body[:w2_body.index("end subroutine mass_matrix_w2")]
+    w2_vals = [float(v) for v in re.findall(r"([0-9.]+E[+-][0-9]+)_r_def", w2_body)]
+    assert len(w2_vals) == 36
+    div_text = (fem / "compute_div_operator_kernel_mod_test.pf").read_text()
+    div_body = div_text[div_text.index("subroutine div_operator( div )"):]
+    div_body = div_body[div_body.index("d32 = "):div_body.index("shape(d32)")]
+    div_body = div_body[div_body.index("["):]
+    div_vals = [float(v) for v in re.findall(r"-?\d+", div_body)]
+    assert len(div_vals) == 6
+    w3_text = (fem / "compute_mass_matrix_kernel_w3_mod_test.pf").read_text()
+    w3_answer = float(re.search(r"answer = ([0-9.]+)_r_double", w3_text).group(1))
+    # W2 basis and divergence at the 3x3x3 Gauss points (get_unit_test_q3x3x3_basis_mod.f90:1026-1263)
+    qb = (ref / "components/science/unit-test/support/get_unit_test_q3x3x3_basis_mod.f90").read_text()
+
+    def basis_table(name, end):
+        body = qb[qb.index(name + " = reshape("):]
+        body = body[:body.index(end)]
+        return [float(v) for v in re.findall(r"(-?[0-9]+\.[0-9]+)_r_def", body)]
+
+    w2_basis = basis_table("basis_w2", "end subroutine get_w2_q3x3x3_basis")
+    w2_div = basis_table("diff_basis_w2", "end subroutine get_w2_q3x3x3_diff_basis")
+    assert len(w2_basis) == 3 * 6 * 27 and len(w2_div) == 6 * 27
     summary = (ref / "rose-stem/app/check_summary/file/summary_cubedsphere_c3_ugrid.kgo.txt").read_text()
     counts = {k: int(re.search(k + r":\s*(\d+)", summary).group(1)) for k in ("Nodes", "Edges", "Faces")}
     kat = {
@@ -170,6 +201,16 @@ def main(ref):
             "dx": dims["dx"], "dy": dims["dy"], "dz": dims["dz"],
             "answer_column_major": vals, "tol": 1e-6,
         },
+        "mass_matrix_w2": {
+            "dx": dims["dx"], "dy": dims["dy"], "dz": dims["dz"], "nlayers": 3,
+            "answer_column_major": w2_vals, "normalise": "(dx + dy + dz) / 3", "tol": 1e-6,
+        },
+        "mass_matrix_w3": {"dx": dims["dx"], "dy": dims["dy"], "dz": dims["dz"], "nlayers": 3,
+                           "answer_over_volume": w3_answer, "tol": 1e-6},
+        "div_operator": {"dx": dims["dx"], "dy": dims["dy"], "dz": dims["dz"], "nlayers": 3,
+                         "answer": div_vals, "tol": 1e-6},
+        "w2_basis_q3x3x3": {"basis_column_major": w2_basis, "shape": [3, 6, 9, 3],
+                            "div_column_major": w2_div, "div_shape": [1, 6, 9, 3], "digits": 15},
         "c3_summary": counts,
     }
     (HERE / "kat.json").write_text(json.dumps(kat, indent=1) + "\n")

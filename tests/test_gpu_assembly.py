@@ -1,10 +1,13 @@
-"""On-device assembly of lowest-order W0 mass matrices from Cartesian coordinate fields
-(compute_mass_matrix_w_scalar_code), against the oracle and the reference's unit-test answer.
+"""On-device assembly of lowest-order mass matrices (W0: compute_mass_matrix_w_scalar_code, W2:
+compute_mass_matrix_w2_code, W3: compute_mass_matrix_w3_code) and of the weak divergence W2 -> W3
+(compute_div_operator_code) from Cartesian coordinate fields, against the oracle and the
+reference's unit-test answers.
 
 Bars: every entry equal to the oracle's bit for bit (both sum the same products in the same
 order without contraction, from the same Gauss rule and basis tables); the reference's known
-answer within its own tolerance; a CG solve with the assembled matrix recovers x = 1 from
-b = M 1."""
+answers within their own tolerance; a CG solve with the assembled W0 matrix recovers x = 1 from
+b = M 1; the Helmholtz operator M3 + tau D diag(M2)^-1 D^T built from assembled operators equals
+its numpy restatement from the oracle's matrices, and GMRES recovers a known solution."""
 import json
 
 import numpy as np
@@ -92,3 +95,122 @@ def test_assembled_mass_matrix_known_answer(hip, orc):
         np.testing.assert_allclose(mm[layer] / vol, ans / vol, rtol=0, atol=k["tol"])
     with pytest.raises(hip.LfricError):  # FP32 operators are converted after assembly
         hip.LmaOperator(fs, precision=hip.R32).assemble_mass_w0(*cv)
+
+
+def _box_w2_w3(nx, ny, nl):
+    """Lowest-order W2 and W3 dofmaps of the box of _box_mesh (same cells, same colouring):
+    x-normal face columns (nl dofs each), then y-normal ones, then one column of horizontal faces
+    per cell (nl + 1 dofs: the top face of layer k is the bottom face of layer k + 1).  Local
+    order W, S, E, N, B, T."""
+    xcol = lambda i, j: (j * (nx + 1) + i) * nl
+    ny0 = (nx + 1) * ny * nl
+    ycol = lambda i, j: ny0 + (j * nx + i) * nl
+    nz0 = ny0 + nx * (ny + 1) * nl
+    zcol = lambda i, j: nz0 + (j * nx + i) * (nl + 1)
+    w2, w3 = [], []
+    for j in range(ny):
+        for i in range(nx):
+            b = zcol(i, j)
+            w2.append([xcol(i, j) + 1, ycol(i, j) + 1, xcol(i + 1, j) + 1, ycol(i, j + 1) + 1, b + 1, b + 2])
+            w3.append([(j * nx + i) * nl + 1])
+    return np.array(w2, np.int32), nz0 + nx * ny * (nl + 1), np.array(w3, np.int32), nx * ny * nl
+
+
+def _box_spaces(hip, nx, ny, nl):
+    dm0, npc, cic, undf0 = _box_mesh(nx, ny, nl)
+    dm2, undf2, dm3, undf3 = _box_w2_w3(nx, ny, nl)
+    w0 = hip.FunctionSpace.from_dofmap(nl, undf0, dm0, npc, cic)
+    w2 = hip.FunctionSpace.from_dofmap(nl, undf2, dm2, npc, cic)
+    w3 = hip.FunctionSpace.from_dofmap(nl, undf3, dm3, npc, cic)
+    return (w0, dm0), (w2, dm2), (w3, dm3)
+
+
+def test_assembled_w2_w3_div_bitwise_equal_oracle(hip, orc):
+    nx, ny, nl = 5, 4, 3
+    (w0, dm0), (w2, _), (w3, _) = _box_spaces(hip, nx, ny, nl)
+    chi = _chi_of_box(nx, ny, nl, np.random.default_rng(7))
+    cv = [hip.FieldVector(w0).import_field(c) for c in chi]
+    nc = nx * ny
+    for nqp in (2, 3, 4):
+        print("assemble nqp", nqp, flush=True)
+        m2 = hip.LmaOperator(w2).assemble_mass(*cv, nqp=nqp).download()
+        np.testing.assert_array_equal(m2, orc.mass_matrix_w2(nc, nl, dm0, *chi, nqp=nqp))
+        for rehab in (True, False):
+            m3 = hip.LmaOperator(w3).assemble_mass(*cv, nqp=nqp, rehabilitate=rehab).download()
+            np.testing.assert_array_equal(m3, orc.mass_matrix_w3(nc, nl, dm0, *chi, nqp=nqp, rehabilitate=rehab))
+            d = hip.LmaOperator(w3, w2).assemble_div(*cv, nqp=nqp, rehabilitate=rehab).download()
+            np.testing.assert_array_equal(d, orc.div_operator(nc, nl, dm0, *chi, nqp=nqp, rehabilitate=rehab))
+    # the W0 path of the general entry point is the W0 assembly
+    m0 = hip.LmaOperator(w0).assemble_mass(*cv).download()
+    np.testing.assert_array_equal(m0, orc.mass_matrix_w0(nc, nl, dm0, *chi, nqp=3))
+
+
+def test_assembled_w2_w3_div_known_answers_and_refusals(hip):
+    kat = json.loads((GOLDEN / "kat.json").read_text())
+    k2, k3, kd = kat["mass_matrix_w2"], kat["mass_matrix_w3"], kat["div_operator"]
+    nl, dx, dy, dz = k2["nlayers"], k2["dx"], k2["dy"], k2["dz"]
+    (w0, dm0), (w2, _), (w3, _) = _box_spaces(hip, 1, 1, nl)
+    chi = np.zeros((3, w0.undf))
+    H = nl + 1
+    for j in range(2):
+        for i in range(2):
+            for kk in range(H):
+                chi[:, (j * 2 + i) * H + kk] = (i * dx, j * dy, kk * dz)
+    cv = [hip.FieldVector(w0).import_field(c) for c in chi]
+    m2 = hip.LmaOperator(w2).assemble_mass(*cv).download()
+    ans = np.array(k2["answer_column_major"]).reshape((6, 6), order="F")
+    norm = (dx + dy + dz) / 3
+    m3 = hip.LmaOperator(w3).assemble_mass(*cv).download()
+    d = hip.LmaOperator(w3, w2).assemble_div(*cv).download()
+    for layer in range(nl):
+        np.testing.assert_allclose(m2[layer] * norm, ans * norm, rtol=0, atol=k2["tol"])
+        np.testing.assert_allclose(m3[layer, 0, 0] / (dx * dy * dz), k3["answer_over_volume"], rtol=0, atol=k3["tol"])
+        np.testing.assert_allclose(d[layer, 0], kd["answer"], rtol=0, atol=kd["tol"])
+    for bad in (lambda: hip.LmaOperator(w2, precision=hip.R32).assemble_mass(*cv),  # FP32: convert afterwards
+                lambda: hip.LmaOperator(w2).assemble_div(*cv),                      # a divergence maps W2 to W3
+                lambda: hip.LmaOperator(w3, w2).assemble_mass(*cv),                 # a mass matrix is square
+                lambda: hip.LmaOperator(w2).assemble_mass(*cv, nqp=5),
+                lambda: hip.LmaOperator(w2).assemble_mass(cv[0], cv[1], hip.FieldVector(w2))):
+        with pytest.raises(hip.LfricError):
+            bad()
+
+
+def test_helmholtz_from_assembled_operators(hip, orc):
+    """H = M3 + tau D diag(M2)^-1 D^T with every operator assembled on the device equals the
+    numpy restatement built from the oracle's element matrices, and FP64 GMRES recovers a known
+    solution of H x = b."""
+    nx, ny, nl, tau = 6, 5, 4, 0.7
+    (w0, dm0), (w2, dm2), (w3, dm3) = _box_spaces(hip, nx, ny, nl)
+    chi = _chi_of_box(nx, ny, nl, np.random.default_rng(9))
+    cv = [hip.FieldVector(w0).import_field(c) for c in chi]
+    M2 = hip.LmaOperator(w2).assemble_mass(*cv)
+    M3 = hip.LmaOperator(w3).assemble_mass(*cv)
+    D = hip.LmaOperator(w3, w2).assemble_div(*cv)
+    H = hip.LinearOperator.helmholtz(M3, D, M2.mm_diagonal(hip.FieldVector(w2)), tau)
+    # dense restatement from the oracle's element matrices and the dofmaps
+    nc = nx * ny
+    m2 = orc.mass_matrix_w2(nc, nl, dm0, *chi)
+    m3 = orc.mass_matrix_w3(nc, nl, dm0, *chi)
+    dd = orc.div_operator(nc, nl, dm0, *chi)
+    n2, n3 = w2.undf, w3.undf
+    diag2 = np.zeros(n2)
+    Dg = np.zeros((n3, n2))
+    M3g = np.zeros((n3, n3))
+    for c in range(nc):
+        for k in range(nl):
+            ik = c * nl + k
+            r2 = dm2[c] - 1 + k
+            r3 = dm3[c, 0] - 1 + k
+            diag2[r2] += np.diag(m2[ik])
+            M3g[r3, r3] += m3[ik, 0, 0]
+            Dg[r3, r2] += dd[ik, 0]
+    Hg = M3g + tau * Dg @ np.diag(1.0 / diag2) @ Dg.T
+    x = np.random.default_rng(10).uniform(-1, 1, n3)
+    y = hip.FieldVector(w3)
+    H.apply(hip.FieldVector(w3).import_field(x), y)
+    np.testing.assert_allclose(y.export_field(), Hg @ x, rtol=1e-12, atol=1e-12 * np.abs(Hg @ x).max())
+    b = hip.FieldVector(w3).import_field(Hg @ x)
+    xs = hip.FieldVector(w3)
+    res = hip.GMRES(H, hip.NullPreconditioner(), 30, 1e-12, 0.0, 200).apply(xs, b)
+    assert res.status == 0
+    np.testing.assert_allclose(xs.export_field(), x, rtol=0, atol=1e-8)

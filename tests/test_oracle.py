@@ -544,3 +544,113 @@ def test_mass_matrix_w0_closed_forms(orc):
             J[..., r, 2] += P[v, r] * lx * ly * sz
     vol = np.linalg.det(J).mean()
     assert abs(m3.sum() - vol) <= 1e-3 * abs(vol)
+
+
+def _kat_box_mesh(k):
+    """The bi-periodic 3x3 mesh of the mass-matrix tests with the coordinates of
+    get_w0_3x3x3_field: each cell's south-west vertex column at (i dx, j dy, k dz).  Cell 1 (the
+    one the reference tests check) is then the box dx x dy x dz in every layer; cells on the
+    periodic seam are not boxes."""
+    nl, dx, dy, dz = k["nlayers"], k["dx"], k["dy"], k["dz"]
+    verts = np.array(json.loads((GOLDEN / "kat.json").read_text())["mass_matrix_w0"]["vertices_of_cell"])
+    dm = np.zeros((9, 8), np.int32)
+    dm[:, :4] = verts * (nl + 1) + 1
+    dm[:, 4:] = verts * (nl + 1) + 2
+    chi = np.zeros((3, 9 * (nl + 1)))
+    for c in range(9):
+        for kk in range(nl + 1):
+            chi[:, dm[c, 0] - 1 + kk] = ((c % 3) * dx, (c // 3) * dy, kk * dz)
+    return dm, chi
+
+
+# lowest-order W2 dofs W, S, E, N, B, T: component, sign of the reference normal, and whether
+# the linear factor is t (1) or 1 - t (0) -- the closed forms below integrate these
+_W2_COMP = np.array([0, 1, 0, 1, 2, 2])
+_W2_SIGN = np.array([1.0, -1.0, 1.0, -1.0, 1.0, 1.0])
+_W2_FAR = np.array([0, 0, 1, 1, 0, 1])
+
+
+def _w2_reference_integrals():
+    """I_ij = integral over the unit cube of v_i v_j (the scalar factors of the basis)."""
+    I = np.zeros((6, 6))
+    for i in range(6):
+        for j in range(6):
+            if _W2_COMP[i] == _W2_COMP[j]:
+                one_d = 1 / 3 if _W2_FAR[i] == _W2_FAR[j] else 1 / 6
+            else:
+                one_d = 1 / 4
+            I[i, j] = _W2_SIGN[i] * _W2_SIGN[j] * one_d
+    return I
+
+
+def test_w2_basis_matches_the_reference_tables(orc):
+    k = json.loads((GOLDEN / "kat.json").read_text())["w2_basis_q3x3x3"]
+    b, d = orc.w2_basis(3)
+    np.testing.assert_allclose(b, np.array(k["basis_column_major"]).reshape(k["shape"], order="F"), rtol=0,
+                               atol=1e-14)
+    np.testing.assert_array_equal(d, np.array(k["div_column_major"]).reshape(k["div_shape"], order="F"))
+
+
+def test_mass_matrix_w2_known_answer_and_closed_forms(orc):
+    k = json.loads((GOLDEN / "kat.json").read_text())["mass_matrix_w2"]
+    dm, chi = _kat_box_mesh(k)
+    mm = orc.mass_matrix_w2(9, k["nlayers"], dm, chi[0], chi[1], chi[2], nqp=3)
+    ans = np.array(k["answer_column_major"]).reshape((6, 6), order="F")
+    norm = (k["dx"] + k["dy"] + k["dz"]) / 3
+    for layer in range(k["nlayers"]):  # cell 1 in every layer
+        np.testing.assert_allclose(mm[layer] * norm, ans * norm, rtol=0, atol=k["tol"])
+    # sheared parallelepiped x = A xi + o: (J b_i).(J b_j) / det = G(c_i, c_j) v_i v_j / det with
+    # G = A^T A; a transposed Jacobian (A A^T) or a dropped 1/det fails it
+    rng = np.random.default_rng(4)
+    A = rng.uniform(-1, 1, (3, 3)) + np.diag([3.0, 2.0, 4.0])
+    O = rng.uniform(-5, 5, 3)
+    X = [O + A @ np.array([_VX[v], _VY[v], _VZ[v]], float) for v in range(8)]
+    dmc, chic = _one_cell(X)
+    G = A.T @ A
+    want = G[_W2_COMP][:, _W2_COMP] * _w2_reference_integrals() / np.linalg.det(A)
+    for nqp in (2, 3, 5):
+        got = orc.mass_matrix_w2(1, 1, dmc, chic[0], chic[1], chic[2], nqp=nqp)[0]
+        np.testing.assert_allclose(got, want, rtol=1e-12, atol=1e-14 * np.abs(want).max())
+    # a general hexahedron: symmetric bit for bit, positive definite
+    Xg = [x + rng.uniform(-0.3, 0.3, 3) for x in X]
+    dmc, chic = _one_cell(Xg)
+    m = orc.mass_matrix_w2(1, 1, dmc, chic[0], chic[1], chic[2], nqp=3)[0]
+    np.testing.assert_array_equal(m, m.T)
+    assert np.all(np.linalg.eigvalsh(m) > 0)
+
+
+def test_mass_matrix_w3_known_answer_and_invariants(orc):
+    k = json.loads((GOLDEN / "kat.json").read_text())["mass_matrix_w3"]
+    dm, chi = _kat_box_mesh(k)
+    vol = k["dx"] * k["dy"] * k["dz"]
+    mm = orc.mass_matrix_w3(9, k["nlayers"], dm, chi[0], chi[1], chi[2], nqp=3)
+    nl = k["nlayers"]
+    np.testing.assert_allclose(mm[:nl, 0, 0] / vol, k["answer_over_volume"], rtol=0, atol=k["tol"])
+    inv = orc.mass_matrix_w3(9, nl, dm, chi[0], chi[1], chi[2], nqp=3, rehabilitate=False)
+    np.testing.assert_allclose(inv[:nl, 0, 0] * vol, 1.0, rtol=1e-14)
+    # a general hexahedron: the W3 entry is the cell volume, which is also the sum of the cell's
+    # W0 mass entries (pinned independently); dj is quadratic per direction, so 2 Gauss points
+    # per direction already integrate it exactly
+    rng = np.random.default_rng(6)
+    X = [np.array([_VX[v] * 2.0, _VY[v] * 1.5, _VZ[v] * 3.0]) + rng.uniform(-0.3, 0.3, 3) for v in range(8)]
+    dmc, chic = _one_cell(X)
+    m0 = orc.mass_matrix_w0(1, 1, dmc, chic[0], chic[1], chic[2], nqp=3)[0]
+    for nqp in (2, 3, 5):
+        m3 = orc.mass_matrix_w3(1, 1, dmc, chic[0], chic[1], chic[2], nqp=nqp)[0, 0, 0]
+        assert abs(m3 - m0.sum()) <= 1e-13 * abs(m3)
+
+
+def test_div_operator_known_answer_and_invariants(orc):
+    k = json.loads((GOLDEN / "kat.json").read_text())["div_operator"]
+    dm, chi = _kat_box_mesh(k)
+    d = orc.div_operator(9, k["nlayers"], dm, chi[0], chi[1], chi[2], nqp=3)
+    for layer in range(k["nlayers"]):
+        np.testing.assert_allclose(d[layer, 0], k["answer"], rtol=0, atol=k["tol"])
+    # without rehabilitation each entry is the divergence sign times the W3 entry without
+    # rehabilitation (the same sums of w / dj), bit for bit, on any cell
+    rng = np.random.default_rng(8)
+    X = [np.array([_VX[v], _VY[v], _VZ[v]], float) + rng.uniform(-0.2, 0.2, 3) for v in range(8)]
+    dmc, chic = _one_cell(X)
+    dn = orc.div_operator(1, 1, dmc, chic[0], chic[1], chic[2], nqp=3, rehabilitate=False)[0, 0]
+    m3n = orc.mass_matrix_w3(1, 1, dmc, chic[0], chic[1], chic[2], nqp=3, rehabilitate=False)[0, 0, 0]
+    np.testing.assert_array_equal(dn, np.array(k["answer"]) * m3n)
