@@ -83,6 +83,10 @@ def lib():
             "orc_mass_matrix_w2": (_I, [_I, _I, _VP, _VP, _VP, _VP, _I, _VP]),
             "orc_mass_matrix_w3": (_I, [_I, _I, _VP, _VP, _VP, _VP, _I, _I, _VP]),
             "orc_div_operator": (_I, [_I, _I, _VP, _VP, _VP, _VP, _I, _I, _VP]),
+            "orc_cma_params": (_I, [_I, _I, _I, _I, _I, _I, _I, _I, _I, _I, _VP]),
+            "orc_cma_dofmaps": (None, [_I, _I, _I, _I, _VP, _VP, _VP]),
+            "orc_tri_solve": (None, [_I, _I, _VP, _VP, _VP, _VP, _VP, _VP]),
+            "orc_tri_solve_r32": (None, [_I, _I, _VP, _VP, _VP, _VP, _VP, _VP]),
             "orc_cg": (_I, [ctypes.POINTER(OrcProblem), _VP, _VP, _D, _D, _I, _I, ctypes.POINTER(OrcResult), _VP]),
             "orc_gmres": (_I, [ctypes.POINTER(OrcProblem), _VP, _VP, _I, _D, _D, _I, _I, ctypes.POINTER(OrcResult), _VP]),
             "orc_bicgstab": (_I, [ctypes.POINTER(OrcProblem), _VP, _VP, _D, _D, _I, _I, ctypes.POINTER(OrcResult), _VP]),
@@ -94,6 +98,17 @@ def lib():
             "orc_prec_only": (_I, [ctypes.POINTER(OrcProblem), _VP, _VP, ctypes.POINTER(OrcResult)]),
             "orc_givens_lsq": (_I, [_I, _VP, _VP, _VP]),
         }
+        for sfx in ("", "_r32"):
+            T = ctypes.c_double if sfx == "" else ctypes.c_float
+            sig.update({
+                "orc_cma_asm" + sfx: (None, [_I, _I, _I, _I, _VP, _VP, _VP, _VP, _VP]),
+                "orc_cma_apply" + sfx: (None, [_I, _VP, _VP, _VP, _VP, _I, _VP, _VP, _I, _VP, _VP]),
+                "orc_cma_appinv" + sfx: (None, [_I, _VP, _VP, _VP, _VP, _I, _VP, _VP]),
+                "orc_cma_mul" + sfx: (None, [_I, _VP, _VP, _VP, _VP, _VP, _VP]),
+                "orc_cma_scaledadd" + sfx: (None, [_I, _VP, _VP, _VP, _VP, _VP, _VP, T, T]),
+                "orc_cma_asm_diag_hmht" + sfx: (None, [_I, _I, _I, _I, _VP, _VP, _VP, _VP, _VP]),
+                "orc_cma_asm_m2v_lumped_inv" + sfx: (None, [_I, _I, _I, _VP, _VP, _VP, _VP]),
+            })
         for name, (res, args) in sig.items():
             f = getattr(L, name)
             f.restype = res
@@ -311,6 +326,115 @@ def div_operator(ncells, nlayers, dofmap, chi1, chi2, chi3, nqp=3, rehabilitate=
                               _p(out)) != 0:
         raise ValueError("div_operator: nqp out of range")
     return out.reshape((6, 1, ncells * nlayers)).transpose(2, 1, 0)
+
+
+# ---- columnwise (CMA) operators --------------------------------------------------------------
+# A CMA matrix is held as a C-order (ncells, nrow, bandwidth) array, which is the reference's
+# Fortran columnwise_matrix(bandwidth, nrow, ncell_2d) byte for byte; p = [alpha, beta, gamma_m,
+# gamma_p, bandwidth, nrow, ncol].
+
+def cma_params(nl, ni_to, nf_to, ni_from, nf_from, custom=None):
+    """Banded-structure parameters of a CMA operator between horizontally discontinuous spaces
+    with ni interior and nf vertical-face dofs per cell; custom = (alpha, beta, gamma_m, gamma_p)."""
+    p = np.zeros(7, np.int32)
+    c = custom or (0, 0, 0, 0)
+    if lib().orc_cma_params(nl, ni_to, nf_to, ni_from, nf_from, int(custom is not None), *c, _p(p)) != 0:
+        raise ValueError("cma_params: alpha and beta are both zero")
+    return p
+
+
+def cma_dofmaps(nl, ndf, ni, nf, map_cell1):
+    """column_banded_dofmap as a Fortran-order (ndf, nl) flat array and indirection_dofmap (nrow)."""
+    m = np.ascontiguousarray(map_cell1, np.int32)
+    banded = np.zeros(ndf * nl, np.int32)
+    ind = np.zeros(nl * (ndf - nf) + nf, np.int32)
+    lib().orc_cma_dofmaps(nl, ndf, ni, nf, _p(m), _p(banded), _p(ind))
+    return banded, ind
+
+
+def _cma_fn(name, r32):
+    return getattr(lib(), name + ("_r32" if r32 else ""))
+
+
+def _cma_t(r32):
+    return np.float32 if r32 else np.float64
+
+
+def cma_asm(ncells, nl, lma, p, banded_to, banded_from, r32=False):
+    """columnwise_op_asm of an LMA operator (ncells*nl, ndf_to, ndf_from)."""
+    T = _cma_t(r32)
+    L = np.ascontiguousarray(flat_matrix(lma), T)
+    out = np.zeros((ncells, p[5], p[4]), T)
+    _cma_fn("orc_cma_asm", r32)(ncells, nl, lma.shape[1], lma.shape[2], _p(L), _p(p), _p(banded_to), _p(banded_from),
+                                _p(out))
+    return out
+
+
+def cma_apply(p, cma, x, lhs, map1, ind_to, map2, ind_from, r32=False):
+    """columnwise_op_app: lhs += A x (returns a new array); map1/map2 are (ncells, ndf) dofmaps."""
+    T = _cma_t(r32)
+    y = np.array(lhs, T)
+    m1, m2 = np.ascontiguousarray(map1, np.int32), np.ascontiguousarray(map2, np.int32)
+    xx, A = np.ascontiguousarray(x, T), np.ascontiguousarray(cma, T)
+    _cma_fn("orc_cma_apply", r32)(A.shape[0], _p(p), _p(A), _p(xx), _p(y), m1.shape[1], _p(m1), _p(ind_to),
+                                  m2.shape[1], _p(m2), _p(ind_from))
+    return y
+
+
+def cma_appinv(p, cma, x, map_, ind, r32=False):
+    """columnwise_op_appinv: the tridiagonal solve of every column."""
+    T = _cma_t(r32)
+    y = np.zeros(len(x), T)
+    m, xx, A = np.ascontiguousarray(map_, np.int32), np.ascontiguousarray(x, T), np.ascontiguousarray(cma, T)
+    _cma_fn("orc_cma_appinv", r32)(A.shape[0], _p(p), _p(A), _p(xx), _p(y), m.shape[1], _p(m), _p(ind))
+    return y
+
+
+def cma_mul(pa, A, pb, B, pc, C, r32=False):
+    """columnwise_op_mul: C + A B (returns a new array)."""
+    T = _cma_t(r32)
+    out = np.array(C, T)
+    Aa, Bb = np.ascontiguousarray(A, T), np.ascontiguousarray(B, T)
+    _cma_fn("orc_cma_mul", r32)(Aa.shape[0], _p(pa), _p(Aa), _p(pb), _p(Bb), _p(pc), _p(out))
+    return out
+
+
+def cma_scaledadd(pa, A, pb, B, pc, alpha, beta, r32=False):
+    """columnwise_op_scaledadd: alpha A + beta B in C's band structure."""
+    T = _cma_t(r32)
+    Aa, Bb = np.ascontiguousarray(A, T), np.ascontiguousarray(B, T)
+    out = np.zeros((Aa.shape[0], pc[5], pc[4]), T)
+    _cma_fn("orc_cma_scaledadd", r32)(Aa.shape[0], _p(pa), _p(Aa), _p(pb), _p(Bb), _p(pc), _p(out), alpha, beta)
+    return out
+
+
+def cma_asm_diag_hmht(ncells, nl, Dh, M2h, p, banded, r32=False):
+    """columnwise_op_asm_diag_hmht from LMA Dh (c3, ndf_w3, ndf_w2h) and M2h (c3, ndf_w2h, ndf_w2h)."""
+    T = _cma_t(r32)
+    D, M = np.ascontiguousarray(flat_matrix(Dh), T), np.ascontiguousarray(flat_matrix(M2h), T)
+    out = np.zeros((ncells, p[5], p[4]), T)
+    _cma_fn("orc_cma_asm_diag_hmht", r32)(ncells, nl, Dh.shape[1], Dh.shape[2], _p(D), _p(M), _p(p), _p(banded),
+                                          _p(out))
+    return out
+
+
+def cma_asm_m2v_lumped_inv(ncells, nl, lma, p, banded, r32=False):
+    """columnwise_op_asm_m2v_lumped_inv of a square LMA operator."""
+    T = _cma_t(r32)
+    L = np.ascontiguousarray(flat_matrix(lma), T)
+    out = np.zeros((ncells, p[5], p[4]), T)
+    _cma_fn("orc_cma_asm_m2v_lumped_inv", r32)(ncells, nl, lma.shape[1], _p(L), _p(p), _p(banded), _p(out))
+    return out
+
+
+def tri_solve(nl, map_, x, t0, tp, tm, r32=False):
+    """tri_solve_code over every W3 column (map: the W3 dofmap, one dof per cell)."""
+    T = _cma_t(r32)
+    m = np.ascontiguousarray(np.asarray(map_).ravel(), np.int32)
+    a = [np.ascontiguousarray(v, T) for v in (x, t0, tp, tm)]
+    y = np.zeros(len(x), T)
+    _cma_fn("orc_tri_solve", r32)(len(m), nl, _p(m), *[_p(v) for v in a], _p(y))
+    return y
 
 
 def make_problem(mesh, A, diag=None):

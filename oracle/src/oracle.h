@@ -41,6 +41,11 @@
  *   orc_div_operator          known answer of compute_div_operator_kernel_mod_test.pf
  *                             (-1, 1, 1, -1, -1, 1); without rehabilitation = divergence sign x
  *                             the W3 entry without rehabilitation, bit for bit
+ *   orc_cma_*                 known answers of the columnwise_op_{asm,app,appinv,mul,scaledadd,
+ *   orc_tri_solve             asm_diag_hmht,asm_m2v_lumped_inv}_kernel_mod_test.pf and
+ *                             sci_tri_solve_kernel_mod_test.pf; dense numpy restatements of the
+ *                             banded products and solves; bit-level comparison with the
+ *                             reference's CMA kernels compiled from source (r_solver = real32)
  *   orc_invert_local          known answers of invert_local_operator_kernel_mod_test.pf (1/5) and
  *                             matrix_invert_mod_test.pf (a a^-1 = I); numpy.linalg.inv on
  *                             nonsymmetric matrices; bit-level comparison with the reference's
@@ -129,6 +134,32 @@ int orc_mass_matrix_w3(int ncells, int nlayers, const int *map, const double *ch
                        const double *chi3, int nqp, int rehabilitate, double *mm);
 int orc_div_operator(int ncells, int nlayers, const int *map, const double *chi1, const double *chi2,
                      const double *chi3, int nqp, int rehabilitate, double *div);
+
+/* ---- columnwise (CMA) operators and the vertical tridiagonal solve (orc_cma.c) ---------- *
+ * p[7] = {alpha, beta, gamma_m, gamma_p, bandwidth, nrow, ncol}; matrices (bandwidth, nrow,
+ * ncells) in Fortran order; _r32 variants are the r_solver = real32 instances. */
+int orc_cma_params(int nl, int ni_to, int nf_to, int ni_from, int nf_from, int custom, int alpha, int beta,
+                   int gamma_m, int gamma_p, int *p);
+void orc_cma_dofmaps(int nl, int ndf, int ni, int nf, const int *map_cell1, int *banded, int *indirection);
+#define ORC_CMA_DECLS(T, SFX)                                                                                  \
+  void orc_cma_asm##SFX(int ncells, int nl, int ndf_to, int ndf_from, const T *lma, const int *p,              \
+                        const int *banded_to, const int *banded_from, T *cma);                                  \
+  void orc_cma_apply##SFX(int ncells, const int *p, const T *cma, const T *x, T *lhs, int ndf1, const int *map1, \
+                          const int *ind_to, int ndf2, const int *map2, const int *ind_from);                   \
+  void orc_cma_appinv##SFX(int ncells, const int *p, const T *cma, const T *x, T *lhs, int ndf, const int *map,  \
+                           const int *ind);                                                                     \
+  void orc_cma_mul##SFX(int ncells, const int *pa, const T *A, const int *pb, const T *B, const int *pc, T *C);   \
+  void orc_cma_scaledadd##SFX(int ncells, const int *pa, const T *A, const int *pb, const T *B, const int *pc,   \
+                              T *C, T alpha, T beta);                                                           \
+  void orc_cma_asm_diag_hmht##SFX(int ncells, int nl, int ndf_w3, int ndf_w2h, const T *Dh, const T *M2h,      \
+                                  const int *p, const int *banded, T *cma);                                     \
+  void orc_cma_asm_m2v_lumped_inv##SFX(int ncells, int nl, int ndf, const T *lma, const int *p,                 \
+                                       const int *banded, T *cma);                                              \
+  void orc_tri_solve##SFX(int ncells, int nl, const int *map, const T *x, const T *t0, const T *tp, const T *tm, \
+                          T *y);
+ORC_CMA_DECLS(double, )
+ORC_CMA_DECLS(float, _r32)
+#undef ORC_CMA_DECLS
 
 /* ---- Krylov solvers ------------------------------------------------------------------- */
 typedef struct orc_problem {

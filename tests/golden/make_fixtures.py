@@ -22,7 +22,9 @@ Everything written is data (inputs and expected outputs); no reference source te
                       (infrastructure/unit-test/utilities/matrix_invert_mod_test.pf:25,39-46) and
                       the W0, W2 and W3 mass-matrix and weak-divergence tests
                       (components/science/unit-test/kernel/fem/compute_mass_matrix_kernel_w{0,2,3}_mod_test.pf,
-                      compute_div_operator_kernel_mod_test.pf), the W2 basis tables at the
+                      compute_div_operator_kernel_mod_test.pf), the tridiagonal-solve test
+                      (components/science/unit-test/kernel/solver/sci_tri_solve_kernel_mod_test.pf),
+                      the W2 basis tables at the
                       3x3x3 Gauss points (components/science/unit-test/support/get_unit_test_q3x3x3_basis_mod.f90)
                       and the C3 mesh-generator summary counts
                       (rose-stem/app/check_summary/file/summary_cubedsphere_c3_ugrid.kgo.txt:8-10).
@@ -163,6 +165,12 @@ def main(ref):
     w2_basis = basis_table("basis_w2", "end subroutine get_w2_q3x3x3_basis")
     w2_div = basis_table("diff_basis_w2", "end subroutine get_w2_q3x3x3_diff_basis")
     assert len(w2_basis) == 3 * 6 * 27 and len(w2_div) == 6 * 27
+    # sci_tri_solve_kernel_mod_test.pf:29-53: one 10-level W3 column, constant diagonals
+    ts = (ref / "components/science/unit-test/kernel/solver/sci_tri_solve_kernel_mod_test.pf").read_text()
+    ts_x = [float(v) for v in re.findall(r"-?\d+", re.search(r"x\s*=\s*real\(\(/(.*?)/\)", ts, re.S).group(1))]
+    ts_ans = [float(v) for v in re.findall(r"(-?[0-9]+\.[0-9]+)_r_def", re.search(r"answer\s*=(.*?)/\)", ts, re.S).group(1))]
+    ts_diag = {n: float(re.search(n + r"\s*=\s*(-?[0-9.]+)_r_def", ts).group(1)) for n in ("A0", "AP", "AM")}
+    assert len(ts_x) == 10 and len(ts_ans) == 10
     summary = (ref / "rose-stem/app/check_summary/file/summary_cubedsphere_c3_ugrid.kgo.txt").read_text()
     counts = {k: int(re.search(k + r":\s*(\d+)", summary).group(1)) for k in ("Nodes", "Edges", "Faces")}
     kat = {
@@ -211,6 +219,8 @@ def main(ref):
                          "answer": div_vals, "tol": 1e-6},
         "w2_basis_q3x3x3": {"basis_column_major": w2_basis, "shape": [3, 6, 9, 3],
                             "div_column_major": w2_div, "div_shape": [1, 6, 9, 3], "digits": 15},
+        "tri_solve": {"x": ts_x, "A0": ts_diag["A0"], "AP": ts_diag["AP"], "AM": ts_diag["AM"],
+                      "answer": ts_ans, "digits": 15},
         "c3_summary": counts,
     }
     (HERE / "kat.json").write_text(json.dumps(kat, indent=1) + "\n")
