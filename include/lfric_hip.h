@@ -48,6 +48,7 @@ typedef struct lfric_op lfric_op;     /* LMA operator local_stencil (device) */
 typedef struct lfric_vec lfric_vec;   /* field vector data (device) */
 typedef struct lfric_prec lfric_prec; /* preconditioner */
 typedef struct lfric_linop lfric_linop; /* abstract linear operator (LMA or composite) */
+typedef struct lfric_cma lfric_cma;   /* columnwise (CMA) banded operator (device) */
 
 /* ---- library ------------------------------------------------------------------------ */
 /* Select the device and create the library stream (default priority). */
@@ -338,6 +339,51 @@ int lfric_prec_only_solve(const lfric_linop *A, const lfric_prec *P, lfric_vec *
  * diagonal in the reference's factory). */
 int lfric_solve(int method, const lfric_linop *A, const lfric_prec *P, lfric_vec *x, const lfric_vec *b,
                 const lfric_solver_opts *opts, lfric_solver_result *res);
+
+/* ---- columnwise (CMA) operators -----------------------------------------------------------
+ * columnwise_operator_type (infrastructure/source/operator/columnwise_operator_mod.f90) between
+ * horizontally discontinuous lowest-order spaces (W3, Wtheta-like: bottom faces, interior, top
+ * faces; every column owned by one cell): one banded matrix per cell column with the
+ * reference's alpha, beta, gamma_m, gamma_p, bandwidth, nrow, ncol (divided by gcd(alpha,
+ * beta)).  Kernels run one thread per column and equal the reference's CMA kernels bit for bit.
+ * params[7] = {alpha, beta, gamma_m, gamma_p, bandwidth, nrow, ncol}. */
+int lfric_cma_create(const lfric_fs *to, const lfric_fs *from, int precision, lfric_cma **A);
+/* columnwise_operator_initialiser_custom: the caller's alpha, beta, gamma_m, gamma_p. */
+int lfric_cma_create_custom(const lfric_fs *to, const lfric_fs *from, int precision, int alpha, int beta, int gamma_m,
+                            int gamma_p, lfric_cma **A);
+int lfric_cma_info(const lfric_cma *A, int *params);
+/* columnwise_matrix(bandwidth, nrow, ncell_2d) in the reference layout (Fortran order, FP64). */
+int lfric_cma_upload(lfric_cma *A, const double *ref);
+int lfric_cma_download(const lfric_cma *A, double *ref);
+/* columnwise_op_asm_kernel_code (components/science/source/kernel/columnwise_operators/
+ * sci_columnwise_op_asm_kernel_mod.F90:96-140): A from an LMA operator on the same spaces. */
+int lfric_cma_assemble(lfric_cma *A, const lfric_op *lma);
+/* columnwise_op_asm_diag_hmht_kernel_code (sci_columnwise_op_asm_diag_hmht_kernel_mod.F90:95-135):
+ * the column band of Dh diag(M2h)^-1 Dh^T (Dh: W2h -> the operator's space, M2h on W2h). */
+int lfric_cma_assemble_diag_hmht(lfric_cma *A, const lfric_op *Dh, const lfric_op *M2h);
+/* columnwise_op_asm_m2v_lumped_inv_krnl_code (sci_columnwise_op_asm_m2v_lumped_inv_krnl_mod.F90:
+ * 85-113): band 1 = reciprocal of the assembled diagonal of a square LMA operator. */
+int lfric_cma_assemble_m2v_lumped_inv(lfric_cma *A, const lfric_op *M);
+/* columnwise_op_app_kernel_code (sci_columnwise_op_app_kernel_mod.F90:107-131): y = A x, or
+ * y += A x with accumulate != 0 (the kernel alone; the reference's invoke zeroes y first). */
+int lfric_cma_apply(const lfric_cma *A, const lfric_vec *x, lfric_vec *y, int accumulate);
+/* columnwise_op_appinv_kernel_code (sci_columnwise_op_appinv_kernel_mod.F90:107-146): y = A^-1 x
+ * for a square tridiagonal (bandwidth 3) A, Thomas algorithm per column. */
+int lfric_cma_apply_inverse(const lfric_cma *A, const lfric_vec *x, lfric_vec *y);
+/* columnwise_op_mul_kernel_code (sci_columnwise_op_mul_kernel_mod.F90:128-160): C += A B. */
+int lfric_cma_multiply(const lfric_cma *A, const lfric_cma *B, lfric_cma *C);
+/* columnwise_op_scaledadd_kernel_code (sci_columnwise_op_scaledadd_kernel_mod.F90:132-166):
+ * C = alpha A + beta B (alpha, beta rounded to the operators' precision). */
+int lfric_cma_scaled_add(const lfric_cma *A, const lfric_cma *B, lfric_cma *C, double alpha, double beta);
+void lfric_cma_destroy(lfric_cma *A);
+/* A preconditioner whose apply is lfric_cma_apply_inverse (the vertical line solve of a
+ * tridiagonal columnwise operator); A must outlive it. */
+int lfric_prec_create_cma_inverse(const lfric_cma *A, lfric_prec **P);
+/* tri_solve_code (components/science/source/kernel/solver/sci_tri_solve_kernel_mod.f90:57-130):
+ * y = T^-1 x on every W3 column, T with diagonal t0, super-diagonal tp and sub-diagonal tm (W3
+ * fields); work is a W3 field of the same precision used as scratch. */
+int lfric_tri_solve(const lfric_vec *x, const lfric_vec *t0, const lfric_vec *tp, const lfric_vec *tm, lfric_vec *y,
+                    lfric_vec *work);
 
 /* ---- batched solves (no reference counterpart: many independent mass-matrix solves at
  *      once) ------------------------------------------------------------------------------

@@ -17,10 +17,14 @@ Reference interfaces mirrored (paths relative to the lfric_core tree):
   BiCGStab, Jacobi,      sci_iterative_solver_mod.f90
   Chebyshev, PreconditionOnly
   solve                  the method factory of sci_mass_matrix_solver_alg_mod.x90:164-181
+  ColumnwiseOperator     infrastructure/source/operator/columnwise_operator_mod.f90 and the
+                         components/science/source/kernel/columnwise_operators/ kernels
+  tri_solve              components/science/source/kernel/solver/sci_tri_solve_kernel_mod.f90
 """
 from ._lib import (LfricError, NotConverged, lib, init, synchronize, timer_start, timer_stop,
                    device_bytes, set_apply_form, R64, R32, BF16, F16, W0, W2, W3)
 from .fem import FunctionSpace, FieldVector, LmaOperator
+from .cma import ColumnwiseOperator, ColumnwiseInversePreconditioner, tri_solve
 from .solvers import (NullPreconditioner, DiagonalPreconditioner, ColumnJacobiPreconditioner,
                       LinearOperator, ConjugateGradient, MixedPrecisionCG, GMRES, MixedPrecisionGMRES,
                       FGMRES, GCR, BiCGStab, Jacobi, Chebyshev, PreconditionOnly, solve, METHODS,
@@ -32,5 +36,5 @@ __all__ = [
     "LmaOperator", "NullPreconditioner", "DiagonalPreconditioner", "ColumnJacobiPreconditioner",
     "LinearOperator", "ConjugateGradient", "MixedPrecisionCG", "GMRES", "MixedPrecisionGMRES",
     "FGMRES", "GCR", "BiCGStab", "Jacobi", "Chebyshev", "PreconditionOnly", "solve", "METHODS",
-    "SolverResult", "SolverOptions",
+    "SolverResult", "SolverOptions", "ColumnwiseOperator", "ColumnwiseInversePreconditioner", "tri_solve",
 ]

@@ -107,6 +107,21 @@ SIGNATURES = {
     "lfric_linop_apply": (_I, [_VP, _VP, _VP]),
     "lfric_linop_destroy": (None, [_VP]),
     "lfric_prec_create_column_jacobi": (_I, [_VP, _I, ctypes.POINTER(_VP)]),
+    "lfric_cma_create": (_I, [_VP, _VP, _I, ctypes.POINTER(_VP)]),
+    "lfric_cma_create_custom": (_I, [_VP, _VP, _I, _I, _I, _I, _I, ctypes.POINTER(_VP)]),
+    "lfric_cma_info": (_I, [_VP, _VP]),
+    "lfric_cma_upload": (_I, [_VP, _VP]),
+    "lfric_cma_download": (_I, [_VP, _VP]),
+    "lfric_cma_assemble": (_I, [_VP, _VP]),
+    "lfric_cma_assemble_diag_hmht": (_I, [_VP, _VP, _VP]),
+    "lfric_cma_assemble_m2v_lumped_inv": (_I, [_VP, _VP]),
+    "lfric_cma_apply": (_I, [_VP, _VP, _VP, _I]),
+    "lfric_cma_apply_inverse": (_I, [_VP, _VP, _VP]),
+    "lfric_cma_multiply": (_I, [_VP, _VP, _VP]),
+    "lfric_cma_scaled_add": (_I, [_VP, _VP, _VP, _D, _D]),
+    "lfric_cma_destroy": (None, [_VP]),
+    "lfric_prec_create_cma_inverse": (_I, [_VP, ctypes.POINTER(_VP)]),
+    "lfric_tri_solve": (_I, [_VP, _VP, _VP, _VP, _VP, _VP]),
     "lfric_gmres_solve": (_I, [_VP, _VP, _VP, _VP, ctypes.POINTER(SolverOpts), ctypes.POINTER(SolverRes)]),
     "lfric_ir_gmres_solve": (_I, [_VP, _VP, _VP, _VP, _VP, ctypes.POINTER(SolverOpts), ctypes.POINTER(SolverRes)]),
     "lfric_solver_opts_default": (None, [ctypes.POINTER(SolverOpts)]),

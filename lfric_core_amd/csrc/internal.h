@@ -130,9 +130,29 @@ struct lfric_op {
   size_t bytes = 0;
 };
 
+// Columnwise (CMA) operator between horizontally discontinuous spaces: one banded matrix per
+// cell column, entry (row i, band b) of column position pos at d[(i*bw + b)*ncols + pos] so that
+// the threads of a wavefront (one column each) read consecutive addresses.
+struct lfric_cma {
+  std::shared_ptr<lfric::FunctionSpace> to, from;
+  int precision = LFRIC_R32;
+  int alpha = 0, beta = 0, gamma_m = 0, gamma_p = 0, bw = 0, nrow = 0, ncol = 0;
+  int ni_to = 0, nf_to = 0, ni_from = 0, nf_from = 0;
+  int ncols = 0;                       // cell columns (positions of the to space)
+  void *d = nullptr;
+  size_t bytes = 0;
+  int *d_ind_to = nullptr, *d_ind_from = nullptr;        // row/column -> offset from the column base
+  int *d_band_to = nullptr, *d_band_from = nullptr;      // [k][df] banded row/column (1-based)
+  size_t ind_to_bytes = 0, ind_from_bytes = 0, band_to_bytes = 0, band_from_bytes = 0;
+  void *work = nullptr;                // [nrow][ncols] forward-sweep factors of the tridiagonal solve
+  size_t work_bytes = 0;
+  ~lfric_cma();
+};
+
 struct lfric_prec {
-  int kind = 0;  // 0 identity, 1 diagonal, 2 vertical-column block-Jacobi
+  int kind = 0;  // 0 identity, 1 diagonal, 2 vertical-column block-Jacobi, 3 columnwise tridiagonal inverse
   const lfric_vec *diag = nullptr;
+  const lfric_cma *cma = nullptr;
   // column block-Jacobi: columns (base dof, height) and per-dof Thomas factors
   int storage = LFRIC_R32;
   long long undf = 0;

@@ -46,6 +46,8 @@ int launch_lma_apply_mrhs(const lfric_op *op, const void *x, void *y, int nrhs, 
 int launch_op_invert_local(const lfric_op *A, lfric_op *Ainv);
 int launch_build_sym(lfric_op *op, int levels_per_thread);
 int launch_assemble_mass_w0(lfric_op *op, const double *chi1, const double *chi2, const double *chi3, int nqp);
+// tridiagonal solve of every column of a square columnwise operator (columnwise_op_appinv)
+int launch_cma_appinv(const lfric_cma *A, const void *x, void *y);
 // kind 0: W3 mass, 1: W2 mass, 2: weak divergence W2 -> W3; chi on the W0 space chi_fs
 int launch_assemble_fem(lfric_op *op, int kind, const FunctionSpace &chi_fs, const double *chi1, const double *chi2,
                         const double *chi3, int nqp, int rehab);

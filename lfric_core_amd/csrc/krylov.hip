@@ -285,6 +285,10 @@ int prec_apply_any(const lfric_prec *P, int prec, const void *x, void *y, long l
     return launch_divide(prec, y, x, P->diag->d, n);
   }
   LF_CHECK(P->undf == n, LFRIC_E_ARG, "column preconditioner does not match vector");
+  if (P->kind == 3) {
+    LF_CHECK(P->cma->precision == prec, LFRIC_E_ARG, "columnwise preconditioner precision differs from the vector");
+    return launch_cma_appinv(P->cma, x, y);
+  }
   const int grid = grid_for(P->ncols, 64);
 #define CT_APPLY(ST)                                                                                           \
   if (P->storage == ST) {                                                                                      \
