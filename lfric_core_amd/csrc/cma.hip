@@ -33,8 +33,12 @@ CmaP params_of(const lfric_cma &A) { return {A.alpha, A.beta, A.gamma_m, A.gamma
 // floor((alpha i + gamma_m) / beta).  The reference divides in r_solver reals; for the small
 // integers of lowest-order operators that equals exact integer division (beta > 0).
 __host__ __device__ inline int floor_div(int a, int b) { return a >= 0 ? a / b : -((-a + b - 1) / b); }
-__host__ __device__ inline int jminus(const CmaP &p, int i) { return -floor_div(-(p.alpha * i - p.gp), p.beta); }
-__host__ __device__ inline int jplus(const CmaP &p, int i) { return floor_div(p.alpha * i + p.gm, p.beta); }
+__host__ __device__ inline int jminus(const CmaP &p, int i) {
+  return p.beta == 1 ? p.alpha * i - p.gp : -floor_div(-(p.alpha * i - p.gp), p.beta);
+}
+__host__ __device__ inline int jplus(const CmaP &p, int i) {
+  return p.beta == 1 ? p.alpha * i + p.gm : floor_div(p.alpha * i + p.gm, p.beta);
+}
 
 __device__ inline size_t cidx(const CmaP &p, int nc, int i1, int b1, int pos) {
   return (static_cast<size_t>(i1 - 1) * p.bw + (b1 - 1)) * nc + pos;
@@ -69,11 +73,18 @@ __global__ __launch_bounds__(kBlock) void k_cma_asm(T *__restrict__ C, CmaP p, i
   }
 }
 
+// Rows are walked in chunks of kRowChunk: the loads of a whole chunk (bands, right-hand side,
+// indirection) are issued before its arithmetic, so a column's thread waits for memory once per
+// chunk rather than once per row.  With one thread per column (55,296 at C96) the chip holds
+// less than one wavefront per SIMD, and this load-level parallelism is what hides HBM latency.
+constexpr int kRowChunk = 8;
+
 // columnwise_op_app_kernel_code (sci_columnwise_op_app_kernel_mod.F90:107-131): row i of the
 // column adds A(j - j_minus + 1, i) x(mu_j) for j from max(1, j_minus) to min(ncol, j_plus)
 // onto lhs(mu_i), mu = column base + indirection.  With accumulate == 0 the sum starts from 0
-// (setval_c(lhs, 0) before the kernel), else from lhs.
-template <typename T>
+// (setval_c(lhs, 0) before the kernel), else from lhs.  BW is the bandwidth when it is a
+// compile-time 1, 2 or 3 (0: any).
+template <typename T, int BW>
 __global__ __launch_bounds__(kBlock) void k_cma_apply(const T *__restrict__ C, CmaP p, int nc,
                                                       const T *__restrict__ x, T *__restrict__ y,
                                                       const int *__restrict__ map_to, int ndf_to,
@@ -82,21 +93,59 @@ __global__ __launch_bounds__(kBlock) void k_cma_apply(const T *__restrict__ C, C
                                                       const int *__restrict__ ind_from, int accumulate) {
   const int pos = blockIdx.x * kBlock + threadIdx.x;
   if (pos >= nc) return;
+  const int bw = BW > 0 ? BW : p.bw;
   const size_t bt = static_cast<size_t>(map_to[static_cast<size_t>(pos) * ndf_to]);
   const size_t bf = static_cast<size_t>(map_from[static_cast<size_t>(pos) * ndf_from]);
-  for (int i = 1; i <= p.nrow; ++i) {
-    const size_t mu_i = bt + ind_to[i - 1];
-    const int jm = jminus(p, i), jp = jplus(p, i);
-    T acc = accumulate ? y[mu_i] : T(0);
-    const int j0 = max(1, jm), j1 = min(p.ncol, jp);
-    for (int j = j0; j <= j1; ++j) acc = acc + C[cidx(p, nc, i, j - jm + 1, pos)] * x[bf + ind_from[j - 1]];
-    y[mu_i] = acc;
+  if constexpr (BW > 0) {
+    for (int i0 = 1; i0 <= p.nrow; i0 += kRowChunk) {
+      T a[kRowChunk][BW], xv[kRowChunk][BW], acc[kRowChunk];
+      int jm[kRowChunk], j0[kRowChunk], j1[kRowChunk];
+      size_t mu[kRowChunk];
+#pragma unroll
+      for (int u = 0; u < kRowChunk; ++u) {
+        const int i = i0 + u;
+        if (i > p.nrow) continue;
+        mu[u] = bt + ind_to[i - 1];
+        jm[u] = jminus(p, i);
+        j0[u] = max(1, jm[u]);
+        j1[u] = min(p.ncol, jplus(p, i));
+        acc[u] = accumulate ? y[mu[u]] : T(0);
+#pragma unroll
+        for (int b = 0; b < BW; ++b) {
+          const int j = jm[u] + b;
+          const bool in = j >= j0[u] && j <= j1[u];
+          a[u][b] = in ? C[cidx(p, nc, i, b + 1, pos)] : T(0);
+          xv[u][b] = in ? x[bf + ind_from[j - 1]] : T(0);
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < kRowChunk; ++u) {
+        const int i = i0 + u;
+        if (i > p.nrow) continue;
+#pragma unroll
+        for (int b = 0; b < BW; ++b) {
+          const int j = jm[u] + b;
+          if (j >= j0[u] && j <= j1[u]) acc[u] = acc[u] + a[u][b] * xv[u][b];
+        }
+        y[mu[u]] = acc[u];
+      }
+    }
+  } else {
+    for (int i = 1; i <= p.nrow; ++i) {
+      const size_t mu_i = bt + ind_to[i - 1];
+      const int jm = jminus(p, i), jp = jplus(p, i);
+      T acc = accumulate ? y[mu_i] : T(0);
+      for (int j = max(1, jm); j <= min(p.ncol, jp); ++j) acc = acc + C[cidx(p, nc, i, j - jm + 1, pos)] * x[bf + ind_from[j - 1]];
+      y[mu_i] = acc;
+    }
   }
+  (void)bw;
 }
 
 // columnwise_op_appinv_kernel_code (sci_columnwise_op_appinv_kernel_mod.F90:107-146): Thomas
 // algorithm on the tridiagonal column, bands (1, 2, 3) = (sub, diagonal, super).  The forward
 // sweep keeps c' in the [nrow][ncols] work array and d' in lhs; the backward sweep finishes lhs.
+// Both sweeps load a chunk of rows ahead of its arithmetic.
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_cma_appinv(const T *__restrict__ C, CmaP p, int nc,
                                                        const T *__restrict__ x, T *__restrict__ y,
@@ -107,31 +156,270 @@ __global__ __launch_bounds__(kBlock) void k_cma_appinv(const T *__restrict__ C, 
   const size_t base = static_cast<size_t>(map[static_cast<size_t>(pos) * ndf]);
   const int nrow = p.nrow;
   T cprev = T(0), dprev = T(0);
-  for (int i = 1; i <= nrow; ++i) {
-    const size_t mu = base + ind[i - 1];
-    const T a1 = C[cidx(p, nc, i, 1, pos)], a2 = C[cidx(p, nc, i, 2, pos)], a3 = C[cidx(p, nc, i, 3, pos)];
-    T inv, cp, dp;
-    if (i == 1) {
-      inv = T(1) / a2;
-      cp = inv * a3;
-      dp = inv * x[mu];
-    } else {
-      inv = T(1) / (a2 - a1 * cprev);
-      cp = i < nrow ? inv * a3 : T(0);
-      dp = inv * (x[mu] - a1 * dprev);
+  for (int i0 = 1; i0 <= nrow; i0 += kRowChunk) {
+    T a1[kRowChunk], a2[kRowChunk], a3[kRowChunk], xv[kRowChunk];
+    size_t mu[kRowChunk];
+#pragma unroll
+    for (int u = 0; u < kRowChunk; ++u) {
+      const int i = i0 + u;
+      if (i > nrow) continue;
+      mu[u] = base + ind[i - 1];
+      a1[u] = C[cidx(p, nc, i, 1, pos)];
+      a2[u] = C[cidx(p, nc, i, 2, pos)];
+      a3[u] = C[cidx(p, nc, i, 3, pos)];
+      xv[u] = x[mu[u]];
     }
-    cw[static_cast<size_t>(i - 1) * nc + pos] = cp;
-    y[mu] = dp;
-    cprev = cp;
-    dprev = dp;
+#pragma unroll
+    for (int u = 0; u < kRowChunk; ++u) {
+      const int i = i0 + u;
+      if (i > nrow) continue;
+      T inv, cp, dp;
+      if (i == 1) {
+        inv = T(1) / a2[u];
+        cp = inv * a3[u];
+        dp = inv * xv[u];
+      } else {
+        inv = T(1) / (a2[u] - a1[u] * cprev);
+        cp = i < nrow ? inv * a3[u] : T(0);
+        dp = inv * (xv[u] - a1[u] * dprev);
+      }
+      cw[static_cast<size_t>(i - 1) * nc + pos] = cp;
+      y[mu[u]] = dp;
+      cprev = cp;
+      dprev = dp;
+    }
   }
   T dnext = dprev;  // d'(nrow) is final
-  for (int i = nrow - 1; i >= 1; --i) {
-    const size_t mu = base + ind[i - 1];
-    const T d = y[mu] - cw[static_cast<size_t>(i - 1) * nc + pos] * dnext;
-    y[mu] = d;
-    dnext = d;
+  for (int i0 = nrow - 1; i0 >= 1; i0 -= kRowChunk) {
+    T cv[kRowChunk], dv[kRowChunk];
+    size_t mu[kRowChunk];
+#pragma unroll
+    for (int u = 0; u < kRowChunk; ++u) {
+      const int i = i0 - u;
+      if (i < 1) continue;
+      mu[u] = base + ind[i - 1];
+      cv[u] = cw[static_cast<size_t>(i - 1) * nc + pos];
+      dv[u] = y[mu[u]];
+    }
+#pragma unroll
+    for (int u = 0; u < kRowChunk; ++u) {
+      const int i = i0 - u;
+      if (i < 1) continue;
+      const T d = dv[u] - cv[u] * dnext;
+      y[mu[u]] = d;
+      dnext = d;
+    }
   }
+}
+
+// LDS-staged forms for columns of up to kMaxStagedRows rows.  A block is one wavefront of 64
+// columns.  The fields are column-contiguous (a column's dofs are consecutive), so reading them
+// one lane per column touches 64 cache lines per instruction; instead the block copies its 64
+// columns of x into LDS with lanes running along each column (64 consecutive values per
+// instruction), the lanes then work one column each out of LDS, and the result goes back to
+// HBM the same way.  Each column's slice is padded to an odd stride S so that lane c's row r,
+// at c*S + r, falls in a different bank for every lane.  The arithmetic is unchanged.
+constexpr int kColBlock = 64;
+constexpr int kMaxStagedRows = 127;
+// dynamic LDS one block may use: the device's per-block limit, read once
+size_t max_block_lds() {
+  static size_t cap = [] {
+    int dev = 0, v = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess || v <= 0)
+      return static_cast<size_t>(64 * 1024);
+    return static_cast<size_t>(v);
+  }();
+  return cap;
+}
+
+__host__ __device__ inline int staged_stride(int n) { return n | 1; }
+
+// Each lane loads the base dof of one column and the offsets of rows lane and lane + 64 once;
+// the bases are then broadcast by lane shuffles, and the loads of kStageBatch columns are all
+// issued before any of them is written to LDS, so a block waits for memory 64 / kStageBatch
+// times.  n <= 2 * kColBlock.
+constexpr int kStageBatch = 16;
+
+template <typename T>
+__device__ inline void stage_columns_in(T *__restrict__ xs, int S, const T *__restrict__ x,
+                                        const int *__restrict__ map, int ndf, const int *__restrict__ ind,
+                                        int n, int pos0, int nc, int cfirst = 0, int clast = kColBlock) {
+  const int lane = threadIdx.x & (kColBlock - 1);
+  const int mybase = pos0 + lane < nc ? map[static_cast<size_t>(pos0 + lane) * ndf] : 0;
+  const int r0 = lane, r1 = lane + kColBlock;
+  const int o0 = r0 < n ? ind[r0] : 0, o1 = r1 < n ? ind[r1] : 0;
+  const int ncol = min(kColBlock, nc - pos0);
+  for (int c0 = cfirst; c0 < clast; c0 += kStageBatch) {
+    T v0[kStageBatch], v1[kStageBatch];
+#pragma unroll
+    for (int u = 0; u < kStageBatch; ++u) {
+      const int base = __shfl(mybase, c0 + u, kColBlock);
+      const bool ok = c0 + u < ncol;
+      v0[u] = ok && r0 < n ? x[static_cast<size_t>(base) + o0] : T(0);
+      v1[u] = ok && r1 < n ? x[static_cast<size_t>(base) + o1] : T(0);
+    }
+#pragma unroll
+    for (int u = 0; u < kStageBatch; ++u) {
+      if (r0 < n) xs[(c0 + u) * S + r0] = v0[u];
+      if (r1 < n) xs[(c0 + u) * S + r1] = v1[u];
+    }
+  }
+}
+
+template <typename T>
+__device__ inline void stage_columns_out(const T *__restrict__ xs, int S, T *__restrict__ y,
+                                         const int *__restrict__ map, int ndf, const int *__restrict__ ind,
+                                         int n, int pos0, int nc, int cfirst = 0, int clast = kColBlock) {
+  const int lane = threadIdx.x & (kColBlock - 1);
+  const int mybase = pos0 + lane < nc ? map[static_cast<size_t>(pos0 + lane) * ndf] : 0;
+  const int r0 = lane, r1 = lane + kColBlock;
+  const int o0 = r0 < n ? ind[r0] : 0, o1 = r1 < n ? ind[r1] : 0;
+  const int ncol = min(kColBlock, nc - pos0);
+#pragma unroll 16
+  for (int c = cfirst; c < clast; ++c) {
+    const int base = __shfl(mybase, c, kColBlock);
+    if (c < ncol) {
+      if (r0 < n) y[static_cast<size_t>(base) + o0] = xs[c * S + r0];
+      if (r1 < n) y[static_cast<size_t>(base) + o1] = xs[c * S + r1];
+    }
+  }
+}
+
+// k_cma_appinv with x and d' in LDS; c' in LDS too when CW_LDS (else in the coalesced work
+// array).  The forward sweep loads CH rows of bands at a time, so 3*CH loads are in flight per
+// lane.
+template <typename T, int CH, bool CW_LDS>
+__global__ __launch_bounds__(kColBlock) void k_cma_appinv_lds(const T *__restrict__ C, CmaP p, int nc,
+                                                              const T *__restrict__ x, T *__restrict__ y,
+                                                              const int *__restrict__ map, int ndf,
+                                                              const int *__restrict__ ind, T *__restrict__ cw) {
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  T *xs = reinterpret_cast<T *>(lds_raw);
+  const int nrow = p.nrow, S = staged_stride(nrow);
+  T *cs = xs + kColBlock * S;
+  const int pos0 = blockIdx.x * kColBlock;
+  stage_columns_in(xs, S, x, map, ndf, ind, nrow, pos0, nc);
+  __syncthreads();
+  const int pos = pos0 + threadIdx.x;
+  if (pos < nc) {
+    T *xc = xs + threadIdx.x * S, *cc = cs + threadIdx.x * S;
+    auto cw_at = [&](int i) -> T & {
+      if constexpr (CW_LDS) return cc[i - 1];
+      else return cw[static_cast<size_t>(i - 1) * nc + pos];
+    };
+    T cprev = T(0), dprev = T(0);
+    for (int i0 = 1; i0 <= nrow; i0 += CH) {
+      T a1[CH], a2[CH], a3[CH];
+#pragma unroll
+      for (int u = 0; u < CH; ++u) {
+        const int i = i0 + u;
+        if (i > nrow) continue;
+        a1[u] = C[cidx(p, nc, i, 1, pos)];
+        a2[u] = C[cidx(p, nc, i, 2, pos)];
+        a3[u] = C[cidx(p, nc, i, 3, pos)];
+      }
+#pragma unroll
+      for (int u = 0; u < CH; ++u) {
+        const int i = i0 + u;
+        if (i > nrow) continue;
+        T inv, cp, dp;
+        if (i == 1) {
+          inv = T(1) / a2[u];
+          cp = inv * a3[u];
+          dp = inv * xc[i - 1];
+        } else {
+          inv = T(1) / (a2[u] - a1[u] * cprev);
+          cp = i < nrow ? inv * a3[u] : T(0);
+          dp = inv * (xc[i - 1] - a1[u] * dprev);
+        }
+        cw_at(i) = cp;
+        xc[i - 1] = dp;
+        cprev = cp;
+        dprev = dp;
+      }
+    }
+    T dnext = dprev;
+    for (int i0 = nrow - 1; i0 >= 1; i0 -= CH) {
+      T cv[CH];
+#pragma unroll
+      for (int u = 0; u < CH; ++u) {
+        const int i = i0 - u;
+        if (i < 1) continue;
+        cv[u] = cw_at(i);
+      }
+#pragma unroll
+      for (int u = 0; u < CH; ++u) {
+        const int i = i0 - u;
+        if (i < 1) continue;
+        const T d = xc[i - 1] - cv[u] * dnext;
+        xc[i - 1] = d;
+        dnext = d;
+      }
+    }
+  }
+  __syncthreads();
+  stage_columns_out(xs, S, y, map, ndf, ind, nrow, pos0, nc);
+}
+
+// k_cma_apply<T, BW> for 64 columns per block of kApplyWaves wavefronts: the waves stage x
+// (and y when accumulating) for a quarter of the columns each, then every wave computes a
+// contiguous quarter of the rows of all 64 columns (lane = column), and the waves store y back a
+// quarter of the columns each.  Rows are independent in an apply, so this splits a column's
+// work four ways; a block needs kColBlock*(Sx + Sy) values of LDS.
+constexpr int kApplyWaves = 4;
+
+template <typename T, int BW>
+__global__ __launch_bounds__(kColBlock * kApplyWaves) void k_cma_apply_lds(
+    const T *__restrict__ C, CmaP p, int nc, const T *__restrict__ x, T *__restrict__ y,
+    const int *__restrict__ map_to, int ndf_to, const int *__restrict__ map_from, int ndf_from,
+    const int *__restrict__ ind_to, const int *__restrict__ ind_from, int accumulate) {
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  T *xs = reinterpret_cast<T *>(lds_raw);
+  const int Sx = staged_stride(p.ncol), Sy = staged_stride(p.nrow);
+  T *ys = xs + kColBlock * Sx;
+  const int pos0 = blockIdx.x * kColBlock;
+  const int w = threadIdx.x / kColBlock, lane = threadIdx.x & (kColBlock - 1);
+  constexpr int kCw = kColBlock / kApplyWaves;
+  stage_columns_in(xs, Sx, x, map_from, ndf_from, ind_from, p.ncol, pos0, nc, w * kCw, (w + 1) * kCw);
+  if (accumulate) stage_columns_in(ys, Sy, y, map_to, ndf_to, ind_to, p.nrow, pos0, nc, w * kCw, (w + 1) * kCw);
+  __syncthreads();
+  const int pos = pos0 + lane;
+  const int q = (p.nrow + kApplyWaves - 1) / kApplyWaves;
+  const int ifirst = 1 + w * q, ilast = min(p.nrow, (w + 1) * q);
+  if (pos < nc) {
+    const T *xc = xs + lane * Sx;
+    T *yc = ys + lane * Sy;
+    for (int i0 = ifirst; i0 <= ilast; i0 += kRowChunk) {
+      T a[kRowChunk][BW];
+#pragma unroll
+      for (int u = 0; u < kRowChunk; ++u) {
+        const int i = i0 + u;
+        if (i > ilast) continue;
+        const int jm = jminus(p, i), j0 = max(1, jm), j1 = min(p.ncol, jplus(p, i));
+#pragma unroll
+        for (int b = 0; b < BW; ++b) {
+          const int j = jm + b;
+          a[u][b] = (j >= j0 && j <= j1) ? C[cidx(p, nc, i, b + 1, pos)] : T(0);
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < kRowChunk; ++u) {
+        const int i = i0 + u;
+        if (i > ilast) continue;
+        const int jm = jminus(p, i), j0 = max(1, jm), j1 = min(p.ncol, jplus(p, i));
+        T acc = accumulate ? yc[i - 1] : T(0);
+#pragma unroll
+        for (int b = 0; b < BW; ++b) {
+          const int j = jm + b;
+          if (j >= j0 && j <= j1) acc = acc + a[u][b] * xc[j - 1];
+        }
+        yc[i - 1] = acc;
+      }
+    }
+  }
+  __syncthreads();
+  stage_columns_out(ys, Sy, y, map_to, ndf_to, ind_to, p.nrow, pos0, nc, w * kCw, (w + 1) * kCw);
 }
 
 // columnwise_op_mul_kernel_code (sci_columnwise_op_mul_kernel_mod.F90:128-160): C += A B.
@@ -438,6 +726,28 @@ int launch_cma_appinv(const lfric_cma *A, const void *x, void *y) {
   }
   const CmaP p = params_of(*A);
   const FunctionSpace &f = *A->to;
+  const size_t lds1 = cma_esize(A->precision) * kColBlock * staged_stride(A->nrow);
+  if (A->nrow <= kMaxStagedRows && lds1 <= max_block_lds()) {
+    const int blocks = (A->ncols + kColBlock - 1) / kColBlock;
+    const bool cw_lds = 2 * lds1 <= 64 * 1024;  // c' beside x while four blocks still fit a CU
+    const size_t lds = cw_lds ? 2 * lds1 : lds1;
+#define APPINV_LDS(T, CH)                                                                                       \
+  if (cw_lds)                                                                                                   \
+    hipLaunchKernelGGL((k_cma_appinv_lds<T, CH, true>), dim3(blocks), dim3(kColBlock), lds, ctx().stream,       \
+                       static_cast<const T *>(A->d), p, A->ncols, static_cast<const T *>(x), static_cast<T *>(y), \
+                       f.d_map, f.ndf, A->d_ind_to, static_cast<T *>(A->work));                                 \
+  else                                                                                                          \
+    hipLaunchKernelGGL((k_cma_appinv_lds<T, CH, false>), dim3(blocks), dim3(kColBlock), lds, ctx().stream,      \
+                       static_cast<const T *>(A->d), p, A->ncols, static_cast<const T *>(x), static_cast<T *>(y), \
+                       f.d_map, f.ndf, A->d_ind_to, static_cast<T *>(A->work));
+    if (A->precision == LFRIC_R64) {
+      APPINV_LDS(double, 16)
+    } else {
+      APPINV_LDS(float, 32)
+    }
+#undef APPINV_LDS
+    return finish("columnwise inverse");
+  }
   if (A->precision == LFRIC_R64)
     CLAUNCH(k_cma_appinv<double>, A->ncols, static_cast<const double *>(A->d), p, A->ncols,
             static_cast<const double *>(x), static_cast<double *>(y), f.d_map, f.ndf, A->d_ind_to,
@@ -597,14 +907,40 @@ int lfric_cma_apply(const lfric_cma *A, const lfric_vec *x, lfric_vec *y, int ac
   LF_CHECK(x->d != y->d, LFRIC_E_ARG, "columnwise apply: x and y alias");
   const CmaP p = params_of(*A);
   const FunctionSpace &ft = *A->to, &ff = *A->from;
-  if (A->precision == LFRIC_R64)
-    CLAUNCH(k_cma_apply<double>, A->ncols, static_cast<const double *>(A->d), p, A->ncols,
-            static_cast<const double *>(x->d), static_cast<double *>(y->d), ft.d_map, ft.ndf, ff.d_map, ff.ndf,
-            A->d_ind_to, A->d_ind_from, accumulate);
-  else
-    CLAUNCH(k_cma_apply<float>, A->ncols, static_cast<const float *>(A->d), p, A->ncols,
-            static_cast<const float *>(x->d), static_cast<float *>(y->d), ft.d_map, ft.ndf, ff.d_map, ff.ndf,
-            A->d_ind_to, A->d_ind_from, accumulate);
+  const int sblocks = (A->ncols + kColBlock - 1) / kColBlock;
+  const size_t slds = cma_esize(A->precision) * kColBlock * (staged_stride(A->nrow) + staged_stride(A->ncol));
+  const bool staged = A->nrow <= kMaxStagedRows && A->ncol <= kMaxStagedRows && slds <= max_block_lds();
+#define CMA_APPLY(T, BW)                                                                                          \
+  if (staged && BW > 0)                                                                                           \
+    hipLaunchKernelGGL((k_cma_apply_lds<T, (BW > 0 ? BW : 1)>), dim3(sblocks), dim3(kColBlock * kApplyWaves), slds,   \
+                       ctx().stream,                                                                              \
+                       static_cast<const T *>(A->d), p, A->ncols, static_cast<const T *>(x->d),                   \
+                       static_cast<T *>(y->d), ft.d_map, ft.ndf, ff.d_map, ff.ndf, A->d_ind_to, A->d_ind_from,    \
+                       accumulate);                                                                               \
+  else                                                                                                            \
+    CLAUNCH((k_cma_apply<T, BW>), A->ncols, static_cast<const T *>(A->d), p, A->ncols, static_cast<const T *>(x->d), \
+            static_cast<T *>(y->d), ft.d_map, ft.ndf, ff.d_map, ff.ndf, A->d_ind_to, A->d_ind_from, accumulate)
+#define CMA_APPLY_T(T)      \
+  switch (A->bw) {          \
+    case 1:                 \
+      CMA_APPLY(T, 1);      \
+      break;                \
+    case 2:                 \
+      CMA_APPLY(T, 2);      \
+      break;                \
+    case 3:                 \
+      CMA_APPLY(T, 3);      \
+      break;                \
+    default:                \
+      CMA_APPLY(T, 0);      \
+  }
+  if (A->precision == LFRIC_R64) {
+    CMA_APPLY_T(double)
+  } else {
+    CMA_APPLY_T(float)
+  }
+#undef CMA_APPLY_T
+#undef CMA_APPLY
   return finish("columnwise apply");
 }
 
