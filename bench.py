@@ -112,6 +112,54 @@ def cpu_baseline(fs, op64, b_host, iters_needed, budget_s):
     }
 
 
+def columnwise_report(L, fs, reps):
+    """Columnwise (CMA) tridiagonal operator on every W3 column of the mesh (nrow = nlayers,
+    bandwidth 3, random diagonally dominant bands): lfric_cma_apply and lfric_cma_apply_inverse,
+    FP32 and FP64.  Algorithmic bytes per call: the bands, x and y once each,
+    ncells * nrow * (3 + 2) * element size.  Beside them, the reference's own
+    columnwise_op_appinv_kernel_code (compiled from source into oracle/_ref) on the same FP32
+    data, one host core, on a bounded sample of columns."""
+    w3 = fs.sibling(L.W3, 0)
+    nc, nl = w3.ncells, w3.nlayers
+    rng = np.random.default_rng(0)
+    bands = rng.uniform(-1, 1, (nc, nl, 3))
+    bands[:, :, 1] += 4.0
+    out = {"space": f"W3 columns {nc} x {nl}"}
+    for P, tag, esz in ((L.R32, "fp32", 4), (L.R64, "fp64", 8)):
+        A = L.ColumnwiseOperator(w3, w3, P, custom=(1, 1, 1, 1)).upload(bands)
+        x, y = L.FieldVector(w3, P).fill_uniform(3, 3), L.FieldVector(w3, P)
+        alg = nc * nl * 5 * esz
+        for name, fn in (("apply", lambda: A.apply(x, y)), ("apply_inverse", lambda: A.apply_inverse(x, y))):
+            fn()
+            L.synchronize()
+            L.timer_start()
+            for _ in range(reps):
+                fn()
+            ms = L.timer_stop() / reps
+            gbs = alg / (ms * 1e-3) / 1e9
+            out[f"{name}_{tag}"] = {"ms": ms, "alg_bytes": alg, "gbs": gbs, "frac": gbs / HBM_PEAK_GBS}
+    lib_path = ROOT / "oracle" / "_ref" / "libref.so"
+    if lib_path.exists():
+        R = ctypes.CDLL(str(lib_path))
+        P_ = lambda a: ctypes.c_void_p(a.ctypes.data)
+        ncs = min(nc, 8192)
+        b32 = np.ascontiguousarray(bands[:ncs], np.float32)
+        p = np.array([1, 1, 1, 1, 3, nl, nl], np.int32)
+        dm = np.array([[c * nl + 1] for c in range(ncs)], np.int32)
+        ind = np.arange(1, nl + 1, dtype=np.int32)
+        xr = rng.uniform(-1, 1, ncs * nl).astype(np.float32)
+        yr = np.zeros_like(xr)
+        t = time.perf_counter()
+        R.ref_cma_appinv_r32(ctypes.c_int(ncs), P_(p), P_(b32), P_(xr), P_(yr), ctypes.c_int(1), ctypes.c_int(ncs * nl),
+                             P_(dm), P_(ind))
+        dt = time.perf_counter() - t
+        out["cpu_reference_apply_inverse_fp32"] = {
+            "ms_full_mesh": dt * 1e3 * nc / ncs, "cores": 1, "kind": "reference",
+            "sample": f"{ncs} columns of {nl} rows with columnwise_op_appinv_kernel_code ({dt * 1e3:.1f} ms), "
+                      f"scaled to {nc} columns"}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -208,6 +256,7 @@ def main():
                                             "fp32_tri_gbs": sym_bytes(fs, 4, 4) / (t32 * 1e-3) / 1e9,
                                             "fp64_tri_gbs": sym_bytes(fs, 8, 8) / (t64 * 1e-3) / 1e9}
         del s32, s64
+        out["columnwise_w3"] = columnwise_report(L, fs, args.apply_reps)
         for _ in range(args.apply_reps):  # plain copies of a known size (undf * 8 bytes each way)
             ys64.copy(xs64)
         L.synchronize()

@@ -52,6 +52,9 @@ module lfric_hip_capi_mod
             lfric_op_destroy, lfric_lma_apply, lfric_mm_diagonal, lfric_op_invert_local, &
             lfric_op_prepare_rows, lfric_op_tune, lfric_op_assemble_mass_w0, lfric_op_prepare_symmetric, &
             lfric_op_assemble_mass, lfric_op_assemble_div
+  public :: lfric_cma_create, lfric_cma_create_custom, lfric_cma_assemble, lfric_cma_assemble_diag_hmht, &
+            lfric_cma_assemble_m2v_lumped_inv, lfric_cma_apply, lfric_cma_apply_inverse, lfric_cma_multiply, &
+            lfric_cma_scaled_add, lfric_cma_destroy, lfric_prec_create_cma_inverse, lfric_tri_solve
   public :: lfric_prec_create_diagonal, lfric_prec_create_identity, lfric_prec_apply, &
             lfric_prec_destroy
   public :: lfric_linop_create_lma, lfric_linop_destroy, lfric_prec_create_column_jacobi
@@ -215,6 +218,69 @@ module lfric_hip_capi_mod
       import :: c_int, c_ptr
       type(c_ptr), value :: op, chi1, chi2, chi3
       integer(c_int), value :: nqp, rehabilitate
+    end function
+    ! columnwise_operator_type between horizontally discontinuous spaces (W3, Wtheta-like)
+    integer(c_int) function lfric_cma_create(fs_to, fs_from, precision, a) bind(c, name="lfric_cma_create")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: fs_to, fs_from
+      integer(c_int), value :: precision
+      type(c_ptr), intent(out) :: a
+    end function
+    integer(c_int) function lfric_cma_create_custom(fs_to, fs_from, precision, alpha, beta, gamma_m, gamma_p, a) &
+        bind(c, name="lfric_cma_create_custom")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: fs_to, fs_from
+      integer(c_int), value :: precision, alpha, beta, gamma_m, gamma_p
+      type(c_ptr), intent(out) :: a
+    end function
+    ! columnwise_op_asm_kernel_code from an LMA operator
+    integer(c_int) function lfric_cma_assemble(a, lma) bind(c, name="lfric_cma_assemble")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: a, lma
+    end function
+    integer(c_int) function lfric_cma_assemble_diag_hmht(a, dh, m2h) bind(c, name="lfric_cma_assemble_diag_hmht")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: a, dh, m2h
+    end function
+    integer(c_int) function lfric_cma_assemble_m2v_lumped_inv(a, m) bind(c, name="lfric_cma_assemble_m2v_lumped_inv")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: a, m
+    end function
+    ! columnwise_op_app_kernel_code (accumulate = 0: y = A x)
+    integer(c_int) function lfric_cma_apply(a, x, y, accumulate) bind(c, name="lfric_cma_apply")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: a, x, y
+      integer(c_int), value :: accumulate
+    end function
+    ! columnwise_op_appinv_kernel_code
+    integer(c_int) function lfric_cma_apply_inverse(a, x, y) bind(c, name="lfric_cma_apply_inverse")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: a, x, y
+    end function
+    ! columnwise_op_mul_kernel_code: c = c + a b
+    integer(c_int) function lfric_cma_multiply(a, b, c) bind(c, name="lfric_cma_multiply")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: a, b, c
+    end function
+    ! columnwise_op_scaledadd_kernel_code: c = alpha a + beta b
+    integer(c_int) function lfric_cma_scaled_add(a, b, c, alpha, beta) bind(c, name="lfric_cma_scaled_add")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: a, b, c
+      real(c_double), value :: alpha, beta
+    end function
+    subroutine lfric_cma_destroy(a) bind(c, name="lfric_cma_destroy")
+      import :: c_ptr
+      type(c_ptr), value :: a
+    end subroutine
+    integer(c_int) function lfric_prec_create_cma_inverse(a, p) bind(c, name="lfric_prec_create_cma_inverse")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: a
+      type(c_ptr), intent(out) :: p
+    end function
+    ! tri_solve_code on W3 fields (work: a W3 scratch field)
+    integer(c_int) function lfric_tri_solve(x, t0, tp, tm, y, work) bind(c, name="lfric_tri_solve")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: x, t0, tp, tm, y, work
     end function
     ! compute_div_operator_code: weak divergence W2 -> W3
     integer(c_int) function lfric_op_assemble_div(op, chi1, chi2, chi3, nqp, rehabilitate) &
