@@ -112,6 +112,43 @@ def cpu_baseline(fs, op64, b_host, iters_needed, budget_s):
     }
 
 
+FP64_VECTOR_PEAK_TFLOPS = 78.6
+# floating-point operations per quadrature point of the assembly kernels (Jacobian 144, its
+# determinant 14, the point weight 1, then per entry: W0 36 x 4, W2 6 x 3 products J b and 21 x 8,
+# W3 2, divergence 6 x 2)
+ASSEMBLY_FLOPS_PER_POINT = {"mass_w0": 303, "mass_w2": 345, "mass_w3": 161, "div": 171}
+
+
+def assembly_report(L, fs, nqp=3):
+    """On-device assembly of the lowest-order W0, W2, W3 mass matrices and the weak divergence on
+    every cell of the mesh, FP64, nqp^3 Gauss points.  The coordinates are synthetic (uniform
+    random W0 fields): the kernels' work does not depend on their values."""
+    w2, w3 = fs.sibling(L.W2, 0), fs.sibling(L.W3, 0)
+    rng = np.random.default_rng(1)
+    chi = [L.FieldVector(fs).import_field(rng.uniform(0, 1, fs.undf)) for _ in range(3)]
+    nc3 = fs.ncell_3d
+    ops = {"mass_w0": (L.LmaOperator(fs), "assemble_mass", 64),
+           "mass_w2": (L.LmaOperator(w2), "assemble_mass", 36),
+           "mass_w3": (L.LmaOperator(w3), "assemble_mass", 1),
+           "div": (L.LmaOperator(w3, w2), "assemble_div", 6)}
+    out = {"cells": nc3, "nqp": nqp, "coordinates": "synthetic uniform random W0 fields"}
+    for name, (op, method, nent) in ops.items():
+        fn = getattr(op, method)
+        fn(*chi, nqp=nqp)
+        L.synchronize()
+        L.timer_start()
+        reps = 3
+        for _ in range(reps):
+            fn(*chi, nqp=nqp)
+        ms = L.timer_stop() / reps
+        flops = nc3 * nqp ** 3 * ASSEMBLY_FLOPS_PER_POINT[name]
+        wbytes = nc3 * nent * 8 + 3 * fs.undf * 8
+        tf = flops / (ms * 1e-3) / 1e12
+        out[name] = {"ms": ms, "tflops": tf, "frac_fp64": tf / FP64_VECTOR_PEAK_TFLOPS,
+                     "bytes": wbytes, "gbs": wbytes / (ms * 1e-3) / 1e9}
+    return out
+
+
 def columnwise_report(L, fs, reps):
     """Columnwise (CMA) tridiagonal operator on every W3 column of the mesh (nrow = nlayers,
     bandwidth 3, random diagonally dominant bands): lfric_cma_apply and lfric_cma_apply_inverse,
@@ -257,6 +294,7 @@ def main():
                                             "fp64_tri_gbs": sym_bytes(fs, 8, 8) / (t64 * 1e-3) / 1e9}
         del s32, s64
         out["columnwise_w3"] = columnwise_report(L, fs, args.apply_reps)
+        out["assembly"] = assembly_report(L, fs)
         for _ in range(args.apply_reps):  # plain copies of a known size (undf * 8 bytes each way)
             ys64.copy(xs64)
         L.synchronize()
