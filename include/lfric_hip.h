@@ -87,6 +87,19 @@ int lfric_fs_create_cubedsphere(int edge_cells, int nlayers, int space, int orde
                                 lfric_fs **fs);
 /* Same mesh as `like` (same cells, layers and colouring), another space/order. */
 int lfric_fs_create_sibling(const lfric_fs *like, int space, int order, lfric_fs **fs);
+/* The global mesh of a UGRID 2D file (infrastructure/source/io/ugrid_2d_mod.F90 reads the same
+ * variables; netCDF classic or 64-bit-offset format): the mesh topology variable mesh_name (NULL:
+ * the file's only one), its face_node_connectivity and face_face_connectivity as 1-based
+ * [face][4] arrays (0 = no neighbour) and its npanels attribute (0 if absent).  Host only (no
+ * device needed); call with null arrays to get nfaces first. */
+int lfric_ugrid_read(const char *path, const char *mesh_name, int *nfaces, int *npanels, int *face_nodes,
+                     int *face_links);
+/* A function space on the extruded mesh of a UGRID file, built as for a generated mesh (local
+ * vertex ids, 3D connectivity, dofmap_setup) and coloured as mesh_colouring_mod.F90:49-128 does:
+ * the cubed-sphere closed form for six panels, the planar breadth-first form for one panel of
+ * quadrilaterals, the generic greedy form otherwise. */
+int lfric_fs_create_ugrid(const char *path, const char *mesh_name, int nlayers, int space, int order,
+                          lfric_fs **fs);
 int lfric_fs_info(const lfric_fs *fs, int *ncells, int *nlayers, int *ndf, int *undf,
                   int *ncolours);
 int lfric_fs_get_dofmap(const lfric_fs *fs, int *dofmap);

@@ -107,6 +107,8 @@ SIGNATURES = {
     "lfric_linop_apply": (_I, [_VP, _VP, _VP]),
     "lfric_linop_destroy": (None, [_VP]),
     "lfric_prec_create_column_jacobi": (_I, [_VP, _I, ctypes.POINTER(_VP)]),
+    "lfric_ugrid_read": (_I, [ctypes.c_char_p, ctypes.c_char_p, _VP, _VP, _VP, _VP]),
+    "lfric_fs_create_ugrid": (_I, [ctypes.c_char_p, ctypes.c_char_p, _I, _I, _I, ctypes.POINTER(_VP)]),
     "lfric_cma_create": (_I, [_VP, _VP, _I, ctypes.POINTER(_VP)]),
     "lfric_cma_create_custom": (_I, [_VP, _VP, _I, _I, _I, _I, _I, ctypes.POINTER(_VP)]),
     "lfric_cma_info": (_I, [_VP, _VP]),

@@ -7,7 +7,7 @@ from pathlib import Path
 PKG = Path(__file__).resolve().parent
 CSRC = PKG / "csrc"
 LIB = PKG / "liblfric_hip.so"
-SOURCES = ["api.hip", "kernels.hip", "krylov.hip", "solvers.hip", "batch.hip", "cma.hip", "mesh_gen.cpp"]
+SOURCES = ["api.hip", "kernels.hip", "krylov.hip", "solvers.hip", "batch.hip", "cma.hip", "mesh_gen.cpp", "ugrid.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function",

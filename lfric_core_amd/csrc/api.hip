@@ -415,6 +415,31 @@ int lfric_fs_create_cubedsphere(int edge_cells, int nlayers, int space, int orde
   return space_from_topology(t, nlayers, space, order, out);
 }
 
+int lfric_ugrid_read(const char *path, const char *mesh_name, int *nfaces, int *npanels, int *face_nodes,
+                     int *face_links) {
+  LF_CHECK(path && nfaces, LFRIC_E_ARG, "null argument");
+  std::vector<int> fn, fl;
+  int nf = 0, np = 0;
+  if (int rc = read_ugrid_mesh(path, mesh_name ? mesh_name : "", fn, fl, nf, np)) return rc;
+  *nfaces = nf;
+  if (npanels) *npanels = np;
+  if (face_nodes) std::copy(fn.begin(), fn.end(), face_nodes);
+  if (face_links) std::copy(fl.begin(), fl.end(), face_links);
+  return LFRIC_OK;
+}
+
+int lfric_fs_create_ugrid(const char *path, const char *mesh_name, int nlayers, int space, int order,
+                          lfric_fs **out) {
+  if (int rc = require_init()) return rc;
+  LF_CHECK(path && out && nlayers > 0, LFRIC_E_ARG, "bad arguments");
+  std::vector<int> fn, fl;
+  int nf = 0, np = 0;
+  if (int rc = read_ugrid_mesh(path, mesh_name ? mesh_name : "", fn, fl, nf, np)) return rc;
+  auto t = std::make_shared<Topology>();
+  if (int rc = build_topology(nf, fl, fn, np, *t)) return rc;
+  return space_from_topology(t, nlayers, space, order, out);
+}
+
 int lfric_fs_create_sibling(const lfric_fs *like, int space, int order, lfric_fs **out) {
   if (int rc = require_init()) return rc;
   LF_CHECK(like && out && like->fs->topo, LFRIC_E_ARG, "sibling spaces need a generated mesh");

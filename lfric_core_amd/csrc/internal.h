@@ -59,6 +59,13 @@ struct Topology {
   int ncolours = 0;
 };
 int build_cubedsphere_topology(int edge_cells, Topology &t);
+// the topology below a global mesh given as [cell][W,S,E,N] neighbours (0: none) and
+// [cell][SW,SE,NE,NW] global vertex ids, all 1-based; npanels as the mesh file states it
+int build_topology(int ncells, const std::vector<int> &cell_next, const std::vector<int> &vert_global, int npanels,
+                   Topology &t);
+// UGRID 2D mesh (netCDF classic file): face-node and face-face connectivity, 1-based
+int read_ugrid_mesh(const std::string &path, const std::string &mesh_name, std::vector<int> &face_nodes,
+                    std::vector<int> &face_links, int &nfaces, int &npanels);
 int build_dofmap(const Topology &t, int nlayers, int space, int order, std::vector<int> &dofmap,
                  int &ndf, int &undf);
 

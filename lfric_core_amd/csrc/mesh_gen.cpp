@@ -13,6 +13,7 @@
 //   colouring                             infrastructure/source/mesh/mesh_colouring_mod.F90:151-308, 703-837
 #include <algorithm>
 #include <array>
+#include <cmath>
 #include <cstring>
 
 #include "internal.h"
@@ -242,6 +243,56 @@ int greedy_colour(Topology &t) {
   return ncol;
 }
 
+// test_mesh_planar_quadrilateral (mesh_colouring_mod.F90:663-691): every neighbour across a
+// side leads back across the opposite side.
+bool planar_quadrilateral(const Topology &t) {
+  static const int opposite[4] = {2, 3, 0, 1};  // W<->E, S<->N
+  for (int c = 1; c <= t.ncells; ++c)
+    for (int d = 0; d < 4; ++d) {
+      const int o = t.cell_next[static_cast<size_t>(c - 1) * 4 + d];
+      if (o > 0 && o <= t.ncells && t.cell_next[static_cast<size_t>(o - 1) * 4 + opposite[d]] != c) return false;
+    }
+  return true;
+}
+
+// set_colours_planar_quadrilateral (mesh_colouring_mod.F90:348-495): breadth-first from each
+// uncoloured cell (the first, then the next uncoloured one by id), visiting the N, E, S, W
+// neighbours of each cell of the current front, a neighbour taking the colour that the
+// direction and the cell's colour give in cell_next_colour.
+void planar_colour(Topology &t) {
+  // next_colour[direction W,S,E,N][colour 1..4]
+  static const int next_colour[4][4] = {{2, 1, 4, 3}, {3, 4, 1, 2}, {2, 1, 4, 3}, {3, 4, 1, 2}};
+  static const int order[4] = {3, 2, 1, 0};  // N, E, S, W
+  const int nc = t.ncells;
+  std::fill(t.colour_of_cell.begin(), t.colour_of_cell.end(), 0);
+  std::vector<int> todo(static_cast<size_t>(nc) + 1, 0);
+  int cell = 1, start = 1, end = 1;
+  todo[1] = 1;
+  while (cell <= nc) {
+    t.colour_of_cell[cell - 1] = 1;
+    while (start <= end) {
+      const int last = end;
+      for (int i = start; i <= last; ++i) {
+        const int c = todo[i];
+        for (int k = 0; k < 4; ++k) {
+          const int d = order[k];
+          const int o = t.cell_next[static_cast<size_t>(c - 1) * 4 + d];
+          if (o > 0 && o <= nc && t.colour_of_cell[o - 1] == 0) {
+            t.colour_of_cell[o - 1] = next_colour[d][t.colour_of_cell[c - 1] - 1];
+            todo[++end] = o;
+          }
+        }
+      }
+      start = last + 1;
+    }
+    while (t.colour_of_cell[cell - 1] != 0) {
+      ++cell;
+      if (cell > nc) break;
+    }
+  }
+  t.ncolours = 4;
+}
+
 }  // namespace
 
 int build_cubedsphere_topology(int n, Topology &t) {
@@ -252,8 +303,26 @@ int build_cubedsphere_topology(int n, Topology &t) {
   panel_adjacency(P);
   vertex_numbering(P);
   orient_panels(P);
+  return build_topology(P.nc, P.next, P.vert, 6, t);
+}
 
-  t.edge_cells = n;
+// Everything below the global mesh: local vertex ids, the 3D connectivity of the base layer
+// and the colouring (mesh_colouring_mod.F90:49-128: the cubed-sphere closed form for a six-panel
+// mesh of 6 n^2 cells, the planar breadth-first form for a one-panel quadrilateral mesh, the
+// generic greedy form otherwise or when the first does not validate).
+int build_topology(int ncells, const std::vector<int> &cell_next, const std::vector<int> &vert_global, int npanels,
+                   Topology &t) {
+  LF_CHECK(ncells > 0 && cell_next.size() == static_cast<size_t>(ncells) * 4 && vert_global.size() == cell_next.size(),
+           LFRIC_E_ARG, "mesh: connectivity sizes");
+  for (int v : vert_global) LF_CHECK(v > 0, LFRIC_E_ARG, "mesh: a face without four corner nodes");
+  for (int c : cell_next) LF_CHECK(c >= 0 && c <= ncells, LFRIC_E_ARG, "mesh: a neighbour id out of range");
+  const int n = static_cast<int>(std::lround(std::sqrt(ncells / 6.0)));
+  const bool cubed = npanels == 6 && 6 * n * n == ncells;
+  struct {
+    std::vector<int> next, vert;
+    int nc;
+  } P{cell_next, vert_global, ncells};
+  t.edge_cells = cubed ? n : 0;
   t.ncells = P.nc;
   t.cell_next = P.next;
   // local vertex ids in order of first appearance
@@ -333,13 +402,20 @@ int build_cubedsphere_topology(int n, Topology &t) {
   }
   t.nedges = nh;
   (void)nside;
-  t.colour_of_cell.resize(nc);
-  for (int c = 1; c <= nc; ++c) t.colour_of_cell[c - 1] = closed_form_colour(n, c);
-  t.ncolours = 4;
-  if (!colouring_valid(t)) {
+  t.colour_of_cell.assign(nc, 0);
+  bool ok = false;
+  if (cubed) {
+    for (int c = 1; c <= nc; ++c) t.colour_of_cell[c - 1] = closed_form_colour(n, c);
+    t.ncolours = 4;
+    ok = colouring_valid(t);
+  } else if (npanels == 1 && planar_quadrilateral(t)) {
+    planar_colour(t);
+    ok = colouring_valid(t);
+  }
+  if (!ok) {
     t.ncolours = greedy_colour(t);
     if (t.ncolours == 0 || !colouring_valid(t))
-      return fail(LFRIC_E_UNSUPPORTED, "no valid colouring found for edge_cells=" + std::to_string(n));
+      return fail(LFRIC_E_UNSUPPORTED, "no valid colouring found for a mesh of " + std::to_string(nc) + " cells");
   }
   return LFRIC_OK;
 }

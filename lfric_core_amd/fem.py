@@ -36,6 +36,16 @@ class FunctionSpace:
                                     ptr(cic), cic.shape[1], ctypes.byref(h)))
         return cls(h)
 
+    @classmethod
+    def from_ugrid(cls, path, nlayers, space=W0, order=0, mesh=None):
+        """A space on the extruded global mesh of a UGRID 2D file (ugrid_2d_mod.F90's
+        face_node and face_face connectivity), coloured as the reference colours that mesh."""
+        init()
+        h = ctypes.c_void_p()
+        check(lib().lfric_fs_create_ugrid(str(path).encode(), mesh.encode() if mesh else None, nlayers, space, order,
+                                          ctypes.byref(h)))
+        return cls(h)
+
     def sibling(self, space, order=0):
         h = ctypes.c_void_p()
         check(lib().lfric_fs_create_sibling(self._h, space, order, ctypes.byref(h)))
@@ -282,3 +292,15 @@ class LmaOperator:
             except Exception:
                 pass
             self._h = None
+
+
+def read_ugrid(path, mesh=None):
+    """(face_nodes, face_links, npanels) of a UGRID 2D file: 1-based (nfaces, 4) int32 arrays,
+    0 for a missing neighbour.  Host only."""
+    n, npan = ctypes.c_int(), ctypes.c_int()
+    m = mesh.encode() if mesh else None
+    check(lib().lfric_ugrid_read(str(path).encode(), m, ctypes.byref(n), ctypes.byref(npan), None, None))
+    fn = np.zeros((n.value, 4), np.int32)
+    fl = np.zeros((n.value, 4), np.int32)
+    check(lib().lfric_ugrid_read(str(path).encode(), m, ctypes.byref(n), ctypes.byref(npan), ptr(fn), ptr(fl)))
+    return fn, fl, npan.value

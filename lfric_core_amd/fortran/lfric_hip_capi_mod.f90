@@ -52,6 +52,7 @@ module lfric_hip_capi_mod
             lfric_op_destroy, lfric_lma_apply, lfric_mm_diagonal, lfric_op_invert_local, &
             lfric_op_prepare_rows, lfric_op_tune, lfric_op_assemble_mass_w0, lfric_op_prepare_symmetric, &
             lfric_op_assemble_mass, lfric_op_assemble_div
+  public :: lfric_fs_create_ugrid
   public :: lfric_cma_create, lfric_cma_create_custom, lfric_cma_assemble, lfric_cma_assemble_diag_hmht, &
             lfric_cma_assemble_m2v_lumped_inv, lfric_cma_apply, lfric_cma_apply_inverse, lfric_cma_multiply, &
             lfric_cma_scaled_add, lfric_cma_destroy, lfric_prec_create_cma_inverse, lfric_tri_solve
@@ -218,6 +219,16 @@ module lfric_hip_capi_mod
       import :: c_int, c_ptr
       type(c_ptr), value :: op, chi1, chi2, chi3
       integer(c_int), value :: nqp, rehabilitate
+    end function
+    ! a function space on the extruded mesh of a UGRID file; path null-terminated
+    ! (trim(path)//c_null_char), mesh_name c_null_ptr for the file's only mesh
+    integer(c_int) function lfric_fs_create_ugrid(path, mesh_name, nlayers, space, order, fs) &
+        bind(c, name="lfric_fs_create_ugrid")
+      import :: c_int, c_ptr, c_char
+      character(kind=c_char), dimension(*), intent(in) :: path
+      type(c_ptr), value :: mesh_name
+      integer(c_int), value :: nlayers, space, order
+      type(c_ptr), intent(out) :: fs
     end function
     ! columnwise_operator_type between horizontally discontinuous spaces (W3, Wtheta-like)
     integer(c_int) function lfric_cma_create(fs_to, fs_from, precision, a) bind(c, name="lfric_cma_create")

@@ -61,6 +61,7 @@ def lib():
             "orc_colour_generic": (_I, [_I, _VP, _VP]),
             "orc_colour_lookup": (_I, [_I, _I, _VP, _VP, _VP, _I]),
             "orc_colour_validate": (_I, [_I, _VP, _VP]),
+            "orc_colour_planar": (_I, [_I, _VP, _VP]),
             "orc_rng_bits": (_U64, [_U64, _U64, _U64]),
             "orc_rng_uniform": (_D, [_U64, _U64, _U64]),
             "orc_rng_normal": (_D, [_U64, _U64, _U64]),
@@ -178,6 +179,34 @@ def colour_cubedsphere(n, cell_next):
     cic = np.zeros((ncol, maxcc), np.int32)
     lib().orc_colour_lookup(nc, ncol, _p(cm), _p(npc), _p(cic), maxcc)
     return cm, npc, cic
+
+
+def _lookup(cm, ncol):
+    nc = len(cm)
+    npc = np.zeros(ncol, np.int32)
+    maxcc = int(np.bincount(cm, minlength=ncol + 1)[1:].max())
+    cic = np.zeros((ncol, maxcc), np.int32)
+    lib().orc_colour_lookup(nc, ncol, _p(cm), _p(npc), _p(cic), maxcc)
+    return npc, cic
+
+
+def colour_generic(cell_next):
+    """set_colours_generic on any mesh: (colour_map, ncells_per_colour, cells_in_colour)."""
+    cn = np.ascontiguousarray(cell_next, np.int32)
+    cm = np.zeros(cn.shape[0], np.int32)
+    ncol = lib().orc_colour_generic(cn.shape[0], _p(cn), _p(cm))
+    assert ncol > 0, "generic colouring did not validate"
+    return (cm,) + _lookup(cm, ncol)
+
+
+def colour_planar(cell_next):
+    """set_colours_planar_quadrilateral, or None when the reference would fall back to the
+    generic colouring."""
+    cn = np.ascontiguousarray(cell_next, np.int32)
+    cm = np.zeros(cn.shape[0], np.int32)
+    if lib().orc_colour_planar(cn.shape[0], _p(cn), _p(cm)) == 0:
+        return None
+    return (cm,) + _lookup(cm, 4)
 
 
 def colour_validate(cell_next, colour_map):

@@ -22,6 +22,9 @@
  *   orc_cg / orc_gmres        numpy.linalg.solve on tiny SPD/nonsymmetric systems; scipy cg
  *                             solutions; Givens least-squares vs numpy.linalg.lstsq
  *   orc_colour_generic        greedy rule checked from vertex lists (independent neighbour sets)
+ *   orc_colour_planar         valid 4-colouring; along W/E the lowest bit of colour-1 alternates
+ *                             and along S/N the next one (the structure the transition table
+ *                             gives), on the reference's planar UGRID meshes
  *   orc_transpose_apply,      known-answer tests of transpose/dg/mm_diagonal unit tests;
  *   orc_dg_apply,             bit-level comparison with the compiled reference kernels
  *   orc_mm_diagonal
@@ -82,6 +85,7 @@ int orc_dofmap(int space, int order, int ncells, int nlayers, int nverts_2d, int
 
 int orc_colour_cubedsphere(int n, const int *cell_next, int *colour_map);
 int orc_colour_generic(int ncells, const int *cell_next, int *colour_map);
+int orc_colour_planar(int ncells, const int *cell_next, int *colour_map);
 int orc_colour_lookup(int ncells, int ncolours, const int *colour_map, int *npc, int *cic, int maxcc);
 int orc_colour_validate(int ncells, const int *cell_next, const int *colour_map);
 

@@ -494,6 +494,53 @@ int orc_colour_lookup(int nc, int ncolours, const int *colour_map, int *npc, int
   return mx;
 }
 
+/* test_mesh_planar_quadrilateral and set_colours_planar_quadrilateral
+ * (infrastructure/source/mesh/mesh_colouring_mod.F90:663-691, 348-495): when every neighbour
+ * leads back across the opposite side, a breadth-first walk from cell 1 (then from the next
+ * uncoloured cell) over the N, E, S, W neighbours of each front, each neighbour taking
+ * cell_next_colour(direction, colour); 4 colours.  Returns 4, or 0 when the mesh is not planar
+ * quadrilateral or the result does not validate (the reference then colours generically). */
+int orc_colour_planar(int nc, const int *cell_next, int *colour_map) {
+  static const int cell_next_colour[5][5] = {{0, 0, 0, 0, 0}, {0, 2, 1, 4, 3}, {0, 3, 4, 1, 2},
+                                             {0, 2, 1, 4, 3}, {0, 3, 4, 1, 2}};  /* [dir W..N][colour] */
+  static const int direction[4] = {N, E, S, W}, opposite[4] = {S, W, N, E};
+#define NX(d, c) cell_next[((c)-1) * 4 + (d)-1]
+  for (int cell = 1; cell <= nc; ++cell)
+    for (int k = 0; k < 4; ++k) {
+      const int t = NX(direction[k], cell);
+      if (t > 0 && t <= nc && NX(opposite[k], t) != cell) return 0;
+    }
+  int *todo = (int *)calloc((size_t)nc + 1, sizeof(int));
+  for (int c = 0; c < nc; ++c) colour_map[c] = 0;
+  int cell = 1, start = 1, end = 1;
+  todo[1] = 1;
+  while (cell <= nc) {
+    colour_map[cell - 1] = 1;
+    while (start <= end) {
+      int i;
+      const int last = end;
+      for (i = start; i <= last; ++i) {
+        const int c = todo[i];
+        for (int k = 0; k < 4; ++k) {
+          const int t = NX(direction[k], c);
+          if (t > 0 && t <= nc && colour_map[t - 1] == 0) {
+            colour_map[t - 1] = cell_next_colour[direction[k]][colour_map[c - 1]];
+            todo[++end] = t;
+          }
+        }
+      }
+      start = i;
+    }
+    while (colour_map[cell - 1] != 0) {
+      ++cell;
+      if (cell > nc) break;
+    }
+  }
+  free(todo);
+#undef NX
+  return orc_colour_validate(nc, cell_next, colour_map) ? 4 : 0;
+}
+
 /* test_set_colours (infrastructure/source/mesh/mesh_colouring_mod.F90:765-837).
  * Returns 1 when the colouring passes. */
 int orc_colour_validate(int nc, const int *cell_next, const int *cm) {

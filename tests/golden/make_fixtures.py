@@ -10,6 +10,8 @@ Everything written is data (inputs and expected outputs); no reference source te
                       applications/coupled/example/mesh_C12.nc,
                       applications/io_demo/example/mesh_C24.nc), read with scipy's netCDF3
                       reader; the C16/C24/C32 arrays are stored as SHA-256 digests only.
+* ugrid/*.nc          copies of four of the reference's UGRID mesh files (C4, C12 cubed spheres,
+                      a bi-periodic 8x8 planar mesh, a bounded planar mesh): data the reader parses.
 * canned_mesh.json    the bi-periodic 3x3x3 unit-test mesh of
                       infrastructure/source/mesh/mesh_mod.F90:3625-3777 (layer-1 cell_next,
                       vert_on_cell, edge_on_cell, face_on_cell) and the W0/W2/W3 dofmaps
@@ -83,6 +85,14 @@ def canned_rows(text, start_marker, array, ncells=9):
     return [rows[c] for c in range(1, ncells + 1)]
 
 
+UGRID_FILES = (
+    "infrastructure/unit-test/data/mesh_C4.nc",
+    "applications/coupled/example/mesh_C12.nc",
+    "infrastructure/unit-test/data/mesh_BiP8x8-750x250.nc",
+    "infrastructure/unit-test/data/mesh_planar.nc",
+)
+
+
 def main(ref):
     ref = Path(ref)
     arrays, digests = {}, {}
@@ -94,6 +104,10 @@ def main(ref):
             arrays[f"C{n}_face_nodes"] = fn
             arrays[f"C{n}_face_links"] = fl
     np.savez_compressed(HERE / "mesh_fixtures.npz", **arrays)
+    # a few of the reference's own UGRID data files, byte for byte, for the mesh reader's tests
+    (HERE / "ugrid").mkdir(exist_ok=True)
+    for rel in UGRID_FILES:
+        (HERE / "ugrid" / Path(rel).name).write_bytes((ref / rel).read_bytes())
     (HERE / "mesh_digests.json").write_text(json.dumps(digests, indent=1, sort_keys=True) + "\n")
 
     mesh_src = (ref / "infrastructure/source/mesh/mesh_mod.F90").read_text()
