@@ -718,11 +718,10 @@ int launch_cma_appinv(const lfric_cma *A, const void *x, void *y) {
            "columnwise inverse: a square tridiagonal operator is needed");
   const size_t wb = cma_esize(A->precision) * static_cast<size_t>(A->nrow) * A->ncols;
   if (A->work_bytes < wb) {
-    auto *M = const_cast<lfric_cma *>(A);  // the factor workspace is scratch, not operator state
-    dfree(M->work, M->work_bytes);
-    M->work_bytes = 0;
-    if (int rc = dmalloc(&M->work, wb)) return rc;
-    M->work_bytes = wb;
+    dfree(A->work, A->work_bytes);
+    A->work_bytes = 0;
+    if (int rc = dmalloc(&A->work, wb)) return rc;
+    A->work_bytes = wb;
   }
   const CmaP p = params_of(*A);
   const FunctionSpace &f = *A->to;

@@ -151,8 +151,10 @@ struct lfric_cma {
   int *d_ind_to = nullptr, *d_ind_from = nullptr;        // row/column -> offset from the column base
   int *d_band_to = nullptr, *d_band_from = nullptr;      // [k][df] banded row/column (1-based)
   size_t ind_to_bytes = 0, ind_from_bytes = 0, band_to_bytes = 0, band_from_bytes = 0;
-  void *work = nullptr;                // [nrow][ncols] forward-sweep factors of the tridiagonal solve
-  size_t work_bytes = 0;
+  // [nrow][ncols] forward-sweep factors of the tridiagonal solve: scratch grown on first use,
+  // not part of the operator's value
+  mutable void *work = nullptr;
+  mutable size_t work_bytes = 0;
   ~lfric_cma();
 };
 
