@@ -45,14 +45,13 @@ def sym_bytes(fs, s_a, s_x):
     return fs.ncell_3d * 36 * s_a + 2 * fs.undf * s_x + 4 * fs.ncells * 8
 
 
-def solve_roofline(fs, mres, pres, t_mixed, t_fp64, folded=True):
+def solve_roofline(fs, mres, pres, t_mixed, t_fp64):
     """Whole-solve HBM rate: sum over iterations of the PCG iteration's algorithmic bytes plus,
     for the mixed solve, each FP64 refinement step (one FP64 apply + 4 FP64 vector passes),
-    divided by the solve time.  Iteration: the apply (matrix, x, y, dofmap once each), then the
-    update (reads x, p, r, A p, diag; writes x, r, z: 8 vector passes) and aypx (reads z, p, writes
-    p: 3) -- or, with aypx folded into the apply, 1 more read (z beside p_old) and 1 more write
-    (p) in the apply instead of aypx: 10 passes beyond the apply."""
-    passes = 10 if folded else 11
+    divided by the solve time.  Iteration: the apply with p.(A p) (matrix, x, y, dofmap once
+    each), the update (reads x, p, r, A p, diag; writes x, r, z: 8 vector passes) and aypx
+    (reads z, p; writes p: 3), i.e. 11 vector passes of undf entries beyond the apply."""
+    passes = 11
     it32 = apply_bytes(fs, 8, 4, 4) + passes * fs.undf * 4
     it64 = apply_bytes(fs, 8, 8, 8) + passes * fs.undf * 8
     ref64 = apply_bytes(fs, 8, 8, 8) + 4 * fs.undf * 8
@@ -379,7 +378,7 @@ def main():
         "iterations": {"mixed_inner": mres.iterations, "mixed_outer": mres.outer_iterations,
                        "fp64": pres.iterations},
         "solve_roofline": solve_roofline(fs, mres, pres, sum(mixed_times) / len(mixed_times),
-                                         sum(plain_times) / len(plain_times), folded=args.apply_form == 1),
+                                         sum(plain_times) / len(plain_times)),
         "verification": {"mixed_true_rel_res_fp64": mixed_true, "fp64_true_rel_res": plain_true,
                          "mixed_vs_fp64_solution_rel_err": err_mixed_vs_fp64},
         "lma_gbs": {"fp32": gbs32, "fp64": gbs64, "fp32_ms": t32, "fp64_ms": t64,

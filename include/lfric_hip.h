@@ -65,12 +65,6 @@ long long lfric_device_bytes(void);
 /* LMA apply used inside the solvers: 0 the colour-ordered scatter (4 launches per apply),
  * 1 the gather form (one launch; in CG it also forms p.(A p)).  Default 1. */
 int lfric_set_apply_form(int form);
-/* CG with the gather form over a row-gathered copy (lfric_op_prepare_rows): fold p%aypx(beta, z)
- * into the next apply (p formed where the apply reads it, two alternating p buffers), saving a
- * vector pass and a launch per iteration with the same arithmetic in the same order, so the
- * iterates are unchanged bit for bit.  No reference counterpart (the reference runs the
- * separate built-in).  Default 1 (on). */
-int lfric_set_cg_fusion(int on);
 /* Free and total device memory (hipMemGetInfo) of the library's device. */
 int lfric_device_memory(long long *free_bytes, long long *total_bytes);
 

@@ -22,7 +22,7 @@ Reference interfaces mirrored (paths relative to the lfric_core tree):
   tri_solve              components/science/source/kernel/solver/sci_tri_solve_kernel_mod.f90
 """
 from ._lib import (LfricError, NotConverged, lib, init, synchronize, timer_start, timer_stop,
-                   device_bytes, set_apply_form, set_cg_fusion, R64, R32, BF16, F16, W0, W2, W3)
+                   device_bytes, set_apply_form, R64, R32, BF16, F16, W0, W2, W3)
 from .fem import FunctionSpace, FieldVector, LmaOperator
 from .cma import ColumnwiseOperator, ColumnwiseInversePreconditioner, tri_solve
 from .solvers import (NullPreconditioner, DiagonalPreconditioner, ColumnJacobiPreconditioner,
@@ -32,7 +32,7 @@ from .solvers import (NullPreconditioner, DiagonalPreconditioner, ColumnJacobiPr
 
 __all__ = [
     "LfricError", "NotConverged", "lib", "init", "synchronize", "timer_start", "timer_stop",
-    "device_bytes", "set_apply_form", "set_cg_fusion", "R64", "R32", "BF16", "F16", "W0", "W2", "W3", "FunctionSpace", "FieldVector",
+    "device_bytes", "set_apply_form", "R64", "R32", "BF16", "F16", "W0", "W2", "W3", "FunctionSpace", "FieldVector",
     "LmaOperator", "NullPreconditioner", "DiagonalPreconditioner", "ColumnJacobiPreconditioner",
     "LinearOperator", "ConjugateGradient", "MixedPrecisionCG", "GMRES", "MixedPrecisionGMRES",
     "FGMRES", "GCR", "BiCGStab", "Jacobi", "Chebyshev", "PreconditionOnly", "solve", "METHODS",

@@ -56,7 +56,6 @@ SIGNATURES = {
     "lfric_timer_stop": (_I, [_DP]),
     "lfric_device_bytes": (ctypes.c_longlong, []),
     "lfric_set_apply_form": (_I, [_I]),
-    "lfric_set_cg_fusion": (_I, [_I]),
     "lfric_device_memory": (_I, [ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong)]),
     "lfric_fs_create": (_I, [_I, _I, _I, _I, _VP, _I, _VP, _VP, _I, ctypes.POINTER(_VP)]),
     "lfric_fs_create_cubedsphere": (_I, [_I, _I, _I, _I, ctypes.POINTER(_VP)]),
@@ -220,8 +219,3 @@ def ptr(a):
 def set_apply_form(form):
     """LMA apply used inside the solvers: 0 colour-ordered scatter, 1 gather (fused p.Ap in CG)."""
     check(lib().lfric_set_apply_form(int(form)))
-
-
-def set_cg_fusion(on):
-    """CG over a row-gathered copy: fold p%aypx into the next apply (same iterates bit for bit)."""
-    check(lib().lfric_set_cg_fusion(int(bool(on))))

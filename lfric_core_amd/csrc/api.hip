@@ -57,14 +57,14 @@ FunctionSpace::~FunctionSpace() {
 }
 
 // Scratch vectors of the solvers, kept between solves (grown on demand).  Slots: 0-2 CG's r, z,
-// p; 3 the true-residual vector; 4 CG's p.Ap partials; 5, 6 and 10 the refinement driver's FP32
+// p; 3 the true-residual vector; 4 CG's p.Ap partials; 5, 6 and 8 the refinement driver's FP32
 // right-hand side, FP32 correction and FP64 residual (live across its inner CG solves); 7 host
-// staging; 8, 9 CG's second p buffer and A p when aypx is folded into the apply.
+// staging.
 struct Workspace {
   void *p = nullptr;
   size_t bytes = 0;
 };
-static Workspace g_ws[11];
+static Workspace g_ws[9];
 static SolverState *g_state = nullptr;
 static SolverState *g_state_host = nullptr;
 
@@ -277,26 +277,8 @@ static int cg_device(const CgArgs &a, SolverState &out) {
     if (int rc = ws_get(4, sizeof(double) * static_cast<size_t>(pz_blocks), &w)) return rc;
     pz_partials = static_cast<double *>(w);
   }
-  // p%aypx folded into the next apply: p alternates between two buffers and A p goes to a third,
-  // the same arithmetic in the same order as the separate kernels (beta starts at 0, so the first
-  // apply forms p = 0 p + z = z)
-  const bool fold = fused && c.cg_fuse && cg_aypx_fusable(a.A);
-  void *pb = nullptr, *ap = nullptr;
-  if (fold) {
-    if (int rc = ws_get(8, vb, &pb)) return rc;
-    if (int rc = ws_get(9, vb, &ap)) return rc;
-  }
-  void *pcur = p, *pnext = pb;
   for (int ch = 0; ch < max_chunks; ++ch) {
     for (int i = 0; i < every; ++i) {
-      if (fold) {
-        if (int rc = launch_cg_apply_aypx(a.A, pcur, z, pnext, ap, pz_partials, &g_state->done, &g_state->beta))
-          return rc;
-        if (int rc = launch_cg_finish_pz(pz_partials, pz_blocks, g_state)) return rc;
-        if (int rc = launch_cg_update(a.prec, a.x, r, z, pnext, a.diag, a.n, g_state, ap)) return rc;
-        std::swap(pcur, pnext);
-        continue;
-      }
       if (fused) {
         if (int rc = launch_lma_apply_gather_dot(a.A, p, z, pz_partials, &g_state->done)) return rc;
         if (int rc = launch_cg_finish_pz(pz_partials, pz_blocks, g_state)) return rc;
@@ -391,12 +373,6 @@ int lfric_set_apply_form(int form) {
   if (int rc = require_init()) return rc;
   LF_CHECK(form == 0 || form == 1, LFRIC_E_ARG, "apply form must be 0 (colour-ordered scatter) or 1 (gather)");
   ctx().apply_form = form;
-  return LFRIC_OK;
-}
-
-int lfric_set_cg_fusion(int on) {
-  if (int rc = require_init()) return rc;
-  ctx().cg_fuse = on != 0;
   return LFRIC_OK;
 }
 
@@ -1019,7 +995,7 @@ int lfric_ir_cg_solve(const lfric_op *A64, const lfric_op *A32, const lfric_prec
   if (int rc = prec_diag_ptr(P32, A32, &diag)) return rc;
   const long long n = x->n;
   void *r64, *b32, *d32;
-  if (int rc = ws_get(10, 8 * static_cast<size_t>(n), &r64)) return rc;
+  if (int rc = ws_get(8, 8 * static_cast<size_t>(n), &r64)) return rc;
   if (int rc = ws_get(5, 4 * static_cast<size_t>(n), &b32)) return rc;
   if (int rc = ws_get(6, 4 * static_cast<size_t>(n), &d32)) return rc;
   // r = b - A x in FP64 and its norm

@@ -39,7 +39,6 @@ struct Context {
   double *h_pinned = nullptr;
   long long bytes = 0;
   int apply_form = 1;  // LMA apply inside the solvers: 0 colour-ordered scatter, 1 gather (+ fused dot)
-  int cg_fuse = 1;     // CG: fold p%aypx into the next apply where the operator allows it
 };
 Context &ctx();
 int require_init();

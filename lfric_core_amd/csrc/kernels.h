@@ -78,12 +78,8 @@ int launch_cg_init_norm(int prec, const void *r, long long n, SolverState *st);
 int launch_cg_init_prec(int prec, const void *r, void *z, void *p, const void *diag, long long n,
                         SolverState *st);
 int launch_cg_dot_pz(int prec, const void *p, const void *z, long long n, SolverState *st);
-int launch_cg_update(int prec, void *x, void *r, void *z, const void *p, const void *diag, long long n,
-                     SolverState *st, const void *ap = nullptr);
-// CG's aypx folded into the apply (lowest-order W0 operators with a row-gathered copy)
-bool cg_aypx_fusable(const lfric_op *op);
-int launch_cg_apply_aypx(const lfric_op *op, const void *pold, const void *z, void *pnew, void *ap, double *partials,
-                         const int *skip, const double *beta);
+int launch_cg_update(int prec, void *x, void *r, void *z, const void *p, const void *diag,
+                     long long n, SolverState *st);
 int launch_cg_aypx(int prec, void *p, const void *z, long long n, SolverState *st);
 int launch_cg_finish_pz(const double *partials, int nblocks, SolverState *st);
 

@@ -704,24 +704,14 @@ __global__ __launch_bounds__(kBlock) void k_build_rows(const T *__restrict__ A, 
 // same dof column (the copy is only built when the dofmap says so), so the x entries of a
 // cell's two rows are its 4 columns at levels m-1, m and m+1, and each is loaded once.  The
 // copy is read once per apply: its loads are nontemporal, so the stream does not evict x.
-template <typename T, bool DOT, bool AYPX = false>
+template <typename T, bool DOT>
 __global__ __launch_bounds__(kBlock) void k_lma_gather_rows_w0(const T *__restrict__ rows, size_t gsr,
                                                                const T *__restrict__ x, T *__restrict__ y,
                                                                const int *__restrict__ map2, const int2 *__restrict__ cols,
                                                                const int *__restrict__ off, const int *__restrict__ ent,
                                                                int ncols, int nl, double *partials, const int *skip,
-                                                               int xwin, const T *__restrict__ zsrc = nullptr,
-                                                               T *__restrict__ pnew = nullptr,
-                                                               const double *__restrict__ beta = nullptr) {
+                                                               int xwin) {
   if (skip && *skip) return;
-  // AYPX: the vector applied is p = beta p_old + z (CG's p%aypx(beta, z), contracted as the
-  // separate kernel is), formed wherever it is read from x = p_old and zsrc = z; each thread also
-  // stores its own dof's p into pnew.  Every read of one dof forms the same value.
-  const T bta = AYPX ? static_cast<T>(*beta) : T(0);
-  auto xat = [&](size_t i) -> T {
-    if constexpr (AYPX) return fma(bta, x[i], zsrc[i]);
-    else return x[i];
-  };
   constexpr int V = chunk_entries<T>(), RS = rows_rs<T, 8>();
   typedef T VecT __attribute__((ext_vector_type(V)));
   const int H = nl + 1;
@@ -749,9 +739,9 @@ __global__ __launch_bounds__(kBlock) void k_lma_gather_rows_w0(const T *__restri
       T xl[3][4];  // x of the cell's 4 columns at levels m-1, m, m+1
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
-        xl[0][j] = has_p ? xat(static_cast<size_t>(b[j]) + m - 1) : T(0);
-        xl[1][j] = xat(static_cast<size_t>(b[j]) + m);
-        xl[2][j] = has_m ? xat(static_cast<size_t>(b[j]) + m + 1) : T(0);
+        xl[0][j] = has_p ? x[b[j] + m - 1] : T(0);
+        xl[1][j] = x[b[j] + m];
+        xl[2][j] = has_m ? x[b[j] + m + 1] : T(0);
       }
 #pragma unroll
       for (int h = 0; h < 2; ++h) {
@@ -770,12 +760,8 @@ __global__ __launch_bounds__(kBlock) void k_lma_gather_rows_w0(const T *__restri
     }
     y[static_cast<size_t>(ci.x) + m] = acc;
   }
-  T own = T(0);
-  if (valid && (DOT || AYPX)) own = xat(static_cast<size_t>(ci.x) + m);
-  if constexpr (AYPX)
-    if (valid) pnew[static_cast<size_t>(ci.x) + m] = own;
   if constexpr (DOT) {
-    double v[1] = {valid ? to_d(acc) * to_d(own) : 0.0};
+    double v[1] = {valid ? to_d(acc) * to_d(x[static_cast<size_t>(ci.x) + m]) : 0.0};
     block_partials<1>(v, partials);
   }
 }
@@ -1323,11 +1309,9 @@ __global__ __launch_bounds__(kBlock) void k_cg_dot_pz(const T *__restrict__ p, c
 }
 
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_cg_update(T *__restrict__ x, T *__restrict__ r, T *z,
+__global__ __launch_bounds__(kBlock) void k_cg_update(T *__restrict__ x, T *__restrict__ r, T *__restrict__ z,
                                                       const T *__restrict__ p, const T *__restrict__ diag,
-                                                      long long n, SolverState *st, double *partials,
-                                                      const T *ap) {
-  // ap: A p (null: it is in z, which the kernel then overwrites with the preconditioned residual)
+                                                      long long n, SolverState *st, double *partials) {
   if (st->done) return;
   const T a = static_cast<T>(st->alpha);
   const T ma = static_cast<T>(-st->alpha);
@@ -1341,7 +1325,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_update(T *__restrict__ x, T *__re
       xv[u] = ok ? x[i] : T(0);
       pv[u] = ok ? p[i] : T(0);
       rv[u] = ok ? r[i] : T(0);
-      zv[u] = ok ? (ap ? ap[i] : z[i]) : T(0);
+      zv[u] = ok ? z[i] : T(0);
       dv[u] = (ok && diag) ? diag[i] : T(1);
     }
 #pragma unroll
@@ -1455,7 +1439,7 @@ template <typename T>
 __global__ void k_cg_aypx(T *__restrict__ p, const T *__restrict__ z, long long n, SolverState *st) {
   if (st->done) return;
   const T b = static_cast<T>(st->beta);
-  GRID_STRIDE(i, n) p[i] = fma(b, p[i], z[i]);  // p%aypx(beta,z)
+  GRID_STRIDE(i, n) p[i] = b * p[i] + z[i];  // p%aypx(beta,z)
 }
 
 // ---------------------------------------------------------------------------------------
@@ -2322,33 +2306,6 @@ int launch_lma_apply_solver(const lfric_op *op, const void *x, void *y, const in
   return ctx().apply_form == 1 ? gather_any(op, x, y, GatherExtra{nullptr, skip}) : launch_lma_apply(op, x, y, skip);
 }
 
-bool cg_aypx_fusable(const lfric_op *op) {
-  return op->rows != nullptr && op->sym == nullptr && op->to == op->from && (op->precision == LFRIC_R64 ||
-                                                                               op->precision == LFRIC_R32);
-}
-
-// ap = A p with p = beta p_old + z formed on the fly (p stored into pnew), and the partial sums
-// of p.ap: CG's aypx folded into the next apply over the row-gathered copy.
-int launch_cg_apply_aypx(const lfric_op *op, const void *pold, const void *z, void *pnew, void *ap, double *partials,
-                         const int *skip, const double *beta) {
-  LF_CHECK(cg_aypx_fusable(op), LFRIC_E_UNSUPPORTED, "fused CG apply: row-gathered copy needed");
-  if (int rc = build_gather(*op->to)) return rc;
-  const FunctionSpace &to = *op->to, &from = *op->from;
-  const int grid = gather_blocks(op);
-  if (op->precision == LFRIC_R64)
-    LAUNCH((k_lma_gather_rows_w0<double, true, true>), grid, static_cast<const double *>(op->rows), op->gsr,
-           static_cast<const double *>(pold), static_cast<double *>(ap), from.d_map, to.d_gcol, to.d_goff, to.d_gent,
-           to.gather_ncols, to.nlayers, partials, skip, op->xwin, static_cast<const double *>(z),
-           static_cast<double *>(pnew), beta);
-  else
-    LAUNCH((k_lma_gather_rows_w0<float, true, true>), grid, static_cast<const float *>(op->rows), op->gsr,
-           static_cast<const float *>(pold), static_cast<float *>(ap), from.d_map, to.d_gcol, to.d_goff, to.d_gent,
-           to.gather_ncols, to.nlayers, partials, skip, op->xwin, static_cast<const float *>(z),
-           static_cast<float *>(pnew), beta);
-  CHECK_LAUNCH();
-  return LFRIC_OK;
-}
-
 int launch_gather_blocks(const lfric_op *op, int *nblocks) {
   if (int rc = build_gather(*op->to)) return rc;
   *nblocks = gather_blocks(op);
@@ -2616,15 +2573,13 @@ int launch_cg_finish_pz(const double *partials, int nblocks, SolverState *st) {
 }
 
 int launch_cg_update(int prec, void *x, void *r, void *z, const void *p, const void *diag, long long n,
-                     SolverState *st, const void *ap) {
+                     SolverState *st) {
   if (prec == LFRIC_R64)
     LAUNCH(k_cg_update<double>, kRedBlocks, static_cast<double *>(x), static_cast<double *>(r), static_cast<double *>(z),
-           static_cast<const double *>(p), static_cast<const double *>(diag), n, st, ctx().d_partials,
-           static_cast<const double *>(ap));
+           static_cast<const double *>(p), static_cast<const double *>(diag), n, st, ctx().d_partials);
   else
     LAUNCH(k_cg_update<float>, kRedBlocks, static_cast<float *>(x), static_cast<float *>(r), static_cast<float *>(z),
-           static_cast<const float *>(p), static_cast<const float *>(diag), n, st, ctx().d_partials,
-           static_cast<const float *>(ap));
+           static_cast<const float *>(p), static_cast<const float *>(diag), n, st, ctx().d_partials);
   LAUNCH((k_finish<2, kEpiUpdate>), 1, ctx().d_partials, kRedBlocks, st, static_cast<double *>(nullptr));
   CHECK_LAUNCH();
   return LFRIC_OK;

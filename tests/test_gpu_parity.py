@@ -399,31 +399,3 @@ def test_symmetric_packed_apply_equals_colour_apply(hip, orc, n, nl, seg, prec):
     op.apply_gather(x, y1)
     nonsym.apply_gather(x, y2)
     np.testing.assert_array_equal(y1.export_raw(), y2.export_raw())
-
-
-def test_cg_aypx_folded_into_apply_is_bit_identical(hip):
-    """CG over the row-gathered copy with p%aypx folded into the next apply gives the same
-    iterates bit for bit as the separate aypx kernel: the same solution and iteration count, FP64,
-    FP32 and mixed precision."""
-    fs = hip.FunctionSpace.cubedsphere(16, 12)
-    op64 = hip.LmaOperator(fs).generate_spd(0, scaled=True)
-    op32 = op64.convert(hip.R32)
-    op64.prepare_rows()
-    op32.prepare_rows()
-    b = hip.FieldVector(fs).fill_uniform(0, 3)
-    out = {}
-    try:
-        for fold in (False, True):
-            hip.set_cg_fusion(fold)
-            prec64 = hip.DiagonalPreconditioner.from_operator(op64)
-            prec32 = hip.DiagonalPreconditioner.from_operator(op32)
-            x64, xm = hip.FieldVector(fs), hip.FieldVector(fs)
-            r64 = hip.ConjugateGradient(op64, prec64, 1e-10, 0.0, 5000, check_every=4).apply(x64, b)
-            rm = hip.MixedPrecisionCG(op64, op32, prec32, 1e-10, 0.0, 5000, inner_r_tol=1e-4, check_every=4).apply(xm, b)
-            out[fold] = (x64.export_field(), r64.iterations, xm.export_field(), rm.iterations, rm.outer_iterations)
-    finally:
-        hip.set_cg_fusion(True)
-    a, f = out[False], out[True]
-    np.testing.assert_array_equal(f[0], a[0])
-    np.testing.assert_array_equal(f[2], a[2])
-    assert f[1] == a[1] and f[3:] == a[3:], (f[1:], a[1:])
