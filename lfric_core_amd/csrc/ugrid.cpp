@@ -174,6 +174,7 @@ int parse(const std::string &path, NcFile &f) {
       NcVar v;
       const std::string nm = r.name();
       const uint32_t nd = r.u32();
+      if (nd > 64) return fail(LFRIC_E_ARG, "UGRID: bad variable " + nm + " in " + path);
       for (uint32_t k = 0; k < nd; ++k) v.dims.push_back(static_cast<int>(r.u32()));
       if (!read_attrs(r, v.attrs)) return fail(LFRIC_E_ARG, "UGRID: bad attributes of " + nm);
       v.type = static_cast<int>(r.u32());
@@ -193,6 +194,8 @@ int read_connectivity(const NcFile &f, const std::string &name, std::vector<int>
   LF_CHECK(it != f.vars.end(), LFRIC_E_ARG, "UGRID: variable " + name + " not found");
   const NcVar &v = it->second;
   LF_CHECK(v.type == kNcInt && v.dims.size() == 2, LFRIC_E_UNSUPPORTED, "UGRID: " + name + " is not an int (n, 4) array");
+  for (int d : v.dims)
+    LF_CHECK(d >= 0 && static_cast<size_t>(d) < f.dims.size(), LFRIC_E_ARG, "UGRID: " + name + " names no dimension");
   const uint64_t n0 = f.dims[v.dims[0]].second, n1 = f.dims[v.dims[1]].second;
   LF_CHECK(n1 == 4 && n0 > 0, LFRIC_E_UNSUPPORTED, "UGRID: " + name + " is not (n, 4) (quadrilateral faces)");
   LF_CHECK(v.begin + n0 * n1 * 4 <= f.bytes.size(), LFRIC_E_ARG, "UGRID: " + name + " runs past the end of the file");

@@ -110,3 +110,30 @@ def test_fullsize_helmholtz_gmres_bf16_block_jacobi(hip):
     assert res64.status == 0
     xm, xf = x.export_field(), x64.export_field()
     assert np.linalg.norm(xm - xf) <= 1e-6 * np.linalg.norm(xf)
+
+
+def test_fullsize_columnwise_solve_inverts_apply(hip):
+    """A tridiagonal columnwise operator on every W3 column of C96x70 (55,296 columns of 70
+    rows): the Thomas solve undoes the apply (FP64 to 1e-13, FP32 to 1e-5 relative), and the
+    FP32 and FP64 solves agree to FP32 rounding."""
+    w3 = hip.FunctionSpace.cubedsphere(96, 70, hip.W3, 0)
+    rng = np.random.default_rng(21)
+    bands = rng.uniform(-1, 1, (w3.ncells, w3.nlayers, 3))
+    bands[:, :, 1] += 4.0
+    xs = {}
+    for P, tol in ((hip.R64, 1e-13), (hip.R32, 1e-5)):
+        A = hip.ColumnwiseOperator(w3, w3, P, custom=(1, 1, 1, 1)).upload(bands)
+        x = hip.FieldVector(w3, P).fill_uniform(5, 3)
+        y, z = hip.FieldVector(w3, P), hip.FieldVector(w3, P)
+        A.apply(x, y)
+        A.apply_inverse(y, z)
+        xv, zv = x.export_field(), z.export_field()
+        assert np.linalg.norm(zv - xv) <= tol * np.linalg.norm(xv)
+        xs[P] = (y.export_field(), A)
+    # the FP32 solve of the FP64 right-hand side rounded to FP32, against the FP64 solve
+    y64 = xs[hip.R64][0]
+    z64, z32 = hip.FieldVector(w3), hip.FieldVector(w3, hip.R32)
+    xs[hip.R64][1].apply_inverse(hip.FieldVector(w3).import_field(y64), z64)
+    xs[hip.R32][1].apply_inverse(hip.FieldVector(w3, hip.R32).import_field(y64), z32)
+    a, b = z64.export_field(), z32.export_field()
+    assert np.linalg.norm(a - b) <= 1e-5 * np.linalg.norm(a)
