@@ -79,7 +79,7 @@ class Reader {
   void skip(uint64_t n) {
     if (need(n)) p_ += n;
   }
-  const unsigned char *here() const { return &b_[p_]; }
+  const unsigned char *here() const { return b_.data() + p_; }
   bool need(uint64_t n) {
     if (!ok_ || p_ + n > b_.size()) ok_ = false;
     return ok_;
