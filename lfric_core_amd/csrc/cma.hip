@@ -241,7 +241,7 @@ __host__ __device__ inline int staged_stride(int n) { return n | 1; }
 // times.  n <= 2 * kColBlock.
 constexpr int kStageBatch = 16;
 
-template <typename T>
+template <typename T, int B = kStageBatch>
 __device__ inline void stage_columns_in(T *__restrict__ xs, int S, const T *__restrict__ x,
                                         const int *__restrict__ map, int ndf, const int *__restrict__ ind,
                                         int n, int pos0, int nc, int cfirst = 0, int clast = kColBlock) {
@@ -250,17 +250,17 @@ __device__ inline void stage_columns_in(T *__restrict__ xs, int S, const T *__re
   const int r0 = lane, r1 = lane + kColBlock;
   const int o0 = r0 < n ? ind[r0] : 0, o1 = r1 < n ? ind[r1] : 0;
   const int ncol = min(kColBlock, nc - pos0);
-  for (int c0 = cfirst; c0 < clast; c0 += kStageBatch) {
-    T v0[kStageBatch], v1[kStageBatch];
+  for (int c0 = cfirst; c0 < clast; c0 += B) {
+    T v0[B], v1[B];
 #pragma unroll
-    for (int u = 0; u < kStageBatch; ++u) {
+    for (int u = 0; u < B; ++u) {
       const int base = __shfl(mybase, c0 + u, kColBlock);
       const bool ok = c0 + u < ncol;
       v0[u] = ok && r0 < n ? x[static_cast<size_t>(base) + o0] : T(0);
       v1[u] = ok && r1 < n ? x[static_cast<size_t>(base) + o1] : T(0);
     }
 #pragma unroll
-    for (int u = 0; u < kStageBatch; ++u) {
+    for (int u = 0; u < B; ++u) {
       if (r0 < n) xs[(c0 + u) * S + r0] = v0[u];
       if (r1 < n) xs[(c0 + u) * S + r1] = v1[u];
     }
@@ -299,9 +299,19 @@ __global__ __launch_bounds__(kColBlock) void k_cma_appinv_lds(const T *__restric
   const int nrow = p.nrow, S = staged_stride(nrow);
   T *cs = xs + kColBlock * S;
   const int pos0 = blockIdx.x * kColBlock;
-  stage_columns_in(xs, S, x, map, ndf, ind, nrow, pos0, nc);
-  __syncthreads();
   const int pos = pos0 + threadIdx.x;
+  // the first chunk of bands is requested before the staging, so the two overlap
+  T a1[CH], a2[CH], a3[CH];
+#pragma unroll
+  for (int u = 0; u < CH; ++u) {
+    const int i = 1 + u;
+    if (i > nrow || pos >= nc) continue;
+    a1[u] = C[cidx(p, nc, i, 1, pos)];
+    a2[u] = C[cidx(p, nc, i, 2, pos)];
+    a3[u] = C[cidx(p, nc, i, 3, pos)];
+  }
+  stage_columns_in<T, sizeof(T) == 4 ? 32 : kStageBatch>(xs, S, x, map, ndf, ind, nrow, pos0, nc);
+  __syncthreads();
   if (pos < nc) {
     T *xc = xs + threadIdx.x * S, *cc = cs + threadIdx.x * S;
     auto cw_at = [&](int i) -> T & {
@@ -310,14 +320,15 @@ __global__ __launch_bounds__(kColBlock) void k_cma_appinv_lds(const T *__restric
     };
     T cprev = T(0), dprev = T(0);
     for (int i0 = 1; i0 <= nrow; i0 += CH) {
-      T a1[CH], a2[CH], a3[CH];
+      if (i0 > 1) {
 #pragma unroll
-      for (int u = 0; u < CH; ++u) {
-        const int i = i0 + u;
-        if (i > nrow) continue;
-        a1[u] = C[cidx(p, nc, i, 1, pos)];
-        a2[u] = C[cidx(p, nc, i, 2, pos)];
-        a3[u] = C[cidx(p, nc, i, 3, pos)];
+        for (int u = 0; u < CH; ++u) {
+          const int i = i0 + u;
+          if (i > nrow) continue;
+          a1[u] = C[cidx(p, nc, i, 1, pos)];
+          a2[u] = C[cidx(p, nc, i, 2, pos)];
+          a3[u] = C[cidx(p, nc, i, 3, pos)];
+        }
       }
 #pragma unroll
       for (int u = 0; u < CH; ++u) {
