@@ -150,6 +150,25 @@ def assembly_report(L, fs, nqp=3):
         tf = flops / (ms * 1e-3) / 1e12
         out[name] = {"ms": ms, "tflops": tf, "frac_fp64": tf / FP64_VECTOR_PEAK_TFLOPS,
                      "bytes": wbytes, "gbs": wbytes / (ms * 1e-3) / 1e9}
+    # the CPU restatement of the same kernels (the reference's assembly kernels need generated
+    # configuration modules and are not compiled here), one host core, on a sample of columns
+    try:
+        from oracle import oracle as O
+        ncs = 1024
+        dm = fs.dofmap()[:ncs]
+        chi_h = [rng.uniform(0, 1, fs.undf) for _ in range(3)]
+        cpu = {}
+        for name, fn in (("mass_w0", O.mass_matrix_w0), ("mass_w2", O.mass_matrix_w2),
+                         ("mass_w3", O.mass_matrix_w3), ("div", O.div_operator)):
+            t = time.perf_counter()
+            fn(ncs, fs.nlayers, dm, *chi_h, nqp=nqp)
+            dt = time.perf_counter() - t
+            cpu[name] = {"ms_full_mesh": dt * 1e3 * fs.ncells / ncs}
+        out["cpu_port"] = {"kind": "port", "cores": 1, "timings": cpu,
+                           "sample": f"{ncs} of {fs.ncells} columns ({fs.nlayers} layers) with the oracle's C restatement, "
+                                     f"scaled to the whole mesh"}
+    except Exception as e:  # the oracle library is test infrastructure; its absence only drops this field
+        out["cpu_port"] = {"error": str(e)}
     return out
 
 
