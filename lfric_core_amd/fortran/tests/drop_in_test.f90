@@ -22,8 +22,14 @@ program drop_in_test
   type(bicgstab_type) :: bs
   type(hip_factory_solver_type) :: fbs
   real(r_def), allocatable :: a1(:), a2(:), a3(:), a4(:), a5(:)
-  real(r_def) :: d_cg, d_gm, d_bs
+  real(r_def) :: d_cg, d_gm, d_bs, d_cma
   integer :: n, nl
+  ! columnwise (CMA) path on W3: an operator assembled from an LMA W3 operator, its apply undone
+  ! by its tridiagonal solve, and the solve used as a preconditioner
+  type(hip_function_space_type) :: fs3
+  type(c_ptr) :: op3, cma, pcma
+  type(hip_field_vector_type) :: x3, y3, z3
+  real(r_def), allocatable :: c1(:), c2(:)
 
   n = 12
   nl = 20
@@ -79,4 +85,29 @@ program drop_in_test
       ', "hip_bicgstab_iterations": ', fbs%result%iterations, ', "stock_vs_hip_bicgstab_rel_diff": ', d_bs, '}'
   if (cg%iterations /= hcg%result%iterations .or. d_cg > 1.0e-10_r_def .or. d_gm > 1.0e-7_r_def) error stop 2
   if (bs%iterations /= fbs%result%iterations .or. d_bs > 1.0e-9_r_def) error stop 3
+
+  call fs3%init_cubedsphere(n, nl, int(LFRIC_W3, i_def), 0)
+  call hip_check(lfric_op_create(fs3%h, fs3%h, LFRIC_R64, op3), "lfric_op_create W3")
+  call hip_check(lfric_op_generate_spd(op3, 1_c_long_long, 0_c_int, 0.5_c_double), "generate W3")
+  call hip_check(lfric_cma_create_custom(fs3%h, fs3%h, LFRIC_R64, 1_c_int, 1_c_int, 1_c_int, 1_c_int, cma), &
+                 "lfric_cma_create_custom")
+  call hip_check(lfric_cma_assemble(cma, op3), "lfric_cma_assemble")
+  call x3%initialise(fs3)
+  call y3%initialise(fs3)
+  call z3%initialise(fs3)
+  call hip_check(lfric_vec_fill_uniform(x3%h, 4_c_long_long, 3_c_long_long), "x3")
+  call hip_check(lfric_cma_apply(cma, x3%h, y3%h, 0_c_int), "lfric_cma_apply")
+  call hip_check(lfric_cma_apply_inverse(cma, y3%h, z3%h), "lfric_cma_apply_inverse")
+  allocate(c1(fs3%undf), c2(fs3%undf))
+  call x3%export_field(c1)
+  call z3%export_field(c2)
+  d_cma = norm2(c2 - c1) / norm2(c1)
+  call hip_check(lfric_prec_create_cma_inverse(cma, pcma), "lfric_prec_create_cma_inverse")
+  call hip_check(lfric_prec_apply(pcma, y3%h, z3%h), "lfric_prec_apply")
+  call z3%export_field(c2)
+  write(*, '(A,ES12.5,A,ES12.5,A)') '{"cma_solve_undoes_apply_rel_diff": ', d_cma, &
+      ', "cma_preconditioner_rel_diff": ', norm2(c2 - c1) / norm2(c1), '}'
+  if (d_cma > 1.0e-13_r_def .or. norm2(c2 - c1) / norm2(c1) > 1.0e-13_r_def) error stop 4
+  call lfric_prec_destroy(pcma)
+  call lfric_cma_destroy(cma)
 end program drop_in_test

@@ -1,7 +1,7 @@
 """The Fortran drop-in layer on the GPU: the stock conjugate_gradient_type / gmres_type run on
 the HIP vector, operator and preconditioner types, agree with the device-resident solve and
 with the oracle's iteration count (program lfric_core_amd/fortran/tests/drop_in_test.f90,
-built by __graft_entry__.build())."""
+built by __graft_entry__.build()), and the columnwise-operator bindings work from Fortran."""
 import json
 import subprocess
 
@@ -17,8 +17,10 @@ def test_fortran_drop_in(orc):
     assert exe.exists(), "Fortran drop-in test not built"
     out = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120)
     assert out.returncode == 0, out.stdout + out.stderr
-    line = [l for l in out.stdout.splitlines() if l.startswith("{")][-1]
-    r = json.loads(line)
+    r = {}
+    for line in out.stdout.splitlines():
+        if line.startswith("{"):
+            r.update(json.loads(line))
     m = orc.Mesh(12, 20)
     A = orc.gen_spd(8, m.ncell_3d, 0)
     _, reso, _ = orc.cg(m, A, orc.gen_uniform(m.undf, 0, 3), diag=orc.mm_diagonal(m, A))
@@ -31,3 +33,7 @@ def test_fortran_drop_in(orc):
                                       r_tol=1e-10, max_iter=300)
     assert r["stock_bicgstab_iterations"] == r["hip_bicgstab_iterations"] == resb.iterations
     assert r["stock_vs_hip_bicgstab_rel_diff"] <= 1e-9
+    # the columnwise bindings from Fortran: a W3 operator's tridiagonal solve undoes its apply,
+    # directly and through the preconditioner wrapper
+    assert r["cma_solve_undoes_apply_rel_diff"] <= 1e-13
+    assert r["cma_preconditioner_rel_diff"] <= 1e-13
