@@ -185,7 +185,8 @@ int lfric_op_assemble_mass_w0(lfric_op *op, const lfric_vec *chi1, const lfric_v
  *   W2 (6 dofs)  compute_mass_matrix_w2_code (components/science/source/kernel/fem/
  *                sci_compute_mass_matrix_kernel_w2_mod.F90:83-164), Piola-mapped face basis;
  *   W3 (1 dof)   compute_mass_matrix_w3_code (sci_compute_mass_matrix_kernel_w3_mod.F90:270-356),
- *                rehabilitate selecting dj (nonzero, the reference's default) or 1/dj.
+ *                rehabilitate selecting dj (nonzero, the reference's default) or 1/dj;
+ *   Wtheta (2)   compute_mass_matrix_w_scalar_code with the Wtheta basis (bottom 1 - z, top z).
  * The space is recognised from its dofmap's column roles; anything else is refused. */
 int lfric_op_assemble_mass(lfric_op *op, const lfric_vec *chi1, const lfric_vec *chi2, const lfric_vec *chi3, int nqp,
                            int rehabilitate);

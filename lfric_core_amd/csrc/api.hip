@@ -744,7 +744,7 @@ int lfric_op_assemble_mass(lfric_op *op, const lfric_vec *chi1, const lfric_vec 
              "assemble: W0 coordinates must share the operator's dofmap");
     return refresh_rows(op, launch_assemble_mass_w0(op, c1, c2, c3, nqp));
   }
-  const int kind = f.ndf == 1 ? 0 : 1;  // W3, else W2 (the launcher checks the roles)
+  const int kind = f.ndf == 1 ? 0 : f.ndf == 2 ? 3 : 1;  // W3, Wtheta, else W2 (the launcher checks the roles)
   return refresh_rows(op, launch_assemble_fem(op, kind, *chi1->fs, c1, c2, c3, nqp, rehabilitate != 0));
 }
 

@@ -286,7 +286,7 @@ __global__ __launch_bounds__(kBlock) void k_assemble_mass_w0(double *__restrict_
 __device__ constexpr int kW2Comp[6] = {0, 1, 0, 1, 2, 2};
 __device__ constexpr double kW2Div[6] = {-1.0, 1.0, 1.0, -1.0, -1.0, 1.0};
 
-enum FemKind { kFemMassW3 = 0, kFemMassW2 = 1, kFemDiv = 2 };
+enum FemKind { kFemMassW3 = 0, kFemMassW2 = 1, kFemDiv = 2, kFemMassWtheta = 3 };
 
 // W3 and W2 mass matrices and the weak divergence W2 -> W3 of every cell from Cartesian
 // coordinate fields on a W0 space of the same mesh (map: that space's [pos][8] column-base
@@ -298,7 +298,10 @@ enum FemKind { kFemMassW3 = 0, kFemMassW2 = 1, kFemDiv = 2 };
 //               entries df >= df2 summed from wh*wv*dot_product(J b_df, J b_df2)/dj, mirrored;
 //               J b is v*J(:, c) for a basis vector whose one nonzero component v sits in c;
 //   kFemDiv     compute_div_operator_code (sci_compute_div_operator_kernel_mod.F90:120-150):
-//               sums of wh*wv*1*div(b_df2), over dj without rehabilitation.
+//               sums of wh*wv*1*div(b_df2), over dj without rehabilitation;
+//   kFemMassWtheta  compute_mass_matrix_w_scalar_code (sci_compute_mass_matrix_kernel_w_scalar_mod.F90:
+//               290-384) with the Wtheta basis 1 - z, z (the W2 B and T factors): entries df >= df2
+//               summed from wh*wv*b_df*b_df2*dj, mirrored.
 // Quadrature points in the reference's order (vertical outer, horizontal inner), one thread per
 // 3D cell, no contraction: each entry is the reference's sum in the reference's order.  Tables:
 // coordinate-basis derivatives [3][8][NP], weights [NH] and [NQ], W2 values [6][NP].
@@ -330,7 +333,7 @@ __global__ __launch_bounds__(kBlock) void k_assemble_fem(double *__restrict__ A,
     c[1][df] = chi2[d];
     c[2][df] = chi3[d];
   }
-  constexpr int NS = KIND == kFemMassW3 ? 1 : KIND == kFemMassW2 ? 21 : 6;
+  constexpr int NS = KIND == kFemMassW3 ? 1 : KIND == kFemMassW2 ? 21 : KIND == kFemMassWtheta ? 3 : 6;
   double s[NS];
 #pragma unroll
   for (int t = 0; t < NS; ++t) s[t] = 0.0;
@@ -350,6 +353,11 @@ __global__ __launch_bounds__(kBlock) void k_assemble_fem(double *__restrict__ A,
       const double w = wh[ic] * wv[kz];
       if constexpr (KIND == kFemMassW3) {
         s[0] = s[0] + (rehab ? w * dj : w / dj);
+      } else if constexpr (KIND == kFemMassWtheta) {
+        const double bq[2] = {w2[4 * NP + q], w2[5 * NP + q]};
+        s[0] = s[0] + w * bq[0] * bq[0] * dj;
+        s[1] = s[1] + w * bq[1] * bq[0] * dj;
+        s[2] = s[2] + w * bq[1] * bq[1] * dj;
       } else if constexpr (KIND == kFemDiv) {
 #pragma unroll
         for (int d = 0; d < 6; ++d) {
@@ -376,6 +384,12 @@ __global__ __launch_bounds__(kBlock) void k_assemble_fem(double *__restrict__ A,
     }
   if constexpr (KIND == kFemMassW3) {
     A[aidx<double>(gs, el, 0)] = s[0];
+  } else if constexpr (KIND == kFemMassWtheta) {
+    const long long e0 = el + (row_top & 1u), e1 = el + ((row_top >> 1) & 1u);
+    A[aidx<double>(gs, e0, 0)] = s[0];
+    A[aidx<double>(gs, e0, 1)] = s[1];
+    A[aidx<double>(gs, e1, 2)] = s[1];
+    A[aidx<double>(gs, e1, 3)] = s[2];
   } else if constexpr (KIND == kFemDiv) {
 #pragma unroll
     for (int d = 0; d < 6; ++d) A[aidx<double>(gs, el, d)] = s[d];
@@ -1733,6 +1747,7 @@ static bool is_w3_space(const FunctionSpace &f) { return f.ndf == 1 && f.top_mas
 static bool is_w2_space(const FunctionSpace &f) {
   return f.ndf == 6 && f.top_mask == (1u << 5) && f.partner[4] == 5;
 }
+static bool is_wtheta_space(const FunctionSpace &f) { return f.ndf == 2 && f.top_mask == 2u && f.partner[0] == 1; }
 static bool is_w0_space(const FunctionSpace &f) {
   bool w0 = f.ndf == 8 && f.top_mask == 0xF0u;
   for (int i = 0; w0 && i < 4; ++i) w0 = f.partner[i] == i + 4;
@@ -1751,6 +1766,9 @@ int launch_assemble_fem(lfric_op *op, int kind, const FunctionSpace &chi_fs, con
     LF_CHECK(op->to == op->from && is_w3_space(f), LFRIC_E_UNSUPPORTED, "assemble: a square lowest-order W3 operator is needed");
   else if (kind == kFemMassW2)
     LF_CHECK(op->to == op->from && is_w2_space(f), LFRIC_E_UNSUPPORTED, "assemble: a square lowest-order W2 operator is needed");
+  else if (kind == kFemMassWtheta)
+    LF_CHECK(op->to == op->from && is_wtheta_space(f), LFRIC_E_UNSUPPORTED,
+             "assemble: a square lowest-order Wtheta operator is needed");
   else
     LF_CHECK(is_w3_space(f) && is_w2_space(*op->from) && op->from->cell_of_pos == f.cell_of_pos, LFRIC_E_UNSUPPORTED,
              "assemble: the divergence maps a lowest-order W2 space to a W3 space of the same mesh");
@@ -1767,10 +1785,10 @@ int launch_assemble_fem(lfric_op *op, int kind, const FunctionSpace &chi_fs, con
            f.nlayers, nelem, op->row_top, rehab);                                                               \
     break;
   switch (nqp * 4 + kind) {
-    FEM_CASE(1, 0) FEM_CASE(1, 1) FEM_CASE(1, 2)
-    FEM_CASE(2, 0) FEM_CASE(2, 1) FEM_CASE(2, 2)
-    FEM_CASE(3, 0) FEM_CASE(3, 1) FEM_CASE(3, 2)
-    FEM_CASE(4, 0) FEM_CASE(4, 1) FEM_CASE(4, 2)
+    FEM_CASE(1, 0) FEM_CASE(1, 1) FEM_CASE(1, 2) FEM_CASE(1, 3)
+    FEM_CASE(2, 0) FEM_CASE(2, 1) FEM_CASE(2, 2) FEM_CASE(2, 3)
+    FEM_CASE(3, 0) FEM_CASE(3, 1) FEM_CASE(3, 2) FEM_CASE(3, 3)
+    FEM_CASE(4, 0) FEM_CASE(4, 1) FEM_CASE(4, 2) FEM_CASE(4, 3)
     default:
       dfree(dt, sizeof(double) * t.size());
       return fail(LFRIC_E_ARG, "assemble: unknown operator kind");

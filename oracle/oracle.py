@@ -83,6 +83,7 @@ def lib():
             "orc_w2_basis": (_I, [_I, _VP, _VP]),
             "orc_mass_matrix_w2": (_I, [_I, _I, _VP, _VP, _VP, _VP, _I, _VP]),
             "orc_mass_matrix_w3": (_I, [_I, _I, _VP, _VP, _VP, _VP, _I, _I, _VP]),
+            "orc_mass_matrix_wtheta": (_I, [_I, _I, _VP, _VP, _VP, _VP, _I, _VP]),
             "orc_div_operator": (_I, [_I, _I, _VP, _VP, _VP, _VP, _I, _I, _VP]),
             "orc_cma_params": (_I, [_I, _I, _I, _I, _I, _I, _I, _I, _I, _I, _VP]),
             "orc_cma_dofmaps": (None, [_I, _I, _I, _I, _VP, _VP, _VP]),
@@ -345,6 +346,15 @@ def mass_matrix_w3(ncells, nlayers, dofmap, chi1, chi2, chi3, nqp=3, rehabilitat
                                 _p(out)) != 0:
         raise ValueError("mass_matrix_w3: nqp out of range")
     return out.reshape((-1, 1, 1))
+
+
+def mass_matrix_wtheta(ncells, nlayers, dofmap, chi1, chi2, chi3, nqp=3):
+    """Lowest-order Wtheta mass matrices (ncells*nlayers, 2, 2)."""
+    dm, c = _chi_args(dofmap, chi1, chi2, chi3)
+    out = np.zeros(ncells * nlayers * 4)
+    if lib().orc_mass_matrix_wtheta(ncells, nlayers, _p(dm), _p(c[0]), _p(c[1]), _p(c[2]), nqp, _p(out)) != 0:
+        raise ValueError("mass_matrix_wtheta: nqp out of range")
+    return out.reshape((2, 2, ncells * nlayers)).transpose(2, 1, 0)
 
 
 def div_operator(ncells, nlayers, dofmap, chi1, chi2, chi3, nqp=3, rehabilitate=True):

@@ -41,6 +41,9 @@
  *   orc_mass_matrix_w3        known answer of compute_mass_matrix_kernel_w3_mod_test.pf (volume);
  *                             equals the sum of the cell's W0 mass entries; independent of nqp
  *                             >= 2 on trilinear cells; 1/det(A) without rehabilitation
+ *   orc_mass_matrix_wtheta    known answer of compute_mass_matrix_kernel_wtheta_mod_test.pf
+ *                             (1/3, 1/6 of the volume); det(A) [[1/3,1/6],[1/6,1/3]] on sheared
+ *                             parallelepipeds; entries add up to the W3 entry (the volume)
  *   orc_div_operator          known answer of compute_div_operator_kernel_mod_test.pf
  *                             (-1, 1, 1, -1, -1, 1); without rehabilitation = divergence sign x
  *                             the W3 entry without rehabilitation, bit for bit
@@ -136,6 +139,8 @@ int orc_mass_matrix_w2(int ncells, int nlayers, const int *map, const double *ch
                        const double *chi3, int nqp, double *mm);
 int orc_mass_matrix_w3(int ncells, int nlayers, const int *map, const double *chi1, const double *chi2,
                        const double *chi3, int nqp, int rehabilitate, double *mm);
+int orc_mass_matrix_wtheta(int ncells, int nlayers, const int *map, const double *chi1, const double *chi2,
+                           const double *chi3, int nqp, double *mm);
 int orc_div_operator(int ncells, int nlayers, const int *map, const double *chi1, const double *chi2,
                      const double *chi3, int nqp, int rehabilitate, double *div);
 

@@ -377,6 +377,36 @@ int orc_mass_matrix_w2(int ncells, int nl, const int *map, const double *chi1, c
   return 0;
 }
 
+/* Lowest-order Wtheta mass matrix: compute_mass_matrix_w_scalar_code
+ * (components/science/source/kernel/fem/sci_compute_mass_matrix_kernel_w_scalar_mod.F90:290-384)
+ * with the Wtheta basis (bottom dof 1 - z, top dof z, constant in x and y: the W2 B and T factors
+ * above).  mm in the reference layout (ncell_3d, 2, 2). */
+int orc_mass_matrix_wtheta(int ncells, int nl, const int *map, const double *chi1, const double *chi2,
+                           const double *chi3, int nqp, double *mm) {
+  static orc_tab t;
+  if (orc_tab_init(&t, nqp)) return 1;
+  const int64_t nc3 = (int64_t)ncells * nl;
+  for (int cell = 0; cell < ncells; ++cell)
+    for (int k = 0; k < nl; ++k) {
+      const int64_t ik = (int64_t)cell * nl + k;
+      double c[3][8], J[3][3], s[2][2] = {{0}};
+      orc_cell_coords(&map[(size_t)cell * 8], k, chi1, chi2, chi3, c);
+      for (int kz = 0; kz < nqp; ++kz)
+        for (int ic = 0; ic < t.nh; ++ic) {
+          const double dj = orc_jacobian(&t, c, ic, kz, J);
+          const double b[2] = {orc_w2_value(&t, 4, ic, kz), orc_w2_value(&t, 5, ic, kz)};
+          for (int df2 = 0; df2 < 2; ++df2)
+            for (int df = df2; df < 2; ++df)
+              s[df][df2] = s[df][df2] + t.wh[ic] * t.w[kz] * b[df] * b[df2] * dj;
+        }
+      for (int df2 = 0; df2 < 2; ++df2) {
+        for (int df = df2; df < 2; ++df) mm[ik + nc3 * (df + 2 * (int64_t)df2)] = s[df][df2];
+        for (int df = df2 - 1; df >= 0; --df) mm[ik + nc3 * (df + 2 * (int64_t)df2)] = s[df2][df];
+      }
+    }
+  return 0;
+}
+
 /* Lowest-order W3 mass matrix: compute_mass_matrix_w3_code_r_double
  * (components/science/source/kernel/fem/sci_compute_mass_matrix_kernel_w3_mod.F90:270-356): the
  * one W3 basis function is 1, so the entry is the sum over (qp_v outer, qp_h inner) of

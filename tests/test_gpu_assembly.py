@@ -214,3 +214,34 @@ def test_helmholtz_from_assembled_operators(hip, orc):
     res = hip.GMRES(H, hip.NullPreconditioner(), 30, 1e-12, 0.0, 200).apply(xs, b)
     assert res.status == 0
     np.testing.assert_allclose(xs.export_field(), x, rtol=0, atol=1e-8)
+
+
+def test_assembled_wtheta_mass_bitwise_equals_oracle_and_known_answer(hip, orc):
+    """The Wtheta mass matrix (compute_mass_matrix_w_scalar_code with the Wtheta basis) on the
+    box: bit-equal to the oracle for 2-4 points per direction; 1/3 and 1/6 of the volume on the
+    reference unit test's box cell."""
+    nx, ny, nl = 5, 4, 3
+    dm0, npc, cic, undf0 = _box_mesh(nx, ny, nl)
+    nc = nx * ny
+    dmt = np.array([[c * (nl + 1) + 1, c * (nl + 1) + 2] for c in range(nc)], np.int32)
+    w0 = hip.FunctionSpace.from_dofmap(nl, undf0, dm0, npc, cic)
+    wt = hip.FunctionSpace.from_dofmap(nl, nc * (nl + 1), dmt, npc, cic)
+    chi = _chi_of_box(nx, ny, nl, np.random.default_rng(15))
+    cv = [hip.FieldVector(w0).import_field(c) for c in chi]
+    for nqp in (2, 3, 4):
+        m = hip.LmaOperator(wt).assemble_mass(*cv, nqp=nqp).download()
+        np.testing.assert_array_equal(m, orc.mass_matrix_wtheta(nc, nl, dm0, *chi, nqp=nqp))
+    k = json.loads((GOLDEN / "kat.json").read_text())["mass_matrix_w3"]  # the unit tests' box
+    dx, dy, dz = k["dx"], k["dy"], k["dz"]
+    dm1, npc1, cic1, undf1 = _box_mesh(1, 1, nl)
+    w01 = hip.FunctionSpace.from_dofmap(nl, undf1, dm1, npc1, cic1)
+    wt1 = hip.FunctionSpace.from_dofmap(nl, nl + 1, np.array([[1, 2]], np.int32), npc1, cic1)
+    c1 = np.zeros((3, undf1))
+    for j in range(2):
+        for i in range(2):
+            for kk in range(nl + 1):
+                c1[:, (j * 2 + i) * (nl + 1) + kk] = (i * dx, j * dy, kk * dz)
+    m = hip.LmaOperator(wt1).assemble_mass(*[hip.FieldVector(w01).import_field(c) for c in c1]).download()
+    vol = dx * dy * dz
+    for ik in range(nl):
+        np.testing.assert_allclose(m[ik] / vol, [[1 / 3, 1 / 6], [1 / 6, 1 / 3]], rtol=0, atol=1e-6)

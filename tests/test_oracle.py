@@ -654,3 +654,29 @@ def test_div_operator_known_answer_and_invariants(orc):
     dn = orc.div_operator(1, 1, dmc, chic[0], chic[1], chic[2], nqp=3, rehabilitate=False)[0, 0]
     m3n = orc.mass_matrix_w3(1, 1, dmc, chic[0], chic[1], chic[2], nqp=3, rehabilitate=False)[0, 0, 0]
     np.testing.assert_array_equal(dn, np.array(k["answer"]) * m3n)
+
+
+def test_mass_matrix_wtheta_known_answer_and_invariants(orc):
+    """compute_mass_matrix_kernel_wtheta_mod_test.pf: entries (1,1) and (1,2) of the first two
+    layers of the 6000 x 1000 x 2000 box cell are 1/3 and 1/6 of its volume."""
+    k = json.loads((GOLDEN / "kat.json").read_text())["mass_matrix_w3"]  # the same box and mesh
+    dm, chi = _kat_box_mesh(k)
+    vol = k["dx"] * k["dy"] * k["dz"]
+    mm = orc.mass_matrix_wtheta(9, k["nlayers"], dm, chi[0], chi[1], chi[2], nqp=3)
+    for ik in (0, 1):
+        np.testing.assert_allclose(mm[ik, 0, 0] / vol, 1 / 3, rtol=0, atol=1e-6)
+        np.testing.assert_allclose(mm[ik, 0, 1] / vol, 1 / 6, rtol=0, atol=1e-6)
+    # sheared parallelepiped: det(A) [[1/3, 1/6], [1/6, 1/3]]; a general hexahedron: symmetric,
+    # entries adding up to the W3 entry (the volume)
+    rng = np.random.default_rng(9)
+    A = rng.uniform(-1, 1, (3, 3)) + np.diag([3.0, 2.0, 4.0])
+    X = [A @ np.array([_VX[v], _VY[v], _VZ[v]], float) for v in range(8)]
+    dmc, chic = _one_cell(X)
+    got = orc.mass_matrix_wtheta(1, 1, dmc, *chic, nqp=3)[0]
+    np.testing.assert_allclose(got, np.linalg.det(A) * np.array([[1 / 3, 1 / 6], [1 / 6, 1 / 3]]), rtol=1e-12)
+    Xg = [x + rng.uniform(-0.3, 0.3, 3) for x in X]
+    dmc, chic = _one_cell(Xg)
+    m = orc.mass_matrix_wtheta(1, 1, dmc, *chic, nqp=3)[0]
+    m3 = orc.mass_matrix_w3(1, 1, dmc, *chic, nqp=3)[0, 0, 0]
+    np.testing.assert_array_equal(m, m.T)
+    assert abs(m.sum() - m3) <= 1e-13 * abs(m3)
