@@ -380,8 +380,30 @@ __global__ __launch_bounds__(kColBlock) void k_cma_appinv_lds(const T *__restric
 // work four ways; a block needs kColBlock*(Sx + Sy) values of LDS.
 constexpr int kApplyWaves = 4;
 
+// A wave's rows of bands are loaded kApplyChunk at a time; the first chunk is issued before
+// the fields are staged, so the two sets of loads are in flight together.
+constexpr int kApplyChunk = 18;
+
 template <typename T, int BW>
-__global__ __launch_bounds__(kColBlock * kApplyWaves) void k_cma_apply_lds(
+__device__ inline void load_band_chunk(T (&a)[kApplyChunk][BW], const T *__restrict__ C, const CmaP &p, int nc,
+                                       int pos, int i0, int ilast) {
+#pragma unroll
+  for (int u = 0; u < kApplyChunk; ++u) {
+    const int i = i0 + u;
+    if (i > ilast) continue;
+    const int jm = jminus(p, i), j0 = max(1, jm), j1 = min(p.ncol, jplus(p, i));
+#pragma unroll
+    for (int b = 0; b < BW; ++b) {
+      const int j = jm + b;
+      a[u][b] = (j >= j0 && j <= j1) ? C[cidx(p, nc, i, b + 1, pos)] : T(0);
+    }
+  }
+}
+
+// four blocks of FP32 staging fit a CU's LDS; at most 128 VGPRs keeps all four resident
+template <typename T, int BW>
+__global__ __launch_bounds__(kColBlock * kApplyWaves) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : 2)))
+void k_cma_apply_lds(
     const T *__restrict__ C, CmaP p, int nc, const T *__restrict__ x, T *__restrict__ y,
     const int *__restrict__ map_to, int ndf_to, const int *__restrict__ map_from, int ndf_from,
     const int *__restrict__ ind_to, const int *__restrict__ ind_from, int accumulate) {
@@ -392,30 +414,21 @@ __global__ __launch_bounds__(kColBlock * kApplyWaves) void k_cma_apply_lds(
   const int pos0 = blockIdx.x * kColBlock;
   const int w = threadIdx.x / kColBlock, lane = threadIdx.x & (kColBlock - 1);
   constexpr int kCw = kColBlock / kApplyWaves;
-  stage_columns_in(xs, Sx, x, map_from, ndf_from, ind_from, p.ncol, pos0, nc, w * kCw, (w + 1) * kCw);
-  if (accumulate) stage_columns_in(ys, Sy, y, map_to, ndf_to, ind_to, p.nrow, pos0, nc, w * kCw, (w + 1) * kCw);
-  __syncthreads();
   const int pos = pos0 + lane;
   const int q = (p.nrow + kApplyWaves - 1) / kApplyWaves;
   const int ifirst = 1 + w * q, ilast = min(p.nrow, (w + 1) * q);
+  T a[kApplyChunk][BW];
+  if (pos < nc) load_band_chunk<T, BW>(a, C, p, nc, pos, ifirst, ilast);
+  stage_columns_in(xs, Sx, x, map_from, ndf_from, ind_from, p.ncol, pos0, nc, w * kCw, (w + 1) * kCw);
+  if (accumulate) stage_columns_in(ys, Sy, y, map_to, ndf_to, ind_to, p.nrow, pos0, nc, w * kCw, (w + 1) * kCw);
+  __syncthreads();
   if (pos < nc) {
     const T *xc = xs + lane * Sx;
     T *yc = ys + lane * Sy;
-    for (int i0 = ifirst; i0 <= ilast; i0 += kRowChunk) {
-      T a[kRowChunk][BW];
+    for (int i0 = ifirst; i0 <= ilast; i0 += kApplyChunk) {
+      if (i0 != ifirst) load_band_chunk<T, BW>(a, C, p, nc, pos, i0, ilast);
 #pragma unroll
-      for (int u = 0; u < kRowChunk; ++u) {
-        const int i = i0 + u;
-        if (i > ilast) continue;
-        const int jm = jminus(p, i), j0 = max(1, jm), j1 = min(p.ncol, jplus(p, i));
-#pragma unroll
-        for (int b = 0; b < BW; ++b) {
-          const int j = jm + b;
-          a[u][b] = (j >= j0 && j <= j1) ? C[cidx(p, nc, i, b + 1, pos)] : T(0);
-        }
-      }
-#pragma unroll
-      for (int u = 0; u < kRowChunk; ++u) {
+      for (int u = 0; u < kApplyChunk; ++u) {
         const int i = i0 + u;
         if (i > ilast) continue;
         const int jm = jminus(p, i), j0 = max(1, jm), j1 = min(p.ncol, jplus(p, i));
