@@ -330,6 +330,15 @@ __global__ __launch_bounds__(kColBlock) void k_cma_appinv_lds(const T *__restric
           a3[u] = C[cidx(p, nc, i, 3, pos)];
         }
       }
+      // the chunk's right-hand sides are read from LDS before any of its results are written,
+      // so the recurrence does not wait on an LDS round trip per row
+      T xv[CH];
+#pragma unroll
+      for (int u = 0; u < CH; ++u) {
+        const int i = i0 + u;
+        if (i > nrow) continue;
+        xv[u] = xc[i - 1];
+      }
 #pragma unroll
       for (int u = 0; u < CH; ++u) {
         const int i = i0 + u;
@@ -338,11 +347,11 @@ __global__ __launch_bounds__(kColBlock) void k_cma_appinv_lds(const T *__restric
         if (i == 1) {
           inv = T(1) / a2[u];
           cp = inv * a3[u];
-          dp = inv * xc[i - 1];
+          dp = inv * xv[u];
         } else {
           inv = T(1) / (a2[u] - a1[u] * cprev);
           cp = i < nrow ? inv * a3[u] : T(0);
-          dp = inv * (xc[i - 1] - a1[u] * dprev);
+          dp = inv * (xv[u] - a1[u] * dprev);
         }
         cw_at(i) = cp;
         xc[i - 1] = dp;
@@ -352,18 +361,19 @@ __global__ __launch_bounds__(kColBlock) void k_cma_appinv_lds(const T *__restric
     }
     T dnext = dprev;
     for (int i0 = nrow - 1; i0 >= 1; i0 -= CH) {
-      T cv[CH];
+      T cv[CH], dv[CH];
 #pragma unroll
       for (int u = 0; u < CH; ++u) {
         const int i = i0 - u;
         if (i < 1) continue;
         cv[u] = cw_at(i);
+        dv[u] = xc[i - 1];
       }
 #pragma unroll
       for (int u = 0; u < CH; ++u) {
         const int i = i0 - u;
         if (i < 1) continue;
-        const T d = xc[i - 1] - cv[u] * dnext;
+        const T d = dv[u] - cv[u] * dnext;
         xc[i - 1] = d;
         dnext = d;
       }
@@ -764,9 +774,9 @@ int launch_cma_appinv(const lfric_cma *A, const void *x, void *y) {
                        static_cast<const T *>(A->d), p, A->ncols, static_cast<const T *>(x), static_cast<T *>(y), \
                        f.d_map, f.ndf, A->d_ind_to, static_cast<T *>(A->work));
     if (A->precision == LFRIC_R64) {
-      APPINV_LDS(double, 16)
+      APPINV_LDS(double, 36)
     } else {
-      APPINV_LDS(float, 32)
+      APPINV_LDS(float, 72)
     }
 #undef APPINV_LDS
     return finish("columnwise inverse");
