@@ -105,6 +105,42 @@ def test_cma_kernels_bitwise_equal_oracle(hip, orc, P):
     np.testing.assert_array_equal(y.export_raw(), orc.tri_solve(nl, dm3, x, t0, tp, tm, r32=r32))
 
 
+@pytest.mark.parametrize("P", ["R32", "R64"])
+@pytest.mark.parametrize("nl", [100, 140])
+def test_cma_tall_columns_bitwise_equal_oracle(hip, orc, P, nl):
+    """Columns taller than one chunk of rows (100: the LDS-staged apply and solve take two or
+    three chunks of bands) and taller than the LDS-staged kernels accept (140: the kernels with
+    one thread per column and no staging), over 70 columns (two blocks of 64, the second
+    partial): apply, accumulate and the tridiagonal solve equal the oracle bit for bit."""
+    P = getattr(hip, P)
+    r32 = P == hip.R32
+    T = np.float32 if r32 else np.float64
+    nx, ny = 10, 7
+    nc = nx * ny
+    rng = np.random.default_rng(nl)
+    print("tall columns", nl, "rows", "R32" if r32 else "R64", flush=True)
+    w3, dm3, _, _ = _spaces(hip, nx, ny, nl)
+    _, i3 = _orc_space(orc, nl, dm3, 1, 0)
+    pc = orc.cma_params(nl, 1, 0, 1, 0, custom=(1, 1, 1, 1))
+    S = hip.ColumnwiseOperator(w3, w3, P, custom=(1, 1, 1, 1))
+    bands = rng.uniform(-1, 1, (nc, pc[5], pc[4])).astype(np.float32).astype(np.float64)
+    bands[:, :, 1] += 4.0
+    S.upload(bands)
+    ref = S.download()
+    x = rng.uniform(-1, 1, w3.undf).astype(T)
+    y0 = rng.uniform(-1, 1, w3.undf).astype(T)
+    xv = hip.FieldVector(w3, P).import_raw(x)
+    y = hip.FieldVector(w3, P)
+    S.apply(xv, y)
+    np.testing.assert_array_equal(y.export_raw(),
+                                  orc.cma_apply(pc, ref, x, np.zeros_like(y0), dm3, i3, dm3, i3, r32=r32))
+    y.import_raw(y0)
+    S.apply(xv, y, accumulate=True)
+    np.testing.assert_array_equal(y.export_raw(), orc.cma_apply(pc, ref, x, y0, dm3, i3, dm3, i3, r32=r32))
+    S.apply_inverse(xv, y)
+    np.testing.assert_array_equal(y.export_raw(), orc.cma_appinv(pc, ref, x, dm3, i3, r32=r32))
+
+
 def test_cma_diag_hmht_bitwise_equals_oracle(hip, orc):
     """asm_diag_hmht from a W3 <- W2h LMA operator and a W2h mass (the side faces of the box's W2
     dofmap, shared between neighbouring cells)."""
