@@ -123,7 +123,7 @@ def _p(a):
     return ctypes.c_void_p(a.ctypes.data) if a is not None else None
 
 
-W0, W2, W3 = 0, 2, 3
+W0, W1, W2, W3 = 0, 1, 2, 3
 
 
 def cubedsphere_topology(n):

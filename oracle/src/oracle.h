@@ -13,7 +13,8 @@
  *   orc_cubedsphere_topology  netCDF UGRID fixtures C4/C8/C12/C16/C24/C32 (face_nodes, face_links)
  *   orc_local_vert_lids       invariant: lid == first encounter; nverts == 6n^2+2
  *   orc_mesh_connectivity     canned bi-periodic 3x3 mesh edge_on_cell / face_on_cell tables
- *   orc_dofmap                fs0k0 / fs2k0 / fs3k0 dofmap fixtures; undf closed forms
+ *   orc_dofmap                fs0k0 / fs1k0 / fs2k0 / fs3k0 dofmap fixtures; undf closed forms;
+ *                             W0 order-1 entity sharing and column lengths
  *   orc_colour_cubedsphere    closed form + validator invariants + brute-force shared-vertex check
  *   orc_rng_*                 libm log/exp to a few ulp; N(0,1) / U(-1,1) moments
  *   orc_lma_apply             known-answer test of matrix_vector_mod_test.pf; assembled
@@ -80,7 +81,7 @@ int orc_mesh_connectivity(int ncells, const int *cell_next, const int *vert_on_c
                           int *nedges_2d);
 
 /* function space ids (match the reference's fs_continuity enumeration by meaning only) */
-enum { ORC_W0 = 0, ORC_W2 = 2, ORC_W3 = 3 };
+enum { ORC_W0 = 0, ORC_W1 = 1, ORC_W2 = 2, ORC_W3 = 3 };
 int orc_ndf(int space, int order);
 int orc_dofmap(int space, int order, int ncells, int nlayers, int nverts_2d, int nedges_2d,
                const int *vert_on_cell, const int *face_on_cell, const int *edge_on_cell,

@@ -248,7 +248,8 @@ int orc_mesh_connectivity(int nc, const int *cell_next, const int *voc, int *foc
 }
 
 /* entity dof counts (infrastructure/source/function_space/
- * function_space_constructor_helper_functions_mod.F90:258-266 W0, :284-295 W2, :391-400 W3) */
+ * function_space_constructor_helper_functions_mod.F90:258-266 W0, :268-282 W1, :284-295 W2,
+ * :391-400 W3; element_order_h = element_order_v = k) */
 typedef struct {
   int vert, edge_h, edge_v, face_h, face_v, vol, cell;
 } ndofs_t;
@@ -259,6 +260,12 @@ static int ndof_setup(int space, int k, ndofs_t *d) {
   case ORC_W0:
     d->vert = 1; d->edge_h = k; d->edge_v = k; d->face_h = k * k; d->face_v = k * k;
     d->vol = k * k * k; d->cell = (k + 2) * (k + 2) * (k + 2);
+    return 0;
+  case ORC_W1:
+    d->edge_h = k + 1; d->edge_v = k + 1;
+    d->face_h = (k + 1) * k + k * (k + 1); d->face_v = 2 * (k + 1) * k;
+    d->vol = 2 * k * (k + 1) * k + k * k * (k + 1);
+    d->cell = 2 * (k + 1) * (k + 2) * (k + 2) + (k + 2) * (k + 2) * (k + 1);
     return 0;
   case ORC_W2:
     d->face_h = (k + 1) * (k + 1); d->face_v = (k + 1) * (k + 1);

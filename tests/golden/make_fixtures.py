@@ -14,9 +14,10 @@ Everything written is data (inputs and expected outputs); no reference source te
                       a bi-periodic 8x8 planar mesh, a bounded planar mesh): data the reader parses.
 * canned_mesh.json    the bi-periodic 3x3x3 unit-test mesh of
                       infrastructure/source/mesh/mesh_mod.F90:3625-3777 (layer-1 cell_next,
-                      vert_on_cell, edge_on_cell, face_on_cell) and the W0/W2/W3 dofmaps
+                      vert_on_cell, edge_on_cell, face_on_cell) and the W0/W1/W2/W3 dofmaps
                       that infrastructure/unit-test/support/fs_test_data_mod.F90 expects on
-                      it (fs0k0_dofmap :24-33, fs2k0_dofmap :1026-1033, fs3k0_dofmap :1824-1826).
+                      it (fs0k0_dofmap :24-33, fs1k0_dofmap :346-359, fs2k0_dofmap :1026-1033,
+                      fs3k0_dofmap :1824-1826).
 * kat.json            the matrix_vector known-answer test
                       (components/science/unit-test/kernel/algebra/matrix_vector_mod_test.pf:57-89),
                       the transpose / dg / mm_diagonal and invert_local_operator kernel tests,
@@ -129,6 +130,7 @@ def main(ref):
         "edge_on_cell": edge_on_cell,
         "face_on_cell": face_on_cell,
         "fs0k0_dofmap": np.array(fortran_int_table(fs, "fs0k0_dofmap")).reshape(10, 8)[1:].tolist(),
+        "fs1k0_dofmap": np.array(fortran_int_table(fs, "fs1k0_dofmap")).reshape(10, 12)[1:].tolist(),
         "fs2k0_dofmap": np.array(fortran_int_table(fs, "fs2k0_dofmap")).reshape(10, 6)[1:].tolist(),
         "fs3k0_dofmap": np.array(fortran_int_table(fs, "fs3k0_dofmap")).reshape(10, 1)[1:].tolist(),
     }

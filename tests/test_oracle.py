@@ -60,9 +60,87 @@ def test_connectivity_and_dofmaps_match_unit_test_fixtures(orc):
     np.testing.assert_array_equal(foc, c["face_on_cell"])
     np.testing.assert_array_equal(eoc, c["edge_on_cell"])
     nv = int(vc.max())
-    for space, key in ((orc.W0, "fs0k0_dofmap"), (orc.W2, "fs2k0_dofmap"), (orc.W3, "fs3k0_dofmap")):
+    for space, key in ((orc.W0, "fs0k0_dofmap"), (orc.W1, "fs1k0_dofmap"), (orc.W2, "fs2k0_dofmap"),
+                       (orc.W3, "fs3k0_dofmap")):
         dm, undf = orc.dofmap(space, 0, c["nlayers"], vc, foc, eoc, nv, ne)
         np.testing.assert_array_equal(dm, c[key])
+
+
+def test_w1_fixture_pins_the_edge_branch(orc):
+    """fs1k0_dofmap holds only edge dofs: the bottom horizontal edges get nlayers+1 ids with the
+    top edge one above, the vertical edges nlayers ids, numbered cell by cell with h-edges first.
+    W0 order 1 takes its edge dofs from this same branch, so a swap of the h-/v-edge loops or a
+    wrong top offset in the oracle would break the fixture equality above; check that it does."""
+    c = json.loads((GOLDEN / "canned_mesh.json").read_text())
+    fx = np.array(c["fs1k0_dofmap"])
+    nl = c["nlayers"]
+    np.testing.assert_array_equal(fx[:, 8:12], fx[:, 0:4] + 1)
+    nh, nvv = np.unique(fx[:, 0:4]).size, np.unique(fx[:, 4:8]).size
+    undf = nh * (nl + 1) + nvv * nl
+    starts = np.unique(np.concatenate([fx[:, 0:4].ravel(), fx[:, 4:8].ravel()]))
+    lengths = dict(zip(starts, np.diff(np.append(starts, undf + 1))))
+    assert {lengths[v] for v in fx[:, 0:4].ravel()} == {nl + 1}
+    assert {lengths[v] for v in fx[:, 4:8].ravel()} == {nl}
+    # cell 1 numbers its four bottom h-edges before its vertical edges
+    assert fx[0, :4].max() < fx[0, 4:8].min()
+
+
+# W0 order 1: 27 local dofs in entity order volume (0), side faces (1-4), bottom and top face
+# (5, 6), bottom h-edges (7-10), vertical edges (11-14), top h-edges (15-18), bottom vertices
+# (19-22), top vertices (23-26)
+_W0K1_KIND = ["vol"] + ["vface"] * 4 + ["hface_b", "hface_t"] + ["hedge_b"] * 4 + ["vedge"] * 4 + \
+             ["hedge_t"] * 4 + ["vert_b"] * 4 + ["vert_t"] * 4
+_W0K1_TOP_OF = {6: 5, **{15 + i: 7 + i for i in range(4)}, **{23 + i: 19 + i for i in range(4)}}
+
+
+def _w0k1_entity_violations(dm, undf, nl):
+    """Entity structure of a W0 order-1 dofmap on a cubed sphere, independent of the numbering
+    rule: every column is nl+1 long for vertices, horizontal edges and horizontal faces and nl
+    long otherwise; each top entity is its bottom partner plus one; each column is reached from
+    one kind of local position only, by as many cells as share that entity (volume 1, side face 2,
+    horizontal face 1, h-edge 2, vertical edge and vertex 3 or 4).  Returns the failed checks."""
+    bad = []
+    for t, b in _W0K1_TOP_OF.items():
+        if not np.array_equal(dm[:, t], dm[:, b] + 1):
+            bad.append(f"top {t} != bottom {b} + 1")
+    base = [p for p in range(27) if p not in _W0K1_TOP_OF]
+    starts = np.unique(dm[:, base])
+    length = dict(zip(starts.tolist(), np.diff(np.append(starts, undf + 1)).tolist()))
+    want_len = {"vol": nl, "vface": nl, "vedge": nl, "hface_b": nl + 1, "hedge_b": nl + 1, "vert_b": nl + 1}
+    kind_of, cells_of = {}, {}
+    for p in base:
+        k = _W0K1_KIND[p]
+        for cell, col in enumerate(dm[:, p].tolist()):
+            if kind_of.setdefault(col, k) != k:
+                bad.append(f"column {col} reached as {kind_of[col]} and {k}")
+            cells_of.setdefault(col, set()).add(cell)
+            if length.get(col) != want_len[k]:
+                bad.append(f"{k} column {col} has length {length.get(col)}")
+    share = {"vol": {1}, "vface": {2}, "hface_b": {1}, "hedge_b": {2}, "vedge": {3, 4}, "vert_b": {3, 4}}
+    for col, k in kind_of.items():
+        if len(cells_of[col]) not in share[k]:
+            bad.append(f"{k} column {col} shared by {len(cells_of[col])} cells")
+    if sum(length.values()) != undf:
+        bad.append("columns do not tile 1..undf")
+    return bad
+
+
+@pytest.mark.parametrize("n,nl", [(3, 4), (6, 5)])
+def test_w0_order1_dofmap_entity_sharing(orc, n, nl):
+    m = orc.Mesh(n, nl, orc.W0, 1)
+    assert _w0k1_entity_violations(m.dofmap, m.undf, nl) == []
+    nv = 6 * n * n + 2
+    # vertical edges sit on the 2D vertices: 8 of their columns are shared by 3 cells
+    vcols = np.unique(m.dofmap[:, 11:15])
+    assert vcols.size == nv and sum(len(set(np.where(m.dofmap[:, 11:15] == v)[0])) == 3 for v in vcols) == 8
+    # planted mistakes in a copy of the oracle's dofmap: h-edges and v-edges swapped, and top
+    # h-edges two above their bottom partners; both must be caught
+    swapped = m.dofmap.copy()
+    swapped[:, 7:11], swapped[:, 11:15] = m.dofmap[:, 11:15], m.dofmap[:, 7:11]
+    assert _w0k1_entity_violations(swapped, m.undf, nl)
+    shifted = m.dofmap.copy()
+    shifted[:, 15:19] += 1
+    assert _w0k1_entity_violations(shifted, m.undf, nl)
 
 
 @pytest.mark.parametrize("n,nl", [(4, 3), (12, 20)])
