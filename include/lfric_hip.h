@@ -135,6 +135,12 @@ int lfric_op_create(const lfric_fs *to, const lfric_fs *from, int precision, lfr
 /* local_stencil(ncell_3d, ndf_to, ndf_from), column-major, FP64. */
 int lfric_op_upload(lfric_op *op, const double *local_stencil);
 int lfric_op_download(const lfric_op *op, double *local_stencil);
+/* The same in real32: r_solver_operator_type's local_stencil (infrastructure/source/operator/
+ * r_solver_operator_mod.F90:17-23, operator_real32 of operator_mod.t90).  Either precision may
+ * be uploaded to an operator of either storage precision (converted on the device); a download
+ * from an FP64 operator rounds to nearest. */
+int lfric_op_upload_r32(lfric_op *op, const float *local_stencil);
+int lfric_op_download_r32(const lfric_op *op, float *local_stencil);
 /* Synthetic SPD element matrices A_e = B_e B_e^T + shift I, B_e ~ N(0,1)/8 (scaled: column j
  * times 10^u_j, u_j ~ U(-2,2)); defined by the counter-based generator (shift 0.1 for the
  * benchmark W0 operators, 1.0 for the W2 mass matrix C C^T + I). */

@@ -84,6 +84,8 @@ SIGNATURES = {
     "lfric_op_create": (_I, [_VP, _VP, _I, ctypes.POINTER(_VP)]),
     "lfric_op_upload": (_I, [_VP, _VP]),
     "lfric_op_download": (_I, [_VP, _VP]),
+    "lfric_op_upload_r32": (_I, [_VP, _VP]),
+    "lfric_op_download_r32": (_I, [_VP, _VP]),
     "lfric_op_generate_spd": (_I, [_VP, _ULL, _I, _D]),
     "lfric_op_generate_uniform": (_I, [_VP, _ULL, _ULL, _D, _D, _VP]),
     "lfric_op_convert": (_I, [_VP, _VP]),

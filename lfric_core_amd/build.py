@@ -16,20 +16,26 @@ FLAGS = [
 ]
 
 
-def build(verbose=False):
-    objs = []
+def _compile(src, build_dir, verbose):
+    obj = build_dir / (src + ".o")
+    cmd = [HIPCC, *FLAGS, "-c", str(CSRC / src), "-o", str(obj)]
+    if src.endswith(".cpp"):
+        cmd = [HIPCC, "-O3", "-std=c++17", "-fPIC", "-Wall", "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include",
+               "-c", str(CSRC / src), "-o", str(obj)]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    return str(obj)
+
+
+def build(verbose=False, jobs=None):
+    """Compile every source (in parallel, one hipcc per file) and link liblfric_hip.so."""
+    from concurrent.futures import ThreadPoolExecutor
     build_dir = PKG / "_build"
     build_dir.mkdir(exist_ok=True)
-    for src in SOURCES:
-        obj = build_dir / (src + ".o")
-        cmd = [HIPCC, *FLAGS, "-c", str(CSRC / src), "-o", str(obj)]
-        if src.endswith(".cpp"):
-            cmd = [HIPCC, "-O3", "-std=c++17", "-fPIC", "-Wall", "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include",
-                   "-c", str(CSRC / src), "-o", str(obj)]
-        if verbose:
-            print(" ".join(cmd), flush=True)
-        subprocess.run(cmd, check=True)
-        objs.append(str(obj))
+    jobs = jobs or min(len(SOURCES), max(1, (os.cpu_count() or 2) - 1))
+    with ThreadPoolExecutor(jobs) as pool:
+        objs = list(pool.map(lambda src: _compile(src, build_dir, verbose), SOURCES))
     tmp = str(LIB) + ".tmp"
     subprocess.run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs], check=True)
     os.replace(tmp, LIB)
@@ -42,26 +48,28 @@ FORTRAN_SOURCES = [
     "standalone/sci_linear_operator_mod.f90", "standalone/sci_preconditioner_mod.f90",
     "standalone/sci_iterative_solver_mod.f90", "lfric_hip_capi_mod.f90", "lfric_hip_solver_api_mod.f90",
 ]
-FORTRAN_TEST = FORTRAN / "_build" / "drop_in_test"
+FORTRAN_LIB = FORTRAN / "_build" / "liblfric_hip_fortran.a"
 
 
 def build_fortran(verbose=False):
-    """Compile the ISO_C_BINDING shim with the standalone solver-API modules and the drop-in
-    test program (flang), linked against liblfric_hip.so."""
+    """Compile the ISO_C_BINDING shim (flang) against the standalone solver-API interfaces into
+    a static library; an LFRic build compiles the two shim modules against its own modules
+    instead, and the drop-in test (oracle/Makefile.dropin) against the reference's."""
     fc = os.environ.get("FC", "/opt/rocm/llvm/bin/flang")
     out = FORTRAN / "_build"
     out.mkdir(exist_ok=True)
     objs = []
-    for src in FORTRAN_SOURCES + ["tests/drop_in_test.f90"]:
+    for src in FORTRAN_SOURCES:
         obj = out / (Path(src).stem + ".o")
         cmd = [fc, "-O2", "-fPIC", "-module-dir", str(out), "-I", str(out), "-c", str(FORTRAN / src), "-o", str(obj)]
         if verbose:
             print(" ".join(cmd), flush=True)
         subprocess.run(cmd, check=True)
         objs.append(str(obj))
-    subprocess.run([fc, "-o", str(FORTRAN_TEST), *objs, "-L", str(PKG), "-llfric_hip",
-                    "-Wl,-rpath," + str(PKG), "-Wl,-rpath,$ORIGIN/../.."], check=True)
-    return FORTRAN_TEST
+    if FORTRAN_LIB.exists():
+        FORTRAN_LIB.unlink()
+    subprocess.run(["ar", "rcs", str(FORTRAN_LIB), *objs], check=True)
+    return FORTRAN_LIB
 
 
 if __name__ == "__main__":

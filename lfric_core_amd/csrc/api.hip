@@ -477,6 +477,44 @@ int lfric_fs_get_colouring(const lfric_fs *fs, int *npc, int *cic) {
 void lfric_fs_destroy(lfric_fs *fs) { delete fs; }
 
 // ---- vectors ----------------------------------------------------------------------------------
+// Buffers of destroyed vectors kept for the next vector of the same size: a Krylov solver's
+// copy() releases and recreates a vector every call, and hipFree waits for the whole device.
+// Only buffers up to 64 MiB are kept, at most 256 MiB in all; they count as free, not as held
+// by live handles.
+static std::mutex g_vec_cache_mu;
+static std::vector<std::pair<void *, size_t>> g_vec_cache;
+static size_t g_vec_cache_bytes = 0;
+static constexpr size_t kVecCacheEntryMax = size_t(64) << 20, kVecCacheTotalMax = size_t(256) << 20;
+
+static int vec_buffer_get(void **p, size_t bytes) {
+  {
+    std::lock_guard<std::mutex> lk(g_vec_cache_mu);
+    for (size_t i = 0; i < g_vec_cache.size(); ++i)
+      if (g_vec_cache[i].second == bytes) {
+        *p = g_vec_cache[i].first;
+        g_vec_cache.erase(g_vec_cache.begin() + static_cast<long>(i));
+        g_vec_cache_bytes -= bytes;
+        g_ctx.bytes += static_cast<long long>(bytes);
+        return LFRIC_OK;
+      }
+  }
+  return dmalloc(p, bytes);
+}
+
+static void vec_buffer_put(void *p, size_t bytes) {
+  if (!p) return;
+  {
+    std::lock_guard<std::mutex> lk(g_vec_cache_mu);
+    if (bytes > 0 && bytes <= kVecCacheEntryMax && g_vec_cache_bytes + bytes <= kVecCacheTotalMax) {
+      g_vec_cache.emplace_back(p, bytes);
+      g_vec_cache_bytes += bytes;
+      g_ctx.bytes -= static_cast<long long>(bytes);
+      return;
+    }
+  }
+  dfree(p, bytes);
+}
+
 int lfric_vec_create(const lfric_fs *fs, int precision, int nrhs, lfric_vec **out) {
   if (int rc = require_init()) return rc;
   LF_CHECK(fs && out && nrhs >= 1, LFRIC_E_ARG, "bad arguments");
@@ -487,7 +525,7 @@ int lfric_vec_create(const lfric_fs *fs, int precision, int nrhs, lfric_vec **ou
   v->nrhs = nrhs;
   v->n = static_cast<long long>(fs->fs->undf) * nrhs;
   v->bytes = esize(precision) * static_cast<size_t>(v->n);
-  if (int rc = dmalloc(&v->d, v->bytes)) {
+  if (int rc = vec_buffer_get(&v->d, v->bytes)) {
     delete v;
     return rc;
   }
@@ -498,7 +536,7 @@ int lfric_vec_create(const lfric_fs *fs, int precision, int nrhs, lfric_vec **ou
 
 void lfric_vec_destroy(lfric_vec *v) {
   if (!v) return;
-  dfree(v->d, v->bytes);
+  vec_buffer_put(v->d, v->bytes);
   delete v;
 }
 
@@ -665,32 +703,39 @@ static int refresh_rows(lfric_op *op, int rc) {
   return LFRIC_OK;
 }
 
-int lfric_op_upload(lfric_op *op, const double *ls) {
+static int op_upload_any(lfric_op *op, const void *ls, int prec) {
   if (int rc = require_init()) return rc;
   LF_CHECK(op && ls, LFRIC_E_ARG, "null argument");
   const size_t n = static_cast<size_t>(op->to->ncells) * op->to->nlayers * op->to->ndf * op->from->ndf;
-  double *tmp = nullptr;
-  if (int rc = dmalloc(reinterpret_cast<void **>(&tmp), n * sizeof(double))) return rc;
-  LF_HIP(hipMemcpyAsync(tmp, ls, n * sizeof(double), hipMemcpyHostToDevice, ctx().stream));
-  int rc = launch_op_from_reference(op, tmp);
-  dfree(tmp, n * sizeof(double));
+  const size_t esize = prec == LFRIC_R64 ? sizeof(double) : sizeof(float);
+  void *tmp = nullptr;
+  if (int rc = dmalloc(&tmp, n * esize)) return rc;
+  LF_HIP(hipMemcpyAsync(tmp, ls, n * esize, hipMemcpyHostToDevice, ctx().stream));
+  int rc = launch_op_from_reference(op, tmp, prec);
+  dfree(tmp, n * esize);
   return refresh_rows(op, rc);
 }
 
-int lfric_op_download(const lfric_op *op, double *ls) {
+static int op_download_any(const lfric_op *op, void *ls, int prec) {
   if (int rc = require_init()) return rc;
   LF_CHECK(op && ls, LFRIC_E_ARG, "null argument");
   const size_t n = static_cast<size_t>(op->to->ncells) * op->to->nlayers * op->to->ndf * op->from->ndf;
-  double *tmp = nullptr;
-  if (int rc = dmalloc(reinterpret_cast<void **>(&tmp), n * sizeof(double))) return rc;
-  int rc = launch_op_to_reference(op, tmp);
+  const size_t esize = prec == LFRIC_R64 ? sizeof(double) : sizeof(float);
+  void *tmp = nullptr;
+  if (int rc = dmalloc(&tmp, n * esize)) return rc;
+  int rc = launch_op_to_reference(op, tmp, prec);
   if (rc == LFRIC_OK) {
-    hipError_t e = hipMemcpy(ls, tmp, n * sizeof(double), hipMemcpyDeviceToHost);
+    hipError_t e = hipMemcpy(ls, tmp, n * esize, hipMemcpyDeviceToHost);
     if (e != hipSuccess) rc = fail(LFRIC_E_HIP, hipGetErrorString(e));
   }
-  dfree(tmp, n * sizeof(double));
+  dfree(tmp, n * esize);
   return rc;
 }
+
+int lfric_op_upload(lfric_op *op, const double *ls) { return op_upload_any(op, ls, LFRIC_R64); }
+int lfric_op_download(const lfric_op *op, double *ls) { return op_download_any(op, ls, LFRIC_R64); }
+int lfric_op_upload_r32(lfric_op *op, const float *ls) { return op_upload_any(op, ls, LFRIC_R32); }
+int lfric_op_download_r32(const lfric_op *op, float *ls) { return op_download_any(op, ls, LFRIC_R32); }
 
 int lfric_op_generate_spd(lfric_op *op, unsigned long long seed, int scaled, double shift) {
   if (int rc = require_init()) return rc;

@@ -28,8 +28,10 @@ int launch_gen_uniform(lfric_op *op, unsigned long long seed, unsigned long long
                        double lo, double hi, const double *col_sign_host);
 int launch_vec_uniform(lfric_vec *v, unsigned long long seed, unsigned long long stream);
 // layout changes between the reference local_stencil and the device plane layout
-int launch_op_from_reference(lfric_op *op, const double *d_ref);
-int launch_op_to_reference(const lfric_op *op, double *d_ref);
+// reference-layout transfer; ref_precision LFRIC_R64 (double) or LFRIC_R32 (float) names the
+// type of d_ref, converted to or from the operator's own storage precision
+int launch_op_from_reference(lfric_op *op, const void *d_ref, int ref_precision);
+int launch_op_to_reference(const lfric_op *op, void *d_ref, int ref_precision);
 int launch_op_convert(const lfric_op *src, lfric_op *dst);
 
 // operator kernels

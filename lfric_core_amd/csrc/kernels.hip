@@ -408,9 +408,9 @@ __global__ __launch_bounds__(kBlock) void k_assemble_fem(double *__restrict__ A,
 #pragma clang fp contract(on)
 
 // reference layout matrix(ncell_3d, ndf1, ndf2) <-> device layout
-template <typename T>
+template <typename T, typename S>
 __global__ void k_from_reference(T *__restrict__ A, size_t gs, const int *__restrict__ ebase,
-                                 const double *__restrict__ R, const int *__restrict__ cell_of_pos, int nl,
+                                 const S *__restrict__ R, const int *__restrict__ cell_of_pos, int nl,
                                  long long nelem, int n1, int n2, uint32_t row_top) {
   const long long e = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (e >= nelem) return;
@@ -424,9 +424,9 @@ __global__ void k_from_reference(T *__restrict__ A, size_t gs, const int *__rest
           static_cast<T>(R[ic + nelem * (i + static_cast<long long>(n1) * j)]);
 }
 
-template <typename T>
+template <typename T, typename D>
 __global__ void k_to_reference(const T *__restrict__ A, size_t gs, const int *__restrict__ ebase,
-                               double *__restrict__ R, const int *__restrict__ cell_of_pos, int nl, long long nelem,
+                               D *__restrict__ R, const int *__restrict__ cell_of_pos, int nl, long long nelem,
                                int n1, int n2, uint32_t row_top) {
   const long long e = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (e >= nelem) return;
@@ -436,7 +436,8 @@ __global__ void k_to_reference(const T *__restrict__ A, size_t gs, const int *__
   const long long el = ebase[pos] + k;
   for (int i = 0; i < n1; ++i)
     for (int j = 0; j < n2; ++j)
-      R[ic + nelem * (i + static_cast<long long>(n1) * j)] = to_d(A[aidx<T>(gs, el + ((row_top >> i) & 1u), i * n2 + j)]);
+      R[ic + nelem * (i + static_cast<long long>(n1) * j)] =
+          static_cast<D>(to_d(A[aidx<T>(gs, el + ((row_top >> i) & 1u), i * n2 + j)]));
 }
 
 // operator precision change; the chunk width differs between precisions, so entries move
@@ -1546,7 +1547,8 @@ static int with_cell_of_pos(const FunctionSpace &f, int **d_cop, size_t &bytes) 
   return LFRIC_OK;
 }
 
-int launch_op_from_reference(lfric_op *op, const double *d_ref) {
+template <typename S>
+static int op_from_reference_t(lfric_op *op, const S *d_ref) {
   const FunctionSpace &f = *op->to;
   int *d_cop = nullptr;
   size_t bytes = 0;
@@ -1554,11 +1556,11 @@ int launch_op_from_reference(lfric_op *op, const double *d_ref) {
   const long long nelem = static_cast<long long>(f.ncells) * f.nlayers;
   const int n1 = op->to->ndf, n2 = op->from->ndf;
   if (op->precision == LFRIC_R64)
-    LAUNCH(k_from_reference<double>, grid_for(nelem), static_cast<double *>(op->d), op->gs, f.d_ebase, d_ref, d_cop,
-           f.nlayers, nelem, n1, n2, op->row_top);
+    LAUNCH((k_from_reference<double, S>), grid_for(nelem), static_cast<double *>(op->d), op->gs, f.d_ebase, d_ref,
+           d_cop, f.nlayers, nelem, n1, n2, op->row_top);
   else
-    LAUNCH(k_from_reference<float>, grid_for(nelem), static_cast<float *>(op->d), op->gs, f.d_ebase, d_ref, d_cop,
-           f.nlayers, nelem, n1, n2, op->row_top);
+    LAUNCH((k_from_reference<float, S>), grid_for(nelem), static_cast<float *>(op->d), op->gs, f.d_ebase, d_ref,
+           d_cop, f.nlayers, nelem, n1, n2, op->row_top);
   hipError_t e = hipGetLastError();
   LF_HIP(hipStreamSynchronize(ctx().stream));
   dfree(d_cop, bytes);
@@ -1566,7 +1568,8 @@ int launch_op_from_reference(lfric_op *op, const double *d_ref) {
   return LFRIC_OK;
 }
 
-int launch_op_to_reference(const lfric_op *op, double *d_ref) {
+template <typename D>
+static int op_to_reference_t(const lfric_op *op, D *d_ref) {
   const FunctionSpace &f = *op->to;
   int *d_cop = nullptr;
   size_t bytes = 0;
@@ -1574,16 +1577,26 @@ int launch_op_to_reference(const lfric_op *op, double *d_ref) {
   const long long nelem = static_cast<long long>(f.ncells) * f.nlayers;
   const int n1 = op->to->ndf, n2 = op->from->ndf;
   if (op->precision == LFRIC_R64)
-    LAUNCH(k_to_reference<double>, grid_for(nelem), static_cast<const double *>(op->d), op->gs, f.d_ebase, d_ref,
-           d_cop, f.nlayers, nelem, n1, n2, op->row_top);
+    LAUNCH((k_to_reference<double, D>), grid_for(nelem), static_cast<const double *>(op->d), op->gs, f.d_ebase,
+           d_ref, d_cop, f.nlayers, nelem, n1, n2, op->row_top);
   else
-    LAUNCH(k_to_reference<float>, grid_for(nelem), static_cast<const float *>(op->d), op->gs, f.d_ebase, d_ref,
-           d_cop, f.nlayers, nelem, n1, n2, op->row_top);
+    LAUNCH((k_to_reference<float, D>), grid_for(nelem), static_cast<const float *>(op->d), op->gs, f.d_ebase,
+           d_ref, d_cop, f.nlayers, nelem, n1, n2, op->row_top);
   hipError_t e = hipGetLastError();
   LF_HIP(hipStreamSynchronize(ctx().stream));
   dfree(d_cop, bytes);
   if (e != hipSuccess) return fail(LFRIC_E_HIP, hipGetErrorString(e));
   return LFRIC_OK;
+}
+
+int launch_op_from_reference(lfric_op *op, const void *d_ref, int ref_precision) {
+  return ref_precision == LFRIC_R64 ? op_from_reference_t(op, static_cast<const double *>(d_ref))
+                                    : op_from_reference_t(op, static_cast<const float *>(d_ref));
+}
+
+int launch_op_to_reference(const lfric_op *op, void *d_ref, int ref_precision) {
+  return ref_precision == LFRIC_R64 ? op_to_reference_t(op, static_cast<double *>(d_ref))
+                                    : op_to_reference_t(op, static_cast<float *>(d_ref));
 }
 
 template <typename T>

@@ -41,14 +41,16 @@ module lfric_hip_capi_mod
     real(c_double) :: true_rel_res
   end type lfric_solver_result
 
-  public :: lfric_init, lfric_last_error_message, lfric_synchronize
+  public :: lfric_init, lfric_last_error_message, lfric_synchronize, lfric_device_bytes
   public :: lfric_fs_create, lfric_fs_create_cubedsphere, lfric_fs_info, lfric_fs_get_dofmap, &
             lfric_fs_get_colouring, lfric_fs_destroy
   public :: lfric_vec_create, lfric_vec_upload, lfric_vec_download, lfric_vec_destroy, &
+            lfric_vec_upload_r32, lfric_vec_download_r32, &
             lfric_vec_set_scalar, lfric_vec_axpy, lfric_vec_aypx, lfric_vec_axpby, &
             lfric_vec_scale, lfric_vec_copy, lfric_vec_dot, lfric_vec_norm, lfric_vec_size, &
             lfric_vec_fill_uniform
   public :: lfric_op_create, lfric_op_upload, lfric_op_generate_spd, lfric_op_convert, &
+            lfric_op_download, lfric_op_upload_r32, lfric_op_download_r32, &
             lfric_op_destroy, lfric_lma_apply, lfric_mm_diagonal, lfric_op_invert_local, &
             lfric_op_prepare_rows, lfric_op_tune, lfric_op_assemble_mass_w0, lfric_op_prepare_symmetric, &
             lfric_op_assemble_mass, lfric_op_assemble_div
@@ -72,6 +74,9 @@ module lfric_hip_capi_mod
     end function
     integer(c_int) function lfric_synchronize() bind(c, name="lfric_synchronize")
       import :: c_int
+    end function
+    integer(c_long_long) function lfric_device_bytes() bind(c, name="lfric_device_bytes")
+      import :: c_long_long
     end function
 
     integer(c_int) function lfric_fs_create(ncells, nlayers, ndf, undf, dofmap, ncolours, npc, cic, &
@@ -123,6 +128,16 @@ module lfric_hip_capi_mod
       import :: c_int, c_ptr, c_double
       type(c_ptr), value :: v
       real(c_double), intent(out) :: host(*)
+    end function
+    integer(c_int) function lfric_vec_upload_r32(v, host) bind(c, name="lfric_vec_upload_r32")
+      import :: c_int, c_ptr, c_float
+      type(c_ptr), value :: v
+      real(c_float), intent(in) :: host(*)
+    end function
+    integer(c_int) function lfric_vec_download_r32(v, host) bind(c, name="lfric_vec_download_r32")
+      import :: c_int, c_ptr, c_float
+      type(c_ptr), value :: v
+      real(c_float), intent(out) :: host(*)
     end function
     subroutine lfric_vec_destroy(v) bind(c, name="lfric_vec_destroy")
       import :: c_ptr
@@ -188,6 +203,21 @@ module lfric_hip_capi_mod
       import :: c_int, c_ptr, c_double
       type(c_ptr), value :: op
       real(c_double), intent(in) :: local_stencil(*)
+    end function
+    integer(c_int) function lfric_op_download(op, local_stencil) bind(c, name="lfric_op_download")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: op
+      real(c_double), intent(out) :: local_stencil(*)
+    end function
+    integer(c_int) function lfric_op_upload_r32(op, local_stencil) bind(c, name="lfric_op_upload_r32")
+      import :: c_int, c_ptr, c_float
+      type(c_ptr), value :: op
+      real(c_float), intent(in) :: local_stencil(*)
+    end function
+    integer(c_int) function lfric_op_download_r32(op, local_stencil) bind(c, name="lfric_op_download_r32")
+      import :: c_int, c_ptr, c_float
+      type(c_ptr), value :: op
+      real(c_float), intent(out) :: local_stencil(*)
     end function
     integer(c_int) function lfric_op_generate_spd(op, seed, scaled, shift) bind(c, name="lfric_op_generate_spd")
       import :: c_int, c_ptr, c_long_long, c_double

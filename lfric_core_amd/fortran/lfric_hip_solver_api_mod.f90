@@ -1,7 +1,15 @@
 !> HIP-resident implementations of LFRic's solver-API types.
 !>
 !>   hip_field_vector_type        extends abstract_vector_type (infrastructure/source/field/
-!>                                vector_mod.F90:17-59); data stays in GPU memory
+!>                                vector_mod.F90:17-59); data stays in GPU memory (real64,
+!>                                field_vector_type of sci_field_vector_mod.x90)
+!>   hip_r_solver_field_vector_type
+!>                                the same over real32 storage (r_solver_field_vector_type,
+!>                                components/science/source/algorithm/solver/
+!>                                sci_r_solver_field_vector_mod.x90:26-252): real(r_solver)
+!>                                import/export, the real(r_def) scalars of axpy, aypx, axpby,
+!>                                scale and set_scalar cast to r_solver before use (:189,213-214,
+!>                                246), inner products accumulated in real64
 !>   hip_lma_operator_type        extends abstract_linear_operator_type: y = A x with the
 !>                                colour-ordered LMA kernels (replaces mass_matrix_operator_type%apply,
 !>                                components/science/source/algorithm/solver/sci_mass_matrix_operator_alg_mod.x90:72-119)
@@ -12,7 +20,7 @@
 !>                                extend abstract_iterative_solver_type and run the whole solve on
 !>                                the device (FP64 CG; FP32 CG inside FP64 refinement)
 !>
-!> With these types the stock conjugate_gradient_type / gmres_type of
+!> With these types LFRic's own conjugate_gradient_type / BiCGstab_type / gmres_type of
 !> components/science/source/solver/sci_iterative_solver_mod.f90 run unmodified: every vector
 !> operation they call becomes a HIP kernel.  Errors follow the reference: a failed library call
 !> is logged at LOG_LEVEL_ERROR (which aborts), non-convergence at INFO or ERROR according to
@@ -25,7 +33,7 @@
 !> is first written.
 module lfric_hip_solver_api_mod
   use iso_c_binding
-  use constants_mod,           only : r_def, i_def, l_def
+  use constants_mod,           only : r_def, r_solver, i_def, l_def
   use log_mod,                 only : log_event, log_scratch_space, LOG_LEVEL_ERROR, LOG_LEVEL_INFO
   use vector_mod,              only : abstract_vector_type
   use sci_linear_operator_mod, only : abstract_linear_operator_type
@@ -71,8 +79,35 @@ module lfric_hip_solver_api_mod
     final :: vec_final
   end type hip_field_vector_type
 
+  type, public, extends(abstract_vector_type) :: hip_r_solver_field_vector_type
+    type(c_ptr) :: h = c_null_ptr
+    type(c_ptr) :: fs = c_null_ptr
+    integer(c_int) :: undf = 0
+    integer(c_intptr_t) :: owner = 0
+  contains
+    procedure :: initialise => rvec_initialise
+    procedure :: import_field => rvec_import
+    procedure :: export_field => rvec_export
+    procedure :: set_scalar => rvec_set_scalar
+    procedure :: axpy => rvec_axpy
+    procedure :: aypx => rvec_aypx
+    procedure :: axpby => rvec_axpby
+    procedure :: norm => rvec_norm
+    procedure :: field_norm => rvec_field_norm
+    procedure :: dot => rvec_dot
+    procedure :: scale => rvec_scale
+    procedure :: duplicate => rvec_duplicate
+    procedure :: copy => rvec_copy
+    procedure :: vector_size => rvec_size
+    procedure, private :: own => rvec_own
+    final :: rvec_final
+  end type hip_r_solver_field_vector_type
+
+  !> y = A x over either vector type (the operator's storage precision must match the
+  !> vectors'); `applies` counts the calls, so a caller can read how many products a solver made
   type, public, extends(abstract_linear_operator_type) :: hip_lma_operator_type
     type(c_ptr) :: op = c_null_ptr
+    integer(i_def) :: applies = 0
   contains
     procedure :: apply => op_apply
   end type hip_lma_operator_type
@@ -287,6 +322,9 @@ contains
     end select
   end subroutine vec_duplicate
 
+  !> self is intent(out), as copy_interface declares it (vector_mod.F90:173-177), so a vector
+  !> passed here is finalised on entry (vec_final releases its device buffer to the library's
+  !> cache) and this call takes a buffer of the source's shape.
   subroutine vec_copy(self, source)
     class(hip_field_vector_type), intent(out) :: self
     class(abstract_vector_type), intent(in) :: source
@@ -309,6 +347,180 @@ contains
     n_fields = 1
   end function vec_size
 
+  ! ---- r_solver (real32) vectors ------------------------------------------------------------
+  !> real(r_def) scalar as the reference passes it on: real(scalar, r_solver)
+  !> (sci_r_solver_field_vector_mod.x90:189,213-214,246), then widened exactly for the C call
+  elemental function to_rsol(a) result(b)
+    real(r_def), intent(in) :: a
+    real(r_def) :: b
+    b = real(real(a, r_solver), r_def)
+  end function to_rsol
+
+  function rhandle_of(x, what) result(h)
+    class(abstract_vector_type), intent(in) :: x
+    character(len=*), intent(in) :: what
+    type(c_ptr) :: h
+    h = c_null_ptr
+    select type (x)
+    type is (hip_r_solver_field_vector_type)
+      h = x%h
+    class default
+      call log_event("hip_r_solver_field_vector_type:" // what // &
+                     ": argument is not a hip_r_solver_field_vector_type", LOG_LEVEL_ERROR)
+    end select
+  end function rhandle_of
+
+  subroutine rvec_initialise(self, fs)
+    class(hip_r_solver_field_vector_type), intent(inout) :: self
+    type(hip_function_space_type), intent(in) :: fs
+    self%fs = fs%h
+    call hip_check(lfric_vec_create(self%fs, LFRIC_R32, 1_c_int, self%h), "lfric_vec_create")
+    self%undf = fs%undf
+    self%owner = loc(self)
+  end subroutine rvec_initialise
+
+  subroutine rvec_own(self)
+    class(hip_r_solver_field_vector_type), intent(inout) :: self
+    type(c_ptr) :: shared
+    if (self%owner == loc(self)) return
+    shared = self%h
+    call hip_check(lfric_vec_create(self%fs, LFRIC_R32, 1_c_int, self%h), "lfric_vec_create")
+    if (c_associated(shared)) call hip_check(lfric_vec_copy(self%h, shared), "lfric_vec_copy")
+    self%owner = loc(self)
+  end subroutine rvec_own
+
+  subroutine rvec_final(self)
+    type(hip_r_solver_field_vector_type), intent(inout) :: self
+    if (c_associated(self%h) .and. self%owner == loc(self)) call lfric_vec_destroy(self%h)
+    self%h = c_null_ptr
+  end subroutine rvec_final
+
+  subroutine rvec_import(self, data)
+    class(hip_r_solver_field_vector_type), intent(inout) :: self
+    real(r_solver), intent(in) :: data(:)
+    call self%own()
+    call hip_check(lfric_vec_upload_r32(self%h, data), "lfric_vec_upload_r32")
+  end subroutine rvec_import
+
+  subroutine rvec_export(self, data)
+    class(hip_r_solver_field_vector_type), intent(in) :: self
+    real(r_solver), intent(out) :: data(:)
+    call hip_check(lfric_vec_download_r32(self%h, data), "lfric_vec_download_r32")
+  end subroutine rvec_export
+
+  subroutine rvec_set_scalar(self, scalar)
+    class(hip_r_solver_field_vector_type), intent(inout) :: self
+    real(kind=r_def), intent(in) :: scalar
+    call self%own()
+    call hip_check(lfric_vec_set_scalar(self%h, to_rsol(scalar)), "set_scalar")
+  end subroutine rvec_set_scalar
+
+  subroutine rvec_axpy(self, alpha, x)
+    class(hip_r_solver_field_vector_type), intent(inout) :: self
+    real(kind=r_def), intent(in) :: alpha
+    class(abstract_vector_type), intent(inout) :: x
+    call self%own()
+    call hip_check(lfric_vec_axpy(self%h, to_rsol(alpha), rhandle_of(x, "axpy")), "axpy")
+  end subroutine rvec_axpy
+
+  subroutine rvec_aypx(self, alpha, x)
+    class(hip_r_solver_field_vector_type), intent(inout) :: self
+    real(kind=r_def), intent(in) :: alpha
+    class(abstract_vector_type), intent(inout) :: x
+    call self%own()
+    call hip_check(lfric_vec_aypx(self%h, to_rsol(alpha), rhandle_of(x, "aypx")), "aypx")
+  end subroutine rvec_aypx
+
+  subroutine rvec_axpby(self, a, x, b, y)
+    class(hip_r_solver_field_vector_type), intent(inout) :: self
+    real(kind=r_def), intent(in) :: a, b
+    class(abstract_vector_type), intent(in) :: x, y
+    call self%own()
+    call hip_check(lfric_vec_axpby(self%h, to_rsol(a), rhandle_of(x, "axpby"), to_rsol(b), &
+                                   rhandle_of(y, "axpby")), "axpby")
+  end subroutine rvec_axpby
+
+  function rvec_norm(self) result(normal)
+    class(hip_r_solver_field_vector_type), intent(in) :: self
+    real(kind=r_def) :: normal
+    call hip_check(lfric_vec_norm(self%h, normal), "norm")
+  end function rvec_norm
+
+  function rvec_field_norm(self, n) result(normal)
+    class(hip_r_solver_field_vector_type), intent(in) :: self
+    integer(kind=i_def), intent(in) :: n
+    real(kind=r_def) :: normal
+    if (n /= 1) call log_event("hip_r_solver_field_vector_type holds a single field", LOG_LEVEL_ERROR)
+    call hip_check(lfric_vec_norm(self%h, normal), "field_norm")
+  end function rvec_field_norm
+
+  function rvec_dot(self, x) result(dot_prod)
+    class(hip_r_solver_field_vector_type), intent(in) :: self
+    class(abstract_vector_type), intent(in) :: x
+    real(kind=r_def) :: dot_prod
+    call hip_check(lfric_vec_dot(self%h, rhandle_of(x, "dot"), dot_prod), "dot")
+  end function rvec_dot
+
+  subroutine rvec_scale(self, scalar)
+    class(hip_r_solver_field_vector_type), intent(inout) :: self
+    real(kind=r_def), intent(in) :: scalar
+    call self%own()
+    call hip_check(lfric_vec_scale(self%h, to_rsol(scalar)), "scale")
+  end subroutine rvec_scale
+
+  subroutine rvec_duplicate(self, vec)
+    class(hip_r_solver_field_vector_type), intent(in) :: self
+    class(abstract_vector_type), allocatable, intent(inout) :: vec
+    if (allocated(vec)) deallocate(vec)
+    allocate(hip_r_solver_field_vector_type :: vec)
+    select type (vec)
+    type is (hip_r_solver_field_vector_type)
+      vec%fs = self%fs
+      vec%undf = self%undf
+      call hip_check(lfric_vec_create(vec%fs, LFRIC_R32, 1_c_int, vec%h), "duplicate")
+      vec%owner = loc(vec)
+    end select
+  end subroutine rvec_duplicate
+
+  !> intent(out) as copy_interface declares it; see vec_copy
+  subroutine rvec_copy(self, source)
+    class(hip_r_solver_field_vector_type), intent(out) :: self
+    class(abstract_vector_type), intent(in) :: source
+    select type (source)
+    type is (hip_r_solver_field_vector_type)
+      self%fs = source%fs
+      self%undf = source%undf
+      call hip_check(lfric_vec_create(self%fs, LFRIC_R32, 1_c_int, self%h), "copy")
+      self%owner = loc(self)
+      call hip_check(lfric_vec_copy(self%h, source%h), "copy")
+    class default
+      call log_event("hip_r_solver_field_vector_type:copy: source is not a hip_r_solver_field_vector_type", &
+                     LOG_LEVEL_ERROR)
+    end select
+  end subroutine rvec_copy
+
+  function rvec_size(self) result(n_fields)
+    class(hip_r_solver_field_vector_type), intent(in) :: self
+    integer(i_def) :: n_fields
+    n_fields = 1
+  end function rvec_size
+
+  !> device handle of a vector of either kind (operators and preconditioners take both)
+  function any_handle(x, what) result(h)
+    class(abstract_vector_type), intent(in) :: x
+    character(len=*), intent(in) :: what
+    type(c_ptr) :: h
+    h = c_null_ptr
+    select type (x)
+    type is (hip_field_vector_type)
+      h = x%h
+    type is (hip_r_solver_field_vector_type)
+      h = x%h
+    class default
+      call log_event(what // ": argument is not a HIP field vector", LOG_LEVEL_ERROR)
+    end select
+  end function any_handle
+
   ! ---- operator and preconditioner ------------------------------------------------------------
   function hip_lma_operator(op) result(self)
     type(c_ptr), intent(in) :: op
@@ -323,16 +535,23 @@ contains
     select type (y)
     type is (hip_field_vector_type)
       call y%own()
-      call hip_check(lfric_lma_apply(self%op, handle_of(x, "apply"), y%h), "lma_apply")
+      call hip_check(lfric_lma_apply(self%op, any_handle(x, "lma_apply"), y%h), "lma_apply")
+    type is (hip_r_solver_field_vector_type)
+      call y%own()
+      call hip_check(lfric_lma_apply(self%op, any_handle(x, "lma_apply"), y%h), "lma_apply")
     class default
-      call log_event("hip_lma_operator_type:apply: y is not a hip_field_vector_type", LOG_LEVEL_ERROR)
+      call log_event("hip_lma_operator_type:apply: y is not a HIP field vector", LOG_LEVEL_ERROR)
     end select
+    self%applies = self%applies + 1
   end subroutine op_apply
 
+  !> diag is a hip_field_vector_type or a hip_r_solver_field_vector_type; the preconditioner
+  !> then divides vectors of that kind
   function hip_diagonal_preconditioner(diag) result(self)
-    type(hip_field_vector_type), intent(in) :: diag
+    class(abstract_vector_type), intent(in) :: diag
     type(hip_diagonal_preconditioner_type) :: self
-    call hip_check(lfric_prec_create_diagonal(diag%h, self%p), "lfric_prec_create_diagonal")
+    call hip_check(lfric_prec_create_diagonal(any_handle(diag, "diagonal preconditioner"), self%p), &
+                   "lfric_prec_create_diagonal")
   end function hip_diagonal_preconditioner
 
   subroutine prec_apply(self, x, y)
@@ -342,9 +561,12 @@ contains
     select type (y)
     type is (hip_field_vector_type)
       call y%own()
-      call hip_check(lfric_prec_apply(self%p, handle_of(x, "prec"), y%h), "prec_apply")
+      call hip_check(lfric_prec_apply(self%p, any_handle(x, "prec"), y%h), "prec_apply")
+    type is (hip_r_solver_field_vector_type)
+      call y%own()
+      call hip_check(lfric_prec_apply(self%p, any_handle(x, "prec"), y%h), "prec_apply")
     class default
-      call log_event("hip_diagonal_preconditioner_type: y is not a hip_field_vector_type", LOG_LEVEL_ERROR)
+      call log_event("hip_diagonal_preconditioner_type: y is not a HIP field vector", LOG_LEVEL_ERROR)
     end select
   end subroutine prec_apply
 

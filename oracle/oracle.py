@@ -497,6 +497,53 @@ def cg(mesh, A, b, x0=None, diag=None, r_tol=1e-8, a_tol=1e-16, max_iter=500, mo
     return x, res, hist[: res.iterations + 1]
 
 
+def cg_r32(mesh, A, b, diag, r_tol=1e-8, a_tol=0.0, max_iter=500):
+    """cg_solve (components/science/source/solver/sci_iterative_solver_mod.f90:495-592, monitor on)
+    over r_solver vectors (sci_r_solver_field_vector_mod.x90:119-252): real32 storage, the
+    real(r_def) scalars rounded to real32 before use (:189,213-214,246), inner products of the
+    real32 entries accumulated in real64 (sci_psykal_light_mod.f90:72-214), the real32 LMA apply
+    (matrix_vector_code_r_single) and z = r / diag in real32.  axpy and aypx round once per
+    entry (a fused multiply-add: the product of two real32 values is exact in real64).
+    Returns (x, iterations, converged)."""
+    f32, f64 = np.float32, np.float64
+    d = np.asarray(diag, f32)
+    b = np.asarray(b, f32)
+
+    def apply(v):
+        return lma_apply(mesh, A, v, r32=True)
+
+    def dot(u, v):
+        return float(np.dot(u.astype(f64), v.astype(f64)))
+
+    def fma(a, u, v):  # v + real(a, r_solver) * u, rounded once
+        return (f64(f32(a)) * u.astype(f64) + v.astype(f64)).astype(f32)
+
+    x = np.zeros(mesh.undf, f32)
+    r = apply(x)
+    r = (f32(-1.0) * r).astype(f32)
+    r = fma(1.0, b, r)
+    r0 = np.sqrt(dot(r, r))
+    if r0 < np.finfo(f64).eps:
+        return x, 0, True
+    z = (r / d).astype(f32)
+    rz = dot(r, z)
+    p = z.copy()
+    for it in range(1, max_iter + 1):
+        z = apply(p)
+        alpha = rz / dot(p, z)
+        x = fma(alpha, p, x)
+        r = fma(-alpha, z, r)
+        rn = np.sqrt(dot(r, r))
+        if rn / r0 <= r_tol or rn <= a_tol:
+            return x, it, True
+        z = (r / d).astype(f32)
+        rz_new = dot(r, z)
+        beta = rz_new / rz
+        p = (f64(f32(beta)) * p.astype(f64) + z.astype(f64)).astype(f32)
+        rz = rz_new
+    return x, max_iter, False
+
+
 def gmres_callbacks(n, apply, prec, b, x0=None, gcrk=30, r_tol=1e-8, a_tol=1e-16, max_iter=50, monitor=True):
     """GMRES(gcrk) of the oracle with python-callable operator and preconditioner."""
 

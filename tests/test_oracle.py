@@ -758,3 +758,20 @@ def test_mass_matrix_wtheta_known_answer_and_invariants(orc):
     m3 = orc.mass_matrix_w3(1, 1, dmc, *chic, nqp=3)[0, 0, 0]
     np.testing.assert_array_equal(m, m.T)
     assert abs(m.sum() - m3) <= 1e-13 * abs(m3)
+
+
+def test_r_solver_cg_converges_like_fp64_cg(orc):
+    """The real32 r_solver CG restatement against scipy's FP64 solve of the same (real32-rounded)
+    system and the FP64 oracle CG: same solution to real32 accuracy, iteration count within one
+    of the FP64 count at a tolerance real32 can reach.  A dropped real32 rounding of alpha or a
+    wrong sign in the residual update fails the solution check."""
+    m = orc.Mesh(4, 3)
+    A32 = orc.gen_spd(8, m.ncell_3d, 0).astype(np.float32).astype(np.float64)
+    b = orc.gen_uniform(m.undf, 0, 3).astype(np.float32).astype(np.float64)
+    diag = orc.mm_diagonal(m, A32)
+    ref = spla.spsolve(orc.assembled_csr(m, A32).tocsc(), b)
+    x32, it32, ok = orc.cg_r32(m, A32, b, diag.astype(np.float32), r_tol=1e-5)
+    assert ok
+    assert np.linalg.norm(x32 - ref) <= 1e-4 * np.linalg.norm(ref)
+    _, res64, _ = orc.cg(m, A32, b, diag=diag, r_tol=1e-5, a_tol=0.0)
+    assert abs(it32 - res64.iterations) <= 1
