@@ -301,6 +301,16 @@ int lfric_residual_norm(const lfric_op *A, const lfric_vec *x, const lfric_vec *
  * FP32 vectors of A's precision; scalars in FP64 on the device. */
 int lfric_cg_solve(const lfric_op *A, const lfric_prec *P, lfric_vec *x, const lfric_vec *b,
                    const lfric_solver_opts *opts, lfric_solver_result *res);
+/* lfric_cg_solve without device allocation, for callers that cannot allocate (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  No counterpart in the
+ * reference.  Same kernels and result as lfric_cg_solve; its r, z, p vectors and block partials
+ * live in a static 64 MiB device array per GPU (3 vectors of undf values of A's precision plus
+ * the partials: undf up to ~2.79M in FP64, ~5.59M in FP32), LFRIC_E_UNSUPPORTED beyond that.
+ * The space's gather lists must exist already (built by the first lfric_lma_apply_gather,
+ * lfric_op_prepare_rows or solve on the operator), else LFRIC_E_STATE.  Every call on a GPU
+ * shares that GPU's array: calls that overlap on one GPU race. */
+int lfric_cg_solve_noalloc(const lfric_op *A, const lfric_prec *P, lfric_vec *x, const lfric_vec *b,
+                           const lfric_solver_opts *opts, lfric_solver_result *res);
 /* FP64 iterative refinement around FP32 (r_solver) CG: r = b - A64 x in FP64, inner CG on
  * A32 with FP32 vectors and FP64 dots, x += d in FP64.  No reference counterpart as one
  * call; the inner solve is cg_solve on r_solver_field_vector_type

@@ -131,6 +131,7 @@ SIGNATURES = {
     "lfric_solver_opts_default": (None, [ctypes.POINTER(SolverOpts)]),
     "lfric_residual_norm": (_I, [_VP, _VP, _VP, _DP]),
     "lfric_cg_solve": (_I, [_VP, _VP, _VP, _VP, ctypes.POINTER(SolverOpts), ctypes.POINTER(SolverRes)]),
+    "lfric_cg_solve_noalloc": (_I, [_VP, _VP, _VP, _VP, ctypes.POINTER(SolverOpts), ctypes.POINTER(SolverRes)]),
     "lfric_ir_cg_solve": (_I, [_VP, _VP, _VP, _VP, _VP, ctypes.POINTER(SolverOpts), ctypes.POINTER(SolverRes)]),
     "lfric_lma_apply_mrhs": (_I, [_VP, _VP, _VP, _I]),
     "lfric_op_tune": (_I, [_VP, _I, ctypes.POINTER(_I), _DP]),

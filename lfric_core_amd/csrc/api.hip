@@ -68,6 +68,16 @@ static Workspace g_ws[9];
 static SolverState *g_state = nullptr;
 static SolverState *g_state_host = nullptr;
 
+// Device memory of the allocation-free CG (lfric_cg_solve_noalloc): its r, z and p vectors and
+// the fused apply's p.Ap block partials, in place of workspace slots 0-2 and 4.  Every GPU the
+// library runs on holds its own copy until the process ends.  The array's name is a host-side
+// stand-in: host code uses only the address hipGetSymbolAddress returned for the device
+// (g_scratch_ptr), recorded by lfric_init with that device current.
+constexpr size_t kScratchBytes = size_t{64} << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+constexpr int kMaxDevices = 64;
+static void *g_scratch_ptr[kMaxDevices] = {};
+
 static int ws_get(int slot, size_t bytes, void **out) {
   Workspace &w = g_ws[slot];
   if (w.bytes < bytes) {
@@ -244,13 +254,34 @@ struct CgArgs {
 // cg_solve (components/science/source/solver/sci_iterative_solver_mod.f90:495-592) with every
 // vector and scalar on the device.  The host enqueues `check_every` iterations at a time and
 // then reads the device state once; kernels of iterations past convergence return at once.
-static int cg_device(const CgArgs &a, SolverState &out) {
+// Byte offsets of r, z, p and the p.Ap partials inside the allocation-free CG's scratch array:
+// each piece starts on a 256-byte boundary.  Returns the bytes used.
+static size_t cg_scratch_layout(size_t vb, int pz_blocks, size_t off[4]) {
+  const auto up = [](size_t v) { return (v + 255) / 256 * 256; };
+  off[0] = 0;
+  off[1] = up(vb);
+  off[2] = off[1] + up(vb);
+  off[3] = off[2] + up(vb);
+  return off[3] + up(sizeof(double) * static_cast<size_t>(std::max(pz_blocks, 1)));
+}
+
+// scratch_base: null for the grown workspace slots; else the device address of the
+// allocation-free CG's scratch array, already checked to hold cg_scratch_layout's bytes.
+static int cg_device(const CgArgs &a, SolverState &out, unsigned char *scratch_base = nullptr) {
   Context &c = ctx();
   const size_t vb = esize(a.prec) * static_cast<size_t>(a.n);
   void *r, *z, *p;
-  if (int rc = ws_get(0, vb, &r)) return rc;
-  if (int rc = ws_get(1, vb, &z)) return rc;
-  if (int rc = ws_get(2, vb, &p)) return rc;
+  size_t soff[4] = {0, 0, 0, 0};
+  if (scratch_base) {
+    cg_scratch_layout(vb, 1, soff);
+    r = scratch_base + soff[0];
+    z = scratch_base + soff[1];
+    p = scratch_base + soff[2];
+  } else {
+    if (int rc = ws_get(0, vb, &r)) return rc;
+    if (int rc = ws_get(1, vb, &z)) return rc;
+    if (int rc = ws_get(2, vb, &p)) return rc;
+  }
   SolverState s{};
   s.r_tol = a.r_tol;
   s.a_tol = a.a_tol;
@@ -274,7 +305,11 @@ static int cg_device(const CgArgs &a, SolverState &out) {
   if (fused) {
     if (int rc = launch_gather_blocks(a.A, &pz_blocks)) return rc;
     void *w;
-    if (int rc = ws_get(4, sizeof(double) * static_cast<size_t>(pz_blocks), &w)) return rc;
+    if (scratch_base) {
+      w = scratch_base + soff[3];
+    } else if (int rc = ws_get(4, sizeof(double) * static_cast<size_t>(pz_blocks), &w)) {
+      return rc;
+    }
     pz_partials = static_cast<double *>(w);
   }
   for (int ch = 0; ch < max_chunks; ++ch) {
@@ -340,6 +375,12 @@ int lfric_init(int device) {
   if (int rc = dmalloc(reinterpret_cast<void **>(&g_state), sizeof(SolverState))) return rc;
   LF_HIP(hipMemset(g_state, 0, sizeof(SolverState)));
   LF_HIP(hipHostMalloc(reinterpret_cast<void **>(&g_state_host), sizeof(SolverState), hipHostMallocDefault));
+  // this device's copy of the allocation-free CG's scratch array
+  LF_CHECK(device < kMaxDevices, LFRIC_E_UNSUPPORTED, "device index beyond the scratch table");
+  void *sp = nullptr;
+  LF_HIP(hipGetSymbolAddress(&sp, HIP_SYMBOL(scratch)));
+  LF_CHECK(sp, LFRIC_E_HIP, "hipGetSymbolAddress returned no address for the scratch array");
+  g_scratch_ptr[device] = sp;
   return LFRIC_OK;
 }
 
@@ -1017,6 +1058,38 @@ int lfric_cg_solve(const lfric_op *A, const lfric_prec *P, lfric_vec *x, const l
   if (int rc = cg_device(a, s)) return rc;
   int st = finish_status(s, *opts, res);
   if (res) res->true_rel_res = -1.0;  // not computed here: see lfric_residual_norm
+  return st;
+}
+
+int lfric_cg_solve_noalloc(const lfric_op *A, const lfric_prec *P, lfric_vec *x, const lfric_vec *b,
+                           const lfric_solver_opts *opts, lfric_solver_result *res) {
+  if (int rc = require_init()) return rc;
+  LF_CHECK(A && x && b && opts, LFRIC_E_ARG, "null argument");
+  LF_CHECK(A->to == A->from, LFRIC_E_ARG, "CG needs a square operator on one space");
+  if (int rc = check_apply(A, x, b, false)) return rc;
+  const void *diag;
+  if (int rc = prec_diag_ptr(P, A, &diag)) return rc;
+  unsigned char *base = static_cast<unsigned char *>(g_scratch_ptr[ctx().device]);
+  LF_CHECK(base, LFRIC_E_STATE, "cg_solve_noalloc: no scratch address recorded for this device");
+  const bool fused = ctx().apply_form == 1;
+  const bool listed = A->to->d_gcol != nullptr;
+  size_t off[4];
+  const size_t need = cg_scratch_layout(esize(A->precision) * static_cast<size_t>(x->n),
+                                        fused && listed ? gather_blocks(A) : 1, off);
+  LF_CHECK(need <= kScratchBytes, LFRIC_E_UNSUPPORTED,
+           "cg_solve_noalloc: the problem needs " + std::to_string(need) + " bytes of scratch, more than its " +
+               std::to_string(kScratchBytes) + "-byte maximum (3 vectors of undf values plus the block partials)");
+  // the gather form's column lists are built on the first apply into a space (an allocation);
+  // this entry point only uses them
+  LF_CHECK(!fused || listed, LFRIC_E_STATE,
+           "cg_solve_noalloc: the space's gather lists are not built yet (they are built by the first "
+           "lfric_lma_apply_gather, lfric_op_prepare_rows or solve on the operator)");
+  CgArgs a{A, diag, x->d, b->d, x->n, A->precision, false, opts->r_tol, opts->a_tol, opts->max_iter,
+           opts->monitor_convergence, opts->check_every};
+  SolverState s{};
+  if (int rc = cg_device(a, s, base)) return rc;
+  int st = finish_status(s, *opts, res);
+  if (res) res->true_rel_res = -1.0;
   return st;
 }
 

@@ -41,6 +41,8 @@ int launch_lma_apply_gather(const lfric_op *op, const void *x, void *y);
 // operators); stops at once when *skip is set
 int launch_lma_apply_gather_dot(const lfric_op *op, const void *x, void *y, double *partials, const int *skip);
 int launch_gather_blocks(const lfric_op *op, int *nblocks);
+// blocks of the gather apply's grid, once the space's column lists exist
+int gather_blocks(const lfric_op *op);
 int launch_build_rows(lfric_op *op);
 // the apply form the solvers use (lfric_set_apply_form)
 int launch_lma_apply_solver(const lfric_op *op, const void *x, void *y, const int *skip);
