@@ -24,6 +24,10 @@ import numpy as np
 
 ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
+# thread placement of the OpenMP CPU baseline: one thread per physical core, packed; read by the
+# OpenMP runtime when the reference library is first loaded
+os.environ.setdefault("OMP_PROC_BIND", "close")
+os.environ.setdefault("OMP_PLACES", "cores")
 
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per FP32 apply of C96x70 W0 from the rocprofv3 FETCH_SIZE / WRITE_SIZE runs recorded in
@@ -73,11 +77,41 @@ def time_applies(L, op, x, y, reps, method="apply"):
     return L.timer_stop() / reps
 
 
-def cpu_baseline(fs, op64, b_host, iters_needed, budget_s):
+def host_cpu_topology():
+    """Physical cores and logical CPUs this process may run on (sysfs topology of the CPUs in
+    its affinity mask, the figures lscpu reports)."""
+    cpus = sorted(os.sched_getaffinity(0))
+    cores = set()
+    for c in cpus:
+        base = Path(f"/sys/devices/system/cpu/cpu{c}/topology")
+        try:
+            cores.add((int((base / "physical_package_id").read_text()), int((base / "core_id").read_text())))
+        except OSError:
+            cores.add((0, c))
+    return {"physical_cores": len(cores), "logical_cpus": len(cpus)}
+
+
+def _ref_cg(R, m, A, diag, b_host, r_tol, max_iter):
+    """One FP64 Jacobi-PCG solve from x = 0 with the reference's matrix_vector_code in a
+    coloured OpenMP loop (oracle/_ref); returns (seconds, iterations)."""
+    P = lambda a: ctypes.c_void_p(a.ctypes.data)
+    I = ctypes.c_int
+    x = np.zeros(m.undf)
+    t = time.perf_counter()
+    it = R.ref_cg_r64(I(m.ncells), I(m.nlayers), I(4), P(m.ncells_per_colour), P(m.cells_in_colour),
+                      I(m.cells_in_colour.shape[1]), I(8), I(m.undf), P(m.dofmap), P(A), P(diag), P(b_host), P(x),
+                      ctypes.c_double(r_tol), I(max_iter))
+    return time.perf_counter() - t, it
+
+
+def cpu_baseline(fs, op64, b_host, iters_needed, budget_s, small=None):
     """The reference's own matrix_vector_code (compiled from its Fortran source into
-    oracle/_ref/libref.so) in a coloured OpenMP loop, inside cg_solve's operation order,
-    on the host cores.  Bounded sample: a fixed number of CG iterations at full size; the
-    solve time is that per-iteration time times the iterations the FP64 GPU solve needed."""
+    oracle/_ref/libref.so) in a coloured OpenMP loop, inside cg_solve's operation order, on the
+    host cores (OMP_PROC_BIND=close, OMP_PLACES=cores).  Bounded sample of the timed workload: a
+    fixed number of CG iterations at full size; the solve time is extrapolated as that
+    per-iteration time times the iterations the FP64 GPU solve needed.  `small` (the C12x20
+    plain FP64 workload: mesh, matrix, diagonal, b) is solved in full, median of 5 solves after
+    one warm-up."""
     lib_path = ROOT / "oracle" / "_ref" / "libref.so"
     if not lib_path.exists():
         return None
@@ -89,31 +123,67 @@ def cpu_baseline(fs, op64, b_host, iters_needed, budget_s):
     A3 = op64.download()
     A = O.flat_matrix(A3)
     diag = O.mm_diagonal(m, A3)
-    P = lambda a: ctypes.c_void_p(a.ctypes.data)
-    I = ctypes.c_int
+    del A3
 
-    def run(iters):
-        x = np.zeros(m.undf)
-        t = time.perf_counter()
-        R.ref_cg_r64(I(m.ncells), I(m.nlayers), I(4), P(m.ncells_per_colour), P(m.cells_in_colour),
-                     I(m.cells_in_colour.shape[1]), I(8), I(m.undf), P(m.dofmap), P(A), P(diag), P(b_host), P(x),
-                     ctypes.c_double(1e-30), I(iters))
-        return time.perf_counter() - t
-
-    t2 = run(2)
+    t2, _ = _ref_cg(R, m, A, diag, b_host, 1e-30, 2)
     n = max(3, min(iters_needed, int(budget_s / max(t2 / 2, 1e-6))))
-    t = run(n)
+    t, _ = _ref_cg(R, m, A, diag, b_host, 1e-30, n)
     per_iter = t / n
     solve_s = per_iter * iters_needed
-    return {
+    out = {
         "value": fs.undf / solve_s,
         "unit": "solved dofs/s",
         "cores": R.ref_num_threads(),
         "kind": "reference",
+        "extrapolated": True,
+        "host": dict(host_cpu_topology(), omp_proc_bind=os.environ.get("OMP_PROC_BIND"),
+                     omp_places=os.environ.get("OMP_PLACES")),
         "sample": (f"{n} FP64 Jacobi-PCG iterations of the full {m.undf}-dof system ({t:.1f} s, "
                    f"{per_iter * 1e3:.1f} ms/iteration) with the reference matrix_vector_code in a coloured "
-                   f"OpenMP loop; solve time = per-iteration time x {iters_needed} iterations of the FP64 solve"),
+                   f"OpenMP loop; solve time extrapolated = per-iteration time x {iters_needed} iterations "
+                   f"of the FP64 solve"),
     }
+    if small is not None:
+        A3s, bs = small
+        ms = O.Mesh(12, 20)
+        As, ds = O.flat_matrix(A3s), O.mm_diagonal(ms, A3s)
+        _ref_cg(R, ms, As, ds, bs, 1e-8, 500)  # warm-up
+        runs = [_ref_cg(R, ms, As, ds, bs, 1e-8, 500) for _ in range(5)]
+        med = float(np.median([r[0] for r in runs]))
+        out["c12x20_fp64_full_solves"] = {
+            "median_ms": med * 1e3, "solved_dofs_per_s": ms.undf / med, "iterations": runs[0][1],
+            "undf": ms.undf, "solves": len(runs), "all_ms": [r[0] * 1e3 for r in runs],
+            "workload": "C12x20 uniform W0 order 0, A_e = B_e B_e^T + 0.1 I (B_e ~ N(0,1)/8, seed 0), "
+                        "b ~ U(-1,1), Jacobi PCG to 1e-8, at most 500 iterations"}
+    return out
+
+
+def small_fp64_report(L):
+    """The C12x20 plain FP64 workload (the reference's CPU-runnable case): the GPU FP64 and mixed
+    solves (median of 5 after one warm-up), and its element matrices and b on the host for the
+    CPU solves beside them."""
+    fs = L.FunctionSpace.cubedsphere(12, 20)
+    op64 = L.LmaOperator(fs).generate_spd(0)
+    op32 = op64.convert(L.R32)
+    prec64 = L.DiagonalPreconditioner.from_operator(op64)
+    prec32 = L.DiagonalPreconditioner.from_operator(op32)
+    b = L.FieldVector(fs).fill_uniform(0, 3)
+    x = L.FieldVector(fs)
+    out = {}
+    for name, solver in (("gpu_fp64", L.ConjugateGradient(op64, prec64, 1e-8, 0.0, 500)),
+                         ("gpu_mixed", L.MixedPrecisionCG(op64, op32, prec32, 1e-8, 0.0, 500, inner_r_tol=1e-4))):
+        ts = []
+        for k in range(6):
+            x.set_scalar(0.0)
+            L.synchronize()
+            t = time.perf_counter()
+            res = solver.apply(x, b)
+            L.synchronize()
+            if k:
+                ts.append(time.perf_counter() - t)
+        med = float(np.median(ts))
+        out[name] = {"median_ms": med * 1e3, "solved_dofs_per_s": fs.undf / med, "iterations": res.iterations}
+    return out, (op64.download(), b.export_field())
 
 
 FP64_VECTOR_PEAK_TFLOPS = 78.6
@@ -413,9 +483,11 @@ def main():
                                 else "k_lma_colour<float,8,8,RolesW0K0> (4 colour launches per apply)")
                                + "; bytes, time and traffic per apply"},
     }
+    small, small_host = small_fp64_report(L)
+    line["c12x20_fp64"] = small
     if not args.no_cpu_baseline:
         bh = b.export_field()
-        line["cpu_baseline"] = cpu_baseline(fs, op64, bh, pres.iterations, args.cpu_budget)
+        line["cpu_baseline"] = cpu_baseline(fs, op64, bh, pres.iterations, args.cpu_budget, small_host)
     print(json.dumps(line))
 
 

@@ -104,6 +104,13 @@ int lfric_fs_info(const lfric_fs *fs, int *ncells, int *nlayers, int *ndf, int *
                   int *ncolours);
 int lfric_fs_get_dofmap(const lfric_fs *fs, int *dofmap);
 int lfric_fs_get_colouring(const lfric_fs *fs, int *ncells_per_colour, int *cells_in_colour);
+/* get_boundary_dofs (infrastructure/source/function_space/function_space_mod.F90:791-801):
+ * boundary_dofs(ndf, 2) in Fortran order, 0 for a dof on the bottom (column 1) or top (column 2)
+ * boundary, else 1.  Generated spaces carry the reference's flags
+ * (function_space_constructor_helper_functions_mod.F90:678-680, 1048-1049: lowest-order W2
+ * flags faces B and T; W0 and W3 none); the setter gives a caller-built space its own. */
+int lfric_fs_get_boundary_dofs(const lfric_fs *fs, int *boundary_dofs);
+int lfric_fs_set_boundary_dofs(lfric_fs *fs, const int *boundary_dofs);
 void lfric_fs_destroy(lfric_fs *fs);
 
 /* ---- field vectors (vector_mod abstract_vector_type, infrastructure/source/field/
@@ -230,6 +237,14 @@ int lfric_lma_apply_transpose(const lfric_op *op, const lfric_vec *x, lfric_vec 
 /* dg_matrix_vector_kernel (components/science/source/kernel/algebra/
  * dg_matrix_vector_kernel_mod.F90): discontinuous output, no colouring. */
 int lfric_dg_apply(const lfric_op *op, const lfric_vec *x, lfric_vec *y);
+/* dg_inc_matrix_vector_kernel (components/science/source/kernel/algebra/
+ * dg_inc_matrix_vector_kernel_mod.F90:69-142): y = y + A x on a discontinuous output space
+ * (GH_READWRITE, no colouring; nothing is zeroed). */
+int lfric_dg_inc_apply(const lfric_op *op, const lfric_vec *x, lfric_vec *y);
+/* enforce_bc_kernel (components/science/source/kernel/algebra/sci_enforce_bc_kernel_mod.F90:
+ * 63-91) over every cell column: the vector's dofs that its space flags in boundary_dofs are
+ * multiplied by 0 (bottom level, top layer); the others keep their values. */
+int lfric_enforce_bc(lfric_vec *v);
 /* mm_diagonal_kernel (components/science/source/kernel/algebra/sci_mm_diagonal_kernel_mod.F90
  * :68-125): assembled diagonal of a square LMA operator. */
 int lfric_mm_diagonal(const lfric_op *op, lfric_vec *diag);
@@ -249,6 +264,10 @@ void lfric_prec_destroy(lfric_prec *p);
 /* y = A x for a square LMA operator (mass_matrix_operator_type,
  * components/science/source/algorithm/solver/sci_mass_matrix_operator_alg_mod.x90:72-119). */
 int lfric_linop_create_lma(const lfric_op *A, lfric_linop **L);
+/* mass_matrix_operator_type(mass_matrix, apply_bc) (sci_mass_matrix_operator_alg_mod.x90:42-63,
+ * apply :72-119): y = A x, then enforce_bc on y when apply_bc is set (the mass-matrix solver
+ * sets it for W1/W2/W2v, sci_mass_matrix_solver_alg_mod.x90:186-191). */
+int lfric_linop_create_mass_matrix(const lfric_op *A, int apply_bc, lfric_linop **L);
 /* Helmholtz-like reduction of a W2/W3 saddle system on W3:
  *   y = M3 x + tau * D (diag(M2)^-1 (D^T x)),
  * D: W2 -> W3 LMA operator (dg_matrix_vector / transpose_matrix_vector kernels),

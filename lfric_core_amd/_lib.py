@@ -131,6 +131,11 @@ SIGNATURES = {
     "lfric_solver_opts_default": (None, [ctypes.POINTER(SolverOpts)]),
     "lfric_residual_norm": (_I, [_VP, _VP, _VP, _DP]),
     "lfric_cg_solve": (_I, [_VP, _VP, _VP, _VP, ctypes.POINTER(SolverOpts), ctypes.POINTER(SolverRes)]),
+    "lfric_fs_get_boundary_dofs": (_I, [_VP, _VP]),
+    "lfric_fs_set_boundary_dofs": (_I, [_VP, _VP]),
+    "lfric_dg_inc_apply": (_I, [_VP, _VP, _VP]),
+    "lfric_enforce_bc": (_I, [_VP]),
+    "lfric_linop_create_mass_matrix": (_I, [_VP, _I, ctypes.POINTER(_VP)]),
     "lfric_cg_solve_noalloc": (_I, [_VP, _VP, _VP, _VP, ctypes.POINTER(SolverOpts), ctypes.POINTER(SolverRes)]),
     "lfric_ir_cg_solve": (_I, [_VP, _VP, _VP, _VP, _VP, ctypes.POINTER(SolverOpts), ctypes.POINTER(SolverRes)]),
     "lfric_lma_apply_mrhs": (_I, [_VP, _VP, _VP, _I]),

@@ -233,6 +233,13 @@ static int space_from_topology(const std::shared_ptr<Topology> &t, int nlayers, 
     const int col = t->colour_of_cell[c - 1] - 1;
     f.cic[static_cast<size_t>(col) * f.maxcc + fill[col]++] = c;
   }
+  // dof_on_vert_boundary of the generated lowest-order spaces
+  // (function_space_constructor_helper_functions_mod.F90:678-680, 1048-1049): W2 flags its
+  // bottom face (dof 5) at the bottom and its top face (dof 6) at the top; W0 and W3 none
+  if (space == LFRIC_W2 && order == 0 && f.ndf == 6) {
+    f.bc_bottom = 1u << 4;
+    f.bc_top = 1u << 5;
+  }
   if (int rc = finish_space(fs)) return rc;
   *out = new lfric_fs{fs};
   return LFRIC_OK;
@@ -249,6 +256,7 @@ struct CgArgs {
   bool x_zero;
   double r_tol, a_tol;
   int max_iter, monitor, check_every;
+  bool bc = false;  // enforce the space's boundary conditions after every apply (mass-matrix operator)
 };
 
 // cg_solve (components/science/source/solver/sci_iterative_solver_mod.f90:495-592) with every
@@ -292,6 +300,8 @@ static int cg_device(const CgArgs &a, SolverState &out, unsigned char *scratch_b
     if (int rc = launch_copy(a.prec, r, a.b, a.n)) return rc;
   } else {
     if (int rc = launch_lma_apply_solver(a.A, a.x, r, nullptr)) return rc;
+    if (a.bc)
+      if (int rc = launch_enforce_bc(*a.A->to, a.prec, r, nullptr)) return rc;
     if (int rc = launch_cg_residual(a.prec, r, a.b, a.n)) return rc;
   }
   if (int rc = launch_cg_init_norm(a.prec, r, a.n, g_state)) return rc;
@@ -299,7 +309,7 @@ static int cg_device(const CgArgs &a, SolverState &out, unsigned char *scratch_b
   const int every = std::max(1, a.check_every);
   const int max_chunks = a.max_iter / every + 2;
   // gather form: z = A p and the partial sums of p.z in one kernel
-  const bool fused = c.apply_form == 1 && a.A->to == a.A->from;
+  const bool fused = c.apply_form == 1 && a.A->to == a.A->from && !a.bc;
   double *pz_partials = nullptr;
   int pz_blocks = 0;
   if (fused) {
@@ -319,6 +329,8 @@ static int cg_device(const CgArgs &a, SolverState &out, unsigned char *scratch_b
         if (int rc = launch_cg_finish_pz(pz_partials, pz_blocks, g_state)) return rc;
       } else {
         if (int rc = launch_lma_apply_solver(a.A, p, z, &g_state->done)) return rc;
+        if (a.bc)
+          if (int rc = launch_enforce_bc(*a.A->to, a.prec, z, &g_state->done)) return rc;
         if (int rc = launch_cg_dot_pz(a.prec, p, z, a.n, g_state)) return rc;
       }
       if (int rc = launch_cg_update(a.prec, a.x, r, z, p, a.diag, a.n, g_state)) return rc;
@@ -971,6 +983,45 @@ int lfric_dg_apply(const lfric_op *op, const lfric_vec *x, lfric_vec *y) {
   return launch_dg_apply(op, x->d, y->d);
 }
 
+int lfric_dg_inc_apply(const lfric_op *op, const lfric_vec *x, lfric_vec *y) {
+  if (int rc = require_init()) return rc;
+  if (int rc = check_apply(op, x, y, false)) return rc;
+  return launch_dg_inc_apply(op, x->d, y->d);
+}
+
+int lfric_fs_set_boundary_dofs(lfric_fs *fs, const int *bv) {
+  if (int rc = require_init()) return rc;
+  LF_CHECK(fs && bv, LFRIC_E_ARG, "null argument");
+  FunctionSpace &f = *fs->fs;
+  uint32_t bot = 0, top = 0;
+  for (int df = 0; df < f.ndf; ++df) {
+    LF_CHECK((bv[df] == 0 || bv[df] == 1) && (bv[f.ndf + df] == 0 || bv[f.ndf + df] == 1), LFRIC_E_ARG,
+             "boundary_dofs entries must be 0 or 1");
+    if (bv[df] == 0) bot |= 1u << df;
+    if (bv[f.ndf + df] == 0) top |= 1u << df;
+  }
+  f.bc_bottom = bot;
+  f.bc_top = top;
+  return LFRIC_OK;
+}
+
+int lfric_fs_get_boundary_dofs(const lfric_fs *fs, int *bv) {
+  LF_CHECK(fs && bv, LFRIC_E_ARG, "null argument");
+  const FunctionSpace &f = *fs->fs;
+  for (int df = 0; df < f.ndf; ++df) {
+    bv[df] = (f.bc_bottom >> df) & 1u ? 0 : 1;
+    bv[f.ndf + df] = (f.bc_top >> df) & 1u ? 0 : 1;
+  }
+  return LFRIC_OK;
+}
+
+int lfric_enforce_bc(lfric_vec *v) {
+  if (int rc = require_init()) return rc;
+  LF_CHECK(v, LFRIC_E_ARG, "null argument");
+  LF_CHECK(v->nrhs == 1, LFRIC_E_UNSUPPORTED, "enforce_bc takes one-RHS vectors");
+  return launch_enforce_bc(*v->fs, v->precision, v->d, nullptr);
+}
+
 int lfric_mm_diagonal(const lfric_op *op, lfric_vec *diag) {
   if (int rc = require_init()) return rc;
   LF_CHECK(op && diag && diag->fs->undf == op->to->undf && diag->precision == op->precision, LFRIC_E_ARG,
@@ -1170,3 +1221,26 @@ int lfric_ir_cg_solve(const lfric_op *A64, const lfric_op *A32, const lfric_prec
 }
 
 }  // extern "C"
+
+// CG on an LMA linear operator (the CG case of lfric_solve); a mass-matrix operator that applies
+// boundary conditions takes the unfused iteration with enforce_bc after each apply, as
+// mass_matrix_operator_type%apply does (sci_mass_matrix_operator_alg_mod.x90:98-101).
+namespace lfric {
+int cg_solve_linop(const lfric_linop *L, const lfric_prec *P, lfric_vec *x, const lfric_vec *b,
+                   const lfric_solver_opts *opts, lfric_solver_result *res) {
+  if (!L->apply_bc) return lfric_cg_solve(L->A, P, x, b, opts, res);
+  const lfric_op *A = L->A;
+  LF_CHECK(x && b && opts, LFRIC_E_ARG, "null argument");
+  if (int rc = check_apply(A, x, b, false)) return rc;
+  const void *diag;
+  if (int rc = prec_diag_ptr(P, A, &diag)) return rc;
+  CgArgs a{A, diag, x->d, b->d, x->n, A->precision, false, opts->r_tol, opts->a_tol, opts->max_iter,
+           opts->monitor_convergence, opts->check_every};
+  a.bc = true;
+  SolverState s{};
+  if (int rc = cg_device(a, s)) return rc;
+  int st = finish_status(s, *opts, res);
+  if (res) res->true_rel_res = -1.0;
+  return st;
+}
+}  // namespace lfric

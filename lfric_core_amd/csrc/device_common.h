@@ -48,14 +48,20 @@ __device__ inline void block_sum(double (&v)[NV], double (&out)[NV]) {
 }
 
 // Every block of a reduction kernel writes its partial sums; a one-block finishing kernel
-// (k_finish) adds them in block order, so sums are deterministic run to run.
+// (k_finish) adds them in block order, so sums are deterministic run to run.  Kernels whose
+// blocks work on a remapped block id (xcd_block) pass that id, so the order of the sum follows
+// the data and not the launch's XCD window.
 template <int NV>
-__device__ inline void block_partials(double (&v)[NV], double *partials) {
+__device__ inline void block_partials(double (&v)[NV], double *partials, int block) {
   double bs[NV];
   block_sum<NV>(v, bs);
   if (threadIdx.x == 0)
 #pragma unroll
-    for (int q = 0; q < NV; ++q) partials[blockIdx.x * NV + q] = bs[q];
+    for (int q = 0; q < NV; ++q) partials[static_cast<size_t>(block) * NV + q] = bs[q];
+}
+template <int NV>
+__device__ inline void block_partials(double (&v)[NV], double *partials) {
+  block_partials<NV>(v, partials, static_cast<int>(blockIdx.x));
 }
 
 template <typename T>

@@ -82,6 +82,9 @@ struct FunctionSpace {
   // column roles: df is "top" when map(df) = map(bottom)+1 in every cell
   uint32_t top_mask = 0;
   std::vector<int> partner;                // bottom df -> top df or -1
+  // dof_on_vert_boundary (function_space_mod.F90:791-801): bit df set when boundary_value(df,1)
+  // (bottom level) or boundary_value(df,2) (top layer) is 0
+  uint32_t bc_bottom = 0, bc_top = 0;
   // operator element slots: column pos owns ebase[pos] + 0..nlayers; colours start 64-aligned
   std::vector<int> ebase;
   long long nelem_padded = 0;
@@ -174,6 +177,7 @@ struct lfric_prec {
 
 struct lfric_linop {
   int kind = 0;  // 0 LMA, 1 Helmholtz composite
+  int apply_bc = 0;  // kind 0: enforce the space's boundary conditions after each apply
   int precision = LFRIC_R64;
   const lfric_op *A = nullptr;
   const lfric_op *M3 = nullptr, *D = nullptr;

@@ -62,6 +62,9 @@ bool mfma_apply_supported(const lfric_op *op);
 int batch_colour_form(const lfric_op *op);
 int launch_lma_transpose(const lfric_op *op, const void *x, void *y);
 int launch_dg_apply(const lfric_op *op, const void *x, void *y);
+int launch_dg_inc_apply(const lfric_op *op, const void *x, void *y);
+// enforce_bc on a vector of space f (no-op when the space flags no boundary dofs)
+int launch_enforce_bc(const FunctionSpace &f, int prec, void *y, const int *skip);
 int launch_mm_diagonal(const lfric_op *op, void *diag);
 
 // vector kernels (precision of the vectors, n elements)
@@ -86,6 +89,11 @@ int launch_cg_update(int prec, void *x, void *r, void *z, const void *p, const v
                      long long n, SolverState *st);
 int launch_cg_aypx(int prec, void *p, const void *z, long long n, SolverState *st);
 int launch_cg_finish_pz(const double *partials, int nblocks, SolverState *st);
+
+// CG on an LMA linear operator, with the mass-matrix operator's boundary conditions when it
+// applies them (api.hip)
+int cg_solve_linop(const lfric_linop *A, const lfric_prec *P, lfric_vec *x, const lfric_vec *b,
+                   const lfric_solver_opts *opts, lfric_solver_result *res);
 
 // abstract operators and preconditioners (krylov.hip)
 int linop_apply(const lfric_linop *L, const void *x, void *y);

@@ -607,7 +607,8 @@ __global__ __launch_bounds__(kBlock) void k_lma_gather(const T *__restrict__ A, 
                                                        const int *skip, int xwin) {
   if (skip && *skip) return;
   const int H = nl + 1;
-  const long long t = static_cast<long long>(xcd_block(blockIdx.x, gridDim.x, xwin)) * kBlock + threadIdx.x;
+  const int lb = xcd_block(blockIdx.x, gridDim.x, xwin);
+  const long long t = static_cast<long long>(lb) * kBlock + threadIdx.x;
   bool valid = t < static_cast<long long>(ncols) * H;
   const int col = valid ? static_cast<int>(t / H) : 0;
   const int m = valid ? static_cast<int>(t - static_cast<long long>(col) * H) : 0;
@@ -662,7 +663,7 @@ __global__ __launch_bounds__(kBlock) void k_lma_gather(const T *__restrict__ A, 
   }
   if constexpr (DOT) {
     double v[1] = {valid ? to_d(acc) * to_d(x[static_cast<size_t>(ci.x) + m]) : 0.0};
-    block_partials<1>(v, partials);
+    block_partials<1>(v, partials, lb);
   }
 }
 
@@ -730,7 +731,8 @@ __global__ __launch_bounds__(kBlock) void k_lma_gather_rows_w0(const T *__restri
   constexpr int V = chunk_entries<T>(), RS = rows_rs<T, 8>();
   typedef T VecT __attribute__((ext_vector_type(V)));
   const int H = nl + 1;
-  const long long t = static_cast<long long>(xcd_block(blockIdx.x, gridDim.x, xwin)) * kBlock + threadIdx.x;
+  const int lb = xcd_block(blockIdx.x, gridDim.x, xwin);
+  const long long t = static_cast<long long>(lb) * kBlock + threadIdx.x;
   bool valid = t < static_cast<long long>(ncols) * H;
   const int col = valid ? static_cast<int>(t / H) : 0;
   const int m = valid ? static_cast<int>(t - static_cast<long long>(col) * H) : 0;
@@ -777,7 +779,7 @@ __global__ __launch_bounds__(kBlock) void k_lma_gather_rows_w0(const T *__restri
   }
   if constexpr (DOT) {
     double v[1] = {valid ? to_d(acc) * to_d(x[static_cast<size_t>(ci.x) + m]) : 0.0};
-    block_partials<1>(v, partials);
+    block_partials<1>(v, partials, lb);
   }
 }
 
@@ -964,7 +966,8 @@ __global__ __launch_bounds__(kBlock) void k_sym_gather(const T *__restrict__ z, 
                                                        int ncols, int nl, double *partials, const int *skip, int xwin) {
   if (skip && *skip) return;
   const int H = nl + 1;
-  const long long t = static_cast<long long>(xcd_block(blockIdx.x, gridDim.x, xwin)) * kBlock + threadIdx.x;
+  const int lb = xcd_block(blockIdx.x, gridDim.x, xwin);
+  const long long t = static_cast<long long>(lb) * kBlock + threadIdx.x;
   bool valid = t < static_cast<long long>(ncols) * H;
   const int col = valid ? static_cast<int>(t / H) : 0;
   const int m = valid ? static_cast<int>(t - static_cast<long long>(col) * H) : 0;
@@ -985,7 +988,7 @@ __global__ __launch_bounds__(kBlock) void k_sym_gather(const T *__restrict__ z, 
   }
   if constexpr (DOT) {
     double v[1] = {valid ? to_d(acc) * to_d(x[static_cast<size_t>(ci.x) + m]) : 0.0};
-    block_partials<1>(v, partials);
+    block_partials<1>(v, partials, lb);
   }
 }
 
@@ -1468,6 +1471,50 @@ Roles roles_of(const FunctionSpace &fs) {
 }
 
 // ----- launchers ---------------------------------------------------------------------------
+// ---- dg_inc apply and boundary conditions ------------------------------------------------------
+// dg_inc_matrix_vector_code (components/science/source/kernel/algebra/
+// dg_inc_matrix_vector_kernel_mod.F90:69-142): y(map1(df)+k) += sum over df2, in df2 order, of
+// A(ik+k,df,df2) x(map2(df2)+k).  The output space is discontinuous (each output dof belongs to
+// one cell), so thread (cell position, layer k) owns its outputs: no colours, no atomics.  Each
+// output's chain starts from the value y holds, as the reference's statement does.
+template <typename T, int N1, int N2>
+__global__ __launch_bounds__(kBlock) void k_dg_inc(const T *__restrict__ A, size_t gs, const int *__restrict__ ebase,
+                                                   const T *__restrict__ x, T *__restrict__ y,
+                                                   const int *__restrict__ map1, const int *__restrict__ map2,
+                                                   int npos, int nl, int n1rt, int n2rt) {
+  const long long t = static_cast<long long>(blockIdx.x) * kBlock + threadIdx.x;
+  if (t >= static_cast<long long>(npos) * nl) return;
+  const int pos = static_cast<int>(t / nl), k = static_cast<int>(t - static_cast<long long>(pos) * nl);
+  const int n1 = N1 > 0 ? N1 : n1rt, n2 = N2 > 0 ? N2 : n2rt;
+  const long long e = static_cast<long long>(ebase[pos]) + k;
+  for (int i = 0; i < n1; ++i) {
+    const size_t d = static_cast<size_t>(map1[static_cast<size_t>(pos) * n1 + i]) + k;
+    T acc = y[d];
+    for (int j = 0; j < n2; ++j)
+      acc = fma(A[aidx<T>(gs, e, i * n2 + j)], x[static_cast<size_t>(map2[static_cast<size_t>(pos) * n2 + j]) + k], acc);
+    y[d] = acc;
+  }
+}
+
+// enforce_bc_code (components/science/source/kernel/algebra/sci_enforce_bc_kernel_mod.F90:63-91):
+// field(map(df)) *= boundary_value(df,1) and field(map(df)+nlayers-1) *= boundary_value(df,2) in
+// every cell column.  The flags are 0 or 1; a product with 1 leaves the value as it is, so only
+// the dofs flagged 0 are touched (one thread per cell position, the bits of `bottom` / `top`).
+// Two cells that flag one shared dof both write x*0 of the same x: the result does not depend
+// on their order, because (x*0)*0 == x*0.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_enforce_bc(T *__restrict__ y, const int *__restrict__ map, int npos, int ndf,
+                                                       int nl, uint32_t bottom, uint32_t top, const int *skip) {
+  if (skip && *skip) return;
+  const int pos = blockIdx.x * kBlock + threadIdx.x;
+  if (pos >= npos) return;
+  for (int df = 0; df < ndf; ++df) {
+    const size_t b = static_cast<size_t>(map[static_cast<size_t>(pos) * ndf + df]);
+    if ((bottom >> df) & 1u) y[b] = y[b] * T(0);
+    if ((top >> df) & 1u) y[b + nl - 1] = y[b + nl - 1] * T(0);
+  }
+}
+
 #define LAUNCH(kernel, grid, ...) hipLaunchKernelGGL((kernel), dim3(grid), dim3(kBlock), 0, ctx().stream, __VA_ARGS__)
 #define CHECK_LAUNCH() LF_HIP(hipGetLastError())
 
@@ -2400,6 +2447,51 @@ static int dg_t(const lfric_op *op, const T *x, T *y) {
   DG_CASE(1, 8)
 #undef DG_CASE
   return fail(LFRIC_E_UNSUPPORTED, "dg apply: unsupported (ndf_to, ndf_from)");
+}
+
+template <typename T>
+static int dg_inc_t(const lfric_op *op, const T *x, T *y) {
+  const FunctionSpace &to = *op->to, &from = *op->from;
+  LF_CHECK(to.top_mask == 0, LFRIC_E_ARG, "dg_inc apply needs a discontinuous output space");
+  LF_CHECK(to.nlayers == from.nlayers && to.ncells == from.ncells, LFRIC_E_ARG, "spaces of different meshes");
+  const int npos = to.ncells, nl = to.nlayers;
+  const int grid = grid_for(static_cast<long long>(npos) * nl);
+#define DGI_CASE(A, B)                                                                                             \
+  if (to.ndf == A && from.ndf == B) {                                                                             \
+    LAUNCH((k_dg_inc<T, A, B>), grid, static_cast<const T *>(op->d), op->gs, to.d_ebase, x, y, to.d_map, from.d_map, \
+           npos, nl, A, B);                                                                                       \
+    CHECK_LAUNCH();                                                                                               \
+    return LFRIC_OK;                                                                                              \
+  }
+  DGI_CASE(1, 6)
+  DGI_CASE(1, 1)
+  DGI_CASE(1, 8)
+#undef DGI_CASE
+  LAUNCH((k_dg_inc<T, 0, 0>), grid, static_cast<const T *>(op->d), op->gs, to.d_ebase, x, y, to.d_map, from.d_map, npos,
+         nl, to.ndf, from.ndf);
+  CHECK_LAUNCH();
+  return LFRIC_OK;
+}
+
+int launch_dg_inc_apply(const lfric_op *op, const void *x, void *y) {
+  if (op->precision == LFRIC_R64) return dg_inc_t<double>(op, static_cast<const double *>(x), static_cast<double *>(y));
+  if (op->precision == LFRIC_R32) return dg_inc_t<float>(op, static_cast<const float *>(x), static_cast<float *>(y));
+  return fail(LFRIC_E_UNSUPPORTED, "dg_inc apply: precision");
+}
+
+int launch_enforce_bc(const FunctionSpace &f, int prec, void *y, const int *skip) {
+  if (!f.bc_bottom && !f.bc_top) return LFRIC_OK;
+  const int grid = grid_for(f.ncells);
+  if (prec == LFRIC_R64)
+    LAUNCH(k_enforce_bc<double>, grid, static_cast<double *>(y), f.d_map, f.ncells, f.ndf, f.nlayers, f.bc_bottom,
+           f.bc_top, skip);
+  else if (prec == LFRIC_R32)
+    LAUNCH(k_enforce_bc<float>, grid, static_cast<float *>(y), f.d_map, f.ncells, f.ndf, f.nlayers, f.bc_bottom,
+           f.bc_top, skip);
+  else
+    return fail(LFRIC_E_UNSUPPORTED, "enforce_bc: precision");
+  CHECK_LAUNCH();
+  return LFRIC_OK;
 }
 
 int launch_dg_apply(const lfric_op *op, const void *x, void *y) {

@@ -262,7 +262,10 @@ static size_t esz(int p) { return p == LFRIC_R64 ? 8 : p == LFRIC_R32 ? 4 : 2; }
 #define LAUNCHK(kernel, grid, block, ...) hipLaunchKernelGGL((kernel), dim3(grid), dim3(block), 0, ctx().stream, __VA_ARGS__)
 
 int linop_apply(const lfric_linop *L, const void *x, void *y) {
-  if (L->kind == 0) return launch_lma_apply_solver(L->A, x, y, nullptr);
+  if (L->kind == 0) {
+    if (int rc = launch_lma_apply_solver(L->A, x, y, nullptr)) return rc;
+    return L->apply_bc ? launch_enforce_bc(*L->space, L->precision, y, nullptr) : LFRIC_OK;
+  }
   // y = M3 x + tau * D (D^T x / m2)
   const long long n2 = L->D->from->undf, n3 = L->space->undf;
   if (int rc = launch_lma_transpose(L->D, x, L->t_w2)) return rc;
@@ -543,6 +546,12 @@ int lfric_linop_create_lma(const lfric_op *A, lfric_linop **out) {
   L->precision = A->precision;
   L->space = A->to;
   *out = L;
+  return LFRIC_OK;
+}
+
+int lfric_linop_create_mass_matrix(const lfric_op *A, int apply_bc, lfric_linop **out) {
+  if (int rc = lfric_linop_create_lma(A, out)) return rc;
+  (*out)->apply_bc = apply_bc ? 1 : 0;
   return LFRIC_OK;
 }
 

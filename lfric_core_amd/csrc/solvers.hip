@@ -567,7 +567,7 @@ int lfric_solve(int method, const lfric_linop *A, const lfric_prec *P, lfric_vec
   switch (method) {
     case LFRIC_METHOD_CG:
       LF_CHECK(A->kind == 0, LFRIC_E_UNSUPPORTED, "the fused CG runs on an LMA operator");
-      return lfric_cg_solve(A->A, P, x, b, opts, res);
+      return cg_solve_linop(A, P, x, b, opts, res);
     case LFRIC_METHOD_GMRES: return lfric_gmres_solve(A, P, x, b, opts, res);
     case LFRIC_METHOD_FGMRES: return lfric_fgmres_solve(A, P, x, b, opts, res);
     case LFRIC_METHOD_BICGSTAB:

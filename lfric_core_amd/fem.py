@@ -46,6 +46,18 @@ class FunctionSpace:
                                           ctypes.byref(h)))
         return cls(h)
 
+    def boundary_dofs(self):
+        """get_boundary_dofs (function_space_mod.F90:791-801): (ndf, 2), column 0 bottom,
+        column 1 top; 0 flags a dof on that boundary."""
+        bv = np.zeros(2 * self.ndf, np.int32)
+        check(lib().lfric_fs_get_boundary_dofs(self._h, ptr(bv)))
+        return bv.reshape(2, self.ndf).T.copy()
+
+    def set_boundary_dofs(self, bv):
+        a = np.ascontiguousarray(np.asarray(bv, np.int32).T).ravel()
+        assert a.size == 2 * self.ndf
+        check(lib().lfric_fs_set_boundary_dofs(self._h, ptr(a)))
+
     def sibling(self, space, order=0):
         h = ctypes.c_void_p()
         check(lib().lfric_fs_create_sibling(self._h, space, order, ctypes.byref(h)))
@@ -131,6 +143,11 @@ class FieldVector:
 
     def copy(self, source):
         check(lib().lfric_vec_copy(self._h, source._h))
+
+    def enforce_bc(self):
+        """enforce_bc_kernel_type on this field (sci_enforce_bc_kernel_mod.F90:63-91)."""
+        check(lib().lfric_enforce_bc(self._h))
+        return self
 
     def divide(self, x, d):
         check(lib().lfric_vec_divide(self._h, x._h, d._h))
@@ -277,6 +294,10 @@ class LmaOperator:
 
     def apply_transpose(self, x, y):
         check(lib().lfric_lma_apply_transpose(self._h, x._h, y._h))
+
+    def apply_dg_inc(self, x, y):
+        """y = y + A x (dg_inc_matrix_vector_kernel_mod.F90:69-142)."""
+        check(lib().lfric_dg_inc_apply(self._h, x._h, y._h))
 
     def apply_dg(self, x, y):
         check(lib().lfric_dg_apply(self._h, x._h, y._h))

@@ -53,7 +53,8 @@ module lfric_hip_capi_mod
             lfric_op_download, lfric_op_upload_r32, lfric_op_download_r32, &
             lfric_op_destroy, lfric_lma_apply, lfric_mm_diagonal, lfric_op_invert_local, &
             lfric_op_prepare_rows, lfric_op_tune, lfric_op_assemble_mass_w0, lfric_op_prepare_symmetric, &
-            lfric_op_assemble_mass, lfric_op_assemble_div
+            lfric_op_assemble_mass, lfric_op_assemble_div, lfric_enforce_bc, lfric_dg_inc_apply, &
+            lfric_fs_get_boundary_dofs, lfric_fs_set_boundary_dofs
   public :: lfric_fs_create_ugrid
   public :: lfric_cma_create, lfric_cma_create_custom, lfric_cma_assemble, lfric_cma_assemble_diag_hmht, &
             lfric_cma_assemble_m2v_lumped_inv, lfric_cma_apply, lfric_cma_apply_inverse, lfric_cma_multiply, &
@@ -357,6 +358,24 @@ module lfric_hip_capi_mod
     integer(c_int) function lfric_lma_apply(op, x, y) bind(c, name="lfric_lma_apply")
       import :: c_int, c_ptr
       type(c_ptr), value :: op, x, y
+    end function
+    integer(c_int) function lfric_dg_inc_apply(op, x, y) bind(c, name="lfric_dg_inc_apply")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: op, x, y
+    end function
+    integer(c_int) function lfric_enforce_bc(v) bind(c, name="lfric_enforce_bc")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: v
+    end function
+    integer(c_int) function lfric_fs_get_boundary_dofs(fs, bv) bind(c, name="lfric_fs_get_boundary_dofs")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: fs
+      integer(c_int), intent(out) :: bv(*)
+    end function
+    integer(c_int) function lfric_fs_set_boundary_dofs(fs, bv) bind(c, name="lfric_fs_set_boundary_dofs")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: fs
+      integer(c_int), intent(in) :: bv(*)
     end function
     integer(c_int) function lfric_mm_diagonal(op, diag) bind(c, name="lfric_mm_diagonal")
       import :: c_int, c_ptr

@@ -104,10 +104,13 @@ module lfric_hip_solver_api_mod
   end type hip_r_solver_field_vector_type
 
   !> y = A x over either vector type (the operator's storage precision must match the
-  !> vectors'); `applies` counts the calls, so a caller can read how many products a solver made
+  !> vectors'); `applies` counts the calls, so a caller can read how many products a solver made.
+  !> With apply_bc set, y then takes the space's boundary conditions, as
+  !> mass_matrix_operator_type%apply does (sci_mass_matrix_operator_alg_mod.x90:98-101).
   type, public, extends(abstract_linear_operator_type) :: hip_lma_operator_type
     type(c_ptr) :: op = c_null_ptr
     integer(i_def) :: applies = 0
+    logical :: apply_bc = .false.
   contains
     procedure :: apply => op_apply
   end type hip_lma_operator_type
@@ -536,9 +539,11 @@ contains
     type is (hip_field_vector_type)
       call y%own()
       call hip_check(lfric_lma_apply(self%op, any_handle(x, "lma_apply"), y%h), "lma_apply")
+      if (self%apply_bc) call hip_check(lfric_enforce_bc(y%h), "enforce_bc")
     type is (hip_r_solver_field_vector_type)
       call y%own()
       call hip_check(lfric_lma_apply(self%op, any_handle(x, "lma_apply"), y%h), "lma_apply")
+      if (self%apply_bc) call hip_check(lfric_enforce_bc(y%h), "enforce_bc")
     class default
       call log_event("hip_lma_operator_type:apply: y is not a HIP field vector", LOG_LEVEL_ERROR)
     end select

@@ -21,6 +21,14 @@ class LinearOperator:
         return cls(h, op.fs_to, op.precision, (op,))
 
     @classmethod
+    def mass_matrix(cls, op, apply_bc=True):
+        """mass_matrix_operator_type(mass_matrix, apply_bc) (sci_mass_matrix_operator_alg_mod.x90:
+        42-119): y = A x, then enforce_bc on y when apply_bc is set."""
+        h = ctypes.c_void_p()
+        check(lib().lfric_linop_create_mass_matrix(op._h, int(bool(apply_bc)), ctypes.byref(h)))
+        return cls(h, op.fs_to, op.precision, (op,))
+
+    @classmethod
     def helmholtz(cls, M3, D, m2_diag, tau):
         """y = M3 x + tau D diag(M2)^-1 D^T x on W3."""
         h = ctypes.c_void_p()

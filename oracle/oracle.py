@@ -76,6 +76,9 @@ def lib():
             "orc_transpose_apply": (None, [_I, _I, _I, _VP, _VP, _I, _I, _I, _VP, _I, _I, _VP, _VP, _VP, _VP]),
             "orc_dg_apply": (None, [_I, _I, _I, _I, _VP, _I, _I, _VP, _VP, _VP, _VP]),
             "orc_mm_diagonal": (None, [_I, _I, _I, _I, _VP, _VP, _VP]),
+            "orc_dg_inc_apply": (None, [_I, _I, _I, _I, _VP, _I, _I, _VP, _VP, _VP, _VP]),
+            "orc_enforce_bc": (None, [_I, _I, _I, _VP, _VP, _VP]),
+            "orc_boundary_dofs": (_I, [_I, _I, _I, _VP]),
             "orc_invert_local": (_I, [_I64, _I, _VP, _VP]),
             "orc_invert_local_r32": (_I, [_I64, _I, _VP, _VP]),
             "orc_gauss_rule": (_I, [_I, _VP, _VP]),
@@ -629,6 +632,31 @@ def dg_apply(to_mesh, from_mesh, A, x):
     lib().orc_dg_apply(to_mesh.ncells, to_mesh.nlayers, to_mesh.ndf, to_mesh.undf, _p(to_mesh.dofmap),
                        from_mesh.ndf, from_mesh.undf, _p(from_mesh.dofmap), _p(flat_matrix(A)),
                        _p(np.ascontiguousarray(x, np.float64)), _p(y))
+    return y
+
+
+def dg_inc_apply(to_mesh, from_mesh, A, x, y0):
+    """y0 + A x on a discontinuous to_mesh (dg_inc_matrix_vector_code); y0 is not modified."""
+    y = np.array(y0, np.float64, copy=True)
+    lib().orc_dg_inc_apply(to_mesh.ncells, to_mesh.nlayers, to_mesh.ndf, to_mesh.undf, _p(to_mesh.dofmap),
+                           from_mesh.ndf, from_mesh.undf, _p(from_mesh.dofmap), _p(flat_matrix(A)),
+                           _p(np.ascontiguousarray(x, np.float64)), _p(y))
+    return y
+
+
+def boundary_dofs(space, order, ndf):
+    """dof_on_vert_boundary (ndf, 2) of a lowest-order space: column 0 bottom, 1 top."""
+    bv = np.zeros(2 * ndf, np.int32)
+    if lib().orc_boundary_dofs(space, order, ndf, _p(bv)) != 0:
+        raise ValueError("boundary dofs not restated for this space")
+    return bv.reshape(2, ndf).T.copy()
+
+
+def enforce_bc(mesh, bv, field):
+    """enforce_bc_code over every cell column; bv is (ndf, 2); returns a new field."""
+    y = np.array(field, np.float64, copy=True)
+    bvf = np.ascontiguousarray(np.asarray(bv, np.int32).T).ravel()  # Fortran order (ndf, 2)
+    lib().orc_enforce_bc(mesh.ncells, mesh.nlayers, mesh.ndf, _p(mesh.dofmap), _p(bvf), _p(y))
     return y
 
 

@@ -26,9 +26,14 @@
  *   orc_colour_planar         valid 4-colouring; along W/E the lowest bit of colour-1 alternates
  *                             and along S/N the next one (the structure the transition table
  *                             gives), on the reference's planar UGRID meshes
- *   orc_transpose_apply,      known-answer tests of transpose/dg/mm_diagonal unit tests;
+ *   orc_transpose_apply,      known-answer tests of transpose/dg/dg_inc/mm_diagonal unit tests;
  *   orc_dg_apply,             bit-level comparison with the compiled reference kernels
+ *   orc_dg_inc_apply,         (dg_inc also: accumulates onto lhs, y0 + D x)
  *   orc_mm_diagonal
+ *   orc_enforce_bc            bit-level comparison with the compiled reference enforce_bc kernel;
+ *   orc_boundary_dofs         W2: exactly the bottom and top dofs of the vertical-flux columns
+ *                             (the domain's bottom and top surfaces) become zero, nothing else
+ *                             changes; a flag on a side face fails that
  *   orc_gauss_rule            numpy.polynomial.legendre.leggauss mapped to [0, 1]; the 3-point
  *                             weights of get_unit_test_q3x3x3_quadrature_mod.f90
  *   orc_mass_matrix_w0        known answer of compute_mass_matrix_kernel_w0_mod_test.pf (a
@@ -122,6 +127,11 @@ void orc_transpose_apply(int ncells, int nlayers, int ncolours, const int *ncell
 void orc_dg_apply(int ncells, int nlayers, int ndf1, int undf1, const int *map1, int ndf2,
                   int undf2, const int *map2, const double *matrix, const double *x,
                   double *y);
+void orc_dg_inc_apply(int ncells, int nlayers, int ndf1, int undf1, const int *map1, int ndf2,
+                      int undf2, const int *map2, const double *matrix, const double *x,
+                      double *y);
+void orc_enforce_bc(int ncells, int nlayers, int ndf, const int *map, const int *bv, double *field);
+int orc_boundary_dofs(int space, int order, int ndf, int *bv);
 void orc_mm_diagonal(int ncells, int nlayers, int ndf, int undf, const int *map,
                      const double *matrix, double *diag);
 /* per-cell dense inverse of a square operator (reference layout), FP64 and FP32 arithmetic */

@@ -110,6 +110,54 @@ void orc_dg_apply(int ncells, int nl, int ndf1, int undf1, const int *map1, int 
   }
 }
 
+/* dg_inc_matrix_vector_code (components/science/source/kernel/algebra/
+ * dg_inc_matrix_vector_kernel_mod.F90:69-104, real64 variant :107-142): lhs is GH_READWRITE on a
+ * discontinuous space, so the cells are visited in order with no colouring and nothing is
+ * zeroed: lhs(i1:i1+nl) = lhs(i1:i1+nl) + matrix(ik:ik+nl,df,df2)*x(i2:i2+nl), df2 outer. */
+void orc_dg_inc_apply(int ncells, int nl, int ndf1, int undf1, const int *map1, int ndf2,
+                      int undf2, const int *map2, const double *matrix, const double *x,
+                      double *y) {
+  (void)undf1; (void)undf2;
+  const int64_t nc3 = (int64_t)ncells * nl;
+  for (int cell = 1; cell <= ncells; ++cell) {
+    const int *m1 = &map1[(size_t)(cell - 1) * ndf1], *m2 = &map2[(size_t)(cell - 1) * ndf2];
+    const int64_t ik = (int64_t)(cell - 1) * nl;
+    for (int df2 = 0; df2 < ndf2; ++df2)
+      for (int df1 = 0; df1 < ndf1; ++df1)
+        for (int k = 0; k < nl; ++k)
+          y[m1[df1] - 1 + k] = y[m1[df1] - 1 + k] + MAT(ik + k, df1, df2, nc3, ndf1) * x[m2[df2] - 1 + k];
+  }
+}
+
+/* enforce_bc_code (components/science/source/kernel/algebra/sci_enforce_bc_kernel_mod.F90:63-91,
+ * real64 variant) over every cell column in cell order: the bottom level of each dof is scaled
+ * by boundary_value(df,1), the top layer's (k = nlayers-1) by boundary_value(df,2).
+ * bv is boundary_value(ndf,2) in Fortran order (bv[df] bottom, bv[ndf+df] top). */
+void orc_enforce_bc(int ncells, int nl, int ndf, const int *map, const int *bv, double *field) {
+  for (int cell = 1; cell <= ncells; ++cell) {
+    const int *m = &map[(size_t)(cell - 1) * ndf];
+    for (int df = 0; df < ndf; ++df) field[m[df] - 1] = field[m[df] - 1] * (double)bv[df];
+    for (int df = 0; df < ndf; ++df)
+      field[m[df] - 1 + nl - 1] = field[m[df] - 1 + nl - 1] * (double)bv[ndf + df];
+  }
+}
+
+/* dof_on_vert_boundary of a lowest-order space
+ * (infrastructure/source/function_space/function_space_constructor_helper_functions_mod.F90:
+ * 678-680: every flag 1, then per space): W2 (:1030-1049) flags its face dofs on face
+ * number_faces-1 (bottom, dof 5) and number_faces (top, dof 6); W0 (:690-800) and W3 (:1172-)
+ * flag none.  space: 0 W0, 2 W2, 3 W3.  Returns 0, or -1 for a space/order not restated. */
+int orc_boundary_dofs(int space, int order, int ndf, int *bv) {
+  for (int i = 0; i < 2 * ndf; ++i) bv[i] = 1;
+  if (space == 0 || space == 3) return 0;
+  if (space == 2 && order == 0 && ndf == 6) {
+    bv[4] = 0;           /* face B: boundary_value(5,1) = 0 */
+    bv[ndf + 5] = 0;     /* face T: boundary_value(6,2) = 0 */
+    return 0;
+  }
+  return -1;
+}
+
 /* mm_diagonal_kernel_code (components/science/source/kernel/algebra/
  * sci_mm_diagonal_kernel_mod.F90:87-92): diag(map(df)+k) += A(ik,df,df) summed over
  * cells in cell order (diag must be zeroed by the caller, as setval_c does). */

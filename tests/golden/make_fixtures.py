@@ -205,6 +205,11 @@ def main(ref):
             "d": d.tolist(), "x": x.tolist(), "answer": (d @ x).tolist(),
             "map1": [1, 2, 3, 4], "map2": [1, 2], "tol": 1e-14,
         },
+        # dg_inc_matrix_vector_mod_test.pf:44-85: y(4) = 0 + D(4x2) x(2), answer = matmul
+        "dg_inc_matrix_vector": {
+            "d": d.tolist(), "x": x.tolist(), "y0": [0.0, 0.0, 0.0, 0.0], "answer": (d @ x).tolist(),
+            "map1": [1, 2, 3, 4], "map2": [1, 2], "tol": 1e-14,
+        },
         # mm_diagonal_kernel_mod_test.pf:42-84: two cells sharing dof 3, literal answer
         "mm_diagonal": {
             "mm": [[4.0, 1.0, 1.0], [2.0, 6.0, 2.0], [1.0, 1.0, 4.0]],  # mm(cell, i, j), same per cell

@@ -275,6 +275,28 @@ def test_cg_fp64_matches_oracle(hip, orc, form):
     assert np.linalg.norm(bo - S @ xg) / np.linalg.norm(bo) <= 1.01e-8
 
 
+@pytest.mark.parametrize("rows", [False, True])
+def test_cg_iterates_do_not_depend_on_the_xcd_window(hip, orc, rows):
+    """The fused p.(A p) partials are stored by the logical block the gather kernels work on, so
+    the order of CG's dot, and with it every iterate, is the same for every block-order window."""
+    fs, m, op, b = _config0(hip, orc, 12, 20, scaled=True)
+    op32 = op.convert(hip.R32)
+    if rows:
+        op32.prepare_rows()
+    prec = hip.DiagonalPreconditioner.from_operator(op32)
+    b32 = hip.FieldVector(fs, hip.R32).import_field(b.export_field())
+    out = []
+    for window in (0, 8, 2):
+        op32.set_window(window)
+        x = hip.FieldVector(fs, hip.R32)
+        x.set_scalar(0.0)
+        res = hip.ConjugateGradient(op32, prec, 1e-6, 0.0, 500).apply(x, b32)
+        out.append((res.iterations, res.r_nrm, x.export_raw()))
+    for it, nrm, xw in out[1:]:
+        assert it == out[0][0] and nrm == out[0][1]
+        assert np.array_equal(xw.view(np.uint32), out[0][2].view(np.uint32))
+
+
 @pytest.mark.parametrize("form", [0, 1])
 def test_mixed_precision_refinement_reaches_fp64_accuracy(hip, orc, form):
     fs, m, op, b = _config0(hip, orc, 12, 20, scaled=True)
@@ -293,10 +315,13 @@ def test_mixed_precision_refinement_reaches_fp64_accuracy(hip, orc, form):
     S = orc.assembled_csr(m, A)
     xg = x.export_field()
     assert np.linalg.norm(bo - S @ xg) / np.linalg.norm(bo) <= 1.01e-8
-    # against the FP64 oracle solve driven to 1e-12: error bounded by cond * tol
+    # against the FP64 oracle solve driven to 1e-12.  The refined solution's error is about 1e-8
+    # here; an inner FP32 error left uncorrected would sit near FP32 precision times the
+    # condition number, orders of magnitude above this bar
     xo, reso, _ = orc.cg(m, A, bo, diag=orc.mm_diagonal(m, A), r_tol=1e-12, max_iter=20000)
     err = np.linalg.norm(xg - xo) / np.linalg.norm(xo)
-    assert err <= 1e-3, err
+    print(f"mixed-precision solution error vs the FP64 oracle: {err:.3e}")
+    assert err <= 1e-6, err
 
 
 def test_solver_status_codes(hip, orc):

@@ -337,6 +337,71 @@ def test_dg_known_answer(orc):
     np.testing.assert_allclose(y, k["answer"], rtol=0, atol=k["tol"])
 
 
+def test_dg_inc_known_answer_and_accumulation(orc):
+    k = json.loads((GOLDEN / "kat.json").read_text())["dg_inc_matrix_vector"]
+    to, fr = _OneCell(k["map1"], 4), _OneCell(k["map2"], 2)
+    d = np.array(k["d"])
+    y = orc.dg_inc_apply(to, fr, d[None, :, :], np.array(k["x"]), np.array(k["y0"]))
+    np.testing.assert_allclose(y, k["answer"], rtol=0, atol=k["tol"])
+    # GH_READWRITE: adds onto what lhs holds (a zeroing kernel, dg_matrix_vector, would lose y0)
+    y0 = np.array([1.0, -2.0, 0.5, 3.0])
+    y = orc.dg_inc_apply(to, fr, d[None, :, :], np.array(k["x"]), y0)
+    np.testing.assert_allclose(y, y0 + d @ np.array(k["x"]), rtol=0, atol=1e-14)
+    assert not np.allclose(orc.dg_apply(to, fr, d[None, :, :], np.array(k["x"])), y)
+
+
+def test_dg_inc_and_enforce_bc_bitwise_equal_compiled_reference_kernels(orc):
+    R = _ref_lib()
+    P = lambda a: ctypes.c_void_p(a.ctypes.data)
+    I = ctypes.c_int
+    H = orc.Helmholtz(4, 3)
+    w2, w3 = H.w2, H.w3
+    x2 = orc.gen_uniform(w2.undf, 1, 3)
+    y0 = orc.gen_uniform(w3.undf, 2, 3)
+    y3 = orc.dg_inc_apply(w3, w2, H.D, x2, y0)                     # y0 + D x: W2 -> W3
+    yr3 = y0.copy()
+    R.ref_dg_inc_apply_r64(I(w3.ncells), I(w3.nlayers), I(w3.ndf), I(w3.undf), P(w3.dofmap), I(w2.ndf),
+                           I(w2.undf), P(w2.dofmap), P(orc.flat_matrix(H.D)), P(x2), P(yr3))
+    np.testing.assert_array_equal(y3, yr3)
+    bv = orc.boundary_dofs(orc.W2, 0, 6)
+    f = orc.gen_uniform(w2.undf, 3, 3)
+    fo = orc.enforce_bc(w2, bv, f)
+    fr = f.copy()
+    bvf = np.ascontiguousarray(bv.T).ravel()
+    R.ref_enforce_bc_r64(I(w2.ncells), I(w2.nlayers), I(w2.ndf), I(w2.undf), P(w2.dofmap), P(bvf), P(fr))
+    np.testing.assert_array_equal(fo, fr)
+
+
+def _w2_surface_dofs(m):
+    """0-based dofs of the lowest-order W2 space on the domain's bottom and top surfaces: the
+    first and last dof of each vertical-flux column (faces B and T share one column of nl+1)."""
+    base = m.dofmap[:, 4] - 1
+    assert np.array_equal(m.dofmap[:, 5] - 1, base + 1)
+    return np.sort(np.concatenate([base, base + m.nlayers]))
+
+
+def test_enforce_bc_zeroes_exactly_the_w2_surface_dofs(orc):
+    m = orc.Mesh(4, 3, orc.W2, 0)
+    bv = orc.boundary_dofs(orc.W2, 0, 6)
+    assert bv.shape == (6, 2) and bv[4, 0] == 0 and bv[5, 1] == 0 and bv.sum() == 10
+    f = orc.gen_uniform(m.undf, 5, 3) + 2.0  # no zero entries
+    g = orc.enforce_bc(m, bv, f)
+    surf = _w2_surface_dofs(m)
+    assert np.all(g[surf] == 0.0)
+    rest = np.setdiff1d(np.arange(m.undf), surf)
+    np.testing.assert_array_equal(g[rest], f[rest])
+    assert len(surf) == 2 * m.ncells
+    # idempotent, and W0 / W3 carry no boundary flags
+    np.testing.assert_array_equal(orc.enforce_bc(m, bv, g), g)
+    for sp, nd in ((orc.W0, 8), (orc.W3, 1)):
+        assert orc.boundary_dofs(sp, 0, nd).min() == 1
+    # a planted mistake: the flag on a side face (W) instead of the bottom face fails the check
+    bad = bv.copy()
+    bad[4, 0], bad[0, 0] = 1, 0
+    gb = orc.enforce_bc(m, bad, f)
+    assert not np.all(gb[surf] == 0.0)
+
+
 def test_mm_diagonal_known_answer(orc):
     k = json.loads((GOLDEN / "kat.json").read_text())["mm_diagonal"]
 
