@@ -30,12 +30,14 @@ os.environ.setdefault("OMP_PROC_BIND", "close")
 os.environ.setdefault("OMP_PLACES", "cores")
 
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per FP32 apply of C96x70 W0 from the rocprofv3 FETCH_SIZE / WRITE_SIZE runs recorded in
-# profiles/ (counters scaled by a plain copy of known size), per apply form: the colour-ordered
-# scatter (4 launches) and the gather kernel over the row-gathered copy (one launch).  Re-measure
+# HBM bytes per launch of the FP32 apply's dominant kernel at C96x70 W0 from the rocprofv3
+# FETCH_SIZE / WRITE_SIZE runs recorded in profiles/ (counters scaled by a plain copy of known
+# size), per apply form: the colour-ordered scatter (per apply, 4 launches), the gather kernel
+# over the row-gathered copy, and the first kernel of the symmetric-packed apply.  Re-measure
 # when a kernel changes.
 TRAFFIC_BYTES_PER_APPLY_FP32 = {("colour", 96, 70): 4 * (288.4e6 + 16.8e6),
-                               ("gather", 96, 70): 1074.05e6 + 15.70e6}
+                               ("gather", 96, 70): 1074.05e6 + 15.70e6,
+                               ("symmetric", 96, 70): 656.5e6 + 63.6e6}
 
 
 def apply_bytes(fs, ndf, s_a, s_x):
@@ -49,16 +51,25 @@ def sym_bytes(fs, s_a, s_x):
     return fs.ncell_3d * 36 * s_a + 2 * fs.undf * s_x + 4 * fs.ncells * 8
 
 
-def solve_roofline(fs, mres, pres, t_mixed, t_fp64):
+def sym_phase1_bytes(fs, s):
+    """Algorithmic bytes of the first kernel of the symmetric-packed apply (k_sym_units1): every
+    element's upper triangle once, x once, the dofmap once, and the 4 outputs of every (cell,
+    level) unit written once."""
+    return fs.ncell_3d * 36 * s + fs.undf * s + 4 * fs.ncells * 8 + fs.ncells * 4 * (fs.nlayers + 1) * s
+
+
+def solve_roofline(fs, mres, pres, t_mixed, t_fp64, packed=False):
     """Whole-solve HBM rate: sum over iterations of the PCG iteration's algorithmic bytes plus,
     for the mixed solve, each FP64 refinement step (one FP64 apply + 4 FP64 vector passes),
     divided by the solve time.  Iteration: the apply with p.(A p) (matrix, x, y, dofmap once
-    each), the update (reads x, p, r, A p, diag; writes x, r, z: 8 vector passes) and aypx
+    each; the matrix counted as the triangles it reads when the apply is the symmetric-packed
+    one), the update (reads x, p, r, A p, diag; writes x, r, z: 8 vector passes) and aypx
     (reads z, p; writes p: 3), i.e. 11 vector passes of undf entries beyond the apply."""
     passes = 11
-    it32 = apply_bytes(fs, 8, 4, 4) + passes * fs.undf * 4
-    it64 = apply_bytes(fs, 8, 8, 8) + passes * fs.undf * 8
-    ref64 = apply_bytes(fs, 8, 8, 8) + 4 * fs.undf * 8
+    ab = (lambda s: sym_bytes(fs, s, s)) if packed else (lambda s: apply_bytes(fs, 8, s, s))
+    it32 = ab(4) + passes * fs.undf * 4
+    it64 = ab(8) + passes * fs.undf * 8
+    ref64 = ab(8) + 4 * fs.undf * 8
     b_mixed = mres.iterations * it32 + (mres.outer_iterations + 1) * ref64
     b_fp64 = pres.iterations * it64
     return {"mixed_gbs": b_mixed / t_mixed / 1e9, "fp64_gbs": b_fp64 / t_fp64 / 1e9,
@@ -77,10 +88,13 @@ def time_applies(L, op, x, y, reps, method="apply"):
     return L.timer_stop() / reps
 
 
+_START_AFFINITY = sorted(os.sched_getaffinity(0))  # before any OpenMP runtime binds this thread
+
+
 def host_cpu_topology():
     """Physical cores and logical CPUs this process may run on (sysfs topology of the CPUs in
-    its affinity mask, the figures lscpu reports)."""
-    cpus = sorted(os.sched_getaffinity(0))
+    its affinity mask at start-up, the figures lscpu reports)."""
+    cpus = _START_AFFINITY
     cores = set()
     for c in cpus:
         base = Path(f"/sys/devices/system/cpu/cpu{c}/topology")
@@ -312,6 +326,9 @@ def main():
                          "with register-staged x) or gather form for every problem (0)")
     ap.add_argument("--apply-form", type=int, default=1, choices=(0, 1),
                     help="LMA apply inside the solvers: 0 colour-ordered scatter, 1 gather with fused p.Ap")
+    ap.add_argument("--operator-copy", choices=("symmetric", "rows"), default="symmetric",
+                    help="gather form's operator copy: symmetric-packed triangles (two kernels) or the "
+                         "row-gathered full element matrices (one kernel)")
     ap.add_argument("--serve-window", type=float, default=20.0, help="seconds of arrivals")
     ap.add_argument("--serve-rate", type=float, default=200.0, help="requests per second")
     ap.add_argument("--serve-limit", type=float, default=30.0, help="wall seconds of admission per server")
@@ -377,7 +394,7 @@ def main():
         # symmetric-packed copies (upper triangles; two-phase apply), on operators of their own
         s32, s64 = op64.convert(L.R32), L.LmaOperator(fs).generate_spd(0, scaled=True)
         out["symmetric_packed"] = {}
-        for seg in (2, 4, 8):
+        for seg in (0, 2, 4, 8):
             s32.prepare_symmetric(seg)
             s64.prepare_symmetric(seg)
             t32 = time_applies(L, s32, xs32, ys32, args.apply_reps, "apply_gather")
@@ -395,7 +412,11 @@ def main():
         print(json.dumps(out))
         return
 
-    if args.apply_form == 1:  # row-gathered copies for the gather apply (set-up, like the operators)
+    packed = args.apply_form == 1 and args.operator_copy == "symmetric"
+    if packed:  # symmetric-packed copies for the gather apply (set-up, like the operators)
+        op32.prepare_symmetric(0)
+        op64.prepare_symmetric(0)
+    elif args.apply_form == 1:  # row-gathered copies
         op32.prepare_rows()
         op64.prepare_rows()
     # the apply autotuner picks each operator's XCD block-order window (once per operator)
@@ -435,10 +456,20 @@ def main():
     method = "apply_gather" if args.apply_form == 1 else "apply"   # the form the solvers used
     t32 = time_applies(L, op32, xs32, ys32, args.apply_reps, method)
     t64 = time_applies(L, op64, xs64, ys64, args.apply_reps, method)
-    b32 = apply_bytes(fs, 8, 4, 4)
-    b64 = apply_bytes(fs, 8, 8, 8)
+    b32 = sym_bytes(fs, 4, 4) if packed else apply_bytes(fs, 8, 4, 4)
+    b64 = sym_bytes(fs, 8, 8) if packed else apply_bytes(fs, 8, 8, 8)
     gbs32 = b32 / (t32 * 1e-3) / 1e9
     gbs64 = b64 / (t64 * 1e-3) / 1e9
+    # per-kernel times of the FP32 apply (HIP events around each kernel, library stream)
+    k32 = op32.time_apply(xs32, ys32, args.apply_reps) if args.apply_form == 1 else (t32, 0.0)
+    if packed:
+        kbytes = sym_phase1_bytes(fs, 4)
+        kname = "k_sym_units1<float> (first of the two kernels of the symmetric-packed apply)"
+    else:
+        kbytes = b32
+        kname = ("k_lma_gather_rows_w0<float> (one launch per apply, row-gathered copy)" if args.apply_form == 1
+                 else "k_lma_colour<float,8,8,RolesW0K0> (4 colour launches per apply)")
+    kgbs = kbytes / (k32[0] * 1e-3) / 1e9
 
     mean_mixed = total / args.steps
     value = undf / mean_mixed
@@ -467,21 +498,22 @@ def main():
         "iterations": {"mixed_inner": mres.iterations, "mixed_outer": mres.outer_iterations,
                        "fp64": pres.iterations},
         "solve_roofline": solve_roofline(fs, mres, pres, sum(mixed_times) / len(mixed_times),
-                                         sum(plain_times) / len(plain_times)),
+                                         sum(plain_times) / len(plain_times), packed),
         "verification": {"mixed_true_rel_res_fp64": mixed_true, "fp64_true_rel_res": plain_true,
                          "mixed_vs_fp64_solution_rel_err": err_mixed_vs_fp64},
         "lma_gbs": {"fp32": gbs32, "fp64": gbs64, "fp32_ms": t32, "fp64_ms": t64,
-                    "fp32_bytes": b32, "fp64_bytes": b64,
+                    "fp32_bytes": b32, "fp64_bytes": b64, "fp32_frac": gbs32 / HBM_PEAK_GBS,
+                    "bytes_counted": ("symmetric-packed: upper triangles (36 of 64 entries), x, y, dofmap once"
+                                      if packed else "full element matrices, x, y, dofmap once"),
+                    "fp32_kernel_ms": {"phase1": k32[0], "phase2": k32[1]},
                     "autotuned_window": {"fp32": tuned["fp32"][0], "fp64": tuned["fp64"][0]}},
-        "roofline": {"bound": "hbm", "achieved": gbs32, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": gbs32 / HBM_PEAK_GBS,
+        "roofline": {"bound": "hbm", "achieved": kgbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": kgbs / HBM_PEAK_GBS,
                      "traffic": TRAFFIC_BYTES_PER_APPLY_FP32.get(
-                         ("gather" if args.apply_form == 1 else "colour", args.edge_cells, args.nlayers)),
-                     "algorithmic_bytes": b32,
-                     "kernel": ("k_lma_gather_rows_w0<float> (one launch per apply, row-gathered copy)"
-                                if args.apply_form == 1
-                                else "k_lma_colour<float,8,8,RolesW0K0> (4 colour launches per apply)")
-                               + "; bytes, time and traffic per apply"},
+                         ("symmetric" if packed else "gather" if args.apply_form == 1 else "colour",
+                          args.edge_cells, args.nlayers)),
+                     "algorithmic_bytes": kbytes, "kernel_ms": k32[0],
+                     "kernel": kname + "; bytes, time and traffic per launch"},
     }
     small, small_host = small_fp64_report(L)
     line["c12x20_fp64"] = small

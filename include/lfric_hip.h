@@ -161,6 +161,11 @@ int lfric_op_generate_uniform(lfric_op *op, unsigned long long seed, unsigned lo
  * 2, ..., 256} and keeps the fastest in the operator (*best_window, *best_ms per apply).
  * No reference counterpart. */
 int lfric_op_tune(lfric_op *op, int form, int *best_window, double *best_ms);
+/* Measurement helper (no reference counterpart): `reps` gather-form applies y = A x timed with
+ * HIP events on the library stream; phase_ms[0], phase_ms[1] = the average milliseconds of the
+ * two kernels of the symmetric-packed apply (per-cell outputs, colour-ordered gather), or of the
+ * one-kernel apply and 0. */
+int lfric_op_time_apply(const lfric_op *op, const lfric_vec *x, lfric_vec *y, int reps, double *phase_ms);
 int lfric_op_set_window(lfric_op *op, int window);
 /* Add a row-gathered copy of a square lowest-order W0 operator: the rows that feed each
  * output dof stored contiguously, so the gather apply streams the matrix in order.  Costs a
@@ -170,11 +175,13 @@ int lfric_op_set_window(lfric_op *op, int window);
 int lfric_op_prepare_rows(lfric_op *op);
 /* Add a symmetric-packed copy of a square lowest-order W0 operator whose every element matrix
  * is symmetric bit for bit (refused otherwise): the upper triangle of each element (36 of 64
- * entries), grouped per cell column in segments of levels_per_thread levels (0: 4, the
- * fastest measured).  From then on the solvers' apply reads it in two phases (per-cell outputs,
- * then a colour-ordered gather) whose result equals the colour-ordered scatter (lfric_lma_apply)
- * bit for bit, reading ~70 % of the matrix bytes.  Writing the operator rebuilds the copy, or drops it if the new entries are
- * not symmetric.  No reference counterpart (the reference stores full element matrices). */
+ * entries).  levels_per_thread 0: one triangle per (cell, level) unit, the previous layer's
+ * top rows passed between neighbouring threads; S > 0: segments of S levels per thread, each
+ * repeating the previous segment's last triangle.  From then on the solvers' apply reads it in
+ * two phases (per-cell outputs, then a colour-ordered gather) whose result equals the
+ * colour-ordered scatter (lfric_lma_apply) bit for bit, reading ~57 % of the matrix bytes
+ * (S = 0).  Writing the operator rebuilds the copy, or drops it if the new entries are not
+ * symmetric.  No reference counterpart (the reference stores full element matrices). */
 int lfric_op_prepare_symmetric(lfric_op *op, int levels_per_thread);
 /* Round an operator into another precision (e.g. the r_solver FP32 copy of an FP64 master). */
 int lfric_op_convert(const lfric_op *src, lfric_op *dst);

@@ -141,6 +141,7 @@ SIGNATURES = {
     "lfric_lma_apply_mrhs": (_I, [_VP, _VP, _VP, _I]),
     "lfric_op_tune": (_I, [_VP, _I, ctypes.POINTER(_I), _DP]),
     "lfric_op_set_window": (_I, [_VP, _I]),
+    "lfric_op_time_apply": (_I, [_VP, _VP, _VP, _I, _DP]),
     "lfric_op_prepare_rows": (_I, [_VP]),
     "lfric_fgmres_solve": (_I, [_VP, _VP, _VP, _VP, ctypes.POINTER(SolverOpts), ctypes.POINTER(SolverRes)]),
     "lfric_bicgstab_solve": (_I, [_VP, _VP, _VP, _VP, ctypes.POINTER(SolverOpts), ctypes.POINTER(SolverRes)]),

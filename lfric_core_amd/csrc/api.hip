@@ -733,6 +733,8 @@ void lfric_op_destroy(lfric_op *op) {
   dfree(op->rows, op->rows_bytes);
   dfree(op->sym, op->sym_bytes);
   dfree(op->symz, op->symz_bytes);
+  dfree(op->d_symzu, op->symzu_bytes);
+  dfree(op->d_symcol, op->symcol_bytes);
   delete op;
 }
 
@@ -747,8 +749,12 @@ static int refresh_rows(lfric_op *op, int rc) {
     if (r == LFRIC_E_UNSUPPORTED) {
       dfree(op->sym, op->sym_bytes);
       dfree(op->symz, op->symz_bytes);
+      dfree(op->d_symzu, op->symzu_bytes);
+      dfree(op->d_symcol, op->symcol_bytes);
       op->sym = op->symz = nullptr;
-      op->sym_bytes = op->symz_bytes = 0;
+      op->d_symzu = nullptr;
+      op->d_symcol = nullptr;
+      op->sym_bytes = op->symz_bytes = op->symzu_bytes = op->symcol_bytes = 0;
     } else if (r) {
       return r;
     }
@@ -882,6 +888,48 @@ int lfric_lma_apply(const lfric_op *op, const lfric_vec *x, lfric_vec *y) {
 
 // Autotuner of the apply's block order: time the apply (form 0 colour-ordered scatter, 1 gather)
 // for each XCD window and keep the fastest in the operator.
+int lfric_op_time_apply(const lfric_op *op, const lfric_vec *x, lfric_vec *y, int reps, double *phase_ms) {
+  if (int rc = require_init()) return rc;
+  LF_CHECK(op && x && y && phase_ms && reps > 0 && reps <= 1000, LFRIC_E_ARG, "bad arguments");
+  if (int rc = check_apply(op, x, y, false)) return rc;
+  Context &c = ctx();
+  std::vector<hipEvent_t> ev(static_cast<size_t>(reps) * 3, nullptr);
+  int rc = LFRIC_OK;
+  for (auto &e : ev)
+    if (hipEventCreate(&e) != hipSuccess) {
+      e = nullptr;
+      rc = fail(LFRIC_E_HIP, "hipEventCreate");
+      break;
+    }
+  bool two = false;
+  if (!rc) rc = launch_lma_apply_gather(op, x->d, y->d);  // warm-up (builds the gather lists)
+  for (int r = 0; r < reps && !rc; ++r) {
+    if (hipEventRecord(ev[3 * r], c.stream) != hipSuccess) rc = fail(LFRIC_E_HIP, "event");
+    if (!rc) rc = launch_lma_apply_gather_marked(op, x->d, y->d, ev[3 * r + 1], &two);
+    if (!rc && hipEventRecord(ev[3 * r + 2], c.stream) != hipSuccess) rc = fail(LFRIC_E_HIP, "event");
+  }
+  double p1 = 0, p2 = 0;
+  if (!rc && hipStreamSynchronize(c.stream) != hipSuccess) rc = fail(LFRIC_E_HIP, "synchronize");
+  for (int r = 0; r < reps && !rc; ++r) {
+    float a = 0.f, b = 0.f;
+    if (two) {
+      if (hipEventElapsedTime(&a, ev[3 * r], ev[3 * r + 1]) != hipSuccess ||
+          hipEventElapsedTime(&b, ev[3 * r + 1], ev[3 * r + 2]) != hipSuccess)
+        rc = fail(LFRIC_E_HIP, "event timing");
+    } else if (hipEventElapsedTime(&a, ev[3 * r], ev[3 * r + 2]) != hipSuccess) {
+      rc = fail(LFRIC_E_HIP, "event timing");
+    }
+    p1 += a;
+    p2 += b;
+  }
+  for (auto e : ev)
+    if (e) (void)hipEventDestroy(e);
+  if (rc) return rc;
+  phase_ms[0] = p1 / reps;
+  phase_ms[1] = p2 / reps;
+  return LFRIC_OK;
+}
+
 int lfric_op_tune(lfric_op *op, int form, int *best_window, double *best_ms) {
   if (int rc = require_init()) return rc;
   LF_CHECK(op && (form == 0 || form == 1), LFRIC_E_ARG, "bad arguments");
@@ -942,7 +990,7 @@ int lfric_op_prepare_rows(lfric_op *op) {
 int lfric_op_prepare_symmetric(lfric_op *op, int levels_per_thread) {
   if (int rc = require_init()) return rc;
   LF_CHECK(op, LFRIC_E_ARG, "null argument");
-  if (int rc = launch_build_sym(op, levels_per_thread > 0 ? levels_per_thread : 4)) return rc;
+  if (int rc = launch_build_sym(op, levels_per_thread)) return rc;
   LF_HIP(hipStreamSynchronize(ctx().stream));
   return LFRIC_OK;
 }

@@ -136,6 +136,10 @@ struct lfric_op {
   void *sym = nullptr, *symz = nullptr;
   size_t sym_bytes = 0, symz_bytes = 0, gss = 0;
   int sym_seg = 0;
+  int4 *d_symzu = nullptr;     // per gather column: its cells' unit ids (cell*4 + i), -1 beyond them
+  size_t symzu_bytes = 0;
+  int4 *d_symcol = nullptr;    // one-unit-per-thread form: the k-th cell's 4 bottom vertex column bases
+  size_t symcol_bytes = 0;
   void *d = nullptr;           // level-aligned AoSoA layout (see kernels.hip)
   size_t bytes = 0;
 };

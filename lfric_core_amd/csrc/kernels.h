@@ -44,6 +44,9 @@ int launch_gather_blocks(const lfric_op *op, int *nblocks);
 // blocks of the gather apply's grid, once the space's column lists exist
 int gather_blocks(const lfric_op *op);
 int launch_build_rows(lfric_op *op);
+// the gather apply with `mark` recorded between the two phases of the symmetric-packed form
+// (two_phase tells whether it was)
+int launch_lma_apply_gather_marked(const lfric_op *op, const void *x, void *y, hipEvent_t mark, bool *two_phase);
 // the apply form the solvers use (lfric_set_apply_form)
 int launch_lma_apply_solver(const lfric_op *op, const void *x, void *y, const int *skip);
 int launch_lma_apply_mrhs(const lfric_op *op, const void *x, void *y, int nrhs, int variant);

@@ -957,13 +957,137 @@ __global__ __launch_bounds__(kBlock) void k_sym_units_s(const T *__restrict__ sy
   }
 }
 
+// One unit per thread (the default copy).  Cells take slots in cell-id order, so that the
+// threads of a window of blocks work on neighbouring cells and share their x entries in one
+// L2: thread t = k*(nl+1) + m owns unit (k-th cell, level m), the cell at colour-major position
+// slotpos[k].  Slot t of the copy holds the triangle of that cell's layer-m element (m < nl; the
+// slot of m = nl is empty), entry q at (t/64)*gss + ((q/V)*64 + t%64)*V + q%V.  The thread forms
+// the bottom rows' chains c_i of its element (x at levels m, m+1) and the top rows' chains of
+// the same element, which belong to unit m+1: the next lane takes them with one shuffle.  Lane 0
+// of a wavefront has no such neighbour and forms them itself from the layer-(m-1) triangle, one
+// extra triangle per 64 units.  Every chain is the colour kernel's (same entries, same x, same
+// order); the unit's 4 outputs go to z[(k*4 + i)*(nl+1) + m].
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_build_sym1(const T *__restrict__ A, size_t gs, const int *__restrict__ ebase,
+                                                       const int *__restrict__ slotpos, T *__restrict__ sym, size_t gss,
+                                                       int ncells, int nl, uint32_t row_top, int *__restrict__ asym) {
+  constexpr int V = chunk_entries<T>();
+  const int H = nl + 1;
+  const long long t = static_cast<long long>(blockIdx.x) * kBlock + threadIdx.x;
+  if (t >= static_cast<long long>(ncells) * H) return;
+  const int k = static_cast<int>(t / H), m = static_cast<int>(t - static_cast<long long>(k) * H);
+  T *St = sym + static_cast<size_t>(t >> 6) * gss + static_cast<size_t>(t & 63) * V;
+  const bool live = m < nl;
+  const long long e = static_cast<long long>(ebase[slotpos[k]]) + (live ? m : 0);
+  bool mismatch = false;
+  for (int i = 0; i < 8; ++i)
+    for (int j = i; j < 8; ++j) {
+      T v = T(0);
+      if (live) {
+        v = A[aidx<T>(gs, e + ((row_top >> i) & 1u), i * 8 + j)];
+        const T w = A[aidx<T>(gs, e + ((row_top >> j) & 1u), j * 8 + i)];
+        if constexpr (sizeof(T) == 8)
+          mismatch = mismatch || (__double_as_longlong(v) != __double_as_longlong(w));
+        else
+          mismatch = mismatch || (__float_as_uint(v) != __float_as_uint(w));
+      }
+      const int q = tri_index(i, j);
+      St[static_cast<size_t>(q / V) * 64 * V + q % V] = v;
+    }
+  if (mismatch) *asym = 1;
+}
+
+// cellcol[k]: the 0-based base dofs of the k-th cell's 4 bottom vertex columns
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_sym_units1(const T *__restrict__ sym, size_t gss, const T *__restrict__ x,
+                                                       const int4 *__restrict__ cellcol, T *__restrict__ z, int ncells,
+                                                       int nl, const int *skip, int xwin) {
+  if (skip && *skip) return;
+  constexpr int V = chunk_entries<T>(), NCH = 36 / V;
+  typedef T VecT __attribute__((ext_vector_type(V)));
+  const int H = nl + 1;
+  const long long t = static_cast<long long>(xcd_block(blockIdx.x, gridDim.x, xwin)) * kBlock + threadIdx.x;
+  const bool valid = t < static_cast<long long>(ncells) * H;
+  const int k = valid ? static_cast<int>(t / H) : 0;
+  const int m = valid ? static_cast<int>(t - static_cast<long long>(k) * H) : 0;
+  const bool has_m = valid && m < nl, has_p = valid && m > 0;
+  const int lane = threadIdx.x & 63;
+  const T *St = sym + static_cast<size_t>(t >> 6) * gss + static_cast<size_t>(t & 63) * V;
+  VecT tri[NCH];
+#pragma unroll
+  for (int c = 0; c < NCH; ++c)
+    tri[c] = has_m ? __builtin_nontemporal_load(reinterpret_cast<const VecT *>(St + static_cast<size_t>(c) * 64 * V))
+                   : VecT{};
+  int b[4] = {0, 0, 0, 0};
+  if (valid) {
+    const int4 bb = cellcol[k];
+    b[0] = bb.x;
+    b[1] = bb.y;
+    b[2] = bb.z;
+    b[3] = bb.w;
+  }
+  T x0[4], x1[4];  // x at levels m and m+1
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    x0[j] = valid ? x[b[j] + m] : T(0);
+    x1[j] = has_m ? x[b[j] + m + 1] : T(0);
+  }
+  auto at = [&](int q) { return tri[q / V][q % V]; };
+  T c[4], cn[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    T s = T(0), sn = T(0);
+    if (has_m) {
+#pragma unroll
+      for (int j = 0; j < 8; ++j) s = fma(at(j >= i ? tri_index(i, j) : tri_index(j, i)), j < 4 ? x0[j] : x1[j - 4], s);
+#pragma unroll
+      for (int j = 0; j < 8; ++j)
+        sn = fma(at(j >= i + 4 ? tri_index(i + 4, j) : tri_index(j, i + 4)), j < 4 ? x0[j] : x1[j - 4], sn);
+    }
+    c[i] = s;
+    cn[i] = sn;
+  }
+  // the top rows of the layer-(m-1) element: from the previous lane, or formed here by lane 0
+  T cp[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) cp[i] = __shfl_up(cn[i], 1, 64);
+  if (lane == 0 && has_p) {
+    const long long tp = t - 1;
+    const T *Sp = sym + static_cast<size_t>(tp >> 6) * gss + static_cast<size_t>(tp & 63) * V;
+    VecT pt[NCH];
+#pragma unroll
+    for (int q = 0; q < NCH; ++q) pt[q] = *reinterpret_cast<const VecT *>(Sp + static_cast<size_t>(q) * 64 * V);
+    T xm[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) xm[j] = x[b[j] + m - 1];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      T sn = T(0);
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const int q = j >= i + 4 ? tri_index(i + 4, j) : tri_index(j, i + 4);
+        sn = fma(pt[q / V][q % V], j < 4 ? xm[j] : x0[j - 4], sn);
+      }
+      cp[i] = sn;
+    }
+  }
+  if (!valid) return;
+  T *zp = z + static_cast<size_t>(k) * 4 * H;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) zp[static_cast<size_t>(i) * H + m] = c[i] + (has_p ? cp[i] : T(0));
+}
+
 // phase 2: y at (column, level) = the units' outputs of the column's cells in colour order, the
-// first written, the rest added (the colour kernel's first-touch write and += order)
+// first written, the rest added (the colour kernel's first-touch write and += order).  With
+// `zu` (columns of at most 4 cells) the column's units come as one int4 (pos*4 + i, or -1 when
+// the column has fewer cells) beside its base dof, so a thread waits for two rounds of loads;
+// otherwise its list entries are loaded before its z values (three rounds).
 template <typename T, bool DOT>
 __global__ __launch_bounds__(kBlock) void k_sym_gather(const T *__restrict__ z, const T *__restrict__ x,
-                                                       T *__restrict__ y, const int2 *__restrict__ cols,
-                                                       const int *__restrict__ off, const int *__restrict__ ent,
-                                                       int ncols, int nl, double *partials, const int *skip, int xwin) {
+                                                        T *__restrict__ y, const int2 *__restrict__ cols,
+                                                        const int *__restrict__ off, const int *__restrict__ ent,
+                                                        const int4 *__restrict__ zu, int ncols, int nl,
+                                                        double *partials, const int *skip, int xwin) {
   if (skip && *skip) return;
   const int H = nl + 1;
   const int lb = xcd_block(blockIdx.x, gridDim.x, xwin);
@@ -977,18 +1101,40 @@ __global__ __launch_bounds__(kBlock) void k_sym_gather(const T *__restrict__ z, 
     if (m == nl && !ci.y) valid = false;
   }
   T acc = T(0);
-  if (valid) {
-    const int q0 = off[col], q1 = off[col + 1];
-    for (int q = q0; q < q1; ++q) {
-      const int en = ent[q];
-      const T v = z[(static_cast<size_t>(en >> 5) * 4 + (en & 31)) * H + m];
-      acc = q == q0 ? v : acc + v;
+  if (valid && zu) {
+    const int4 u = zu[col];
+    const int us[4] = {u.x, u.y, u.z, u.w};
+    T v[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) v[q] = us[q] >= 0 ? z[static_cast<size_t>(us[q]) * H + m] : T(0);
+    acc = v[0];
+#pragma unroll
+    for (int q = 1; q < 4; ++q)
+      if (us[q] >= 0) acc = acc + v[q];
+    y[static_cast<size_t>(ci.x) + m] = acc;
+  } else if (valid) {
+    const int q0 = off[col], cnt = off[col + 1] - q0;
+    constexpr int KU = 4;
+    int en[KU];
+#pragma unroll
+    for (int q = 0; q < KU; ++q) en[q] = q < cnt ? ent[q0 + q] : 0;
+    T v[KU];
+#pragma unroll
+    for (int q = 0; q < KU; ++q)
+      v[q] = q < cnt ? z[(static_cast<size_t>(en[q] >> 5) * 4 + (en[q] & 31)) * H + m] : T(0);
+    acc = v[0];
+#pragma unroll
+    for (int q = 1; q < KU; ++q)
+      if (q < cnt) acc = acc + v[q];
+    for (int q = KU; q < cnt; ++q) {
+      const int e2 = ent[q0 + q];
+      acc = acc + z[(static_cast<size_t>(e2 >> 5) * 4 + (e2 & 31)) * H + m];
     }
     y[static_cast<size_t>(ci.x) + m] = acc;
   }
   if constexpr (DOT) {
-    double v[1] = {valid ? to_d(acc) * to_d(x[static_cast<size_t>(ci.x) + m]) : 0.0};
-    block_partials<1>(v, partials, lb);
+    double d[1] = {valid ? to_d(acc) * to_d(x[static_cast<size_t>(ci.x) + m]) : 0.0};
+    block_partials<1>(d, partials, lb);
   }
 }
 
@@ -2046,6 +2192,18 @@ int gather_blocks(const lfric_op *op) {
   return grid_for(static_cast<long long>(to.gather_ncols) * (to.nlayers + 1));
 }
 
+// set by lfric_op_time_apply: an event recorded between the two phases of the symmetric-packed
+// apply, so that each phase can be timed on its own
+static hipEvent_t g_phase_mark = nullptr;
+
+int launch_lma_apply_gather_marked(const lfric_op *op, const void *x, void *y, hipEvent_t mark, bool *two_phase) {
+  *two_phase = op->sym != nullptr && op->to == op->from && role_kind(*op->to) == RoleKind::W0K0;
+  g_phase_mark = *two_phase ? mark : nullptr;
+  const int rc = launch_lma_apply_gather(op, x, y);
+  g_phase_mark = nullptr;
+  return rc;
+}
+
 struct GatherExtra {
   double *partials = nullptr;  // non-null: fused x.y partial sums, one per block
   const int *skip = nullptr;
@@ -2057,8 +2215,12 @@ static void launch_gather_r(const lfric_op *op, const T *x, T *y, R roles, Gathe
   const int grid = gather_blocks(op);
   if constexpr (N2 == 8 && std::is_same<R, RolesW0K0>::value) {
     if (op->sym) {  // symmetric-packed copy (launch_build_sym): two phases
-      const int nseg = (to.nlayers + 1 + op->sym_seg - 1) / op->sym_seg;
-      if (op->sym_seg == 4)
+      const int nseg = op->sym_seg > 0 ? (to.nlayers + 1 + op->sym_seg - 1) / op->sym_seg : 0;
+      if (op->sym_seg == 0)
+        LAUNCH((k_sym_units1<T>), grid_for(static_cast<long long>(to.ncells) * (to.nlayers + 1)),
+               static_cast<const T *>(op->sym), op->gss, x, op->d_symcol, static_cast<T *>(op->symz), to.ncells,
+               to.nlayers, ex.skip, op->xwin);
+      else if (op->sym_seg == 4)
         LAUNCH((k_sym_units_s<T, 4>), grid_for(static_cast<long long>(to.ncells) * nseg),
                static_cast<const T *>(op->sym), op->gss, x, from.d_map, static_cast<T *>(op->symz), to.ncells,
                to.nlayers, nseg, ex.skip);
@@ -2069,12 +2231,13 @@ static void launch_gather_r(const lfric_op *op, const T *x, T *y, R roles, Gathe
       else
         LAUNCH((k_sym_units<T>), grid_for(static_cast<long long>(to.ncells) * nseg), static_cast<const T *>(op->sym),
                op->gss, x, from.d_map, static_cast<T *>(op->symz), to.ncells, to.nlayers, op->sym_seg, nseg, ex.skip);
+      if (g_phase_mark) (void)hipEventRecord(g_phase_mark, ctx().stream);
       if (ex.partials)
         LAUNCH((k_sym_gather<T, true>), grid, static_cast<const T *>(op->symz), x, y, to.d_gcol, to.d_goff, to.d_gent,
-               to.gather_ncols, to.nlayers, ex.partials, ex.skip, op->xwin);
+               op->d_symzu, to.gather_ncols, to.nlayers, ex.partials, ex.skip, op->xwin);
       else
         LAUNCH((k_sym_gather<T, false>), grid, static_cast<const T *>(op->symz), x, y, to.d_gcol, to.d_goff,
-               to.d_gent, to.gather_ncols, to.nlayers, ex.partials, ex.skip, op->xwin);
+               to.d_gent, op->d_symzu, to.gather_ncols, to.nlayers, ex.partials, ex.skip, op->xwin);
       return;
     }
     if (op->rows) {  // built only for square lowest-order W0 operators (launch_build_rows)
@@ -2313,51 +2476,108 @@ int launch_build_rows(lfric_op *op) {
   return op->precision == LFRIC_R64 ? build_rows_r<double, 8>(op, RolesW0K0{}) : build_rows_r<float, 8>(op, RolesW0K0{});
 }
 
+// device buffers of a symmetric-packed copy under construction, freed unless handed over
+struct SymBuild {
+  void *sym = nullptr, *z = nullptr;
+  int4 *zu = nullptr, *col = nullptr;
+  int *slotpos = nullptr, *flag = nullptr;
+  size_t sym_b = 0, z_b = 0, zu_b = 0, col_b = 0, slotpos_b = 0;
+  ~SymBuild() {
+    dfree(sym, sym_b);
+    dfree(z, z_b);
+    dfree(zu, zu_b);
+    dfree(col, col_b);
+    dfree(slotpos, slotpos_b);
+    dfree(flag, sizeof(int));
+  }
+};
+
 template <typename T>
 static int build_sym_t(lfric_op *op, int S) {
   const FunctionSpace &f = *op->to;
   constexpr int V = chunk_entries<T>(), NCH = 36 / V;
-  const int nseg = (f.nlayers + 1 + S - 1) / S;
-  const long long threads = static_cast<long long>(f.ncells) * nseg;
-  const size_t gss = static_cast<size_t>(64) * (S + 1) * NCH * V;
-  const size_t bytes = sizeof(T) * gss * static_cast<size_t>((threads + 63) / 64);
-  const size_t zbytes = sizeof(T) * static_cast<size_t>(f.ncells) * 4 * (f.nlayers + 1);
-  void *sym = nullptr, *z = nullptr;
-  int *flag = nullptr;
-  if (int rc = dmalloc(&sym, bytes)) return rc;
-  if (int rc = dmalloc(&z, zbytes)) {
-    dfree(sym, bytes);
-    return rc;
+  const int nc = f.ncells, H = f.nlayers + 1;
+  // S = 0: one unit per thread, one triangle per slot, cells in cell-id order; else S+1
+  // triangles per thread, cells in colour-major position order
+  const int nseg = S > 0 ? (H + S - 1) / S : H;
+  const long long threads = static_cast<long long>(nc) * nseg;
+  const size_t gss = static_cast<size_t>(64) * (S > 0 ? S + 1 : 1) * NCH * V;
+  SymBuild b;
+  b.sym_b = sizeof(T) * gss * static_cast<size_t>((threads + 63) / 64);
+  b.z_b = sizeof(T) * static_cast<size_t>(nc) * 4 * H;
+  if (int rc = dmalloc(&b.sym, b.sym_b)) return rc;
+  if (int rc = dmalloc(&b.z, b.z_b)) return rc;
+  if (int rc = dmalloc(reinterpret_cast<void **>(&b.flag), sizeof(int))) return rc;
+  // unit id of (position, bottom vertex i): slot-cell * 4 + i
+  std::vector<int> unit_cell(nc);
+  for (int p = 0; p < nc; ++p) unit_cell[p] = S == 0 ? f.cell_of_pos[p] : p;
+  if (S == 0) {
+    std::vector<int> slotpos(nc);
+    std::vector<int4> col(nc);
+    for (int p = 0; p < nc; ++p) slotpos[f.cell_of_pos[p]] = p;
+    for (int k = 0; k < nc; ++k) {
+      const int *dm = &f.dofmap[static_cast<size_t>(k) * 8];
+      col[k] = make_int4(dm[0] - 1, dm[1] - 1, dm[2] - 1, dm[3] - 1);
+    }
+    b.slotpos_b = sizeof(int) * nc;
+    b.col_b = sizeof(int4) * nc;
+    if (int rc = dmalloc(reinterpret_cast<void **>(&b.slotpos), b.slotpos_b)) return rc;
+    if (int rc = dmalloc(reinterpret_cast<void **>(&b.col), b.col_b)) return rc;
+    LF_HIP(hipMemcpy(b.slotpos, slotpos.data(), b.slotpos_b, hipMemcpyHostToDevice));
+    LF_HIP(hipMemcpy(b.col, col.data(), b.col_b, hipMemcpyHostToDevice));
   }
-  if (int rc = dmalloc(reinterpret_cast<void **>(&flag), sizeof(int))) {
-    dfree(sym, bytes);
-    dfree(z, zbytes);
-    return rc;
+  // the gather's per-column unit table (columns of at most 4 cells), in colour order
+  {
+    const int ncol = f.gather_ncols;
+    std::vector<int> off(ncol + 1), ent;
+    LF_HIP(hipMemcpy(off.data(), f.d_goff, sizeof(int) * (ncol + 1), hipMemcpyDeviceToHost));
+    ent.resize(off[ncol]);
+    LF_HIP(hipMemcpy(ent.data(), f.d_gent, sizeof(int) * ent.size(), hipMemcpyDeviceToHost));
+    bool fits = true;
+    for (int c = 0; fits && c < ncol; ++c) fits = off[c + 1] - off[c] <= 4;
+    if (fits) {
+      std::vector<int4> h(ncol);
+      for (int c = 0; c < ncol; ++c) {
+        int u[4] = {-1, -1, -1, -1};
+        for (int q = off[c]; q < off[c + 1]; ++q) u[q - off[c]] = unit_cell[ent[q] >> 5] * 4 + (ent[q] & 31);
+        h[c] = make_int4(u[0], u[1], u[2], u[3]);
+      }
+      b.zu_b = sizeof(int4) * h.size();
+      if (int rc = dmalloc(reinterpret_cast<void **>(&b.zu), b.zu_b)) return rc;
+      LF_HIP(hipMemcpy(b.zu, h.data(), b.zu_b, hipMemcpyHostToDevice));
+    } else {
+      LF_CHECK(S > 0, LFRIC_E_UNSUPPORTED,
+               "the one-unit-per-thread symmetric copy needs at most 4 cells around each vertex column");
+    }
   }
   int h_flag = 0;
-  hipError_t e = hipMemcpyAsync(flag, &h_flag, sizeof(int), hipMemcpyHostToDevice, ctx().stream);
-  if (e == hipSuccess) {
+  LF_HIP(hipMemcpyAsync(b.flag, &h_flag, sizeof(int), hipMemcpyHostToDevice, ctx().stream));
+  if (S == 0)
+    LAUNCH((k_build_sym1<T>), grid_for(threads), static_cast<const T *>(op->d), op->gs, f.d_ebase, b.slotpos,
+           static_cast<T *>(b.sym), gss, nc, f.nlayers, op->row_top, b.flag);
+  else
     LAUNCH((k_build_sym<T>), grid_for(threads), static_cast<const T *>(op->d), op->gs, f.d_ebase,
-           static_cast<T *>(sym), gss, f.ncells, f.nlayers, S, nseg, op->row_top, flag);
-    e = hipGetLastError();
-  }
-  if (e == hipSuccess) e = hipMemcpyAsync(&h_flag, flag, sizeof(int), hipMemcpyDeviceToHost, ctx().stream);
-  if (e == hipSuccess) e = hipStreamSynchronize(ctx().stream);
-  dfree(flag, sizeof(int));
-  if (e != hipSuccess || h_flag) {
-    dfree(sym, bytes);
-    dfree(z, zbytes);
-    if (e != hipSuccess) return fail(LFRIC_E_HIP, hipGetErrorString(e));
-    return fail(LFRIC_E_UNSUPPORTED, "the operator's element matrices are not symmetric");
-  }
+           static_cast<T *>(b.sym), gss, nc, f.nlayers, S, nseg, op->row_top, b.flag);
+  CHECK_LAUNCH();
+  LF_HIP(hipMemcpyAsync(&h_flag, b.flag, sizeof(int), hipMemcpyDeviceToHost, ctx().stream));
+  LF_HIP(hipStreamSynchronize(ctx().stream));
+  LF_CHECK(!h_flag, LFRIC_E_UNSUPPORTED, "the operator's element matrices are not symmetric");
   dfree(op->sym, op->sym_bytes);
   dfree(op->symz, op->symz_bytes);
-  op->sym = sym;
-  op->sym_bytes = bytes;
-  op->symz = z;
-  op->symz_bytes = zbytes;
+  dfree(op->d_symzu, op->symzu_bytes);
+  dfree(op->d_symcol, op->symcol_bytes);
+  op->sym = b.sym;
+  op->sym_bytes = b.sym_b;
+  op->symz = b.z;
+  op->symz_bytes = b.z_b;
+  op->d_symzu = b.zu;
+  op->symzu_bytes = b.zu_b;
+  op->d_symcol = b.col;
+  op->symcol_bytes = b.col_b;
   op->gss = gss;
   op->sym_seg = S;
+  b.sym = b.z = nullptr;
+  b.zu = b.col = nullptr;
   return LFRIC_OK;
 }
 
@@ -2366,7 +2586,7 @@ int launch_build_sym(lfric_op *op, int levels_per_thread) {
   LF_CHECK(op->to == op->from, LFRIC_E_ARG, "the symmetric-packed copy is for square operators");
   LF_CHECK(role_kind(f) == RoleKind::W0K0 && f.ndf == 8, LFRIC_E_UNSUPPORTED,
            "the symmetric-packed copy is built for lowest-order W0 operators");
-  LF_CHECK(levels_per_thread >= 1 && levels_per_thread <= 64, LFRIC_E_ARG, "levels per thread must be 1..64");
+  LF_CHECK(levels_per_thread >= 0 && levels_per_thread <= 64, LFRIC_E_ARG, "levels per thread must be 0..64");
   for (int c = 0; c < f.ncells; ++c)
     for (int j = 0; j < 4; ++j)
       LF_CHECK(f.dofmap[static_cast<size_t>(c) * 8 + j + 4] == f.dofmap[static_cast<size_t>(c) * 8 + j] + 1,

@@ -274,9 +274,19 @@ class LmaOperator:
     def prepare_symmetric(self, levels_per_thread=0):
         """Symmetric-packed copy (square lowest-order W0 operators with symmetric element
         matrices): the solvers' apply then reads ~70 % of the matrix bytes, same result as the
-        colour-ordered apply bit for bit; levels_per_thread 0 picks 4."""
+        colour-ordered apply bit for bit.  levels_per_thread 0: one (cell, level) unit per thread,
+        the previous layer's top rows passed between lanes; S > 0: S units per thread, each
+        thread's copy repeating the previous segment's last triangle."""
         check(lib().lfric_op_prepare_symmetric(self._h, int(levels_per_thread)))
         return self
+
+    def time_apply(self, x, y, reps=20):
+        """Average ms of the gather apply's kernels over `reps` applies (HIP events): (phase 1,
+        phase 2) of the symmetric-packed form, or (the one kernel, 0)."""
+        out = np.zeros(2, np.float64)
+        check(lib().lfric_op_time_apply(self._h, x._h, y._h, int(reps),
+                                        out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+        return float(out[0]), float(out[1])
 
     def set_window(self, window):
         check(lib().lfric_op_set_window(self._h, int(window)))
