@@ -321,6 +321,8 @@ def main():
                          "in one batch vs sequential FP64 solves; served: Poisson request stream with "
                          "continuous batching vs sequential FP64 solves")
     ap.add_argument("--problems", type=int, default=64)
+    ap.add_argument("--batched-problems", type=int, default=64,
+                    help="single workload: also run the batched configuration with this many problems (0: skip)")
     ap.add_argument("--batch-colour", type=int, default=1, choices=(0, 1),
                     help="batched: colour-ordered multi-RHS apply (1: lowest order on the matrix cores, order 1 "
                          "with register-staged x) or gather form for every problem (0)")
@@ -520,6 +522,10 @@ def main():
     if not args.no_cpu_baseline:
         bh = b.export_field()
         line["cpu_baseline"] = cpu_baseline(fs, op64, bh, pres.iterations, args.cpu_budget, small_host)
+    if args.batched_problems > 0:  # the batched configuration beside the single solve
+        del op32, op64, prec32, prec64, xs32, ys32, xs64, ys64, mixed, plain
+        say = lambda msg: print(f"[batched] {msg}", file=sys.stderr, flush=True)
+        line["batched"] = batched_record(L, args.batched_problems, 1, say)
     print(json.dumps(line))
 
 
@@ -530,56 +536,35 @@ def _common(args, metric, value, unit, ms, config, dtype):
             "config": config}
 
 
-def batched_mix(n_problems, seed=0):
-    """64 W0 SPD problems: mesh C12/C24/C48, 10/30/70 levels, element order 0 or 1 (p = 0.5),
-    4 right-hand sides; element matrices B B^T + 0.1 I with B entries N(0,1)/8 (unscaled)."""
-    rng = np.random.default_rng(seed)
-    return [(int(rng.choice((12, 24, 48))), int(rng.choice((10, 30, 70))), int(rng.integers(0, 2)), 4,
-             int(rng.integers(0, 2**31))) for _ in range(n_problems)]
-
-
-def bench_batched(args):
-    import lfric_core_amd as L
-    from lfric_core_amd.batch import Batch
-
-    L.init(0)
-    mix = batched_mix(args.problems)
-    probs = []
-    say = lambda msg: print(f"[batched] {msg}", file=sys.stderr, flush=True)
+def batched_record(L, problems, steps, say, colour=True):
+    """The batched configuration (64 uneven W0 problems, 4 RHS each) solved as one batch: a
+    warm-up batch with the autotuner, then `steps` timed batches (host polls every 8 inner
+    iterations), one latency batch polled every 2 iterations (each problem's completion time),
+    the batch apply's time and HBM rate, and the same 256 right-hand sides solved one after
+    another with plain FP64 CG."""
+    from lfric_core_amd.batch import Batch, batched_mix, build_problem
+    mix = batched_mix(problems)
     say(f"building {len(mix)} problems")
-    for (n, nl, order, nrhs, seed) in mix:
-        fs = L.FunctionSpace.cubedsphere(n, nl, L.W0, order)
-        op64 = L.LmaOperator(fs).generate_spd(seed, scaled=False)
-        op32 = op64.convert(L.R32)
-        d32 = op32.mm_diagonal(L.FieldVector(fs, L.R32))
-        b = L.FieldVector(fs, L.R64, nrhs).fill_uniform(seed, 3)
-        x = L.FieldVector(fs, L.R64, nrhs)
-        probs.append(dict(fs=fs, op64=op64, op32=op32, d32=d32, b=b, x=x, nrhs=nrhs, key=(n, nl, order)))
+    probs = [build_problem(*k) for k in mix]
     total_dofs = sum(p["fs"].undf * p["nrhs"] for p in probs)
-
     say(f"built: {total_dofs} dofs x RHS in total")
 
-    def run_batch(tune):
-        bt = Batch(r_tol=1e-8, a_tol=0.0, max_iter=4000, check_every=8)
-        bt.config(colour=bool(args.batch_colour))
+    def run_batch(tune, check_every):
+        bt = Batch(r_tol=1e-8, a_tol=0.0, max_iter=4000, check_every=check_every)
+        bt.config(colour=colour)
         ids = []
         for p in probs:
             p["x"].set_scalar(0.0)
             ids.append(bt.add(p["op64"], p["op32"], p["d32"], p["x"], p["b"]))
         L.synchronize()
         t0 = time.perf_counter()
-        done_at = {}
-        tuned = None
-        last = 0.0
+        done_at, tuned = {}, None
         while True:
             na = bt.run(1)
             if tune and tuned is None:
                 tuned = bt.tune()
             now = time.perf_counter() - t0
-            if now - last > 5.0:
-                last = now
-                say(f"{now:.1f} s: {na} problems active, {bt.stats()['iterations']} batch iterations")
-            for i, pid in enumerate(ids):
+            for pid in ids:
                 if pid not in done_at and bt.finished(pid):
                     done_at[pid] = now
             if na == 0:
@@ -590,35 +575,35 @@ def bench_batched(args):
         bt.close()
         return wall, np.array(sorted(done_at.values())), res, st, tuned
 
-    w, _, _, st, tuned = run_batch(tune=True)  # warm-up, and the autotuner's choice for the timed runs
+    w, _, _, st, tuned = run_batch(True, 8)  # warm-up, and the autotuner's choice
     say(f"warm-up batch {w:.2f} s, autotuned apply {tuned:.3f} ms: {st}")
-    # the batch's multi-RHS apply of all 64 problems at once, matrix-core and gather forms
-    apply_ms = {}
-    for form, on in (("colour_mfma", True), ("gather", False)):
-        bt = Batch(r_tol=1e-8, a_tol=0.0, max_iter=4000, check_every=8)
-        bt.config(colour=on)
-        for p in probs:
-            p["x"].set_scalar(0.0)
-            bt.add(p["op64"], p["op32"], p["d32"], p["x"], p["b"])
-        bt.run(1)
-        apply_ms[form] = bt.tune()
-        bt.close()
-    say(f"batch apply (autotuned): {apply_ms}")
     times = []
-    for _ in range(max(1, args.steps)):
-        wall, done, res, st, _ = run_batch(tune=False)
+    for _ in range(max(1, steps)):
+        wall, _, res, st, _ = run_batch(False, 8)
         times.append(wall)
-        say(f"timed batch {wall:.2f} s")
-    worst = max(r.true_rel_res for rr in res for r in rr)
-    ok = all(r.status == 0 for rr in res for r in rr)
+        say(f"timed batch {wall:.3f} s")
+    lat_wall, done, lat_res, _, _ = run_batch(False, 2)
+    say(f"latency batch {lat_wall:.3f} s")
+    worst = max(r.true_rel_res for rr in res + lat_res for r in rr)
+    ok = all(r.status == 0 for rr in res + lat_res for r in rr)
+    # the batch's multi-RHS apply of all problems at once (autotuned), and its algorithmic bytes:
+    # every element matrix once in FP32, x and y once (nrhs values per dof), the dofmaps once
+    bt = Batch(r_tol=1e-8, a_tol=0.0, max_iter=4000, check_every=8)
+    bt.config(colour=colour)
+    for p in probs:
+        p["x"].set_scalar(0.0)
+        bt.add(p["op64"], p["op32"], p["d32"], p["x"], p["b"])
+    bt.run(1)
+    apply_ms = bt.tune()
+    bt.close()
+    apply_bytes_total = sum(p["fs"].ncell_3d * p["fs"].ndf ** 2 * 4 + 2 * p["fs"].undf * p["nrhs"] * 4
+                            + 4 * p["fs"].ncells * p["fs"].ndf for p in probs)
     # sequential plain FP64 solves of the same systems, one RHS at a time
-    seq_t = 0.0
-    seq_it = 0
+    seq_t, seq_it = 0.0, 0
     for p in probs:
         B = p["b"].export_field().reshape(p["fs"].undf, p["nrhs"])
         prec = L.DiagonalPreconditioner.from_operator(p["op64"])
         cg = L.ConjugateGradient(p["op64"], prec, 1e-8, 0.0, 20000, check_every=16)
-        say(f"sequential FP64: C{p['key'][0]}x{p['key'][1]} order {p['key'][2]}, {seq_t:.1f} s so far")
         for r in range(p["nrhs"]):
             bv = L.FieldVector(p["fs"]).import_field(B[:, r])
             xv = L.FieldVector(p["fs"])
@@ -629,29 +614,46 @@ def bench_batched(args):
             L.synchronize()
             seq_t += time.perf_counter() - t0
             seq_it += out.iterations
+    say(f"sequential FP64 {seq_t:.2f} s")
     wall = float(np.median(times))
-    line = _common(args, "solved dofs/s at rel-res 1e-8 (batched mixed precision vs sequential FP64), p50/p99 "
-                         "solve ms", total_dofs / wall, "solved dofs/s", wall * 1e3,
-                   {"workload": f"{len(probs)} uneven W0 SPD problems (C12/C24/C48, 10/30/70 levels, order 0/1), "
-                                "element matrices B B^T + 0.1 I with B ~ N(0,1)/8, 4 RHS each, Jacobi CG to 1e-8 "
-                                "in FP64 refinement, one batch",
-                    "multi_rhs_apply": "colour-ordered: matrix cores (order 0), register-staged FMA (order 1)"
-                    if args.batch_colour else "gather form",
-                    "problems": len(probs), "rhs_total": sum(p["nrhs"] for p in probs),
-                    "dofs_total": total_dofs, "parallelism": "one GPU, one batch"},
-                   "f32 inner CG / f64 refinement")
-    line.update({
-        "solves_per_s": len(probs) / wall,
-        "rhs_solves_per_s": sum(p["nrhs"] for p in probs) / wall,
-        "solve_ms": {"batch_wall": wall * 1e3, "problem_done_p50": float(np.percentile(done, 50) * 1e3),
-                     "problem_done_p99": float(np.percentile(done, 99) * 1e3)},
+    nrhs_total = sum(p["nrhs"] for p in probs)
+    return {
+        "workload": f"{len(probs)} uneven W0 SPD problems (C12/C24/C48, 10/30/70 levels, order 0/1), element "
+                    "matrices B B^T + 0.1 I with B ~ N(0,1)/8, 4 RHS each, Jacobi CG to 1e-8 in FP64 refinement, "
+                    "one batch",
+        "multi_rhs_apply": "colour-ordered: matrix cores (order 0), register-staged FMA (order 1)"
+        if colour else "gather form",
+        "problems": len(probs), "rhs_total": nrhs_total, "dofs_total": total_dofs,
+        "solved_dofs_per_s": total_dofs / wall, "solves_per_s": len(probs) / wall,
+        "rhs_solves_per_s": nrhs_total / wall, "batch_wall_ms": wall * 1e3,
+        "batch_wall_ms_all": [t * 1e3 for t in times],
+        "latency_ms": {"poll_every_iterations": 2, "batch_wall": lat_wall * 1e3,
+                       "problem_done_p50": float(np.percentile(done, 50) * 1e3),
+                       "problem_done_p99": float(np.percentile(done, 99) * 1e3),
+                       "problem_done_min": float(done.min() * 1e3)},
         "sequential_fp64": {"wall_ms": seq_t * 1e3, "dofs_per_s": total_dofs / seq_t,
                             "speedup_of_batch": seq_t / wall, "iterations": seq_it},
         "verification": {"all_converged": ok, "worst_true_rel_res_fp64": worst},
         "autotuned": {"block": st["block"], "grid_mult": st["grid_mult"]},
-        "batch_apply_ms": apply_ms,
+        "batch_apply": {"ms": apply_ms, "alg_bytes": apply_bytes_total,
+                        "gbs": apply_bytes_total / (apply_ms * 1e-3) / 1e9,
+                        "frac": apply_bytes_total / (apply_ms * 1e-3) / 1e9 / HBM_PEAK_GBS},
         "batch_iterations": st["iterations"],
-    })
+    }
+
+
+def bench_batched(args):
+    import lfric_core_amd as L
+    L.init(0)
+    say = lambda msg: print(f"[batched] {msg}", file=sys.stderr, flush=True)
+    rec = batched_record(L, args.problems, args.steps, say, colour=bool(args.batch_colour))
+    line = _common(args, "solved dofs/s at rel-res 1e-8 (batched mixed precision vs sequential FP64), p50/p99 "
+                         "solve ms", rec["solved_dofs_per_s"], "solved dofs/s", rec["batch_wall_ms"],
+                   {"workload": rec["workload"], "multi_rhs_apply": rec["multi_rhs_apply"],
+                    "problems": rec["problems"], "rhs_total": rec["rhs_total"], "dofs_total": rec["dofs_total"],
+                    "parallelism": "one GPU, one batch"},
+                   "f32 inner CG / f64 refinement")
+    line.update({k: v for k, v in rec.items() if k not in ("workload", "multi_rhs_apply")})
     print(json.dumps(line))
 
 

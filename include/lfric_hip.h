@@ -113,6 +113,23 @@ int lfric_fs_get_boundary_dofs(const lfric_fs *fs, int *boundary_dofs);
 int lfric_fs_set_boundary_dofs(lfric_fs *fs, const int *boundary_dofs);
 void lfric_fs_destroy(lfric_fs *fs);
 
+/* extrusions of infrastructure/source/mesh/extrusion_mod.f90: uniform k/nl (:292-305),
+ * quadratic (k/nl)^2 (:341-354), geometric with stretching 1.03 (:390-409, :618-629) */
+#define LFRIC_EXTRUSION_UNIFORM 0
+#define LFRIC_EXTRUSION_QUADRATIC 1
+#define LFRIC_EXTRUSION_GEOMETRIC 2
+/* eta(0:nlayers) of an extrusion (the extrude methods of extrusion_mod.f90). */
+int lfric_extrusion_eta(int kind, int nlayers, double *eta);
+/* The Cartesian coordinate fields chi1..3 (FP64 vectors on fs) of a generated lowest-order W0
+ * cubed-sphere space: vertex longitude/latitude of the mesh generator
+ * (mesh_tools/source/support/gencube_ps_mod.F90:887-1113), levels from the extrusion over a
+ * zero surface height to domain_top (mesh_constructor_helper_functions_mod.F90:120-252,
+ * 450-477), projected onto spheres of radius scaled_radius + height as assign_coordinate_field
+ * does for Cartesian coordinates (components/driver/source/driver_coordinates_mod.F90:59-290,
+ * 357-454).  Computed on the host, uploaded once. */
+int lfric_fs_coordinates(const lfric_fs *fs, int extrusion, double domain_top, double scaled_radius,
+                         lfric_vec *chi1, lfric_vec *chi2, lfric_vec *chi3);
+
 /* ---- field vectors (vector_mod abstract_vector_type, infrastructure/source/field/
  *      vector_mod.F90:17-59; field_vector_type / r_solver_field_vector_type) ----------- */
 int lfric_vec_create(const lfric_fs *fs, int precision, int nrhs, lfric_vec **v);

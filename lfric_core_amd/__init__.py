@@ -23,7 +23,7 @@ Reference interfaces mirrored (paths relative to the lfric_core tree):
 """
 from ._lib import (LfricError, NotConverged, lib, init, synchronize, timer_start, timer_stop,
                    device_bytes, set_apply_form, R64, R32, BF16, F16, W0, W2, W3)
-from .fem import FunctionSpace, FieldVector, LmaOperator
+from .fem import FunctionSpace, FieldVector, LmaOperator, extrusion_eta
 from .cma import ColumnwiseOperator, ColumnwiseInversePreconditioner, tri_solve
 from .solvers import (NullPreconditioner, DiagonalPreconditioner, ColumnJacobiPreconditioner,
                       LinearOperator, ConjugateGradient, MixedPrecisionCG, GMRES, MixedPrecisionGMRES,

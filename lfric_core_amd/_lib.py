@@ -131,6 +131,8 @@ SIGNATURES = {
     "lfric_solver_opts_default": (None, [ctypes.POINTER(SolverOpts)]),
     "lfric_residual_norm": (_I, [_VP, _VP, _VP, _DP]),
     "lfric_cg_solve": (_I, [_VP, _VP, _VP, _VP, ctypes.POINTER(SolverOpts), ctypes.POINTER(SolverRes)]),
+    "lfric_extrusion_eta": (_I, [_I, _I, _VP]),
+    "lfric_fs_coordinates": (_I, [_VP, _I, _D, _D, _VP, _VP, _VP]),
     "lfric_fs_get_boundary_dofs": (_I, [_VP, _VP]),
     "lfric_fs_set_boundary_dofs": (_I, [_VP, _VP]),
     "lfric_dg_inc_apply": (_I, [_VP, _VP, _VP]),

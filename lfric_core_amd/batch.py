@@ -4,8 +4,34 @@ device (FP32 CG inside FP64 iterative refinement per RHS)."""
 import ctypes
 from dataclasses import dataclass
 
-from ._lib import LFRIC_OK, R32, SolverRes, check, lib
+import numpy as np
+
+from ._lib import LFRIC_OK, R32, R64, W0, SolverRes, check, lib
 from .solvers import SolverOptions, SolverResult
+
+
+def batched_mix(n_problems, seed=0):
+    """The batched configuration's problems: mesh C12/C24/C48, 10/30/70 levels, W0 element order
+    0 or 1 (p = 0.5), 4 right-hand sides, a generator seed each, drawn from `seed`:
+    [(edge cells, levels, order, nrhs, problem seed)]."""
+    rng = np.random.default_rng(seed)
+    return [(int(rng.choice((12, 24, 48))), int(rng.choice((10, 30, 70))), int(rng.integers(0, 2)), 4,
+             int(rng.integers(0, 2**31))) for _ in range(n_problems)]
+
+
+def build_problem(n, nl, order, nrhs, seed, scaled=False):
+    """One batch problem on the device: the cubed-sphere W0 space, the FP64 operator
+    (A_e = B_e B_e^T + 0.1 I from the counter-based generator) and its FP32 copy, the FP32
+    Jacobi diagonal, b ~ U(-1, 1) with nrhs columns and x = 0."""
+    from .fem import FieldVector, FunctionSpace, LmaOperator
+    fs = FunctionSpace.cubedsphere(n, nl, W0, order)
+    op64 = LmaOperator(fs).generate_spd(seed, scaled=scaled)
+    op32 = op64.convert(R32)
+    d32 = op32.mm_diagonal(FieldVector(fs, R32))
+    b = FieldVector(fs, R64, nrhs).fill_uniform(seed, 3)
+    x = FieldVector(fs, R64, nrhs)
+    x.set_scalar(0.0)
+    return dict(fs=fs, op64=op64, op32=op32, d32=d32, b=b, x=x, nrhs=nrhs, key=(n, nl, order, nrhs, seed))
 
 
 @dataclass

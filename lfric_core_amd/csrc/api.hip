@@ -1037,6 +1037,31 @@ int lfric_dg_inc_apply(const lfric_op *op, const lfric_vec *x, lfric_vec *y) {
   return launch_dg_inc_apply(op, x->d, y->d);
 }
 
+int lfric_extrusion_eta(int kind, int nlayers, double *eta) {
+  LF_CHECK(eta, LFRIC_E_ARG, "null argument");
+  std::vector<double> e;
+  if (int rc = extrusion_eta(kind, nlayers, e)) return rc;
+  std::copy(e.begin(), e.end(), eta);
+  return LFRIC_OK;
+}
+
+int lfric_fs_coordinates(const lfric_fs *fs, int extrusion, double domain_top, double scaled_radius, lfric_vec *chi1,
+                         lfric_vec *chi2, lfric_vec *chi3) {
+  if (int rc = require_init()) return rc;
+  LF_CHECK(fs && chi1 && chi2 && chi3, LFRIC_E_ARG, "null argument");
+  for (const lfric_vec *c : {chi1, chi2, chi3})
+    LF_CHECK(c->fs == fs->fs && c->precision == LFRIC_R64 && c->nrhs == 1, LFRIC_E_ARG,
+             "coordinate fields must be FP64 one-RHS vectors on the space");
+  const size_t n = static_cast<size_t>(fs->fs->undf);
+  std::vector<double> h(3 * n, 0.0);
+  if (int rc = cubedsphere_chi(*fs->fs, extrusion, domain_top, scaled_radius, h.data(), h.data() + n, h.data() + 2 * n))
+    return rc;
+  LF_HIP(hipMemcpy(chi1->d, h.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+  LF_HIP(hipMemcpy(chi2->d, h.data() + n, sizeof(double) * n, hipMemcpyHostToDevice));
+  LF_HIP(hipMemcpy(chi3->d, h.data() + 2 * n, sizeof(double) * n, hipMemcpyHostToDevice));
+  return LFRIC_OK;
+}
+
 int lfric_fs_set_boundary_dofs(lfric_fs *fs, const int *bv) {
   if (int rc = require_init()) return rc;
   LF_CHECK(fs && bv, LFRIC_E_ARG, "null argument");

@@ -57,6 +57,7 @@ struct Topology {
   std::vector<int> edge_on_cell;          // [cell][12]
   std::vector<int> colour_of_cell;        // 1..ncolours
   int ncolours = 0;
+  std::vector<double> vert_lon, vert_lat; // by local vertex id, radians (generated cubed spheres)
 };
 int build_cubedsphere_topology(int edge_cells, Topology &t);
 // the topology below a global mesh given as [cell][W,S,E,N] neighbours (0: none) and
@@ -68,6 +69,10 @@ int read_ugrid_mesh(const std::string &path, const std::string &mesh_name, std::
                     std::vector<int> &face_links, int &nfaces, int &npanels);
 int build_dofmap(const Topology &t, int nlayers, int space, int order, std::vector<int> &dofmap,
                  int &ndf, int &undf);
+int extrusion_eta(int kind, int nlayers, std::vector<double> &eta);
+struct FunctionSpace;
+int cubedsphere_chi(const FunctionSpace &f, int kind, double domain_top, double scaled_radius, double *chi1,
+                    double *chi2, double *chi3);
 
 // ---- function space -------------------------------------------------------------------------
 struct FunctionSpace {

@@ -11,8 +11,13 @@
 //   3D face / edge ids                    infrastructure/source/mesh/mesh_constructor_helper_functions_mod.F90:266-438
 //   dofmap                                infrastructure/source/function_space/function_space_constructor_helper_functions_mod.F90:1715-2300
 //   colouring                             infrastructure/source/mesh/mesh_colouring_mod.F90:151-308, 703-837
+//   vertex longitude / latitude           gencube_ps_mod.F90:887-1113 (calc_coords, before orientation)
+//   extrusion eta                         infrastructure/source/mesh/extrusion_mod.f90:292-305, 341-354, 390-409, 618-629
+//   Cartesian coordinate fields           mesh_constructor_helper_functions_mod.F90:120-252 (mesh_extruder),
+//                                         :450-477 (set_dz), components/driver/source/driver_coordinates_mod.F90:357-454
 #include <algorithm>
 #include <array>
+#include <cfloat>
 #include <cmath>
 #include <cstring>
 
@@ -146,6 +151,80 @@ void vertex_numbering(Panelled &P) {
     P.vx(nb(c, DN), CSW) = P.vx(c, CNW);
     P.vx(nb(c, DN), CSE) = P.vx(c, CNE);
   }
+}
+
+// xyz2ll of coord_transform_mod (infrastructure/source/utilities/coord_transform_mod.F90:226-283):
+// longitude from atan(y/x) moved to the x < 0 half-plane by +-pi, latitude atan(z / sqrt(x^2+y^2))
+constexpr double kPi = 3.141592653589793115997963468544185161590576171875;  // 4 atan(1) in FP64
+void xyz_to_lonlat(double x, double y, double z, double &lon, double &lat) {
+  if (x == 0.0) {
+    lon = (y >= 0.0 ? 0.5 : -0.5) * kPi;
+  } else {
+    lon = std::atan(y / x);
+    if (x < 0.0) lon = y > 0.0 ? lon + kPi : lon - kPi;
+  }
+  const double r = std::sqrt(x * x + y * y);
+  if (r <= DBL_EPSILON) {
+    lat = (z > 0.0 ? 0.5 : -0.5) * kPi;
+    lon = 0.0;
+  } else {
+    lat = std::atan(z / r);
+  }
+}
+
+// Longitude and negated latitude (radians) of every global vertex, on the equiangular grid of
+// each panel: grid point (i, j) of a panel is the unit cube-face point (1, tan(l1), tan(l2))
+// normalised, l = -pi/4 + (0..n) pi/(2n), turned onto the panel's face.  Panels 1-4 give their
+// vertices ids in grid order; the top and bottom panels' vertices are found through the corners
+// of the unoriented cells.  P.vert must hold the numbering before orient_panels.
+void vertex_lonlat(const Panelled &P, std::vector<double> &lon, std::vector<double> &lat) {
+  const int n = P.n, cpp = P.cpp;
+  lon.assign(static_cast<size_t>(6) * cpp + 2, 0.0);
+  lat.assign(lon.size(), 0.0);
+  const double dl = 0.5 * kPi / n, pio4 = kPi / 4.0;
+  auto unit = [&](int i, int j, double &xs, double &ys, double &zs) {
+    const double t1 = std::tan(i * dl - pio4), t2 = std::tan(j * dl - pio4);
+    xs = 1.0 / std::sqrt(1.0 + t1 * t1 + t2 * t2);
+    ys = xs * t1;
+    zs = xs * t2;
+  };
+  auto put = [&](int gid, double x, double y, double z) {
+    double lo, la;
+    xyz_to_lonlat(x, y, z, lo, la);
+    lon[gid - 1] = lo;
+    lat[gid - 1] = -la;
+  };
+  auto V = [&](int cell, int corner) { return P.vert[(cell - 1) * 4 + corner]; };
+  double xs, ys, zs;
+  for (int y = 1; y <= n; ++y)
+    for (int x = 1; x <= n; ++x) {
+      unit(x - 1, y - 1, xs, ys, zs);
+      const int v = (y - 1) * n + x;
+      put(v, xs, ys, zs);                  // panel 1
+      put(v + cpp, -ys, xs, zs);           // panel 2
+      put(v + 2 * cpp, -xs, -ys, zs);      // panel 3
+      put(v + 3 * cpp, ys, -xs, zs);       // panel 4
+    }
+  for (int y = 1; y <= n - 1; ++y)        // top panel: SE corners of its interior cells
+    for (int x = 1; x <= n - 1; ++x) {
+      unit(x, y, xs, ys, zs);
+      put(V(4 * cpp + (y - 1) * n + x, CSE), -ys, zs, -xs);
+    }
+  for (int y = 1; y <= n; ++y)            // bottom panel: NW corners
+    for (int x = 1; x <= n; ++x) {
+      unit(x - 1, y - 1, xs, ys, zs);
+      put(V(5 * cpp + (y - 1) * n + x, CNW), -ys, -zs, xs);
+    }
+  for (int x = 1; x <= n; ++x) {          // bottom panel: SW corners of its last row
+    unit(x - 1, n, xs, ys, zs);
+    put(V(6 * cpp - n + x, CSW), -ys, -zs, xs);
+  }
+  for (int y = 1; y <= n; ++y) {          // bottom panel: NE corners of its last column
+    unit(n, y - 1, xs, ys, zs);
+    put(V(5 * cpp + y * n, CNE), -ys, -zs, xs);
+  }
+  unit(n, n, xs, ys, zs);                 // the last vertex
+  put(6 * cpp + 2, -ys, -zs, xs);
 }
 
 // Panels 3 and 4 are turned by one slot (cshift +1), panel 5 by one slot the other way.
@@ -302,8 +381,88 @@ int build_cubedsphere_topology(int n, Topology &t) {
   P.vert.assign(static_cast<size_t>(P.nc) * 4, 0);
   panel_adjacency(P);
   vertex_numbering(P);
+  std::vector<double> lon, lat;
+  vertex_lonlat(P, lon, lat);
   orient_panels(P);
-  return build_topology(P.nc, P.next, P.vert, 6, t);
+  if (int rc = build_topology(P.nc, P.next, P.vert, 6, t)) return rc;
+  // coordinates by local vertex id (orientation only permutes the corner slots)
+  t.vert_lon.assign(t.nverts, 0.0);
+  t.vert_lat.assign(t.nverts, 0.0);
+  for (size_t i = 0; i < P.vert.size(); ++i) {
+    t.vert_lon[t.vert_on_cell[i] - 1] = lon[P.vert[i] - 1];
+    t.vert_lat[t.vert_on_cell[i] - 1] = lat[P.vert[i] - 1];
+  }
+  return LFRIC_OK;
+}
+
+// eta(0:nl) of the uniform (k/nl), quadratic ((k/nl)^2) and geometric (stretching 1.03)
+// extrusions
+int extrusion_eta(int kind, int nl, std::vector<double> &eta) {
+  LF_CHECK(nl >= 1, LFRIC_E_ARG, "extrusion needs at least one layer");
+  eta.assign(static_cast<size_t>(nl) + 1, 0.0);
+  switch (kind) {
+    case LFRIC_EXTRUSION_UNIFORM:
+      for (int k = 0; k <= nl; ++k) eta[k] = static_cast<double>(k) / static_cast<double>(nl);
+      return LFRIC_OK;
+    case LFRIC_EXTRUSION_QUADRATIC:
+      for (int k = 0; k <= nl; ++k) {
+        const double r = static_cast<double>(k) / static_cast<double>(nl);
+        eta[k] = r * r;
+      }
+      return LFRIC_OK;
+    case LFRIC_EXTRUSION_GEOMETRIC: {
+      // stretching_factor**number_of_layers is an integer power (repeated products here);
+      // geometric_func's stretching_factor**real(k) a real one
+      const double sf = 1.03;
+      double sfn = 1.0;
+      for (int k = 0; k < nl; ++k) sfn *= sf;
+      const double deta = (sf - 1.0) / (sfn - 1.0);
+      for (int k = 1; k <= nl; ++k) eta[k] = deta * (std::pow(sf, static_cast<double>(k)) - 1.0) / (sf - 1.0);
+      return LFRIC_OK;
+    }
+  }
+  return fail(LFRIC_E_ARG, "unknown extrusion");
+}
+
+// Cartesian coordinates of the lowest-order W0 dofs of a generated cubed-sphere space, as
+// assign_coordinate_field's xyz branch leaves them: dz = (eta(k) - eta(k-1)) * depth; vertex at
+// level j: (x, y, z) = llr2xyz(lon, lat, 1 + sum(dz(1:j))) (the mesh extruder's radius over a
+// zero surface height), projected onto the shell of radius scaled_radius + sum(dz(1:j)); the
+// top level's radius is (scaled_radius + sum(dz(1:nl-1))) + dz(nl), the top-node form the last
+// layer writes.
+int cubedsphere_chi(const FunctionSpace &f, int kind, double domain_top, double scaled_radius, double *chi1,
+                    double *chi2, double *chi3) {
+  LF_CHECK(f.topo && !f.topo->vert_lon.empty(), LFRIC_E_UNSUPPORTED,
+           "coordinates are known for generated cubed-sphere spaces only");
+  LF_CHECK(f.space == LFRIC_W0 && f.order == 0, LFRIC_E_UNSUPPORTED, "coordinate fields live on lowest-order W0");
+  LF_CHECK(domain_top > 0.0 && scaled_radius > 0.0, LFRIC_E_ARG, "domain top and radius must be positive");
+  const int nl = f.nlayers;
+  std::vector<double> eta;
+  if (int rc = extrusion_eta(kind, nl, eta)) return rc;
+  std::vector<double> dz(nl), sdz(static_cast<size_t>(nl) + 1, 0.0);
+  for (int k = 1; k <= nl; ++k) dz[k - 1] = (eta[k] - eta[k - 1]) * domain_top;
+  for (int k = 1; k <= nl; ++k) sdz[k] = sdz[k - 1] + dz[k - 1];
+  const Topology &t = *f.topo;
+  std::vector<char> done(t.nverts, 0);
+  for (int c = 0; c < f.ncells; ++c)
+    for (int i = 0; i < 4; ++i) {
+      const int v = t.vert_on_cell[static_cast<size_t>(c) * 4 + i] - 1;
+      if (done[v]) continue;
+      done[v] = 1;
+      const int base = f.dofmap[static_cast<size_t>(c) * 8 + i] - 1;
+      const double sl = std::sin(t.vert_lon[v]), cl = std::cos(t.vert_lon[v]);
+      const double sa = std::sin(t.vert_lat[v]), ca = std::cos(t.vert_lat[v]);
+      for (int j = 0; j <= nl; ++j) {
+        const double r = 1.0 + sdz[j];
+        const double x = r * cl * ca, y = r * sl * ca, z = r * sa;
+        double rc = j < nl ? (scaled_radius + sdz[j]) + 0.0 * dz[j] : (scaled_radius + sdz[nl - 1]) + 1.0 * dz[nl - 1];
+        rc = rc / std::sqrt(x * x + y * y + z * z);
+        chi1[base + j] = x * rc;
+        chi2[base + j] = y * rc;
+        chi3[base + j] = z * rc;
+      }
+    }
+  return LFRIC_OK;
 }
 
 // Everything below the global mesh: local vertex ids, the 3D connectivity of the base layer

@@ -46,6 +46,16 @@ class FunctionSpace:
                                           ctypes.byref(h)))
         return cls(h)
 
+    def coordinates(self, extrusion=1, domain_top=80000.0, scaled_radius=6371229.0):
+        """The Cartesian coordinate fields (chi1, chi2, chi3) of a generated lowest-order W0
+        cubed-sphere space: the mesh generator's vertices on spheres of radius scaled_radius +
+        height, heights from the extrusion (0 uniform, 1 quadratic, 2 geometric) to domain_top
+        (assign_coordinate_field, components/driver/source/driver_coordinates_mod.F90:59-454)."""
+        chi = [FieldVector(self) for _ in range(3)]
+        check(lib().lfric_fs_coordinates(self._h, int(extrusion), float(domain_top), float(scaled_radius),
+                                         chi[0]._h, chi[1]._h, chi[2]._h))
+        return chi
+
     def boundary_dofs(self):
         """get_boundary_dofs (function_space_mod.F90:791-801): (ndf, 2), column 0 bottom,
         column 1 top; 0 flags a dof on that boundary."""
@@ -323,6 +333,13 @@ class LmaOperator:
             except Exception:
                 pass
             self._h = None
+
+
+def extrusion_eta(kind, nlayers):
+    """eta(0:nlayers) of an extrusion: 0 uniform, 1 quadratic, 2 geometric (extrusion_mod.f90)."""
+    e = np.zeros(nlayers + 1)
+    check(lib().lfric_extrusion_eta(int(kind), int(nlayers), ptr(e)))
+    return e
 
 
 def read_ugrid(path, mesh=None):

@@ -76,6 +76,9 @@ def lib():
             "orc_transpose_apply": (None, [_I, _I, _I, _VP, _VP, _I, _I, _I, _VP, _I, _I, _VP, _VP, _VP, _VP]),
             "orc_dg_apply": (None, [_I, _I, _I, _I, _VP, _I, _I, _VP, _VP, _VP, _VP]),
             "orc_mm_diagonal": (None, [_I, _I, _I, _I, _VP, _VP, _VP]),
+            "orc_cubedsphere_lonlat": (_I, [_I, _VP, _VP]),
+            "orc_eta": (_I, [_I, _I, _VP]),
+            "orc_chi_w0": (_I, [_I, _I, _VP, _VP, _VP, _VP, _I, _D, _D, _VP, _VP, _VP]),
             "orc_dg_inc_apply": (None, [_I, _I, _I, _I, _VP, _I, _I, _VP, _VP, _VP, _VP]),
             "orc_enforce_bc": (None, [_I, _I, _I, _VP, _VP, _VP]),
             "orc_boundary_dofs": (_I, [_I, _I, _I, _VP]),
@@ -224,6 +227,7 @@ class Mesh:
     def __init__(self, n, nlayers, space=W0, order=0):
         self.n, self.nlayers, self.space, self.order = n, nlayers, space, order
         self.cell_next, gvoc = cubedsphere_topology(n)
+        self.vert_gid = gvoc
         self.vert_on_cell, self.nverts = local_vert_lids(gvoc)
         self.face_on_cell, self.edge_on_cell, self.nfaces_2d, self.nedges_2d = mesh_connectivity(
             self.cell_next, self.vert_on_cell)
@@ -633,6 +637,37 @@ def dg_apply(to_mesh, from_mesh, A, x):
                        from_mesh.ndf, from_mesh.undf, _p(from_mesh.dofmap), _p(flat_matrix(A)),
                        _p(np.ascontiguousarray(x, np.float64)), _p(y))
     return y
+
+
+def cubedsphere_lonlat(n):
+    """(longitude, latitude) in radians of the mesh generator's vertices, by global vertex id."""
+    nv = 6 * n * n + 2
+    lon, lat = np.zeros(nv), np.zeros(nv)
+    if lib().orc_cubedsphere_lonlat(n, _p(lon), _p(lat)) != 0:
+        raise ValueError(n)
+    return lon, lat
+
+
+def eta(kind, nl):
+    """eta(0:nl) of an extrusion: 0 uniform, 1 quadratic, 2 geometric."""
+    e = np.zeros(nl + 1)
+    if lib().orc_eta(kind, nl, _p(e)) != 0:
+        raise ValueError((kind, nl))
+    return e
+
+
+def chi_w0(mesh, kind, domain_top, scaled_radius):
+    """The Cartesian coordinate fields (chi1, chi2, chi3) on a lowest-order W0 cubed-sphere mesh."""
+    lon, lat = cubedsphere_lonlat(mesh.n)
+    vlon, vlat = np.zeros(mesh.nverts), np.zeros(mesh.nverts)
+    g, l = np.asarray(mesh.vert_gid).ravel(), np.asarray(mesh.vert_on_cell).ravel()
+    vlon[l - 1], vlat[l - 1] = lon[g - 1], lat[g - 1]
+    chi = [np.zeros(mesh.undf) for _ in range(3)]
+    voc = np.ascontiguousarray(mesh.vert_on_cell, np.int32)
+    if lib().orc_chi_w0(mesh.ncells, mesh.nlayers, _p(mesh.dofmap), _p(voc), _p(vlon), _p(vlat), kind,
+                        float(domain_top), float(scaled_radius), _p(chi[0]), _p(chi[1]), _p(chi[2])) != 0:
+        raise ValueError(kind)
+    return chi
 
 
 def dg_inc_apply(to_mesh, from_mesh, A, x, y0):

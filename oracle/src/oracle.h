@@ -12,6 +12,13 @@
  * Parity pins (what checks each restatement against something other than itself):
  *   orc_cubedsphere_topology  netCDF UGRID fixtures C4/C8/C12/C16/C24/C32 (face_nodes, face_links)
  *   orc_local_vert_lids       invariant: lid == first encounter; nverts == 6n^2+2
+ *   orc_cubedsphere_lonlat    node longitudes / latitudes of the reference's UGRID files C4, C12
+ *                             (degrees, to 1e-10); every vertex on the unit sphere at the
+ *                             equiangular grid's angles (a closed form per panel)
+ *   orc_eta                   extrusion_mod_test.pf values (uniform, quadratic, geometric);
+ *                             closed forms k/nl and (k/nl)^2
+ *   orc_chi_w0                |chi| = scaled_radius + eta(k) (top - surface); chi / |chi| = the
+ *                             vertex's unit vector (lon, lat)
  *   orc_mesh_connectivity     canned bi-periodic 3x3 mesh edge_on_cell / face_on_cell tables
  *   orc_dofmap                fs0k0 / fs1k0 / fs2k0 / fs3k0 dofmap fixtures; undf closed forms;
  *                             W0 order-1 entity sharing and column lengths
@@ -80,6 +87,11 @@ extern "C" {
 
 /* ---- mesh / topology ---------------------------------------------------------------- */
 int orc_cubedsphere_topology(int n, int *cell_next, int *verts_on_cell);
+int orc_cubedsphere_lonlat(int n, double *lon, double *lat);
+int orc_eta(int kind, int nl, double *eta);
+int orc_chi_w0(int ncells, int nl, const int *dofmap, const int *vert_on_cell, const double *vlon,
+               const double *vlat, int kind, double domain_top, double scaled_radius, double *chi1,
+               double *chi2, double *chi3);
 int orc_local_vert_lids(int ncells, const int *verts_on_cell_gid, int *vert_on_cell_lid);
 int orc_mesh_connectivity(int ncells, const int *cell_next, const int *vert_on_cell,
                           int *face_on_cell, int *edge_on_cell, int *nfaces_2d,

@@ -31,7 +31,12 @@ static int pec(int ec, int i, int dir, int panel) {
 /* calc_adjacency, calc_face_to_vert and orient_lfric
  * (mesh_tools/source/support/gencube_ps_mod.F90:336-546, 558-720, 1561-1601).
  * cell_next[c*4 + d] (W,S,E,N) and verts_on_cell[c*4 + v] (SW,SE,NE,NW) are 1-based ids. */
+static int topology_impl(int ec, int *cell_next, int *verts_on_cell, int orient);
 int orc_cubedsphere_topology(int ec, int *cell_next, int *verts_on_cell) {
+  return topology_impl(ec, cell_next, verts_on_cell, 1);
+}
+
+static int topology_impl(int ec, int *cell_next, int *verts_on_cell, int orient) {
   if (ec < 1) return -1;
   const int cpp = ec * ec, ncells = 6 * cpp;
   int *voc = verts_on_cell;
@@ -137,7 +142,7 @@ int orc_cubedsphere_topology(int ec, int *cell_next, int *verts_on_cell) {
   /* --- orient_lfric with PANEL_ROTATIONS = (0,0,1,1,-1,0) (:69, :1561-1601):
    * cshift(.,+1,dim=1) -> [a2,a3,a4,a1]; cshift(.,-1,dim=1) -> [a4,a1,a2,a3] --- */
   static const int rot[7] = {0, 0, 0, 1, 1, -1, 0};
-  for (int p = 1; p <= 6; ++p) {
+  for (int p = 1; p <= 6 && orient; ++p) {
     if (rot[p] == 0) continue;
     for (int c = (p - 1) * cpp + 1; c <= p * cpp; ++c) {
       int *arrs[2] = {&voc[(c - 1) * 4], &cell_next[(c - 1) * 4]};
@@ -578,4 +583,217 @@ int orc_colour_validate(int nc, const int *cell_next, const int *cm) {
   return 1;
 #undef NX
 #undef CM
+}
+
+/* ---- coordinates ------------------------------------------------------------------------ */
+#include <float.h>
+#include <math.h>
+
+/* xyz2ll_r_double (infrastructure/source/utilities/coord_transform_mod.F90:226-283) */
+static void xyz2ll(double x, double y, double z, double *longitude, double *latitude) {
+  const double pi = 4.0 * atan(1.0);
+  if (x == 0.0) {
+    *longitude = y >= 0.0 ? 0.5f * pi : -0.5f * pi;
+  } else {
+    *longitude = atan(y / x);
+    if (x < 0.0) {
+      if (y > 0.0)
+        *longitude = *longitude + pi;
+      else
+        *longitude = *longitude - pi;
+    }
+  }
+  const double radius = sqrt(x * x + y * y);
+  if (radius <= DBL_EPSILON) {
+    *latitude = z > 0.0 ? 0.5 * pi : -0.5 * pi;
+    *longitude = 0.0;
+  } else {
+    *latitude = atan(z / radius);
+  }
+}
+
+/* calc_coords (mesh_tools/source/support/gencube_ps_mod.F90:887-1113), called by generate
+ * (:1302-1348) after calc_face_to_vert and before orient_lfric: vert_coords(1,v) = longitude,
+ * vert_coords(2,v) = -latitude (radians), v the global vertex id (1-based; lon[v-1]). */
+int orc_cubedsphere_lonlat(int ec, double *lon, double *lat) {
+  if (ec < 1) return -1;
+  const int cpp = ec * ec, ncells = 6 * cpp;
+  int *cn = malloc(sizeof(int) * 4 * (size_t)ncells), *voc = malloc(sizeof(int) * 4 * (size_t)ncells);
+  if (!cn || !voc) {
+    free(cn);
+    free(voc);
+    return -1;
+  }
+  topology_impl(ec, cn, voc, 0);
+  const double pi = 4.0 * atan(1.0), pio4 = pi / 4.0;
+  const double dlambda = 0.5 * pi / ec;
+  double lo, la, xs, ys, zs, x0, y0, z0, t1, t2, lambda1, lambda2;
+#define SET(vid, X, Y, Z)            \
+  do {                               \
+    xyz2ll((X), (Y), (Z), &lo, &la); \
+    lon[(vid)-1] = lo;               \
+    lat[(vid)-1] = -la;              \
+  } while (0)
+#define CUBE()                                        \
+  do {                                                \
+    xs = 1.0 / sqrt(1.0 + t1 * t1 + t2 * t2);         \
+    ys = xs * t1;                                     \
+    zs = xs * t2;                                     \
+  } while (0)
+  int vert = 1;
+  for (int y = 1; y <= ec; ++y) { /* Panels I-IV */
+    lambda2 = (y - 1) * dlambda - pio4;
+    t2 = tan(lambda2);
+    for (int x = 1; x <= ec; ++x) {
+      lambda1 = (x - 1) * dlambda - pio4;
+      t1 = tan(lambda1);
+      CUBE();
+      SET(vert, xs, ys, zs);
+      x0 = -ys; y0 = xs; z0 = zs;
+      SET(vert + cpp, x0, y0, z0);
+      x0 = -xs; y0 = -ys; z0 = zs;
+      SET(vert + 2 * cpp, x0, y0, z0);
+      x0 = ys; y0 = -xs; /* z0 keeps Panel III's value */
+      SET(vert + 3 * cpp, x0, y0, z0);
+      vert++;
+    }
+  }
+  int cell = 4 * cpp; /* Panel V */
+  for (int y = 1; y <= ec - 1; ++y) {
+    lambda2 = y * dlambda - pio4;
+    t2 = tan(lambda2);
+    for (int x = 1; x <= ec - 1; ++x) {
+      lambda1 = x * dlambda - pio4;
+      t1 = tan(lambda1);
+      CUBE();
+      SET(VOC(SE, cell + x), -ys, zs, -xs);
+    }
+    cell += ec;
+  }
+  cell = 5 * cpp + 1; /* Panel VI */
+  for (int y = 1; y <= ec; ++y) {
+    lambda2 = (y - 1) * dlambda - pio4;
+    t2 = tan(lambda2);
+    for (int x = 1; x <= ec; ++x) {
+      lambda1 = (x - 1) * dlambda - pio4;
+      t1 = tan(lambda1);
+      CUBE();
+      SET(VOC(NW, cell), -ys, -zs, xs);
+      cell++;
+    }
+  }
+  cell = 6 * cpp - ec + 1; /* Panel VI: bottom edge */
+  lambda2 = ec * dlambda - pio4;
+  t2 = tan(lambda2);
+  for (int x = 1; x <= ec; ++x) {
+    lambda1 = (x - 1) * dlambda - pio4;
+    t1 = tan(lambda1);
+    CUBE();
+    SET(VOC(SW, cell), -ys, -zs, xs);
+    cell++;
+  }
+  cell = 5 * cpp + ec; /* Panel VI: right edge */
+  lambda1 = ec * dlambda - pio4;
+  t1 = tan(lambda1);
+  for (int y = 1; y <= ec; ++y) {
+    lambda2 = (y - 1) * dlambda - pio4;
+    t2 = tan(lambda2);
+    CUBE();
+    SET(VOC(NE, cell), -ys, -zs, xs);
+    cell += ec;
+  }
+  lambda1 = ec * dlambda - pio4;
+  t1 = tan(lambda1);
+  lambda2 = ec * dlambda - pio4;
+  t2 = tan(lambda2);
+  CUBE();
+  SET(6 * cpp + 2, -ys, -zs, xs);
+#undef SET
+#undef CUBE
+  free(cn);
+  free(voc);
+  return 0;
+}
+
+/* uniform_extrude / quadratic_extrude / geometric_extrude with geometric_func
+ * (infrastructure/source/mesh/extrusion_mod.f90:292-305, 341-354, 390-409, 618-629); kind 0 uniform,
+ * 1 quadratic, 2 geometric.  eta has nl+1 entries. */
+int orc_eta(int kind, int nl, double *eta) {
+  if (nl < 1) return -1;
+  if (kind == 0) {
+    for (int k = 0; k <= nl; ++k) eta[k] = (double)k / (double)nl;
+  } else if (kind == 1) {
+    for (int k = 0; k <= nl; ++k) {
+      const double q = (double)k / (double)nl;
+      eta[k] = q * q;
+    }
+  } else if (kind == 2) {
+    const double sf = 1.03;
+    double p = 1.0;
+    for (int i = 0; i < nl; ++i) p = p * sf; /* stretching_factor**number_of_layers */
+    const double delta_eta = (sf - 1.0) / (p - 1.0);
+    eta[0] = 0.0;
+    for (int k = 1; k <= nl; ++k) eta[k] = delta_eta * (pow(sf, (double)k) - 1.0) / (sf - 1.0);
+  } else {
+    return -1;
+  }
+  return 0;
+}
+
+/* Cartesian coordinate fields on lowest-order W0 (dofmap(8, ncells), 1-based columns), as
+ * assign_coordinate_field's xyz branch (components/driver/source/driver_coordinates_mod.F90:
+ * 357-454) leaves them for a mesh extruded by mesh_extruder (infrastructure/source/mesh/
+ * mesh_constructor_helper_functions_mod.F90:120-252) with dz from set_dz (:450-477) over a zero
+ * surface height: the cells are visited in order, layers k = 0..nl-1 and the 8 nodes in order,
+ * and every node is (re)written, so the last write stands.  vlon/vlat by the vertex id in
+ * vert_on_cell(4, ncells). */
+int orc_chi_w0(int ncells, int nl, const int *dofmap, const int *vert_on_cell, const double *vlon,
+               const double *vlat, int kind, double domain_top, double scaled_radius, double *chi1,
+               double *chi2, double *chi3) {
+  double *eta = malloc(sizeof(double) * (size_t)(nl + 1)), *dz = malloc(sizeof(double) * (size_t)nl);
+  if (!eta || !dz || orc_eta(kind, nl, eta)) {
+    free(eta);
+    free(dz);
+    return -1;
+  }
+  for (int k = 1; k <= nl; ++k) dz[k - 1] = (eta[k] - eta[k - 1]) * domain_top;
+  static const double hat[8][3] = {{0, 0, 0}, {1, 0, 0}, {1, 1, 0}, {0, 1, 0},
+                                   {0, 0, 1}, {1, 0, 1}, {1, 1, 1}, {0, 1, 1}};
+  for (int c = 0; c < ncells; ++c) {
+    for (int k = 0; k < nl; ++k) {
+      double vc[8][3];
+      for (int v = 0; v < 8; ++v) { /* column_coords: the cell's vertices of layer k */
+        const int vid = vert_on_cell[c * 4 + (v % 4)] - 1;
+        const int lev = k + v / 4;
+        double s = 0.0;
+        for (int j = 1; j <= lev; ++j) s = s + dz[j - 1];
+        const double r = 1.0 + s;
+        const double sl = sin(vlon[vid]), cl = cos(vlon[vid]), sa = sin(vlat[vid]), ca = cos(vlat[vid]);
+        vc[v][0] = r * cl * ca;
+        vc[v][1] = r * sl * ca;
+        vc[v][2] = r * sa;
+      }
+      for (int df = 0; df < 8; ++df) {
+        double x = 0.0, y = 0.0, z = 0.0;
+        for (int v = 0; v < 8; ++v) {
+          const double w = (1.0 - fabs(hat[v][0] - hat[df][0])) * (1.0 - fabs(hat[v][1] - hat[df][1])) *
+                           (1.0 - fabs(hat[v][2] - hat[df][2]));
+          x = x + w * vc[v][0];
+          y = y + w * vc[v][1];
+          z = z + w * vc[v][2];
+        }
+        double s = 0.0;
+        for (int j = 1; j <= k; ++j) s = s + dz[j - 1];
+        double rc = scaled_radius + s + hat[df][2] * dz[k];
+        rc = rc / sqrt(x * x + y * y + z * z);
+        const int dfk = dofmap[c * 8 + df] - 1 + k;
+        chi1[dfk] = x * rc;
+        chi2[dfk] = y * rc;
+        chi3[dfk] = z * rc;
+      }
+    }
+  }
+  free(eta);
+  free(dz);
+  return 0;
 }

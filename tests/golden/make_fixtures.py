@@ -10,6 +10,8 @@ Everything written is data (inputs and expected outputs); no reference source te
                       applications/coupled/example/mesh_C12.nc,
                       applications/io_demo/example/mesh_C24.nc), read with scipy's netCDF3
                       reader; the C16/C24/C32 arrays are stored as SHA-256 digests only.
+* node_coords.npz     node longitudes / latitudes (degrees) of the unrotated unit-test cubed spheres
+                      infrastructure/unit-test/data/mesh_C{4,8,16,32}.nc (north pole at 90 N).
 * ugrid/*.nc          copies of four of the reference's UGRID mesh files (C4, C12 cubed spheres,
                       a bi-periodic 8x8 planar mesh, a bounded planar mesh): data the reader parses.
 * canned_mesh.json    the bi-periodic 3x3x3 unit-test mesh of
@@ -20,7 +22,9 @@ Everything written is data (inputs and expected outputs); no reference source te
                       fs3k0_dofmap :1824-1826).
 * kat.json            the matrix_vector known-answer test
                       (components/science/unit-test/kernel/algebra/matrix_vector_mod_test.pf:57-89),
-                      the transpose / dg / mm_diagonal and invert_local_operator kernel tests,
+                      the transpose / dg / dg_inc / mm_diagonal and invert_local_operator kernel
+                      tests, the uniform / quadratic / geometric extrusions
+                      (infrastructure/unit-test/mesh/extrusion_mod_test.pf),
                       the matrix_invert_lu test
                       (infrastructure/unit-test/utilities/matrix_invert_mod_test.pf:25,39-46) and
                       the W0, W2 and W3 mass-matrix and weak-divergence tests
@@ -105,6 +109,14 @@ def main(ref):
             arrays[f"C{n}_face_nodes"] = fn
             arrays[f"C{n}_face_links"] = fl
     np.savez_compressed(HERE / "mesh_fixtures.npz", **arrays)
+    coords = {}
+    for n in (4, 8, 16, 32):
+        nc = netcdf_file(ref / MESHES[n], "r", mmap=False)
+        pre = [k for k in nc.variables if k.endswith("_node_x")][0][: -len("_node_x")]
+        coords[f"C{n}_node_x"] = np.array(nc.variables[pre + "_node_x"][:], dtype=np.float64)
+        coords[f"C{n}_node_y"] = np.array(nc.variables[pre + "_node_y"][:], dtype=np.float64)
+        nc.close()
+    np.savez_compressed(HERE / "node_coords.npz", **coords)
     # a few of the reference's own UGRID data files, byte for byte, for the mesh reader's tests
     (HERE / "ugrid").mkdir(exist_ok=True)
     for rel in UGRID_FILES:
@@ -204,6 +216,13 @@ def main(ref):
         "dg_matrix_vector": {
             "d": d.tolist(), "x": x.tolist(), "answer": (d @ x).tolist(),
             "map1": [1, 2, 3, 4], "map2": [1, 2], "tol": 1e-14,
+        },
+        # extrusion_mod_test.pf:181-210 (uniform), 250-271 (quadratic), 306-328 (geometric): eta(0:5)
+        "extrusion": {
+            "layers": 5, "tol": 1e-3,
+            "uniform": [0.0, 0.2, 0.4, 0.6, 0.8, 1.0],
+            "quadratic": [0.0, 0.04, 0.16, 0.36, 0.64, 1.0],
+            "geometric": [0.0, 0.188, 0.382, 0.582, 0.788, 1.0],
         },
         # dg_inc_matrix_vector_mod_test.pf:44-85: y(4) = 0 + D(4x2) x(2), answer = matmul
         "dg_inc_matrix_vector": {

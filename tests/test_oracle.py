@@ -840,3 +840,58 @@ def test_r_solver_cg_converges_like_fp64_cg(orc):
     assert np.linalg.norm(x32 - ref) <= 1e-4 * np.linalg.norm(ref)
     _, res64, _ = orc.cg(m, A32, b, diag=diag, r_tol=1e-5, a_tol=0.0)
     assert abs(it32 - res64.iterations) <= 1
+
+
+def test_extrusion_eta_pinned(orc):
+    k = json.loads((GOLDEN / "kat.json").read_text())["extrusion"]
+    for kind, name in ((0, "uniform"), (1, "quadratic"), (2, "geometric")):
+        np.testing.assert_allclose(orc.eta(kind, k["layers"]), k[name], rtol=0, atol=k["tol"])
+    for nl in (1, 5, 70):
+        kk = np.arange(nl + 1) / nl
+        assert np.array_equal(orc.eta(0, nl), kk)
+        assert np.array_equal(orc.eta(1, nl), kk * kk)
+        g = orc.eta(2, nl)
+        assert g[0] == 0.0 and abs(g[-1] - 1.0) < 1e-12 and np.all(np.diff(g) > 0)
+        # geometric layers grow by the stretching factor 1.03
+        np.testing.assert_allclose(np.diff(g)[1:] / np.diff(g)[:-1], 1.03, rtol=1e-9)
+
+
+def test_cubedsphere_node_coordinates_pinned_to_mesh_files(orc):
+    """The mesh generator's vertex longitudes and latitudes against the reference's own unrotated
+    unit-test meshes (degrees as the files hold them), and on the unit sphere at the equiangular
+    grid angles."""
+    ref = np.load(GOLDEN / "node_coords.npz")
+    for n in (4, 8, 16, 32):
+        lon, lat = orc.cubedsphere_lonlat(n)
+        dx = (np.degrees(lon) - ref[f"C{n}_node_x"] + 180.0) % 360.0 - 180.0
+        assert np.abs(dx).max() <= 1e-10, n
+        assert np.abs(np.degrees(lat) - ref[f"C{n}_node_y"]).max() <= 1e-10, n
+    # panel I's vertices sit at (1, tan a, tan b) / |.| on the grid a, b = -pi/4 + j pi/(2n)
+    n = 6
+    lon, lat = orc.cubedsphere_lonlat(n)
+    xyz = np.stack([np.cos(lon) * np.cos(-lat), np.sin(lon) * np.cos(-lat), np.sin(-lat)], 1)
+    a = -np.pi / 4 + np.arange(n) * np.pi / (2 * n)
+    t1, t2 = np.meshgrid(np.tan(a), np.tan(a))
+    p1 = np.stack([np.ones_like(t1), t1, t2], -1).reshape(-1, 3)
+    p1 /= np.linalg.norm(p1, axis=1)[:, None]
+    np.testing.assert_allclose(xyz[: n * n], p1, atol=1e-14)
+    # a planted mistake, latitude not negated, moves every off-equator vertex
+    assert np.abs(np.sin(lat[: n * n]) + p1[:, 2]).max() < 1e-14
+
+
+def test_coordinate_fields_on_shells(orc):
+    m = orc.Mesh(5, 6)
+    top, R = 80000.0, 6371229.0
+    for kind in (0, 1, 2):
+        chi = orc.chi_w0(m, kind, top, R)
+        r = np.sqrt(chi[0] ** 2 + chi[1] ** 2 + chi[2] ** 2)
+        e = orc.eta(kind, m.nlayers)
+        lon, lat = orc.cubedsphere_lonlat(5)
+        for c in (0, 17, 149):
+            for i in range(4):
+                base = m.dofmap[c, i] - 1
+                np.testing.assert_allclose(r[base:base + m.nlayers + 1], R + e * top, rtol=1e-14)
+                g = m.vert_gid[c, i] - 1
+                u = np.array([np.cos(lon[g]) * np.cos(lat[g]), np.sin(lon[g]) * np.cos(lat[g]), np.sin(lat[g])])
+                np.testing.assert_allclose(np.stack(chi, 1)[base:base + m.nlayers + 1] / r[base:base + m.nlayers + 1, None],
+                                           np.tile(u, (m.nlayers + 1, 1)), atol=1e-14)
