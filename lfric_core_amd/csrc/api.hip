@@ -322,19 +322,28 @@ static int cg_device(const CgArgs &a, SolverState &out, unsigned char *scratch_b
     }
     pz_partials = static_cast<double *>(w);
   }
+  // the p.(A p) partials: the fused apply's, or those of a separate dot (first kRedBlocks
+  // partials of the context's buffer); the update adds them itself when there are few
+  const double *pzp = fused ? pz_partials : c.d_partials;
+  const int npz = fused ? pz_blocks : kRedBlocks;
+  const bool fold = npz <= kFoldMaxPartials;
+  long long it = 0;  // iterations enqueued: their parity picks the rz slot
   for (int ch = 0; ch < max_chunks; ++ch) {
-    for (int i = 0; i < every; ++i) {
+    for (int i = 0; i < every; ++i, ++it) {
+      const int par = static_cast<int>(it & 1);
       if (fused) {
         if (int rc = launch_lma_apply_gather_dot(a.A, p, z, pz_partials, &g_state->done)) return rc;
-        if (int rc = launch_cg_finish_pz(pz_partials, pz_blocks, g_state)) return rc;
       } else {
         if (int rc = launch_lma_apply_solver(a.A, p, z, &g_state->done)) return rc;
         if (a.bc)
           if (int rc = launch_enforce_bc(*a.A->to, a.prec, z, &g_state->done)) return rc;
-        if (int rc = launch_cg_dot_pz(a.prec, p, z, a.n, g_state)) return rc;
+        if (int rc = launch_cg_dot_pz(a.prec, p, z, a.n, g_state, c.d_partials)) return rc;
       }
-      if (int rc = launch_cg_update(a.prec, a.x, r, z, p, a.diag, a.n, g_state)) return rc;
-      if (int rc = launch_cg_aypx(a.prec, p, z, a.n, g_state)) return rc;
+      if (!fold)
+        if (int rc = launch_cg_finish_pz(pzp, npz, g_state, par)) return rc;
+      if (int rc = launch_cg_update(a.prec, a.x, r, z, p, a.diag, a.n, g_state, fold ? pzp : nullptr, npz, par))
+        return rc;
+      if (int rc = launch_cg_aypx(a.prec, p, z, a.n, g_state, par)) return rc;
     }
     LF_HIP(hipMemcpyAsync(g_state_host, g_state, sizeof(SolverState), hipMemcpyDeviceToHost, c.stream));
     LF_HIP(hipStreamSynchronize(c.stream));
@@ -1199,7 +1208,7 @@ int lfric_cg_solve_noalloc(const lfric_op *A, const lfric_prec *P, lfric_vec *x,
   const bool listed = A->to->d_gcol != nullptr;
   size_t off[4];
   const size_t need = cg_scratch_layout(esize(A->precision) * static_cast<size_t>(x->n),
-                                        fused && listed ? gather_blocks(A) : 1, off);
+                                        fused && listed ? gather_dot_blocks(A) : 1, off);
   LF_CHECK(need <= kScratchBytes, LFRIC_E_UNSUPPORTED,
            "cg_solve_noalloc: the problem needs " + std::to_string(need) + " bytes of scratch, more than its " +
                std::to_string(kScratchBytes) + "-byte maximum (3 vectors of undf values plus the block partials)");

@@ -14,6 +14,7 @@ Roles roles_of(const FunctionSpace &fs);
 // device solver state (FP64 scalars stay on the GPU between kernels)
 struct SolverState {
   double rz, alpha, beta, r_nrm0, r_nrm, r_tol, a_tol, pz;
+  double rzb[2];      // rz of even / odd iterations (read and written by different kernels' blocks)
   double red0, red1;  // last reduction results
   int iter, max_iter, done, status, monitor, converged;
   unsigned int counter;
@@ -41,8 +42,10 @@ int launch_lma_apply_gather(const lfric_op *op, const void *x, void *y);
 // operators); stops at once when *skip is set
 int launch_lma_apply_gather_dot(const lfric_op *op, const void *x, void *y, double *partials, const int *skip);
 int launch_gather_blocks(const lfric_op *op, int *nblocks);
-// blocks of the gather apply's grid, once the space's column lists exist
+// blocks of the gather apply's grid, once the space's column lists exist, and the number of
+// p.(A p) partials its CG form leaves
 int gather_blocks(const lfric_op *op);
+int gather_dot_blocks(const lfric_op *op);
 int launch_build_rows(lfric_op *op);
 // the gather apply with `mark` recorded between the two phases of the symmetric-packed form
 // (two_phase tells whether it was)
@@ -87,11 +90,18 @@ int launch_cg_residual(int prec, void *r, const void *b, long long n);  // r = b
 int launch_cg_init_norm(int prec, const void *r, long long n, SolverState *st);
 int launch_cg_init_prec(int prec, const void *r, void *z, void *p, const void *diag, long long n,
                         SolverState *st);
-int launch_cg_dot_pz(int prec, const void *p, const void *z, long long n, SolverState *st);
-int launch_cg_update(int prec, void *x, void *r, void *z, const void *p, const void *diag,
-                     long long n, SolverState *st);
-int launch_cg_aypx(int prec, void *p, const void *z, long long n, SolverState *st);
-int launch_cg_finish_pz(const double *partials, int nblocks, SolverState *st);
+// p.(A p) partials (kRedBlocks) into `partials`
+int launch_cg_dot_pz(int prec, const void *p, const void *z, long long n, SolverState *st, double *partials);
+// x += alpha p, r -= alpha A p, z = P^-1 r and the (r.r, r.z) partials.  With pz_partials the
+// kernel forms alpha from those npz partials itself; otherwise it reads alpha from st.  par:
+// the parity of the iteration (its rz slot)
+int launch_cg_update(int prec, void *x, void *r, void *z, const void *p, const void *diag, long long n,
+                     SolverState *st, const double *pz_partials, int npz, int par);
+// convergence test, beta and p = z + beta p from the update's partials
+int launch_cg_aypx(int prec, void *p, const void *z, long long n, SolverState *st, int par);
+int launch_cg_finish_pz(const double *partials, int nblocks, SolverState *st, int par);
+// partials the folded update adds up itself, at most
+constexpr int kFoldMaxPartials = 4 * kRedBlocks;
 
 // CG on an LMA linear operator, with the mass-matrix operator's boundary conditions when it
 // applies them (api.hip)

@@ -1079,19 +1079,16 @@ __global__ __launch_bounds__(kBlock) void k_sym_units1(const T *__restrict__ sym
 
 // phase 2: y at (column, level) = the units' outputs of the column's cells in colour order, the
 // first written, the rest added (the colour kernel's first-touch write and += order).  With
-// `zu` (columns of at most 4 cells) the column's units come as one int4 (pos*4 + i, or -1 when
+// `zu` (columns of at most 4 cells) the column's units come as one int4 (cell*4 + i, or -1 when
 // the column has fewer cells) beside its base dof, so a thread waits for two rounds of loads;
-// otherwise its list entries are loaded before its z values (three rounds).
-template <typename T, bool DOT>
-__global__ __launch_bounds__(kBlock) void k_sym_gather(const T *__restrict__ z, const T *__restrict__ x,
-                                                        T *__restrict__ y, const int2 *__restrict__ cols,
-                                                        const int *__restrict__ off, const int *__restrict__ ent,
-                                                        const int4 *__restrict__ zu, int ncols, int nl,
-                                                        double *partials, const int *skip, int xwin) {
-  if (skip && *skip) return;
+// otherwise its list entries are loaded before its z values (three rounds).  Returns x.y for
+// the point (0 when t is past the end).
+template <typename T>
+__device__ inline double sym_gather_point(long long t, const T *__restrict__ z, const T *__restrict__ x,
+                                          T *__restrict__ y, const int2 *__restrict__ cols,
+                                          const int *__restrict__ off, const int *__restrict__ ent,
+                                          const int4 *__restrict__ zu, int ncols, int nl) {
   const int H = nl + 1;
-  const int lb = xcd_block(blockIdx.x, gridDim.x, xwin);
-  const long long t = static_cast<long long>(lb) * kBlock + threadIdx.x;
   bool valid = t < static_cast<long long>(ncols) * H;
   const int col = valid ? static_cast<int>(t / H) : 0;
   const int m = valid ? static_cast<int>(t - static_cast<long long>(col) * H) : 0;
@@ -1100,8 +1097,9 @@ __global__ __launch_bounds__(kBlock) void k_sym_gather(const T *__restrict__ z, 
     ci = cols[col];
     if (m == nl && !ci.y) valid = false;
   }
+  if (!valid) return 0.0;
   T acc = T(0);
-  if (valid && zu) {
+  if (zu) {
     const int4 u = zu[col];
     const int us[4] = {u.x, u.y, u.z, u.w};
     T v[4];
@@ -1111,8 +1109,7 @@ __global__ __launch_bounds__(kBlock) void k_sym_gather(const T *__restrict__ z, 
 #pragma unroll
     for (int q = 1; q < 4; ++q)
       if (us[q] >= 0) acc = acc + v[q];
-    y[static_cast<size_t>(ci.x) + m] = acc;
-  } else if (valid) {
+  } else {
     const int q0 = off[col], cnt = off[col + 1] - q0;
     constexpr int KU = 4;
     int en[KU];
@@ -1130,12 +1127,39 @@ __global__ __launch_bounds__(kBlock) void k_sym_gather(const T *__restrict__ z, 
       const int e2 = ent[q0 + q];
       acc = acc + z[(static_cast<size_t>(e2 >> 5) * 4 + (e2 & 31)) * H + m];
     }
-    y[static_cast<size_t>(ci.x) + m] = acc;
   }
-  if constexpr (DOT) {
-    double d[1] = {valid ? to_d(acc) * to_d(x[static_cast<size_t>(ci.x) + m]) : 0.0};
-    block_partials<1>(d, partials, lb);
-  }
+  y[static_cast<size_t>(ci.x) + m] = acc;
+  return to_d(acc) * to_d(x[static_cast<size_t>(ci.x) + m]);
+}
+
+// one point per thread
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_sym_gather(const T *__restrict__ z, const T *__restrict__ x,
+                                                        T *__restrict__ y, const int2 *__restrict__ cols,
+                                                        const int *__restrict__ off, const int *__restrict__ ent,
+                                                        const int4 *__restrict__ zu, int ncols, int nl, const int *skip,
+                                                        int xwin) {
+  if (skip && *skip) return;
+  const long long t = static_cast<long long>(xcd_block(blockIdx.x, gridDim.x, xwin)) * kBlock + threadIdx.x;
+  (void)sym_gather_point<T>(t, z, x, y, cols, off, ent, zu, ncols, nl);
+}
+
+// with the x.y partial sums of the CG's p.(A p): a fixed grid of kRedBlocks blocks walks the
+// points (grid stride), so the next kernel can add the kRedBlocks partials itself
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_sym_gather_dot(const T *__restrict__ z, const T *__restrict__ x,
+                                                            T *__restrict__ y, const int2 *__restrict__ cols,
+                                                            const int *__restrict__ off, const int *__restrict__ ent,
+                                                            const int4 *__restrict__ zu, int ncols, int nl,
+                                                            double *partials, const int *skip, int xwin) {
+  if (skip && *skip) return;
+  const int lb = xcd_block(blockIdx.x, gridDim.x, xwin);
+  const long long total = static_cast<long long>(ncols) * (nl + 1);
+  const long long stride = static_cast<long long>(gridDim.x) * kBlock;
+  double d[1] = {0.0};
+  for (long long t = static_cast<long long>(lb) * kBlock + threadIdx.x; t < total; t += stride)
+    d[0] += sym_gather_point<T>(t, z, x, y, cols, off, ent, zu, ncols, nl);
+  block_partials<1>(d, partials, lb);
 }
 
 // Several right-hand sides over the row-gathered copy: each RHS accumulates exactly as
@@ -1510,16 +1534,14 @@ __global__ __launch_bounds__(kBlock) void k_cg_update(T *__restrict__ x, T *__re
 
 // Finishing step of every reduction: sum the block partials in block order, then update the
 // solver scalars (cg_solve's scalar arithmetic, sci_iterative_solver_mod.f90:521-575).
-enum Epilogue { kEpiDot = 0, kEpiInitNorm, kEpiInitPrec, kEpiDotPz, kEpiUpdate };
+enum Epilogue { kEpiDot = 0, kEpiInitNorm, kEpiInitPrec, kEpiDotPz };
 
-template <int NV, int EPI>
-__global__ __launch_bounds__(kBlock) void k_finish(const double *__restrict__ partials, int nblocks, SolverState *st,
-                                                   double *out) {
-  if (EPI == kEpiInitPrec || EPI == kEpiDotPz || EPI == kEpiUpdate) {
-    if (st->done) return;
-  }
-  // thread t adds partials t, t + kBlock, ... in kChains independent chains (chain u takes
-  // every kChains-th of them), so the loads overlap; the chains are then added in order
+// The sum of a reduction's block partials, the same in every block that forms it: thread t
+// adds partials t, t + kBlock, ... in kChains independent chains (chain u takes every
+// kChains-th of them), so the loads overlap; the chains are added in order, then the block's
+// threads in block_sum's fixed tree.  Every thread of the block gets the total.
+template <int NV>
+__device__ inline void reduce_partials(const double *__restrict__ partials, int nblocks, double (&tot)[NV]) {
   constexpr int kChains = 8;
   double c[kChains][NV];
 #pragma unroll
@@ -1544,8 +1566,17 @@ __global__ __launch_bounds__(kBlock) void k_finish(const double *__restrict__ pa
 #pragma unroll
     for (int u = 1; u < kChains; ++u) s[q] += c[u][q];
   }
-  double tot[NV];
   block_sum<NV>(s, tot);
+}
+
+template <int NV, int EPI>
+__global__ __launch_bounds__(kBlock) void k_finish(const double *__restrict__ partials, int nblocks, SolverState *st,
+                                                   double *out, int par) {
+  if (EPI == kEpiInitPrec || EPI == kEpiDotPz) {
+    if (st->done) return;
+  }
+  double tot[NV];
+  reduce_partials<NV>(partials, nblocks, tot);
   if (threadIdx.x != 0) return;
   if (EPI == kEpiDot) {
     *out = tot[0];
@@ -1564,6 +1595,7 @@ __global__ __launch_bounds__(kBlock) void k_finish(const double *__restrict__ pa
     }
   } else if (EPI == kEpiInitPrec) {
     st->rz = tot[0];
+    st->rzb[0] = tot[0];
   } else if (EPI == kEpiDotPz) {
     const double pz = tot[0];
     st->pz = pz;
@@ -1574,27 +1606,115 @@ __global__ __launch_bounds__(kBlock) void k_finish(const double *__restrict__ pa
       st->status = LFRIC_BREAKDOWN;
       st->done = 1;
     } else {
-      st->alpha = st->rz / pz;  // alpha = <r,z> / <p,A.p>
+      st->alpha = st->rzb[par] / pz;  // alpha = <r,z> / <p,A.p>
     }
-  } else if (EPI == kEpiUpdate) {
-    st->iter += 1;
-    const double rn = sqrt(tot[0]);
-    st->r_nrm = rn;
-    st->red0 = tot[0];
-    st->red1 = tot[NV - 1];
-    if (rn != rn) {
+  }
+}
+
+// CG's update with the p.(A p) finish folded in: every block adds the npz partials of the apply
+// itself (reduce_partials: the same total in every block) and forms alpha = rz / pz; block 0
+// records it (and a NaN or zero pz, which stops the solve in every block alike).  par selects
+// the iteration's rz slot (the host counts the iterations it enqueues).
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_cg_update_fold(T *__restrict__ x, T *__restrict__ r, T *__restrict__ z,
+                                                           const T *__restrict__ p, const T *__restrict__ diag,
+                                                           long long n, SolverState *st,
+                                                           const double *__restrict__ pzp, int npz,
+                                                           double *partials, int par) {
+  if (st->done) return;
+  double tp[1];
+  reduce_partials<1>(pzp, npz, tp);
+  const double pz = tp[0];
+  if (pz != pz || pz == 0.0) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      st->pz = pz;
+      st->status = pz != pz ? LFRIC_NAN : LFRIC_BREAKDOWN;
+      st->done = 1;
+    }
+    return;
+  }
+  const double alpha = st->rzb[par] / pz;  // alpha = <r,z> / <p,A.p>
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    st->pz = pz;
+    st->alpha = alpha;
+  }
+  const T a = static_cast<T>(alpha);
+  const T ma = static_cast<T>(-alpha);
+  double v[2] = {0.0, 0.0};
+  UNROLLED_STRIDE(n) {
+    T xv[4], pv[4], rv[4], zv[4], dv[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const long long i = i0_ + u * s_;
+      const bool ok = i < n;
+      xv[u] = ok ? x[i] : T(0);
+      pv[u] = ok ? p[i] : T(0);
+      rv[u] = ok ? r[i] : T(0);
+      zv[u] = ok ? z[i] : T(0);
+      dv[u] = (ok && diag) ? diag[i] : T(1);
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const long long i = i0_ + u * s_;
+      if (i >= n) continue;
+      x[i] = xv[u] + a * pv[u];              // x%axpy(alpha,p)
+      const T ri = rv[u] + ma * zv[u];       // r%axpy(-alpha,z)
+      r[i] = ri;
+      const T zi = diag ? ri / dv[u] : ri;   // prec%apply(r,z)
+      z[i] = zi;
+      v[0] += to_d(ri) * to_d(ri);
+      v[1] += to_d(ri) * to_d(zi);
+    }
+  }
+  block_partials<2>(v, partials);
+}
+
+// CG's aypx with the update's finish folded in: every block adds the update's partials (r.r,
+// r.z), tests convergence as cg_solve does (monitor on: ||r|| / ||r0|| <= r_tol or ||r|| <=
+// a_tol) and, when the solve goes on, forms beta = rz_new / rz and p = beta p + z.  Block 0
+// keeps the scalars: the iteration count, ||r||, the next rz slot, the done flag.  A block that
+// starts after block 0 has set done skips p, which the finished solve no longer reads.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_cg_aypx_fold(T *__restrict__ p, const T *__restrict__ z, long long n,
+                                                         SolverState *st, const double *__restrict__ partials,
+                                                         int nblocks, int par) {
+  if (st->done) return;
+  double tot[2];
+  reduce_partials<2>(partials, nblocks, tot);
+  const double rn = sqrt(tot[0]);
+  const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+  if (rn != rn) {
+    if (lead) {
+      st->iter += 1;
+      st->r_nrm = rn;
       st->status = LFRIC_NAN;
       st->done = 1;
-      return;
     }
-    if (st->monitor && (rn / st->r_nrm0 <= st->r_tol || rn <= st->a_tol)) {
+    return;
+  }
+  if (st->monitor && (rn / st->r_nrm0 <= st->r_tol || rn <= st->a_tol)) {
+    if (lead) {
+      st->iter += 1;
+      st->r_nrm = rn;
+      st->red0 = tot[0];
+      st->red1 = tot[1];
       st->converged = 1;
       st->done = 1;
-      return;
     }
-    const double rz_new = tot[NV - 1];
-    st->beta = rz_new / st->rz;
+    return;
+  }
+  const double rz_new = tot[1];
+  const double beta = rz_new / st->rzb[par];
+  const T b = static_cast<T>(beta);
+  GRID_STRIDE(i, n) p[i] = b * p[i] + z[i];  // p%aypx(beta,z)
+  if (lead) {
+    st->iter += 1;
+    st->r_nrm = rn;
+    st->red0 = tot[0];
+    st->red1 = tot[1];
+    st->beta = beta;
     st->rz = rz_new;
+    st->rzb[par ^ 1] = rz_new;
     if (st->iter >= st->max_iter) st->done = 1;
   }
 }
@@ -2192,6 +2312,11 @@ int gather_blocks(const lfric_op *op) {
   return grid_for(static_cast<long long>(to.gather_ncols) * (to.nlayers + 1));
 }
 
+int gather_dot_blocks(const lfric_op *op) {
+  const bool sym = op->sym && op->to == op->from && role_kind(*op->to) == RoleKind::W0K0;
+  return sym ? kRedBlocks : gather_blocks(op);
+}
+
 // set by lfric_op_time_apply: an event recorded between the two phases of the symmetric-packed
 // apply, so that each phase can be timed on its own
 static hipEvent_t g_phase_mark = nullptr;
@@ -2233,11 +2358,11 @@ static void launch_gather_r(const lfric_op *op, const T *x, T *y, R roles, Gathe
                op->gss, x, from.d_map, static_cast<T *>(op->symz), to.ncells, to.nlayers, op->sym_seg, nseg, ex.skip);
       if (g_phase_mark) (void)hipEventRecord(g_phase_mark, ctx().stream);
       if (ex.partials)
-        LAUNCH((k_sym_gather<T, true>), grid, static_cast<const T *>(op->symz), x, y, to.d_gcol, to.d_goff, to.d_gent,
-               op->d_symzu, to.gather_ncols, to.nlayers, ex.partials, ex.skip, op->xwin);
-      else
-        LAUNCH((k_sym_gather<T, false>), grid, static_cast<const T *>(op->symz), x, y, to.d_gcol, to.d_goff,
+        LAUNCH((k_sym_gather_dot<T>), kRedBlocks, static_cast<const T *>(op->symz), x, y, to.d_gcol, to.d_goff,
                to.d_gent, op->d_symzu, to.gather_ncols, to.nlayers, ex.partials, ex.skip, op->xwin);
+      else
+        LAUNCH((k_sym_gather<T>), grid, static_cast<const T *>(op->symz), x, y, to.d_gcol, to.d_goff, to.d_gent,
+               op->d_symzu, to.gather_ncols, to.nlayers, ex.skip, op->xwin);
       return;
     }
     if (op->rows) {  // built only for square lowest-order W0 operators (launch_build_rows)
@@ -2606,7 +2731,7 @@ int launch_lma_apply_solver(const lfric_op *op, const void *x, void *y, const in
 
 int launch_gather_blocks(const lfric_op *op, int *nblocks) {
   if (int rc = build_gather(*op->to)) return rc;
-  *nblocks = gather_blocks(op);
+  *nblocks = gather_dot_blocks(op);
   return LFRIC_OK;
 }
 
@@ -2859,7 +2984,7 @@ int launch_dot(int prec, const void *x, const void *y, long long n, double *d_ou
            c.d_counter, d_out);
   else
     return fail(LFRIC_E_UNSUPPORTED, "vector precision");
-  LAUNCH((k_finish<1, kEpiDot>), 1, c.d_partials, kRedBlocks, static_cast<SolverState *>(nullptr), d_out);
+  LAUNCH((k_finish<1, kEpiDot>), 1, c.d_partials, kRedBlocks, static_cast<SolverState *>(nullptr), d_out, 0);
   CHECK_LAUNCH();
   return LFRIC_OK;
 }
@@ -2879,7 +3004,7 @@ int launch_cg_init_norm(int prec, const void *r, long long n, SolverState *st) {
     LAUNCH(k_cg_init_norm<double>, kRedBlocks, static_cast<const double *>(r), n, st, ctx().d_partials);
   else
     LAUNCH(k_cg_init_norm<float>, kRedBlocks, static_cast<const float *>(r), n, st, ctx().d_partials);
-  LAUNCH((k_finish<1, kEpiInitNorm>), 1, ctx().d_partials, kRedBlocks, st, static_cast<double *>(nullptr));
+  LAUNCH((k_finish<1, kEpiInitNorm>), 1, ctx().d_partials, kRedBlocks, st, static_cast<double *>(nullptr), 0);
   CHECK_LAUNCH();
   return LFRIC_OK;
 }
@@ -2891,48 +3016,65 @@ int launch_cg_init_prec(int prec, const void *r, void *z, void *p, const void *d
   else
     LAUNCH(k_cg_init_prec<float>, kRedBlocks, static_cast<const float *>(r), static_cast<float *>(z),
            static_cast<float *>(p), static_cast<const float *>(diag), n, st, ctx().d_partials);
-  LAUNCH((k_finish<1, kEpiInitPrec>), 1, ctx().d_partials, kRedBlocks, st, static_cast<double *>(nullptr));
+  LAUNCH((k_finish<1, kEpiInitPrec>), 1, ctx().d_partials, kRedBlocks, st, static_cast<double *>(nullptr), 0);
   CHECK_LAUNCH();
   return LFRIC_OK;
 }
 
-int launch_cg_dot_pz(int prec, const void *p, const void *z, long long n, SolverState *st) {
+// partial sums of p.(A p), one per block, into partials (kRedBlocks of them)
+int launch_cg_dot_pz(int prec, const void *p, const void *z, long long n, SolverState *st, double *partials) {
   if (prec == LFRIC_R64)
     LAUNCH(k_cg_dot_pz<double>, kRedBlocks, static_cast<const double *>(p), static_cast<const double *>(z), n, st,
-           ctx().d_partials);
+           partials);
   else
     LAUNCH(k_cg_dot_pz<float>, kRedBlocks, static_cast<const float *>(p), static_cast<const float *>(z), n, st,
-           ctx().d_partials);
-  LAUNCH((k_finish<1, kEpiDotPz>), 1, ctx().d_partials, kRedBlocks, st, static_cast<double *>(nullptr));
+           partials);
   CHECK_LAUNCH();
   return LFRIC_OK;
 }
 
-// alpha from partial sums of p.(A p) left by the fused gather apply
-int launch_cg_finish_pz(const double *partials, int nblocks, SolverState *st) {
-  LAUNCH((k_finish<1, kEpiDotPz>), 1, partials, nblocks, st, static_cast<double *>(nullptr));
+// alpha from partial sums of p.(A p) (one-block finish, for applies with more partials than
+// the folded update adds up itself)
+int launch_cg_finish_pz(const double *partials, int nblocks, SolverState *st, int par) {
+  LAUNCH((k_finish<1, kEpiDotPz>), 1, partials, nblocks, st, static_cast<double *>(nullptr), par);
   CHECK_LAUNCH();
   return LFRIC_OK;
 }
+
+// the update's (r.r, r.z) partials: past the first kRedBlocks partials, which hold p.(A p)
+static double *update_partials() { return ctx().d_partials + 2 * kRedBlocks; }
 
 int launch_cg_update(int prec, void *x, void *r, void *z, const void *p, const void *diag, long long n,
-                     SolverState *st) {
-  if (prec == LFRIC_R64)
+                     SolverState *st, const double *pz_partials, int npz, int par) {
+  double *out = update_partials();
+  if (pz_partials) {
+    if (prec == LFRIC_R64)
+      LAUNCH(k_cg_update_fold<double>, kRedBlocks, static_cast<double *>(x), static_cast<double *>(r),
+             static_cast<double *>(z), static_cast<const double *>(p), static_cast<const double *>(diag), n, st,
+             pz_partials, npz, out, par);
+    else
+      LAUNCH(k_cg_update_fold<float>, kRedBlocks, static_cast<float *>(x), static_cast<float *>(r),
+             static_cast<float *>(z), static_cast<const float *>(p), static_cast<const float *>(diag), n, st,
+             pz_partials, npz, out, par);
+  } else if (prec == LFRIC_R64) {
     LAUNCH(k_cg_update<double>, kRedBlocks, static_cast<double *>(x), static_cast<double *>(r), static_cast<double *>(z),
-           static_cast<const double *>(p), static_cast<const double *>(diag), n, st, ctx().d_partials);
-  else
+           static_cast<const double *>(p), static_cast<const double *>(diag), n, st, out);
+  } else {
     LAUNCH(k_cg_update<float>, kRedBlocks, static_cast<float *>(x), static_cast<float *>(r), static_cast<float *>(z),
-           static_cast<const float *>(p), static_cast<const float *>(diag), n, st, ctx().d_partials);
-  LAUNCH((k_finish<2, kEpiUpdate>), 1, ctx().d_partials, kRedBlocks, st, static_cast<double *>(nullptr));
+           static_cast<const float *>(p), static_cast<const float *>(diag), n, st, out);
+  }
   CHECK_LAUNCH();
   return LFRIC_OK;
 }
 
-int launch_cg_aypx(int prec, void *p, const void *z, long long n, SolverState *st) {
+int launch_cg_aypx(int prec, void *p, const void *z, long long n, SolverState *st, int par) {
+  // kRedBlocks blocks (grid stride): each block adds the update's partials once
   if (prec == LFRIC_R64)
-    LAUNCH(k_cg_aypx<double>, vgrid(n), static_cast<double *>(p), static_cast<const double *>(z), n, st);
+    LAUNCH(k_cg_aypx_fold<double>, kRedBlocks, static_cast<double *>(p), static_cast<const double *>(z), n, st,
+           update_partials(), kRedBlocks, par);
   else
-    LAUNCH(k_cg_aypx<float>, vgrid(n), static_cast<float *>(p), static_cast<const float *>(z), n, st);
+    LAUNCH(k_cg_aypx_fold<float>, kRedBlocks, static_cast<float *>(p), static_cast<const float *>(z), n, st,
+           update_partials(), kRedBlocks, par);
   CHECK_LAUNCH();
   return LFRIC_OK;
 }
