@@ -57,6 +57,13 @@ int lfric_init(int device);
 const char *lfric_last_error(void);
 /* Wait for all work queued by the library. */
 int lfric_synchronize(void);
+/* Launch this host thread's library calls on `stream` (a hipStream_t the caller created, at
+ * default priority; NULL: the library's own stream), so that one host thread per stream drives
+ * its own work (the threading of a serving loop).  Calls that wait for results wait on that
+ * stream.  Applies and vector operations on different streams may overlap; the solvers share
+ * the library's device scratch and state, so their calls must not overlap one another.  No
+ * reference counterpart. */
+int lfric_set_stream(void *stream);
 /* Time work on the library stream with HIP events: start, then stop returns milliseconds. */
 int lfric_timer_start(void);
 int lfric_timer_stop(double *ms);
@@ -257,6 +264,10 @@ int lfric_lma_apply_gather(const lfric_op *op, const lfric_vec *x, lfric_vec *y)
 int lfric_lma_apply_mrhs(const lfric_op *op, const lfric_vec *x, lfric_vec *y, int variant);
 /* transpose_matrix_vector_kernel (components/science/source/kernel/algebra/
  * transpose_matrix_vector_kernel_mod.F90): y = A^T x, A maps `from` -> `to`, y on `from`. */
+/* lfric_lma_apply on the caller's stream (a hipStream_t; NULL: the library's), returning
+ * without waiting: the invoke of setval_c + matrix_vector_kernel_type
+ * (sci_mass_matrix_operator_alg_mod.x90:98-100) queued behind the caller's earlier work. */
+int lfric_lma_apply_stream(const lfric_op *op, const lfric_vec *x, lfric_vec *y, void *stream);
 int lfric_lma_apply_transpose(const lfric_op *op, const lfric_vec *x, lfric_vec *y);
 /* dg_matrix_vector_kernel (components/science/source/kernel/algebra/
  * dg_matrix_vector_kernel_mod.F90): discontinuous output, no colouring. */

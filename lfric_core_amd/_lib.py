@@ -132,6 +132,8 @@ SIGNATURES = {
     "lfric_residual_norm": (_I, [_VP, _VP, _VP, _DP]),
     "lfric_cg_solve": (_I, [_VP, _VP, _VP, _VP, ctypes.POINTER(SolverOpts), ctypes.POINTER(SolverRes)]),
     "lfric_extrusion_eta": (_I, [_I, _I, _VP]),
+    "lfric_set_stream": (_I, [_VP]),
+    "lfric_lma_apply_stream": (_I, [_VP, _VP, _VP, _VP]),
     "lfric_fs_coordinates": (_I, [_VP, _I, _D, _D, _VP, _VP, _VP]),
     "lfric_fs_get_boundary_dofs": (_I, [_VP, _VP]),
     "lfric_fs_set_boundary_dofs": (_I, [_VP, _VP]),

@@ -22,6 +22,8 @@ int fail(int code, const std::string &m) {
 
 static Context g_ctx;
 Context &ctx() { return g_ctx; }
+static thread_local hipStream_t t_stream = nullptr;
+hipStream_t cur_stream() { return t_stream ? t_stream : g_ctx.stream; }
 
 int require_init() {
   if (g_ctx.device < 0) return fail(LFRIC_E_STATE, "lfric_init has not been called");
@@ -295,7 +297,7 @@ static int cg_device(const CgArgs &a, SolverState &out, unsigned char *scratch_b
   s.a_tol = a.a_tol;
   s.max_iter = a.max_iter;
   s.monitor = a.monitor;
-  LF_HIP(hipMemcpyAsync(g_state, &s, sizeof(s), hipMemcpyHostToDevice, c.stream));
+  LF_HIP(hipMemcpyAsync(g_state, &s, sizeof(s), hipMemcpyHostToDevice, cur_stream()));
   if (a.x_zero) {
     if (int rc = launch_copy(a.prec, r, a.b, a.n)) return rc;
   } else {
@@ -345,8 +347,8 @@ static int cg_device(const CgArgs &a, SolverState &out, unsigned char *scratch_b
         return rc;
       if (int rc = launch_cg_aypx(a.prec, p, z, a.n, g_state, par)) return rc;
     }
-    LF_HIP(hipMemcpyAsync(g_state_host, g_state, sizeof(SolverState), hipMemcpyDeviceToHost, c.stream));
-    LF_HIP(hipStreamSynchronize(c.stream));
+    LF_HIP(hipMemcpyAsync(g_state_host, g_state, sizeof(SolverState), hipMemcpyDeviceToHost, cur_stream()));
+    LF_HIP(hipStreamSynchronize(cur_stream()));
     if (g_state_host->done) break;
   }
   out = *g_state_host;
@@ -361,8 +363,8 @@ static int true_residual(const lfric_op *A, const void *x, const void *b, long l
   double *d_out = &g_state->red0;
   if (int rc = launch_dot(prec, r, r, n, d_out)) return rc;
   double h = 0;
-  LF_HIP(hipMemcpyAsync(g_state_host, g_state, sizeof(SolverState), hipMemcpyDeviceToHost, ctx().stream));
-  LF_HIP(hipStreamSynchronize(ctx().stream));
+  LF_HIP(hipMemcpyAsync(g_state_host, g_state, sizeof(SolverState), hipMemcpyDeviceToHost, cur_stream()));
+  LF_HIP(hipStreamSynchronize(cur_stream()));
   h = g_state_host->red0;
   *nrm = std::sqrt(h);
   return LFRIC_OK;
@@ -407,21 +409,27 @@ int lfric_init(int device) {
 
 const char *lfric_last_error(void) { return g_err.c_str(); }
 
+int lfric_set_stream(void *stream) {
+  if (int rc = require_init()) return rc;
+  t_stream = static_cast<hipStream_t>(stream);
+  return LFRIC_OK;
+}
+
 int lfric_synchronize(void) {
   if (int rc = require_init()) return rc;
-  LF_HIP(hipStreamSynchronize(ctx().stream));
+  LF_HIP(hipStreamSynchronize(cur_stream()));
   return LFRIC_OK;
 }
 
 int lfric_timer_start(void) {
   if (int rc = require_init()) return rc;
-  LF_HIP(hipEventRecord(ctx().t0, ctx().stream));
+  LF_HIP(hipEventRecord(ctx().t0, cur_stream()));
   return LFRIC_OK;
 }
 
 int lfric_timer_stop(double *ms) {
   if (int rc = require_init()) return rc;
-  LF_HIP(hipEventRecord(ctx().t1, ctx().stream));
+  LF_HIP(hipEventRecord(ctx().t1, cur_stream()));
   LF_HIP(hipEventSynchronize(ctx().t1));
   float f = 0.f;
   LF_HIP(hipEventElapsedTime(&f, ctx().t0, ctx().t1));
@@ -591,7 +599,7 @@ int lfric_vec_create(const lfric_fs *fs, int precision, int nrhs, lfric_vec **ou
     delete v;
     return rc;
   }
-  LF_HIP(hipMemsetAsync(v->d, 0, v->bytes, ctx().stream));
+  LF_HIP(hipMemsetAsync(v->d, 0, v->bytes, cur_stream()));
   *out = v;
   return LFRIC_OK;
 }
@@ -606,16 +614,16 @@ static int upload_any(lfric_vec *v, const void *host, int hprec) {
   if (int rc = require_init()) return rc;
   LF_CHECK(v && host, LFRIC_E_ARG, "null argument");
   if (hprec == v->precision) {
-    LF_HIP(hipMemcpyAsync(v->d, host, v->bytes, hipMemcpyHostToDevice, ctx().stream));
-    LF_HIP(hipStreamSynchronize(ctx().stream));
+    LF_HIP(hipMemcpyAsync(v->d, host, v->bytes, hipMemcpyHostToDevice, cur_stream()));
+    LF_HIP(hipStreamSynchronize(cur_stream()));
     return LFRIC_OK;
   }
   void *tmp;
   const size_t hb = esize(hprec) * static_cast<size_t>(v->n);
   if (int rc = ws_get(7, hb, &tmp)) return rc;
-  LF_HIP(hipMemcpyAsync(tmp, host, hb, hipMemcpyHostToDevice, ctx().stream));
+  LF_HIP(hipMemcpyAsync(tmp, host, hb, hipMemcpyHostToDevice, cur_stream()));
   if (int rc = launch_convert(v->precision, v->d, hprec, tmp, v->n, 1.0)) return rc;
-  LF_HIP(hipStreamSynchronize(ctx().stream));
+  LF_HIP(hipStreamSynchronize(cur_stream()));
   return LFRIC_OK;
 }
 
@@ -623,16 +631,16 @@ static int download_any(const lfric_vec *v, void *host, int hprec) {
   if (int rc = require_init()) return rc;
   LF_CHECK(v && host, LFRIC_E_ARG, "null argument");
   if (hprec == v->precision) {
-    LF_HIP(hipMemcpyAsync(host, v->d, v->bytes, hipMemcpyDeviceToHost, ctx().stream));
-    LF_HIP(hipStreamSynchronize(ctx().stream));
+    LF_HIP(hipMemcpyAsync(host, v->d, v->bytes, hipMemcpyDeviceToHost, cur_stream()));
+    LF_HIP(hipStreamSynchronize(cur_stream()));
     return LFRIC_OK;
   }
   void *tmp;
   const size_t hb = esize(hprec) * static_cast<size_t>(v->n);
   if (int rc = ws_get(7, hb, &tmp)) return rc;
   if (int rc = launch_convert(hprec, tmp, v->precision, v->d, v->n, 1.0)) return rc;
-  LF_HIP(hipMemcpyAsync(host, tmp, hb, hipMemcpyDeviceToHost, ctx().stream));
-  LF_HIP(hipStreamSynchronize(ctx().stream));
+  LF_HIP(hipMemcpyAsync(host, tmp, hb, hipMemcpyDeviceToHost, cur_stream()));
+  LF_HIP(hipStreamSynchronize(cur_stream()));
   return LFRIC_OK;
 }
 
@@ -693,8 +701,8 @@ int lfric_vec_dot(const lfric_vec *x, const lfric_vec *y, double *out) {
   SAME_SHAPE(x, y);
   LF_CHECK(out, LFRIC_E_ARG, "null output");
   if (int rc = launch_dot(x->precision, x->d, y->d, x->n, &g_state->red0)) return rc;
-  LF_HIP(hipMemcpyAsync(out, &g_state->red0, sizeof(double), hipMemcpyDeviceToHost, ctx().stream));
-  LF_HIP(hipStreamSynchronize(ctx().stream));
+  LF_HIP(hipMemcpyAsync(out, &g_state->red0, sizeof(double), hipMemcpyDeviceToHost, cur_stream()));
+  LF_HIP(hipStreamSynchronize(cur_stream()));
   return LFRIC_OK;
 }
 int lfric_vec_norm(const lfric_vec *x, double *out) {
@@ -731,7 +739,7 @@ int lfric_op_create(const lfric_fs *to, const lfric_fs *from, int precision, lfr
     delete op;
     return rc;
   }
-  LF_HIP(hipMemsetAsync(op->d, 0, op->bytes, ctx().stream));
+  LF_HIP(hipMemsetAsync(op->d, 0, op->bytes, cur_stream()));
   *out = op;
   return LFRIC_OK;
 }
@@ -778,7 +786,7 @@ static int op_upload_any(lfric_op *op, const void *ls, int prec) {
   const size_t esize = prec == LFRIC_R64 ? sizeof(double) : sizeof(float);
   void *tmp = nullptr;
   if (int rc = dmalloc(&tmp, n * esize)) return rc;
-  LF_HIP(hipMemcpyAsync(tmp, ls, n * esize, hipMemcpyHostToDevice, ctx().stream));
+  LF_HIP(hipMemcpyAsync(tmp, ls, n * esize, hipMemcpyHostToDevice, cur_stream()));
   int rc = launch_op_from_reference(op, tmp, prec);
   dfree(tmp, n * esize);
   return refresh_rows(op, rc);
@@ -913,12 +921,12 @@ int lfric_op_time_apply(const lfric_op *op, const lfric_vec *x, lfric_vec *y, in
   bool two = false;
   if (!rc) rc = launch_lma_apply_gather(op, x->d, y->d);  // warm-up (builds the gather lists)
   for (int r = 0; r < reps && !rc; ++r) {
-    if (hipEventRecord(ev[3 * r], c.stream) != hipSuccess) rc = fail(LFRIC_E_HIP, "event");
+    if (hipEventRecord(ev[3 * r], cur_stream()) != hipSuccess) rc = fail(LFRIC_E_HIP, "event");
     if (!rc) rc = launch_lma_apply_gather_marked(op, x->d, y->d, ev[3 * r + 1], &two);
-    if (!rc && hipEventRecord(ev[3 * r + 2], c.stream) != hipSuccess) rc = fail(LFRIC_E_HIP, "event");
+    if (!rc && hipEventRecord(ev[3 * r + 2], cur_stream()) != hipSuccess) rc = fail(LFRIC_E_HIP, "event");
   }
   double p1 = 0, p2 = 0;
-  if (!rc && hipStreamSynchronize(c.stream) != hipSuccess) rc = fail(LFRIC_E_HIP, "synchronize");
+  if (!rc && hipStreamSynchronize(cur_stream()) != hipSuccess) rc = fail(LFRIC_E_HIP, "synchronize");
   for (int r = 0; r < reps && !rc; ++r) {
     float a = 0.f, b = 0.f;
     if (two) {
@@ -964,11 +972,11 @@ int lfric_op_tune(lfric_op *op, int form, int *best_window, double *best_ms) {
     op->xwin = w;
     auto go = [&]() { return form == 1 ? launch_lma_apply_gather(op, x, y) : launch_lma_apply(op, x, y, nullptr); };
     if ((rc = go())) break;
-    if (hipEventRecord(c.t0, c.stream) != hipSuccess) { rc = fail(LFRIC_E_HIP, "event"); break; }
+    if (hipEventRecord(c.t0, cur_stream()) != hipSuccess) { rc = fail(LFRIC_E_HIP, "event"); break; }
     for (int r = 0; r < 20 && !rc; ++r) rc = go();
     if (rc) break;
     float ms = 0.f;
-    if (hipEventRecord(c.t1, c.stream) != hipSuccess || hipEventSynchronize(c.t1) != hipSuccess ||
+    if (hipEventRecord(c.t1, cur_stream()) != hipSuccess || hipEventSynchronize(c.t1) != hipSuccess ||
         hipEventElapsedTime(&ms, c.t0, c.t1) != hipSuccess) {
       rc = fail(LFRIC_E_HIP, "event timing");
       break;
@@ -979,7 +987,7 @@ int lfric_op_tune(lfric_op *op, int form, int *best_window, double *best_ms) {
     }
   }
   op->xwin = bw;
-  (void)hipStreamSynchronize(c.stream);
+  (void)hipStreamSynchronize(cur_stream());
   dfree(x, es * static_cast<size_t>(op->from->undf));
   dfree(y, es * static_cast<size_t>(op->to->undf));
   if (rc) return rc;
@@ -992,7 +1000,7 @@ int lfric_op_prepare_rows(lfric_op *op) {
   if (int rc = require_init()) return rc;
   LF_CHECK(op, LFRIC_E_ARG, "null argument");
   if (int rc = launch_build_rows(op)) return rc;
-  LF_HIP(hipStreamSynchronize(ctx().stream));
+  LF_HIP(hipStreamSynchronize(cur_stream()));
   return LFRIC_OK;
 }
 
@@ -1000,7 +1008,7 @@ int lfric_op_prepare_symmetric(lfric_op *op, int levels_per_thread) {
   if (int rc = require_init()) return rc;
   LF_CHECK(op, LFRIC_E_ARG, "null argument");
   if (int rc = launch_build_sym(op, levels_per_thread)) return rc;
-  LF_HIP(hipStreamSynchronize(ctx().stream));
+  LF_HIP(hipStreamSynchronize(cur_stream()));
   return LFRIC_OK;
 }
 
@@ -1026,6 +1034,15 @@ int lfric_lma_apply_gather(const lfric_op *op, const lfric_vec *x, lfric_vec *y)
   if (int rc = require_init()) return rc;
   if (int rc = check_apply(op, x, y, false)) return rc;
   return launch_lma_apply_gather(op, x->d, y->d);
+}
+
+int lfric_lma_apply_stream(const lfric_op *op, const lfric_vec *x, lfric_vec *y, void *stream) {
+  if (int rc = require_init()) return rc;
+  const hipStream_t keep = t_stream;
+  t_stream = static_cast<hipStream_t>(stream);
+  const int rc = lfric_lma_apply(op, x, y);
+  t_stream = keep;
+  return rc;
 }
 
 int lfric_lma_apply_transpose(const lfric_op *op, const lfric_vec *x, lfric_vec *y) {
@@ -1254,8 +1271,8 @@ int lfric_ir_cg_solve(const lfric_op *A64, const lfric_op *A32, const lfric_prec
     if (int rc = launch_lma_apply_solver(A64, x->d, r64, nullptr)) return rc;
     if (int rc = launch_cg_residual(LFRIC_R64, r64, b->d, n)) return rc;
     if (int rc = launch_dot(LFRIC_R64, r64, r64, n, &g_state->red0)) return rc;
-    LF_HIP(hipMemcpyAsync(g_state_host, g_state, sizeof(SolverState), hipMemcpyDeviceToHost, ctx().stream));
-    LF_HIP(hipStreamSynchronize(ctx().stream));
+    LF_HIP(hipMemcpyAsync(g_state_host, g_state, sizeof(SolverState), hipMemcpyDeviceToHost, cur_stream()));
+    LF_HIP(hipStreamSynchronize(cur_stream()));
     *nrm = std::sqrt(g_state_host->red0);
     return LFRIC_OK;
   };

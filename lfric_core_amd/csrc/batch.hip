@@ -472,7 +472,7 @@ namespace lfric {
 namespace {
 
 #define LAUNCHB(kernel, grid, block, ...) \
-  hipLaunchKernelGGL((kernel), dim3(grid), dim3(block), 0, ctx().stream, __VA_ARGS__)
+  hipLaunchKernelGGL((kernel), dim3(grid), dim3(block), 0, cur_stream(), __VA_ARGS__)
 
 constexpr double kEps = 2.220446049250313e-16;
 
@@ -528,8 +528,8 @@ int alloc_state(lfric_batch &B, int nr, int *scal0) {
       return fail(LFRIC_E_NOMEM, "pinned host state");
     }
     if (B.used_state) {
-      LF_HIP(hipMemcpyAsync(d, B.d_state, B.used_state * sizeof(RhsDev), hipMemcpyDeviceToDevice, ctx().stream));
-      LF_HIP(hipStreamSynchronize(ctx().stream));
+      LF_HIP(hipMemcpyAsync(d, B.d_state, B.used_state * sizeof(RhsDev), hipMemcpyDeviceToDevice, cur_stream()));
+      LF_HIP(hipStreamSynchronize(cur_stream()));
     }
     dfree(B.d_state, B.cap_state * sizeof(RhsDev));
     if (B.h_state) (void)hipHostFree(B.h_state);
@@ -633,12 +633,12 @@ int rebuild(lfric_batch &B) {
                       static_cast<size_t>(std::max(B.nvtiles, kIrBlocks)) * 2 * kMaxR, sizeof(double)))
     return rc;
   if (np) {
-    LF_HIP(hipMemcpyAsync(B.d_desc, desc.data(), sizeof(ProbDev) * np, hipMemcpyHostToDevice, ctx().stream));
-    LF_HIP(hipMemcpyAsync(B.d_tile0, tile0.data(), sizeof(int) * (np + 1), hipMemcpyHostToDevice, ctx().stream));
-    LF_HIP(hipMemcpyAsync(B.d_vtile0, vtile0.data(), sizeof(int) * (np + 1), hipMemcpyHostToDevice, ctx().stream));
-    LF_HIP(hipMemcpyAsync(B.d_ctile0, ctile0.data(), sizeof(int) * ctile0.size(), hipMemcpyHostToDevice, ctx().stream));
+    LF_HIP(hipMemcpyAsync(B.d_desc, desc.data(), sizeof(ProbDev) * np, hipMemcpyHostToDevice, cur_stream()));
+    LF_HIP(hipMemcpyAsync(B.d_tile0, tile0.data(), sizeof(int) * (np + 1), hipMemcpyHostToDevice, cur_stream()));
+    LF_HIP(hipMemcpyAsync(B.d_vtile0, vtile0.data(), sizeof(int) * (np + 1), hipMemcpyHostToDevice, cur_stream()));
+    LF_HIP(hipMemcpyAsync(B.d_ctile0, ctile0.data(), sizeof(int) * ctile0.size(), hipMemcpyHostToDevice, cur_stream()));
   }
-  LF_HIP(hipStreamSynchronize(ctx().stream));
+  LF_HIP(hipStreamSynchronize(cur_stream()));
   drop_graph(B);
   B.dirty = false;
   return LFRIC_OK;
@@ -731,16 +731,16 @@ int run_iterations(lfric_batch &B, int k) {
   if (B.active.empty()) return LFRIC_OK;
   if (B.use_graph) {
     if (!B.exec) {
-      LF_HIP(hipStreamBeginCapture(ctx().stream, hipStreamCaptureModeThreadLocal));
+      LF_HIP(hipStreamBeginCapture(cur_stream(), hipStreamCaptureModeThreadLocal));
       const int rc = enqueue_iteration(B);
       hipGraph_t g = nullptr;
-      const hipError_t e = hipStreamEndCapture(ctx().stream, &g);
+      const hipError_t e = hipStreamEndCapture(cur_stream(), &g);
       if (rc) return rc;
       if (e != hipSuccess) return fail(LFRIC_E_HIP, std::string("graph capture: ") + hipGetErrorString(e));
       B.graph = g;
       LF_HIP(hipGraphInstantiate(&B.exec, B.graph, nullptr, nullptr, 0));
     }
-    for (int i = 0; i < k; ++i) LF_HIP(hipGraphLaunch(B.exec, ctx().stream));
+    for (int i = 0; i < k; ++i) LF_HIP(hipGraphLaunch(B.exec, cur_stream()));
     B.replays += k;
   } else {
     for (int i = 0; i < k; ++i)
@@ -762,14 +762,14 @@ int refine(lfric_batch &B, Problem &p) {
   double *nrm = B.d_scratch + 4 * kMaxR;
   if (p.started) {
     for (int r = 0; r < nr; ++r) B.h_scratch[r] = p.conv[r] ? 0.0 : p.nr[r];
-    LF_HIP(hipMemcpyAsync(scale, B.h_scratch, sizeof(double) * nr, hipMemcpyHostToDevice, ctx().stream));
+    LF_HIP(hipMemcpyAsync(scale, B.h_scratch, sizeof(double) * nr, hipMemcpyHostToDevice, cur_stream()));
     LAUNCHB(k_ir_accumulate, vg, kBlock, static_cast<double *>(p.x->d), p.d, scale, n, nr);
   }
   if (int rc = launch_lma_apply_mrhs(p.A64, p.x->d, p.r64, nr, LFRIC_MRHS_AUTO)) return rc;
   LAUNCHB(k_ir_residual, kIrBlocks, kVecBlock, p.r64, static_cast<const double *>(p.b->d), n, nr, B.d_partials);
   LAUNCHB(k_ir_norms, 1, kFinBlock, B.d_partials, kIrBlocks, nr, nrm);
-  LF_HIP(hipMemcpyAsync(B.h_scratch + 4 * kMaxR, nrm, sizeof(double) * nr, hipMemcpyDeviceToHost, ctx().stream));
-  LF_HIP(hipStreamSynchronize(ctx().stream));
+  LF_HIP(hipMemcpyAsync(B.h_scratch + 4 * kMaxR, nrm, sizeof(double) * nr, hipMemcpyDeviceToHost, cur_stream()));
+  LF_HIP(hipStreamSynchronize(cur_stream()));
   bool all = true, nan = false;
   for (int r = 0; r < nr; ++r) {
     const double v = B.h_scratch[4 * kMaxR + r];
@@ -795,12 +795,12 @@ int refine(lfric_batch &B, Problem &p) {
     reinterpret_cast<int *>(&hv[3 * kMaxR])[r] = p.conv[r] ? 0 : 1;
   }
   std::memcpy(B.h_scratch, hv, sizeof(hv));
-  LF_HIP(hipMemcpyAsync(B.d_scratch, B.h_scratch, sizeof(hv), hipMemcpyHostToDevice, ctx().stream));
+  LF_HIP(hipMemcpyAsync(B.d_scratch, B.h_scratch, sizeof(hv), hipMemcpyHostToDevice, cur_stream()));
   LAUNCHB(k_ir_start, kIrBlocks, kVecBlock, p.r64, inv, p.d, p.r, p.z, p.p, static_cast<const float *>(p.diag->d), n,
           nr, B.d_partials);
   LAUNCHB(k_ir_start_fin, 1, kFinBlock, B.d_partials, kIrBlocks, nr, B.d_state + p.scal0, tol, live);
   LF_HIP(hipGetLastError());
-  LF_HIP(hipStreamSynchronize(ctx().stream));
+  LF_HIP(hipStreamSynchronize(cur_stream()));
   if (!p.inner_running) B.dirty = true;
   p.inner_running = true;
   return LFRIC_OK;
@@ -832,8 +832,8 @@ int run_rounds(lfric_batch &B, int rounds, int *nactive) {
     if (B.active.empty()) break;
     if (int rc = run_iterations(B, k)) return rc;
     LF_HIP(hipMemcpyAsync(B.h_state, B.d_state, sizeof(RhsDev) * B.used_state, hipMemcpyDeviceToHost,
-                          ctx().stream));
-    LF_HIP(hipStreamSynchronize(ctx().stream));
+                          cur_stream()));
+    LF_HIP(hipStreamSynchronize(cur_stream()));
     for (Problem *p : std::vector<Problem *>(B.active)) {
       bool ended = true;
       for (int r = 0; r < p->nrhs; ++r)
@@ -988,7 +988,7 @@ int lfric_batch_remove(lfric_batch *B, int id) {
   for (size_t i = 0; i < B->probs.size(); ++i)
     if (B->probs[i]->id == id) {
       if (B->probs[i]->inner_running) B->dirty = true;
-      LF_HIP(hipStreamSynchronize(ctx().stream));
+      LF_HIP(hipStreamSynchronize(cur_stream()));
       free_problem(*B, *B->probs[i]);
       B->probs.erase(B->probs.begin() + static_cast<long>(i));
       return LFRIC_OK;
@@ -1013,9 +1013,9 @@ int lfric_batch_tune(lfric_batch *B, double *best_ms) {
     if (int rc = rebuild(*B)) return rc;  // tiles follow the block size
     for (int gm : mults) {
       launch_apply(*B, blk, gm);  // warm
-      LF_HIP(hipEventRecord(ctx().t0, ctx().stream));
+      LF_HIP(hipEventRecord(ctx().t0, cur_stream()));
       for (int rep = 0; rep < 5; ++rep) launch_apply(*B, blk, gm);
-      LF_HIP(hipEventRecord(ctx().t1, ctx().stream));
+      LF_HIP(hipEventRecord(ctx().t1, cur_stream()));
       LF_HIP(hipEventSynchronize(ctx().t1));
       float ms = 0.f;
       LF_HIP(hipEventElapsedTime(&ms, ctx().t0, ctx().t1));
@@ -1061,7 +1061,7 @@ int lfric_batch_stats(const lfric_batch *B, long long *iterations, int *block, i
 
 void lfric_batch_destroy(lfric_batch *B) {
   if (!B) return;
-  (void)hipStreamSynchronize(ctx().stream);
+  (void)hipStreamSynchronize(cur_stream());
   for (auto &u : B->probs) free_problem(*B, *u);
   drop_graph(*B);
   dfree(B->d_state, B->cap_state * sizeof(RhsDev));

@@ -617,7 +617,7 @@ __global__ void k_cma_to_ref(const T *__restrict__ C, CmaP p, int nc, double *__
 
 }  // namespace
 
-#define CLAUNCH(kernel, n, ...) hipLaunchKernelGGL((kernel), dim3(grid_for(n)), dim3(kBlock), 0, ctx().stream, __VA_ARGS__)
+#define CLAUNCH(kernel, n, ...) hipLaunchKernelGGL((kernel), dim3(grid_for(n)), dim3(kBlock), 0, cur_stream(), __VA_ARGS__)
 
 // Interior / vertical-face dof counts of a horizontally discontinuous lowest-order space, from
 // its column roles: nf bottom faces first, then ni interior dofs, then the nf top faces (the
@@ -766,11 +766,11 @@ int launch_cma_appinv(const lfric_cma *A, const void *x, void *y) {
     const size_t lds = cw_lds ? 2 * lds1 : lds1;
 #define APPINV_LDS(T, CH)                                                                                       \
   if (cw_lds)                                                                                                   \
-    hipLaunchKernelGGL((k_cma_appinv_lds<T, CH, true>), dim3(blocks), dim3(kColBlock), lds, ctx().stream,       \
+    hipLaunchKernelGGL((k_cma_appinv_lds<T, CH, true>), dim3(blocks), dim3(kColBlock), lds, cur_stream(),       \
                        static_cast<const T *>(A->d), p, A->ncols, static_cast<const T *>(x), static_cast<T *>(y), \
                        f.d_map, f.ndf, A->d_ind_to, static_cast<T *>(A->work));                                 \
   else                                                                                                          \
-    hipLaunchKernelGGL((k_cma_appinv_lds<T, CH, false>), dim3(blocks), dim3(kColBlock), lds, ctx().stream,      \
+    hipLaunchKernelGGL((k_cma_appinv_lds<T, CH, false>), dim3(blocks), dim3(kColBlock), lds, cur_stream(),      \
                        static_cast<const T *>(A->d), p, A->ncols, static_cast<const T *>(x), static_cast<T *>(y), \
                        f.d_map, f.ndf, A->d_ind_to, static_cast<T *>(A->work));
     if (A->precision == LFRIC_R64) {
@@ -853,7 +853,7 @@ static int cma_ref_copy(const lfric_cma *A, const double *in, double *out) {
     }
     e = hipGetLastError();
   }
-  if (e == hipSuccess) e = hipStreamSynchronize(ctx().stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(cur_stream());
   if (e == hipSuccess && out) e = hipMemcpy(out, d_ref, sizeof(double) * n, hipMemcpyDeviceToHost);
   dfree(d_ref, sizeof(double) * n);
   dfree(d_cop, sizeof(int) * f.cell_of_pos.size());
@@ -946,7 +946,7 @@ int lfric_cma_apply(const lfric_cma *A, const lfric_vec *x, lfric_vec *y, int ac
 #define CMA_APPLY(T, BW)                                                                                          \
   if (staged && BW > 0)                                                                                           \
     hipLaunchKernelGGL((k_cma_apply_lds<T, (BW > 0 ? BW : 1)>), dim3(sblocks), dim3(kColBlock * kApplyWaves), slds,   \
-                       ctx().stream,                                                                              \
+                       cur_stream(),                                                                              \
                        static_cast<const T *>(A->d), p, A->ncols, static_cast<const T *>(x->d),                   \
                        static_cast<T *>(y->d), ft.d_map, ft.ndf, ff.d_map, ff.ndf, A->d_ind_to, A->d_ind_from,    \
                        accumulate);                                                                               \

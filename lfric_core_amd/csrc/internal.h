@@ -41,6 +41,9 @@ struct Context {
   int apply_form = 1;  // LMA apply inside the solvers: 0 colour-ordered scatter, 1 gather (+ fused dot)
 };
 Context &ctx();
+// the stream the calls of this host thread launch on: the one lfric_set_stream gave, else the
+// library's own
+hipStream_t cur_stream();
 int require_init();
 int dmalloc(void **p, size_t bytes);
 void dfree(void *p, size_t bytes);

@@ -1781,7 +1781,7 @@ __global__ __launch_bounds__(kBlock) void k_enforce_bc(T *__restrict__ y, const 
   }
 }
 
-#define LAUNCH(kernel, grid, ...) hipLaunchKernelGGL((kernel), dim3(grid), dim3(kBlock), 0, ctx().stream, __VA_ARGS__)
+#define LAUNCH(kernel, grid, ...) hipLaunchKernelGGL((kernel), dim3(grid), dim3(kBlock), 0, cur_stream(), __VA_ARGS__)
 #define CHECK_LAUNCH() LF_HIP(hipGetLastError())
 
 int launch_gen_spd(lfric_op *op, unsigned long long seed, int scaled, double shift) {
@@ -1791,7 +1791,7 @@ int launch_gen_spd(lfric_op *op, unsigned long long seed, int scaled, double shi
   int *d_cop = nullptr;
   size_t bytes = sizeof(int) * f.cell_of_pos.size();
   if (int rc = dmalloc(reinterpret_cast<void **>(&d_cop), bytes)) return rc;
-  LF_HIP(hipMemcpyAsync(d_cop, f.cell_of_pos.data(), bytes, hipMemcpyHostToDevice, ctx().stream));
+  LF_HIP(hipMemcpyAsync(d_cop, f.cell_of_pos.data(), bytes, hipMemcpyHostToDevice, cur_stream()));
   const int grid = grid_for(nelem, kBlock / 64);
   int rc = LFRIC_OK;
   auto go = [&](auto tag, auto ndfc) {
@@ -1808,7 +1808,7 @@ int launch_gen_spd(lfric_op *op, unsigned long long seed, int scaled, double shi
     default: rc = fail(LFRIC_E_UNSUPPORTED, "generator: unsupported ndf");
   }
   hipError_t e = hipGetLastError();
-  LF_HIP(hipStreamSynchronize(ctx().stream));
+  LF_HIP(hipStreamSynchronize(cur_stream()));
   dfree(d_cop, bytes);
   if (e != hipSuccess) return fail(LFRIC_E_HIP, hipGetErrorString(e));
   return rc;
@@ -1825,9 +1825,9 @@ int launch_gen_uniform(lfric_op *op, unsigned long long seed, unsigned long long
   if (int rc = dmalloc(reinterpret_cast<void **>(&d_cop), bytes)) return rc;
   if (col_sign_host) {
     if (int rc = dmalloc(reinterpret_cast<void **>(&d_sign), sizeof(double) * n2)) return rc;
-    LF_HIP(hipMemcpyAsync(d_sign, col_sign_host, sizeof(double) * n2, hipMemcpyHostToDevice, ctx().stream));
+    LF_HIP(hipMemcpyAsync(d_sign, col_sign_host, sizeof(double) * n2, hipMemcpyHostToDevice, cur_stream()));
   }
-  LF_HIP(hipMemcpyAsync(d_cop, f.cell_of_pos.data(), bytes, hipMemcpyHostToDevice, ctx().stream));
+  LF_HIP(hipMemcpyAsync(d_cop, f.cell_of_pos.data(), bytes, hipMemcpyHostToDevice, cur_stream()));
   const int grid = grid_for(nelem);
   if (op->precision == LFRIC_R64)
     LAUNCH(k_gen_uniform<double>, grid, static_cast<double *>(op->d), op->gs, f.d_ebase, d_cop, f.nlayers, nelem, n1,
@@ -1836,7 +1836,7 @@ int launch_gen_uniform(lfric_op *op, unsigned long long seed, unsigned long long
     LAUNCH(k_gen_uniform<float>, grid, static_cast<float *>(op->d), op->gs, f.d_ebase, d_cop, f.nlayers, nelem, n1,
            n2, seed, stream, lo, hi, d_sign, op->row_top);
   hipError_t e = hipGetLastError();
-  LF_HIP(hipStreamSynchronize(ctx().stream));
+  LF_HIP(hipStreamSynchronize(cur_stream()));
   dfree(d_cop, bytes);
   if (d_sign) dfree(d_sign, sizeof(double) * n2);
   if (e != hipSuccess) return fail(LFRIC_E_HIP, hipGetErrorString(e));
@@ -1856,7 +1856,7 @@ int launch_vec_uniform(lfric_vec *v, unsigned long long seed, unsigned long long
 static int with_cell_of_pos(const FunctionSpace &f, int **d_cop, size_t &bytes) {
   bytes = sizeof(int) * f.cell_of_pos.size();
   if (int rc = dmalloc(reinterpret_cast<void **>(d_cop), bytes)) return rc;
-  LF_HIP(hipMemcpyAsync(*d_cop, f.cell_of_pos.data(), bytes, hipMemcpyHostToDevice, ctx().stream));
+  LF_HIP(hipMemcpyAsync(*d_cop, f.cell_of_pos.data(), bytes, hipMemcpyHostToDevice, cur_stream()));
   return LFRIC_OK;
 }
 
@@ -1875,7 +1875,7 @@ static int op_from_reference_t(lfric_op *op, const S *d_ref) {
     LAUNCH((k_from_reference<float, S>), grid_for(nelem), static_cast<float *>(op->d), op->gs, f.d_ebase, d_ref,
            d_cop, f.nlayers, nelem, n1, n2, op->row_top);
   hipError_t e = hipGetLastError();
-  LF_HIP(hipStreamSynchronize(ctx().stream));
+  LF_HIP(hipStreamSynchronize(cur_stream()));
   dfree(d_cop, bytes);
   if (e != hipSuccess) return fail(LFRIC_E_HIP, hipGetErrorString(e));
   return LFRIC_OK;
@@ -1896,7 +1896,7 @@ static int op_to_reference_t(const lfric_op *op, D *d_ref) {
     LAUNCH((k_to_reference<float, D>), grid_for(nelem), static_cast<const float *>(op->d), op->gs, f.d_ebase,
            d_ref, d_cop, f.nlayers, nelem, n1, n2, op->row_top);
   hipError_t e = hipGetLastError();
-  LF_HIP(hipStreamSynchronize(ctx().stream));
+  LF_HIP(hipStreamSynchronize(cur_stream()));
   dfree(d_cop, bytes);
   if (e != hipSuccess) return fail(LFRIC_E_HIP, hipGetErrorString(e));
   return LFRIC_OK;
@@ -1936,7 +1936,7 @@ static int invert_local_t(const lfric_op *A, lfric_op *Ainv) {
                static_cast<T *>(work));
       }
       const hipError_t e = hipGetLastError();
-      const hipError_t es = hipStreamSynchronize(ctx().stream);
+      const hipError_t es = hipStreamSynchronize(cur_stream());
       dfree(work, bytes);
       if (e != hipSuccess) return fail(LFRIC_E_HIP, hipGetErrorString(e));
       if (es != hipSuccess) return fail(LFRIC_E_HIP, hipGetErrorString(es));
@@ -2017,7 +2017,7 @@ int launch_assemble_mass_w0(lfric_op *op, const double *chi1, const double *chi2
   const std::vector<double> t = w0_tables(nqp);
   void *dt = nullptr;
   if (int rc = dmalloc(&dt, sizeof(double) * t.size())) return rc;
-  LF_HIP(hipMemcpyAsync(dt, t.data(), sizeof(double) * t.size(), hipMemcpyHostToDevice, ctx().stream));
+  LF_HIP(hipMemcpyAsync(dt, t.data(), sizeof(double) * t.size(), hipMemcpyHostToDevice, cur_stream()));
   const long long nelem = static_cast<long long>(f.ncells) * f.nlayers;
   double *A = static_cast<double *>(op->d);
   const double *tb = static_cast<const double *>(dt);
@@ -2034,7 +2034,7 @@ int launch_assemble_mass_w0(lfric_op *op, const double *chi1, const double *chi2
   }
 #undef ASM_CASE
   const hipError_t e = hipGetLastError();
-  const hipError_t es = hipStreamSynchronize(ctx().stream);
+  const hipError_t es = hipStreamSynchronize(cur_stream());
   dfree(dt, sizeof(double) * t.size());
   if (e != hipSuccess) return fail(LFRIC_E_HIP, hipGetErrorString(e));
   if (es != hipSuccess) return fail(LFRIC_E_HIP, hipGetErrorString(es));
@@ -2101,7 +2101,7 @@ int launch_assemble_fem(lfric_op *op, int kind, const FunctionSpace &chi_fs, con
   const std::vector<double> t = fem_tables(nqp);
   void *dt = nullptr;
   if (int rc = dmalloc(&dt, sizeof(double) * t.size())) return rc;
-  LF_HIP(hipMemcpyAsync(dt, t.data(), sizeof(double) * t.size(), hipMemcpyHostToDevice, ctx().stream));
+  LF_HIP(hipMemcpyAsync(dt, t.data(), sizeof(double) * t.size(), hipMemcpyHostToDevice, cur_stream()));
   const long long nelem = static_cast<long long>(f.ncells) * f.nlayers;
   double *A = static_cast<double *>(op->d);
   const double *tb = static_cast<const double *>(dt);
@@ -2121,7 +2121,7 @@ int launch_assemble_fem(lfric_op *op, int kind, const FunctionSpace &chi_fs, con
   }
 #undef FEM_CASE
   const hipError_t e = hipGetLastError();
-  const hipError_t es = hipStreamSynchronize(ctx().stream);
+  const hipError_t es = hipStreamSynchronize(cur_stream());
   dfree(dt, sizeof(double) * t.size());
   if (e != hipSuccess) return fail(LFRIC_E_HIP, hipGetErrorString(e));
   if (es != hipSuccess) return fail(LFRIC_E_HIP, hipGetErrorString(es));
@@ -2356,7 +2356,7 @@ static void launch_gather_r(const lfric_op *op, const T *x, T *y, R roles, Gathe
       else
         LAUNCH((k_sym_units<T>), grid_for(static_cast<long long>(to.ncells) * nseg), static_cast<const T *>(op->sym),
                op->gss, x, from.d_map, static_cast<T *>(op->symz), to.ncells, to.nlayers, op->sym_seg, nseg, ex.skip);
-      if (g_phase_mark) (void)hipEventRecord(g_phase_mark, ctx().stream);
+      if (g_phase_mark) (void)hipEventRecord(g_phase_mark, cur_stream());
       if (ex.partials)
         LAUNCH((k_sym_gather_dot<T>), kRedBlocks, static_cast<const T *>(op->symz), x, y, to.d_gcol, to.d_goff,
                to.d_gent, op->d_symzu, to.gather_ncols, to.nlayers, ex.partials, ex.skip, op->xwin);
@@ -2676,7 +2676,7 @@ static int build_sym_t(lfric_op *op, int S) {
     }
   }
   int h_flag = 0;
-  LF_HIP(hipMemcpyAsync(b.flag, &h_flag, sizeof(int), hipMemcpyHostToDevice, ctx().stream));
+  LF_HIP(hipMemcpyAsync(b.flag, &h_flag, sizeof(int), hipMemcpyHostToDevice, cur_stream()));
   if (S == 0)
     LAUNCH((k_build_sym1<T>), grid_for(threads), static_cast<const T *>(op->d), op->gs, f.d_ebase, b.slotpos,
            static_cast<T *>(b.sym), gss, nc, f.nlayers, op->row_top, b.flag);
@@ -2684,8 +2684,8 @@ static int build_sym_t(lfric_op *op, int S) {
     LAUNCH((k_build_sym<T>), grid_for(threads), static_cast<const T *>(op->d), op->gs, f.d_ebase,
            static_cast<T *>(b.sym), gss, nc, f.nlayers, S, nseg, op->row_top, b.flag);
   CHECK_LAUNCH();
-  LF_HIP(hipMemcpyAsync(&h_flag, b.flag, sizeof(int), hipMemcpyDeviceToHost, ctx().stream));
-  LF_HIP(hipStreamSynchronize(ctx().stream));
+  LF_HIP(hipMemcpyAsync(&h_flag, b.flag, sizeof(int), hipMemcpyDeviceToHost, cur_stream()));
+  LF_HIP(hipStreamSynchronize(cur_stream()));
   LF_CHECK(!h_flag, LFRIC_E_UNSUPPORTED, "the operator's element matrices are not symmetric");
   dfree(op->sym, op->sym_bytes);
   dfree(op->symz, op->symz_bytes);
@@ -2946,7 +2946,7 @@ int launch_scale(int prec, void *v, long long n, double a) {
 
 int launch_copy(int prec, void *dst, const void *src, long long n) {
   const size_t es = prec == LFRIC_R64 ? 8 : 4;
-  LF_HIP(hipMemcpyAsync(dst, src, es * static_cast<size_t>(n), hipMemcpyDeviceToDevice, ctx().stream));
+  LF_HIP(hipMemcpyAsync(dst, src, es * static_cast<size_t>(n), hipMemcpyDeviceToDevice, cur_stream()));
   return LFRIC_OK;
 }
 

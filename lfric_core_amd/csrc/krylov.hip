@@ -259,7 +259,7 @@ __global__ void k_gmres_update(T *__restrict__ x, const T *__restrict__ V, long 
 // ======================================================================================
 static size_t esz(int p) { return p == LFRIC_R64 ? 8 : p == LFRIC_R32 ? 4 : 2; }
 
-#define LAUNCHK(kernel, grid, block, ...) hipLaunchKernelGGL((kernel), dim3(grid), dim3(block), 0, ctx().stream, __VA_ARGS__)
+#define LAUNCHK(kernel, grid, block, ...) hipLaunchKernelGGL((kernel), dim3(grid), dim3(block), 0, cur_stream(), __VA_ARGS__)
 
 int linop_apply(const lfric_linop *L, const void *x, void *y) {
   if (L->kind == 0) {
@@ -351,8 +351,8 @@ static int dot_into(const T *a, T *b, long long n, double *dst, bool sq) {
 }
 
 static int read_scalar(const double *d, double *h) {
-  LF_HIP(hipMemcpyAsync(g_host_scalar, d, sizeof(double), hipMemcpyDeviceToHost, ctx().stream));
-  LF_HIP(hipStreamSynchronize(ctx().stream));
+  LF_HIP(hipMemcpyAsync(g_host_scalar, d, sizeof(double), hipMemcpyDeviceToHost, cur_stream()));
+  LF_HIP(hipStreamSynchronize(cur_stream()));
   *h = *g_host_scalar;
   return LFRIC_OK;
 }
@@ -668,7 +668,7 @@ int lfric_prec_create_column_jacobi(const lfric_linop *L, int storage, lfric_pre
   if (rc == LFRIC_OK) rc = dmalloc(reinterpret_cast<void **>(&P->d_cols), P->cols_bytes);
   if (rc == LFRIC_OK) rc = dmalloc(&P->d_fac, P->fac_bytes);
   if (rc == LFRIC_OK) {
-    hipError_t e = hipMemcpyAsync(P->d_cols, cols.data(), P->cols_bytes, hipMemcpyHostToDevice, ctx().stream);
+    hipError_t e = hipMemcpyAsync(P->d_cols, cols.data(), P->cols_bytes, hipMemcpyHostToDevice, cur_stream());
     if (e != hipSuccess) rc = fail(LFRIC_E_HIP, hipGetErrorString(e));
   }
   if (rc == LFRIC_OK) {
@@ -680,7 +680,7 @@ int lfric_prec_create_column_jacobi(const lfric_linop *L, int storage, lfric_pre
     else
       LAUNCHK(k_coltri_factor<LFRIC_F16>, grid, 64, diag, sup, P->d_cols, P->ncols, static_cast<_Float16 *>(P->d_fac), n);
     hipError_t e = hipGetLastError();
-    if (e == hipSuccess) e = hipStreamSynchronize(ctx().stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(cur_stream());
     if (e != hipSuccess) rc = fail(LFRIC_E_HIP, hipGetErrorString(e));
   }
   dfree(diag, sizeof(double) * n);

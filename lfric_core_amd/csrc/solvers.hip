@@ -147,7 +147,7 @@ __global__ __launch_bounds__(kBlock) void k_lincomb(T *out, const T *a, Coef ca,
 }  // namespace
 
 // ---- host side ----------------------------------------------------------------------------
-#define LAUNCHS(kernel, grid, ...) hipLaunchKernelGGL((kernel), dim3(grid), dim3(kBlock), 0, ctx().stream, __VA_ARGS__)
+#define LAUNCHS(kernel, grid, ...) hipLaunchKernelGGL((kernel), dim3(grid), dim3(kBlock), 0, cur_stream(), __VA_ARGS__)
 
 namespace {
 
@@ -179,14 +179,14 @@ int bank_init() {
     LF_HIP(hipHostMalloc(reinterpret_cast<void **>(&g_bank_host), sizeof(ScalarBank), hipHostMallocDefault));
   }
   ScalarBank z{};
-  LF_HIP(hipMemcpyAsync(g_bank, &z, sizeof(ScalarBank), hipMemcpyHostToDevice, ctx().stream));
-  LF_HIP(hipStreamSynchronize(ctx().stream));
+  LF_HIP(hipMemcpyAsync(g_bank, &z, sizeof(ScalarBank), hipMemcpyHostToDevice, cur_stream()));
+  LF_HIP(hipStreamSynchronize(cur_stream()));
   return LFRIC_OK;
 }
 
 int bank_read(ScalarBank &h) {
-  LF_HIP(hipMemcpyAsync(g_bank_host, g_bank, sizeof(ScalarBank), hipMemcpyDeviceToHost, ctx().stream));
-  LF_HIP(hipStreamSynchronize(ctx().stream));
+  LF_HIP(hipMemcpyAsync(g_bank_host, g_bank, sizeof(ScalarBank), hipMemcpyDeviceToHost, cur_stream()));
+  LF_HIP(hipStreamSynchronize(cur_stream()));
   h = *g_bank_host;
   return LFRIC_OK;
 }

@@ -425,3 +425,35 @@ def test_symmetric_packed_apply_equals_colour_apply(hip, orc, n, nl, seg, prec):
     op.apply_gather(x, y1)
     nonsym.apply_gather(x, y2)
     np.testing.assert_array_equal(y1.export_raw(), y2.export_raw())
+
+
+def test_apply_and_solve_on_a_caller_stream(hip, orc):
+    """lfric_lma_apply_stream and lfric_set_stream run the same kernels on a stream the caller
+    created (default priority): the results equal those on the library's stream bit for bit."""
+    import ctypes
+    from lfric_core_amd import _lib
+    L = _lib.lib()
+    hiprt = ctypes.CDLL("libamdhip64.so")
+    stream = ctypes.c_void_p()
+    assert hiprt.hipStreamCreate(ctypes.byref(stream)) == 0
+    try:
+        fs, m, op, b = _config0(hip, orc, 8, 6)
+        y0, y1 = hip.FieldVector(fs), hip.FieldVector(fs)
+        op.apply(b, y0)
+        assert L.lfric_lma_apply_stream(op._h, b._h, y1._h, stream) == 0
+        assert hiprt.hipStreamSynchronize(stream) == 0
+        np.testing.assert_array_equal(y1.export_field(), y0.export_field())
+        prec = hip.DiagonalPreconditioner.from_operator(op)
+        xs = []
+        for s in (None, stream):
+            assert L.lfric_set_stream(s) == 0
+            x = hip.FieldVector(fs)
+            x.set_scalar(0.0)
+            res = hip.ConjugateGradient(op, prec, 1e-8, 0.0, 500).apply(x, b)
+            xs.append((res.iterations, x.export_field()))
+        assert L.lfric_set_stream(None) == 0
+        assert xs[0][0] == xs[1][0]
+        np.testing.assert_array_equal(xs[0][1], xs[1][1])
+    finally:
+        L.lfric_set_stream(None)
+        hiprt.hipStreamDestroy(stream)
