@@ -596,8 +596,21 @@ def batched_record(L, problems, steps, say, colour=True):
     bt.run(1)
     apply_ms = bt.tune()
     bt.close()
-    apply_bytes_total = sum(p["fs"].ncell_3d * p["fs"].ndf ** 2 * 4 + 2 * p["fs"].undf * p["nrhs"] * 4
-                            + 4 * p["fs"].ncells * p["fs"].ndf for p in probs)
+    # with the operators' copies (the default) an order-1 problem reads the 378 entries of each
+    # element's upper triangle (symmetric-packed copy) and a lowest-order one its 64 entries (the
+    # row-gathered copy); the colour-ordered forms read every element in full.  The per-unit row
+    # sums that the packed form writes and reads back between its two phases are not algorithmic.
+    def ent(p):
+        nd = p["fs"].ndf
+        return nd * (nd + 1) // 2 if (colour and nd == 27) else nd * nd
+
+    def xyz(p):
+        return 2 * p["fs"].undf * p["nrhs"] * 4 + 4 * p["fs"].ncells * p["fs"].ndf
+
+    apply_bytes_total = sum(p["fs"].ncell_3d * ent(p) * 4 + xyz(p) for p in probs)
+    apply_bytes_full = sum(p["fs"].ncell_3d * p["fs"].ndf ** 2 * 4 + xyz(p) for p in probs)
+    rowsum_bytes = sum(2 * p["fs"].ncells * 18 * (p["fs"].nlayers + 1) * p["nrhs"] * 4 for p in probs
+                       if colour and p["fs"].ndf == 27)
     # sequential plain FP64 solves of the same systems, one RHS at a time
     seq_t, seq_it = 0.0, 0
     for p in probs:
@@ -621,8 +634,8 @@ def batched_record(L, problems, steps, say, colour=True):
         "workload": f"{len(probs)} uneven W0 SPD problems (C12/C24/C48, 10/30/70 levels, order 0/1), element "
                     "matrices B B^T + 0.1 I with B ~ N(0,1)/8, 4 RHS each, Jacobi CG to 1e-8 in FP64 refinement, "
                     "one batch",
-        "multi_rhs_apply": "colour-ordered: matrix cores (order 0), register-staged FMA (order 1)"
-        if colour else "gather form",
+        "multi_rhs_apply": "operator copies: gather over the row-gathered copy (order 0), symmetric-packed "
+                           "upper triangles in two phases (order 1)" if colour else "gather form",
         "problems": len(probs), "rhs_total": nrhs_total, "dofs_total": total_dofs,
         "solved_dofs_per_s": total_dofs / wall, "solves_per_s": len(probs) / wall,
         "rhs_solves_per_s": nrhs_total / wall, "batch_wall_ms": wall * 1e3,
@@ -630,14 +643,22 @@ def batched_record(L, problems, steps, say, colour=True):
         "latency_ms": {"poll_every_iterations": 2, "batch_wall": lat_wall * 1e3,
                        "problem_done_p50": float(np.percentile(done, 50) * 1e3),
                        "problem_done_p99": float(np.percentile(done, 99) * 1e3),
-                       "problem_done_min": float(done.min() * 1e3)},
+                       "problem_done_min": float(done.min() * 1e3),
+                       "problem_done_p10": float(np.percentile(done, 10) * 1e3)},
+        # a problem is done when its slowest RHS is: inner FP32 iterations (all refinement steps)
+        # of that RHS per problem, which set the completion times above
+        "problem_inner_iterations": {q: float(np.percentile([max(r.iterations for r in rr) for rr in lat_res], v))
+                                     for q, v in (("min", 0), ("p10", 10), ("p50", 50), ("max", 100))},
         "sequential_fp64": {"wall_ms": seq_t * 1e3, "dofs_per_s": total_dofs / seq_t,
                             "speedup_of_batch": seq_t / wall, "iterations": seq_it},
         "verification": {"all_converged": ok, "worst_true_rel_res_fp64": worst},
         "autotuned": {"block": st["block"], "grid_mult": st["grid_mult"]},
         "batch_apply": {"ms": apply_ms, "alg_bytes": apply_bytes_total,
                         "gbs": apply_bytes_total / (apply_ms * 1e-3) / 1e9,
-                        "frac": apply_bytes_total / (apply_ms * 1e-3) / 1e9 / HBM_PEAK_GBS},
+                        "frac": apply_bytes_total / (apply_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                        "full_matrix_bytes": apply_bytes_full,
+                        "full_matrix_equivalent_gbs": apply_bytes_full / (apply_ms * 1e-3) / 1e9,
+                        "row_sum_bytes": rowsum_bytes},
         "batch_iterations": st["iterations"],
     }
 

@@ -197,15 +197,18 @@ int lfric_op_set_window(lfric_op *op, int window);
  * the same order).  Writing the operator (upload, generate, convert, invert, assemble)
  * rebuilds the copy. */
 int lfric_op_prepare_rows(lfric_op *op);
-/* Add a symmetric-packed copy of a square lowest-order W0 operator whose every element matrix
- * is symmetric bit for bit (refused otherwise): the upper triangle of each element (36 of 64
- * entries).  levels_per_thread 0: one triangle per (cell, level) unit, the previous layer's
- * top rows passed between neighbouring threads; S > 0: segments of S levels per thread, each
- * repeating the previous segment's last triangle.  From then on the solvers' apply reads it in
- * two phases (per-cell outputs, then a colour-ordered gather) whose result equals the
- * colour-ordered scatter (lfric_lma_apply) bit for bit, reading ~57 % of the matrix bytes
- * (S = 0).  Writing the operator rebuilds the copy, or drops it if the new entries are not
- * symmetric.  No reference counterpart (the reference stores full element matrices). */
+/* Add a symmetric-packed copy of a square W0 operator of order 0 or 1 whose every element
+ * matrix is symmetric bit for bit (refused otherwise): the upper triangle of each element (36
+ * of 64 entries at order 0, 378 of 729 at order 1).  Order 0, levels_per_thread 0: one triangle
+ * per (cell, level) unit, the previous layer's top rows passed between neighbouring threads;
+ * S > 0: segments of S levels per thread, each repeating the previous segment's last triangle.
+ * Order 1: one triangle per (cell, layer) unit (levels_per_thread must be 0), the layer below's
+ * top rows passed between neighbouring threads, a unit's 18 outputs kept for the second phase.  From then on the gather apply (lfric_lma_apply_gather,
+ * the solvers' apply, lfric_lma_apply_mrhs with LFRIC_MRHS_GATHER or AUTO, the batch) reads it
+ * in two phases (per-unit row sums, then a colour-ordered gather) whose result equals the
+ * colour-ordered scatter (lfric_lma_apply, LFRIC_MRHS_COLOUR) bit for bit.  Writing the
+ * operator rebuilds the copy, or drops it if the new entries are not symmetric.  No reference
+ * counterpart (the reference stores full element matrices). */
 int lfric_op_prepare_symmetric(lfric_op *op, int levels_per_thread);
 /* Round an operator into another precision (e.g. the r_solver FP32 copy of an FP64 master). */
 int lfric_op_convert(const lfric_op *src, lfric_op *dst);
@@ -497,10 +500,13 @@ int lfric_batch_remove(lfric_batch *b, int id);
 int lfric_batch_tune(lfric_batch *b, double *best_ms);
 /* use_graph 0/1 (-1 keeps), block 64/128/256 (0 keeps), grid_mult > 0 (0 keeps). */
 int lfric_batch_config(lfric_batch *b, int use_graph, int block, int grid_mult);
-/* Multi-RHS apply of the batch: 1 (default) colour-ordered where supported (FP32 W0 problems
- * with at most 8 colours: lowest order on the matrix cores as LFRIC_MRHS_MFMA, order 1 one
- * (cell, level) unit per thread as LFRIC_MRHS_COLOUR), gather form for the rest; 0 gather form
- * for every problem. */
+/* Multi-RHS apply of the batch: 2 (default) an FP32 operator's copy where it has one (the
+ * gather over the row-gathered copy of lowest-order W0, lfric_op_prepare_rows; the two phases
+ * of the symmetric-packed order-1 W0 copy, lfric_op_prepare_symmetric), else as 1; 1 colour-
+ * ordered where supported (FP32 W0 problems with at most 8 colours: lowest order on the matrix
+ * cores as LFRIC_MRHS_MFMA, order 1 one (cell, level) unit per thread as LFRIC_MRHS_COLOUR),
+ * gather form for the rest; 0 gather form for every problem.  The symmetric-packed order-1
+ * apply equals the colour-ordered one bit for bit, the row-gathered gather the gather form. */
 int lfric_batch_set_apply(lfric_batch *b, int form);
 int lfric_batch_stats(const lfric_batch *b, long long *iterations, int *block, int *grid_mult, int *ntiles);
 void lfric_batch_destroy(lfric_batch *b);

@@ -19,14 +19,27 @@ def batched_mix(n_problems, seed=0):
              int(rng.integers(0, 2**31))) for _ in range(n_problems)]
 
 
-def build_problem(n, nl, order, nrhs, seed, scaled=False):
+def prepare_batch_copies(op32):
+    """The FP32 operator's copy for the batch's fastest apply: the row-gathered copy of a
+    lowest-order operator, the symmetric-packed copy of an order-1 one."""
+    if op32.fs_to.ndf == 8:
+        op32.prepare_rows()
+    elif op32.fs_to.ndf == 27:
+        op32.prepare_symmetric()
+    return op32
+
+
+def build_problem(n, nl, order, nrhs, seed, scaled=False, copies=True):
     """One batch problem on the device: the cubed-sphere W0 space, the FP64 operator
-    (A_e = B_e B_e^T + 0.1 I from the counter-based generator) and its FP32 copy, the FP32
-    Jacobi diagonal, b ~ U(-1, 1) with nrhs columns and x = 0."""
+    (A_e = B_e B_e^T + 0.1 I from the counter-based generator) and its FP32 copy (with the
+    batch copies of prepare_batch_copies when `copies`), the FP32 Jacobi diagonal,
+    b ~ U(-1, 1) with nrhs columns and x = 0."""
     from .fem import FieldVector, FunctionSpace, LmaOperator
     fs = FunctionSpace.cubedsphere(n, nl, W0, order)
     op64 = LmaOperator(fs).generate_spd(seed, scaled=scaled)
     op32 = op64.convert(R32)
+    if copies:
+        prepare_batch_copies(op32)
     d32 = op32.mm_diagonal(FieldVector(fs, R32))
     b = FieldVector(fs, R64, nrhs).fill_uniform(seed, 3)
     x = FieldVector(fs, R64, nrhs)
@@ -91,13 +104,19 @@ class Batch:
         check(lib().lfric_batch_tune(self._h, ctypes.byref(ms)))
         return ms.value
 
-    def config(self, use_graph=-1, block=0, grid_mult=0, colour=None):
-        """use_graph 0/1, block 64/128/256, grid_mult > 0 (-1/0 keep); colour True/False: the
-        W0 problems apply in colour-ordered form (lowest order on the matrix cores; default) or
-        every problem in gather form."""
+    APPLY = {"gather": 0, "colour": 1, "copies": 2}
+
+    def config(self, use_graph=-1, block=0, grid_mult=0, colour=None, apply=None):
+        """use_graph 0/1, block 64/128/256, grid_mult > 0 (-1/0 keep).  apply: "copies"
+        (default: an operator's row-gathered or symmetric-packed copy where it has one, else the
+        colour-ordered forms), "colour" (colour-ordered forms where supported: lowest order on
+        the matrix cores, order 1 register-staged) or "gather" (every problem in gather form);
+        colour=True/False is "copies"/"gather"."""
         check(lib().lfric_batch_config(self._h, int(use_graph), int(block), int(grid_mult)))
         if colour is not None:
-            check(lib().lfric_batch_set_apply(self._h, int(bool(colour))))
+            apply = "copies" if colour else "gather"
+        if apply is not None:
+            check(lib().lfric_batch_set_apply(self._h, self.APPLY[apply]))
 
     def stats(self):
         it, blk, gm, nt = ctypes.c_longlong(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int()

@@ -748,10 +748,7 @@ void lfric_op_destroy(lfric_op *op) {
   if (!op) return;
   dfree(op->d, op->bytes);
   dfree(op->rows, op->rows_bytes);
-  dfree(op->sym, op->sym_bytes);
-  dfree(op->symz, op->symz_bytes);
-  dfree(op->d_symzu, op->symzu_bytes);
-  dfree(op->d_symcol, op->symcol_bytes);
+  drop_sym(op);
   delete op;
 }
 
@@ -764,14 +761,7 @@ static int refresh_rows(lfric_op *op, int rc) {
   if (op->sym) {
     const int r = launch_build_sym(op, op->sym_seg);
     if (r == LFRIC_E_UNSUPPORTED) {
-      dfree(op->sym, op->sym_bytes);
-      dfree(op->symz, op->symz_bytes);
-      dfree(op->d_symzu, op->symzu_bytes);
-      dfree(op->d_symcol, op->symcol_bytes);
-      op->sym = op->symz = nullptr;
-      op->d_symzu = nullptr;
-      op->d_symcol = nullptr;
-      op->sym_bytes = op->symz_bytes = op->symzu_bytes = op->symcol_bytes = 0;
+      drop_sym(op);
     } else if (r) {
       return r;
     }
@@ -903,13 +893,12 @@ int lfric_lma_apply(const lfric_op *op, const lfric_vec *x, lfric_vec *y) {
   return launch_lma_apply(op, x->d, y->d, nullptr);
 }
 
-// Autotuner of the apply's block order: time the apply (form 0 colour-ordered scatter, 1 gather)
-// for each XCD window and keep the fastest in the operator.
+// Average time of the gather apply's kernels over `reps` applies, with HIP events on the
+// library's stream: both phases of a symmetric-packed copy on their own, else the one kernel.
 int lfric_op_time_apply(const lfric_op *op, const lfric_vec *x, lfric_vec *y, int reps, double *phase_ms) {
   if (int rc = require_init()) return rc;
   LF_CHECK(op && x && y && phase_ms && reps > 0 && reps <= 1000, LFRIC_E_ARG, "bad arguments");
   if (int rc = check_apply(op, x, y, false)) return rc;
-  Context &c = ctx();
   std::vector<hipEvent_t> ev(static_cast<size_t>(reps) * 3, nullptr);
   int rc = LFRIC_OK;
   for (auto &e : ev)
@@ -947,6 +936,8 @@ int lfric_op_time_apply(const lfric_op *op, const lfric_vec *x, lfric_vec *y, in
   return LFRIC_OK;
 }
 
+// Autotuner of the apply's block order: time the apply (form 0 colour-ordered scatter, 1 gather)
+// for each XCD window and keep the fastest in the operator.
 int lfric_op_tune(lfric_op *op, int form, int *best_window, double *best_ms) {
   if (int rc = require_init()) return rc;
   LF_CHECK(op && (form == 0 || form == 1), LFRIC_E_ARG, "bad arguments");

@@ -23,6 +23,7 @@
 #include "kernels.h"
 #include "colour_apply.h"
 #include "mfma_apply.h"
+#include "sym27_apply.h"
 
 namespace lfric {
 
@@ -40,7 +41,7 @@ struct ProbDev {
   const int *ebase, *map2, *goff, *gent;
   const int2 *gcol;
   const float *diag;
-  float *d, *r, *z, *p, *q;
+  float *d, *r, *p, *q;
   long long n;      // undf * nrhs
   int ncols, nl, ndf, nrhs;
   int apply_tile0, vec_tile0, vec_ntiles, scal0;
@@ -50,6 +51,17 @@ struct ProbDev {
   const int *map1;
   const uint32_t *first;
   int ctile0[kMaxC], cpos0[kMaxC], cnpos[kMaxC];
+  // row-gathered copy (lowest-order W0): gather tiles over it
+  const float *rows;
+  unsigned long long gsr;
+  // symmetric-packed order-1 copy: phase-1 unit tiles in colour slot 0, phase-2 gather tiles;
+  // zs holds the per-unit row sums between the two
+  const float *sym;
+  unsigned long long gss;
+  const int *sympos;
+  const int4 *symzu;
+  float *zs;
+  long long units;
 };
 
 // per (problem, RHS) inner CG state
@@ -178,6 +190,74 @@ __global__ __launch_bounds__(kBlock) void k_batch_apply_colour27(const ProbDev *
   }
 }
 
+// q = A32 p over the row-gathered copy for the active lowest-order W0 problems of one nrhs that
+// have it, tiles of kBlock gather points [t_begin, t_end); persistent like k_batch_apply
+template <int NR>
+__global__ __launch_bounds__(kBlock) void k_batch_apply_rows(const ProbDev *__restrict__ P,
+                                                             const int *__restrict__ tile0, int nprob, int t_begin,
+                                                             int t_end, unsigned int *counter) {
+  __shared__ int s_tile;
+  for (;;) {
+    if (threadIdx.x == 0) s_tile = t_begin + static_cast<int>(atomicAdd(counter, 1u));
+    __syncthreads();
+    const int tile = __builtin_amdgcn_readfirstlane(s_tile);
+    __syncthreads();
+    if (tile >= t_end) return;
+    const ProbDev &d = P[find_problem(tile0, nprob, tile)];
+    rows_point_w0_mrhs<float, NR>(d.rows, d.gsr, d.p, d.q, d.map2, d.gcol, d.goff, d.gent, d.ncols, d.nl,
+                                  static_cast<long long>(tile - d.apply_tile0) * kBlock + threadIdx.x);
+  }
+}
+
+// phase 1 of the symmetric-packed order-1 apply (sym27_apply.h) for the active problems of one
+// nrhs that have the copy: tiles of kBlock threads (NR/NRL per unit, whole wavefronts) in
+// colour slot 0
+template <int NR>
+__global__ __launch_bounds__(kBlock) void k_batch_sym27_units(const ProbDev *__restrict__ P,
+                                                              const int *__restrict__ ctile0, int nprob, int t_begin,
+                                                              int t_end, unsigned int *counter) {
+  constexpr int NRL = s27_nrl<float, NR>();
+  __shared__ int s_tile;
+  for (;;) {
+    if (threadIdx.x == 0) s_tile = t_begin + static_cast<int>(atomicAdd(counter, 1u));
+    __syncthreads();
+    const int tile = __builtin_amdgcn_readfirstlane(s_tile);
+    __syncthreads();
+    if (tile >= t_end) return;
+    const ProbDev &d = P[find_problem(ctile0, nprob, tile)];
+    sym27_unit<float, NR, NRL>(d.sym, d.gss, d.p, d.map2, d.sympos, d.zs, d.units, d.nl,
+                               static_cast<long long>(tile - d.ctile0[0]) * kBlock + threadIdx.x);
+  }
+}
+
+// phase 2: q at each gather point from the row sums, tiles of kBlock * kSymPts points (a thread
+// takes kSymPts points kBlock apart: a point is a few loads, and one tile per kBlock points
+// would make the work counter's atomics the bottleneck)
+constexpr int kSymPts = 8;
+template <int NR>
+__global__ __launch_bounds__(kBlock) void k_batch_sym27_gather(const ProbDev *__restrict__ P,
+                                                               const int *__restrict__ tile0, int nprob, int t_begin,
+                                                               int t_end, unsigned int *counter) {
+  typedef typename RVec<float, NR>::type VR;
+  __shared__ int s_tile;
+  for (;;) {
+    if (threadIdx.x == 0) s_tile = t_begin + static_cast<int>(atomicAdd(counter, 1u));
+    __syncthreads();
+    const int tile = __builtin_amdgcn_readfirstlane(s_tile);
+    __syncthreads();
+    if (tile >= t_end) return;
+    const ProbDev &d = P[find_problem(tile0, nprob, tile)];
+    const long long t0 = static_cast<long long>(tile - d.apply_tile0) * kBlock * kSymPts + threadIdx.x;
+#pragma unroll
+    for (int j = 0; j < kSymPts; ++j) {
+      VR acc;
+      size_t dof = 0;
+      if (sym27_gather_point<float, NR>(t0 + j * kBlock, d.zs, d.gcol, d.symzu, d.ncols, d.nl, acc, dof))
+        *reinterpret_cast<VR *>(d.q + dof * NR) = acc;
+    }
+  }
+}
+
 // work counters: one per (apply group, colour)
 constexpr int kMaxGroups = 16;
 constexpr int kCounters = kMaxGroups * kMaxC;
@@ -262,7 +342,8 @@ __global__ __launch_bounds__(kFinBlock) void k_batch_alpha(const ProbDev *__rest
   }
 }
 
-// d += alpha p; r -= alpha q; z = r / diag; partials of r.r and r.z
+// d += alpha p; r -= alpha q; partials of r.r and r.z with z = r / diag (z is formed again
+// where it is used, in k_batch_aypx, rather than stored)
 __global__ __launch_bounds__(kVecBlock) void k_batch_update(const ProbDev *__restrict__ P,
                                                             const int *__restrict__ vtile0, int nprob,
                                                             const RhsDev *__restrict__ S, double *__restrict__ partials) {
@@ -282,7 +363,6 @@ __global__ __launch_bounds__(kVecBlock) void k_batch_update(const ProbDev *__res
       ri = ri + ma * d->q[f];
       d->r[f] = ri;
       const float zi = ri / d->diag[f / nr];
-      d->z[f] = zi;
       v[0] += static_cast<double>(ri) * ri;
       v[1] += static_cast<double>(ri) * zi;
     }
@@ -323,7 +403,7 @@ __global__ __launch_bounds__(kFinBlock) void k_batch_beta(const ProbDev *__restr
 }
 
 
-// p = z + beta p for the live RHS
+// p = z + beta p for the live RHS, z = r / diag
 __global__ __launch_bounds__(kVecBlock) void k_batch_aypx(const ProbDev *__restrict__ P, const int *__restrict__ vtile0,
                                                           int nprob, const RhsDev *__restrict__ S) {
   const ProbDev *d;
@@ -333,7 +413,8 @@ __global__ __launch_bounds__(kVecBlock) void k_batch_aypx(const ProbDev *__restr
   const RhsDev &s = S[d->scal0 + r];
   if (s.done) return;
   const float b = static_cast<float>(s.beta);
-  for (long long f = f0 + threadIdx.x; f < f1; f += kVecBlock) d->p[f] = b * d->p[f] + d->z[f];
+  const int nr = d->nrhs;
+  for (long long f = f0 + threadIdx.x; f < f1; f += kVecBlock) d->p[f] = b * d->p[f] + d->r[f] / d->diag[f / nr];
 }
 
 // ---- refinement step kernels (one problem at a time) ---------------------------------------
@@ -370,7 +451,7 @@ __global__ __launch_bounds__(kFinBlock) void k_ir_norms(const double *__restrict
 // partials of r.r and r.z per RHS (done RHS get zeros)
 __global__ __launch_bounds__(kVecBlock) void k_ir_start(const double *__restrict__ r64, const double *__restrict__ inv,
                                                         float *__restrict__ d, float *__restrict__ r,
-                                                        float *__restrict__ z, float *__restrict__ p,
+                                                        float *__restrict__ p,
                                                         const float *__restrict__ diag, long long n, int nr,
                                                         double *__restrict__ partials) {
   double v[2] = {0.0, 0.0};
@@ -380,7 +461,6 @@ __global__ __launch_bounds__(kVecBlock) void k_ir_start(const double *__restrict
     const float zi = ri / diag[f / nr];
     d[f] = 0.f;
     r[f] = ri;
-    z[f] = zi;
     p[f] = zi;
     v[0] += static_cast<double>(ri) * ri;
     v[1] += static_cast<double>(ri) * zi;
@@ -419,9 +499,11 @@ struct Problem {
   int nrhs = 1;
   long long n = 0;
   int scal0 = 0;  // first of this problem's nrhs entries of the batch's RhsDev array
-  float *d = nullptr, *r = nullptr, *z = nullptr, *p = nullptr, *q = nullptr;
+  float *d = nullptr, *r = nullptr, *p = nullptr, *q = nullptr;  // z = r / diag is not stored
   double *r64 = nullptr;
   size_t bytes32 = 0, bytes64 = 0;
+  float *zs = nullptr;  // per-unit row sums of the symmetric-packed order-1 apply
+  size_t zs_bytes = 0;
   // refinement state per RHS
   std::vector<double> nr0, nr;
   std::vector<int> conv, inner_iters;
@@ -445,12 +527,14 @@ struct lfric_batch {
   int *d_tile0 = nullptr, *d_vtile0 = nullptr, *d_ctile0 = nullptr;
   size_t cap_desc = 0, cap_tile0 = 0, cap_vtile0 = 0, cap_ctile0 = 0, cap_partials = 0;
   double *d_partials = nullptr, *d_scratch = nullptr, *h_scratch = nullptr;
+  size_t scratch_slots = 0;  // problems whose refinement step the scratch holds at once
   unsigned int *d_counter = nullptr;
   std::vector<lfric::Problem *> active;
   int ntiles = 0, nvtiles = 0;
-  // apply groups of equal (ndf, nrhs, form): gather tiles [begin, end), or for the colour-
-  // ordered forms (1 matrix cores, 2 register-staged FMA) tiles [cbegin[c], cend[c]) of each
-  // colour c
+  // apply groups of equal (ndf, nrhs, form): gather tiles [begin, end) (forms 0, 3 and the
+  // second phase of 4), colour tiles [cbegin[c], cend[c]) of each colour c of the colour-
+  // ordered forms (1 matrix cores, 2 register-staged FMA), the first phase's unit tiles of form
+  // 4 in [cbegin[0], cend[0])
   struct Group {
     int ndf, nrhs, form, begin, end;
     int cbegin[lfric::kMaxC], cend[lfric::kMaxC];
@@ -462,7 +546,7 @@ struct lfric_batch {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   int use_graph = 1;
-  int use_colour = 1;  // colour-ordered apply where supported (batch_colour_form)
+  int use_colour = 2;  // apply forms (lfric_batch_set_apply): copies, then colour-ordered forms
   // launch shape of the apply (autotuned)
   int block = 256, grid_mult = 2, ncu = 256;
   long long replays = 0, iterations = 0;
@@ -500,14 +584,38 @@ int apply_tiles(const Problem &p, int block) {
 }
 
 // the problem's apply form in the batch: 0 gather, 1 colour-ordered on the matrix cores, 2
-// colour-ordered with register-staged x
+// colour-ordered with register-staged x, 3 gather over the row-gathered copy, 4 the two phases
+// of the symmetric-packed order-1 copy.  B.use_colour: 0 gather for every problem, 1 the
+// colour-ordered forms where supported, 2 (default) the operator's copies where it has one,
+// else as 1.
+constexpr int kFormRows = 3, kFormSym27 = 4;
+
+bool is_lowest_w0_fp32(const lfric_op *op) { return mfma_apply_supported(op); }
+
 int apply_form(const lfric_batch &B, const Problem &p) {
-  if (!B.use_colour || p.A32->to->ncolours > kMaxC) return 0;
+  if (!B.use_colour) return 0;
+  if (B.use_colour == 2) {
+    if (p.A32->rows && is_lowest_w0_fp32(p.A32)) return kFormRows;
+    if (p.A32->precision == LFRIC_R32 && sym27_ready(p.A32)) return kFormSym27;
+  }
+  if (p.A32->to->ncolours > kMaxC) return 0;
   return batch_colour_form(p.A32);
 }
 
+bool colour_form(int form) { return form == 1 || form == 2; }
+
 // (cell, level) units per tile of a colour-ordered form
 int units_per_tile(int form) { return form == 1 ? kBlock / 4 : kBlock; }
+
+// threads of the symmetric-packed order-1 apply's first phase
+long long sym27_threads(int nrhs, long long units) {
+  switch (nrhs) {
+    case 1: return s27_threads<float, 1>(units);
+    case 2: return s27_threads<float, 2>(units);
+    case 4: return s27_threads<float, 4>(units);
+    default: return s27_threads<float, 8>(units);
+  }
+}
 
 // state entries for a new problem (reusing a freed range of the same size), growing the array
 // with a device copy when needed
@@ -548,6 +656,12 @@ int rebuild(lfric_batch &B) {
   B.active.clear();
   for (auto &u : B.probs)
     if (u->inner_running) B.active.push_back(u.get());
+  for (Problem *p : B.active)
+    if (apply_form(B, *p) == kFormSym27 && !p->zs) {
+      const size_t need = sizeof(float) * s27_zalloc(p->A32->to->ncells, p->A32->to->nlayers, p->nrhs);
+      if (int rc = dmalloc(reinterpret_cast<void **>(&p->zs), need)) return rc;
+      p->zs_bytes = need;
+    }
   // order-1 problems first (their own apply kernel), then the gather-form and matrix-core groups,
   // heaviest tiles first within each kind
   std::stable_sort(B.active.begin(), B.active.end(), [&B](const Problem *a, const Problem *b) {
@@ -577,7 +691,6 @@ int rebuild(lfric_batch &B) {
     d.diag = static_cast<const float *>(p.diag->d);
     d.d = p.d;
     d.r = p.r;
-    d.z = p.z;
     d.p = p.p;
     d.q = p.q;
     d.n = p.n;
@@ -589,18 +702,35 @@ int rebuild(lfric_batch &B) {
     mf[i] = apply_form(B, p);
     d.map1 = to.d_map;
     d.first = to.d_first;
+    if (mf[i] == kFormRows) {
+      d.rows = static_cast<const float *>(p.A32->rows);
+      d.gsr = p.A32->gsr;
+    }
+    if (mf[i] == kFormSym27) {
+      d.sym = static_cast<const float *>(p.A32->sym);
+      d.gss = p.A32->gss;
+      d.sympos = p.A32->d_sympos;
+      d.symzu = p.A32->d_symzu;
+      d.zs = p.zs;
+      d.units = static_cast<long long>(to.ncells) * to.nlayers;
+    }
     for (int c = 0; c < kMaxC; ++c) {
       int *row = ctile0.data() + static_cast<size_t>(c) * (np + 1);
-      const bool has = mf[i] && c < to.ncolours;
+      const bool has = colour_form(mf[i]) && c < to.ncolours;
       d.cpos0[c] = has ? to.colour_start[c] : 0;
       d.cnpos[c] = has ? to.colour_start[c + 1] - to.colour_start[c] : 0;
       d.ctile0[c] = row[i];
-      const long long units = static_cast<long long>(d.cnpos[c]) * (to.nlayers + 1);
-      const int upt = units_per_tile(mf[i]);
-      row[i + 1] = row[i] + static_cast<int>((units + upt - 1) / upt);
+      long long threads = static_cast<long long>(d.cnpos[c]) * (to.nlayers + 1);
+      int upt = units_per_tile(mf[i]);
+      if (mf[i] == kFormSym27 && c == 0) {  // phase-1 tiles (whole wavefronts)
+        threads = sym27_threads(p.nrhs, d.units);
+        upt = kBlock;
+      }
+      row[i + 1] = row[i] + static_cast<int>((threads + upt - 1) / upt);
     }
     d.apply_tile0 = tile0[i];
-    tile0[i + 1] = tile0[i] + (mf[i] ? 0 : apply_tiles(p, B.block));
+    const int gtile = mf[i] == kFormSym27 ? kBlock * kSymPts : mf[i] == kFormRows ? kBlock : B.block;
+    tile0[i + 1] = tile0[i] + (colour_form(mf[i]) ? 0 : apply_tiles(p, gtile));
     d.vec_tile0 = vtile0[i];
     d.vec_ntiles = static_cast<int>((p.n + kVecTile - 1) / kVecTile);
     vtile0[i + 1] = vtile0[i] + d.vec_ntiles;
@@ -688,10 +818,63 @@ void launch_group_colour(lfric_batch &B, int g, int grid_mult) {
   }
 }
 
+// the gather tiles of a group over its row-gathered copies
+void launch_group_rows(lfric_batch &B, int g, int cap) {
+  const auto &G = B.groups[g];
+  if (G.end == G.begin) return;
+  const int np = B.np_tables, grid = std::min(G.end - G.begin, cap);
+  unsigned int *cnt = B.d_counter + g * kMaxC;
+#define ROWS_LAUNCH(K) LAUNCHB(K, grid, kBlock, B.d_desc, B.d_tile0, np, G.begin, G.end, cnt)
+  switch (G.nrhs) {
+    case 1: ROWS_LAUNCH(k_batch_apply_rows<1>); break;
+    case 2: ROWS_LAUNCH(k_batch_apply_rows<2>); break;
+    case 4: ROWS_LAUNCH(k_batch_apply_rows<4>); break;
+    default: ROWS_LAUNCH(k_batch_apply_rows<8>); break;
+  }
+#undef ROWS_LAUNCH
+}
+
+// one phase of a symmetric-packed order-1 group: 1 the unit tiles (colour slot 0 of the
+// colour tile table), 2 the gather tiles
+void launch_group_sym27(lfric_batch &B, int g, int phase, int cap) {
+  const auto &G = B.groups[g];
+  const int np = B.np_tables;
+  const int b0 = phase == 1 ? G.cbegin[0] : G.begin, b1 = phase == 1 ? G.cend[0] : G.end;
+  if (b1 == b0) return;
+  const int grid = std::min(b1 - b0, cap);
+  unsigned int *cnt = B.d_counter + g * kMaxC + (phase == 1 ? 0 : 1);
+  const int *tab = phase == 1 ? B.d_ctile0 : B.d_tile0;
+#define SYM_LAUNCH(K) LAUNCHB(K, grid, kBlock, B.d_desc, tab, np, b0, b1, cnt)
+  if (phase == 1) {
+    switch (G.nrhs) {
+      case 1: SYM_LAUNCH(k_batch_sym27_units<1>); break;
+      case 2: SYM_LAUNCH(k_batch_sym27_units<2>); break;
+      case 4: SYM_LAUNCH(k_batch_sym27_units<4>); break;
+      default: SYM_LAUNCH(k_batch_sym27_units<8>); break;
+    }
+  } else {
+    switch (G.nrhs) {
+      case 1: SYM_LAUNCH(k_batch_sym27_gather<1>); break;
+      case 2: SYM_LAUNCH(k_batch_sym27_gather<2>); break;
+      case 4: SYM_LAUNCH(k_batch_sym27_gather<4>); break;
+      default: SYM_LAUNCH(k_batch_sym27_gather<8>); break;
+    }
+  }
+#undef SYM_LAUNCH
+}
+
 template <int BS>
 void launch_apply_bs(lfric_batch &B, int grid_mult) {
-  const int cap = grid_mult * B.ncu * (1024 / BS);
-  for (int g = 0; g < static_cast<int>(B.groups.size()); ++g) {
+  const int cap = grid_mult * B.ncu * (1024 / BS), capk = grid_mult * B.ncu * (1024 / kBlock);
+  const int ng = static_cast<int>(B.groups.size());
+  for (int g = 0; g < ng; ++g)
+    if (B.groups[g].form == kFormSym27) launch_group_sym27(B, g, 1, capk);
+  for (int g = 0; g < ng; ++g) {
+    if (B.groups[g].form == kFormSym27) continue;
+    if (B.groups[g].form == kFormRows) {
+      launch_group_rows(B, g, capk);
+      continue;
+    }
     if (B.groups[g].form) {
       launch_group_colour(B, g, grid_mult);
       continue;
@@ -702,6 +885,8 @@ void launch_apply_bs(lfric_batch &B, int grid_mult) {
     else
       launch_group<BS, 8>(B, g, cap);
   }
+  for (int g = 0; g < ng; ++g)
+    if (B.groups[g].form == kFormSym27) launch_group_sym27(B, g, 2, capk);
 }
 
 void launch_apply(lfric_batch &B, int block, int grid_mult) {
@@ -753,64 +938,102 @@ int run_iterations(lfric_batch &B, int k) {
 // FP64 refinement step of one problem (the outer loop of the mixed-precision CG): fold the
 // scaled inner solution into x, take b - A64 x in FP64, test each RHS, and start the next
 // inner solve on r / ||r|| for the RHS that still need one
-int refine(lfric_batch &B, Problem &p) {
-  const int nr = p.nrhs;
-  const long long n = p.n;
-  const int vg = grid_for(n, kBlock, 8192);
-  double *scale = B.d_scratch, *inv = B.d_scratch + kMaxR, *tol = B.d_scratch + 2 * kMaxR;
-  int *live = reinterpret_cast<int *>(B.d_scratch + 3 * kMaxR);
-  double *nrm = B.d_scratch + 4 * kMaxR;
-  if (p.started) {
-    for (int r = 0; r < nr; ++r) B.h_scratch[r] = p.conv[r] ? 0.0 : p.nr[r];
-    LF_HIP(hipMemcpyAsync(scale, B.h_scratch, sizeof(double) * nr, hipMemcpyHostToDevice, cur_stream()));
-    LAUNCHB(k_ir_accumulate, vg, kBlock, static_cast<double *>(p.x->d), p.d, scale, n, nr);
-  }
-  if (int rc = launch_lma_apply_mrhs(p.A64, p.x->d, p.r64, nr, LFRIC_MRHS_AUTO)) return rc;
-  LAUNCHB(k_ir_residual, kIrBlocks, kVecBlock, p.r64, static_cast<const double *>(p.b->d), n, nr, B.d_partials);
-  LAUNCHB(k_ir_norms, 1, kFinBlock, B.d_partials, kIrBlocks, nr, nrm);
-  LF_HIP(hipMemcpyAsync(B.h_scratch + 4 * kMaxR, nrm, sizeof(double) * nr, hipMemcpyDeviceToHost, cur_stream()));
+// per problem of a refinement pass: scale, 1/||r||, inner tolerance, live flags (as ints) and
+// the residual norms, kMaxR each, in the batch's device and pinned host scratch
+constexpr int kSlot = 5 * kMaxR;
+
+int ensure_slots(lfric_batch &B, size_t n) {
+  if (n <= B.scratch_slots) return LFRIC_OK;
+  const size_t slots = std::max<size_t>(n, 2 * B.scratch_slots);
   LF_HIP(hipStreamSynchronize(cur_stream()));
-  bool all = true, nan = false;
-  for (int r = 0; r < nr; ++r) {
-    const double v = B.h_scratch[4 * kMaxR + r];
-    if (!p.started) p.nr0[r] = v;
-    if (!p.conv[r]) p.nr[r] = v;
-    if (v != v) nan = true;
-    if (!p.conv[r] && (v <= B.opts.r_tol * p.nr0[r] || v <= B.opts.a_tol || p.nr0[r] < kEps)) p.conv[r] = 1;
-    all = all && p.conv[r];
+  dfree(B.d_scratch, sizeof(double) * kSlot * B.scratch_slots);
+  if (B.h_scratch) (void)hipHostFree(B.h_scratch);
+  B.d_scratch = B.h_scratch = nullptr;
+  B.scratch_slots = 0;
+  if (int rc = dmalloc(reinterpret_cast<void **>(&B.d_scratch), sizeof(double) * kSlot * slots)) return rc;
+  if (hipHostMalloc(reinterpret_cast<void **>(&B.h_scratch), sizeof(double) * kSlot * slots, hipHostMallocDefault) !=
+      hipSuccess) {
+    dfree(B.d_scratch, sizeof(double) * kSlot * slots);
+    B.d_scratch = nullptr;
+    return fail(LFRIC_E_NOMEM, "pinned host scratch");
   }
-  if (p.started) ++p.outer;
-  p.started = true;
-  if (nan || all || p.outer >= B.opts.max_outer) {
-    p.status = nan ? LFRIC_NAN : all ? LFRIC_OK : LFRIC_NOT_CONVERGED;
-    p.finished = true;
-    if (p.inner_running) B.dirty = true;
-    p.inner_running = false;
-    return LFRIC_OK;
+  B.scratch_slots = slots;
+  return LFRIC_OK;
+}
+
+// The FP64 refinement step of the problems in ps (each the same as on its own): add the scaled
+// inner solution into x, take b - A64 x in FP64 and its norm per RHS, then either retire the
+// problem or restart its inner solve on r/||r||.  Every problem's kernels are queued before the
+// host waits, twice in all.
+int refine_all(lfric_batch &B, const std::vector<Problem *> &ps) {
+  if (ps.empty()) return LFRIC_OK;
+  if (int rc = ensure_slots(B, ps.size())) return rc;
+  for (size_t i = 0; i < ps.size(); ++i) {
+    Problem &p = *ps[i];
+    double *d = B.d_scratch + i * kSlot, *h = B.h_scratch + i * kSlot;
+    const int nr = p.nrhs;
+    const long long n = p.n;
+    if (p.started) {
+      for (int r = 0; r < nr; ++r) h[r] = p.conv[r] ? 0.0 : p.nr[r];
+      LF_HIP(hipMemcpyAsync(d, h, sizeof(double) * nr, hipMemcpyHostToDevice, cur_stream()));
+      LAUNCHB(k_ir_accumulate, grid_for(n, kBlock, 8192), kBlock, static_cast<double *>(p.x->d), p.d, d, n, nr);
+    }
+    if (int rc = launch_lma_apply_mrhs(p.A64, p.x->d, p.r64, nr, LFRIC_MRHS_AUTO)) return rc;
+    LAUNCHB(k_ir_residual, kIrBlocks, kVecBlock, p.r64, static_cast<const double *>(p.b->d), n, nr, B.d_partials);
+    LAUNCHB(k_ir_norms, 1, kFinBlock, B.d_partials, kIrBlocks, nr, d + 4 * kMaxR);
+    LF_HIP(hipMemcpyAsync(h + 4 * kMaxR, d + 4 * kMaxR, sizeof(double) * nr, hipMemcpyDeviceToHost, cur_stream()));
   }
-  double hv[4 * kMaxR] = {};
-  for (int r = 0; r < nr; ++r) {
-    hv[kMaxR + r] = p.conv[r] ? 0.0 : 1.0 / p.nr[r];
-    hv[2 * kMaxR + r] = std::max(B.opts.inner_r_tol, 0.5 * B.opts.r_tol * p.nr0[r] / p.nr[r]);
-    reinterpret_cast<int *>(&hv[3 * kMaxR])[r] = p.conv[r] ? 0 : 1;
+  LF_HIP(hipStreamSynchronize(cur_stream()));
+  for (size_t i = 0; i < ps.size(); ++i) {
+    Problem &p = *ps[i];
+    double *d = B.d_scratch + i * kSlot, *h = B.h_scratch + i * kSlot;
+    const int nr = p.nrhs;
+    bool all = true, nan = false;
+    for (int r = 0; r < nr; ++r) {
+      const double v = h[4 * kMaxR + r];
+      if (!p.started) p.nr0[r] = v;
+      if (!p.conv[r]) p.nr[r] = v;
+      if (v != v) nan = true;
+      if (!p.conv[r] && (v <= B.opts.r_tol * p.nr0[r] || v <= B.opts.a_tol || p.nr0[r] < kEps)) p.conv[r] = 1;
+      all = all && p.conv[r];
+    }
+    if (p.started) ++p.outer;
+    p.started = true;
+    if (nan || all || p.outer >= B.opts.max_outer) {
+      p.status = nan ? LFRIC_NAN : all ? LFRIC_OK : LFRIC_NOT_CONVERGED;
+      p.finished = true;
+      if (p.inner_running) B.dirty = true;
+      p.inner_running = false;
+      continue;
+    }
+    double *inv = d + kMaxR, *tol = d + 2 * kMaxR;
+    int *live = reinterpret_cast<int *>(d + 3 * kMaxR);
+    for (int r = 0; r < 3 * kMaxR; ++r) h[kMaxR + r] = 0.0;
+    for (int r = 0; r < nr; ++r) {
+      h[kMaxR + r] = p.conv[r] ? 0.0 : 1.0 / p.nr[r];
+      h[2 * kMaxR + r] = std::max(B.opts.inner_r_tol, 0.5 * B.opts.r_tol * p.nr0[r] / p.nr[r]);
+      reinterpret_cast<int *>(h + 3 * kMaxR)[r] = p.conv[r] ? 0 : 1;
+    }
+    LF_HIP(hipMemcpyAsync(inv, h + kMaxR, sizeof(double) * 3 * kMaxR, hipMemcpyHostToDevice, cur_stream()));
+    LAUNCHB(k_ir_start, kIrBlocks, kVecBlock, p.r64, inv, p.d, p.r, p.p, static_cast<const float *>(p.diag->d),
+            p.n, nr, B.d_partials);
+    LAUNCHB(k_ir_start_fin, 1, kFinBlock, B.d_partials, kIrBlocks, nr, B.d_state + p.scal0, tol, live);
+    if (!p.inner_running) B.dirty = true;
+    p.inner_running = true;
   }
-  std::memcpy(B.h_scratch, hv, sizeof(hv));
-  LF_HIP(hipMemcpyAsync(B.d_scratch, B.h_scratch, sizeof(hv), hipMemcpyHostToDevice, cur_stream()));
-  LAUNCHB(k_ir_start, kIrBlocks, kVecBlock, p.r64, inv, p.d, p.r, p.z, p.p, static_cast<const float *>(p.diag->d), n,
-          nr, B.d_partials);
-  LAUNCHB(k_ir_start_fin, 1, kFinBlock, B.d_partials, kIrBlocks, nr, B.d_state + p.scal0, tol, live);
   LF_HIP(hipGetLastError());
   LF_HIP(hipStreamSynchronize(cur_stream()));
-  if (!p.inner_running) B.dirty = true;
-  p.inner_running = true;
   return LFRIC_OK;
 }
 
 void free_problem(lfric_batch &B, Problem &p) {
   dfree(p.d, p.bytes32);
   dfree(p.r64, p.bytes64);
+  dfree(p.zs, p.zs_bytes);
   p.d = nullptr;
   p.r64 = nullptr;
+  p.zs = nullptr;
+  p.zs_bytes = 0;
   if (p.scal0 >= 0) B.free_ranges.push_back({p.scal0, p.nrhs});
   p.scal0 = -1;
 }
@@ -824,9 +1047,10 @@ Problem *find(lfric_batch &B, int id) {
 int run_rounds(lfric_batch &B, int rounds, int *nactive) {
   const int k = std::max(1, B.opts.check_every);
   for (int round = 0; round < rounds; ++round) {
+    std::vector<Problem *> due;
     for (auto &u : B.probs)  // problems added since the last round: first residual
-      if (!u->started && !u->finished)
-        if (int rc = refine(B, *u)) return rc;
+      if (!u->started && !u->finished) due.push_back(u.get());
+    if (int rc = refine_all(B, due)) return rc;
     if (B.dirty)
       if (int rc = rebuild(B)) return rc;
     if (B.active.empty()) break;
@@ -834,15 +1058,17 @@ int run_rounds(lfric_batch &B, int rounds, int *nactive) {
     LF_HIP(hipMemcpyAsync(B.h_state, B.d_state, sizeof(RhsDev) * B.used_state, hipMemcpyDeviceToHost,
                           cur_stream()));
     LF_HIP(hipStreamSynchronize(cur_stream()));
-    for (Problem *p : std::vector<Problem *>(B.active)) {
+    due.clear();
+    for (Problem *p : B.active) {
       bool ended = true;
       for (int r = 0; r < p->nrhs; ++r)
         if (!p->conv[r] && !B.h_state[p->scal0 + r].done) ended = false;
       if (!ended) continue;
       for (int r = 0; r < p->nrhs; ++r)
         if (!p->conv[r]) p->inner_iters[r] += B.h_state[p->scal0 + r].iters;
-      if (int rc = refine(B, *p)) return rc;
+      due.push_back(p);
     }
+    if (int rc = refine_all(B, due)) return rc;
   }
   if (nactive) {
     int a = 0;
@@ -867,14 +1093,11 @@ int lfric_batch_create(const lfric_solver_opts *opts, lfric_batch **out) {
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, ctx().device) == hipSuccess) B->ncu = prop.multiProcessorCount;
   int rc = dmalloc(reinterpret_cast<void **>(&B->d_counter), sizeof(unsigned int) * kCounters);
-  if (!rc) rc = dmalloc(reinterpret_cast<void **>(&B->d_scratch), sizeof(double) * 8 * kMaxR);
+  if (!rc) rc = ensure_slots(*B, 8);
   // the refinement kernels use kIrBlocks partials before any table exists
   if (!rc)
     rc = ensure(reinterpret_cast<void **>(&B->d_partials), B->cap_partials, static_cast<size_t>(kIrBlocks) * 2 * kMaxR,
                 sizeof(double));
-  if (!rc && hipHostMalloc(reinterpret_cast<void **>(&B->h_scratch), sizeof(double) * 8 * kMaxR,
-                           hipHostMallocDefault) != hipSuccess)
-    rc = fail(LFRIC_E_NOMEM, "pinned host scratch");
   if (rc) {
     lfric_batch_destroy(B);
     return rc;
@@ -906,7 +1129,7 @@ int lfric_batch_add(lfric_batch *B, const lfric_op *A64, const lfric_op *A32, co
   p->b = b;
   p->nrhs = nr;
   p->n = x->n;
-  p->bytes32 = sizeof(float) * static_cast<size_t>(p->n) * 5;
+  p->bytes32 = sizeof(float) * static_cast<size_t>(p->n) * 4;
   p->bytes64 = sizeof(double) * static_cast<size_t>(p->n);
   p->scal0 = -1;
   if (int rc = dmalloc(reinterpret_cast<void **>(&p->d), p->bytes32)) return rc;
@@ -919,8 +1142,7 @@ int lfric_batch_add(lfric_batch *B, const lfric_op *A64, const lfric_op *A32, co
     return rc;
   }
   p->r = p->d + p->n;
-  p->z = p->r + p->n;
-  p->p = p->z + p->n;
+  p->p = p->r + p->n;
   p->q = p->p + p->n;
   p->nr0.assign(nr, 0.0);
   p->nr.assign(nr, 0.0);
@@ -1044,7 +1266,7 @@ int lfric_batch_config(lfric_batch *B, int use_graph, int block, int grid_mult) 
 }
 
 int lfric_batch_set_apply(lfric_batch *B, int form) {
-  LF_CHECK(B && (form == 0 || form == 1), LFRIC_E_ARG, "bad arguments");
+  LF_CHECK(B && form >= 0 && form <= 2, LFRIC_E_ARG, "bad arguments");
   B->use_colour = form;
   B->dirty = true;
   return LFRIC_OK;
@@ -1070,7 +1292,7 @@ void lfric_batch_destroy(lfric_batch *B) {
   dfree(B->d_tile0, B->cap_tile0 * sizeof(int));
   dfree(B->d_vtile0, B->cap_vtile0 * sizeof(int));
   dfree(B->d_partials, B->cap_partials * sizeof(double));
-  dfree(B->d_scratch, sizeof(double) * 8 * kMaxR);
+  dfree(B->d_scratch, sizeof(double) * kSlot * B->scratch_slots);
   dfree(B->d_counter, sizeof(unsigned int) * kCounters);
   dfree(B->d_ctile0, B->cap_ctile0 * sizeof(int));
   if (B->h_scratch) (void)hipHostFree(B->h_scratch);

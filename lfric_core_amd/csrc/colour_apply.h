@@ -134,5 +134,73 @@ __device__ inline void colour_unit_mrhs(const T *__restrict__ A, size_t gs, long
   }
 }
 
+// The row-gathered copy of a square operator whose dof columns each take rows from at most
+// kRowsKmax cells: the rows that gather point t = (column, level m) reads are stored together,
+// slot s = 2k + h holding, for the k-th cell of the column's list, its top-partner row of layer
+// m-1 (h = 0) or its bottom row of layer m (h = 1), each padded to RS entries (whole chunks).
+// Entry j of slot s of point t: (t/64)*gsr + ((s*RS + j)/V*64 + t%64)*V + (s*RS + j)%V.
+constexpr int kRowsKmax = 4;
+template <typename T, int N2>
+constexpr int rows_rs() {
+  return (N2 + chunk_entries<T>() - 1) / chunk_entries<T>() * chunk_entries<T>();
+}
+
+// Gather point t = (column, level m) of a lowest-order W0 operator over its row-gathered copy,
+// NR right-hand sides: per cell of the column's list the top-partner row of layer m-1, then the
+// bottom row of layer m, all in one fma chain per RHS (the gather form's order).  The cell's
+// top vertex j+4 lies one level above its bottom vertex j, so the x entries are the cell's 4
+// columns at levels m-1, m and m+1, each loaded once.  Nothing past the end.
+template <typename T, int NR>
+__device__ inline void rows_point_w0_mrhs(const T *__restrict__ rows, size_t gsr, const T *__restrict__ x,
+                                          T *__restrict__ y, const int *__restrict__ map2,
+                                          const int2 *__restrict__ cols, const int *__restrict__ off,
+                                          const int *__restrict__ ent, int ncols, int nl, long long t) {
+  typedef typename RVec<T, NR>::type VR;
+  constexpr int V = chunk_entries<T>(), RS = rows_rs<T, 8>();
+  typedef T VecT __attribute__((ext_vector_type(V)));
+  const int H = nl + 1;
+  if (t >= static_cast<long long>(ncols) * H) return;
+  const int col = static_cast<int>(t / H);
+  const int m = static_cast<int>(t - static_cast<long long>(col) * H);
+  const int2 ci = cols[col];
+  if (m == nl && !ci.y) return;
+  const bool has_m = m < nl, has_p = m > 0;
+  const int q0 = off[col], cnt = off[col + 1] - q0;
+  const T *Rt = rows + static_cast<size_t>(t >> 6) * gsr + static_cast<size_t>(t & 63) * V;
+  VR acc = {};
+#pragma unroll
+  for (int k = 0; k < kRowsKmax; ++k) {
+    if (k >= cnt) break;
+    const int pos = ent[q0 + k] >> 5;
+    int b[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) b[j] = map2[static_cast<size_t>(pos) * 8 + j];
+    VR xl[3][4];  // x of the cell's 4 columns at levels m-1, m, m+1
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      xl[0][j] = has_p ? *reinterpret_cast<const VR *>(x + (static_cast<size_t>(b[j]) + m - 1) * NR) : VR{};
+      xl[1][j] = *reinterpret_cast<const VR *>(x + (static_cast<size_t>(b[j]) + m) * NR);
+      xl[2][j] = has_m ? *reinterpret_cast<const VR *>(x + (static_cast<size_t>(b[j]) + m + 1) * NR) : VR{};
+    }
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      if (h == 0 ? !has_p : !has_m) continue;
+#pragma unroll
+      for (int c = 0; c < RS / V; ++c) {
+        const VecT v = __builtin_nontemporal_load(
+            reinterpret_cast<const VecT *>(Rt + static_cast<size_t>((2 * k + h) * (RS / V) + c) * 64 * V));
+#pragma unroll
+        for (int q = 0; q < V; ++q) {
+          const int j = c * V + q;
+          const VR &xv = j < 4 ? xl[h][j] : xl[h + 1][j - 4];
+#pragma unroll
+          for (int r = 0; r < NR; ++r) acc[r] = fma(v[q], xv[r], acc[r]);
+        }
+      }
+    }
+  }
+  *reinterpret_cast<VR *>(y + (static_cast<size_t>(ci.x) + m) * NR) = acc;
+}
+
 }  // namespace
 }  // namespace lfric

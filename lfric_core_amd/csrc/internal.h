@@ -148,6 +148,13 @@ struct lfric_op {
   size_t symzu_bytes = 0;
   int4 *d_symcol = nullptr;    // one-unit-per-thread form: the k-th cell's 4 bottom vertex column bases
   size_t symcol_bytes = 0;
+  // order-1 W0 copy: symz holds one right-hand side's per-unit row sums, d_symzu the gather
+  // columns' cells as pos*18 + (slot of dof i), d_sympos the k-th cell's colour-major position; symz_mr the
+  // row sums of several right-hand sides, grown on first use
+  int *d_sympos = nullptr;
+  size_t sympos_bytes = 0;
+  mutable void *symz_mr = nullptr;
+  mutable size_t symz_mr_bytes = 0;
   void *d = nullptr;           // level-aligned AoSoA layout (see kernels.hip)
   size_t bytes = 0;
 };

@@ -55,6 +55,10 @@ int launch_lma_apply_solver(const lfric_op *op, const void *x, void *y, const in
 int launch_lma_apply_mrhs(const lfric_op *op, const void *x, void *y, int nrhs, int variant);
 int launch_op_invert_local(const lfric_op *A, lfric_op *Ainv);
 int launch_build_sym(lfric_op *op, int levels_per_thread);
+// free the operator's symmetric-packed copy and its buffers (if any)
+void drop_sym(lfric_op *op);
+// the operator's symmetric-packed copy is the order-1 W0 kind (sym27_apply.h)
+bool sym27_ready(const lfric_op *op);
 int launch_assemble_mass_w0(lfric_op *op, const double *chi1, const double *chi2, const double *chi3, int nqp);
 // tridiagonal solve of every column of a square columnwise operator (columnwise_op_appinv)
 int launch_cma_appinv(const lfric_cma *A, const void *x, void *y);

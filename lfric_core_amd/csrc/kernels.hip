@@ -33,6 +33,7 @@
 #include "colour_apply.h"
 #include "mfma_apply.h"
 #include "rng.h"
+#include "sym27_apply.h"
 
 namespace lfric {
 
@@ -676,11 +677,7 @@ __global__ __launch_bounds__(kBlock) void k_lma_gather(const T *__restrict__ A, 
 // wavefront reads every chunk of its 64 threads as one contiguous block and the whole copy is
 // streamed once in order, as the cell layout is by the colour-ordered kernel.  Rows that do
 // not exist (m = 0 top, m = nlayers bottom, k beyond the list) hold zeros.
-constexpr int kRowsKmax = 4;
-template <typename T, int N2>
-constexpr int rows_rs() {
-  return (N2 + chunk_entries<T>() - 1) / chunk_entries<T>() * chunk_entries<T>();
-}
+// (kRowsKmax and rows_rs: colour_apply.h)
 
 template <typename T, int N2, class R>
 __global__ __launch_bounds__(kBlock) void k_build_rows(const T *__restrict__ A, size_t gs, const int *__restrict__ ebase,
@@ -1162,6 +1159,68 @@ __global__ __launch_bounds__(kBlock) void k_sym_gather_dot(const T *__restrict__
   block_partials<1>(d, partials, lb);
 }
 
+// ---- symmetric-packed order-1 W0 apply (sym27_apply.h) ----------------------------------
+// Build: thread u = k*nl + m fills unit u's slot with the upper triangle of the k-th cell's
+// layer-m element (rows of top dofs are stored one level up in the level-aligned layout) and
+// raises *asym when an entry differs from its transpose in any bit.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_build_sym27(const T *__restrict__ A, size_t gs, const int *__restrict__ ebase,
+                                                        const int *__restrict__ slotpos, T *__restrict__ sym, size_t gss,
+                                                        long long units, int nl, uint32_t row_top,
+                                                        int *__restrict__ asym) {
+  constexpr int V = chunk_entries<T>(), NQ = s27_chunks<T>() * V;
+  const long long u = static_cast<long long>(blockIdx.x) * kBlock + threadIdx.x;
+  if (u >= units) return;
+  const int k = static_cast<int>(u / nl), m = static_cast<int>(u - static_cast<long long>(k) * nl);
+  T *St = sym + static_cast<size_t>(u >> 6) * gss + static_cast<size_t>(u & 63) * V;
+  const long long e = static_cast<long long>(ebase[slotpos[k]]) + m;
+  bool mismatch = false;
+  for (int q = 0; q < NQ; ++q) {
+    T v = T(0);
+    if (q < kS27Tri) {
+      const int i = s27_row(q), j = s27_col(q);
+      v = A[aidx<T>(gs, e + ((row_top >> i) & 1u), i * kS27 + j)];
+      const T w = A[aidx<T>(gs, e + ((row_top >> j) & 1u), j * kS27 + i)];
+      if constexpr (sizeof(T) == 8)
+        mismatch = mismatch || (__double_as_longlong(v) != __double_as_longlong(w));
+      else
+        mismatch = mismatch || (__float_as_uint(v) != __float_as_uint(w));
+    }
+    St[static_cast<size_t>(q / V) * 64 * V + q % V] = v;
+  }
+  if (mismatch) *asym = 1;
+}
+
+// phase 1: L = NR/NRL lanes per unit, s27_threads(units) threads
+template <typename T, int NR, int NRL>
+__global__ __launch_bounds__(kBlock) void k_sym27_units(const T *__restrict__ sym, size_t gss, const T *__restrict__ x,
+                                                        const int *__restrict__ map2, const int *__restrict__ slotpos,
+                                                        T *__restrict__ z, long long units, int nl, const int *skip) {
+  if (skip && *skip) return;
+  sym27_unit<T, NR, NRL>(sym, gss, x, map2, slotpos, z, units, nl,
+                         static_cast<long long>(blockIdx.x) * kBlock + threadIdx.x);
+}
+
+// phase 2, one point per thread; DOT (one RHS): the block's partial sum of x.y as well
+template <typename T, int NR, bool DOT>
+__global__ __launch_bounds__(kBlock) void k_sym27_gather(const T *__restrict__ z, const T *__restrict__ x,
+                                                         T *__restrict__ y, const int2 *__restrict__ cols,
+                                                         const int4 *__restrict__ zu, int ncols, int nl,
+                                                         double *partials, const int *skip) {
+  if (skip && *skip) return;
+  typedef typename RVec<T, NR>::type VR;
+  VR acc;
+  size_t dof = 0;
+  const bool valid = sym27_gather_point<T, NR>(static_cast<long long>(blockIdx.x) * kBlock + threadIdx.x, z, cols,
+                                               zu, ncols, nl, acc, dof);
+  if (valid) *reinterpret_cast<VR *>(y + dof * NR) = acc;
+  if constexpr (DOT) {
+    static_assert(NR == 1, "the fused dot product is for one right-hand side");
+    double d[1] = {valid ? to_d(acc[0]) * to_d(x[dof]) : 0.0};
+    block_partials<1>(d, partials);
+  }
+}
+
 // Several right-hand sides over the row-gathered copy: each RHS accumulates exactly as
 // k_lma_gather_rows_w0 (and so as k_lma_gather_mrhs), x and y hold NR values per dof, RHS
 // fastest.  The matrix streams once for all right-hand sides.
@@ -1173,52 +1232,8 @@ __global__ __launch_bounds__(kBlock) void k_lma_gather_rows_w0_mrhs(const T *__r
                                                                     const int *__restrict__ off,
                                                                     const int *__restrict__ ent, int ncols, int nl,
                                                                     int xwin) {
-  typedef typename RVec<T, NR>::type VR;
-  constexpr int V = chunk_entries<T>(), RS = rows_rs<T, 8>();
-  typedef T VecT __attribute__((ext_vector_type(V)));
-  const int H = nl + 1;
-  const long long t = static_cast<long long>(xcd_block(blockIdx.x, gridDim.x, xwin)) * kBlock + threadIdx.x;
-  if (t >= static_cast<long long>(ncols) * H) return;
-  const int col = static_cast<int>(t / H);
-  const int m = static_cast<int>(t - static_cast<long long>(col) * H);
-  const int2 ci = cols[col];
-  if (m == nl && !ci.y) return;
-  const bool has_m = m < nl, has_p = m > 0;
-  const int q0 = off[col], cnt = off[col + 1] - q0;
-  const T *Rt = rows + static_cast<size_t>(t >> 6) * gsr + static_cast<size_t>(t & 63) * V;
-  VR acc = {};
-#pragma unroll
-  for (int k = 0; k < kRowsKmax; ++k) {
-    if (k >= cnt) break;
-    const int pos = ent[q0 + k] >> 5;
-    int b[4];
-#pragma unroll
-    for (int j = 0; j < 4; ++j) b[j] = map2[static_cast<size_t>(pos) * 8 + j];
-    VR xl[3][4];  // x of the cell's 4 columns at levels m-1, m, m+1
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      xl[0][j] = has_p ? *reinterpret_cast<const VR *>(x + (static_cast<size_t>(b[j]) + m - 1) * NR) : VR{};
-      xl[1][j] = *reinterpret_cast<const VR *>(x + (static_cast<size_t>(b[j]) + m) * NR);
-      xl[2][j] = has_m ? *reinterpret_cast<const VR *>(x + (static_cast<size_t>(b[j]) + m + 1) * NR) : VR{};
-    }
-#pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      if (h == 0 ? !has_p : !has_m) continue;
-#pragma unroll
-      for (int c = 0; c < RS / V; ++c) {
-        const VecT v = __builtin_nontemporal_load(
-            reinterpret_cast<const VecT *>(Rt + static_cast<size_t>((2 * k + h) * (RS / V) + c) * 64 * V));
-#pragma unroll
-        for (int q = 0; q < V; ++q) {
-          const int j = c * V + q;
-          const VR &xv = j < 4 ? xl[h][j] : xl[h + 1][j - 4];
-#pragma unroll
-          for (int r = 0; r < NR; ++r) acc[r] = fma(v[q], xv[r], acc[r]);
-        }
-      }
-    }
-  }
-  *reinterpret_cast<VR *>(y + (static_cast<size_t>(ci.x) + m) * NR) = acc;
+  rows_point_w0_mrhs<T, NR>(rows, gsr, x, y, map2, cols, off, ent, ncols, nl,
+                            static_cast<long long>(xcd_block(blockIdx.x, gridDim.x, xwin)) * kBlock + threadIdx.x);
 }
 
 // ---- several right-hand sides ------------------------------------------------------------
@@ -2322,7 +2337,8 @@ int gather_dot_blocks(const lfric_op *op) {
 static hipEvent_t g_phase_mark = nullptr;
 
 int launch_lma_apply_gather_marked(const lfric_op *op, const void *x, void *y, hipEvent_t mark, bool *two_phase) {
-  *two_phase = op->sym != nullptr && op->to == op->from && role_kind(*op->to) == RoleKind::W0K0;
+  *two_phase = op->sym != nullptr && op->to == op->from &&
+               (role_kind(*op->to) == RoleKind::W0K0 || role_kind(*op->to) == RoleKind::W0K1);
   g_phase_mark = *two_phase ? mark : nullptr;
   const int rc = launch_lma_apply_gather(op, x, y);
   g_phase_mark = nullptr;
@@ -2334,10 +2350,50 @@ struct GatherExtra {
   const int *skip = nullptr;
 };
 
+// the two phases of the order-1 symmetric-packed apply (sym27_apply.h), NR right-hand sides;
+// with partials (one RHS) the second phase adds the x.y partial sum of each of its blocks
+template <typename T, int NR>
+static int launch_sym27(const lfric_op *op, const T *x, T *y, double *partials, const int *skip) {
+  const FunctionSpace &to = *op->to, &from = *op->from;
+  const long long units = static_cast<long long>(to.ncells) * to.nlayers;
+  constexpr int NRL = s27_nrl<T, NR>();
+  T *z = static_cast<T *>(op->symz);
+  if (NR > 1) {
+    const size_t need = sizeof(T) * s27_zalloc(to.ncells, to.nlayers, NR);
+    if (op->symz_mr_bytes < need) {
+      dfree(op->symz_mr, op->symz_mr_bytes);
+      op->symz_mr = nullptr;
+      op->symz_mr_bytes = 0;
+      if (int rc = dmalloc(&op->symz_mr, need)) return rc;
+      op->symz_mr_bytes = need;
+    }
+    z = static_cast<T *>(op->symz_mr);
+  }
+  LAUNCH((k_sym27_units<T, NR, NRL>), grid_for(s27_threads<T, NR>(units)), static_cast<const T *>(op->sym), op->gss,
+         x, from.d_map, op->d_sympos, z, units, to.nlayers, skip);
+  if (g_phase_mark) (void)hipEventRecord(g_phase_mark, cur_stream());
+  const int grid = gather_blocks(op);
+  if (partials) {
+    if constexpr (NR == 1)
+      LAUNCH((k_sym27_gather<T, 1, true>), grid, z, x, y, to.d_gcol, op->d_symzu, to.gather_ncols, to.nlayers,
+             partials, skip);
+  } else {
+    LAUNCH((k_sym27_gather<T, NR, false>), grid, z, x, y, to.d_gcol, op->d_symzu, to.gather_ncols, to.nlayers,
+           partials, skip);
+  }
+  return LFRIC_OK;
+}
+
 template <typename T, int N2, class R>
 static void launch_gather_r(const lfric_op *op, const T *x, T *y, R roles, GatherExtra ex) {
   const FunctionSpace &to = *op->to, &from = *op->from;
   const int grid = gather_blocks(op);
+  if constexpr (N2 == 27 && std::is_same<R, RolesW0K1>::value) {
+    if (sym27_ready(op)) {
+      (void)launch_sym27<T, 1>(op, x, y, ex.partials, ex.skip);
+      return;
+    }
+  }
   if constexpr (N2 == 8 && std::is_same<R, RolesW0K0>::value) {
     if (op->sym) {  // symmetric-packed copy (launch_build_sym): two phases
       const int nseg = op->sym_seg > 0 ? (to.nlayers + 1 + op->sym_seg - 1) / op->sym_seg : 0;
@@ -2421,18 +2477,22 @@ static int gather_any(const lfric_op *op, const void *x, void *y, GatherExtra ex
 
 // ---- multi-RHS dispatch ---------------------------------------------------------------------
 template <typename T, int N2, int NR, class R>
-static void launch_gather_mrhs_r(const lfric_op *op, const T *x, T *y, R roles) {
+static int launch_gather_mrhs_r(const lfric_op *op, const T *x, T *y, R roles) {
   const FunctionSpace &to = *op->to, &from = *op->from;
   const int grid = grid_for(static_cast<long long>(to.gather_ncols) * (to.nlayers + 1));
+  if constexpr (N2 == 27 && std::is_same<R, RolesW0K1>::value) {
+    if (sym27_ready(op)) return launch_sym27<T, NR>(op, x, y, nullptr, nullptr);
+  }
   if constexpr (N2 == 8 && std::is_same<R, RolesW0K0>::value) {
     if (op->rows) {  // the row-gathered copy (lowest-order W0, launch_build_rows)
       LAUNCH((k_lma_gather_rows_w0_mrhs<T, NR>), grid, static_cast<const T *>(op->rows), op->gsr, x, y, from.d_map,
              to.d_gcol, to.d_goff, to.d_gent, to.gather_ncols, to.nlayers, op->xwin);
-      return;
+      return LFRIC_OK;
     }
   }
   LAUNCH((k_lma_gather_mrhs<T, N2, NR, R>), grid, static_cast<const T *>(op->d), op->gs, to.d_ebase, x, y,
          from.d_map, to.d_gcol, to.d_goff, to.d_gent, to.gather_ncols, to.nlayers, roles, op->xwin);
+  return LFRIC_OK;
 }
 
 template <typename T, int N1, int N2, int NR, class R>
@@ -2448,7 +2508,7 @@ static void launch_colour_mrhs_r(const lfric_op *op, const T *x, T *y, R roles) 
 }
 
 template <typename T, int N1, int N2, int NR>
-static void launch_mrhs_n(int variant, RoleKind rk, const lfric_op *op, const T *x, T *y) {
+static int launch_mrhs_n(int variant, RoleKind rk, const lfric_op *op, const T *x, T *y) {
   const Roles rr = roles_of(*op->to);
   if (variant == LFRIC_MRHS_COLOUR) {
     switch (rk) {
@@ -2458,14 +2518,14 @@ static void launch_mrhs_n(int variant, RoleKind rk, const lfric_op *op, const T 
       case RoleKind::None: launch_colour_mrhs_r<T, N1, N2, NR>(op, x, y, RolesNone{}); break;
       default: launch_colour_mrhs_r<T, N1, N2, NR>(op, x, y, RolesRuntime{rr});
     }
-    return;
+    return LFRIC_OK;
   }
   switch (rk) {
-    case RoleKind::W0K0: launch_gather_mrhs_r<T, N2, NR>(op, x, y, RolesW0K0{}); break;
-    case RoleKind::W0K1: launch_gather_mrhs_r<T, N2, NR>(op, x, y, RolesW0K1{}); break;
-    case RoleKind::W2K0: launch_gather_mrhs_r<T, N2, NR>(op, x, y, RolesW2K0{}); break;
-    case RoleKind::None: launch_gather_mrhs_r<T, N2, NR>(op, x, y, RolesNone{}); break;
-    default: launch_gather_mrhs_r<T, N2, NR>(op, x, y, RolesRuntime{rr});
+    case RoleKind::W0K0: return launch_gather_mrhs_r<T, N2, NR>(op, x, y, RolesW0K0{});
+    case RoleKind::W0K1: return launch_gather_mrhs_r<T, N2, NR>(op, x, y, RolesW0K1{});
+    case RoleKind::W2K0: return launch_gather_mrhs_r<T, N2, NR>(op, x, y, RolesW2K0{});
+    case RoleKind::None: return launch_gather_mrhs_r<T, N2, NR>(op, x, y, RolesNone{});
+    default: return launch_gather_mrhs_r<T, N2, NR>(op, x, y, RolesRuntime{rr});
   }
 }
 
@@ -2475,7 +2535,7 @@ static int mrhs_nr(int variant, const lfric_op *op, const T *x, T *y) {
   const int n1 = op->to->ndf, n2 = op->from->ndf;
 #define MRHS_CASE(A, B)                                      \
   if (n1 == A && n2 == B) {                                  \
-    launch_mrhs_n<T, A, B, NR>(variant, rk, op, x, y);       \
+    if (int rc = launch_mrhs_n<T, A, B, NR>(variant, rk, op, x, y)) return rc; \
     CHECK_LAUNCH();                                          \
     return LFRIC_OK;                                         \
   }
@@ -2545,7 +2605,7 @@ int launch_lma_apply_mrhs(const lfric_op *op, const void *x, void *y, int nrhs, 
     // hundred thousand cells up (C96x70 W0, 4 FP32 RHS: 0.310 ms vs gather 0.354); on small
     // meshes its one launch per colour leaves the GPU idle and the gather form wins
     const long long nc3 = static_cast<long long>(op->to->ncells) * op->to->nlayers;
-    variant = (nrhs > 1 && nc3 >= 200000 && !op->rows) ? LFRIC_MRHS_COLOUR : LFRIC_MRHS_GATHER;
+    variant = (nrhs > 1 && nc3 >= 200000 && !op->rows && !sym27_ready(op)) ? LFRIC_MRHS_COLOUR : LFRIC_MRHS_GATHER;
   }
   LF_CHECK(variant == LFRIC_MRHS_GATHER || variant == LFRIC_MRHS_COLOUR || variant == LFRIC_MRHS_MFMA, LFRIC_E_ARG,
            "unknown multi-RHS variant");
@@ -2687,10 +2747,7 @@ static int build_sym_t(lfric_op *op, int S) {
   LF_HIP(hipMemcpyAsync(&h_flag, b.flag, sizeof(int), hipMemcpyDeviceToHost, cur_stream()));
   LF_HIP(hipStreamSynchronize(cur_stream()));
   LF_CHECK(!h_flag, LFRIC_E_UNSUPPORTED, "the operator's element matrices are not symmetric");
-  dfree(op->sym, op->sym_bytes);
-  dfree(op->symz, op->symz_bytes);
-  dfree(op->d_symzu, op->symzu_bytes);
-  dfree(op->d_symcol, op->symcol_bytes);
+  drop_sym(op);
   op->sym = b.sym;
   op->sym_bytes = b.sym_b;
   op->symz = b.z;
@@ -2706,11 +2763,98 @@ static int build_sym_t(lfric_op *op, int S) {
   return LFRIC_OK;
 }
 
+void drop_sym(lfric_op *op) {
+  dfree(op->sym, op->sym_bytes);
+  dfree(op->symz, op->symz_bytes);
+  dfree(op->d_symzu, op->symzu_bytes);
+  dfree(op->d_symcol, op->symcol_bytes);
+  dfree(op->d_sympos, op->sympos_bytes);
+  dfree(op->symz_mr, op->symz_mr_bytes);
+  op->sym = op->symz = op->symz_mr = nullptr;
+  op->d_symzu = op->d_symcol = nullptr;
+  op->d_sympos = nullptr;
+  op->sym_bytes = op->symz_bytes = op->symzu_bytes = op->symcol_bytes = op->sympos_bytes = op->symz_mr_bytes = 0;
+  op->gss = 0;
+  op->sym_seg = 0;
+}
+
+bool sym27_ready(const lfric_op *op) {
+  return op->sym && op->d_sympos && op->to == op->from && role_kind(*op->to) == RoleKind::W0K1;
+}
+
+// the order-1 W0 symmetric-packed copy (sym27_apply.h): one unit per (cell, layer), cells in
+// cell-id order; the gather columns' cell tables; one right-hand side's row-sum buffer
+template <typename T>
+static int build_sym27_t(lfric_op *op) {
+  const FunctionSpace &f = *op->to;
+  const int nc = f.ncells, nl = f.nlayers;
+  const long long units = static_cast<long long>(nc) * nl;
+  const size_t gss = s27_group_stride<T>();
+  SymBuild b;
+  b.sym_b = sizeof(T) * gss * static_cast<size_t>((units + 63) / 64);
+  b.z_b = sizeof(T) * s27_zalloc(nc, nl, 1);
+  b.slotpos_b = sizeof(int) * nc;
+  if (int rc = dmalloc(&b.sym, b.sym_b)) return rc;
+  if (int rc = dmalloc(&b.z, b.z_b)) return rc;
+  if (int rc = dmalloc(reinterpret_cast<void **>(&b.flag), sizeof(int))) return rc;
+  if (int rc = dmalloc(reinterpret_cast<void **>(&b.slotpos), b.slotpos_b)) return rc;
+  std::vector<int> slotpos(nc);
+  for (int p = 0; p < nc; ++p) slotpos[f.cell_of_pos[p]] = p;
+  LF_HIP(hipMemcpy(b.slotpos, slotpos.data(), b.slotpos_b, hipMemcpyHostToDevice));
+  {
+    const int ncol = f.gather_ncols;
+    std::vector<int> off(ncol + 1), ent;
+    LF_HIP(hipMemcpy(off.data(), f.d_goff, sizeof(int) * (ncol + 1), hipMemcpyDeviceToHost));
+    ent.resize(off[ncol]);
+    LF_HIP(hipMemcpy(ent.data(), f.d_gent, sizeof(int) * ent.size(), hipMemcpyDeviceToHost));
+    std::vector<int4> h(ncol);
+    for (int c = 0; c < ncol; ++c) {
+      LF_CHECK(off[c + 1] - off[c] <= 4, LFRIC_E_UNSUPPORTED,
+               "the order-1 symmetric copy needs at most 4 cells around each dof column");
+      int u[4] = {-1, -1, -1, -1};
+      for (int q = off[c]; q < off[c + 1]; ++q) u[q - off[c]] = (ent[q] >> 5) * 18 + s27_slot(ent[q] & 31);
+      h[c] = make_int4(u[0], u[1], u[2], u[3]);
+    }
+    b.zu_b = sizeof(int4) * h.size();
+    if (int rc = dmalloc(reinterpret_cast<void **>(&b.zu), b.zu_b)) return rc;
+    LF_HIP(hipMemcpy(b.zu, h.data(), b.zu_b, hipMemcpyHostToDevice));
+  }
+  int h_flag = 0;
+  LF_HIP(hipMemcpyAsync(b.flag, &h_flag, sizeof(int), hipMemcpyHostToDevice, cur_stream()));
+  LAUNCH((k_build_sym27<T>), grid_for(units), static_cast<const T *>(op->d), op->gs, f.d_ebase, b.slotpos,
+         static_cast<T *>(b.sym), gss, units, nl, op->row_top, b.flag);
+  CHECK_LAUNCH();
+  LF_HIP(hipMemcpyAsync(&h_flag, b.flag, sizeof(int), hipMemcpyDeviceToHost, cur_stream()));
+  LF_HIP(hipStreamSynchronize(cur_stream()));
+  LF_CHECK(!h_flag, LFRIC_E_UNSUPPORTED, "the operator's element matrices are not symmetric");
+  drop_sym(op);
+  op->sym = b.sym;
+  op->sym_bytes = b.sym_b;
+  op->symz = b.z;
+  op->symz_bytes = b.z_b;
+  op->d_symzu = b.zu;
+  op->symzu_bytes = b.zu_b;
+  op->d_sympos = b.slotpos;
+  op->sympos_bytes = b.slotpos_b;
+  op->gss = gss;
+  op->sym_seg = 0;
+  b.sym = b.z = nullptr;
+  b.zu = nullptr;
+  b.slotpos = nullptr;
+  return LFRIC_OK;
+}
+
 int launch_build_sym(lfric_op *op, int levels_per_thread) {
   FunctionSpace &f = *op->to;
   LF_CHECK(op->to == op->from, LFRIC_E_ARG, "the symmetric-packed copy is for square operators");
+  if (role_kind(f) == RoleKind::W0K1 && f.ndf == 27) {
+    LF_CHECK(levels_per_thread == 0, LFRIC_E_UNSUPPORTED,
+             "the order-1 symmetric-packed copy takes one (cell, layer) unit per thread");
+    if (int rc = build_gather(f)) return rc;
+    return op->precision == LFRIC_R64 ? build_sym27_t<double>(op) : build_sym27_t<float>(op);
+  }
   LF_CHECK(role_kind(f) == RoleKind::W0K0 && f.ndf == 8, LFRIC_E_UNSUPPORTED,
-           "the symmetric-packed copy is built for lowest-order W0 operators");
+           "the symmetric-packed copy is built for W0 operators of order 0 and 1");
   LF_CHECK(levels_per_thread >= 0 && levels_per_thread <= 64, LFRIC_E_ARG, "levels per thread must be 0..64");
   for (int c = 0; c < f.ncells; ++c)
     for (int j = 0; j < 4; ++j)

@@ -282,11 +282,13 @@ class LmaOperator:
         return self
 
     def prepare_symmetric(self, levels_per_thread=0):
-        """Symmetric-packed copy (square lowest-order W0 operators with symmetric element
-        matrices): the solvers' apply then reads ~70 % of the matrix bytes, same result as the
-        colour-ordered apply bit for bit.  levels_per_thread 0: one (cell, level) unit per thread,
-        the previous layer's top rows passed between lanes; S > 0: S units per thread, each
-        thread's copy repeating the previous segment's last triangle."""
+        """Symmetric-packed copy (square W0 operators of order 0 or 1 with symmetric element
+        matrices): the gather apply (one or several right-hand sides, the solvers' apply and the
+        batch's) then reads the upper triangles only, with the colour-ordered apply's result bit
+        for bit.  Order 0: levels_per_thread 0 is one (cell, level) unit per thread, the previous
+        layer's top rows passed between lanes; S > 0: S units per thread, each thread's copy
+        repeating the previous segment's last triangle.  Order 1: one (cell, layer) unit per
+        thread (levels_per_thread 0), its 18 outputs kept between the two phases."""
         check(lib().lfric_op_prepare_symmetric(self._h, int(levels_per_thread)))
         return self
 

@@ -21,7 +21,7 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from . import _lib
-from .batch import Batch
+from .batch import Batch, prepare_batch_copies
 from .fem import FieldVector, FunctionSpace, LmaOperator
 
 
@@ -55,8 +55,10 @@ def padded_rhs(nrhs):
 
 def footprint_bytes(n, nl, order, nrhs):
     """Device bytes a request holds while admitted: FP64 and FP32 operators (level-aligned
-    layout with 64-slot colour padding), x and b in FP64, the batch's five FP32 vectors and
-    FP64 residual, the Jacobi diagonal and the space's maps."""
+    layout with 64-slot colour padding), the FP32 operator's batch copy (row-gathered for
+    order 0; for order 1 the symmetric-packed triangles, one RHS's row sums and the batch's
+    row sums of every RHS), x and b in FP64, the batch's four FP32 vectors and FP64 residual,
+    the Jacobi diagonal and the space's maps."""
     nc = 6 * n * n
     ndf = 8 if order == 0 else 27
     undf = (nc + 2) * (nl + 1) if order == 0 else (nc + 12 * n * n + nc + 2) * (2 * nl + 1)
@@ -65,9 +67,14 @@ def footprint_bytes(n, nl, order, nrhs):
     ent64 = -(-ndf * ndf // 2) * 2
     ops = nelem * (ent32 * 4 + ent64 * 8)
     r = padded_rhs(nrhs)
-    vecs = undf * r * (8 + 8 + 5 * 4 + 8) + undf * 4 * 2
+    if order == 0:
+        copy = (undf + 64) * 2 * 4 * 8 * 4
+    else:
+        units = nc * nl
+        copy = (units + 64) * 380 * 4 + units * 27 * 4 * (1 + r) + undf * 16
+    vecs = undf * r * (8 + 8 + 4 * 4 + 8) + undf * 4 * 2
     maps = nc * ndf * 4 * 4
-    return ops + vecs + maps
+    return ops + copy + vecs + maps
 
 
 def poisson_stream(rate, window_s, seed=0, meshes=(24, 48, 96, 192), weights=(0.4, 0.3, 0.2, 0.1),
@@ -108,7 +115,7 @@ class _Live:
         fs = FunctionSpace.cubedsphere(req.n, req.nl, _lib.W0, req.order)
         self.fs = fs
         self.op64 = LmaOperator(fs).generate_spd(req.seed, scaled=True)
-        self.op32 = self.op64.convert(_lib.R32)
+        self.op32 = prepare_batch_copies(self.op64.convert(_lib.R32))
         self.diag32 = self.op32.mm_diagonal(FieldVector(fs, _lib.R32))
         r = padded_rhs(req.nrhs)
         self.nrhs = r

@@ -99,3 +99,36 @@ def test_batch_gather_form_converges_and_agrees(hip, orc):
             assert res.status == 0 and res.true_rel_res <= 1e-8, (p["key"], res)
         d = np.linalg.norm(col[i][0] - gat[i][0]) / np.linalg.norm(gat[i][0])
         assert d <= 1e-5, (p["key"], d)
+
+
+def test_batch_copy_forms_equal_colour_and_gather_forms(hip, orc):
+    """The batch's default forms over the operators' copies: the symmetric-packed order-1 apply
+    equals the colour-ordered register-staged one bit for bit, and the gather over the
+    row-gathered copy equals the gather form bit for bit, so each problem's solution and
+    iteration counts are those of the batch run in that form."""
+    from lfric_core_amd.batch import prepare_batch_copies
+    probs = [_make(hip, *k) for k in MIX]
+    for p in probs:
+        prepare_batch_copies(p["op32"])
+
+    def run(apply):
+        from lfric_core_amd.batch import Batch
+        bt = Batch(r_tol=1e-8, check_every=4)
+        bt.config(apply=apply)
+        ids = {}
+        for i, p in enumerate(probs):
+            p["x"].set_scalar(0.0)
+            ids[i] = bt.add(p["op64"], p["op32"], p["diag32"], p["x"], p["b"])
+        bt.solve()
+        out = {i: (probs[i]["x"].export_field().copy(), bt.result(pid)) for i, pid in ids.items()}
+        bt.close()
+        return out
+
+    cop, col, gat = run("copies"), run("colour"), run("gather")
+    for i, p in enumerate(probs):
+        order = p["key"][2]
+        for res in cop[i][1]:
+            assert res.status == 0 and res.true_rel_res <= 1e-8, (p["key"], res)
+        same = col if order == 1 else gat
+        np.testing.assert_array_equal(cop[i][0], same[i][0])
+        assert [r.iterations for r in cop[i][1]] == [r.iterations for r in same[i][1]]
