@@ -88,6 +88,40 @@ def time_applies(L, op, x, y, reps, method="apply"):
     return L.timer_stop() / reps
 
 
+def order1_report(L, reps, n=48, nl=70):
+    """FP32 order-1 W0 multi-RHS applies at C48x70 (3.87M dofs, scaled SPD elements), 1 and 4
+    right-hand sides: colour-ordered, gather, and the two phases over the symmetric-packed copy.
+    GB/s on each form's own algorithmic bytes (full 729-entry elements, or 378-entry triangles)
+    and on full-element bytes."""
+    fs = L.FunctionSpace.cubedsphere(n, nl, L.W0, 1)
+    op = L.LmaOperator(fs).generate_spd(1, scaled=True).convert(L.R32)
+    xs = {nr: L.FieldVector(fs, L.R32, nr).fill_uniform(1, 3) for nr in (1, 4)}
+    ys = {nr: L.FieldVector(fs, L.R32, nr) for nr in (1, 4)}
+
+    def timed(nr, variant):
+        op.apply_mrhs(xs[nr], ys[nr], variant)
+        L.synchronize()
+        L.timer_start()
+        for _ in range(reps):
+            op.apply_mrhs(xs[nr], ys[nr], variant)
+        return L.timer_stop() / reps
+
+    out = {}
+    for nr in (1, 4):
+        out[f"rhs{nr}"] = {"colour_ms": timed(nr, 2), "gather_ms": timed(nr, 1)}
+    op.prepare_symmetric()
+    for nr in (1, 4):
+        r = out[f"rhs{nr}"]
+        r["symmetric_ms"] = timed(nr, 1)
+        xy = 2 * fs.undf * nr * 4 + fs.ncells * 27 * 4
+        full, tri = fs.ncell_3d * 729 * 4 + xy, fs.ncell_3d * 378 * 4 + xy
+        r["gbs"] = {"colour": full / (r["colour_ms"] * 1e-3) / 1e9, "gather": full / (r["gather_ms"] * 1e-3) / 1e9,
+                    "symmetric": tri / (r["symmetric_ms"] * 1e-3) / 1e9,
+                    "symmetric_full_element_equivalent": full / (r["symmetric_ms"] * 1e-3) / 1e9}
+    out["symmetric_phases_ms_rhs1"] = op.time_apply(xs[1], ys[1], reps)
+    return out
+
+
 _START_AFFINITY = sorted(os.sched_getaffinity(0))  # before any OpenMP runtime binds this thread
 
 
@@ -315,7 +349,8 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--apply-only", action="store_true",
-                    help="time only the LMA applies and a known-size vector copy (for hardware-counter runs)")
+                    help="time only the LMA applies and a known-size vector copy (for hardware-counter runs); "
+                         "with --workload batched, the batch apply only")
     ap.add_argument("--workload", choices=("single", "batched", "served"), default="single",
                     help="single: C96x70 mixed vs FP64 solve (default); batched: 64 uneven multi-RHS problems "
                          "in one batch vs sequential FP64 solves; served: Poisson request stream with "
@@ -324,14 +359,14 @@ def main():
     ap.add_argument("--batched-problems", type=int, default=64,
                     help="single workload: also run the batched configuration with this many problems (0: skip)")
     ap.add_argument("--batch-colour", type=int, default=1, choices=(0, 1),
-                    help="batched: colour-ordered multi-RHS apply (1: lowest order on the matrix cores, order 1 "
-                         "with register-staged x) or gather form for every problem (0)")
+                    help="batched: the operators' copies (1: gather over the row-gathered copy at lowest order, "
+                         "symmetric-packed triangles at order 1) or gather form for every problem (0)")
     ap.add_argument("--apply-form", type=int, default=1, choices=(0, 1),
                     help="LMA apply inside the solvers: 0 colour-ordered scatter, 1 gather with fused p.Ap")
     ap.add_argument("--operator-copy", choices=("symmetric", "rows"), default="symmetric",
                     help="gather form's operator copy: symmetric-packed triangles (two kernels) or the "
                          "row-gathered full element matrices (one kernel)")
-    ap.add_argument("--serve-window", type=float, default=20.0, help="seconds of arrivals")
+    ap.add_argument("--serve-window", type=float, default=60.0, help="seconds of arrivals")
     ap.add_argument("--serve-rate", type=float, default=200.0, help="requests per second")
     ap.add_argument("--serve-limit", type=float, default=30.0, help="wall seconds of admission per server")
     args = ap.parse_args()
@@ -405,6 +440,7 @@ def main():
                                             "fp32_tri_gbs": sym_bytes(fs, 4, 4) / (t32 * 1e-3) / 1e9,
                                             "fp64_tri_gbs": sym_bytes(fs, 8, 8) / (t64 * 1e-3) / 1e9}
         del s32, s64
+        out["order1_c48x70_fp32"] = order1_report(L, args.apply_reps)
         out["columnwise_w3"] = columnwise_report(L, fs, args.apply_reps)
         out["assembly"] = assembly_report(L, fs)
         for _ in range(args.apply_reps):  # plain copies of a known size (undf * 8 bytes each way)
@@ -536,6 +572,46 @@ def _common(args, metric, value, unit, ms, config, dtype):
             "config": config}
 
 
+def batch_apply_bytes(probs, copies):
+    """Algorithmic bytes of one batch apply: with the operators' copies an order-1 problem reads
+    the 378 entries of each element's upper triangle (symmetric-packed copy) and a lowest-order
+    one its 64 entries (row-gathered copy); x and y once (nrhs values per dof); the dofmaps once.
+    Also the same with every element read in full, and the per-unit outputs that the packed form
+    writes and reads back between its two phases (not algorithmic)."""
+    def ent(p):
+        nd = p["fs"].ndf
+        return nd * (nd + 1) // 2 if (copies and nd == 27) else nd * nd
+
+    def xyz(p):
+        return 2 * p["fs"].undf * p["nrhs"] * 4 + 4 * p["fs"].ncells * p["fs"].ndf
+
+    alg = sum(p["fs"].ncell_3d * ent(p) * 4 + xyz(p) for p in probs)
+    full = sum(p["fs"].ncell_3d * p["fs"].ndf ** 2 * 4 + xyz(p) for p in probs)
+    rowsum = sum(2 * p["fs"].ncells * 18 * (p["fs"].nlayers + 1) * p["nrhs"] * 4 for p in probs
+                 if copies and p["fs"].ndf == 27)
+    return alg, full, rowsum
+
+
+def batched_apply_only(L, problems, say):
+    """The batched configuration's problems in one batch, one round run, then the autotuner's
+    applies (block sizes x grid sizes, 6 each): for hardware-counter and kernel-trace runs of
+    the batch apply."""
+    from lfric_core_amd.batch import Batch, batched_mix, build_problem
+    probs = [build_problem(*k) for k in batched_mix(problems)]
+    bt = Batch(r_tol=1e-8, a_tol=0.0, max_iter=4000, check_every=8)
+    for p in probs:
+        bt.add(p["op64"], p["op32"], p["d32"], p["x"], p["b"])
+    bt.run(1)
+    ms = bt.tune()
+    st = bt.stats()
+    bt.close()
+    alg, full, rowsum = batch_apply_bytes(probs, True)
+    say(f"tuned apply {ms:.3f} ms {st}")
+    return {"apply_only": True, "problems": len(probs), "batch_apply_ms": ms, "alg_bytes": alg,
+            "gbs": alg / (ms * 1e-3) / 1e9, "full_matrix_bytes": full, "row_sum_bytes": rowsum,
+            "autotuned": {"block": st["block"], "grid_mult": st["grid_mult"]}}
+
+
 def batched_record(L, problems, steps, say, colour=True):
     """The batched configuration (64 uneven W0 problems, 4 RHS each) solved as one batch: a
     warm-up batch with the autotuner, then `steps` timed batches (host polls every 8 inner
@@ -596,21 +672,7 @@ def batched_record(L, problems, steps, say, colour=True):
     bt.run(1)
     apply_ms = bt.tune()
     bt.close()
-    # with the operators' copies (the default) an order-1 problem reads the 378 entries of each
-    # element's upper triangle (symmetric-packed copy) and a lowest-order one its 64 entries (the
-    # row-gathered copy); the colour-ordered forms read every element in full.  The per-unit row
-    # sums that the packed form writes and reads back between its two phases are not algorithmic.
-    def ent(p):
-        nd = p["fs"].ndf
-        return nd * (nd + 1) // 2 if (colour and nd == 27) else nd * nd
-
-    def xyz(p):
-        return 2 * p["fs"].undf * p["nrhs"] * 4 + 4 * p["fs"].ncells * p["fs"].ndf
-
-    apply_bytes_total = sum(p["fs"].ncell_3d * ent(p) * 4 + xyz(p) for p in probs)
-    apply_bytes_full = sum(p["fs"].ncell_3d * p["fs"].ndf ** 2 * 4 + xyz(p) for p in probs)
-    rowsum_bytes = sum(2 * p["fs"].ncells * 18 * (p["fs"].nlayers + 1) * p["nrhs"] * 4 for p in probs
-                       if colour and p["fs"].ndf == 27)
+    apply_bytes_total, apply_bytes_full, rowsum_bytes = batch_apply_bytes(probs, colour)
     # sequential plain FP64 solves of the same systems, one RHS at a time
     seq_t, seq_it = 0.0, 0
     for p in probs:
@@ -667,6 +729,9 @@ def bench_batched(args):
     import lfric_core_amd as L
     L.init(0)
     say = lambda msg: print(f"[batched] {msg}", file=sys.stderr, flush=True)
+    if args.apply_only:
+        print(json.dumps(batched_apply_only(L, args.problems, say)))
+        return
     rec = batched_record(L, args.problems, args.steps, say, colour=bool(args.batch_colour))
     line = _common(args, "solved dofs/s at rel-res 1e-8 (batched mixed precision vs sequential FP64), p50/p99 "
                          "solve ms", rec["solved_dofs_per_s"], "solved dofs/s", rec["batch_wall_ms"],
