@@ -207,8 +207,10 @@ int lfric_op_prepare_rows(lfric_op *op);
  * the solvers' apply, lfric_lma_apply_mrhs with LFRIC_MRHS_GATHER or AUTO, the batch) reads it
  * in two phases (per-unit row sums, then a colour-ordered gather) whose result equals the
  * colour-ordered scatter (lfric_lma_apply, LFRIC_MRHS_COLOUR) bit for bit.  Writing the
- * operator rebuilds the copy, or drops it if the new entries are not symmetric.  No reference
- * counterpart (the reference stores full element matrices). */
+ * operator rebuilds the copy, or drops it if the new entries are not symmetric.  The two
+ * phases pass the per-unit outputs through one buffer per operator (for several right-hand
+ * sides grown on first use): applies of one operator that overlap on two streams race.  No
+ * reference counterpart (the reference stores full element matrices). */
 int lfric_op_prepare_symmetric(lfric_op *op, int levels_per_thread);
 /* Round an operator into another precision (e.g. the r_solver FP32 copy of an FP64 master). */
 int lfric_op_convert(const lfric_op *src, lfric_op *dst);
