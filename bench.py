@@ -621,7 +621,7 @@ def batched_record(L, problems, steps, say, colour=True):
     from lfric_core_amd.batch import Batch, batched_mix, build_problem
     mix = batched_mix(problems)
     say(f"building {len(mix)} problems")
-    probs = [build_problem(*k) for k in mix]
+    probs = [build_problem(*k, fp64_copies=True) for k in mix]
     total_dofs = sum(p["fs"].undf * p["nrhs"] for p in probs)
     say(f"built: {total_dofs} dofs x RHS in total")
 
@@ -695,7 +695,8 @@ def batched_record(L, problems, steps, say, colour=True):
     return {
         "workload": f"{len(probs)} uneven W0 SPD problems (C12/C24/C48, 10/30/70 levels, order 0/1), element "
                     "matrices B B^T + 0.1 I with B ~ N(0,1)/8, 4 RHS each, Jacobi CG to 1e-8 in FP64 refinement, "
-                    "one batch",
+                    "one batch; FP32 and FP64 operators with their row-gathered (order 0) or symmetric-packed "
+                    "(order 1) copies",
         "multi_rhs_apply": "operator copies: gather over the row-gathered copy (order 0), symmetric-packed "
                            "upper triangles in two phases (order 1)" if colour else "gather form",
         "problems": len(probs), "rhs_total": nrhs_total, "dofs_total": total_dofs,

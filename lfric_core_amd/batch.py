@@ -29,17 +29,20 @@ def prepare_batch_copies(op32):
     return op32
 
 
-def build_problem(n, nl, order, nrhs, seed, scaled=False, copies=True):
+def build_problem(n, nl, order, nrhs, seed, scaled=False, copies=True, fp64_copies=False):
     """One batch problem on the device: the cubed-sphere W0 space, the FP64 operator
     (A_e = B_e B_e^T + 0.1 I from the counter-based generator) and its FP32 copy (with the
-    batch copies of prepare_batch_copies when `copies`), the FP32 Jacobi diagonal,
-    b ~ U(-1, 1) with nrhs columns and x = 0."""
+    batch copies of prepare_batch_copies when `copies`; on the FP64 operator too, for the
+    refinement steps' residuals, when `fp64_copies`), the FP32 Jacobi diagonal, b ~ U(-1, 1)
+    with nrhs columns and x = 0."""
     from .fem import FieldVector, FunctionSpace, LmaOperator
     fs = FunctionSpace.cubedsphere(n, nl, W0, order)
     op64 = LmaOperator(fs).generate_spd(seed, scaled=scaled)
     op32 = op64.convert(R32)
     if copies:
         prepare_batch_copies(op32)
+    if fp64_copies:
+        prepare_batch_copies(op64)
     d32 = op32.mm_diagonal(FieldVector(fs, R32))
     b = FieldVector(fs, R64, nrhs).fill_uniform(seed, 3)
     x = FieldVector(fs, R64, nrhs)

@@ -426,6 +426,13 @@ __global__ void k_ir_accumulate(double *__restrict__ x, const float *__restrict_
     x[f] = x[f] + scale[f % nr] * static_cast<double>(d[f]);
 }
 
+// *flag = 1 if any entry of x is not zero (set to 0 beforehand)
+__global__ __launch_bounds__(kVecBlock) void k_nonzero(const double *__restrict__ x, long long n, int *flag) {
+  for (long long f = static_cast<long long>(blockIdx.x) * kVecBlock + threadIdx.x; f < n;
+       f += static_cast<long long>(gridDim.x) * kVecBlock)
+    if (x[f] != 0.0) *flag = 1;
+}
+
 // r = b - r (in place), per-block partial sums of r.r per RHS
 __global__ __launch_bounds__(kVecBlock) void k_ir_residual(double *__restrict__ r, const double *__restrict__ b,
                                                            long long n, int nr, double *__restrict__ partials) {
@@ -938,9 +945,9 @@ int run_iterations(lfric_batch &B, int k) {
 // FP64 refinement step of one problem (the outer loop of the mixed-precision CG): fold the
 // scaled inner solution into x, take b - A64 x in FP64, test each RHS, and start the next
 // inner solve on r / ||r|| for the RHS that still need one
-// per problem of a refinement pass: scale, 1/||r||, inner tolerance, live flags (as ints) and
-// the residual norms, kMaxR each, in the batch's device and pinned host scratch
-constexpr int kSlot = 5 * kMaxR;
+// per problem of a refinement pass: scale, 1/||r||, inner tolerance, live flags (as ints), the
+// residual norms and a nonzero-x flag, kMaxR each, in the batch's device and pinned host scratch
+constexpr int kSlot = 6 * kMaxR;
 
 int ensure_slots(lfric_batch &B, size_t n) {
   if (n <= B.scratch_slots) return LFRIC_OK;
@@ -968,6 +975,20 @@ int ensure_slots(lfric_batch &B, size_t n) {
 int refine_all(lfric_batch &B, const std::vector<Problem *> &ps) {
   if (ps.empty()) return LFRIC_OK;
   if (int rc = ensure_slots(B, ps.size())) return rc;
+  // a problem's first step with x = 0 (the usual start) has A64 x = 0: its FP64 apply is
+  // replaced by zeros (one more host wait for all such problems together)
+  bool first = false;
+  for (size_t i = 0; i < ps.size(); ++i) {
+    Problem &p = *ps[i];
+    if (p.started) continue;
+    int *flag = reinterpret_cast<int *>(B.d_scratch + i * kSlot + 5 * kMaxR);
+    LF_HIP(hipMemsetAsync(flag, 0, sizeof(int), cur_stream()));
+    LAUNCHB(k_nonzero, grid_for(p.n, kVecBlock, 2048), kVecBlock, static_cast<const double *>(p.x->d), p.n, flag);
+    LF_HIP(hipMemcpyAsync(B.h_scratch + i * kSlot + 5 * kMaxR, flag, sizeof(int), hipMemcpyDeviceToHost,
+                          cur_stream()));
+    first = true;
+  }
+  if (first) LF_HIP(hipStreamSynchronize(cur_stream()));
   for (size_t i = 0; i < ps.size(); ++i) {
     Problem &p = *ps[i];
     double *d = B.d_scratch + i * kSlot, *h = B.h_scratch + i * kSlot;
@@ -978,7 +999,11 @@ int refine_all(lfric_batch &B, const std::vector<Problem *> &ps) {
       LF_HIP(hipMemcpyAsync(d, h, sizeof(double) * nr, hipMemcpyHostToDevice, cur_stream()));
       LAUNCHB(k_ir_accumulate, grid_for(n, kBlock, 8192), kBlock, static_cast<double *>(p.x->d), p.d, d, n, nr);
     }
-    if (int rc = launch_lma_apply_mrhs(p.A64, p.x->d, p.r64, nr, LFRIC_MRHS_AUTO)) return rc;
+    if (!p.started && *reinterpret_cast<const int *>(h + 5 * kMaxR) == 0) {
+      LF_HIP(hipMemsetAsync(p.r64, 0, sizeof(double) * static_cast<size_t>(n), cur_stream()));
+    } else if (int rc = launch_lma_apply_mrhs(p.A64, p.x->d, p.r64, nr, LFRIC_MRHS_AUTO)) {
+      return rc;
+    }
     LAUNCHB(k_ir_residual, kIrBlocks, kVecBlock, p.r64, static_cast<const double *>(p.b->d), n, nr, B.d_partials);
     LAUNCHB(k_ir_norms, 1, kFinBlock, B.d_partials, kIrBlocks, nr, d + 4 * kMaxR);
     LF_HIP(hipMemcpyAsync(h + 4 * kMaxR, d + 4 * kMaxR, sizeof(double) * nr, hipMemcpyDeviceToHost, cur_stream()));

@@ -22,7 +22,7 @@ def test_batched_configuration_64_problems(hip):
     mix = batched_mix(64, seed=0)
     assert len(mix) == 64 and {m[0] for m in mix} == {12, 24, 48} and {m[2] for m in mix} == {0, 1}
     t0 = time.perf_counter()
-    probs = [build_problem(*k) for k in mix]
+    probs = [build_problem(*k, fp64_copies=True) for k in mix]
     print(f"built 64 problems in {time.perf_counter() - t0:.1f} s", flush=True)
     bt = Batch(r_tol=1e-8, a_tol=0.0, max_iter=4000, check_every=8)
     ids = [bt.add(p["op64"], p["op32"], p["d32"], p["x"], p["b"]) for p in probs]
