@@ -956,14 +956,13 @@ __global__ __launch_bounds__(kBlock) void k_sym_units_s(const T *__restrict__ sy
 
 // One unit per thread (the default copy).  Cells take slots in cell-id order, so that the
 // threads of a window of blocks work on neighbouring cells and share their x entries in one
-// L2: thread t = k*(nl+1) + m owns unit (k-th cell, level m), the cell at colour-major position
+// L2: unit t = k*(nl+1) + m is (k-th cell, level m), the cell at colour-major position
 // slotpos[k].  Slot t of the copy holds the triangle of that cell's layer-m element (m < nl; the
-// slot of m = nl is empty), entry q at (t/64)*gss + ((q/V)*64 + t%64)*V + q%V.  The thread forms
-// the bottom rows' chains c_i of its element (x at levels m, m+1) and the top rows' chains of
-// the same element, which belong to unit m+1: the next lane takes them with one shuffle.  Lane 0
-// of a wavefront has no such neighbour and forms them itself from the layer-(m-1) triangle, one
-// extra triangle per 64 units.  Every chain is the colour kernel's (same entries, same x, same
-// order); the unit's 4 outputs go to z[(k*4 + i)*(nl+1) + m].
+// slot of m = nl is empty), entry q at (t/64)*gss + ((q/V)*64 + t%64)*V + q%V.  A unit's thread
+// forms the bottom rows' chains c_i of its element (x at levels m, m+1) and the top rows' chains
+// of the same element, which belong to unit m+1: the next lane takes them with one shuffle.
+// Every chain is the colour kernel's (same entries, same x, same order); the unit's 4 outputs
+// go to z[(k*4 + i)*(nl+1) + m].
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_build_sym1(const T *__restrict__ A, size_t gs, const int *__restrict__ ebase,
                                                        const int *__restrict__ slotpos, T *__restrict__ sym, size_t gss,
@@ -994,7 +993,12 @@ __global__ __launch_bounds__(kBlock) void k_build_sym1(const T *__restrict__ A, 
   if (mismatch) *asym = 1;
 }
 
-// cellcol[k]: the 0-based base dofs of the k-th cell's 4 bottom vertex columns
+// cellcol[k]: the 0-based base dofs of the k-th cell's 4 bottom vertex columns.  A wavefront
+// takes 63 new units and repeats, in lane 0, the previous wavefront's last unit (whose outputs
+// it does not write), so that every unit's lower neighbour is the previous lane; s1_threads
+// gives the launch's thread count.
+inline long long s1_threads(long long units) { return (units + 62) / 63 * 64; }
+
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_sym_units1(const T *__restrict__ sym, size_t gss, const T *__restrict__ x,
                                                        const int4 *__restrict__ cellcol, T *__restrict__ z, int ncells,
@@ -1003,13 +1007,15 @@ __global__ __launch_bounds__(kBlock) void k_sym_units1(const T *__restrict__ sym
   constexpr int V = chunk_entries<T>(), NCH = 36 / V;
   typedef T VecT __attribute__((ext_vector_type(V)));
   const int H = nl + 1;
-  const long long t = static_cast<long long>(xcd_block(blockIdx.x, gridDim.x, xwin)) * kBlock + threadIdx.x;
-  const bool valid = t < static_cast<long long>(ncells) * H;
+  const long long g = static_cast<long long>(xcd_block(blockIdx.x, gridDim.x, xwin)) * kBlock + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  const long long t = (g >> 6) * 63 + lane - 1;
+  const bool valid = t >= 0 && t < static_cast<long long>(ncells) * H;
   const int k = valid ? static_cast<int>(t / H) : 0;
   const int m = valid ? static_cast<int>(t - static_cast<long long>(k) * H) : 0;
   const bool has_m = valid && m < nl, has_p = valid && m > 0;
-  const int lane = threadIdx.x & 63;
-  const T *St = sym + static_cast<size_t>(t >> 6) * gss + static_cast<size_t>(t & 63) * V;
+  const long long tc = valid ? t : 0;
+  const T *St = sym + static_cast<size_t>(tc >> 6) * gss + static_cast<size_t>(tc & 63) * V;
   VecT tri[NCH];
 #pragma unroll
   for (int c = 0; c < NCH; ++c)
@@ -1044,31 +1050,11 @@ __global__ __launch_bounds__(kBlock) void k_sym_units1(const T *__restrict__ sym
     c[i] = s;
     cn[i] = sn;
   }
-  // the top rows of the layer-(m-1) element: from the previous lane, or formed here by lane 0
+  // the top rows of the layer-(m-1) element: the previous lane's unit (m-1 of the same cell)
   T cp[4];
 #pragma unroll
   for (int i = 0; i < 4; ++i) cp[i] = __shfl_up(cn[i], 1, 64);
-  if (lane == 0 && has_p) {
-    const long long tp = t - 1;
-    const T *Sp = sym + static_cast<size_t>(tp >> 6) * gss + static_cast<size_t>(tp & 63) * V;
-    VecT pt[NCH];
-#pragma unroll
-    for (int q = 0; q < NCH; ++q) pt[q] = *reinterpret_cast<const VecT *>(Sp + static_cast<size_t>(q) * 64 * V);
-    T xm[4];
-#pragma unroll
-    for (int j = 0; j < 4; ++j) xm[j] = x[b[j] + m - 1];
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      T sn = T(0);
-#pragma unroll
-      for (int j = 0; j < 8; ++j) {
-        const int q = j >= i + 4 ? tri_index(i + 4, j) : tri_index(j, i + 4);
-        sn = fma(pt[q / V][q % V], j < 4 ? xm[j] : x0[j - 4], sn);
-      }
-      cp[i] = sn;
-    }
-  }
-  if (!valid) return;
+  if (!valid || lane == 0) return;
   T *zp = z + static_cast<size_t>(k) * 4 * H;
 #pragma unroll
   for (int i = 0; i < 4; ++i) zp[static_cast<size_t>(i) * H + m] = c[i] + (has_p ? cp[i] : T(0));
@@ -2398,7 +2384,7 @@ static void launch_gather_r(const lfric_op *op, const T *x, T *y, R roles, Gathe
     if (op->sym) {  // symmetric-packed copy (launch_build_sym): two phases
       const int nseg = op->sym_seg > 0 ? (to.nlayers + 1 + op->sym_seg - 1) / op->sym_seg : 0;
       if (op->sym_seg == 0)
-        LAUNCH((k_sym_units1<T>), grid_for(static_cast<long long>(to.ncells) * (to.nlayers + 1)),
+        LAUNCH((k_sym_units1<T>), grid_for(s1_threads(static_cast<long long>(to.ncells) * (to.nlayers + 1))),
                static_cast<const T *>(op->sym), op->gss, x, op->d_symcol, static_cast<T *>(op->symz), to.ncells,
                to.nlayers, ex.skip, op->xwin);
       else if (op->sym_seg == 4)
