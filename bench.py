@@ -37,7 +37,7 @@ HBM_PEAK_GBS = 8000.0
 # when a kernel changes.
 TRAFFIC_BYTES_PER_APPLY_FP32 = {("colour", 96, 70): 4 * (288.4e6 + 16.8e6),
                                ("gather", 96, 70): 1074.05e6 + 15.70e6,
-                               ("symmetric", 96, 70): 673.5e6 + 65.9e6}
+                               ("symmetric", 96, 70): 649.4e6 + 65.9e6}
 
 
 def apply_bytes(fs, ndf, s_a, s_x):

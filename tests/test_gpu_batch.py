@@ -107,7 +107,7 @@ def test_batch_copy_forms_equal_colour_and_gather_forms(hip, orc):
     row-gathered copy equals the gather form bit for bit, so each problem's solution and
     iteration counts are those of the batch run in that form."""
     from lfric_core_amd.batch import prepare_batch_copies
-    probs = [_make(hip, *k) for k in MIX]
+    probs = [_make(hip, *k) for k in MIX + [(6, 5, 1, 4, 6), (3, 3, 1, 1, 7)]]
     for p in probs:
         prepare_batch_copies(p["op32"])
 
