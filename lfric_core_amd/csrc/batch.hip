@@ -248,7 +248,6 @@ __global__ __launch_bounds__(kBlock) void k_batch_sym27_gather(const ProbDev *__
     if (tile >= t_end) return;
     const ProbDev &d = P[find_problem(tile0, nprob, tile)];
     const long long t0 = static_cast<long long>(tile - d.apply_tile0) * kBlock * kSymPts + threadIdx.x;
-#pragma unroll
     for (int j = 0; j < kSymPts; ++j) {
       VR acc;
       size_t dof = 0;
