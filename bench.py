@@ -110,6 +110,7 @@ def order1_report(L, reps, n=48, nl=70):
     for nr in (1, 4):
         out[f"rhs{nr}"] = {"colour_ms": timed(nr, 2), "gather_ms": timed(nr, 1)}
     op.prepare_symmetric()
+    out["symmetric_window"] = op.tune(1)[0]  # XCD block order of the first phase, by timing
     for nr in (1, 4):
         r = out[f"rhs{nr}"]
         r["symmetric_ms"] = timed(nr, 1)

@@ -1177,14 +1177,15 @@ __global__ __launch_bounds__(kBlock) void k_build_sym27(const T *__restrict__ A,
   if (mismatch) *asym = 1;
 }
 
-// phase 1: L = NR/NRL lanes per unit, s27_threads(units) threads
+// phase 1: L = NR/NRL lanes per unit, s27_threads(units) threads; xwin: the XCD block order
 template <typename T, int NR, int NRL>
 __global__ __launch_bounds__(kBlock) void k_sym27_units(const T *__restrict__ sym, size_t gss, const T *__restrict__ x,
                                                         const int *__restrict__ map2, const int *__restrict__ slotpos,
-                                                        T *__restrict__ z, long long units, int nl, const int *skip) {
+                                                        T *__restrict__ z, long long units, int nl, const int *skip,
+                                                        int xwin) {
   if (skip && *skip) return;
   sym27_unit<T, NR, NRL>(sym, gss, x, map2, slotpos, z, units, nl,
-                         static_cast<long long>(blockIdx.x) * kBlock + threadIdx.x);
+                         static_cast<long long>(xcd_block(blockIdx.x, gridDim.x, xwin)) * kBlock + threadIdx.x);
 }
 
 // phase 2, one point per thread; DOT (one RHS): the block's partial sum of x.y as well
@@ -2356,7 +2357,7 @@ static int launch_sym27(const lfric_op *op, const T *x, T *y, double *partials, 
     z = static_cast<T *>(op->symz_mr);
   }
   LAUNCH((k_sym27_units<T, NR, NRL>), grid_for(s27_threads<T, NR>(units)), static_cast<const T *>(op->sym), op->gss,
-         x, from.d_map, op->d_sympos, z, units, to.nlayers, skip);
+         x, from.d_map, op->d_sympos, z, units, to.nlayers, skip, op->xwin);
   if (g_phase_mark) (void)hipEventRecord(g_phase_mark, cur_stream());
   const int grid = gather_blocks(op);
   if (partials) {
