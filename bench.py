@@ -356,6 +356,8 @@ def main():
                     help="single: C96x70 mixed vs FP64 solve (default); batched: 64 uneven multi-RHS problems "
                          "in one batch vs sequential FP64 solves; served: Poisson request stream with "
                          "continuous batching vs sequential FP64 solves")
+    ap.add_argument("--no-helmholtz", action="store_true",
+                    help="single workload: skip the Helmholtz-like GMRES configuration's record")
     ap.add_argument("--problems", type=int, default=64)
     ap.add_argument("--batched-problems", type=int, default=64,
                     help="single workload: also run the batched configuration with this many problems (0: skip)")
@@ -559,8 +561,10 @@ def main():
     if not args.no_cpu_baseline:
         bh = b.export_field()
         line["cpu_baseline"] = cpu_baseline(fs, op64, bh, pres.iterations, args.cpu_budget, small_host)
+    del op32, op64, prec32, prec64, xs32, ys32, xs64, ys64, mixed, plain
+    if not args.no_helmholtz:  # the Helmholtz-like configuration beside the single solve
+        line["helmholtz"] = helmholtz_record(L)
     if args.batched_problems > 0:  # the batched configuration beside the single solve
-        del op32, op64, prec32, prec64, xs32, ys32, xs64, ys64, mixed, plain
         say = lambda msg: print(f"[batched] {msg}", file=sys.stderr, flush=True)
         line["batched"] = batched_record(L, args.batched_problems, 1, say)
     print(json.dumps(line))
@@ -571,6 +575,65 @@ def _common(args, metric, value, unit, ms, config, dtype):
             "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
             "vs_baseline": None, "dtype": dtype, "data": "synthetic (counter-based generator)",
             "config": config}
+
+
+# divergence sign pattern of the Helmholtz-like configuration: outward normals of the faces
+# W, S, E, N, B, T of a W2 cell
+HELMHOLTZ_FACE_SIGNS = (-1.0, -1.0, 1.0, 1.0, -1.0, 1.0)
+
+
+def helmholtz_record(L, reps=3):
+    """The Helmholtz-like configuration: C48 cubed sphere, 50 levels, W2 (2,087,424 dofs) and W3
+    (691,200 dofs); M2 = C C^T + I, D = face signs x U(0.5, 1.5), M3 SPD, and on W3
+    H = M3 + 0.5 D diag(M2)^-1 D^T.  GMRES(30) to 1e-8 with FP32 applies and the BF16
+    vertical-column block-Jacobi preconditioner (FP32 arithmetic) inside FP64 refinement with
+    FP64 Givens rotations: median of `reps` solves after one warm-up; beside it plain FP64
+    GMRES(30) without a preconditioner, and the FP32 H apply (HIP events)."""
+    w2 = L.FunctionSpace.cubedsphere(48, 50, L.W2, 0)
+    w3 = w2.sibling(L.W3, 0)
+    M2 = L.LmaOperator(w2).generate_spd(0, shift=1.0)
+    D = L.LmaOperator(w3, w2).generate_uniform(0, 5, 0.5, 1.5, HELMHOLTZ_FACE_SIGNS)
+    M3 = L.LmaOperator(w3).generate_spd(1, shift=1.0)
+    H64 = L.LinearOperator.helmholtz(M3, D, M2.mm_diagonal(L.FieldVector(w2)), 0.5)
+    M2s, Ds, M3s = M2.convert(L.R32), D.convert(L.R32), M3.convert(L.R32)
+    H32 = L.LinearOperator.helmholtz(M3s, Ds, M2s.mm_diagonal(L.FieldVector(w2, L.R32)), 0.5)
+    P = L.ColumnJacobiPreconditioner(H32, L.BF16)
+    b = L.FieldVector(w3).fill_uniform(9, 3)
+    x = L.FieldVector(w3)
+
+    def solve(solver):
+        x.set_scalar(0.0)
+        L.synchronize()
+        t = time.perf_counter()
+        res = solver.apply(x, b)
+        L.synchronize()
+        return time.perf_counter() - t, res
+
+    mixed = L.MixedPrecisionGMRES(H64, H32, P, 30, 1e-8, 0.0, 50, inner_r_tol=1e-4)
+    solve(mixed)
+    runs = [solve(mixed) for _ in range(reps)]
+    t_mixed = float(np.median([t for t, _ in runs]))
+    res = runs[-1][1]
+    plain = L.GMRES(H64, L.NullPreconditioner(), 30, 1e-8, 0.0, 200)
+    t_plain, res64 = solve(plain)
+    x32, y32 = L.FieldVector(w3, L.R32).fill_uniform(1, 3), L.FieldVector(w3, L.R32)
+    t_apply = time_applies(L, H32, x32, y32, 20, "apply")
+    # one H apply: D^T (6 entries per cell-layer) and D read once each, M3 once; x read twice, the
+    # W2 intermediate written and read, y written
+    nc3 = w3.ncell_3d
+    apply_bytes = nc3 * (6 + 6 + 1) * 4 + w3.undf * 3 * 4 + w2.undf * 2 * 4
+    return {
+        "workload": "C48x50 W2/W3 Helmholtz-like H = M3 + 0.5 D diag(M2)^-1 D^T on W3 (691,200 dofs), GMRES(30) "
+                    "to 1e-8: FP32 applies, BF16 column block-Jacobi, FP64 refinement and Givens",
+        "mixed_ms": t_mixed * 1e3, "mixed_ms_all": [t * 1e3 for t, _ in runs],
+        "solved_dofs_per_s": w3.undf / t_mixed,
+        "iterations": {"inner": res.iterations, "outer": res.outer_iterations},
+        "true_rel_res": res.true_rel_res,
+        "plain_fp64_gmres_no_preconditioner": {"ms": t_plain * 1e3, "iterations": res64.iterations,
+                                                "status": res64.status},
+        "fp32_h_apply": {"ms": t_apply, "bytes": apply_bytes,
+                         "gbs": apply_bytes / (t_apply * 1e-3) / 1e9},
+    }
 
 
 def batch_apply_bytes(probs, copies):
