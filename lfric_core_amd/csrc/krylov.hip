@@ -186,13 +186,54 @@ __global__ __launch_bounds__(kBlock) void k_mgs_step(T *__restrict__ w, const T 
   block_partials<1>(v, partials);
 }
 
-__global__ __launch_bounds__(kBlock) void k_finish_to(const double *__restrict__ partials, int nblocks, double *dst,
-                                                      int take_sqrt) {
+// the sum of a reduction's partials by one block in a fixed order (strided per-thread sums,
+// then block_sum's tree); every thread gets it
+__device__ inline double finish_sum(const double *__restrict__ partials, int nblocks) {
   double s[1] = {0.0};
   for (int b = threadIdx.x; b < nblocks; b += kBlock) s[0] += partials[b];
   double tot[1];
   block_sum<1>(s, tot);
-  if (threadIdx.x == 0) *dst = take_sqrt ? sqrt(tot[0]) : tot[0];
+  return tot[0];
+}
+
+__global__ __launch_bounds__(kBlock) void k_finish_to(const double *__restrict__ partials, int nblocks, double *dst,
+                                                      int take_sqrt) {
+  const double tot = finish_sum(partials, nblocks);
+  if (threadIdx.x == 0) *dst = take_sqrt ? sqrt(tot) : tot;
+}
+
+// k_mgs_step for j > 0 with the previous step's finish folded in: every block adds the previous
+// step's partials itself (finish_sum, the same sum as k_finish_to's) to get h(j-1, iv), block 0
+// stores it, then w = w - h(j-1) v_{j-1} and the partials of v_j . w (or w . w) go to partials.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_mgs_fold(T *__restrict__ w, const T *__restrict__ vprev,
+                                                     const T *__restrict__ vj, long long n,
+                                                     const double *__restrict__ pprev, int nprev, double *hprev,
+                                                     double *partials) {
+  const double h = finish_sum(pprev, nprev);
+  if (blockIdx.x == 0 && threadIdx.x == 0) *hprev = h;
+  const T a = static_cast<T>(-h);
+  double v[1] = {0.0};
+  for (long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += static_cast<long long>(gridDim.x) * blockDim.x) {
+    const T wi = w[i] + a * vprev[i];
+    w[i] = wi;
+    v[0] += vj ? to_d(vj[i]) * to_d(wi) : to_d(wi) * to_d(wi);
+  }
+  block_partials<1>(v, partials);
+}
+
+// k_scale_by_inv with the finish of ||w||^2 folded in: h = sqrt(sum of the partials), block 0
+// stores it, v = w * (1 / h)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_scale_fold(T *__restrict__ v, const T *__restrict__ w, long long n,
+                                                       const double *__restrict__ partials, int np, double *hout) {
+  const double h = sqrt(finish_sum(partials, np));
+  if (blockIdx.x == 0 && threadIdx.x == 0) *hout = h;
+  const T s = static_cast<T>(1.0 / h);
+  for (long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += static_cast<long long>(gridDim.x) * blockDim.x)
+    v[i] = w[i] * s;
 }
 
 template <typename T>
@@ -335,8 +376,34 @@ static int gm_ws(int slot, size_t bytes, void **out) {
 static int gm_init() {
   if (g_gstate) return LFRIC_OK;
   if (int rc = dmalloc(reinterpret_cast<void **>(&g_gstate), sizeof(GmresState))) return rc;
-  if (int rc = dmalloc(reinterpret_cast<void **>(&g_gpart), sizeof(double) * kGmBlocks)) return rc;
+  if (int rc = dmalloc(reinterpret_cast<void **>(&g_gpart), sizeof(double) * 2 * kGmBlocks)) return rc;
   LF_HIP(hipHostMalloc(reinterpret_cast<void **>(&g_host_scalar), sizeof(double) * 4, hipHostMallocDefault));
+  return LFRIC_OK;
+}
+
+// The Gram-Schmidt steps j = 0 .. iv+1 of Arnoldi column iv (w against v_0 .. v_iv, then
+// ||w||), and v_{iv+1} = w / h(iv+1, iv) unless iv is the cycle's last column.  Each step's
+// partial sums are finished by the next kernel's blocks themselves (partials in two alternating
+// halves of the scratch), so a step is one launch; the h values are those of separate
+// finishing kernels bit for bit.
+template <typename T>
+static int arnoldi_mgs(T *w, T *V, long long n, int iv, int m, double *h) {
+  const int vg = grid_for(n);
+  double *part[2] = {g_gpart, g_gpart + kGmBlocks};
+  LAUNCHK(k_mgs_step<T>, kGmBlocks, kBlock, w, static_cast<const T *>(nullptr), static_cast<const T *>(V), n,
+          static_cast<const double *>(nullptr), part[0]);
+  for (int j = 1; j <= iv + 1; ++j) {
+    const T *vj = j <= iv ? V + static_cast<size_t>(j) * n : nullptr;
+    LAUNCHK(k_mgs_fold<T>, kGmBlocks, kBlock, w, V + static_cast<size_t>(j - 1) * n, vj, n, part[(j - 1) & 1],
+            kGmBlocks, &h[iv * (m + 1) + (j - 1)], part[j & 1]);
+  }
+  double *hlast = &h[iv * (m + 1) + iv + 1];
+  if (iv < m - 1)
+    LAUNCHK(k_scale_fold<T>, std::min(vg, kGmBlocks), kBlock, V + static_cast<size_t>(iv + 1) * n, w, n,
+            part[(iv + 1) & 1], kGmBlocks, hlast);
+  else
+    LAUNCHK(k_finish_to, 1, kBlock, part[(iv + 1) & 1], kGmBlocks, hlast, 1);
+  LF_HIP(hipGetLastError());
   return LFRIC_OK;
 }
 
@@ -406,16 +473,7 @@ static int gmres_t(const lfric_linop *A, const lfric_prec *P, T *x, const T *b, 
       T *viv = V + static_cast<size_t>(iv) * n;
       if (int rc = linop_apply(A, viv, s)) return rc;  // s = A w (w = v_iv)
       if (int rc = prec_apply_any(P, prec, s, w, n)) return rc;  // w = P^-1 s
-      for (int j = 0; j <= iv + 1; ++j) {
-        const T *vprev = j > 0 ? V + static_cast<size_t>(j - 1) * n : nullptr;
-        const T *vj = j <= iv ? V + static_cast<size_t>(j) * n : nullptr;
-        const double *hprev = j > 0 ? &st->h[iv * (m + 1) + (j - 1)] : nullptr;
-        LAUNCHK(k_mgs_step<T>, kGmBlocks, kBlock, w, vprev, vj, n, hprev, g_gpart);
-        LAUNCHK(k_finish_to, 1, kBlock, g_gpart, kGmBlocks, &st->h[iv * (m + 1) + j], j == iv + 1 ? 1 : 0);
-      }
-      if (iv < m - 1)
-        LAUNCHK(k_scale_by_inv<T>, vg, kBlock, V + static_cast<size_t>(iv + 1) * n, w, n,
-                &st->h[iv * (m + 1) + iv + 1]);
+      if (int rc = arnoldi_mgs<T>(w, V, n, iv, m, st->h)) return rc;
     }
     LAUNCHK(k_givens, 1, 64, st, m);
     LAUNCHK(k_gmres_update<T>, vg, kBlock, x, V, n, m, st->u);
@@ -485,16 +543,7 @@ static int fgmres_t(const lfric_linop *A, const lfric_prec *P, T *x, const T *b,
       T *viv = V + static_cast<size_t>(iv) * n, *pviv = PV + static_cast<size_t>(iv) * n;
       if (int rc = prec_apply_any(P, prec, viv, pviv, n)) return rc;  // Pv = P^-1 v
       if (int rc = linop_apply(A, pviv, w)) return rc;               // w = s = A Pv
-      for (int j = 0; j <= iv + 1; ++j) {
-        const T *vprev = j > 0 ? V + static_cast<size_t>(j - 1) * n : nullptr;
-        const T *vj = j <= iv ? V + static_cast<size_t>(j) * n : nullptr;
-        const double *hprev = j > 0 ? &st->h[iv * (m + 1) + (j - 1)] : nullptr;
-        LAUNCHK(k_mgs_step<T>, kGmBlocks, kBlock, w, vprev, vj, n, hprev, g_gpart);
-        LAUNCHK(k_finish_to, 1, kBlock, g_gpart, kGmBlocks, &st->h[iv * (m + 1) + j], j == iv + 1 ? 1 : 0);
-      }
-      if (iv < m - 1)
-        LAUNCHK(k_scale_by_inv<T>, vg, kBlock, V + static_cast<size_t>(iv + 1) * n, w, n,
-                &st->h[iv * (m + 1) + iv + 1]);
+      if (int rc = arnoldi_mgs<T>(w, V, n, iv, m, st->h)) return rc;
     }
     LAUNCHK(k_givens, 1, 64, st, m);
     LAUNCHK(k_gmres_update<T>, vg, kBlock, dx, PV, n, m, st->u);  // dx += sum u_iv Pv_iv
