@@ -205,5 +205,6 @@ struct lfric_linop {
   std::shared_ptr<lfric::FunctionSpace> space;  // domain and range
   void *t_w2 = nullptr, *t_w3 = nullptr;      // scratch of the composite apply
   size_t t_w2_bytes = 0, t_w3_bytes = 0;
+  mutable int fused = -1;  // kind 1: the two-kernel composite applies (set on first use)
   ~lfric_linop();
 };

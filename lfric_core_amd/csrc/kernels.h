@@ -71,6 +71,11 @@ bool mfma_apply_supported(const lfric_op *op);
 // per thread with register-staged x (order-1 W0), 0 none (gather form)
 int batch_colour_form(const lfric_op *op);
 int launch_lma_transpose(const lfric_op *op, const void *x, void *y);
+// the Helmholtz composite in two kernels (lowest-order W3 <- W2), equal to the chain of
+// transpose, division, divergence, M3 apply and axpy bit for bit; helmholtz_fusable says
+// whether it applies to the operator
+bool helmholtz_fusable(const lfric_linop *L);
+int launch_helmholtz_fused(const lfric_linop *L, const void *x, void *y);
 int launch_dg_apply(const lfric_op *op, const void *x, void *y);
 int launch_dg_inc_apply(const lfric_op *op, const void *x, void *y);
 // enforce_bc on a vector of space f (no-op when the space flags no boundary dofs)

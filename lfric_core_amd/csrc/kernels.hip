@@ -1348,6 +1348,63 @@ __global__ __launch_bounds__(kBlock) void k_transpose_colour(const T *__restrict
   }
 }
 
+// ---- Helmholtz composite y = M3 x + tau D diag(M2)^-1 D^T x in two kernels ------------------
+// (W3 <- W2 lowest order: D holds 1 x 6 per element, M3 1 x 1.)  The first is the transpose in
+// gather form with the division by m2 folded in: W2 dof (column, level m) adds its cells'
+// contributions in colour order, the first written and the rest added, each the colour-
+// ordered transpose's chain (row of layer m, then the top partner's of layer m-1), then divides
+// by m2: the transpose followed by k_div_inplace, bit for bit.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_helm_transpose_div(const T *__restrict__ D, size_t gs,
+                                                              const int *__restrict__ ebase, const T *__restrict__ x3,
+                                                              const int *__restrict__ map3, const T *__restrict__ m2,
+                                                              T *__restrict__ t, const int2 *__restrict__ cols,
+                                                              const int *__restrict__ off, const int *__restrict__ ent,
+                                                              int ncols, int nl, Roles roles) {
+  const int H = nl + 1;
+  const long long p = static_cast<long long>(blockIdx.x) * kBlock + threadIdx.x;
+  if (p >= static_cast<long long>(ncols) * H) return;
+  const int col = static_cast<int>(p / H), m = static_cast<int>(p - static_cast<long long>(col) * H);
+  const int2 ci = cols[col];
+  if (m == nl && !ci.y) return;
+  const bool has_m = m < nl, has_p = m > 0;
+  T total = T(0);
+  const int q0 = off[col], q1 = off[col + 1];
+  for (int q = q0; q < q1; ++q) {
+    const int en = ent[q], pos = en >> 5, j = en & 31;
+    const int pt = roles.partner[j];
+    const long long e = static_cast<long long>(ebase[pos]) + m;
+    const int b3 = map3[pos];
+    T acc = T(0);
+    if (has_m) acc = fma(D[aidx<T>(gs, e, j)], x3[b3 + m], acc);
+    if (pt >= 0 && has_p) acc = fma(D[aidx<T>(gs, e - 1, pt)], x3[b3 + m - 1], acc);
+    total = q == q0 ? acc : total + acc;
+  }
+  const size_t d = static_cast<size_t>(ci.x) + m;
+  t[d] = total / m2[d];
+}
+
+// the second: per W3 dof (cell, layer m) the divergence row (k_lma_colour's chain over the 6
+// faces), M3's 1 x 1 product and y = M3 x + tau (D t) as k_axpy forms it
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_helm_dg_m3(const T *__restrict__ D, size_t gsd, const T *__restrict__ M3,
+                                                      size_t gs3, const int *__restrict__ ebase,
+                                                      const T *__restrict__ t, const int *__restrict__ map2,
+                                                      const T *__restrict__ x3, const int *__restrict__ map3,
+                                                      T *__restrict__ y, int ncells, int nl, T tau) {
+  const long long u = static_cast<long long>(blockIdx.x) * kBlock + threadIdx.x;
+  if (u >= static_cast<long long>(ncells) * nl) return;
+  const int pos = static_cast<int>(u / nl), m = static_cast<int>(u - static_cast<long long>(pos) * nl);
+  const long long e = static_cast<long long>(ebase[pos]) + m;
+  const int *b2 = map2 + static_cast<size_t>(pos) * 6;
+  T acc = T(0);
+#pragma unroll
+  for (int j = 0; j < 6; ++j) acc = fma(D[aidx<T>(gsd, e, j)], t[b2[j] + m], acc);
+  const size_t d = static_cast<size_t>(map3[pos]) + m;
+  const T y3 = fma(M3[aidx<T>(gs3, e, 0)], x3[d], T(0));
+  y[d] = y3 + tau * acc;
+}
+
 // assembled diagonal: sum over cells of A_k(df,df) (sci_mm_diagonal_kernel_mod.F90:87-92)
 template <typename T, int N>
 __global__ __launch_bounds__(kBlock) void k_diag_colour(const T *__restrict__ A, size_t gs, const int *__restrict__ ebase,
@@ -3208,6 +3265,35 @@ int launch_cg_aypx(int prec, void *p, const void *z, long long n, SolverState *s
            update_partials(), kRedBlocks, par);
   CHECK_LAUNCH();
   return LFRIC_OK;
+}
+
+bool helmholtz_fusable(const lfric_linop *L) {
+  const FunctionSpace &w3 = *L->D->to, &w2 = *L->D->from;
+  return L->kind == 1 && L->M3->to == L->D->to && L->M3->from == L->D->to && w3.ndf == 1 && w2.ndf == 6 &&
+         w3.top_mask == 0 && role_kind(w2) == RoleKind::W2K0 && w2.ncells == w3.ncells && w2.nlayers == w3.nlayers &&
+         w2.ebase == w3.ebase && (L->precision == LFRIC_R64 || L->precision == LFRIC_R32);
+}
+
+template <typename T>
+static int helmholtz_t(const lfric_linop *L, const T *x, T *y) {
+  FunctionSpace &w2 = *L->D->from;
+  const FunctionSpace &w3 = *L->D->to;
+  if (int rc = build_gather(w2)) return rc;
+  T *t = static_cast<T *>(L->t_w2);
+  LAUNCH((k_helm_transpose_div<T>), grid_for(static_cast<long long>(w2.gather_ncols) * (w2.nlayers + 1)),
+         static_cast<const T *>(L->D->d), L->D->gs, w3.d_ebase, x, w3.d_map, static_cast<const T *>(L->m2->d), t,
+         w2.d_gcol, w2.d_goff, w2.d_gent, w2.gather_ncols, w2.nlayers, roles_of(w2));
+  LAUNCH((k_helm_dg_m3<T>), grid_for(static_cast<long long>(w3.ncells) * w3.nlayers),
+         static_cast<const T *>(L->D->d), L->D->gs, static_cast<const T *>(L->M3->d), L->M3->gs, w3.d_ebase, t,
+         w2.d_map, x, w3.d_map, y, w3.ncells, w3.nlayers, static_cast<T>(L->tau));
+  CHECK_LAUNCH();
+  return LFRIC_OK;
+}
+
+int launch_helmholtz_fused(const lfric_linop *L, const void *x, void *y) {
+  if (L->precision == LFRIC_R64)
+    return helmholtz_t<double>(L, static_cast<const double *>(x), static_cast<double *>(y));
+  return helmholtz_t<float>(L, static_cast<const float *>(x), static_cast<float *>(y));
 }
 
 }  // namespace lfric

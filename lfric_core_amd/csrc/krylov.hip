@@ -308,6 +308,8 @@ int linop_apply(const lfric_linop *L, const void *x, void *y) {
     return L->apply_bc ? launch_enforce_bc(*L->space, L->precision, y, nullptr) : LFRIC_OK;
   }
   // y = M3 x + tau * D (D^T x / m2)
+  if (L->fused < 0) L->fused = helmholtz_fusable(L) ? 1 : 0;
+  if (L->fused) return launch_helmholtz_fused(L, x, y);
   const long long n2 = L->D->from->undf, n3 = L->space->undf;
   if (int rc = launch_lma_transpose(L->D, x, L->t_w2)) return rc;
   if (L->precision == LFRIC_R64)

@@ -71,6 +71,27 @@ def test_helmholtz_apply_parity(hip, orc, n, nl):
     assert np.linalg.norm(y.export_field() - ref) <= 1e-13 * np.linalg.norm(ref)
 
 
+@pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("n,nl", [(4, 3), (5, 7), (12, 10)])
+def test_helmholtz_two_kernel_apply_equals_the_kernel_chain(hip, orc, n, nl, precision):
+    """The Helmholtz composite's two-kernel apply (transpose in gather form with the division,
+    then divergence, M3 and the axpy in one kernel) equals the chain of separate applies -
+    colour-ordered transpose, division by diag(M2), divergence, M3, y += tau t - bit for bit."""
+    tau = 0.5
+    w2, w3, H, (M2, D, M3, m2d) = _helmholtz(hip, orc, n, nl, precision, tau)
+    x = hip.FieldVector(w3, precision).fill_uniform(3, 3)
+    y = hip.FieldVector(w3, precision)
+    H.apply(x, y)
+    t2, t2d = hip.FieldVector(w2, precision), hip.FieldVector(w2, precision)
+    t3, yc = hip.FieldVector(w3, precision), hip.FieldVector(w3, precision)
+    D.apply_transpose(x, t2)
+    t2d.divide(t2, m2d)
+    D.apply_dg(t2d, t3)
+    M3.apply_dg(x, yc)
+    yc.axpy(tau, t3)
+    np.testing.assert_array_equal(y.export_raw(), yc.export_raw())
+
+
 def _column_solves(Hd, columns, r):
     z = np.zeros_like(r)
     for base, h in columns:
