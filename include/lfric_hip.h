@@ -384,7 +384,10 @@ int lfric_ir_cg_solve(const lfric_op *A64, const lfric_op *A32, const lfric_prec
 int lfric_gmres_solve(const lfric_linop *A, const lfric_prec *P, lfric_vec *x, const lfric_vec *b,
                       const lfric_solver_opts *opts, lfric_solver_result *res);
 /* FP64 iterative refinement around FP32 GMRES: r = b - A64 x in FP64, inner GMRES on A32 with
- * FP32 vectors until the residual falls by inner_r_tol, x += d in FP64. */
+ * FP32 vectors until the residual falls by inner_r_tol, x += d in FP64.  The inner solve reads
+ * the Givens least-squares residual estimate after every Arnoldi column (one column behind the
+ * device) and ends its cycle once the estimate is below the inner tolerance; res->iterations
+ * counts the Arnoldi columns made. */
 int lfric_ir_gmres_solve(const lfric_linop *A64, const lfric_linop *A32, const lfric_prec *P32,
                          lfric_vec *x, const lfric_vec *b, const lfric_solver_opts *opts,
                          lfric_solver_result *res);

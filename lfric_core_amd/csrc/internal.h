@@ -188,7 +188,8 @@ struct lfric_prec {
   int storage = LFRIC_R32;
   long long undf = 0;
   int ncols = 0;
-  int *d_cols = nullptr;   // [ncols][2] base (0-based), height
+  int *d_cols = nullptr;   // [ncols][2] base (0-based), height; sorted by base
+  int tile = 0, cap = 0;   // columns per block of the LDS-tiled apply (0: not used), dofs per tile at most
   void *d_fac = nullptr;   // [3][undf]: lower multiplier, 1/pivot, super-diagonal
   size_t cols_bytes = 0, fac_bytes = 0;
   ~lfric_prec();
@@ -206,5 +207,8 @@ struct lfric_linop {
   void *t_w2 = nullptr, *t_w3 = nullptr;      // scratch of the composite apply
   size_t t_w2_bytes = 0, t_w3_bytes = 0;
   mutable int fused = -1;  // kind 1: the two-kernel composite applies (set on first use)
+  mutable int hcol_state = 0;  // kind 1: 1 the W2 column table is built, -1 the mesh does not allow one
+  mutable void *d_hcol = nullptr;
+  mutable size_t hcol_bytes = 0;
   ~lfric_linop();
 };

@@ -1384,6 +1384,38 @@ __global__ __launch_bounds__(kBlock) void k_helm_transpose_div(const T *__restri
   t[d] = total / m2[d];
 }
 
+// The same transpose with each W2 column's (at most two) cells read from one int4 of a table
+// built for the composite: {ebase, map3 * 8 + face} per cell in the column's colour order, the
+// second ebase -1 when the column has one cell.  The W2 lowest-order roles are fixed (face 4
+// is the bottom, its partner the top face 5), so one level of dependent index loads is left
+// where the gather lists took three; the sums are k_helm_transpose_div's, term for term.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_helm_transpose_div_tab(const T *__restrict__ D, size_t gs,
+                                                                  const T *__restrict__ x3, const T *__restrict__ m2,
+                                                                  T *__restrict__ t, const int2 *__restrict__ cols,
+                                                                  const int4 *__restrict__ tab, int ncols, int nl) {
+  const int H = nl + 1;
+  const int p = blockIdx.x * kBlock + threadIdx.x;  // the table exists only when ncols * H < 2^31
+  if (p >= ncols * H) return;
+  const int col = p / H, m = p - col * H;
+  const int2 ci = cols[col];
+  if (m == nl && !ci.y) return;
+  const int4 c = tab[col];
+  const bool has_m = m < nl, has_p = m > 0;
+  auto cell = [&](int eb, int bj) {
+    const int j = bj & 7, b3 = bj >> 3;
+    const long long e = static_cast<long long>(eb) + m;
+    T acc = T(0);
+    if (has_m) acc = fma(D[aidx<T>(gs, e, j)], x3[b3 + m], acc);
+    if (j == 4 && has_p) acc = fma(D[aidx<T>(gs, e - 1, 5)], x3[b3 + m - 1], acc);
+    return acc;
+  };
+  T total = cell(c.x, c.y);
+  if (c.z >= 0) total = total + cell(c.z, c.w);
+  const size_t d = static_cast<size_t>(ci.x) + m;
+  t[d] = total / m2[d];
+}
+
 // the second: per W3 dof (cell, layer m) the divergence row (k_lma_colour's chain over the 6
 // faces), M3's 1 x 1 product and y = M3 x + tau (D t) as k_axpy forms it
 template <typename T>
@@ -3274,15 +3306,60 @@ bool helmholtz_fusable(const lfric_linop *L) {
          w2.ebase == w3.ebase && (L->precision == LFRIC_R64 || L->precision == LFRIC_R32);
 }
 
+// the composite's column table for k_helm_transpose_div_tab (the gather lists of the W2 space
+// resolved to ebase and W3 dof per cell); none when a column has more than two cells
+static int build_helm_table(const lfric_linop *L) {
+  if (L->hcol_state) return LFRIC_OK;
+  const FunctionSpace &w2 = *L->D->from, &w3 = *L->D->to;
+  struct Item {
+    int base, pos, df;
+  };
+  std::vector<Item> items;
+  for (int p = 0; p < w2.ncells; ++p)
+    for (int d = 0; d < 6; ++d)
+      if (!((w2.top_mask >> d) & 1u)) items.push_back({w2.dofmap[static_cast<size_t>(w2.cell_of_pos[p]) * 6 + d] - 1, p, d});
+  std::stable_sort(items.begin(), items.end(), [](const Item &a, const Item &b) { return a.base < b.base; });
+  std::vector<int4> tab;
+  bool ok = true;
+  for (size_t q = 0; q < items.size() && ok; ++q) {
+    const Item &it = items[q];
+    const int ent = w2.ebase[it.pos], bj = (w3.dofmap[w3.cell_of_pos[it.pos]] - 1) * 8 + it.df;
+    if (q == 0 || it.base != items[q - 1].base) {
+      tab.push_back(make_int4(ent, bj, -1, 0));
+    } else if (tab.back().z < 0) {
+      tab.back().z = ent;
+      tab.back().w = bj;
+    } else {
+      ok = false;
+    }
+  }
+  ok = ok && static_cast<int>(tab.size()) == w2.gather_ncols && static_cast<long long>(w3.undf) * 8 < (1LL << 31) &&
+       static_cast<long long>(w2.gather_ncols) * (w2.nlayers + 1) + kBlock < (1LL << 31);
+  L->hcol_state = -1;
+  if (!ok) return LFRIC_OK;
+  L->hcol_bytes = sizeof(int4) * tab.size();
+  if (int rc = dmalloc(reinterpret_cast<void **>(&L->d_hcol), L->hcol_bytes)) return rc;
+  LF_HIP(hipMemcpy(L->d_hcol, tab.data(), L->hcol_bytes, hipMemcpyHostToDevice));
+  L->hcol_state = 1;
+  return LFRIC_OK;
+}
+
 template <typename T>
 static int helmholtz_t(const lfric_linop *L, const T *x, T *y) {
   FunctionSpace &w2 = *L->D->from;
   const FunctionSpace &w3 = *L->D->to;
   if (int rc = build_gather(w2)) return rc;
+  if (int rc = build_helm_table(L)) return rc;
   T *t = static_cast<T *>(L->t_w2);
-  LAUNCH((k_helm_transpose_div<T>), grid_for(static_cast<long long>(w2.gather_ncols) * (w2.nlayers + 1)),
-         static_cast<const T *>(L->D->d), L->D->gs, w3.d_ebase, x, w3.d_map, static_cast<const T *>(L->m2->d), t,
-         w2.d_gcol, w2.d_goff, w2.d_gent, w2.gather_ncols, w2.nlayers, roles_of(w2));
+  const int tgrid = grid_for(static_cast<long long>(w2.gather_ncols) * (w2.nlayers + 1));
+  if (L->hcol_state == 1)
+    LAUNCH((k_helm_transpose_div_tab<T>), tgrid, static_cast<const T *>(L->D->d), L->D->gs, x,
+           static_cast<const T *>(L->m2->d), t, w2.d_gcol, static_cast<const int4 *>(L->d_hcol), w2.gather_ncols,
+           w2.nlayers);
+  else
+    LAUNCH((k_helm_transpose_div<T>), tgrid, static_cast<const T *>(L->D->d), L->D->gs, w3.d_ebase, x, w3.d_map,
+           static_cast<const T *>(L->m2->d), t, w2.d_gcol, w2.d_goff, w2.d_gent, w2.gather_ncols, w2.nlayers,
+           roles_of(w2));
   LAUNCH((k_helm_dg_m3<T>), grid_for(static_cast<long long>(w3.ncells) * w3.nlayers),
          static_cast<const T *>(L->D->d), L->D->gs, static_cast<const T *>(L->M3->d), L->M3->gs, w3.d_ebase, t,
          w2.d_map, x, w3.d_map, y, w3.ncells, w3.nlayers, static_cast<T>(L->tau));

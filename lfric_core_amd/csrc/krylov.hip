@@ -21,8 +21,11 @@ struct GmresState {
   double h[(kMaxKrylov + 1) * kMaxKrylov];
   double g[kMaxKrylov + 1];
   double u[kMaxKrylov];
+  double cs[kMaxKrylov], sn[kMaxKrylov];  // the cycle's Givens rotations, formed column by column
   double beta;
   double rnorm2;
+  double beta0;  // beta of the first cycle
+  double ratio;  // |g(iv+1)| / beta0 after column iv: the least-squares residual estimate
 };
 
 namespace {
@@ -41,18 +44,21 @@ struct Store;
 template <>
 struct Store<LFRIC_R32> {
   using T = float;
+  __device__ static float val(T v) { return v; }
   __device__ static float get(const T *p, size_t i) { return p[i]; }
   __device__ static void put(T *p, size_t i, float v) { p[i] = v; }
 };
 template <>
 struct Store<LFRIC_BF16> {
   using T = uint16_t;
+  __device__ static float val(T v) { return bf16_to_f(v); }
   __device__ static float get(const T *p, size_t i) { return bf16_to_f(p[i]); }
   __device__ static void put(T *p, size_t i, float v) { p[i] = f_to_bf16(v); }
 };
 template <>
 struct Store<LFRIC_F16> {
   using T = _Float16;
+  __device__ static float val(T v) { return static_cast<float>(v); }
   __device__ static float get(const T *p, size_t i) { return static_cast<float>(p[i]); }
   __device__ static void put(T *p, size_t i, float v) { p[i] = static_cast<_Float16>(v); }
 };
@@ -163,6 +169,70 @@ __global__ void k_coltri_apply(const typename Store<STORE>::T *__restrict__ fac,
   }
 }
 
+// The same solves with the columns sorted by base dof and tiling the dofs, so that the `tile`
+// columns of one block are one contiguous dof range: the block's kColThreads threads read the
+// range's r and factors into LDS with coalesced loads (kColAhead per thread requested before
+// any is stored), the first `tile` lanes solve one column each there (the operations of
+// k_coltri_apply, in its order), and the block writes z back coalesced.  cap bounds every
+// tile's range.
+constexpr int kColThreads = 256, kColAhead = 4;
+
+template <int STORE, typename V>
+__global__ __launch_bounds__(kColThreads) void k_coltri_apply_tile(const typename Store<STORE>::T *__restrict__ fac,
+                                                                   long long undf, const int *__restrict__ cols,
+                                                                   int ncols, int tile, int cap,
+                                                                   const V *__restrict__ r, V *__restrict__ z) {
+  using S = Store<STORE>;
+  using FT = typename S::T;
+  extern __shared__ __align__(16) unsigned char coltri_lds[];
+  float *sv = reinterpret_cast<float *>(coltri_lds);
+  FT *sl = reinterpret_cast<FT *>(sv + cap), *si = sl + cap, *ss = si + cap;
+  const int c0 = blockIdx.x * tile, c1 = min(c0 + tile, ncols);
+  const int lo = cols[2 * c0], len = cols[2 * (c1 - 1)] + cols[2 * (c1 - 1) + 1] - lo;
+  for (int i0 = 0; i0 < len; i0 += kColThreads * kColAhead) {
+    float rv[kColAhead];
+    FT lv[kColAhead], iv[kColAhead], sv4[kColAhead];
+#pragma unroll
+    for (int a = 0; a < kColAhead; ++a) {
+      const int i = i0 + a * kColThreads + static_cast<int>(threadIdx.x);
+      const size_t d = static_cast<size_t>(lo) + (i < len ? i : 0);
+      rv[a] = static_cast<float>(r[d]);
+      lv[a] = fac[d];
+      iv[a] = fac[undf + d];
+      sv4[a] = fac[2 * undf + d];
+    }
+#pragma unroll
+    for (int a = 0; a < kColAhead; ++a) {
+      const int i = i0 + a * kColThreads + static_cast<int>(threadIdx.x);
+      if (i < len) {
+        sv[i] = rv[a];
+        sl[i] = lv[a];
+        si[i] = iv[a];
+        ss[i] = sv4[a];
+      }
+    }
+  }
+  __syncthreads();
+  const int c = c0 + static_cast<int>(threadIdx.x);
+  if (c < c1) {
+    const int b = cols[2 * c] - lo, h = cols[2 * c + 1];
+    float y = 0.f;
+#pragma unroll 8
+    for (int k = 0; k < h; ++k) {
+      y = sv[b + k] - S::val(sl[b + k]) * y;
+      sv[b + k] = y;
+    }
+    float zn = 0.f;
+#pragma unroll 8
+    for (int k = h - 1; k >= 0; --k) {
+      zn = (sv[b + k] - S::val(ss[b + k]) * zn) * S::val(si[b + k]);
+      sv[b + k] = zn;
+    }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < len; i += kColThreads) z[static_cast<size_t>(lo) + i] = static_cast<V>(sv[i]);
+}
+
 // ---- GMRES -------------------------------------------------------------------------------
 // one fused modified Gram-Schmidt step j of Arnoldi column iv:
 //   j > 0      : w = w - h(j-1,iv) v_{j-1}
@@ -205,17 +275,39 @@ __global__ __launch_bounds__(kBlock) void k_finish_to(const double *__restrict__
 // k_mgs_step for j > 0 with the previous step's finish folded in: every block adds the previous
 // step's partials itself (finish_sum, the same sum as k_finish_to's) to get h(j-1, iv), block 0
 // stores it, then w = w - h(j-1) v_{j-1} and the partials of v_j . w (or w . w) go to partials.
+// The thread's first kMgsAhead elements of w, v_{j-1} and v_j are requested before the finish,
+// so their latency overlaps it; the elements are taken in the grid-stride order all the same.
+constexpr int kMgsAhead = 4;
+
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_mgs_fold(T *__restrict__ w, const T *__restrict__ vprev,
                                                      const T *__restrict__ vj, long long n,
                                                      const double *__restrict__ pprev, int nprev, double *hprev,
                                                      double *partials) {
+  const long long i0 = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const long long stride = static_cast<long long>(gridDim.x) * blockDim.x;
+  T wr[kMgsAhead], pr[kMgsAhead], jr[kMgsAhead];
+#pragma unroll
+  for (int r = 0; r < kMgsAhead; ++r) {
+    const long long i = i0 + r * stride;
+    wr[r] = i < n ? w[i] : T(0);
+    pr[r] = i < n ? vprev[i] : T(0);
+    jr[r] = (vj && i < n) ? vj[i] : T(0);
+  }
   const double h = finish_sum(pprev, nprev);
   if (blockIdx.x == 0 && threadIdx.x == 0) *hprev = h;
   const T a = static_cast<T>(-h);
   double v[1] = {0.0};
-  for (long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
-       i += static_cast<long long>(gridDim.x) * blockDim.x) {
+#pragma unroll
+  for (int r = 0; r < kMgsAhead; ++r) {
+    const long long i = i0 + r * stride;
+    if (i < n) {
+      const T wi = wr[r] + a * pr[r];
+      w[i] = wi;
+      v[0] += vj ? to_d(jr[r]) * to_d(wi) : to_d(wi) * to_d(wi);
+    }
+  }
+  for (long long i = i0 + kMgsAhead * stride; i < n; i += stride) {
     const T wi = w[i] + a * vprev[i];
     w[i] = wi;
     v[0] += vj ? to_d(vj[i]) * to_d(wi) : to_d(wi) * to_d(wi);
@@ -228,12 +320,18 @@ __global__ __launch_bounds__(kBlock) void k_mgs_fold(T *__restrict__ w, const T 
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_scale_fold(T *__restrict__ v, const T *__restrict__ w, long long n,
                                                        const double *__restrict__ partials, int np, double *hout) {
+  const long long i0 = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const long long stride = static_cast<long long>(gridDim.x) * blockDim.x;
+  T wr[kMgsAhead];
+#pragma unroll
+  for (int r = 0; r < kMgsAhead; ++r) wr[r] = i0 + r * stride < n ? w[i0 + r * stride] : T(0);
   const double h = sqrt(finish_sum(partials, np));
   if (blockIdx.x == 0 && threadIdx.x == 0) *hout = h;
   const T s = static_cast<T>(1.0 / h);
-  for (long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
-       i += static_cast<long long>(gridDim.x) * blockDim.x)
-    v[i] = w[i] * s;
+#pragma unroll
+  for (int r = 0; r < kMgsAhead; ++r)
+    if (i0 + r * stride < n) v[i0 + r * stride] = wr[r] * s;
+  for (long long i = i0 + kMgsAhead * stride; i < n; i += stride) v[i] = w[i] * s;
 }
 
 template <typename T>
@@ -279,6 +377,59 @@ __global__ void k_givens(GmresState *st, int m) {
   for (int iv = m - 2; iv >= 0; --iv) {
     double s = g[iv];
     for (int j = iv + 1; j < m; ++j) s = s - H(iv, j) * u[j];
+    u[iv] = s / H(iv, iv);
+  }
+#undef H
+}
+
+// The same rotations formed one Hessenberg column at a time, as each Arnoldi column is made:
+// column iv gets rotations 0 .. iv-1 in order, then its own rotation iv is formed from its
+// entries (iv, iv) and (iv+1, iv), applied to it and to g.  Every entry goes through the
+// operations of the loop above in the same order (rotation i reaches column j > i before
+// rotation i+1 does), so a full cycle ends with the same H and g; a cycle can also stop after
+// any column.  Column 0 starts g = (beta, 0, ...); first marks the first cycle, whose beta is
+// kept as beta0 for ratio = |g(iv+1)| / beta0 (the preconditioned residual estimate).
+__global__ void k_givens_col(GmresState *st, int iv, int m, int first) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double *h = st->h + static_cast<size_t>(iv) * (m + 1), *g = st->g;
+  if (iv == 0) {
+    g[0] = st->beta;
+    for (int i = 1; i <= m; ++i) g[i] = 0.0;
+    if (first) st->beta0 = st->beta;
+  }
+  for (int i = 0; i < iv; ++i) {
+    const double ci = st->cs[i], si = st->sn[i];
+    const double h1 = ci * h[i] + si * h[i + 1];
+    const double h2 = -si * h[i] + ci * h[i + 1];
+    h[i] = h1;
+    h[i + 1] = h2;
+  }
+  const double nrm = sqrt(h[iv] * h[iv] + h[iv + 1] * h[iv + 1]);
+  const double si = h[iv + 1] / nrm, ci = h[iv] / nrm;
+  st->cs[iv] = ci;
+  st->sn[iv] = si;
+  const double p = ci * g[iv] + si * g[iv + 1];
+  const double q = -si * g[iv] + ci * g[iv + 1];
+  g[iv] = p;
+  g[iv + 1] = q;
+  const double h1 = ci * h[iv] + si * h[iv + 1];
+  const double h2 = -si * h[iv] + ci * h[iv + 1];
+  h[iv] = h1;
+  h[iv + 1] = h2;
+  st->ratio = fabs(q) / st->beta0;
+}
+
+// back substitution over the first k columns (k = m after a full cycle)
+__global__ void k_givens_solve(GmresState *st, int k, int m) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const int ld = m + 1;
+  const double *h = st->h, *g = st->g;
+  double *u = st->u;
+#define H(i, j) h[(j) * ld + (i)]
+  u[k - 1] = g[k - 1] / H(k - 1, k - 1);
+  for (int iv = k - 2; iv >= 0; --iv) {
+    double s = g[iv];
+    for (int j = iv + 1; j < k; ++j) s = s - H(iv, j) * u[j];
     u[iv] = s / H(iv, iv);
   }
 #undef H
@@ -336,10 +487,20 @@ int prec_apply_any(const lfric_prec *P, int prec, const void *x, void *y, long l
     return launch_cma_appinv(P->cma, x, y);
   }
   const int grid = grid_for(P->ncols, 64);
+  const int tgrid = P->tile ? (P->ncols + P->tile - 1) / P->tile : 0;
 #define CT_APPLY(ST)                                                                                           \
   if (P->storage == ST) {                                                                                      \
     using FT = typename Store<ST>::T;                                                                          \
-    if (prec == LFRIC_R64)                                                                                     \
+    const size_t lds = static_cast<size_t>(P->cap) * (sizeof(float) + 3 * sizeof(FT));                       \
+    if (P->tile && prec == LFRIC_R64)                                                                          \
+      hipLaunchKernelGGL((k_coltri_apply_tile<ST, double>), dim3(tgrid), dim3(kColThreads), lds,               \
+                         cur_stream(), static_cast<const FT *>(P->d_fac), P->undf, P->d_cols, P->ncols,        \
+                         P->tile, P->cap, static_cast<const double *>(x), static_cast<double *>(y));           \
+    else if (P->tile)                                                                                          \
+      hipLaunchKernelGGL((k_coltri_apply_tile<ST, float>), dim3(tgrid), dim3(kColThreads), lds,                \
+                         cur_stream(), static_cast<const FT *>(P->d_fac), P->undf, P->d_cols, P->ncols,        \
+                         P->tile, P->cap, static_cast<const float *>(x), static_cast<float *>(y));             \
+    else if (prec == LFRIC_R64)                                                                                \
       LAUNCHK((k_coltri_apply<ST, double>), grid, 64, static_cast<const FT *>(P->d_fac), P->undf, P->d_cols,   \
               P->ncols, static_cast<const double *>(x), static_cast<double *>(y));                             \
     else                                                                                                       \
@@ -361,6 +522,7 @@ static size_t g_gm_bytes[6] = {0};
 static GmresState *g_gstate = nullptr;
 static double *g_gpart = nullptr;
 static double *g_host_scalar = nullptr;
+static hipEvent_t g_gev[2] = {nullptr, nullptr};  // the inner solve's per-column residual reads
 constexpr int kGmBlocks = 1024;
 
 static int gm_ws(int slot, size_t bytes, void **out) {
@@ -380,6 +542,7 @@ static int gm_init() {
   if (int rc = dmalloc(reinterpret_cast<void **>(&g_gstate), sizeof(GmresState))) return rc;
   if (int rc = dmalloc(reinterpret_cast<void **>(&g_gpart), sizeof(double) * 2 * kGmBlocks)) return rc;
   LF_HIP(hipHostMalloc(reinterpret_cast<void **>(&g_host_scalar), sizeof(double) * 4, hipHostMallocDefault));
+  for (auto &e : g_gev) LF_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
   return LFRIC_OK;
 }
 
@@ -427,15 +590,22 @@ static int read_scalar(const double *d, double *h) {
 }
 
 struct GmresOut {
-  int cycles = 0, status = LFRIC_NOT_CONVERGED;
+  int cycles = 0, iterations = 0, status = LFRIC_NOT_CONVERGED;
   double r0 = 0, rn = 0;
 };
 
 // gmres_solve (sci_iterative_solver_mod.f90:830-1003) with every vector on the device.  One
 // host synchronisation per restart cycle (the convergence test on the true residual).
+//
+// early (the inner solve of the refinement only; the reference's GMRES runs whole cycles):
+// after each column the host also reads the least-squares residual estimate |g(iv+1)| / beta0,
+// one column behind the device (the read of column iv-1 waits while column iv runs), and ends
+// the cycle with the columns made so far once it falls below r_tol, and the solve with it (the
+// refinement's FP64 residual is the test that follows); a cycle that runs to m columns ends
+// with the true-residual test as usual.
 template <typename T>
 static int gmres_t(const lfric_linop *A, const lfric_prec *P, T *x, const T *b, long long n, int m, double r_tol,
-                   double a_tol, int max_cycles, int monitor, GmresOut &out) {
+                   double a_tol, int max_cycles, int monitor, GmresOut &out, bool early = false) {
   if (int rc = gm_init()) return rc;
   LF_CHECK(m >= 1 && m <= kMaxKrylov, LFRIC_E_ARG, "gcrk must be in [1, 64]");
   const int prec = sizeof(T) == 8 ? LFRIC_R64 : LFRIC_R32;
@@ -471,15 +641,35 @@ static int gmres_t(const lfric_linop *A, const lfric_prec *P, T *x, const T *b, 
     if (int rc = prec_apply_any(P, prec, r, s, n)) return rc;
     if (int rc = dot_into<T>(s, s, n, &st->beta, true)) return rc;
     LAUNCHK(k_scale_by_inv<T>, vg, kBlock, V, s, n, &st->beta);
+    int k = m;  // columns made in this cycle
     for (int iv = 0; iv < m; ++iv) {
       T *viv = V + static_cast<size_t>(iv) * n;
       if (int rc = linop_apply(A, viv, s)) return rc;  // s = A w (w = v_iv)
       if (int rc = prec_apply_any(P, prec, s, w, n)) return rc;  // w = P^-1 s
       if (int rc = arnoldi_mgs<T>(w, V, n, iv, m, st->h)) return rc;
+      LAUNCHK(k_givens_col, 1, 64, st, iv, m, cyc == 1 ? 1 : 0);
+      if (early && iv < m - 1) {
+        double *slot = g_host_scalar + 2 + (iv & 1);
+        LF_HIP(hipMemcpyAsync(slot, &st->ratio, sizeof(double), hipMemcpyDeviceToHost, cur_stream()));
+        LF_HIP(hipEventRecord(g_gev[iv & 1], cur_stream()));
+        if (iv >= 1) {
+          LF_HIP(hipEventSynchronize(g_gev[(iv - 1) & 1]));
+          if (g_host_scalar[2 + ((iv - 1) & 1)] < r_tol) {  // column iv is made too: use it
+            k = iv + 1;
+            break;
+          }
+        }
+      }
     }
-    LAUNCHK(k_givens, 1, 64, st, m);
-    LAUNCHK(k_gmres_update<T>, vg, kBlock, x, V, n, m, st->u);
+    out.iterations += k;
+    LAUNCHK(k_givens_solve, 1, 64, st, k, m);
+    LAUNCHK(k_gmres_update<T>, vg, kBlock, x, V, n, k, st->u);
     LF_HIP(hipGetLastError());
+    if (k < m) {  // ended on the estimate: the refinement's own residual is the test
+      out.cycles = cyc;
+      out.status = LFRIC_OK;
+      return LFRIC_OK;
+    }
     if (int rc = linop_apply(A, x, Ax)) return rc;
     if (int rc = launch_copy(prec, r, Ax, n)) return rc;
     if (int rc = launch_cg_residual(prec, r, b, n)) return rc;  // res%aypx(-1, b)
@@ -581,6 +771,7 @@ lfric_prec::~lfric_prec() {
 }
 
 lfric_linop::~lfric_linop() {
+  dfree(d_hcol, hcol_bytes);
   dfree(t_w2, t_w2_bytes);
   dfree(t_w3, t_w3_bytes);
 }
@@ -671,6 +862,15 @@ int lfric_prec_create_column_jacobi(const lfric_linop *L, int storage, lfric_pre
         cols.push_back(f.nlayers + (f.partner[d] >= 0 ? 1 : 0));
       }
     }
+    // by base dof: each column's solve is independent of the others, so the order only decides
+    // which columns share a block of the tiled apply
+    std::vector<std::pair<int, int>> sorted(cols.size() / 2);
+    for (size_t i = 0; i < sorted.size(); ++i) sorted[i] = {cols[2 * i], cols[2 * i + 1]};
+    std::sort(sorted.begin(), sorted.end());
+    for (size_t i = 0; i < sorted.size(); ++i) {
+      cols[2 * i] = sorted[i].first;
+      cols[2 * i + 1] = sorted[i].second;
+    }
   }
   int rc = LFRIC_OK;
   if (L->kind == 0) {
@@ -714,6 +914,28 @@ int lfric_prec_create_column_jacobi(const lfric_linop *L, int storage, lfric_pre
   P->storage = storage;
   P->undf = n;
   P->ncols = static_cast<int>(cols.size() / 2);
+  // tiles of kColTile consecutive columns when the sorted columns tile [0, undf) and a tile's
+  // range fits the LDS budget; otherwise one thread per column straight from memory
+  {
+    constexpr int kColTile = 32;
+    constexpr size_t kColLds = 48 * 1024;
+    bool tiles = true;
+    long long next = 0;
+    for (size_t i = 0; i < cols.size(); i += 2) {
+      tiles = tiles && cols[i] == next;
+      next = static_cast<long long>(cols[i]) + cols[i + 1];
+    }
+    tiles = tiles && next == n && P->ncols > 0;
+    int cap = 0;
+    for (int c0 = 0; tiles && c0 < P->ncols; c0 += kColTile) {
+      const int c1 = std::min(c0 + kColTile, P->ncols);
+      cap = std::max(cap, cols[2 * (c1 - 1)] + cols[2 * (c1 - 1) + 1] - cols[2 * c0]);
+    }
+    if (tiles && static_cast<size_t>(cap) * (4 + 3 * esz(storage)) <= kColLds) {
+      P->tile = kColTile;
+      P->cap = cap;
+    }
+  }
   P->cols_bytes = sizeof(int) * cols.size();
   P->fac_bytes = 3 * esz(storage) * static_cast<size_t>(n);
   if (rc == LFRIC_OK) rc = dmalloc(reinterpret_cast<void **>(&P->d_cols), P->cols_bytes);
@@ -745,12 +967,12 @@ int lfric_prec_create_column_jacobi(const lfric_linop *L, int storage, lfric_pre
 }
 
 static int gmres_common(const lfric_linop *A, const lfric_prec *P, void *x, const void *b, long long n, int prec,
-                        const lfric_solver_opts *o, GmresOut &g) {
+                        const lfric_solver_opts *o, GmresOut &g, bool early = false) {
   if (prec == LFRIC_R64)
     return gmres_t<double>(A, P, static_cast<double *>(x), static_cast<const double *>(b), n, o->gcrk, o->r_tol,
-                           o->a_tol, o->max_iter, o->monitor_convergence, g);
+                           o->a_tol, o->max_iter, o->monitor_convergence, g, early);
   return gmres_t<float>(A, P, static_cast<float *>(x), static_cast<const float *>(b), n, o->gcrk, o->r_tol, o->a_tol,
-                        o->max_iter, o->monitor_convergence, g);
+                        o->max_iter, o->monitor_convergence, g, early);
 }
 
 int lfric_gmres_solve(const lfric_linop *A, const lfric_prec *P, lfric_vec *x, const lfric_vec *b,
@@ -825,7 +1047,7 @@ int lfric_ir_gmres_solve(const lfric_linop *A64, const lfric_linop *A32, const l
   double nr0, nr;
   if (int rc = residual(&nr0)) return rc;
   nr = nr0;
-  int status = LFRIC_NOT_CONVERGED, cycles = 0, outer = 0;
+  int status = LFRIC_NOT_CONVERGED, cycles = 0, outer = 0, iterations = 0;
   if (!(nr0 == nr0)) status = LFRIC_NAN;
   else if (nr0 < 2.220446049250313e-16) status = LFRIC_OK;
   while (status == LFRIC_NOT_CONVERGED && outer < opts->max_outer && cycles < opts->max_iter) {
@@ -841,8 +1063,9 @@ int lfric_ir_gmres_solve(const lfric_linop *A64, const lfric_linop *A32, const l
     in.monitor_convergence = 1;
     in.max_iter = opts->max_iter - cycles;
     GmresOut g;
-    if (int rc = gmres_common(A32, P32, d32, b32, n, LFRIC_R32, &in, g)) return rc;
+    if (int rc = gmres_common(A32, P32, d32, b32, n, LFRIC_R32, &in, g, true)) return rc;
     cycles += g.cycles;
+    iterations += g.iterations;
     ++outer;
     if (g.status == LFRIC_NAN) {
       status = LFRIC_NAN;
@@ -855,7 +1078,7 @@ int lfric_ir_gmres_solve(const lfric_linop *A64, const lfric_linop *A32, const l
   if (status == LFRIC_NOT_CONVERGED && (nr / nr0 < opts->r_tol || nr <= opts->a_tol)) status = LFRIC_OK;
   if (res) {
     res->status = status;
-    res->iterations = cycles * opts->gcrk;
+    res->iterations = iterations;
     res->outer_iterations = outer;
     res->r_nrm_0 = nr0;
     res->r_nrm = nr;

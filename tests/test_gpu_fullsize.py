@@ -105,6 +105,7 @@ def test_fullsize_helmholtz_gmres_bf16_block_jacobi(hip):
     x = hip.FieldVector(w3)
     res = hip.MixedPrecisionGMRES(H64, H32, P, 30, 1e-8, 0.0, 50, inner_r_tol=1e-4).apply(x, b)
     assert res.status == 0 and res.true_rel_res <= 1e-8, res
+    assert 0 < res.iterations < 30 * res.outer_iterations, res
     x64 = hip.FieldVector(w3)
     res64 = hip.GMRES(H64, hip.NullPreconditioner(), 30, 1e-10, 0.0, 200).apply(x64, b)
     assert res64.status == 0
