@@ -755,6 +755,7 @@ void lfric_op_destroy(lfric_op *op) {
 // an operator whose entries change keeps its row-gathered and symmetric-packed copies (those it
 // has) in step; new entries that are not symmetric drop the symmetric copy
 static int refresh_rows(lfric_op *op, int rc) {
+  ++op->version;
   if (rc != LFRIC_OK) return rc;
   if (op->rows)
     if (int r = launch_build_rows(op)) return r;

@@ -157,6 +157,7 @@ struct lfric_op {
   mutable size_t symz_mr_bytes = 0;
   void *d = nullptr;           // level-aligned AoSoA layout (see kernels.hip)
   size_t bytes = 0;
+  unsigned long long version = 0;  // counts the rewrites of the entries (copies held elsewhere follow it)
 };
 
 // Columnwise (CMA) operator between horizontally discontinuous spaces: one banded matrix per
@@ -210,5 +211,9 @@ struct lfric_linop {
   mutable int hcol_state = 0;  // kind 1: 1 the W2 column table is built, -1 the mesh does not allow one
   mutable void *d_hcol = nullptr;
   mutable size_t hcol_bytes = 0;
+  // kind 1: D's entries in transpose-gather order (two per W2 dof) and the D version they hold
+  mutable void *d_dtc = nullptr;
+  mutable size_t dtc_bytes = 0;
+  mutable unsigned long long dtc_version = ~0ull;
   ~lfric_linop();
 };

@@ -1384,35 +1384,67 @@ __global__ __launch_bounds__(kBlock) void k_helm_transpose_div(const T *__restri
   t[d] = total / m2[d];
 }
 
-// The same transpose with each W2 column's (at most two) cells read from one int4 of a table
-// built for the composite: {ebase, map3 * 8 + face} per cell in the column's colour order, the
-// second ebase -1 when the column has one cell.  The W2 lowest-order roles are fixed (face 4
-// is the bottom, its partner the top face 5), so one level of dependent index loads is left
-// where the gather lists took three; the sums are k_helm_transpose_div's, term for term.
+// The composite's copy of D in transpose-gather order.  A table built for the composite holds
+// each W2 column's (at most two) cells as one int4, {ebase, W3 dof * 8 + face} per cell in
+// the column's colour order (the second ebase -1 when the column has one cell); the W2 lowest-
+// order roles are fixed (face 4 the bottom, its partner the top face 5).  Per W2 dof (column,
+// level m) the copy holds the two entries of D that its row of D^T takes (a side face: the
+// column's first and second cell at layer m; a bottom/top face column: the bottom face at
+// layer m and the top face at layer m-1), zero where there is none, so the transpose reads one
+// coalesced pair per dof instead of one value out of each cell's 16-byte chunk of faces, and
+// one level of dependent index loads where the gather lists take three.
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_helm_transpose_div_tab(const T *__restrict__ D, size_t gs,
-                                                                  const T *__restrict__ x3, const T *__restrict__ m2,
-                                                                  T *__restrict__ t, const int2 *__restrict__ cols,
-                                                                  const int4 *__restrict__ tab, int ncols, int nl) {
+__global__ __launch_bounds__(kBlock) void k_build_dtc(const T *__restrict__ D, size_t gs, const int2 *__restrict__ cols,
+                                                     const int4 *__restrict__ tab, int ncols, int nl,
+                                                     T *__restrict__ out) {
   const int H = nl + 1;
-  const int p = blockIdx.x * kBlock + threadIdx.x;  // the table exists only when ncols * H < 2^31
+  const int p = blockIdx.x * kBlock + threadIdx.x;
   if (p >= ncols * H) return;
   const int col = p / H, m = p - col * H;
   const int2 ci = cols[col];
   if (m == nl && !ci.y) return;
   const int4 c = tab[col];
-  const bool has_m = m < nl, has_p = m > 0;
-  auto cell = [&](int eb, int bj) {
-    const int j = bj & 7, b3 = bj >> 3;
-    const long long e = static_cast<long long>(eb) + m;
-    T acc = T(0);
-    if (has_m) acc = fma(D[aidx<T>(gs, e, j)], x3[b3 + m], acc);
-    if (j == 4 && has_p) acc = fma(D[aidx<T>(gs, e - 1, 5)], x3[b3 + m - 1], acc);
-    return acc;
-  };
-  T total = cell(c.x, c.y);
-  if (c.z >= 0) total = total + cell(c.z, c.w);
+  T a = T(0), b = T(0);
+  if ((c.y & 7) == 4) {
+    if (m < nl) a = D[aidx<T>(gs, static_cast<long long>(c.x) + m, 4)];
+    if (m > 0) b = D[aidx<T>(gs, static_cast<long long>(c.x) + m - 1, 5)];
+  } else {
+    a = D[aidx<T>(gs, static_cast<long long>(c.x) + m, c.y & 7)];
+    if (c.z >= 0) b = D[aidx<T>(gs, static_cast<long long>(c.z) + m, c.w & 7)];
+  }
   const size_t d = static_cast<size_t>(ci.x) + m;
+  out[2 * d] = a;
+  out[2 * d + 1] = b;
+}
+
+// k_helm_transpose_div over that copy and the table: the same terms in the same order, bit for
+// bit
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_helm_transpose_div_dtc(const T *__restrict__ dtc,
+                                                                  const T *__restrict__ x3, const T *__restrict__ m2,
+                                                                  T *__restrict__ t, const int2 *__restrict__ cols,
+                                                                  const int4 *__restrict__ tab, int ncols, int nl) {
+  using T2 = typename std::conditional<sizeof(T) == 8, double2, float2>::type;
+  const int H = nl + 1;
+  const int p = blockIdx.x * kBlock + threadIdx.x;
+  if (p >= ncols * H) return;
+  const int col = p / H, m = p - col * H;
+  const int2 ci = cols[col];
+  if (m == nl && !ci.y) return;
+  const int4 c = tab[col];
+  const size_t d = static_cast<size_t>(ci.x) + m;
+  const T2 v = reinterpret_cast<const T2 *>(dtc)[d];
+  const int b3 = c.y >> 3;
+  T total;
+  if ((c.y & 7) == 4) {
+    T acc = T(0);
+    if (m < nl) acc = fma(v.x, x3[b3 + m], acc);
+    if (m > 0) acc = fma(v.y, x3[b3 + m - 1], acc);
+    total = acc;
+  } else {
+    total = fma(v.x, x3[b3 + m], T(0));
+    if (c.z >= 0) total = total + fma(v.y, x3[(c.w >> 3) + m], T(0));
+  }
   t[d] = total / m2[d];
 }
 
@@ -3306,8 +3338,9 @@ bool helmholtz_fusable(const lfric_linop *L) {
          w2.ebase == w3.ebase && (L->precision == LFRIC_R64 || L->precision == LFRIC_R32);
 }
 
-// the composite's column table for k_helm_transpose_div_tab (the gather lists of the W2 space
-// resolved to ebase and W3 dof per cell); none when a column has more than two cells
+// the composite's column table for k_build_dtc and k_helm_transpose_div_dtc (the gather lists
+// of the W2 space resolved to ebase and W3 dof per cell); none when a column has more than two
+// cells
 static int build_helm_table(const lfric_linop *L) {
   if (L->hcol_state) return LFRIC_OK;
   const FunctionSpace &w2 = *L->D->from, &w3 = *L->D->to;
@@ -3352,8 +3385,18 @@ static int helmholtz_t(const lfric_linop *L, const T *x, T *y) {
   if (int rc = build_helm_table(L)) return rc;
   T *t = static_cast<T *>(L->t_w2);
   const int tgrid = grid_for(static_cast<long long>(w2.gather_ncols) * (w2.nlayers + 1));
+  if (L->hcol_state == 1 && L->dtc_version != L->D->version) {  // (re)build the copy of D
+    if (!L->d_dtc) {
+      L->dtc_bytes = 2 * sizeof(T) * static_cast<size_t>(w2.undf);
+      if (int rc = dmalloc(&L->d_dtc, L->dtc_bytes)) return rc;
+    }
+    LAUNCH((k_build_dtc<T>), tgrid, static_cast<const T *>(L->D->d), L->D->gs, w2.d_gcol,
+           static_cast<const int4 *>(L->d_hcol), w2.gather_ncols, w2.nlayers, static_cast<T *>(L->d_dtc));
+    CHECK_LAUNCH();
+    L->dtc_version = L->D->version;
+  }
   if (L->hcol_state == 1)
-    LAUNCH((k_helm_transpose_div_tab<T>), tgrid, static_cast<const T *>(L->D->d), L->D->gs, x,
+    LAUNCH((k_helm_transpose_div_dtc<T>), tgrid, static_cast<const T *>(L->d_dtc), x,
            static_cast<const T *>(L->m2->d), t, w2.d_gcol, static_cast<const int4 *>(L->d_hcol), w2.gather_ncols,
            w2.nlayers);
   else

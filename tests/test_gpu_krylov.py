@@ -75,21 +75,26 @@ def test_helmholtz_apply_parity(hip, orc, n, nl):
 @pytest.mark.parametrize("n,nl", [(4, 3), (5, 7), (12, 10)])
 def test_helmholtz_two_kernel_apply_equals_the_kernel_chain(hip, orc, n, nl, precision):
     """The Helmholtz composite's two-kernel apply (transpose in gather form with the division,
-    then divergence, M3 and the axpy in one kernel) equals the chain of separate applies -
-    colour-ordered transpose, division by diag(M2), divergence, M3, y += tau t - bit for bit."""
+    over the composite's copy of D in transpose-gather order, then divergence, M3 and the axpy
+    in one kernel) equals the chain of separate applies - colour-ordered transpose, division by
+    diag(M2), divergence, M3, y += tau t - bit for bit; after D's entries are rewritten the
+    copy follows them."""
     tau = 0.5
     w2, w3, H, (M2, D, M3, m2d) = _helmholtz(hip, orc, n, nl, precision, tau)
     x = hip.FieldVector(w3, precision).fill_uniform(3, 3)
-    y = hip.FieldVector(w3, precision)
-    H.apply(x, y)
-    t2, t2d = hip.FieldVector(w2, precision), hip.FieldVector(w2, precision)
-    t3, yc = hip.FieldVector(w3, precision), hip.FieldVector(w3, precision)
-    D.apply_transpose(x, t2)
-    t2d.divide(t2, m2d)
-    D.apply_dg(t2d, t3)
-    M3.apply_dg(x, yc)
-    yc.axpy(tau, t3)
-    np.testing.assert_array_equal(y.export_raw(), yc.export_raw())
+    for rewrite in (False, True):
+        if rewrite:
+            D.upload(np.roll(D.download(), 1, axis=0))
+        y = hip.FieldVector(w3, precision)
+        H.apply(x, y)
+        t2, t2d = hip.FieldVector(w2, precision), hip.FieldVector(w2, precision)
+        t3, yc = hip.FieldVector(w3, precision), hip.FieldVector(w3, precision)
+        D.apply_transpose(x, t2)
+        t2d.divide(t2, m2d)
+        D.apply_dg(t2d, t3)
+        M3.apply_dg(x, yc)
+        yc.axpy(tau, t3)
+        np.testing.assert_array_equal(y.export_raw(), yc.export_raw())
 
 
 def _column_solves(Hd, columns, r):
