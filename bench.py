@@ -352,10 +352,11 @@ def main():
     ap.add_argument("--apply-only", action="store_true",
                     help="time only the LMA applies and a known-size vector copy (for hardware-counter runs); "
                          "with --workload batched, the batch apply only")
-    ap.add_argument("--workload", choices=("single", "batched", "served"), default="single",
+    ap.add_argument("--workload", choices=("single", "batched", "served", "helmholtz"), default="single",
                     help="single: C96x70 mixed vs FP64 solve (default); batched: 64 uneven multi-RHS problems "
                          "in one batch vs sequential FP64 solves; served: Poisson request stream with "
-                         "continuous batching vs sequential FP64 solves")
+                         "continuous batching vs sequential FP64 solves; helmholtz: only the C48x50 "
+                         "Helmholtz-like GMRES record (for kernel-trace and counter runs)")
     ap.add_argument("--no-helmholtz", action="store_true",
                     help="single workload: skip the Helmholtz-like GMRES configuration's record")
     ap.add_argument("--problems", type=int, default=64)
@@ -381,6 +382,11 @@ def main():
         return bench_batched(args)
     if args.workload == "served":
         return bench_served(args)
+    if args.workload == "helmholtz":
+        import lfric_core_amd as L
+        L.init(0)
+        print(json.dumps({"helmholtz": helmholtz_record(L)}))
+        return
 
     import lfric_core_amd as L
     from lfric_core_amd.solvers import residual_norm
