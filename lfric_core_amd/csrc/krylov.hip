@@ -538,11 +538,15 @@ static int gm_ws(int slot, size_t bytes, void **out) {
 }
 
 static int gm_init() {
-  if (g_gstate) return LFRIC_OK;
-  if (int rc = dmalloc(reinterpret_cast<void **>(&g_gstate), sizeof(GmresState))) return rc;
-  if (int rc = dmalloc(reinterpret_cast<void **>(&g_gpart), sizeof(double) * 2 * kGmBlocks)) return rc;
-  LF_HIP(hipHostMalloc(reinterpret_cast<void **>(&g_host_scalar), sizeof(double) * 4, hipHostMallocDefault));
-  for (auto &e : g_gev) LF_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  // each piece on its own, so that a call after a failed one finishes the set
+  if (!g_gstate)
+    if (int rc = dmalloc(reinterpret_cast<void **>(&g_gstate), sizeof(GmresState))) return rc;
+  if (!g_gpart)
+    if (int rc = dmalloc(reinterpret_cast<void **>(&g_gpart), sizeof(double) * 2 * kGmBlocks)) return rc;
+  if (!g_host_scalar)
+    LF_HIP(hipHostMalloc(reinterpret_cast<void **>(&g_host_scalar), sizeof(double) * 4, hipHostMallocDefault));
+  for (auto &e : g_gev)
+    if (!e) LF_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
   return LFRIC_OK;
 }
 

@@ -328,6 +328,12 @@ contains
   !> self is intent(out), as copy_interface declares it (vector_mod.F90:173-177), so a vector
   !> passed here is finalised on entry (vec_final releases its device buffer to the library's
   !> cache) and this call takes a buffer of the source's shape.
+  !> self is intent(out) because copy_interface declares it so (vector_mod.F90:173-177), and an
+  !> override must keep the dummy's characteristics.  The processor finalizes the actual argument
+  !> on entry (vec_final frees the handle it owns: F2008 4.5.6.3), and lfric_vec_create takes
+  !> the freed buffer back from the library's vector cache, so the copies that the solvers make
+  !> every iteration neither leak nor call hipMalloc/hipFree (the drop-in test checks the
+  !> library's device bytes before and after a BiCGStab solve).
   subroutine vec_copy(self, source)
     class(hip_field_vector_type), intent(out) :: self
     class(abstract_vector_type), intent(in) :: source
@@ -486,6 +492,7 @@ contains
   end subroutine rvec_duplicate
 
   !> intent(out) as copy_interface declares it; see vec_copy
+  !> intent(out) as copy_interface requires; see vec_copy for the handle's life
   subroutine rvec_copy(self, source)
     class(hip_r_solver_field_vector_type), intent(out) :: self
     class(abstract_vector_type), intent(in) :: source
