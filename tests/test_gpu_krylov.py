@@ -97,6 +97,77 @@ def test_helmholtz_two_kernel_apply_equals_the_kernel_chain(hip, orc, n, nl, pre
         np.testing.assert_array_equal(y.export_raw(), yc.export_raw())
 
 
+def _ir_gmres_columns(apply, b, m, r_tol, inner_tol, max_outer=50):
+    """FP64 numpy restatement of the refinement around GMRES(m) whose inner solve ends its
+    cycle on the least-squares residual estimate: the estimate of column iv-1 is looked at once
+    column iv is made (and only while iv < m-1), and the cycle then keeps iv+1 columns; a cycle
+    of m columns ends with the inner true-residual test.  Returns (columns, outer steps)."""
+    x = np.zeros_like(b)
+    r = b.copy()
+    nr0 = nr = np.linalg.norm(r)
+    total = outer = 0
+    while nr / nr0 >= r_tol and outer < max_outer:
+        rhs = r / nr
+        tol = max(inner_tol, 0.5 * r_tol * nr0 / nr)
+        d = np.zeros_like(b)
+        beta0 = None
+        while True:
+            res = rhs - apply(d)
+            beta = np.linalg.norm(res)
+            beta0 = beta if beta0 is None else beta0
+            V, H, g = [res / beta], np.zeros((m + 1, m)), np.zeros(m + 1)
+            g[0] = beta
+            cs, sn, ratios, k = np.zeros(m), np.zeros(m), [], m
+            for iv in range(m):
+                w = apply(V[iv])
+                for j in range(iv + 1):
+                    H[j, iv] = V[j] @ w
+                    w = w - H[j, iv] * V[j]
+                H[iv + 1, iv] = np.linalg.norm(w)
+                V.append(w / H[iv + 1, iv])
+                for i in range(iv):
+                    a, c = H[i, iv], H[i + 1, iv]
+                    H[i, iv], H[i + 1, iv] = cs[i] * a + sn[i] * c, -sn[i] * a + cs[i] * c
+                nrm = np.hypot(H[iv, iv], H[iv + 1, iv])
+                cs[iv], sn[iv] = H[iv, iv] / nrm, H[iv + 1, iv] / nrm
+                g[iv], g[iv + 1] = cs[iv] * g[iv] + sn[iv] * g[iv + 1], -sn[iv] * g[iv] + cs[iv] * g[iv + 1]
+                H[iv, iv] = cs[iv] * H[iv, iv] + sn[iv] * H[iv + 1, iv]
+                ratios.append(abs(g[iv + 1]) / beta0)
+                if 1 <= iv < m - 1 and ratios[iv - 1] < tol:
+                    k = iv + 1
+                    break
+            d = d + np.column_stack(V[:k]) @ np.linalg.solve(np.triu(H[:k, :k]), g[:k])
+            total += k
+            if k < m or np.linalg.norm(rhs - apply(d)) < tol * np.linalg.norm(rhs):
+                break
+        x = x + nr * d
+        r = b - apply(x)
+        nr = np.linalg.norm(r)
+        outer += 1
+    return total, outer
+
+
+@pytest.mark.parametrize("gcrk", [30, 6])
+def test_mixed_gmres_inner_exit_matches_restatement(hip, orc, gcrk):
+    """The inner solves' exits on the residual estimate (no preconditioner, so the estimate is
+    the residual's): as many refinement steps as the FP64 restatement, and the same number of
+    Arnoldi columns within one per step (FP32 inner arithmetic against FP64).  gcrk 6 needs
+    whole restarted cycles before the exit."""
+    w2, w3, H64, (M2, D, M3, m2d) = _helmholtz(hip, orc, 12, 10)
+    M2s, Ds, M3s = M2.convert(hip.R32), D.convert(hip.R32), M3.convert(hip.R32)
+    H32 = hip.LinearOperator.helmholtz(M3s, Ds, M2s.mm_diagonal(hip.FieldVector(w2, hip.R32)), 0.5)
+    o = orc.Helmholtz(12, 10)
+    b = np.random.default_rng(5).uniform(-1, 1, w3.undf)
+    x = hip.FieldVector(w3)
+    res = hip.MixedPrecisionGMRES(H64, H32, hip.NullPreconditioner(), gcrk, 1e-8, 0.0, 50,
+                                  inner_r_tol=1e-4).apply(x, hip.FieldVector(w3).import_field(b))
+    cols, outer = _ir_gmres_columns(o.apply, b, gcrk, 1e-8, 1e-4)
+    print(f"gcrk {gcrk}: device {res.iterations} columns / {res.outer_iterations} steps, "
+          f"restatement {cols} / {outer}", flush=True)
+    assert res.status == 0 and res.outer_iterations == outer
+    assert abs(res.iterations - cols) <= outer
+
+
 def _column_solves(Hd, columns, r):
     z = np.zeros_like(r)
     for base, h in columns:
