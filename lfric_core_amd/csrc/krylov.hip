@@ -919,25 +919,28 @@ int lfric_prec_create_column_jacobi(const lfric_linop *L, int storage, lfric_pre
   P->storage = storage;
   P->undf = n;
   P->ncols = static_cast<int>(cols.size() / 2);
-  // tiles of kColTile consecutive columns when the sorted columns tile [0, undf) and a tile's
-  // range fits the LDS budget; otherwise one thread per column straight from memory
+  // tiles of up to 32 consecutive columns when the sorted columns tile [0, undf), as many as
+  // fit 48 KB of LDS at the tallest column; otherwise one thread per column straight from memory
   {
-    constexpr int kColTile = 32;
     constexpr size_t kColLds = 48 * 1024;
-    bool tiles = true;
+    const size_t per_dof = 4 + 3 * esz(storage);
+    bool tiles = P->ncols > 0;
     long long next = 0;
+    int hmax = 0;
     for (size_t i = 0; i < cols.size(); i += 2) {
       tiles = tiles && cols[i] == next;
       next = static_cast<long long>(cols[i]) + cols[i + 1];
+      hmax = std::max(hmax, cols[i + 1]);
     }
-    tiles = tiles && next == n && P->ncols > 0;
+    tiles = tiles && next == n && hmax > 0;
+    const int tile = tiles ? static_cast<int>(std::min<size_t>(32, kColLds / (per_dof * hmax))) : 0;
     int cap = 0;
-    for (int c0 = 0; tiles && c0 < P->ncols; c0 += kColTile) {
-      const int c1 = std::min(c0 + kColTile, P->ncols);
+    for (int c0 = 0; tile > 0 && c0 < P->ncols; c0 += tile) {
+      const int c1 = std::min(c0 + tile, P->ncols);
       cap = std::max(cap, cols[2 * (c1 - 1)] + cols[2 * (c1 - 1) + 1] - cols[2 * c0]);
     }
-    if (tiles && static_cast<size_t>(cap) * (4 + 3 * esz(storage)) <= kColLds) {
-      P->tile = kColTile;
+    if (tile > 0 && static_cast<size_t>(cap) * per_dof <= kColLds) {
+      P->tile = tile;
       P->cap = cap;
     }
   }

@@ -172,7 +172,8 @@ def _column_solves(Hd, columns, r):
     z = np.zeros_like(r)
     for base, h in columns:
         idx = np.arange(base, base + h)
-        T = Hd[np.ix_(idx, idx)]
+        T = Hd[idx][:, idx]
+        T = T.toarray() if hasattr(T, "toarray") else T
         ab = np.zeros((3, h))
         ab[0, 1:] = np.diag(T, 1)
         ab[1] = np.diag(T)
@@ -195,18 +196,21 @@ def test_column_jacobi_helmholtz(hip, orc, storage, tol):
     assert np.linalg.norm(z.export_field() - ref) <= tol * np.linalg.norm(ref)
 
 
-def test_column_jacobi_w0_lma(hip, orc):
-    fs = hip.FunctionSpace.cubedsphere(4, 6)
-    m = orc.Mesh(4, 6)
+@pytest.mark.parametrize("nl", [6, 120])
+def test_column_jacobi_w0_lma(hip, orc, nl):
+    """W0 vertex columns of nl+1 dofs; at 120 levels fewer than 32 columns fit one block's LDS
+    tile (FP32 factors: 25 per block)."""
+    fs = hip.FunctionSpace.cubedsphere(4, nl)
+    m = orc.Mesh(4, nl)
     op = hip.LmaOperator(fs).generate_spd(3)
     A = orc.gen_spd(8, m.ncell_3d, 3)
-    S = orc.assembled_csr(m, A).toarray()
+    S = orc.assembled_csr(m, A).tocsr()
     bases = sorted(set(m.dofmap[:, :4].ravel() - 1))
     P = hip.ColumnJacobiPreconditioner(hip.LinearOperator.from_lma(op), hip.R32)
     r = np.random.default_rng(3).standard_normal(m.undf)
     z = hip.FieldVector(fs)
     P.apply(hip.FieldVector(fs).import_field(r), z)
-    ref = _column_solves(S, [(b, 7) for b in bases], r)
+    ref = _column_solves(S, [(b, nl + 1) for b in bases], r)
     assert np.linalg.norm(z.export_field() - ref) <= 2e-5 * np.linalg.norm(ref)
 
 
