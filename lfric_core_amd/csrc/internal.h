@@ -215,5 +215,10 @@ struct lfric_linop {
   mutable void *d_dtc = nullptr;
   mutable size_t dtc_bytes = 0;
   mutable unsigned long long dtc_version = ~0ull;
+  // kind 1: D's rows and M3 compacted in the divergence kernel's thread order, and the D and
+  // M3 versions they hold
+  mutable void *d_drc = nullptr;
+  mutable size_t drc_bytes = 0;
+  mutable unsigned long long drc_dver = ~0ull, drc_mver = ~0ull;
   ~lfric_linop();
 };

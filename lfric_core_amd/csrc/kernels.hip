@@ -1469,6 +1469,43 @@ __global__ __launch_bounds__(kBlock) void k_helm_dg_m3(const T *__restrict__ D, 
   y[d] = y3 + tau * acc;
 }
 
+// The composite's compact copy of D's rows and M3 in k_helm_dg_m3's thread order u = pos * nl
+// + m: row j < 6 of D at out[j * nu + u], M3 at out[6 * nu + u].  The element layout pads a
+// 1 x 6 element to two 16-byte chunks and a 1 x 1 element to one (each thread of the apply then
+// pulls whole chunks for one value); the copy holds the values alone, coalesced per entry.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_build_drc(const T *__restrict__ D, size_t gsd, const T *__restrict__ M3,
+                                                     size_t gs3, const int *__restrict__ ebase, int ncells, int nl,
+                                                     T *__restrict__ out) {
+  const int nu = ncells * nl;
+  const int u = blockIdx.x * kBlock + threadIdx.x;
+  if (u >= nu) return;
+  const int pos = u / nl, m = u - pos * nl;
+  const long long e = static_cast<long long>(ebase[pos]) + m;
+#pragma unroll
+  for (int j = 0; j < 6; ++j) out[static_cast<size_t>(j) * nu + u] = D[aidx<T>(gsd, e, j)];
+  out[static_cast<size_t>(6) * nu + u] = M3[aidx<T>(gs3, e, 0)];
+}
+
+// k_helm_dg_m3 over that copy: the same products in the same order, bit for bit
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_helm_dg_m3_drc(const T *__restrict__ drc, const T *__restrict__ t,
+                                                          const int *__restrict__ map2, const T *__restrict__ x3,
+                                                          const int *__restrict__ map3, T *__restrict__ y, int ncells,
+                                                          int nl, T tau) {
+  const int nu = ncells * nl;
+  const int u = blockIdx.x * kBlock + threadIdx.x;
+  if (u >= nu) return;
+  const int pos = u / nl, m = u - pos * nl;
+  const int *b2 = map2 + static_cast<size_t>(pos) * 6;
+  T acc = T(0);
+#pragma unroll
+  for (int j = 0; j < 6; ++j) acc = fma(drc[static_cast<size_t>(j) * nu + u], t[b2[j] + m], acc);
+  const size_t d = static_cast<size_t>(map3[pos]) + m;
+  const T y3 = fma(drc[static_cast<size_t>(6) * nu + u], x3[d], T(0));
+  y[d] = y3 + tau * acc;
+}
+
 // assembled diagonal: sum over cells of A_k(df,df) (sci_mm_diagonal_kernel_mod.F90:87-92)
 template <typename T, int N>
 __global__ __launch_bounds__(kBlock) void k_diag_colour(const T *__restrict__ A, size_t gs, const int *__restrict__ ebase,
@@ -3403,9 +3440,24 @@ static int helmholtz_t(const lfric_linop *L, const T *x, T *y) {
     LAUNCH((k_helm_transpose_div<T>), tgrid, static_cast<const T *>(L->D->d), L->D->gs, w3.d_ebase, x, w3.d_map,
            static_cast<const T *>(L->m2->d), t, w2.d_gcol, w2.d_goff, w2.d_gent, w2.gather_ncols, w2.nlayers,
            roles_of(w2));
-  LAUNCH((k_helm_dg_m3<T>), grid_for(static_cast<long long>(w3.ncells) * w3.nlayers),
-         static_cast<const T *>(L->D->d), L->D->gs, static_cast<const T *>(L->M3->d), L->M3->gs, w3.d_ebase, t,
-         w2.d_map, x, w3.d_map, y, w3.ncells, w3.nlayers, static_cast<T>(L->tau));
+  const int ugrid = grid_for(static_cast<long long>(w3.ncells) * w3.nlayers);
+  if (L->hcol_state == 1 && (L->drc_dver != L->D->version || L->drc_mver != L->M3->version)) {
+    if (!L->d_drc) {  // (re)build the compact rows of D and M3
+      L->drc_bytes = 7 * sizeof(T) * static_cast<size_t>(w3.ncells) * w3.nlayers;
+      if (int rc = dmalloc(&L->d_drc, L->drc_bytes)) return rc;
+    }
+    LAUNCH((k_build_drc<T>), ugrid, static_cast<const T *>(L->D->d), L->D->gs, static_cast<const T *>(L->M3->d),
+           L->M3->gs, w3.d_ebase, w3.ncells, w3.nlayers, static_cast<T *>(L->d_drc));
+    CHECK_LAUNCH();
+    L->drc_dver = L->D->version;
+    L->drc_mver = L->M3->version;
+  }
+  if (L->hcol_state == 1)
+    LAUNCH((k_helm_dg_m3_drc<T>), ugrid, static_cast<const T *>(L->d_drc), t, w2.d_map, x, w3.d_map, y, w3.ncells,
+           w3.nlayers, static_cast<T>(L->tau));
+  else
+    LAUNCH((k_helm_dg_m3<T>), ugrid, static_cast<const T *>(L->D->d), L->D->gs, static_cast<const T *>(L->M3->d),
+           L->M3->gs, w3.d_ebase, t, w2.d_map, x, w3.d_map, y, w3.ncells, w3.nlayers, static_cast<T>(L->tau));
   CHECK_LAUNCH();
   return LFRIC_OK;
 }

@@ -777,6 +777,7 @@ lfric_prec::~lfric_prec() {
 lfric_linop::~lfric_linop() {
   dfree(d_hcol, hcol_bytes);
   dfree(d_dtc, dtc_bytes);
+  dfree(d_drc, drc_bytes);
   dfree(t_w2, t_w2_bytes);
   dfree(t_w3, t_w3_bytes);
 }

@@ -77,14 +77,14 @@ def test_helmholtz_two_kernel_apply_equals_the_kernel_chain(hip, orc, n, nl, pre
     """The Helmholtz composite's two-kernel apply (transpose in gather form with the division,
     over the composite's copy of D in transpose-gather order, then divergence, M3 and the axpy
     in one kernel) equals the chain of separate applies - colour-ordered transpose, division by
-    diag(M2), divergence, M3, y += tau t - bit for bit; after D's entries are rewritten the
-    copy follows them."""
+    diag(M2), divergence, M3, y += tau t - bit for bit; after D's or M3's entries are
+    rewritten the composite's copies follow them."""
     tau = 0.5
     w2, w3, H, (M2, D, M3, m2d) = _helmholtz(hip, orc, n, nl, precision, tau)
     x = hip.FieldVector(w3, precision).fill_uniform(3, 3)
-    for rewrite in (False, True):
-        if rewrite:
-            D.upload(np.roll(D.download(), 1, axis=0))
+    for rewrite in (None, D, M3):
+        if rewrite is not None:
+            rewrite.upload(np.roll(rewrite.download(), 1, axis=0))
         y = hip.FieldVector(w3, precision)
         H.apply(x, y)
         t2, t2d = hip.FieldVector(w2, precision), hip.FieldVector(w2, precision)
