@@ -312,7 +312,9 @@ int lfric_linop_create_mass_matrix(const lfric_op *A, int apply_bc, lfric_linop 
  *   y = M3 x + tau * D (diag(M2)^-1 (D^T x)),
  * D: W2 -> W3 LMA operator (dg_matrix_vector / transpose_matrix_vector kernels),
  * m2_diag: assembled diagonal of the W2 mass matrix (mm_diagonal), M3: W3 -> W3 LMA operator.
- * All three in one precision. */
+ * All three in one precision.  For lowest-order W2/W3 on one mesh the first apply builds
+ * copies of D (transpose-gather order) and of D's rows with M3 (compacted per W3 dof), held by
+ * L and rebuilt when D's or M3's entries are rewritten; L must not outlive M3, D or m2_diag. */
 int lfric_linop_create_helmholtz(const lfric_op *M3, const lfric_op *D, const lfric_vec *m2_diag,
                                  double tau, lfric_linop **L);
 int lfric_linop_apply(const lfric_linop *L, const lfric_vec *x, lfric_vec *y);
