@@ -315,18 +315,70 @@ __global__ __launch_bounds__(kBlock) void k_mgs_fold(T *__restrict__ w, const T 
   block_partials<1>(v, partials);
 }
 
+// The Givens rotations of sci_iterative_solver_mod.f90:929-953 (k_givens below) formed one
+// Hessenberg column at a time, as each Arnoldi column is made:
+// column iv gets rotations 0 .. iv-1 in order, then its own rotation iv is formed from its
+// entries (iv, iv) and (iv+1, iv), applied to it and to g.  Every entry goes through the
+// operations of k_givens's loop in the same order (rotation i reaches column j > i before
+// rotation i+1 does), so a full cycle ends with the same H and g; a cycle can also stop after
+// any column.  Column 0 starts g = (beta, 0, ...); first marks the first cycle, whose beta is
+// kept as beta0 for ratio = |g(iv+1)| / beta0 (the preconditioned residual estimate).  One
+// thread runs it, in the kernel that finishes h(iv+1, iv) (k_scale_fold, k_finish_givens).
+__device__ inline void givens_column(GmresState *st, int iv, int m, int first) {
+  double *h = st->h + static_cast<size_t>(iv) * (m + 1), *g = st->g;
+  if (iv == 0) {
+    g[0] = st->beta;
+    for (int i = 1; i <= m; ++i) g[i] = 0.0;
+    if (first) st->beta0 = st->beta;
+  }
+  for (int i = 0; i < iv; ++i) {
+    const double ci = st->cs[i], si = st->sn[i];
+    const double h1 = ci * h[i] + si * h[i + 1];
+    const double h2 = -si * h[i] + ci * h[i + 1];
+    h[i] = h1;
+    h[i + 1] = h2;
+  }
+  const double nrm = sqrt(h[iv] * h[iv] + h[iv + 1] * h[iv + 1]);
+  const double si = h[iv + 1] / nrm, ci = h[iv] / nrm;
+  st->cs[iv] = ci;
+  st->sn[iv] = si;
+  const double p = ci * g[iv] + si * g[iv + 1];
+  const double q = -si * g[iv] + ci * g[iv + 1];
+  g[iv] = p;
+  g[iv + 1] = q;
+  const double h1 = ci * h[iv] + si * h[iv + 1];
+  const double h2 = -si * h[iv] + ci * h[iv + 1];
+  h[iv] = h1;
+  h[iv + 1] = h2;
+  st->ratio = fabs(q) / st->beta0;
+}
+
+// k_finish_to (sqrt) for the cycle's last column, then its Givens column
+__global__ __launch_bounds__(kBlock) void k_finish_givens(const double *__restrict__ partials, int nblocks, double *dst,
+                                                          GmresState *st, int iv, int m, int first) {
+  const double tot = finish_sum(partials, nblocks);
+  if (threadIdx.x == 0) {
+    *dst = sqrt(tot);
+    givens_column(st, iv, m, first);
+  }
+}
+
 // k_scale_by_inv with the finish of ||w||^2 folded in: h = sqrt(sum of the partials), block 0
-// stores it, v = w * (1 / h)
+// stores it (and, with st, forms Givens column iv), v = w * (1 / h)
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_scale_fold(T *__restrict__ v, const T *__restrict__ w, long long n,
-                                                       const double *__restrict__ partials, int np, double *hout) {
+                                                       const double *__restrict__ partials, int np, double *hout,
+                                                       GmresState *st, int iv, int m, int first) {
   const long long i0 = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
   const long long stride = static_cast<long long>(gridDim.x) * blockDim.x;
   T wr[kMgsAhead];
 #pragma unroll
   for (int r = 0; r < kMgsAhead; ++r) wr[r] = i0 + r * stride < n ? w[i0 + r * stride] : T(0);
   const double h = sqrt(finish_sum(partials, np));
-  if (blockIdx.x == 0 && threadIdx.x == 0) *hout = h;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    *hout = h;
+    if (st) givens_column(st, iv, m, first);
+  }
   const T s = static_cast<T>(1.0 / h);
 #pragma unroll
   for (int r = 0; r < kMgsAhead; ++r)
@@ -380,43 +432,6 @@ __global__ void k_givens(GmresState *st, int m) {
     u[iv] = s / H(iv, iv);
   }
 #undef H
-}
-
-// The same rotations formed one Hessenberg column at a time, as each Arnoldi column is made:
-// column iv gets rotations 0 .. iv-1 in order, then its own rotation iv is formed from its
-// entries (iv, iv) and (iv+1, iv), applied to it and to g.  Every entry goes through the
-// operations of the loop above in the same order (rotation i reaches column j > i before
-// rotation i+1 does), so a full cycle ends with the same H and g; a cycle can also stop after
-// any column.  Column 0 starts g = (beta, 0, ...); first marks the first cycle, whose beta is
-// kept as beta0 for ratio = |g(iv+1)| / beta0 (the preconditioned residual estimate).
-__global__ void k_givens_col(GmresState *st, int iv, int m, int first) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  double *h = st->h + static_cast<size_t>(iv) * (m + 1), *g = st->g;
-  if (iv == 0) {
-    g[0] = st->beta;
-    for (int i = 1; i <= m; ++i) g[i] = 0.0;
-    if (first) st->beta0 = st->beta;
-  }
-  for (int i = 0; i < iv; ++i) {
-    const double ci = st->cs[i], si = st->sn[i];
-    const double h1 = ci * h[i] + si * h[i + 1];
-    const double h2 = -si * h[i] + ci * h[i + 1];
-    h[i] = h1;
-    h[i + 1] = h2;
-  }
-  const double nrm = sqrt(h[iv] * h[iv] + h[iv + 1] * h[iv + 1]);
-  const double si = h[iv + 1] / nrm, ci = h[iv] / nrm;
-  st->cs[iv] = ci;
-  st->sn[iv] = si;
-  const double p = ci * g[iv] + si * g[iv + 1];
-  const double q = -si * g[iv] + ci * g[iv + 1];
-  g[iv] = p;
-  g[iv + 1] = q;
-  const double h1 = ci * h[iv] + si * h[iv + 1];
-  const double h2 = -si * h[iv] + ci * h[iv + 1];
-  h[iv] = h1;
-  h[iv + 1] = h2;
-  st->ratio = fabs(q) / st->beta0;
 }
 
 // back substitution over the first k columns (k = m after a full cycle)
@@ -556,7 +571,8 @@ static int gm_init() {
 // halves of the scratch), so a step is one launch; the h values are those of separate
 // finishing kernels bit for bit.
 template <typename T>
-static int arnoldi_mgs(T *w, T *V, long long n, int iv, int m, double *h) {
+static int arnoldi_mgs(T *w, T *V, long long n, int iv, int m, double *h, GmresState *gst = nullptr,
+                       int first = 0) {
   const int vg = grid_for(n);
   double *part[2] = {g_gpart, g_gpart + kGmBlocks};
   LAUNCHK(k_mgs_step<T>, kGmBlocks, kBlock, w, static_cast<const T *>(nullptr), static_cast<const T *>(V), n,
@@ -569,7 +585,9 @@ static int arnoldi_mgs(T *w, T *V, long long n, int iv, int m, double *h) {
   double *hlast = &h[iv * (m + 1) + iv + 1];
   if (iv < m - 1)
     LAUNCHK(k_scale_fold<T>, std::min(vg, kGmBlocks), kBlock, V + static_cast<size_t>(iv + 1) * n, w, n,
-            part[(iv + 1) & 1], kGmBlocks, hlast);
+            part[(iv + 1) & 1], kGmBlocks, hlast, gst, iv, m, first);
+  else if (gst)
+    LAUNCHK(k_finish_givens, 1, kBlock, part[(iv + 1) & 1], kGmBlocks, hlast, gst, iv, m, first);
   else
     LAUNCHK(k_finish_to, 1, kBlock, part[(iv + 1) & 1], kGmBlocks, hlast, 1);
   LF_HIP(hipGetLastError());
@@ -650,8 +668,7 @@ static int gmres_t(const lfric_linop *A, const lfric_prec *P, T *x, const T *b, 
       T *viv = V + static_cast<size_t>(iv) * n;
       if (int rc = linop_apply(A, viv, s)) return rc;  // s = A w (w = v_iv)
       if (int rc = prec_apply_any(P, prec, s, w, n)) return rc;  // w = P^-1 s
-      if (int rc = arnoldi_mgs<T>(w, V, n, iv, m, st->h)) return rc;
-      LAUNCHK(k_givens_col, 1, 64, st, iv, m, cyc == 1 ? 1 : 0);
+      if (int rc = arnoldi_mgs<T>(w, V, n, iv, m, st->h, st, cyc == 1 ? 1 : 0)) return rc;  // + Givens column iv
       if (early && iv < m - 1) {
         double *slot = g_host_scalar + 2 + (iv & 1);
         LF_HIP(hipMemcpyAsync(slot, &st->ratio, sizeof(double), hipMemcpyDeviceToHost, cur_stream()));
