@@ -322,45 +322,57 @@ __global__ __launch_bounds__(kBlock) void k_mgs_fold(T *__restrict__ w, const T 
 // operations of k_givens's loop in the same order (rotation i reaches column j > i before
 // rotation i+1 does), so a full cycle ends with the same H and g; a cycle can also stop after
 // any column.  Column 0 starts g = (beta, 0, ...); first marks the first cycle, whose beta is
-// kept as beta0 for ratio = |g(iv+1)| / beta0 (the preconditioned residual estimate).  One
-// thread runs it, in the kernel that finishes h(iv+1, iv) (k_scale_fold, k_finish_givens).
-__device__ inline void givens_column(GmresState *st, int iv, int m, int first) {
+// kept as beta0 for ratio = |g(iv+1)| / beta0 (the preconditioned residual estimate).  Block 0
+// of the kernel that finishes hnew = h(iv+1, iv) runs it (k_scale_fold, k_finish_givens): its
+// threads read the earlier rotations and the column into LDS together, and thread 0 does the
+// arithmetic there and writes the results back.
+__device__ inline void givens_column_block(GmresState *st, int iv, int m, int first, double hnew) {
+  __shared__ double s_cs[kMaxKrylov], s_sn[kMaxKrylov], s_h[kMaxKrylov + 1];
   double *h = st->h + static_cast<size_t>(iv) * (m + 1), *g = st->g;
-  if (iv == 0) {
-    g[0] = st->beta;
-    for (int i = 1; i <= m; ++i) g[i] = 0.0;
-    if (first) st->beta0 = st->beta;
+  for (int i = threadIdx.x; i < iv; i += blockDim.x) {
+    s_cs[i] = st->cs[i];
+    s_sn[i] = st->sn[i];
   }
-  for (int i = 0; i < iv; ++i) {
-    const double ci = st->cs[i], si = st->sn[i];
-    const double h1 = ci * h[i] + si * h[i + 1];
-    const double h2 = -si * h[i] + ci * h[i + 1];
-    h[i] = h1;
-    h[i + 1] = h2;
+  for (int i = threadIdx.x; i <= iv; i += blockDim.x) s_h[i] = h[i];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    s_h[iv + 1] = hnew;
+    if (iv == 0) {
+      g[0] = st->beta;
+      for (int i = 1; i <= m; ++i) g[i] = 0.0;
+      if (first) st->beta0 = st->beta;
+    }
+    for (int i = 0; i < iv; ++i) {
+      const double ci = s_cs[i], si = s_sn[i];
+      const double h1 = ci * s_h[i] + si * s_h[i + 1];
+      const double h2 = -si * s_h[i] + ci * s_h[i + 1];
+      s_h[i] = h1;
+      s_h[i + 1] = h2;
+    }
+    const double nrm = sqrt(s_h[iv] * s_h[iv] + s_h[iv + 1] * s_h[iv + 1]);
+    const double si = s_h[iv + 1] / nrm, ci = s_h[iv] / nrm;
+    st->cs[iv] = ci;
+    st->sn[iv] = si;
+    const double gi = g[iv], gn = g[iv + 1];
+    const double p = ci * gi + si * gn;
+    const double q = -si * gi + ci * gn;
+    g[iv] = p;
+    g[iv + 1] = q;
+    const double h1 = ci * s_h[iv] + si * s_h[iv + 1];
+    const double h2 = -si * s_h[iv] + ci * s_h[iv + 1];
+    s_h[iv] = h1;
+    s_h[iv + 1] = h2;
+    for (int i = 0; i <= iv + 1; ++i) h[i] = s_h[i];
+    st->ratio = fabs(q) / st->beta0;
   }
-  const double nrm = sqrt(h[iv] * h[iv] + h[iv + 1] * h[iv + 1]);
-  const double si = h[iv + 1] / nrm, ci = h[iv] / nrm;
-  st->cs[iv] = ci;
-  st->sn[iv] = si;
-  const double p = ci * g[iv] + si * g[iv + 1];
-  const double q = -si * g[iv] + ci * g[iv + 1];
-  g[iv] = p;
-  g[iv + 1] = q;
-  const double h1 = ci * h[iv] + si * h[iv + 1];
-  const double h2 = -si * h[iv] + ci * h[iv + 1];
-  h[iv] = h1;
-  h[iv + 1] = h2;
-  st->ratio = fabs(q) / st->beta0;
 }
 
 // k_finish_to (sqrt) for the cycle's last column, then its Givens column
 __global__ __launch_bounds__(kBlock) void k_finish_givens(const double *__restrict__ partials, int nblocks, double *dst,
                                                           GmresState *st, int iv, int m, int first) {
   const double tot = finish_sum(partials, nblocks);
-  if (threadIdx.x == 0) {
-    *dst = sqrt(tot);
-    givens_column(st, iv, m, first);
-  }
+  if (threadIdx.x == 0) *dst = sqrt(tot);
+  givens_column_block(st, iv, m, first, sqrt(tot));
 }
 
 // k_scale_by_inv with the finish of ||w||^2 folded in: h = sqrt(sum of the partials), block 0
@@ -375,10 +387,8 @@ __global__ __launch_bounds__(kBlock) void k_scale_fold(T *__restrict__ v, const 
 #pragma unroll
   for (int r = 0; r < kMgsAhead; ++r) wr[r] = i0 + r * stride < n ? w[i0 + r * stride] : T(0);
   const double h = sqrt(finish_sum(partials, np));
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    *hout = h;
-    if (st) givens_column(st, iv, m, first);
-  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) *hout = h;
+  if (st && blockIdx.x == 0) givens_column_block(st, iv, m, first, h);
   const T s = static_cast<T>(1.0 / h);
 #pragma unroll
   for (int r = 0; r < kMgsAhead; ++r)
