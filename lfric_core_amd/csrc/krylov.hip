@@ -414,50 +414,67 @@ __global__ void k_scale_by_inv(T *__restrict__ v, const T *__restrict__ w, long 
 
 // Givens rotations and back substitution (sci_iterative_solver_mod.f90:929-953), FP64, on the
 // device; g(0) = beta, the rest zero.
-__global__ void k_givens(GmresState *st, int m) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+// The block's threads read H (and g) into LDS together; thread 0 runs the sequential
+// arithmetic there and the block writes the results back.
+__global__ __launch_bounds__(64) void k_givens(GmresState *st, int m) {
+  __shared__ double s_h[(kMaxKrylov + 1) * kMaxKrylov], s_g[kMaxKrylov + 1], s_u[kMaxKrylov];
   const int ld = m + 1;
-  double *h = st->h, *g = st->g, *u = st->u;
-  g[0] = st->beta;
-  for (int i = 1; i <= m; ++i) g[i] = 0.0;
+  for (int i = threadIdx.x; i < ld * m; i += blockDim.x) s_h[i] = st->h[i];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double *h = s_h, *g = s_g, *u = s_u;
+    g[0] = st->beta;
+    for (int i = 1; i <= m; ++i) g[i] = 0.0;
 #define H(i, j) h[(j) * ld + (i)]
-  for (int iv = 0; iv < m; ++iv) {
-    const double nrm = sqrt(H(iv, iv) * H(iv, iv) + H(iv + 1, iv) * H(iv + 1, iv));
-    const double si = H(iv + 1, iv) / nrm, ci = H(iv, iv) / nrm;
-    const double p = ci * g[iv] + si * g[iv + 1];
-    const double q = -si * g[iv] + ci * g[iv + 1];
-    g[iv] = p;
-    g[iv + 1] = q;
-    for (int j = iv; j < m; ++j) {
-      const double h1 = ci * H(iv, j) + si * H(iv + 1, j);
-      const double h2 = -si * H(iv, j) + ci * H(iv + 1, j);
-      H(iv, j) = h1;
-      H(iv + 1, j) = h2;
+    for (int iv = 0; iv < m; ++iv) {
+      const double nrm = sqrt(H(iv, iv) * H(iv, iv) + H(iv + 1, iv) * H(iv + 1, iv));
+      const double si = H(iv + 1, iv) / nrm, ci = H(iv, iv) / nrm;
+      const double p = ci * g[iv] + si * g[iv + 1];
+      const double q = -si * g[iv] + ci * g[iv + 1];
+      g[iv] = p;
+      g[iv + 1] = q;
+      for (int j = iv; j < m; ++j) {
+        const double h1 = ci * H(iv, j) + si * H(iv + 1, j);
+        const double h2 = -si * H(iv, j) + ci * H(iv + 1, j);
+        H(iv, j) = h1;
+        H(iv + 1, j) = h2;
+      }
     }
-  }
-  u[m - 1] = g[m - 1] / H(m - 1, m - 1);
-  for (int iv = m - 2; iv >= 0; --iv) {
-    double s = g[iv];
-    for (int j = iv + 1; j < m; ++j) s = s - H(iv, j) * u[j];
-    u[iv] = s / H(iv, iv);
-  }
+    u[m - 1] = g[m - 1] / H(m - 1, m - 1);
+    for (int iv = m - 2; iv >= 0; --iv) {
+      double s = g[iv];
+      for (int j = iv + 1; j < m; ++j) s = s - H(iv, j) * u[j];
+      u[iv] = s / H(iv, iv);
+    }
 #undef H
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < ld * m; i += blockDim.x) st->h[i] = s_h[i];
+  for (int i = threadIdx.x; i <= m; i += blockDim.x) st->g[i] = s_g[i];
+  for (int i = threadIdx.x; i < m; i += blockDim.x) st->u[i] = s_u[i];
 }
 
-// back substitution over the first k columns (k = m after a full cycle)
-__global__ void k_givens_solve(GmresState *st, int k, int m) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+// back substitution over the first k columns (k = m after a full cycle), from LDS as k_givens
+__global__ __launch_bounds__(64) void k_givens_solve(GmresState *st, int k, int m) {
+  __shared__ double s_h[(kMaxKrylov + 1) * kMaxKrylov], s_g[kMaxKrylov], s_u[kMaxKrylov];
   const int ld = m + 1;
-  const double *h = st->h, *g = st->g;
-  double *u = st->u;
+  for (int i = threadIdx.x; i < ld * k; i += blockDim.x) s_h[i] = st->h[i];
+  for (int i = threadIdx.x; i < k; i += blockDim.x) s_g[i] = st->g[i];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const double *h = s_h, *g = s_g;
+    double *u = s_u;
 #define H(i, j) h[(j) * ld + (i)]
-  u[k - 1] = g[k - 1] / H(k - 1, k - 1);
-  for (int iv = k - 2; iv >= 0; --iv) {
-    double s = g[iv];
-    for (int j = iv + 1; j < k; ++j) s = s - H(iv, j) * u[j];
-    u[iv] = s / H(iv, iv);
-  }
+    u[k - 1] = g[k - 1] / H(k - 1, k - 1);
+    for (int iv = k - 2; iv >= 0; --iv) {
+      double s = g[iv];
+      for (int j = iv + 1; j < k; ++j) s = s - H(iv, j) * u[j];
+      u[iv] = s / H(iv, iv);
+    }
 #undef H
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < k; i += blockDim.x) st->u[i] = s_u[i];
 }
 
 // x = x + sum_iv u(iv) v_iv, iv in order (sci_iterative_solver_mod.f90:956-960)
