@@ -693,8 +693,12 @@ static int gmres_t(const lfric_linop *A, const lfric_prec *P, T *x, const T *b, 
     int k = m;  // columns made in this cycle
     for (int iv = 0; iv < m; ++iv) {
       T *viv = V + static_cast<size_t>(iv) * n;
-      if (int rc = linop_apply(A, viv, s)) return rc;  // s = A w (w = v_iv)
-      if (int rc = prec_apply_any(P, prec, s, w, n)) return rc;  // w = P^-1 s
+      if (!P || P->kind == 0) {  // identity: w = P^-1 (A v_iv) is the apply's output itself
+        if (int rc = linop_apply(A, viv, w)) return rc;
+      } else {
+        if (int rc = linop_apply(A, viv, s)) return rc;  // s = A w (w = v_iv)
+        if (int rc = prec_apply_any(P, prec, s, w, n)) return rc;  // w = P^-1 s
+      }
       if (int rc = arnoldi_mgs<T>(w, V, n, iv, m, st->h, st, cyc == 1 ? 1 : 0)) return rc;  // + Givens column iv
       if (early && iv < m - 1) {
         double *slot = g_host_scalar + 2 + (iv & 1);
