@@ -174,14 +174,18 @@ __global__ void k_coltri_apply(const typename Store<STORE>::T *__restrict__ fac,
 // range's r and factors into LDS with coalesced loads (kColAhead per thread requested before
 // any is stored), the first `tile` lanes solve one column each there (the operations of
 // k_coltri_apply, in its order), and the block writes z back coalesced.  cap bounds every
-// tile's range.
+// tile's range.  With vdot the block also leaves the partial sum of vdot . z over its range
+// (FP64, as the Gram-Schmidt kernels form theirs) in partials[block]: GMRES's first
+// Gram-Schmidt step of the column.
 constexpr int kColThreads = 256, kColAhead = 4;
 
 template <int STORE, typename V>
 __global__ __launch_bounds__(kColThreads) void k_coltri_apply_tile(const typename Store<STORE>::T *__restrict__ fac,
                                                                    long long undf, const int *__restrict__ cols,
                                                                    int ncols, int tile, int cap,
-                                                                   const V *__restrict__ r, V *__restrict__ z) {
+                                                                   const V *__restrict__ r, V *__restrict__ z,
+                                                                   const V *__restrict__ vdot,
+                                                                   double *__restrict__ partials) {
   using S = Store<STORE>;
   using FT = typename S::T;
   extern __shared__ __align__(16) unsigned char coltri_lds[];
@@ -230,7 +234,13 @@ __global__ __launch_bounds__(kColThreads) void k_coltri_apply_tile(const typenam
     }
   }
   __syncthreads();
-  for (int i = threadIdx.x; i < len; i += kColThreads) z[static_cast<size_t>(lo) + i] = static_cast<V>(sv[i]);
+  double dot[1] = {0.0};
+  for (int i = threadIdx.x; i < len; i += kColThreads) {
+    const V zi = static_cast<V>(sv[i]);
+    z[static_cast<size_t>(lo) + i] = zi;
+    if (vdot) dot[0] += to_d(vdot[static_cast<size_t>(lo) + i]) * to_d(zi);
+  }
+  if (vdot) block_partials<1>(dot, partials);  // the same for every thread of the block
 }
 
 // ---- GMRES -------------------------------------------------------------------------------
@@ -517,7 +527,15 @@ int linop_apply(const lfric_linop *L, const void *x, void *y) {
   return launch_axpy(L->precision, y, L->t_w3, n3, L->tau);
 }
 
-int prec_apply_any(const lfric_prec *P, int prec, const void *x, void *y, long long n) {
+// the tiled column block-Jacobi's blocks (one partial sum each when it also forms a dot), 0
+// when P does not apply that way
+static int coltri_tiles(const lfric_prec *P) {
+  return (P && P->kind == 2 && P->tile) ? (P->ncols + P->tile - 1) / P->tile : 0;
+}
+
+// y = P^-1 x; vdot/partials only with coltri_tiles(P) > 0: also the partials of vdot . y
+static int prec_apply_impl(const lfric_prec *P, int prec, const void *x, void *y, long long n, const void *vdot,
+                           double *partials) {
   if (!P || P->kind == 0) return launch_copy(prec, y, x, n);
   if (P->kind == 1) {
     LF_CHECK(P->diag->precision == prec && P->diag->n == n, LFRIC_E_ARG, "diagonal does not match vector");
@@ -537,11 +555,13 @@ int prec_apply_any(const lfric_prec *P, int prec, const void *x, void *y, long l
     if (P->tile && prec == LFRIC_R64)                                                                          \
       hipLaunchKernelGGL((k_coltri_apply_tile<ST, double>), dim3(tgrid), dim3(kColThreads), lds,               \
                          cur_stream(), static_cast<const FT *>(P->d_fac), P->undf, P->d_cols, P->ncols,        \
-                         P->tile, P->cap, static_cast<const double *>(x), static_cast<double *>(y));           \
+                         P->tile, P->cap, static_cast<const double *>(x), static_cast<double *>(y),            \
+                         static_cast<const double *>(vdot), partials);                                         \
     else if (P->tile)                                                                                          \
       hipLaunchKernelGGL((k_coltri_apply_tile<ST, float>), dim3(tgrid), dim3(kColThreads), lds,                \
                          cur_stream(), static_cast<const FT *>(P->d_fac), P->undf, P->d_cols, P->ncols,        \
-                         P->tile, P->cap, static_cast<const float *>(x), static_cast<float *>(y));             \
+                         P->tile, P->cap, static_cast<const float *>(x), static_cast<float *>(y),              \
+                         static_cast<const float *>(vdot), partials);                                          \
     else if (prec == LFRIC_R64)                                                                                \
       LAUNCHK((k_coltri_apply<ST, double>), grid, 64, static_cast<const FT *>(P->d_fac), P->undf, P->d_cols,   \
               P->ncols, static_cast<const double *>(x), static_cast<double *>(y));                             \
@@ -556,6 +576,10 @@ int prec_apply_any(const lfric_prec *P, int prec, const void *x, void *y, long l
   CT_APPLY(LFRIC_F16)
 #undef CT_APPLY
   return fail(LFRIC_E_UNSUPPORTED, "column preconditioner storage");
+}
+
+int prec_apply_any(const lfric_prec *P, int prec, const void *x, void *y, long long n) {
+  return prec_apply_impl(P, prec, x, y, n, nullptr, nullptr);
 }
 
 // scratch of the GMRES driver, grown on demand
@@ -597,17 +621,21 @@ static int gm_init() {
 // partial sums are finished by the next kernel's blocks themselves (partials in two alternating
 // halves of the scratch), so a step is one launch; the h values are those of separate
 // finishing kernels bit for bit.
+// first_parts > 0: step 0's partials (v_0 . w) are already in the first half of the scratch,
+// that many of them (left there by the preconditioner's apply).
 template <typename T>
 static int arnoldi_mgs(T *w, T *V, long long n, int iv, int m, double *h, GmresState *gst = nullptr,
-                       int first = 0) {
+                       int first = 0, int first_parts = 0) {
   const int vg = grid_for(n);
   double *part[2] = {g_gpart, g_gpart + kGmBlocks};
-  LAUNCHK(k_mgs_step<T>, kGmBlocks, kBlock, w, static_cast<const T *>(nullptr), static_cast<const T *>(V), n,
-          static_cast<const double *>(nullptr), part[0]);
+  if (!first_parts)
+    LAUNCHK(k_mgs_step<T>, kGmBlocks, kBlock, w, static_cast<const T *>(nullptr), static_cast<const T *>(V), n,
+            static_cast<const double *>(nullptr), part[0]);
   for (int j = 1; j <= iv + 1; ++j) {
     const T *vj = j <= iv ? V + static_cast<size_t>(j) * n : nullptr;
+    const int nprev = (j == 1 && first_parts) ? first_parts : kGmBlocks;
     LAUNCHK(k_mgs_fold<T>, kGmBlocks, kBlock, w, V + static_cast<size_t>(j - 1) * n, vj, n, part[(j - 1) & 1],
-            kGmBlocks, &h[iv * (m + 1) + (j - 1)], part[j & 1]);
+            nprev, &h[iv * (m + 1) + (j - 1)], part[j & 1]);
   }
   double *hlast = &h[iv * (m + 1) + iv + 1];
   if (iv < m - 1)
@@ -693,13 +721,20 @@ static int gmres_t(const lfric_linop *A, const lfric_prec *P, T *x, const T *b, 
     int k = m;  // columns made in this cycle
     for (int iv = 0; iv < m; ++iv) {
       T *viv = V + static_cast<size_t>(iv) * n;
+      int first_parts = 0;
       if (!P || P->kind == 0) {  // identity: w = P^-1 (A v_iv) is the apply's output itself
         if (int rc = linop_apply(A, viv, w)) return rc;
+      } else if (coltri_tiles(P) > 0 && coltri_tiles(P) <= kGmBlocks) {
+        // the tiled column block-Jacobi also leaves Gram-Schmidt step 0's partials (v_0 . w)
+        if (int rc = linop_apply(A, viv, s)) return rc;
+        if (int rc = prec_apply_impl(P, prec, s, w, n, V, g_gpart)) return rc;
+        first_parts = coltri_tiles(P);
       } else {
         if (int rc = linop_apply(A, viv, s)) return rc;  // s = A w (w = v_iv)
         if (int rc = prec_apply_any(P, prec, s, w, n)) return rc;  // w = P^-1 s
       }
-      if (int rc = arnoldi_mgs<T>(w, V, n, iv, m, st->h, st, cyc == 1 ? 1 : 0)) return rc;  // + Givens column iv
+      // Gram-Schmidt and Givens column iv
+      if (int rc = arnoldi_mgs<T>(w, V, n, iv, m, st->h, st, cyc == 1 ? 1 : 0, first_parts)) return rc;
       if (early && iv < m - 1) {
         double *slot = g_host_scalar + 2 + (iv & 1);
         LF_HIP(hipMemcpyAsync(slot, &st->ratio, sizeof(double), hipMemcpyDeviceToHost, cur_stream()));
