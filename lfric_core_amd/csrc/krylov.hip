@@ -716,8 +716,12 @@ static int gmres_t(const lfric_linop *A, const lfric_prec *P, T *x, const T *b, 
   if (monitor) sc_err = std::max(sc_err, a_tol);
   for (int cyc = 1; cyc <= max_cycles; ++cyc) {
     if (int rc = prec_apply_any(P, prec, r, s, n)) return rc;
-    if (int rc = dot_into<T>(s, s, n, &st->beta, true)) return rc;
-    LAUNCHK(k_scale_by_inv<T>, vg, kBlock, V, s, n, &st->beta);
+    // beta = ||s|| and v_0 = s / beta: the partials of s . s, then one kernel that finishes them
+    // (as k_finish_to) and scales (as k_scale_by_inv)
+    LAUNCHK(k_mgs_step<T>, kGmBlocks, kBlock, s, static_cast<const T *>(nullptr), static_cast<const T *>(nullptr), n,
+            static_cast<const double *>(nullptr), g_gpart);
+    LAUNCHK(k_scale_fold<T>, std::min(grid_for(n), kGmBlocks), kBlock, V, s, n, g_gpart, kGmBlocks, &st->beta,
+            static_cast<GmresState *>(nullptr), 0, m, 0);
     int k = m;  // columns made in this cycle
     for (int iv = 0; iv < m; ++iv) {
       T *viv = V + static_cast<size_t>(iv) * n;
